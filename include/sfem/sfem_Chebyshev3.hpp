@@ -1,0 +1,102 @@
+// sfem::Chebyshev3<T> (algebra/sfem_Chebyshev3.hpp:57-303): three-term Chebyshev
+// smoother over [scale_eig_min*lmax, lmax], lmax = scale_eig_max * eig_max.
+#pragma once
+
+#include "sfem_Buffer.hpp"
+#include "sfem_MatrixFreeLinearSolver.hpp"
+#include "sfem_PowerMethod.hpp"
+
+namespace sfem {
+
+template <typename T>
+class Chebyshev3 final : public MatrixFreeLinearSolver<T> {
+public:
+    SharedOperator<T> op;
+    std::shared_ptr<BLAS<T>> blas;
+    std::shared_ptr<PowerMethod<T>> power_method;
+    SharedBuffer<T> p_, temp_;
+
+    T eigen_solver_tol{1e-6};
+    int eigen_solver_max_it{1000};
+    int max_it{3};
+    int iterations_{0};
+    T eig_max{0};
+    T scale_eig_max{1};
+    T scale_eig_min{0.06};
+    std::ptrdiff_t n_dofs{-1};
+    bool is_initial_guess_zero{false};
+    bool apply_overwrites_output{false};
+    void *stream{nullptr};
+
+    void default_init(void *s = nullptr) {
+        stream = s;
+        blas = make_device_blas<T>(s);
+        power_method = std::make_shared<PowerMethod<T>>();
+        power_method->blas = blas;
+    }
+    int iterations() const override { return iterations_; }
+    void set_initial_guess_zero(bool v) override { is_initial_guess_zero = v; }
+    void set_op(const SharedOperator<T> &o) override {
+        op = o;
+        n_dofs = o->rows();
+    }
+    void set_preconditioner_op(const SharedOperator<T> &) override { fatal("Chebyshev3 is unpreconditioned"); }
+    void set_max_it(int it) override { max_it = it; }
+    void set_n_dofs(std::ptrdiff_t n) override { n_dofs = n; }
+    std::ptrdiff_t rows() const override { return n_dofs; }
+    std::ptrdiff_t cols() const override { return n_dofs; }
+    ExecutionSpace execution_space() const override { return EXECUTION_SPACE_DEVICE; }
+
+    void init_with_ones() {
+        auto ones = Buffer<T>::own(n_dofs, blas->allocate(n_dofs), [b = blas](void *q) { b->destroy(q); });
+        blas->values(n_dofs, 1, ones->data());
+        init(ones->data());
+    }
+    // algebra/sfem_Chebyshev3.hpp:165-175
+    void init(const T *guess) {
+        T *ev = blas->allocate(n_dofs);
+        T *work = blas->allocate(n_dofs);
+        blas->copy(n_dofs, guess, ev);
+        auto apply_fn = [this](const T *const x, T *const y) { op->apply(x, y); };
+        eig_max = power_method->max_eigen_value(apply_fn, eigen_solver_max_it, eigen_solver_tol, n_dofs, ev, work);
+        auto b = blas;
+        p_ = Buffer<T>::own(n_dofs, work, [b](void *q) { b->destroy(q); });
+        temp_ = Buffer<T>::own(n_dofs, ev, [b](void *q) { b->destroy(q); });
+    }
+
+    int apply(const T *const rhs, T *const x) override {
+        if (!op || !p_) return SFEM_FAILURE;
+        if (max_it <= 0) {
+            iterations_ = 0;
+            return SFEM_SUCCESS;
+        }
+        const std::ptrdiff_t n = n_dofs;
+        T *p = p_->data(), *temp = temp_->data();
+        const T lmax = eig_max * scale_eig_max;
+        const T lmin = scale_eig_min * lmax;
+        const T avg = (lmin + lmax) / 2, diff = (lmin - lmax) / 2;
+        T alpha = T(1) / avg, beta = 0, dea = 0;
+        // iteration 0: p = A x - rhs (or -rhs when x == 0); x -= alpha p
+        if (is_initial_guess_zero) {
+            blas->zaxpby(n, T(-1), rhs, T(0), rhs, p);
+        } else {
+            if (!apply_overwrites_output) blas->zeros(n, p);
+            op->apply(x, p);
+            blas->axpy(n, T(-1), rhs, p);
+        }
+        blas->axpy(n, -alpha, p, x);
+        iterations_ = 1;
+        for (; iterations_ < max_it; iterations_++) {
+            dea = diff * alpha;
+            beta = (iterations_ == 1 ? T(0.5) : T(0.25)) * dea * dea;
+            alpha = T(1) / (avg - beta / alpha);
+            if (!apply_overwrites_output) blas->zeros(n, temp);
+            op->apply(x, temp);
+            // fused: p = beta p - rhs + A x; x -= alpha p
+            sfem_amd_cheb3_update(real_type_of<T>::value, n, alpha, beta, rhs, temp, p, x, stream);
+        }
+        return SFEM_SUCCESS;
+    }
+};
+
+}  // namespace sfem

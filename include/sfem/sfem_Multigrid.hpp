@@ -1,0 +1,130 @@
+// sfem::Multigrid<T> (algebra/sfem_Multigrid.hpp:27-347).  Level 0 is the finest;
+// add_level(op, smoother_or_solver, prolongation, restriction).  apply(rhs, x) wraps the
+// caller's arrays as finest-level storage and runs up to max_it cycles, stopping when the
+// finest residual norm is below atol.
+#pragma once
+
+#include <cmath>
+#include <vector>
+
+#include "sfem_Buffer.hpp"
+#include "sfem_Operator.hpp"
+#include "sfem_tpl_blas.hpp"
+
+namespace sfem {
+
+template <typename T>
+class Multigrid final : public Operator<T> {
+public:
+    enum CycleType { V_CYCLE = 1, W_CYCLE = 2 };
+    enum CycleReturnCode { CYCLE_CONTINUE = 0, CYCLE_CONVERGED = 1, CYCLE_FAILURE = 2 };
+
+    std::shared_ptr<BLAS<T>> blas_;
+    bool verbose{false};
+    std::vector<double> residual_history;  // finest residual norm per cycle
+    // Called after each cycle with (cycle, finest residual norm); may replace smoothers.
+    std::function<void(int, double)> on_cycle;
+
+    void default_init(void *stream = nullptr) { blas_ = make_device_blas<T>(stream); }
+    std::shared_ptr<BLAS<T>> &blas() { return blas_; }
+    ExecutionSpace execution_space() const override { return EXECUTION_SPACE_DEVICE; }
+
+    void add_level(const SharedOperator<T> &op, const SharedOperator<T> &smoother_or_solver,
+                   const SharedOperator<T> &prolongation, const SharedOperator<T> &restriction) {
+        operator_.push_back(op);
+        smoother_.push_back(smoother_or_solver);
+        prolongation_.push_back(prolongation);
+        restriction_.push_back(restriction);
+        memory_.clear();
+    }
+    void set_cycle_type(int v) { cycle_type_ = v; }
+    void set_max_it(int v) { max_it_ = v; }
+    void set_atol(T v) { atol_ = v; }
+    int n_levels() const { return (int)smoother_.size(); }
+    int iterations() const { return iterations_; }
+    std::ptrdiff_t rows() const override { return operator_.front()->rows(); }
+    std::ptrdiff_t cols() const override { return operator_.front()->cols(); }
+    std::vector<SharedOperator<T>> &operators() { return operator_; }
+    std::vector<SharedOperator<T>> &smoothers() { return smoother_; }
+
+    int apply(const T *const rhs, T *const x) override {
+        ensure_init();
+        memory_[0].solution = Buffer<T>::wrap(rows(), x);
+        memory_[0].rhs = Buffer<T>::wrap(rows(), (T *)rhs);
+        residual_history.clear();
+        for (iterations_ = 0; iterations_ < max_it_; iterations_++) {
+            const CycleReturnCode ret = cycle(0);
+            if (ret == CYCLE_FAILURE) return SFEM_FAILURE;
+            if (ret == CYCLE_CONVERGED) break;
+        }
+        return SFEM_SUCCESS;
+    }
+
+private:
+    struct Memory {
+        SharedBuffer<T> rhs, solution, work;
+        std::ptrdiff_t size() const { return work->size(); }
+    };
+    std::vector<SharedOperator<T>> operator_, smoother_, prolongation_, restriction_;
+    std::vector<Memory> memory_;
+    int max_it_{10};
+    int iterations_{0};
+    int cycle_type_{V_CYCLE};
+    T atol_{1e-10};
+
+    void ensure_init() {
+        if (!memory_.empty()) return;
+        memory_.resize(n_levels());
+        auto b = blas_;
+        for (int l = 0; l < n_levels(); l++) {
+            const std::ptrdiff_t n = operator_[l]->rows();
+            if (l != 0) {
+                memory_[l].solution = Buffer<T>::own(n, b->allocate(n), [b](void *q) { b->destroy(q); });
+                memory_[l].rhs = Buffer<T>::own(n, b->allocate(n), [b](void *q) { b->destroy(q); });
+            }
+            memory_[l].work = Buffer<T>::own(n, b->allocate(n), [b](void *q) { b->destroy(q); });
+        }
+    }
+
+    // algebra/sfem_Multigrid.hpp:242-346
+    CycleReturnCode cycle(int level) {
+        Memory &mem = memory_[level];
+        auto &smoother = smoother_[level];
+        if (level == n_levels() - 1) {
+            blas_->zeros(mem.solution->size(), mem.solution->data());
+            return smoother->apply(mem.rhs->data(), mem.solution->data()) == SFEM_SUCCESS ? CYCLE_CONTINUE
+                                                                                         : CYCLE_FAILURE;
+        }
+        auto &op = operator_[level];
+        auto &restriction = restriction_[level];
+        auto &prolongation = prolongation_[level + 1];
+        Memory &coarse = memory_[level + 1];
+        for (int k = 0; k < cycle_type_; k++) {
+            smoother->apply(mem.rhs->data(), mem.solution->data());
+            // residual
+            blas_->zeros(mem.size(), mem.work->data());
+            op->apply(mem.solution->data(), mem.work->data());
+            blas_->axpby(mem.size(), T(1), mem.rhs->data(), T(-1), mem.work->data());
+            if (level == 0) {
+                const double nr = (double)blas_->norm2(mem.size(), mem.work->data());
+                residual_history.push_back(nr);
+                if (verbose) std::printf("Multigrid %d\t%g\n", iterations_, nr);
+                if (on_cycle) on_cycle(iterations_, nr);
+                if (nr < (double)atol_) return CYCLE_CONVERGED;
+            }
+            // restriction
+            blas_->zeros(coarse.rhs->size(), coarse.rhs->data());
+            restriction->apply(mem.work->data(), coarse.rhs->data());
+            blas_->zeros(coarse.solution->size(), coarse.solution->data());
+            if (cycle(level + 1) == CYCLE_FAILURE) return CYCLE_FAILURE;
+            // prolongation and correction
+            blas_->zeros(mem.size(), mem.work->data());
+            prolongation->apply(coarse.solution->data(), mem.work->data());
+            blas_->axpby(mem.size(), T(1), mem.work->data(), T(1), mem.solution->data());
+            smoother->apply(mem.rhs->data(), mem.solution->data());
+        }
+        return CYCLE_CONTINUE;
+    }
+};
+
+}  // namespace sfem

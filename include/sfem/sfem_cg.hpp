@@ -1,0 +1,230 @@
+// sfem::ConjugateGradient<T> (algebra/sfem_cg.hpp:80-446).
+//  * plain CG (aux_apply_basic :301-363) and left-preconditioned CG with a fixed
+//    preconditioner (aux_apply_precond :365-445), stopping rules as the reference;
+//  * flexible PCG (an extension; the reference has none): beta = r_new.(z_new - z_old) /
+//    r_old.z_old (Polak-Ribiere form), for preconditioners that are not a fixed SPD map
+//    (FP32 or learned V-cycles).  It stops on the true relative residual ||b - A x||/||b||,
+//    and recomputes r = b - A x in FP64 every `residual_replacement` iterations and at
+//    the end to guard against drift.
+#pragma once
+
+#include <cmath>
+#include <vector>
+
+#include "sfem_MatrixFreeLinearSolver.hpp"
+#include "sfem_tpl_blas.hpp"
+
+namespace sfem {
+
+template <typename T>
+class ConjugateGradient final : public MatrixFreeLinearSolver<T> {
+public:
+    SharedOperator<T> apply_op;
+    SharedOperator<T> preconditioner_op;
+    std::shared_ptr<BLAS<T>> blas;
+    void *stream{nullptr};
+
+    T rtol{1e-10};
+    T atol{1e-16};
+    int max_it{10000};
+    int check_each{1};
+    std::ptrdiff_t n_dofs{-1};
+    int iterations_{0};
+    bool verbose{false};
+    bool apply_overwrites_output{false};
+    bool flexible{false};
+    int residual_replacement{50};
+    int last_status{SFEM_FAILURE};
+    std::vector<double> history;  // relative residual per iteration
+
+    ~ConjugateGradient() { release(); }
+
+    void default_init(void *s = nullptr) {
+        stream = s;
+        blas = make_device_blas<T>(s);
+    }
+    int iterations() const override { return iterations_; }
+    void set_atol(T v) { atol = v; }
+    void set_rtol(T v) { rtol = v; }
+    void set_verbose(bool v) { verbose = v; }
+    void set_op(const SharedOperator<T> &op) override {
+        apply_op = op;
+        n_dofs = op->rows();
+    }
+    void set_preconditioner_op(const SharedOperator<T> &op) override { preconditioner_op = op; }
+    void set_max_it(int it) override { max_it = it; }
+    void set_n_dofs(std::ptrdiff_t n) override { n_dofs = n; }
+    std::ptrdiff_t rows() const override { return n_dofs; }
+    std::ptrdiff_t cols() const override { return n_dofs; }
+    ExecutionSpace execution_space() const override { return EXECUTION_SPACE_DEVICE; }
+
+    int apply(const T *const b, T *const x) override {
+        if (n_dofs < 0 || !apply_op || !blas) return SFEM_FAILURE;
+        history.clear();
+        int ret;
+        if (!preconditioner_op)
+            ret = basic(n_dofs, b, x);
+        else if (flexible)
+            ret = flexible_pcg(n_dofs, b, x);
+        else
+            ret = pcg(n_dofs, b, x);
+        last_status = ret;
+        return ret;
+    }
+
+private:
+    T *r_{nullptr}, *z_{nullptr}, *p_{nullptr}, *Ap_{nullptr}, *zold_{nullptr};
+    std::ptrdiff_t wn_{-1};
+
+    void release() {
+        for (T **q : {&r_, &z_, &p_, &Ap_, &zold_}) {
+            if (*q && blas) blas->destroy(*q);
+            *q = nullptr;
+        }
+        wn_ = -1;
+    }
+    void ensure_workspace(std::ptrdiff_t n) {
+        if (wn_ == n) return;
+        release();
+        r_ = blas->allocate(n);
+        z_ = blas->allocate(n);
+        p_ = blas->allocate(n);
+        Ap_ = blas->allocate(n);
+        zold_ = blas->allocate(n);
+        wn_ = n;
+    }
+    void zero_for_apply(std::ptrdiff_t n, T *y) {
+        if (!apply_overwrites_output) blas->zeros(n, y);
+    }
+    T update_x_r(std::ptrdiff_t n, T alpha, const T *p, const T *Ap, T *x, T *r, bool with_rtr) {
+        return (T)sfem_amd_cg_update_x_r(real_type_of<T>::value, n, alpha, p, Ap, x, r, with_rtr ? 1 : 0, stream);
+    }
+    void residual(std::ptrdiff_t n, const T *b, const T *x, T *r) {
+        zero_for_apply(n, r);
+        apply_op->apply(x, r);
+        blas->axpby(n, T(1), b, T(-1), r);
+    }
+
+    // algebra/sfem_cg.hpp:301-363
+    int basic(std::ptrdiff_t n, const T *b, T *x) {
+        ensure_workspace(n);
+        T *r = r_, *p = p_, *Ap = Ap_;
+        residual(n, b, x, r);
+        const T rtr0 = blas->dot(n, r, r);
+        const T r_norm0 = std::sqrt(rtr0);
+        iterations_ = 0;
+        if (rtr0 == 0) return SFEM_SUCCESS;
+        blas->copy(n, r, p);
+        T rtr = rtr0;
+        for (iterations_ = 0; iterations_ < max_it; iterations_++) {
+            zero_for_apply(n, Ap);
+            apply_op->apply(p, Ap);
+            const T ptAp = blas->dot(n, p, Ap);
+            const T alpha = rtr / ptAp;
+            const T rtr_new = update_x_r(n, alpha, p, Ap, x, r, true);
+            const T beta = rtr_new / rtr;
+            rtr = rtr_new;
+            blas->axpby(n, T(1), r, beta, p);
+            const T r_norm = std::sqrt(rtr_new);
+            history.push_back((double)(r_norm / r_norm0));
+            if (verbose) std::printf("%d: residual abs: %g, rel: %g\n", iterations_ + 1, (double)r_norm, (double)(r_norm / r_norm0));
+            // the reference breaks without incrementing its counter (:351-355)
+            if (r_norm < atol || rtr_new == 0 || r_norm / r_norm0 < rtol) return SFEM_SUCCESS;
+        }
+        return SFEM_FAILURE;
+    }
+
+    // algebra/sfem_cg.hpp:365-445 (stops on the preconditioned residual sqrt(r.z))
+    int pcg(std::ptrdiff_t n, const T *b, T *x) {
+        ensure_workspace(n);
+        T *r = r_, *z = z_, *p = p_, *Ap = Ap_;
+        residual(n, b, x, r);
+        const T rtr0 = blas->dot(n, r, r);
+        iterations_ = 0;
+        if (rtr0 == 0) return SFEM_SUCCESS;
+        zero_for_apply(n, z);
+        preconditioner_op->apply(r, z);
+        blas->copy(n, z, p);
+        zero_for_apply(n, Ap);
+        apply_op->apply(p, Ap);
+        const T rtz0 = blas->dot(n, r, z);
+        T rtz = rtz0;
+        if (rtz == 0) return SFEM_SUCCESS;
+        {
+            const T ptAp = blas->dot(n, p, Ap);
+            update_x_r(n, rtz / ptAp, p, Ap, x, r, false);
+        }
+        for (iterations_ = 0; iterations_ < max_it; iterations_++) {
+            zero_for_apply(n, z);
+            preconditioner_op->apply(r, z);
+            const T rtz_new = blas->dot(n, r, z);
+            const T beta = rtz_new / rtz;
+            rtz = rtz_new;
+            blas->axpby(n, T(1), z, beta, p);
+            zero_for_apply(n, Ap);
+            apply_op->apply(p, Ap);
+            const T ptAp = blas->dot(n, p, Ap);
+            const T alpha = rtz / ptAp;
+            update_x_r(n, alpha, p, Ap, x, r, false);
+            const T anorm = std::sqrt(std::abs(rtz));
+            const T rnorm = anorm / std::sqrt(rtz0);
+            history.push_back((double)rnorm);
+            if (verbose) std::printf("%d: sqrt(r.z) abs: %g, rel: %g\n", iterations_ + 1, (double)anorm, (double)rnorm);
+            if (anorm < atol || rnorm < rtol) return SFEM_SUCCESS;
+        }
+        return SFEM_FAILURE;
+    }
+
+    // Flexible PCG with true-residual stopping (extension)
+    int flexible_pcg(std::ptrdiff_t n, const T *b, T *x) {
+        ensure_workspace(n);
+        T *r = r_, *z = z_, *p = p_, *Ap = Ap_, *zold = zold_;
+        const T bnorm = blas->norm2(n, b);
+        residual(n, b, x, r);
+        T rnorm = blas->norm2(n, r);
+        iterations_ = 0;
+        const T ref = bnorm > 0 ? bnorm : T(1);
+        if (rnorm == 0 || rnorm / ref < rtol || rnorm < atol) return SFEM_SUCCESS;
+        zero_for_apply(n, z);
+        preconditioner_op->apply(r, z);
+        blas->copy(n, z, p);
+        T rtz = blas->dot(n, r, z);
+        for (iterations_ = 0; iterations_ < max_it; iterations_++) {
+            zero_for_apply(n, Ap);
+            apply_op->apply(p, Ap);
+            const T ptAp = blas->dot(n, p, Ap);
+            const T alpha = rtz / ptAp;
+            const bool replace = residual_replacement > 0 && (iterations_ + 1) % residual_replacement == 0;
+            T rtr = update_x_r(n, alpha, p, Ap, x, r, !replace);
+            if (replace) {
+                residual(n, b, x, r);
+                rtr = blas->dot(n, r, r);
+            }
+            rnorm = std::sqrt(rtr);
+            history.push_back((double)(rnorm / ref));
+            if (verbose) std::printf("%d: residual rel: %g\n", iterations_ + 1, (double)(rnorm / ref));
+            if (rnorm / ref < rtol || rnorm < atol) {
+                // confirm with the true residual (recomputed in the operator's precision)
+                residual(n, b, x, r);
+                const T true_norm = blas->norm2(n, r);
+                if (true_norm / ref < rtol || true_norm < atol) {
+                    iterations_++;
+                    if (!history.empty()) history.back() = (double)(true_norm / ref);
+                    return SFEM_SUCCESS;
+                }
+                rnorm = true_norm;
+            }
+            blas->copy(n, z, zold);
+            zero_for_apply(n, z);
+            preconditioner_op->apply(r, z);
+            const T rtz_new = blas->dot(n, r, z);
+            const T rtzold = blas->dot(n, r, zold);
+            const T beta = (rtz_new - rtzold) / rtz;
+            rtz = rtz_new;
+            blas->axpby(n, T(1), z, beta, p);
+        }
+        return SFEM_FAILURE;
+    }
+};
+
+}  // namespace sfem

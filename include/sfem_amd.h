@@ -1,0 +1,226 @@
+/*
+ * sfem_amd.h -- C ABI of the MI355X (gfx950) matrix-free FEM + multigrid library.
+ *
+ * Everything here is extern "C", takes plain pointers and sizes and returns
+ * SFEM_SUCCESS (0) or SFEM_FAILURE (1) as the reference does (base/sfem_base.hpp:12-13).
+ * Host arrays use the reference's layouts: connectivity SoA int32 [nxe][stride]
+ * (the reference's `idx_t **elements` with elements[v] = &elements[v*stride]),
+ * FFF float32 AoS [E][6], adjugate float32 AoS [E][9], determinant float32 [E].
+ * Vectors passed to apply-like calls are DEVICE pointers; element-type vectors of
+ * dofs are interleaved [node*block_size + component] like the reference.
+ *
+ * Handles:
+ *   sfem_amd_mesh  -- connectivity resident in HBM in the device layout
+ *                     (element-major for sshex8, SoA for tet4/hex8);
+ *   sfem_amd_op    -- anything that maps x -> y: an FE operator, a transfer, a
+ *                     solver or multigrid used as an operator
+ *                     (the reference's sfem::Operator<T>, algebra/sfem_Operator.hpp:12-33).
+ * Operator apply ACCUMULATES, y += A x (frontend/sfem_Function.cpp:481-496); solvers
+ * use x as initial guess and output (algebra/sfem_cg.hpp:207-213).
+ * `stream` is a hipStream_t passed as void*; NULL is the default stream.  Calls do
+ * not synchronise except where they return a host scalar (dot, norm, solve).
+ * Objects are not re-entrant: calls that overlap on one handle race.  A mesh must outlive
+ * the operators and transfers created on it; operators built from other operators (solvers,
+ * constrained operators, multigrid) keep shared references to them.
+ */
+#ifndef SFEM_AMD_H
+#define SFEM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFEM_SUCCESS 0
+#define SFEM_FAILURE 1
+
+/* element types: numeric values of the reference's smesh::ElemType for TET4/HEX8;
+ * SSHEX8 is HEX8 refined to `level` (frontend/sfem_FunctionSpace.hpp:17-78). */
+enum sfem_amd_element_type { SFEM_AMD_TET4 = 4, SFEM_AMD_HEX8 = 8, SFEM_AMD_SSHEX8 = 108 };
+
+/* value types of an operator's x/y: FP64 (the plain method) or FP32 (chosen form 1) */
+enum sfem_amd_real_type { SFEM_AMD_FLOAT64 = 0, SFEM_AMD_FLOAT32 = 1 };
+
+typedef struct sfem_amd_mesh sfem_amd_mesh;
+typedef struct sfem_amd_op sfem_amd_op;
+
+/* ------------------------------------------------------------------ library */
+/* Last error message of this thread ("" if none). */
+const char *sfem_amd_last_error(void);
+/* Number of visible GPUs; 0 when no HIP device is present. */
+int sfem_amd_device_count(void);
+int sfem_amd_device_synchronize(void);
+
+/* ------------------------------------------------------------------ device memory */
+void *sfem_amd_malloc(size_t bytes);
+int sfem_amd_free(void *ptr);
+int sfem_amd_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int sfem_amd_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int sfem_amd_memset(void *dst, int value, size_t bytes, void *stream);
+
+/* ------------------------------------------------------------------ host mesh setup */
+/* python/sfem/mesh/box_mesh.py:479-700: nx,ny,nz node counts, node id yi*nx*nz+xi*nz+zi,
+ * hex8 (8 rows) or 6 tets per cube (4 rows). elements [nv][E], points [3][N] float32. */
+ptrdiff_t sfem_amd_box_nnodes(int nx, int ny, int nz);
+ptrdiff_t sfem_amd_box_nelements(int nx, int ny, int nz, int element_type);
+int sfem_amd_box_mesh(int nx, int ny, int nz, double w, double h, double t, int element_type,
+                      int32_t *elements, float *points);
+/* Hex8 -> sshex8 level L with hierarchical numbering (derefined levels are prefixes);
+ * replaces smesh::to_semistructured (ssmg/tests/sfem_SSGMGTest.cpp:77-88).
+ * With elements == NULL returns the node count only. */
+ptrdiff_t sfem_amd_sshex8_refine(int level, ptrdiff_t nelements, ptrdiff_t nnodes,
+                                 const int32_t *macro_elements, const float *macro_points,
+                                 int32_t *elements, float *points);
+/* Geometry caches, evaluated in float64 and rounded once to float32:
+ * FFF = J^-1 J^-T det (tet4: /6), adjugate and det of J at the element centre.
+ * Replaces smesh::FFF / JacobianAdjugateAndDeterminant (frontend/ops/sfem_Laplacian.cpp:88,
+ * frontend/ops/sfem_LinearElasticity.cpp:276-278).  For SSHEX8 the macro element's corners
+ * are used (level given). */
+int sfem_amd_fff(int element_type, int level, ptrdiff_t nelements, ptrdiff_t stride,
+                 const int32_t *elements, ptrdiff_t nnodes, const float *points, float *fff);
+int sfem_amd_adjugate_det(int element_type, int level, ptrdiff_t nelements, ptrdiff_t stride,
+                          const int32_t *elements, ptrdiff_t nnodes, const float *points,
+                          float *adjugate, float *det);
+
+/* ------------------------------------------------------------------ device meshes */
+/* Upload connectivity (host, reference layout) and keep it in the device layout. */
+int sfem_amd_mesh_create(int element_type, int level, ptrdiff_t nelements, ptrdiff_t stride,
+                         const int32_t *elements, ptrdiff_t nnodes, sfem_amd_mesh **out);
+/* frontend/sfem_API.hpp:1412-1475 sshex8_derefine_element_connectivity, on device.
+ * level 1 yields a HEX8 mesh in the standard corner order. */
+int sfem_amd_mesh_derefine(const sfem_amd_mesh *fine, int level, sfem_amd_mesh **out);
+ptrdiff_t sfem_amd_mesh_nnodes(const sfem_amd_mesh *m);
+ptrdiff_t sfem_amd_mesh_nelements(const sfem_amd_mesh *m);
+int sfem_amd_mesh_level(const sfem_amd_mesh *m);
+int sfem_amd_mesh_element_type(const sfem_amd_mesh *m);
+/* Copy the connectivity back in the reference layout ([nxe][nelements], host). */
+int sfem_amd_mesh_download(const sfem_amd_mesh *m, int32_t *elements);
+void sfem_amd_mesh_destroy(sfem_amd_mesh *m);
+
+/* ------------------------------------------------------------------ FE operators */
+/* Laplacian apply: the device counterpart of laplacian_apply_opt (operators/laplacian.cpp:197-236),
+ * tet4_laplacian_apply_opt (operators/tet4/tet4_laplacian.cpp:237-285),
+ * hex8_laplacian_apply_opt (operators/hex8/hex8_laplacian.cpp:309-345),
+ * affine_sshex8_laplacian_stencil_apply_fff (operators/sshex8/sshex8_laplacian.cpp:698-769)
+ * and cu_laplacian_apply (operators/cuda/cu_laplacian.hpp).
+ * fff: host float32 [E][6]; coeff: host float32 [E] per-element multiplier or NULL.
+ * real_type selects FP64 or FP32 vectors; FP32 keeps geometry in BF16 (chosen form 1). */
+int sfem_amd_laplacian_create(const sfem_amd_mesh *mesh, const float *fff, const float *coeff,
+                              int real_type, sfem_amd_op **out);
+/* Linear elasticity (3 dofs/node, interleaved): linear_elasticity_apply_adjugate_{aos,soa}
+ * (operators/linear_elasticity.cpp:303-392), affine_hex8_linear_elasticity_apply_adjugate_soa
+ * (operators/hex8/hex8_linear_elasticity.cpp:625-760),
+ * affine_sshex8_linear_elasticity_apply_macro_adjugate
+ * (operators/sshex8/sshex8_linear_elasticity.cpp:275-425), cu_linear_elasticity_apply
+ * (operators/cuda/cu_linear_elasticity.hpp).  mu, lambda: Lame parameters; coeff scales both. */
+int sfem_amd_linear_elasticity_create(const sfem_amd_mesh *mesh, const float *adjugate,
+                                      const float *det, double mu, double lambda,
+                                      const float *coeff, int real_type, sfem_amd_op **out);
+/* y += A x (device vectors of the op's real type). */
+int sfem_amd_op_apply(sfem_amd_op *op, const void *x, void *y, void *stream);
+/* d += diag(A): laplacian_diag (operators/laplacian.cpp:156-195),
+ * affine_sshex8_laplacian_diag (operators/sshex8/sshex8_laplacian.cpp:333-435),
+ * affine_sshex8_linear_elasticity_diag (sshex8_linear_elasticity.cpp:971-). */
+int sfem_amd_op_diag(sfem_amd_op *op, void *d, void *stream);
+/* d += sym 3x3 block diagonal [node][6] = (xx,xy,xz,yy,yz,zz) of an elasticity op:
+ * affine_sshex8_linear_elasticity_block_diag_sym (sshex8_linear_elasticity.cpp:1102-). */
+int sfem_amd_op_block_diag_sym(sfem_amd_op *op, void *d, void *stream);
+ptrdiff_t sfem_amd_op_rows(const sfem_amd_op *op);
+ptrdiff_t sfem_amd_op_cols(const sfem_amd_op *op);
+int sfem_amd_op_real_type(const sfem_amd_op *op);
+void sfem_amd_op_destroy(sfem_amd_op *op);
+
+/* Function::apply with constraints (frontend/sfem_Function.cpp:481-496): y += A x, then
+ * y[c] = x[c] on constrained dofs c = nodes[i]*block_size + component for each listed
+ * component (components: array of n_components ints; NULL = all block components).
+ * nodes: host int32. */
+int sfem_amd_constrained_op_create(sfem_amd_op *op, ptrdiff_t n_nodes, const int32_t *nodes,
+                                   int block_size, int n_components, const int *components,
+                                   sfem_amd_op **out);
+/* x[c] = value on the constrained dofs of a constrained op (dirichlet.cpp:95-106). */
+int sfem_amd_constrained_op_set_value(sfem_amd_op *constrained, double value, void *x, void *stream);
+/* diag-style vectors: x[c] = value (f->set_value_to_constrained_dofs). */
+
+/* ------------------------------------------------------------------ transfers */
+/* sshex8_prolongate (operators/sshex8/sshex8_interpolate.cpp:259-632) between nested
+ * levels of one macro mesh, or sshex8_hierarchical_prolongation (:129-258) when `coarse`
+ * is the level-1 HEX8 mesh.  Overwrites the fine vector.  Optional zero constraints on the
+ * output (create_gmg_data's apply_zero_constraints, algebra/mg/sfem_GeometricMultigrid.cpp:39-66):
+ * constrained: a constrained op on the fine level (or NULL). */
+int sfem_amd_prolongation_create(const sfem_amd_mesh *coarse, const sfem_amd_mesh *fine,
+                                 int block_size, int real_type, const sfem_amd_op *constrained,
+                                 sfem_amd_op **out);
+/* sshex8_restrict (:633-767) / sshex8_hierarchical_restriction (:12-128): accumulates into
+ * the coarse vector (the caller zeroes it, as Multigrid::cycle does); zeroes the coarse
+ * constrained dofs afterwards when `constrained` (a coarse-level constrained op) is given. */
+int sfem_amd_restriction_create(const sfem_amd_mesh *fine, const sfem_amd_mesh *coarse,
+                                int block_size, int real_type, const sfem_amd_op *constrained,
+                                sfem_amd_op **out);
+
+/* ------------------------------------------------------------------ BLAS-1 (device) */
+/* The reference's BLAS<T> (algebra/sfem_tpl_blas.hpp:11-35) on device vectors of real type
+ * `rt` (SFEM_AMD_FLOAT64 / SFEM_AMD_FLOAT32); scalars are passed as double.  Reductions are
+ * deterministic (fixed two-stage order, FP64 accumulation) and return host scalars
+ * (synchronising the stream). */
+int sfem_amd_blas_axpby(int rt, ptrdiff_t n, double alpha, const void *x, double beta, void *y, void *stream);
+int sfem_amd_blas_zaxpby(int rt, ptrdiff_t n, double alpha, const void *x, double beta, const void *y, void *z,
+                         void *stream);
+int sfem_amd_blas_scal(int rt, ptrdiff_t n, double alpha, void *x, void *stream);
+int sfem_amd_blas_values(int rt, ptrdiff_t n, double value, void *x, void *stream);
+int sfem_amd_blas_copy(int rt, ptrdiff_t n, const void *x, void *y, void *stream);
+int sfem_amd_blas_reciprocal(int rt, ptrdiff_t n, double alpha, void *x, void *stream);  /* x = alpha / x */
+int sfem_amd_blas_xypaz(int rt, ptrdiff_t n, const void *x, const void *y, double alpha, void *z,
+                        void *stream);                                                   /* z = x*y + alpha*z */
+double sfem_amd_blas_dot(int rt, ptrdiff_t n, const void *x, const void *y, void *stream);
+double sfem_amd_blas_norm2(int rt, ptrdiff_t n, const void *x, void *stream);
+/* y = (real type rt_to) x */
+int sfem_amd_blas_cast(int rt_from, ptrdiff_t n, const void *x, int rt_to, void *y, void *stream);
+/* Fused CG step (algebra/sfem_cg.hpp:26-72): x += alpha p, r -= alpha Ap; returns (r,r)
+ * when with_rtr != 0 (synchronising), else 0. */
+double sfem_amd_cg_update_x_r(int rt, ptrdiff_t n, double alpha, const void *p, const void *Ap, void *x, void *r,
+                              int with_rtr, void *stream);
+/* Chebyshev3 iteration_update (algebra/sfem_Chebyshev3.hpp:37-52): p = beta p - rhs + Ax; x -= alpha p */
+int sfem_amd_cheb3_update(int rt, ptrdiff_t n, double alpha, double beta, const void *rhs, const void *Ax, void *p,
+                          void *x, void *stream);
+
+/* ------------------------------------------------------------------ solvers as operators */
+/* sfem::ConjugateGradient<real_t> (algebra/sfem_cg.hpp:80-446): plain CG, or left-PCG with
+ * a fixed preconditioner when `precond` != NULL.  flexible != 0 selects flexible PCG
+ * (beta from r_new.(z_new - z_old)) with the true residual ||b - A x|| / ||b|| as stopping
+ * test (the reference has no flexible CG).  Applying the returned op solves A x = b
+ * (x: initial guess and output).  The op keeps shared references to A and precond. */
+int sfem_amd_cg_create(sfem_amd_op *A, sfem_amd_op *precond, int flexible, double rtol,
+                       double atol, int max_it, sfem_amd_op **out);
+/* sfem::Chebyshev3<real_t> (algebra/sfem_Chebyshev3.hpp:57-303) of degree `max_it`; eig_max
+ * from PowerMethod (algebra/sfem_PowerMethod.hpp:18-55) started from ones, with
+ * eigen_solver_max_it iterations (rounded up to odd, like the reference) and tolerance. */
+int sfem_amd_cheb3_create(sfem_amd_op *A, int max_it, double scale_eig_max, double scale_eig_min,
+                          int eigen_solver_max_it, double eigen_solver_tol, int initial_guess_zero,
+                          sfem_amd_op **out);
+double sfem_amd_cheb3_eig_max(const sfem_amd_op *cheb);
+/* StationaryIteration with (scaled) Jacobi: x += omega D^-1 (b - A x), `steps` sweeps
+ * (algebra/sfem_Stationary.hpp:50-80, algebra/sfem_ShiftableJacobi.hpp).  diag: device. */
+int sfem_amd_jacobi_create(sfem_amd_op *A, const void *diag, double omega, int steps,
+                           sfem_amd_op **out);
+/* sfem::Multigrid<real_t> (algebra/sfem_Multigrid.hpp:27-347).  Level 0 is the finest;
+ * add_level(op, smoother_or_solver, prolongation (NULL on level 0), restriction (NULL on
+ * the coarsest)). */
+int sfem_amd_mg_create(int real_type, int cycle_type, int max_it, double atol, sfem_amd_op **out);
+int sfem_amd_mg_add_level(sfem_amd_op *mg, sfem_amd_op *op, sfem_amd_op *smoother,
+                          sfem_amd_op *prolongation, sfem_amd_op *restriction);
+/* An FP64 operator that runs an FP32 operator: casts x, zeroes and applies, casts back
+ * (chosen form 1: the FP32 preconditioner inside FP64 flexible CG). */
+int sfem_amd_precision_bridge_create(sfem_amd_op *op32, sfem_amd_op **out);
+/* Iterations of the last solve of a solver op; -1 for non-solvers. */
+int sfem_amd_op_iterations(const sfem_amd_op *op);
+/* Residual history of the last CG solve (relative norms), up to `cap` entries. */
+int sfem_amd_cg_history(const sfem_amd_op *op, double *out, int cap);
+/* Status of the last solve (SFEM_SUCCESS if converged). */
+int sfem_amd_op_last_status(const sfem_amd_op *op);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

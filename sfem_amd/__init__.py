@@ -1,0 +1,371 @@
+"""Python view of the sfem_amd C ABI (include/sfem_amd.h).
+
+Mirrors the reference's operator/solver interface (pysfem, python/bindings/pysfem.cpp:585-719):
+Operator.apply accumulates y += A x, solvers take x as initial guess and output.  Device
+vectors are torch tensors on the GPU (torch is only the allocator here); every computation
+runs in the library's own HIP kernels.  There is no CPU fallback: importing this module
+raises if the compiled library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsfem_amd.so")
+
+TET4, HEX8, SSHEX8 = 4, 8, 108
+FLOAT64, FLOAT32 = 0, 1
+SUCCESS, FAILURE = 0, 1
+
+_vp, _pd, _i, _d = C.c_void_p, C.c_ssize_t, C.c_int, C.c_double
+
+# name: (restype, argtypes) -- exactly the declarations of include/sfem_amd.h
+SIGNATURES = {
+    "sfem_amd_last_error": (C.c_char_p, []),
+    "sfem_amd_device_count": (_i, []),
+    "sfem_amd_device_synchronize": (_i, []),
+    "sfem_amd_malloc": (_vp, [C.c_size_t]),
+    "sfem_amd_free": (_i, [_vp]),
+    "sfem_amd_memcpy_h2d": (_i, [_vp, _vp, C.c_size_t]),
+    "sfem_amd_memcpy_d2h": (_i, [_vp, _vp, C.c_size_t]),
+    "sfem_amd_memset": (_i, [_vp, _i, C.c_size_t, _vp]),
+    "sfem_amd_box_nnodes": (_pd, [_i, _i, _i]),
+    "sfem_amd_box_nelements": (_pd, [_i, _i, _i, _i]),
+    "sfem_amd_box_mesh": (_i, [_i, _i, _i, _d, _d, _d, _i, _vp, _vp]),
+    "sfem_amd_sshex8_refine": (_pd, [_i, _pd, _pd, _vp, _vp, _vp, _vp]),
+    "sfem_amd_fff": (_i, [_i, _i, _pd, _pd, _vp, _pd, _vp, _vp]),
+    "sfem_amd_adjugate_det": (_i, [_i, _i, _pd, _pd, _vp, _pd, _vp, _vp, _vp]),
+    "sfem_amd_mesh_create": (_i, [_i, _i, _pd, _pd, _vp, _pd, C.POINTER(_vp)]),
+    "sfem_amd_mesh_derefine": (_i, [_vp, _i, C.POINTER(_vp)]),
+    "sfem_amd_mesh_nnodes": (_pd, [_vp]),
+    "sfem_amd_mesh_nelements": (_pd, [_vp]),
+    "sfem_amd_mesh_level": (_i, [_vp]),
+    "sfem_amd_mesh_element_type": (_i, [_vp]),
+    "sfem_amd_mesh_download": (_i, [_vp, _vp]),
+    "sfem_amd_mesh_destroy": (None, [_vp]),
+    "sfem_amd_laplacian_create": (_i, [_vp, _vp, _vp, _i, C.POINTER(_vp)]),
+    "sfem_amd_linear_elasticity_create": (_i, [_vp, _vp, _vp, _d, _d, _vp, _i, C.POINTER(_vp)]),
+    "sfem_amd_op_apply": (_i, [_vp, _vp, _vp, _vp]),
+    "sfem_amd_op_diag": (_i, [_vp, _vp, _vp]),
+    "sfem_amd_op_block_diag_sym": (_i, [_vp, _vp, _vp]),
+    "sfem_amd_op_rows": (_pd, [_vp]),
+    "sfem_amd_op_cols": (_pd, [_vp]),
+    "sfem_amd_op_real_type": (_i, [_vp]),
+    "sfem_amd_op_destroy": (None, [_vp]),
+    "sfem_amd_constrained_op_create": (_i, [_vp, _pd, _vp, _i, _i, _vp, C.POINTER(_vp)]),
+    "sfem_amd_constrained_op_set_value": (_i, [_vp, _d, _vp, _vp]),
+    "sfem_amd_prolongation_create": (_i, [_vp, _vp, _i, _i, _vp, C.POINTER(_vp)]),
+    "sfem_amd_restriction_create": (_i, [_vp, _vp, _i, _i, _vp, C.POINTER(_vp)]),
+    "sfem_amd_blas_axpby": (_i, [_i, _pd, _d, _vp, _d, _vp, _vp]),
+    "sfem_amd_blas_zaxpby": (_i, [_i, _pd, _d, _vp, _d, _vp, _vp, _vp]),
+    "sfem_amd_blas_scal": (_i, [_i, _pd, _d, _vp, _vp]),
+    "sfem_amd_blas_values": (_i, [_i, _pd, _d, _vp, _vp]),
+    "sfem_amd_blas_copy": (_i, [_i, _pd, _vp, _vp, _vp]),
+    "sfem_amd_blas_reciprocal": (_i, [_i, _pd, _d, _vp, _vp]),
+    "sfem_amd_blas_xypaz": (_i, [_i, _pd, _vp, _vp, _d, _vp, _vp]),
+    "sfem_amd_blas_dot": (_d, [_i, _pd, _vp, _vp, _vp]),
+    "sfem_amd_blas_norm2": (_d, [_i, _pd, _vp, _vp]),
+    "sfem_amd_blas_cast": (_i, [_i, _pd, _vp, _i, _vp, _vp]),
+    "sfem_amd_cg_update_x_r": (_d, [_i, _pd, _d, _vp, _vp, _vp, _vp, _i, _vp]),
+    "sfem_amd_cheb3_update": (_i, [_i, _pd, _d, _d, _vp, _vp, _vp, _vp, _vp]),
+    "sfem_amd_cg_create": (_i, [_vp, _vp, _i, _d, _d, _i, C.POINTER(_vp)]),
+    "sfem_amd_cheb3_create": (_i, [_vp, _i, _d, _d, _i, _d, _i, C.POINTER(_vp)]),
+    "sfem_amd_cheb3_eig_max": (_d, [_vp]),
+    "sfem_amd_jacobi_create": (_i, [_vp, _vp, _d, _i, C.POINTER(_vp)]),
+    "sfem_amd_mg_create": (_i, [_i, _i, _i, _d, C.POINTER(_vp)]),
+    "sfem_amd_mg_add_level": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "sfem_amd_precision_bridge_create": (_i, [_vp, C.POINTER(_vp)]),
+    "sfem_amd_op_iterations": (_i, [_vp]),
+    "sfem_amd_cg_history": (_i, [_vp, _vp, _i]),
+    "sfem_amd_op_last_status": (_i, [_vp]),
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"sfem_amd: compiled library not found at {LIB_PATH}; build it with "
+            "`make -C sfem_amd` or __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+class SfemError(RuntimeError):
+    pass
+
+
+def _check(ret, what):
+    if ret != SUCCESS:
+        raise SfemError(f"{what}: {lib.sfem_amd_last_error().decode()}")
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(_vp) if a is not None else None
+
+
+def _host(a, dtype):
+    return None if a is None else np.ascontiguousarray(a, dtype=dtype)
+
+
+# ---------------------------------------------------------------- host mesh setup
+def box_mesh(nx, ny, nz, w=1.0, h=1.0, t=1.0, element_type=HEX8):
+    """box_mesh.py:479-700 layout: (elements [nv][E] int32, points [3][N] float32)."""
+    nn = lib.sfem_amd_box_nnodes(nx, ny, nz)
+    ne = lib.sfem_amd_box_nelements(nx, ny, nz, element_type)
+    el = np.zeros((4 if element_type == TET4 else 8, ne), np.int32)
+    pts = np.zeros((3, nn), np.float32)
+    _check(lib.sfem_amd_box_mesh(nx, ny, nz, w, h, t, element_type, _np_ptr(el), _np_ptr(pts)), "box_mesh")
+    return el, pts
+
+
+def sshex8_refine(level, macro_elements, macro_points):
+    me = _host(macro_elements, np.int32)
+    mp = _host(macro_points, np.float32)
+    n = lib.sfem_amd_sshex8_refine(level, me.shape[1], mp.shape[1], _np_ptr(me), _np_ptr(mp), None, None)
+    if n < 0:
+        raise SfemError(lib.sfem_amd_last_error().decode())
+    el = np.zeros(((level + 1) ** 3, me.shape[1]), np.int32)
+    pts = np.zeros((3, n), np.float32)
+    n2 = lib.sfem_amd_sshex8_refine(level, me.shape[1], mp.shape[1], _np_ptr(me), _np_ptr(mp), _np_ptr(el),
+                                    _np_ptr(pts))
+    if n2 != n:
+        raise SfemError(lib.sfem_amd_last_error().decode())
+    return el, pts
+
+
+def fff(element_type, level, elements, points):
+    el = _host(elements, np.int32)
+    p = _host(points, np.float32)
+    out = np.zeros((el.shape[1], 6), np.float32)
+    _check(lib.sfem_amd_fff(element_type, level, el.shape[1], el.shape[1], _np_ptr(el), p.shape[1], _np_ptr(p),
+                            _np_ptr(out)), "fff")
+    return out
+
+
+def adjugate_det(element_type, level, elements, points):
+    el = _host(elements, np.int32)
+    p = _host(points, np.float32)
+    adj = np.zeros((el.shape[1], 9), np.float32)
+    det = np.zeros(el.shape[1], np.float32)
+    _check(lib.sfem_amd_adjugate_det(element_type, level, el.shape[1], el.shape[1], _np_ptr(el), p.shape[1],
+                                     _np_ptr(p), _np_ptr(adj), _np_ptr(det)), "adjugate_det")
+    return adj, det
+
+
+# ---------------------------------------------------------------- device objects
+def _dptr(t):
+    """Device pointer of a torch tensor (must be contiguous and on the GPU)."""
+    if t is None:
+        return None
+    if not (hasattr(t, "is_cuda") and t.is_cuda and t.is_contiguous()):
+        raise SfemError("expected a contiguous GPU tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    return C.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+
+class Mesh:
+    """Device-resident connectivity (sfem_amd_mesh)."""
+
+    def __init__(self, element_type, level, elements, nnodes, _handle=None):
+        self._keep = []
+        if _handle is not None:
+            self.h = _handle
+        else:
+            el = _host(elements, np.int32)
+            h = _vp()
+            _check(lib.sfem_amd_mesh_create(element_type, level, el.shape[1], el.shape[1], _np_ptr(el), nnodes,
+                                            C.byref(h)), "mesh_create")
+            self.h = h
+        self.element_type = lib.sfem_amd_mesh_element_type(self.h)
+        self.level = lib.sfem_amd_mesh_level(self.h)
+        self.nnodes = lib.sfem_amd_mesh_nnodes(self.h)
+        self.nelements = lib.sfem_amd_mesh_nelements(self.h)
+
+    def derefine(self, level):
+        h = _vp()
+        _check(lib.sfem_amd_mesh_derefine(self.h, level, C.byref(h)), "mesh_derefine")
+        m = Mesh(None, None, None, None, _handle=h)
+        m._keep.append(self)
+        return m
+
+    def download(self):
+        nv = 8 if self.element_type == HEX8 else 4 if self.element_type == TET4 else (self.level + 1) ** 3
+        out = np.zeros((nv, self.nelements), np.int32)
+        _check(lib.sfem_amd_mesh_download(self.h, _np_ptr(out)), "mesh_download")
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and lib is not None:
+            lib.sfem_amd_mesh_destroy(self.h)
+            self.h = None
+
+
+class Operator:
+    """An sfem_amd_op handle: FE operator, transfer, constrained operator or solver."""
+
+    def __init__(self, handle, keep=()):
+        self.h = handle
+        self._keep = list(keep)
+
+    @property
+    def rows(self):
+        return lib.sfem_amd_op_rows(self.h)
+
+    @property
+    def cols(self):
+        return lib.sfem_amd_op_cols(self.h)
+
+    @property
+    def real_type(self):
+        return lib.sfem_amd_op_real_type(self.h)
+
+    def _check_vec(self, t, n):
+        import torch
+        want = torch.float64 if self.real_type == FLOAT64 else torch.float32
+        if t.dtype != want or t.numel() != n:
+            raise SfemError(f"vector must be {want} with {n} entries (got {t.dtype}, {t.numel()})")
+
+    def apply(self, x, y, stream=None):
+        self._check_vec(x, self.cols)
+        self._check_vec(y, self.rows)
+        _check(lib.sfem_amd_op_apply(self.h, _dptr(x), _dptr(y), _stream_ptr(stream)), "op_apply")
+        return y
+
+    def diag(self, d, stream=None):
+        self._check_vec(d, self.rows)
+        _check(lib.sfem_amd_op_diag(self.h, _dptr(d), _stream_ptr(stream)), "op_diag")
+        return d
+
+    def block_diag_sym(self, d, stream=None):
+        _check(lib.sfem_amd_op_block_diag_sym(self.h, _dptr(d), _stream_ptr(stream)), "op_block_diag_sym")
+        return d
+
+    @property
+    def iterations(self):
+        return lib.sfem_amd_op_iterations(self.h)
+
+    @property
+    def last_status(self):
+        return lib.sfem_amd_op_last_status(self.h)
+
+    def history(self):
+        n = lib.sfem_amd_cg_history(self.h, None, 0)
+        out = np.zeros(max(n, 0))
+        if n > 0:
+            lib.sfem_amd_cg_history(self.h, _np_ptr(out), n)
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and lib is not None:
+            lib.sfem_amd_op_destroy(self.h)
+            self.h = None
+
+
+def _new_op(fn, *args, keep=()):
+    h = _vp()
+    _check(getattr(lib, fn)(*args, C.byref(h)), fn)
+    return Operator(h, keep)
+
+
+def laplacian(mesh, fff_aos, coeff=None, real_type=FLOAT64):
+    f = _host(fff_aos, np.float32)
+    c = _host(coeff, np.float32)
+    return _new_op("sfem_amd_laplacian_create", mesh.h, _np_ptr(f), _np_ptr(c), real_type, keep=[mesh])
+
+
+def linear_elasticity(mesh, adjugate, det, mu, lam, coeff=None, real_type=FLOAT64):
+    a = _host(adjugate, np.float32)
+    d = _host(det, np.float32)
+    c = _host(coeff, np.float32)
+    return _new_op("sfem_amd_linear_elasticity_create", mesh.h, _np_ptr(a), _np_ptr(d), mu, lam, _np_ptr(c),
+                   real_type, keep=[mesh])
+
+
+def constrained(op, nodes, block_size=1, components=None):
+    n = _host(nodes, np.int32)
+    comps = None if components is None else np.ascontiguousarray(components, np.int32)
+    nc = 0 if comps is None else len(comps)
+    return _new_op("sfem_amd_constrained_op_create", op.h, len(n), _np_ptr(n), block_size, nc, _np_ptr(comps),
+                   keep=[op])
+
+
+def set_constrained_value(cop, value, x, stream=None):
+    _check(lib.sfem_amd_constrained_op_set_value(cop.h, value, _dptr(x), _stream_ptr(stream)), "set_value")
+
+
+def prolongation(coarse, fine, block_size=1, real_type=FLOAT64, constraints=None):
+    return _new_op("sfem_amd_prolongation_create", coarse.h, fine.h, block_size, real_type,
+                   constraints.h if constraints else None, keep=[coarse, fine, constraints])
+
+
+def restriction(fine, coarse, block_size=1, real_type=FLOAT64, constraints=None):
+    return _new_op("sfem_amd_restriction_create", fine.h, coarse.h, block_size, real_type,
+                   constraints.h if constraints else None, keep=[coarse, fine, constraints])
+
+
+def cg(A, precond=None, flexible=False, rtol=1e-10, atol=1e-16, max_it=10000):
+    return _new_op("sfem_amd_cg_create", A.h, precond.h if precond else None, int(flexible), rtol, atol, max_it,
+                   keep=[A, precond])
+
+
+def cheb3(A, max_it=3, scale_eig_max=1.0, scale_eig_min=0.06, eigen_solver_max_it=1000, eigen_solver_tol=1e-6,
+          initial_guess_zero=False):
+    return _new_op("sfem_amd_cheb3_create", A.h, max_it, scale_eig_max, scale_eig_min, eigen_solver_max_it,
+                   eigen_solver_tol, int(initial_guess_zero), keep=[A])
+
+
+def cheb3_eig_max(op):
+    return lib.sfem_amd_cheb3_eig_max(op.h)
+
+
+def jacobi(A, diag, omega=1.0, steps=3):
+    return _new_op("sfem_amd_jacobi_create", A.h, _dptr(diag), omega, steps, keep=[A])
+
+
+class Multigrid(Operator):
+    """sfem::Multigrid (algebra/sfem_Multigrid.hpp:27-347); level 0 is the finest."""
+
+    def __init__(self, real_type=FLOAT64, cycle_type=1, max_it=10, atol=1e-10):
+        h = _vp()
+        _check(lib.sfem_amd_mg_create(real_type, cycle_type, max_it, atol, C.byref(h)), "mg_create")
+        super().__init__(h)
+
+    def add_level(self, op, smoother, prolongation=None, restriction=None):
+        _check(lib.sfem_amd_mg_add_level(self.h, op.h, smoother.h, prolongation.h if prolongation else None,
+                                         restriction.h if restriction else None), "mg_add_level")
+        self._keep += [op, smoother, prolongation, restriction]
+
+
+def precision_bridge(op32):
+    return _new_op("sfem_amd_precision_bridge_create", op32.h, keep=[op32])
+
+
+def dot(x, y, stream=None):
+    rt = FLOAT64 if x.element_size() == 8 else FLOAT32
+    return lib.sfem_amd_blas_dot(rt, x.numel(), _dptr(x), _dptr(y), _stream_ptr(stream))
+
+
+def norm2(x, stream=None):
+    rt = FLOAT64 if x.element_size() == 8 else FLOAT32
+    return lib.sfem_amd_blas_norm2(rt, x.numel(), _dptr(x), _stream_ptr(stream))
+
+
+def device_count():
+    return lib.sfem_amd_device_count()
+
+
+def synchronize():
+    _check(lib.sfem_amd_device_synchronize(), "synchronize")

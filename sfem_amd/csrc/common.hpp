@@ -1,0 +1,119 @@
+// Shared helpers for the gfx950 kernels and the host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "sfem_amd.h"
+
+namespace sfem_amd {
+
+// ----------------------------------------------------------------- errors
+void set_last_error(const std::string &msg);
+
+#define SFEM_AMD_HIP_CHECK(call)                                                               \
+    do {                                                                                       \
+        hipError_t err__ = (call);                                                             \
+        if (err__ != hipSuccess) {                                                             \
+            ::sfem_amd::set_last_error(std::string(#call " failed: ") + hipGetErrorString(err__) + \
+                                       " at " __FILE__ ":" + std::to_string(__LINE__));        \
+            return SFEM_FAILURE;                                                               \
+        }                                                                                      \
+    } while (0)
+
+#define SFEM_AMD_REQUIRE(cond, msg)                                \
+    do {                                                           \
+        if (!(cond)) {                                             \
+            ::sfem_amd::set_last_error(std::string("sfem_amd: ") + (msg)); \
+            return SFEM_FAILURE;                                   \
+        }                                                          \
+    } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ----------------------------------------------------------------- structured helpers
+// operators/sshex8/sshex8.hpp:18-24: local id = z*(L+1)^2 + y*(L+1) + x
+__host__ __device__ constexpr inline int lidx(int L, int x, int y, int z) { return (z * (L + 1) + y) * (L + 1) + x; }
+__host__ __device__ constexpr inline int nxe_of(int L) { return (L + 1) * (L + 1) * (L + 1); }
+
+// hex8 corner order (python/sfem/mesh/box_mesh.py:520-534, sshex8.hpp:53-62):
+// bit pattern of corner v: x = (v==1||v==2||v==5||v==6), y = (v==2||v==3||v==6||v==7), z = v>=4
+__host__ __device__ constexpr inline int hex8_cx(int v) { return (v == 1 || v == 2 || v == 5 || v == 6) ? 1 : 0; }
+__host__ __device__ constexpr inline int hex8_cy(int v) { return (v == 2 || v == 3 || v == 6 || v == 7) ? 1 : 0; }
+__host__ __device__ constexpr inline int hex8_cz(int v) { return v >= 4 ? 1 : 0; }
+// inverse: (x,y,z) bits -> hex8 corner id
+__host__ __device__ constexpr inline int hex8_corner(int x, int y, int z) {
+    return z * 4 + (y ? (x ? 2 : 3) : (x ? 1 : 0));
+}
+
+// 1-D Q1 integrals on [0,1]: mass, stiffness, and C[i][j] = int psi_i' psi_j.
+__host__ __device__ inline double q1_mass(int i, int j) { return i == j ? 1.0 / 3.0 : 1.0 / 6.0; }
+__host__ __device__ inline double q1_stiff(int i, int j) { return i == j ? 1.0 : -1.0; }
+__host__ __device__ inline double q1_mixed(int i, int /*j*/) { return i ? 0.5 : -0.5; }
+
+// I_ab(i,j) = int_{[0,1]^3} d_a phi_i d_b phi_j for hex8 corners i, j
+__host__ __device__ inline double q1_grad_integral(int a, int b, int i, int j) {
+    const int bi[3] = {hex8_cx(i), hex8_cy(i), hex8_cz(i)};
+    const int bj[3] = {hex8_cx(j), hex8_cy(j), hex8_cz(j)};
+    double r = 1.0;
+    for (int d = 0; d < 3; d++) {
+        double t;
+        if (d == a && d == b)
+            t = q1_stiff(bi[d], bj[d]);
+        else if (d == a)
+            t = q1_mixed(bi[d], bj[d]);
+        else if (d == b)
+            t = q1_mixed(bj[d], bi[d]);
+        else
+            t = q1_mass(bi[d], bj[d]);
+        r *= t;
+    }
+    return r;
+}
+
+// Q1 Laplacian element matrix entry with constant metric G (packed FFF, 6 entries):
+// the exact integral evaluated by operators/hex8/hex8_laplacian_inline_cpu.hpp:213-330.
+__host__ __device__ inline double hex8_laplacian_entry(const double *f, int i, int j) {
+    const double G[3][3] = {{f[0], f[1], f[2]}, {f[1], f[3], f[4]}, {f[2], f[4], f[5]}};
+    double s = 0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) s += G[a][b] * q1_grad_integral(a, b, i, j);
+    return s;
+}
+
+// Q1 elasticity element matrix entry, index = component*8 + node, constant Jacobian:
+// K = det * [lambda M_{ci,cj} + mu M_{cj,ci} + delta_{ci,cj} mu sum_k M_kk],
+// M_kl(i,j) = sum_ab P_ka P_lb I_ab(i,j), P = J^-T = adj^T / det
+// (operators/hex8/hex8_linear_elasticity_inline_cpu.hpp:207-).
+__host__ __device__ inline double hex8_elasticity_entry(const double *adj, double det, double mu, double lambda,
+                                                        int row, int col) {
+    const int ci = row / 8, i = row % 8, cj = col / 8, j = col % 8;
+    double I[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) I[a][b] = q1_grad_integral(a, b, i, j);
+    // P_ka = adj[a*3+k] / det
+    auto M = [&](int k, int l) {
+        double s = 0;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) s += adj[a * 3 + k] * adj[b * 3 + l] * I[a][b];
+        return s / (det * det);
+    };
+    double v = lambda * M(ci, cj) + mu * M(cj, ci);
+    if (ci == cj) v += mu * (M(0, 0) + M(1, 1) + M(2, 2));
+    return v * det;
+}
+
+// ----------------------------------------------------------------- launch helpers
+constexpr int kBlock = 256;
+inline unsigned grid_for(ptrdiff_t n, int block = kBlock, unsigned cap = 1u << 20) {
+    ptrdiff_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > (ptrdiff_t)cap) g = cap;
+    return (unsigned)g;
+}
+
+}  // namespace sfem_amd

@@ -1,0 +1,890 @@
+// C ABI runtime: device meshes, operators, transfers, BLAS entry points and solver
+// handles over the sfem:: templates (include/sfem/).  See include/sfem_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "sfem/sfem_Chebyshev3.hpp"
+#include "sfem/sfem_Multigrid.hpp"
+#include "sfem/sfem_Stationary.hpp"
+#include "sfem/sfem_cg.hpp"
+#include "sshex8_kernels.hpp"
+#include "transfer_blas_kernels.hpp"
+#include "unstructured_kernels.hpp"
+
+namespace sfem_amd {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+// ----------------------------------------------------------------- device memory helper
+template <typename T>
+struct DevArray {
+    T *p{nullptr};
+    size_t n{0};
+    DevArray() = default;
+    DevArray(const DevArray &) = delete;
+    DevArray &operator=(const DevArray &) = delete;
+    ~DevArray() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        reset();
+        if (count == 0) return SFEM_SUCCESS;
+        SFEM_AMD_HIP_CHECK(hipMalloc((void **)&p, sizeof(T) * count));
+        n = count;
+        return SFEM_SUCCESS;
+    }
+    int upload(const T *h, size_t count) {
+        if (alloc(count)) return SFEM_FAILURE;
+        if (count) SFEM_AMD_HIP_CHECK(hipMemcpy(p, h, sizeof(T) * count, hipMemcpyHostToDevice));
+        return SFEM_SUCCESS;
+    }
+};
+
+static double *reduction_ws() {
+    static double *ws = nullptr;
+    if (!ws) {
+        if (hipMalloc((void **)&ws, sizeof(double) * reduction_workspace_doubles()) != hipSuccess) ws = nullptr;
+    }
+    return ws;
+}
+
+static uint16_t to_bf16(float f) {
+    // round to nearest even
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+    const uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (uint16_t)(u >> 16);
+}
+
+// ----------------------------------------------------------------- operator base
+struct OpBase {
+    virtual ~OpBase() = default;
+    virtual int apply(const void *x, void *y, hipStream_t s) = 0;
+    virtual ptrdiff_t rows() const = 0;
+    virtual ptrdiff_t cols() const { return rows(); }
+    virtual int real_type() const = 0;
+    virtual int diag(void *, hipStream_t) {
+        set_last_error("diag not available for this operator");
+        return SFEM_FAILURE;
+    }
+    virtual int block_diag(void *, hipStream_t) {
+        set_last_error("block diagonal not available for this operator");
+        return SFEM_FAILURE;
+    }
+    virtual int iterations() const { return -1; }
+    virtual int last_status() const { return SFEM_SUCCESS; }
+    virtual const std::vector<double> *history() const { return nullptr; }
+};
+
+}  // namespace sfem_amd
+
+struct sfem_amd_op {
+    std::shared_ptr<sfem_amd::OpBase> impl;
+};
+
+struct sfem_amd_mesh {
+    int type{0};
+    int level{1};
+    ptrdiff_t E{0};
+    ptrdiff_t nnodes{0};
+    int nv{0};                             // nodes per element
+    sfem_amd::DevArray<int32_t> idx;       // SoA [nv][E] (tet4/hex8) or element-major [E][nv] (sshex8)
+    mutable sfem_amd::DevArray<uint16_t> count;  // incidence counts (lazy)
+    mutable sfem_amd::DevArray<uint8_t> own;     // owner mask (lazy)
+    const sfem_amd_mesh *parent{nullptr};        // set by sfem_amd_mesh_derefine
+};
+
+namespace sfem_amd {
+
+static int mesh_counts(const sfem_amd_mesh *m, hipStream_t s) {
+    if (m->count.p) return SFEM_SUCCESS;
+    if (m->count.alloc(m->nnodes)) return SFEM_FAILURE;
+    return node_incidence(m->E * m->nv, m->nnodes, m->idx.p, m->count.p, s);
+}
+static int mesh_owner(const sfem_amd_mesh *m, hipStream_t s) {
+    if (m->own.p) return SFEM_SUCCESS;
+    if (m->own.alloc((size_t)m->E * m->nv)) return SFEM_FAILURE;
+    return node_owner_mask(m->E, m->nv, m->nnodes, m->idx.p, m->own.p, s);
+}
+
+// ----------------------------------------------------------------- Laplacian
+template <typename T>
+struct LaplacianOp final : OpBase {
+    const sfem_amd_mesh *m{nullptr};  // the mesh outlives the operator (documented contract)
+    DevArray<float> fff32;      // SoA [6][E]
+    DevArray<uint16_t> fff16;   // SoA [6][E] bf16 (FP32 path, coefficient folded in)
+    DevArray<float> coeff;
+    int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
+    ptrdiff_t rows() const override { return m->nnodes; }
+
+    int apply(const void *xv, void *yv, hipStream_t s) override {
+        const T *x = (const T *)xv;
+        T *y = (T *)yv;
+        if constexpr (std::is_same<T, double>::value) {
+            switch (m->type) {
+                case SFEM_AMD_TET4:
+                    return tet4_laplacian_apply(m->E, m->idx.p, fff32.p, x, y, s);
+                case SFEM_AMD_HEX8:
+                    return hex8_laplacian_apply(m->E, m->idx.p, fff32.p, coeff.p, x, y, s);
+                default:
+                    return sshex8_laplacian_apply<double, float>(m->level, m->E, m->idx.p, fff32.p, m->E, coeff.p, x,
+                                                                 y, s);
+            }
+        } else {
+            return sshex8_laplacian_apply<float, uint16_t>(m->level, m->E, m->idx.p, fff16.p, m->E, nullptr, x, y,
+                                                           s);
+        }
+    }
+    int diag(void *dv, hipStream_t s) override {
+        T *d = (T *)dv;
+        if constexpr (std::is_same<T, double>::value) {
+            switch (m->type) {
+                case SFEM_AMD_TET4:
+                    return tet4_laplacian_diag(m->E, m->idx.p, fff32.p, d, s);
+                case SFEM_AMD_HEX8:
+                    return hex8_laplacian_diag(m->E, m->idx.p, fff32.p, coeff.p, d, s);
+                default:
+                    return sshex8_laplacian_diag<double, float>(m->level, m->E, m->idx.p, fff32.p, m->E, coeff.p, d, s);
+            }
+        } else {
+            return sshex8_laplacian_diag<float, uint16_t>(m->level, m->E, m->idx.p, fff16.p, m->E, nullptr, d, s);
+        }
+    }
+};
+
+// ----------------------------------------------------------------- elasticity
+template <typename T>
+struct ElasticityOp final : OpBase {
+    const sfem_amd_mesh *m{nullptr};
+    DevArray<float> adj, det, coeff;  // hex8: SoA [9][E], [E]
+    DevArray<double> K;               // sshex8: per-macro 24x24 matrices
+    double mu{0}, lambda{0};
+    int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
+    ptrdiff_t rows() const override { return m->nnodes * 3; }
+    int apply(const void *xv, void *yv, hipStream_t s) override {
+        if (m->type == SFEM_AMD_HEX8) {
+            if constexpr (std::is_same<T, double>::value)
+                return hex8_elasticity_apply(m->E, m->idx.p, adj.p, det.p, mu, lambda, coeff.p, (const double *)xv,
+                                             (double *)yv, s);
+            set_last_error("hex8 elasticity: FP64 only");
+            return SFEM_FAILURE;
+        }
+        return sshex8_elasticity_apply<T>(m->level, m->E, m->idx.p, K.p, (const T *)xv, (T *)yv, s);
+    }
+    int diag(void *d, hipStream_t s) override {
+        if (m->type == SFEM_AMD_HEX8) {
+            if constexpr (std::is_same<T, double>::value)
+                return hex8_elasticity_diag(m->E, m->idx.p, adj.p, det.p, mu, lambda, coeff.p, 0, (double *)d, s);
+            return SFEM_FAILURE;
+        }
+        return sshex8_elasticity_diag<T>(m->level, m->E, m->idx.p, K.p, 0, (T *)d, s);
+    }
+    int block_diag(void *d, hipStream_t s) override {
+        if (m->type == SFEM_AMD_HEX8) {
+            if constexpr (std::is_same<T, double>::value)
+                return hex8_elasticity_diag(m->E, m->idx.p, adj.p, det.p, mu, lambda, coeff.p, 1, (double *)d, s);
+            return SFEM_FAILURE;
+        }
+        return sshex8_elasticity_diag<T>(m->level, m->E, m->idx.p, K.p, 1, (T *)d, s);
+    }
+};
+
+// ----------------------------------------------------------------- constraints
+struct ConstrainedOp final : OpBase {
+    std::shared_ptr<OpBase> inner;
+    DevArray<int32_t> dofs;
+    int apply(const void *x, void *y, hipStream_t s) override {
+        if (inner->apply(x, y, s)) return SFEM_FAILURE;
+        return copy_constrained(x, y, s);
+    }
+    int copy_constrained(const void *x, void *y, hipStream_t s) const {
+        if (inner->real_type() == SFEM_AMD_FLOAT64)
+            return dofs_copy<double>((ptrdiff_t)dofs.n, dofs.p, (const double *)x, (double *)y, s);
+        return dofs_copy<float>((ptrdiff_t)dofs.n, dofs.p, (const float *)x, (float *)y, s);
+    }
+    int set_value(double v, void *x, hipStream_t s) const {
+        if (inner->real_type() == SFEM_AMD_FLOAT64) return dofs_set_value<double>((ptrdiff_t)dofs.n, dofs.p, v, (double *)x, s);
+        return dofs_set_value<float>((ptrdiff_t)dofs.n, dofs.p, (float)v, (float *)x, s);
+    }
+    ptrdiff_t rows() const override { return inner->rows(); }
+    int real_type() const override { return inner->real_type(); }
+    int diag(void *d, hipStream_t s) override {
+        if (inner->diag(d, s)) return SFEM_FAILURE;
+        return set_value(1.0, d, s);  // f->set_value_to_constrained_dofs(1, diag)
+    }
+    int block_diag(void *d, hipStream_t s) override { return inner->block_diag(d, s); }
+};
+
+// ----------------------------------------------------------------- transfers
+template <typename T>
+struct TransferOp final : OpBase {
+    const sfem_amd_mesh *fine{nullptr};
+    int Lc{1};
+    ptrdiff_t n_coarse{0};
+    int vs{1};
+    bool prolong{true};
+    std::shared_ptr<OpBase> constrained;  // ConstrainedOp on the output level (or null)
+    int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
+    ptrdiff_t rows() const override { return prolong ? fine->nnodes * vs : n_coarse * vs; }
+    ptrdiff_t cols() const override { return prolong ? n_coarse * vs : fine->nnodes * vs; }
+    int apply(const void *x, void *y, hipStream_t s) override {
+        int err;
+        if (prolong) {
+            if (mesh_owner(fine, s)) return SFEM_FAILURE;
+            err = sshex8_prolongate<T>(fine->E, fine->level, Lc, fine->idx.p, fine->own.p, vs, (const T *)x, (T *)y, s);
+        } else {
+            if (mesh_counts(fine, s)) return SFEM_FAILURE;
+            err = sshex8_restrict<T>(fine->E, fine->level, Lc, fine->idx.p, fine->count.p, vs, (const T *)x, (T *)y, s);
+        }
+        if (err) return err;
+        if (constrained) return static_cast<ConstrainedOp *>(constrained.get())->set_value(0.0, y, s);
+        return SFEM_SUCCESS;
+    }
+};
+
+// ----------------------------------------------------------------- sfem:: template bridges
+template <typename T>
+struct OpView final : sfem::Operator<T> {
+    std::shared_ptr<OpBase> impl;
+    hipStream_t s{nullptr};
+    explicit OpView(std::shared_ptr<OpBase> o) : impl(std::move(o)) {}
+    int apply(const T *const x, T *const y) override { return impl->apply(x, y, s); }
+    std::ptrdiff_t rows() const override { return impl->rows(); }
+    std::ptrdiff_t cols() const override { return impl->cols(); }
+    sfem::ExecutionSpace execution_space() const override { return sfem::EXECUTION_SPACE_DEVICE; }
+};
+
+template <typename T>
+static sfem::SharedOperator<T> view(const std::shared_ptr<OpBase> &o) {
+    if (!o) return nullptr;
+    return std::make_shared<OpView<T>>(o);
+}
+
+template <typename T>
+struct SolverOp final : OpBase {
+    std::shared_ptr<sfem::Operator<T>> op;
+    std::function<int()> its;
+    std::function<int()> status;
+    std::function<const std::vector<double> *()> hist;
+    std::vector<std::shared_ptr<OpBase>> keep;  // children kept alive
+    ptrdiff_t n{0};
+    int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
+    ptrdiff_t rows() const override { return n; }
+    int apply(const void *x, void *y, hipStream_t) override { return op->apply((const T *)x, (T *)y); }
+    int iterations() const override { return its ? its() : -1; }
+    int last_status() const override { return status ? status() : SFEM_SUCCESS; }
+    const std::vector<double> *history() const override { return hist ? hist() : nullptr; }
+};
+
+// FP64 -> FP32 operator -> FP64: y += (double) op32((float) x)
+struct PrecisionBridge final : OpBase {
+    std::shared_ptr<OpBase> op32;
+    DevArray<float> xf, yf;
+    int real_type() const override { return SFEM_AMD_FLOAT64; }
+    ptrdiff_t rows() const override { return op32->rows(); }
+    int apply(const void *x, void *y, hipStream_t s) override {
+        const ptrdiff_t n = rows();
+        if ((ptrdiff_t)xf.n != n && (xf.alloc(n) || yf.alloc(n))) return SFEM_FAILURE;
+        if (blas_cast<double, float>(n, (const double *)x, xf.p, s)) return SFEM_FAILURE;
+        SFEM_AMD_HIP_CHECK(hipMemsetAsync(yf.p, 0, sizeof(float) * n, s));
+        if (op32->apply(xf.p, yf.p, s)) return SFEM_FAILURE;
+        return cast_accumulate(n, yf.p, (double *)y, s);  // y += (double) yf
+    }
+};
+
+}  // namespace sfem_amd
+
+using namespace sfem_amd;
+
+static sfem_amd_op *wrap(std::shared_ptr<OpBase> o) {
+    auto *h = new sfem_amd_op;
+    h->impl = std::move(o);
+    return h;
+}
+
+#define CHECK_HANDLE(h)                                  \
+    do {                                                 \
+        if (!(h)) {                                      \
+            set_last_error("null handle");               \
+            return SFEM_FAILURE;                         \
+        }                                                \
+    } while (0)
+
+extern "C" {
+
+const char *sfem_amd_last_error(void) { return g_last_error.c_str(); }
+
+int sfem_amd_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int sfem_amd_device_synchronize(void) {
+    SFEM_AMD_HIP_CHECK(hipDeviceSynchronize());
+    return SFEM_SUCCESS;
+}
+
+void *sfem_amd_malloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+        set_last_error("hipMalloc failed");
+        return nullptr;
+    }
+    return p;
+}
+int sfem_amd_free(void *p) {
+    SFEM_AMD_HIP_CHECK(hipFree(p));
+    return SFEM_SUCCESS;
+}
+int sfem_amd_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    SFEM_AMD_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return SFEM_SUCCESS;
+}
+int sfem_amd_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    SFEM_AMD_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return SFEM_SUCCESS;
+}
+int sfem_amd_memset(void *dst, int v, size_t bytes, void *stream) {
+    if (!bytes) return SFEM_SUCCESS;
+    SFEM_AMD_HIP_CHECK(hipMemsetAsync(dst, v, bytes, as_stream(stream)));
+    return SFEM_SUCCESS;
+}
+
+// ----------------------------------------------------------------- meshes
+static int nodes_per_element(int type, int level) {
+    switch (type) {
+        case SFEM_AMD_TET4:
+            return 4;
+        case SFEM_AMD_HEX8:
+            return 8;
+        case SFEM_AMD_SSHEX8:
+            return nxe_of(level);
+        default:
+            return -1;
+    }
+}
+
+int sfem_amd_mesh_create(int type, int level, ptrdiff_t ne, ptrdiff_t stride, const int32_t *el, ptrdiff_t nnodes,
+                         sfem_amd_mesh **out) {
+    const int nv = nodes_per_element(type, level);
+    SFEM_AMD_REQUIRE(nv > 0, "mesh_create: unknown element type");
+    SFEM_AMD_REQUIRE(type != SFEM_AMD_SSHEX8 || sshex8_level_supported(level), "mesh_create: unsupported sshex8 level");
+    SFEM_AMD_REQUIRE(stride >= ne, "mesh_create: stride < nelements");
+    for (ptrdiff_t i = 0; i < (ptrdiff_t)nv; i++)
+        for (ptrdiff_t e = 0; e < ne; e++) {
+            const int32_t g = el[i * stride + e];
+            SFEM_AMD_REQUIRE(g >= 0 && g < nnodes, "mesh_create: node id out of range");
+        }
+    auto m = std::make_unique<sfem_amd_mesh>();
+    m->type = type;
+    m->level = type == SFEM_AMD_SSHEX8 ? level : 1;
+    m->E = ne;
+    m->nnodes = nnodes;
+    m->nv = nv;
+    std::vector<int32_t> h((size_t)nv * ne);
+    if (type == SFEM_AMD_SSHEX8) {
+        for (ptrdiff_t e = 0; e < ne; e++)
+            for (int v = 0; v < nv; v++) h[(size_t)e * nv + v] = el[v * stride + e];
+    } else {
+        for (int v = 0; v < nv; v++) std::memcpy(&h[(size_t)v * ne], &el[v * stride], sizeof(int32_t) * ne);
+    }
+    if (m->idx.upload(h.data(), h.size())) return SFEM_FAILURE;
+    *out = m.release();
+    return SFEM_SUCCESS;
+}
+
+int sfem_amd_mesh_derefine(const sfem_amd_mesh *fine, int level, sfem_amd_mesh **out) {
+    CHECK_HANDLE(fine);
+    SFEM_AMD_REQUIRE(fine->type == SFEM_AMD_SSHEX8, "derefine: sshex8 mesh required");
+    SFEM_AMD_REQUIRE(level >= 1 && fine->level % level == 0, "derefine: level must divide the fine level");
+    auto m = std::make_unique<sfem_amd_mesh>();
+    m->E = fine->E;
+    if (level == 1) {
+        m->type = SFEM_AMD_HEX8;
+        m->level = 1;
+        m->nv = 8;
+        if (m->idx.alloc((size_t)8 * m->E)) return SFEM_FAILURE;
+        if (sshex8_to_hex8(m->E, fine->level, fine->idx.p, m->idx.p, nullptr)) return SFEM_FAILURE;
+    } else {
+        SFEM_AMD_REQUIRE(sshex8_level_supported(level), "derefine: unsupported level");
+        m->type = SFEM_AMD_SSHEX8;
+        m->level = level;
+        m->nv = nxe_of(level);
+        if (m->idx.alloc((size_t)m->nv * m->E)) return SFEM_FAILURE;
+        if (sshex8_derefine(m->E, fine->level, level, fine->idx.p, m->idx.p, nullptr)) return SFEM_FAILURE;
+    }
+    m->parent = fine;
+    // nested levels are prefixes of the fine numbering: node count = max id + 1
+    std::vector<int32_t> h(m->idx.n);
+    SFEM_AMD_HIP_CHECK(hipMemcpy(h.data(), m->idx.p, sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost));
+    int32_t mx = -1;
+    for (int32_t v : h) mx = v > mx ? v : mx;
+    m->nnodes = (ptrdiff_t)mx + 1;
+    *out = m.release();
+    return SFEM_SUCCESS;
+}
+
+ptrdiff_t sfem_amd_mesh_nnodes(const sfem_amd_mesh *m) { return m ? m->nnodes : -1; }
+ptrdiff_t sfem_amd_mesh_nelements(const sfem_amd_mesh *m) { return m ? m->E : -1; }
+int sfem_amd_mesh_level(const sfem_amd_mesh *m) { return m ? m->level : -1; }
+int sfem_amd_mesh_element_type(const sfem_amd_mesh *m) { return m ? m->type : -1; }
+
+int sfem_amd_mesh_download(const sfem_amd_mesh *m, int32_t *el) {
+    CHECK_HANDLE(m);
+    std::vector<int32_t> h(m->idx.n);
+    SFEM_AMD_HIP_CHECK(hipMemcpy(h.data(), m->idx.p, sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost));
+    if (m->type == SFEM_AMD_SSHEX8) {
+        for (ptrdiff_t e = 0; e < m->E; e++)
+            for (int v = 0; v < m->nv; v++) el[(size_t)v * m->E + e] = h[(size_t)e * m->nv + v];
+    } else {
+        std::memcpy(el, h.data(), sizeof(int32_t) * h.size());
+    }
+    return SFEM_SUCCESS;
+}
+
+void sfem_amd_mesh_destroy(sfem_amd_mesh *m) { delete m; }
+
+// ----------------------------------------------------------------- operators
+static int soa_upload(DevArray<float> &dst, const float *aos, ptrdiff_t E, int k) {
+    std::vector<float> h((size_t)k * E);
+    for (ptrdiff_t e = 0; e < E; e++)
+        for (int q = 0; q < k; q++) h[(size_t)q * E + e] = aos[e * k + q];
+    return dst.upload(h.data(), h.size());
+}
+
+int sfem_amd_laplacian_create(const sfem_amd_mesh *mesh, const float *fff, const float *coeff, int rt,
+                              sfem_amd_op **out) {
+    CHECK_HANDLE(mesh);
+    SFEM_AMD_REQUIRE(fff, "laplacian_create: fff required");
+    const ptrdiff_t E = mesh->E;
+    if (rt == SFEM_AMD_FLOAT64) {
+        auto op = std::make_shared<LaplacianOp<double>>();
+        op->m = mesh;
+        SFEM_AMD_REQUIRE(mesh->type != SFEM_AMD_TET4 || !coeff, "tet4 Laplacian: no coefficient");
+        if (soa_upload(op->fff32, fff, E, 6)) return SFEM_FAILURE;
+        if (coeff && op->coeff.upload(coeff, E)) return SFEM_FAILURE;
+        *out = wrap(op);
+        return SFEM_SUCCESS;
+    }
+    SFEM_AMD_REQUIRE(rt == SFEM_AMD_FLOAT32, "laplacian_create: real type");
+    SFEM_AMD_REQUIRE(mesh->type == SFEM_AMD_SSHEX8, "FP32 Laplacian: sshex8 meshes only");
+    auto op = std::make_shared<LaplacianOp<float>>();
+    op->m = mesh;
+    std::vector<uint16_t> h((size_t)6 * E);
+    for (ptrdiff_t e = 0; e < E; e++) {
+        const double c = coeff ? (double)coeff[e] : 1.0;
+        for (int q = 0; q < 6; q++) h[(size_t)q * E + e] = to_bf16((float)((double)fff[e * 6 + q] * c));
+    }
+    if (op->fff16.upload(h.data(), h.size())) return SFEM_FAILURE;
+    *out = wrap(op);
+    return SFEM_SUCCESS;
+}
+
+int sfem_amd_linear_elasticity_create(const sfem_amd_mesh *mesh, const float *adj, const float *det, double mu,
+                                      double lambda, const float *coeff, int rt, sfem_amd_op **out) {
+    CHECK_HANDLE(mesh);
+    SFEM_AMD_REQUIRE(adj && det, "elasticity_create: adjugate and determinant required");
+    SFEM_AMD_REQUIRE(mesh->type == SFEM_AMD_HEX8 || mesh->type == SFEM_AMD_SSHEX8, "elasticity: HEX8 or SSHEX8");
+    const ptrdiff_t E = mesh->E;
+    auto setup = [&](auto op) -> int {
+        op->m = mesh;
+        op->mu = mu;
+        op->lambda = lambda;
+        if (mesh->type == SFEM_AMD_HEX8) {
+            if (soa_upload(op->adj, adj, E, 9) || op->det.upload(det, E)) return SFEM_FAILURE;
+            if (coeff && op->coeff.upload(coeff, E)) return SFEM_FAILURE;
+            return SFEM_SUCCESS;
+        }
+        // sshex8: per-macro 24x24 matrices, built on device from the float32 words (FP64
+        // path) or from BF16-rounded words (FP32 path)
+        DevArray<float> dcoeff;
+        if (coeff && dcoeff.upload(coeff, E)) return SFEM_FAILURE;
+        if (op->K.alloc((size_t)E * 576)) return SFEM_FAILURE;
+        if (rt == SFEM_AMD_FLOAT64) {
+            DevArray<float> a, d;
+            if (soa_upload(a, adj, E, 9) || d.upload(det, E)) return SFEM_FAILURE;
+            if (sshex8_elasticity_matrices<float>(mesh->level, E, a.p, d.p, E, mu, lambda, dcoeff.p, op->K.p, nullptr))
+                return SFEM_FAILURE;
+        } else {
+            std::vector<uint16_t> ha((size_t)9 * E), hd(E);
+            for (ptrdiff_t e = 0; e < E; e++) {
+                for (int q = 0; q < 9; q++) ha[(size_t)q * E + e] = to_bf16(adj[e * 9 + q]);
+                hd[e] = to_bf16(det[e]);
+            }
+            DevArray<uint16_t> a, d;
+            if (a.upload(ha.data(), ha.size()) || d.upload(hd.data(), hd.size())) return SFEM_FAILURE;
+            if (sshex8_elasticity_matrices<uint16_t>(mesh->level, E, a.p, d.p, E, mu, lambda, dcoeff.p, op->K.p,
+                                                     nullptr))
+                return SFEM_FAILURE;
+        }
+        SFEM_AMD_HIP_CHECK(hipDeviceSynchronize());
+        return SFEM_SUCCESS;
+    };
+    if (rt == SFEM_AMD_FLOAT64) {
+        auto op = std::make_shared<ElasticityOp<double>>();
+        if (setup(op)) return SFEM_FAILURE;
+        *out = wrap(op);
+    } else {
+        SFEM_AMD_REQUIRE(mesh->type == SFEM_AMD_SSHEX8, "FP32 elasticity: sshex8 only");
+        auto op = std::make_shared<ElasticityOp<float>>();
+        if (setup(op)) return SFEM_FAILURE;
+        *out = wrap(op);
+    }
+    return SFEM_SUCCESS;
+}
+
+int sfem_amd_op_apply(sfem_amd_op *op, const void *x, void *y, void *stream) {
+    CHECK_HANDLE(op);
+    return op->impl->apply(x, y, as_stream(stream));
+}
+int sfem_amd_op_diag(sfem_amd_op *op, void *d, void *stream) {
+    CHECK_HANDLE(op);
+    return op->impl->diag(d, as_stream(stream));
+}
+int sfem_amd_op_block_diag_sym(sfem_amd_op *op, void *d, void *stream) {
+    CHECK_HANDLE(op);
+    return op->impl->block_diag(d, as_stream(stream));
+}
+ptrdiff_t sfem_amd_op_rows(const sfem_amd_op *op) { return op ? op->impl->rows() : -1; }
+ptrdiff_t sfem_amd_op_cols(const sfem_amd_op *op) { return op ? op->impl->cols() : -1; }
+int sfem_amd_op_real_type(const sfem_amd_op *op) { return op ? op->impl->real_type() : -1; }
+void sfem_amd_op_destroy(sfem_amd_op *op) { delete op; }
+int sfem_amd_op_iterations(const sfem_amd_op *op) { return op ? op->impl->iterations() : -1; }
+int sfem_amd_op_last_status(const sfem_amd_op *op) { return op ? op->impl->last_status() : SFEM_FAILURE; }
+int sfem_amd_cg_history(const sfem_amd_op *op, double *out, int cap) {
+    if (!op) return -1;
+    const std::vector<double> *h = op->impl->history();
+    if (!h) return 0;
+    const int n = (int)h->size() < cap ? (int)h->size() : cap;
+    for (int i = 0; i < n; i++) out[i] = (*h)[i];
+    return (int)h->size();
+}
+
+int sfem_amd_constrained_op_create(sfem_amd_op *op, ptrdiff_t n_nodes, const int32_t *nodes, int bs, int nc,
+                                   const int *comps, sfem_amd_op **out) {
+    CHECK_HANDLE(op);
+    SFEM_AMD_REQUIRE(bs >= 1, "constrained_op: block size");
+    std::vector<int> cs;
+    if (comps)
+        cs.assign(comps, comps + nc);
+    else
+        for (int c = 0; c < bs; c++) cs.push_back(c);
+    std::vector<int32_t> dofs;
+    dofs.reserve((size_t)n_nodes * cs.size());
+    for (int c : cs) {
+        SFEM_AMD_REQUIRE(c >= 0 && c < bs, "constrained_op: component out of range");
+        for (ptrdiff_t i = 0; i < n_nodes; i++) {
+            const int64_t d = (int64_t)nodes[i] * bs + c;
+            SFEM_AMD_REQUIRE(d >= 0 && d < op->impl->rows(), "constrained_op: node out of range");
+            dofs.push_back((int32_t)d);
+        }
+    }
+    auto c = std::make_shared<ConstrainedOp>();
+    c->inner = op->impl;
+    if (c->dofs.upload(dofs.data(), dofs.size())) return SFEM_FAILURE;
+    *out = wrap(c);
+    return SFEM_SUCCESS;
+}
+
+int sfem_amd_constrained_op_set_value(sfem_amd_op *constrained, double value, void *x, void *stream) {
+    CHECK_HANDLE(constrained);
+    auto *c = dynamic_cast<ConstrainedOp *>(constrained->impl.get());
+    SFEM_AMD_REQUIRE(c, "not a constrained operator");
+    return c->set_value(value, x, as_stream(stream));
+}
+
+// ----------------------------------------------------------------- transfers
+static int transfer_create(const sfem_amd_mesh *fine, const sfem_amd_mesh *coarse, int bs, int rt,
+                           const sfem_amd_op *constrained, bool prolong, sfem_amd_op **out) {
+    CHECK_HANDLE(fine);
+    CHECK_HANDLE(coarse);
+    SFEM_AMD_REQUIRE(fine->type == SFEM_AMD_SSHEX8, "transfer: fine mesh must be sshex8");
+    SFEM_AMD_REQUIRE(fine->E == coarse->E, "transfer: meshes must share macro elements");
+    SFEM_AMD_REQUIRE(fine->level % coarse->level == 0 && coarse->level < fine->level, "transfer: nested levels");
+    SFEM_AMD_REQUIRE(!constrained || dynamic_cast<ConstrainedOp *>(constrained->impl.get()),
+                     "transfer: constraints must be a constrained operator");
+    if (coarse->parent != fine) {
+        // The kernels address coarse dofs through the fine connectivity (nested numbering):
+        // every coarse-position id must index the coarse vector.
+        DevArray<int32_t> tmp;
+        if (tmp.alloc((size_t)nxe_of(coarse->level) * fine->E)) return SFEM_FAILURE;
+        if (sshex8_derefine(fine->E, fine->level, coarse->level, fine->idx.p, tmp.p, nullptr)) return SFEM_FAILURE;
+        std::vector<int32_t> h(tmp.n);
+        SFEM_AMD_HIP_CHECK(hipMemcpy(h.data(), tmp.p, sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost));
+        for (int32_t v : h) SFEM_AMD_REQUIRE(v >= 0 && v < coarse->nnodes, "transfer: meshes are not nested (coarse id out of range)");
+    }
+    auto make = [&](auto op) {
+        op->fine = fine;
+        op->Lc = coarse->level;
+        op->n_coarse = coarse->nnodes;
+        op->vs = bs;
+        op->prolong = prolong;
+        if (constrained) op->constrained = constrained->impl;
+        *out = wrap(op);
+    };
+    if (rt == SFEM_AMD_FLOAT64)
+        make(std::make_shared<TransferOp<double>>());
+    else
+        make(std::make_shared<TransferOp<float>>());
+    // build the lazily needed tables now (setup time)
+    if (prolong ? mesh_owner(fine, nullptr) : mesh_counts(fine, nullptr)) return SFEM_FAILURE;
+    return SFEM_SUCCESS;
+}
+
+int sfem_amd_prolongation_create(const sfem_amd_mesh *coarse, const sfem_amd_mesh *fine, int bs, int rt,
+                                 const sfem_amd_op *constrained, sfem_amd_op **out) {
+    return transfer_create(fine, coarse, bs, rt, constrained, true, out);
+}
+int sfem_amd_restriction_create(const sfem_amd_mesh *fine, const sfem_amd_mesh *coarse, int bs, int rt,
+                                const sfem_amd_op *constrained, sfem_amd_op **out) {
+    return transfer_create(fine, coarse, bs, rt, constrained, false, out);
+}
+
+// ----------------------------------------------------------------- BLAS
+#define DISPATCH_RT(rt, CALL_D, CALL_F)                               \
+    do {                                                              \
+        if ((rt) == SFEM_AMD_FLOAT64) return CALL_D;                  \
+        if ((rt) == SFEM_AMD_FLOAT32) return CALL_F;                  \
+        set_last_error("unknown real type");                          \
+        return SFEM_FAILURE;                                          \
+    } while (0)
+
+int sfem_amd_blas_axpby(int rt, ptrdiff_t n, double a, const void *x, double b, void *y, void *s) {
+    DISPATCH_RT(rt, blas_axpby<double>(n, a, (const double *)x, b, (double *)y, as_stream(s)),
+                blas_axpby<float>(n, (float)a, (const float *)x, (float)b, (float *)y, as_stream(s)));
+}
+int sfem_amd_blas_zaxpby(int rt, ptrdiff_t n, double a, const void *x, double b, const void *y, void *z, void *s) {
+    DISPATCH_RT(rt, blas_zaxpby<double>(n, a, (const double *)x, b, (const double *)y, (double *)z, as_stream(s)),
+                blas_zaxpby<float>(n, (float)a, (const float *)x, (float)b, (const float *)y, (float *)z, as_stream(s)));
+}
+int sfem_amd_blas_scal(int rt, ptrdiff_t n, double a, void *x, void *s) {
+    DISPATCH_RT(rt, blas_scal<double>(n, a, (double *)x, as_stream(s)), blas_scal<float>(n, (float)a, (float *)x, as_stream(s)));
+}
+int sfem_amd_blas_values(int rt, ptrdiff_t n, double v, void *x, void *s) {
+    DISPATCH_RT(rt, blas_values<double>(n, v, (double *)x, as_stream(s)), blas_values<float>(n, (float)v, (float *)x, as_stream(s)));
+}
+int sfem_amd_blas_copy(int rt, ptrdiff_t n, const void *x, void *y, void *s) {
+    DISPATCH_RT(rt, blas_copy<double>(n, (const double *)x, (double *)y, as_stream(s)),
+                blas_copy<float>(n, (const float *)x, (float *)y, as_stream(s)));
+}
+int sfem_amd_blas_reciprocal(int rt, ptrdiff_t n, double a, void *x, void *s) {
+    DISPATCH_RT(rt, blas_reciprocal<double>(n, a, (double *)x, as_stream(s)),
+                blas_reciprocal<float>(n, (float)a, (float *)x, as_stream(s)));
+}
+int sfem_amd_blas_xypaz(int rt, ptrdiff_t n, const void *x, const void *y, double a, void *z, void *s) {
+    DISPATCH_RT(rt, blas_xypaz<double>(n, (const double *)x, (const double *)y, a, (double *)z, as_stream(s)),
+                blas_xypaz<float>(n, (const float *)x, (const float *)y, (float)a, (float *)z, as_stream(s)));
+}
+double sfem_amd_blas_dot(int rt, ptrdiff_t n, const void *x, const void *y, void *s) {
+    double r = NAN;
+    double *ws = reduction_ws();
+    if (!ws) return NAN;
+    int err = rt == SFEM_AMD_FLOAT64 ? blas_dot<double>(n, (const double *)x, (const double *)y, ws, &r, as_stream(s))
+                                     : blas_dot<float>(n, (const float *)x, (const float *)y, ws, &r, as_stream(s));
+    return err ? NAN : r;
+}
+double sfem_amd_blas_norm2(int rt, ptrdiff_t n, const void *x, void *s) { return std::sqrt(sfem_amd_blas_dot(rt, n, x, x, s)); }
+int sfem_amd_blas_cast(int rt_from, ptrdiff_t n, const void *x, int rt_to, void *y, void *s) {
+    if (rt_from == SFEM_AMD_FLOAT64 && rt_to == SFEM_AMD_FLOAT32) return blas_cast<double, float>(n, (const double *)x, (float *)y, as_stream(s));
+    if (rt_from == SFEM_AMD_FLOAT32 && rt_to == SFEM_AMD_FLOAT64) return blas_cast<float, double>(n, (const float *)x, (double *)y, as_stream(s));
+    if (rt_from == rt_to) return sfem_amd_blas_copy(rt_from, n, x, y, s);
+    set_last_error("cast: unknown real types");
+    return SFEM_FAILURE;
+}
+double sfem_amd_cg_update_x_r(int rt, ptrdiff_t n, double a, const void *p, const void *Ap, void *x, void *r,
+                              int with_rtr, void *s) {
+    double *ws = reduction_ws();
+    double rtr = 0;
+    int err = rt == SFEM_AMD_FLOAT64
+                      ? cg_update<double>(n, a, (const double *)p, (const double *)Ap, (double *)x, (double *)r, ws,
+                                          with_rtr ? &rtr : nullptr, as_stream(s))
+                      : cg_update<float>(n, (float)a, (const float *)p, (const float *)Ap, (float *)x, (float *)r, ws,
+                                         with_rtr ? &rtr : nullptr, as_stream(s));
+    return err ? NAN : rtr;
+}
+int sfem_amd_cheb3_update(int rt, ptrdiff_t n, double alpha, double beta, const void *rhs, const void *Ax, void *p,
+                          void *x, void *s) {
+    DISPATCH_RT(rt,
+                cheb3_update<double>(n, alpha, beta, (const double *)rhs, (const double *)Ax, (double *)p, (double *)x,
+                                     as_stream(s)),
+                cheb3_update<float>(n, (float)alpha, (float)beta, (const float *)rhs, (const float *)Ax, (float *)p,
+                                    (float *)x, as_stream(s)));
+}
+
+// ----------------------------------------------------------------- solvers
+extern "C++" {
+template <typename T>
+static int cg_create_t(sfem_amd_op *A, sfem_amd_op *P, int flexible, double rtol, double atol, int max_it,
+                       sfem_amd_op **out) {
+    auto cg = std::make_shared<sfem::ConjugateGradient<T>>();
+    cg->default_init();
+    cg->set_op(view<T>(A->impl));
+    if (P) cg->set_preconditioner_op(view<T>(P->impl));
+    cg->flexible = flexible != 0;
+    cg->rtol = (T)rtol;
+    cg->atol = (T)atol;
+    cg->max_it = max_it;
+    auto s = std::make_shared<SolverOp<T>>();
+    s->op = cg;
+    s->n = A->impl->rows();
+    sfem::ConjugateGradient<T> *raw = cg.get();
+    s->its = [raw] { return raw->iterations(); };
+    s->status = [raw] { return raw->last_status; };
+    s->hist = [raw] { return &raw->history; };
+    *out = wrap(s);
+    return SFEM_SUCCESS;
+}
+}  // extern "C++"
+
+int sfem_amd_cg_create(sfem_amd_op *A, sfem_amd_op *P, int flexible, double rtol, double atol, int max_it,
+                       sfem_amd_op **out) {
+    CHECK_HANDLE(A);
+    SFEM_AMD_REQUIRE(!P || P->impl->real_type() == A->impl->real_type(), "cg: preconditioner real type differs");
+    if (A->impl->real_type() == SFEM_AMD_FLOAT64) return cg_create_t<double>(A, P, flexible, rtol, atol, max_it, out);
+    return cg_create_t<float>(A, P, flexible, rtol, atol, max_it, out);
+}
+
+extern "C++" {
+template <typename T>
+static int cheb3_create_t(sfem_amd_op *A, int max_it, double smax, double smin, int eig_its, double eig_tol, int zero,
+                          sfem_amd_op **out) {
+    auto c = std::make_shared<sfem::Chebyshev3<T>>();
+    c->default_init();
+    c->set_op(view<T>(A->impl));
+    c->max_it = max_it;
+    c->scale_eig_max = (T)smax;
+    c->scale_eig_min = (T)smin;
+    c->eigen_solver_max_it = eig_its;
+    c->eigen_solver_tol = (T)eig_tol;
+    c->is_initial_guess_zero = zero != 0;
+    c->init_with_ones();
+    if (!std::isfinite((double)c->eig_max) || c->eig_max <= 0) {
+        set_last_error("cheb3: power method failed");
+        return SFEM_FAILURE;
+    }
+    auto s = std::make_shared<SolverOp<T>>();
+    s->op = c;
+    s->n = A->impl->rows();
+    sfem::Chebyshev3<T> *raw = c.get();
+    s->its = [raw] { return raw->iterations(); };
+    *out = wrap(s);
+    return SFEM_SUCCESS;
+}
+}  // extern "C++"
+
+int sfem_amd_cheb3_create(sfem_amd_op *A, int max_it, double smax, double smin, int eig_its, double eig_tol, int zero,
+                          sfem_amd_op **out) {
+    CHECK_HANDLE(A);
+    if (A->impl->real_type() == SFEM_AMD_FLOAT64) return cheb3_create_t<double>(A, max_it, smax, smin, eig_its, eig_tol, zero, out);
+    return cheb3_create_t<float>(A, max_it, smax, smin, eig_its, eig_tol, zero, out);
+}
+
+double sfem_amd_cheb3_eig_max(const sfem_amd_op *op) {
+    if (!op) return NAN;
+    if (auto *s = dynamic_cast<SolverOp<double> *>(op->impl.get()))
+        if (auto *c = dynamic_cast<sfem::Chebyshev3<double> *>(s->op.get())) return c->eig_max;
+    if (auto *s = dynamic_cast<SolverOp<float> *>(op->impl.get()))
+        if (auto *c = dynamic_cast<sfem::Chebyshev3<float> *>(s->op.get())) return c->eig_max;
+    return NAN;
+}
+
+extern "C++" {
+template <typename T>
+static int jacobi_create_t(sfem_amd_op *A, const void *diag, double omega, int steps, sfem_amd_op **out) {
+    auto st = std::make_shared<sfem::StationaryIteration<T>>();
+    st->default_init();
+    st->set_op(view<T>(A->impl));
+    st->set_preconditioner_op(std::make_shared<sfem::Jacobi<T>>((const T *)diag, A->impl->rows(), (T)omega, st->blas));
+    st->set_max_it(steps);
+    auto s = std::make_shared<SolverOp<T>>();
+    s->op = st;
+    s->n = A->impl->rows();
+    *out = wrap(s);
+    return SFEM_SUCCESS;
+}
+}  // extern "C++"
+
+int sfem_amd_jacobi_create(sfem_amd_op *A, const void *diag, double omega, int steps, sfem_amd_op **out) {
+    CHECK_HANDLE(A);
+    if (A->impl->real_type() == SFEM_AMD_FLOAT64) return jacobi_create_t<double>(A, diag, omega, steps, out);
+    return jacobi_create_t<float>(A, diag, omega, steps, out);
+}
+
+extern "C++" {
+template <typename T>
+static int mg_create_t(int cycle_type, int max_it, double atol, sfem_amd_op **out) {
+    auto s = std::make_shared<SolverOp<T>>();
+    auto mg = std::make_shared<sfem::Multigrid<T>>();
+    mg->default_init();
+    mg->set_cycle_type(cycle_type);
+    mg->set_max_it(max_it);
+    mg->set_atol((T)atol);
+    s->op = mg;
+    sfem::Multigrid<T> *raw = mg.get();
+    s->its = [raw] { return raw->iterations(); };
+    s->hist = [raw] { return &raw->residual_history; };
+    *out = wrap(s);
+    return SFEM_SUCCESS;
+}
+}  // extern "C++"
+
+int sfem_amd_mg_create(int rt, int cycle_type, int max_it, double atol, sfem_amd_op **out) {
+    if (rt == SFEM_AMD_FLOAT64) return mg_create_t<double>(cycle_type, max_it, atol, out);
+    SFEM_AMD_REQUIRE(rt == SFEM_AMD_FLOAT32, "mg_create: real type");
+    return mg_create_t<float>(cycle_type, max_it, atol, out);
+}
+
+extern "C++" {
+template <typename T>
+static int mg_add_level_t(SolverOp<T> *s, sfem_amd_op *op, sfem_amd_op *sm, sfem_amd_op *P, sfem_amd_op *R) {
+    auto *mg = dynamic_cast<sfem::Multigrid<T> *>(s->op.get());
+    SFEM_AMD_REQUIRE(mg, "mg_add_level: not a multigrid handle");
+    mg->add_level(view<T>(op->impl), view<T>(sm->impl), P ? view<T>(P->impl) : nullptr, R ? view<T>(R->impl) : nullptr);
+    if (s->n == 0) s->n = op->impl->rows();
+    return SFEM_SUCCESS;
+}
+}  // extern "C++"
+
+int sfem_amd_mg_add_level(sfem_amd_op *mgh, sfem_amd_op *op, sfem_amd_op *sm, sfem_amd_op *P, sfem_amd_op *R) {
+    CHECK_HANDLE(mgh);
+    CHECK_HANDLE(op);
+    CHECK_HANDLE(sm);
+    const int rt = op->impl->real_type();
+    SFEM_AMD_REQUIRE(sm->impl->real_type() == rt && (!P || P->impl->real_type() == rt) &&
+                             (!R || R->impl->real_type() == rt),
+                     "mg_add_level: mixed real types");
+    if (auto *s = dynamic_cast<SolverOp<double> *>(mgh->impl.get())) {
+        SFEM_AMD_REQUIRE(rt == SFEM_AMD_FLOAT64, "mg_add_level: FP64 multigrid needs FP64 levels");
+        return mg_add_level_t<double>(s, op, sm, P, R);
+    }
+    if (auto *s = dynamic_cast<SolverOp<float> *>(mgh->impl.get())) {
+        SFEM_AMD_REQUIRE(rt == SFEM_AMD_FLOAT32, "mg_add_level: FP32 multigrid needs FP32 levels");
+        return mg_add_level_t<float>(s, op, sm, P, R);
+    }
+    set_last_error("mg_add_level: not a multigrid handle");
+    return SFEM_FAILURE;
+}
+
+int sfem_amd_precision_bridge_create(sfem_amd_op *op32, sfem_amd_op **out) {
+    CHECK_HANDLE(op32);
+    SFEM_AMD_REQUIRE(op32->impl->real_type() == SFEM_AMD_FLOAT32, "precision bridge: FP32 operator required");
+    auto b = std::make_shared<PrecisionBridge>();
+    b->op32 = op32->impl;
+    *out = wrap(b);
+    return SFEM_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,499 @@
+// gfx950 kernels for semi-structured hex8 (sshex8) macro elements.
+//
+// Device layout (set up by mesh.cpp from the reference's SoA [nxe][E]):
+//   idx   int32 [E][nxe]   element-major, so one macro element's (L+1)^3 node ids are
+//                          one contiguous, coalesced block;
+//   fff   float/bf16 [6][E] SoA per-macro FFF (operators/sshex8/sshex8_laplacian.cpp:698-769
+//                          reads the macro FFF, scales by h = 1/L: hex8_sub_fff_0 :19-28);
+//   coeff float [E]        optional per-macro multiplier.
+// Vectors are T (double for the plain method, float for the FP32 preconditioner).
+#include "common.hpp"
+#include "sshex8_kernels.hpp"
+
+#include <hip/hip_bf16.h>
+
+namespace sfem_amd {
+
+__device__ __forceinline__ float bf16_to_float(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
+
+template <typename G>
+__device__ __forceinline__ double load_geom(const G *g, ptrdiff_t i);
+template <>
+__device__ __forceinline__ double load_geom<float>(const float *g, ptrdiff_t i) {
+    return (double)g[i];
+}
+template <>
+__device__ __forceinline__ double load_geom<uint16_t>(const uint16_t *g, ptrdiff_t i) {
+    return (double)bf16_to_float(g[i]);
+}
+
+template <typename T>
+__device__ __forceinline__ void atomic_add(T *p, T v) {
+    atomicAdd(p, v);
+}
+
+constexpr int round64(int n) { return ((n + 63) / 64) * 64; }
+
+template <int L>
+struct LapShape {
+    static constexpr int N1 = L + 1;
+    static constexpr int NXE = N1 * N1 * N1;
+    static constexpr int LINES = N1 * N1;
+    static constexpr int EPB = LINES >= 256 ? 1 : 256 / LINES;  // macro elements per block
+    static constexpr int BLOCK = round64(EPB * LINES);
+};
+
+// Computes the 8x8 sub-element matrices of the block's macro elements into sK.
+template <int L, int EPB, int BLOCK, typename G>
+__device__ __forceinline__ void lap_block_matrices(ptrdiff_t E, ptrdiff_t e0, const G *__restrict__ fff,
+                                                   ptrdiff_t fstride, const float *__restrict__ coeff,
+                                                   double (*sK)[64]) {
+    const double h = 1.0 / L;
+    for (int k = threadIdx.x; k < EPB * 64; k += BLOCK) {
+        const int le = k >> 6, ij = k & 63;
+        const ptrdiff_t e = e0 + le;
+        double v = 0;
+        if (e < E) {
+            double f[6];
+            const double c = (coeff ? (double)coeff[e] : 1.0) * h;
+#pragma unroll
+            for (int q = 0; q < 6; q++) f[q] = load_geom<G>(fff, q * fstride + e) * c;
+            v = hex8_laplacian_entry(f, ij >> 3, ij & 7);
+        }
+        sK[le][ij] = v;
+    }
+}
+
+// y += A x for the sshex8 Laplacian.  One thread per x-line (y,z) of a macro element;
+// the line's node values sit in LDS, a 3x3x3 register window slides along x.  The
+// node's stencil is split into S_a (micro elements whose lower x-corner is the node) and
+// S_b (upper x-corner), each masked by which y/z micro elements exist inside the macro
+// element, so x = 0 and x = L (skeleton) need no divergent code: the whole wave is at
+// the same x.  Interior nodes are owned by one macro element (plain read-modify-write);
+// skeleton nodes are shared and use FP64/FP32 atomics.
+template <typename T, typename G, int L>
+__global__ void __launch_bounds__(LapShape<L>::BLOCK)
+    sshex8_laplacian_apply_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx, const G *__restrict__ fff,
+                                  const ptrdiff_t fstride, const float *__restrict__ coeff,
+                                  const T *__restrict__ x, T *__restrict__ y) {
+    using S = LapShape<L>;
+    constexpr int N1 = S::N1, NXE = S::NXE, LINES = S::LINES, EPB = S::EPB;
+    __shared__ T su[EPB][NXE];
+    __shared__ double sK[EPB][64];
+
+    const int tid = threadIdx.x;
+    const int le = tid / LINES;
+    const int line = tid - le * LINES;
+    const int yy = line % N1, zz = line / N1;
+    const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
+    const ptrdiff_t e = e0 + le;
+    const bool active = (le < EPB) && (e < E);
+
+    int32_t gid[N1];
+    if (active) {
+        const int32_t *row = idx + e * NXE + lidx(L, 0, yy, zz);
+#pragma unroll
+        for (int i = 0; i < N1; i++) {
+            gid[i] = row[i];
+            su[le][lidx(L, i, yy, zz)] = x[gid[i]];
+        }
+    }
+    lap_block_matrices<L, EPB, S::BLOCK, G>(E, e0, fff, fstride, coeff, sK);
+    __syncthreads();
+    if (!active) return;
+
+    // masked stencil halves: [dz+1][dy+1][dx'] with dx' = dx (S_a, dx in {0,1}) or dx+1 (S_b)
+    T Sa[3][3][2], Sb[3][3][2];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                Sa[a][b][c] = 0;
+                Sb[a][b][c] = 0;
+            }
+    const double *K = sK[le];
+#pragma unroll
+    for (int cz = 0; cz < 2; cz++) {
+        const bool okz = cz ? (zz > 0) : (zz < L);
+#pragma unroll
+        for (int cy = 0; cy < 2; cy++) {
+            const bool oky = cy ? (yy > 0) : (yy < L);
+            const double m = (okz && oky) ? 1.0 : 0.0;
+#pragma unroll
+            for (int dz = -cz; dz <= 1 - cz; dz++)
+#pragma unroll
+                for (int dy = -cy; dy <= 1 - cy; dy++) {
+                    const int ra = hex8_corner(0, cy, cz), rb = hex8_corner(1, cy, cz);
+#pragma unroll
+                    for (int dx = 0; dx < 2; dx++) {
+                        Sa[dz + 1][dy + 1][dx] += (T)(m * K[ra * 8 + hex8_corner(dx, cy + dy, cz + dz)]);
+                        Sb[dz + 1][dy + 1][dx] += (T)(m * K[rb * 8 + hex8_corner(dx, cy + dy, cz + dz)]);
+                    }
+                }
+        }
+    }
+
+    // sliding window: w[s][dz+1][dy+1], s = 0: x-1, 1: x, 2: x+1
+    const int yl = yy > 0 ? yy - 1 : 0, yh = yy < L ? yy + 1 : L;
+    const int zl = zz > 0 ? zz - 1 : 0, zh = zz < L ? zz + 1 : L;
+    const int ys[3] = {yl, yy, yh}, zs[3] = {zl, zz, zh};
+    const T *u = su[le];
+    T w[3][3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            w[0][a][b] = 0;
+            w[1][a][b] = u[lidx(L, 0, ys[b], zs[a])];
+            w[2][a][b] = u[lidx(L, 1, ys[b], zs[a])];
+        }
+    const bool interior_line = (yy > 0 && yy < L && zz > 0 && zz < L);
+
+#pragma unroll
+    for (int i = 0; i <= L; i++) {
+        T acc = 0;
+        if (i < L) {
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) acc += Sa[a][b][0] * w[1][a][b] + Sa[a][b][1] * w[2][a][b];
+        }
+        if (i > 0) {
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) acc += Sb[a][b][0] * w[0][a][b] + Sb[a][b][1] * w[1][a][b];
+        }
+        if (interior_line && i > 0 && i < L) {
+            y[gid[i]] += acc;
+        } else {
+            atomic_add(&y[gid[i]], acc);
+        }
+        // slide
+        if (i < L) {
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) {
+                    w[0][a][b] = w[1][a][b];
+                    w[1][a][b] = w[2][a][b];
+                    w[2][a][b] = (i + 2 <= L) ? u[lidx(L, i + 2 <= L ? i + 2 : L, ys[b], zs[a])] : T(0);
+                }
+        }
+    }
+}
+
+// d += diag(A): node-wise sum of the 8x8 diagonal over the micro elements present
+// (operators/sshex8/sshex8_laplacian.cpp:333-435).
+template <typename T, typename G, int L>
+__global__ void __launch_bounds__(LapShape<L>::BLOCK)
+    sshex8_laplacian_diag_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx, const G *__restrict__ fff,
+                                 const ptrdiff_t fstride, const float *__restrict__ coeff, T *__restrict__ d) {
+    using S = LapShape<L>;
+    constexpr int N1 = S::N1, NXE = S::NXE, LINES = S::LINES, EPB = S::EPB;
+    __shared__ double sK[EPB][64];
+    const int tid = threadIdx.x;
+    const int le = tid / LINES;
+    const int line = tid - le * LINES;
+    const int yy = line % N1, zz = line / N1;
+    const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
+    const ptrdiff_t e = e0 + le;
+    const bool active = (le < EPB) && (e < E);
+    lap_block_matrices<L, EPB, S::BLOCK, G>(E, e0, fff, fstride, coeff, sK);
+    __syncthreads();
+    if (!active) return;
+    const double *K = sK[le];
+    for (int i = 0; i <= L; i++) {
+        double acc = 0;
+        for (int c = 0; c < 8; c++) {
+            const int cx = hex8_cx(c), cy = hex8_cy(c), cz = hex8_cz(c);
+            const bool ok = (cx ? i > 0 : i < L) && (cy ? yy > 0 : yy < L) && (cz ? zz > 0 : zz < L);
+            if (ok) acc += K[c * 9];
+        }
+        atomic_add(&d[idx[e * NXE + lidx(L, i, yy, zz)]], (T)acc);
+    }
+}
+
+// ------------------------------------------------------------------ elasticity
+// Per-macro 24x24 sub-element matrix (index component*8 + node), computed once per
+// operator setup: adj * h^2, det * h^3 (operators/sshex8/sshex8.hpp:26-42), mu, lambda
+// scaled by the optional coefficient.
+template <typename G>
+__global__ void sshex8_elasticity_matrices_kernel(const ptrdiff_t E, const int L, const G *__restrict__ adj,
+                                                  const G *__restrict__ det, const ptrdiff_t gstride,
+                                                  const double mu, const double lambda,
+                                                  const float *__restrict__ coeff, double *__restrict__ Kout) {
+    const ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= E * 576) return;
+    const ptrdiff_t e = t / 576;
+    const int rc = (int)(t - e * 576);
+    const double h = 1.0 / L;
+    double a[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) a[q] = load_geom<G>(adj, q * gstride + e) * (h * h);
+    const double dt = load_geom<G>(det, e) * (h * h * h);
+    const double c = coeff ? (double)coeff[e] : 1.0;
+    Kout[t] = hex8_elasticity_entry(a, dt, mu * c, lambda * c, rc / 24, rc % 24);
+}
+
+template <int L>
+struct ElShape {
+    static constexpr int N1 = L + 1;
+    static constexpr int PLANE = N1 * N1;
+    // keep the per-block copies of the 24x24 matrices small in LDS
+    static constexpr int EPB = PLANE >= 64 ? (PLANE >= 256 ? 1 : 256 / PLANE) : 4;
+    static constexpr int BLOCK = round64(EPB * PLANE);
+};
+
+// y += A x for sshex8 linear elasticity (x, y interleaved xyz).  One block per macro
+// element (or EPB small ones); the element is swept plane by plane in z through a ring of
+// three LDS planes (2.5-D streaming, so L = 16 fits), one thread per node of the plane.
+// Each node sums over the (up to 8) micro elements that contain it, 3 rows x 24 columns
+// of the per-macro matrix each (K rows broadcast from LDS).
+template <typename T, int L>
+__global__ void __launch_bounds__(ElShape<L>::BLOCK)
+    sshex8_elasticity_apply_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
+                                   const double *__restrict__ Kall, const T *__restrict__ x,
+                                   T *__restrict__ y) {
+    using S = ElShape<L>;
+    constexpr int N1 = S::N1, NXE = N1 * N1 * N1, PLANE = S::PLANE, EPB = S::EPB;
+    __shared__ T su[EPB][3][PLANE][3];
+    __shared__ T sK[EPB][576];
+
+    const int tid = threadIdx.x;
+    const int le = tid / PLANE;
+    const int p = tid - le * PLANE;
+    const int xx = p % N1, yy = p / N1;
+    const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
+    const ptrdiff_t e = e0 + le;
+    const bool active = (le < EPB) && (e < E);
+
+    for (int k = tid; k < EPB * 576; k += S::BLOCK) {
+        const int l2 = k / 576;
+        const ptrdiff_t e2 = e0 + l2;
+        sK[l2][k - l2 * 576] = (e2 < E) ? (T)Kall[e2 * 576 + (k - l2 * 576)] : T(0);
+    }
+    const int32_t *row = idx + (active ? e : 0) * NXE;
+    auto load_plane = [&](int z) {
+        if (active) {
+            const int32_t g = row[lidx(L, xx, yy, z)];
+#pragma unroll
+            for (int c = 0; c < 3; c++) su[le][z % 3][p][c] = x[(ptrdiff_t)g * 3 + c];
+        }
+    };
+    load_plane(0);
+    __syncthreads();
+
+    for (int z = 0; z <= L; z++) {
+        if (z + 1 <= L) {
+            // slot (z+1)%3 held plane z-2, which nobody reads any more
+            load_plane(z + 1);
+        }
+        __syncthreads();
+        if (active) {
+            T acc[3] = {0, 0, 0};
+            const T *K = sK[le];
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                const int cx = hex8_cx(c), cy = hex8_cy(c), cz = hex8_cz(c);
+                const bool ok = (cx ? xx > 0 : xx < L) && (cy ? yy > 0 : yy < L) && (cz ? z > 0 : z < L);
+                if (ok) {
+                    // micro element with lower corner (xx-cx, yy-cy, z-cz); this node is its corner c
+                    T ue[24];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const int nx = xx - cx + hex8_cx(j), ny = yy - cy + hex8_cy(j), nz = z - cz + hex8_cz(j);
+                        const T *src = su[le][nz % 3][nx + ny * N1];
+                        ue[0 * 8 + j] = src[0];
+                        ue[1 * 8 + j] = src[1];
+                        ue[2 * 8 + j] = src[2];
+                    }
+#pragma unroll
+                    for (int ci = 0; ci < 3; ci++) {
+                        const T *Kr = K + (ci * 8 + c) * 24;
+                        T s = 0;
+#pragma unroll
+                        for (int j = 0; j < 24; j++) s += Kr[j] * ue[j];
+                        acc[ci] += s;
+                    }
+                }
+            }
+            const int32_t g = row[lidx(L, xx, yy, z)];
+            const bool interior = xx > 0 && xx < L && yy > 0 && yy < L && z > 0 && z < L;
+#pragma unroll
+            for (int ci = 0; ci < 3; ci++) {
+                if (interior)
+                    y[(ptrdiff_t)g * 3 + ci] += acc[ci];
+                else
+                    atomic_add(&y[(ptrdiff_t)g * 3 + ci], acc[ci]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// d += diagonal (block = 0) or symmetric 3x3 block diagonal [node][6] (block = 1).
+template <typename T, int L>
+__global__ void sshex8_elasticity_diag_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
+                                              const double *__restrict__ Kall, const int block, T *__restrict__ d) {
+    constexpr int N1 = L + 1, NXE = N1 * N1 * N1;
+    const ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= E * NXE) return;
+    const ptrdiff_t e = t / NXE;
+    const int v = (int)(t - e * NXE);
+    const int xx = v % N1, yy = (v / N1) % N1, zz = v / (N1 * N1);
+    const double *K = Kall + e * 576;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const int ca[6] = {0, 0, 0, 1, 1, 2}, cb[6] = {0, 1, 2, 1, 2, 2};
+    for (int c = 0; c < 8; c++) {
+        const int cx = hex8_cx(c), cy = hex8_cy(c), cz = hex8_cz(c);
+        const bool ok = (cx ? xx > 0 : xx < L) && (cy ? yy > 0 : yy < L) && (cz ? zz > 0 : zz < L);
+        if (!ok) continue;
+        for (int k = 0; k < 6; k++) acc[k] += K[(ca[k] * 8 + c) * 24 + cb[k] * 8 + c];
+    }
+    const ptrdiff_t g = idx[e * NXE + v];
+    if (block) {
+        for (int k = 0; k < 6; k++) atomic_add(&d[g * 6 + k], (T)acc[k]);
+    } else {
+        atomic_add(&d[g * 3 + 0], (T)acc[0]);
+        atomic_add(&d[g * 3 + 1], (T)acc[3]);
+        atomic_add(&d[g * 3 + 2], (T)acc[5]);
+    }
+}
+
+// ------------------------------------------------------------------ dispatch
+#define SFEM_AMD_SSHEX8_LEVELS(X) X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(10) X(12) X(16)
+
+bool sshex8_level_supported(int L) {
+    switch (L) {
+#define CASE(LV) \
+    case LV:     \
+        return true;
+        SFEM_AMD_SSHEX8_LEVELS(CASE)
+#undef CASE
+        default:
+            return false;
+    }
+}
+
+template <typename T, typename G>
+int sshex8_laplacian_apply(int L, ptrdiff_t E, const int32_t *idx, const G *fff, ptrdiff_t fstride,
+                           const float *coeff, const T *x, T *y, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    switch (L) {
+#define CASE(LV)                                                                                            \
+    case LV: {                                                                                              \
+        using Sh = LapShape<LV>;                                                                            \
+        const unsigned grid = (unsigned)((E + Sh::EPB - 1) / Sh::EPB);                                      \
+        sshex8_laplacian_apply_kernel<T, G, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, fff, fstride, coeff, x, y); \
+        break;                                                                                              \
+    }
+        SFEM_AMD_SSHEX8_LEVELS(CASE)
+#undef CASE
+        default:
+            set_last_error("sshex8 level not supported: " + std::to_string(L));
+            return SFEM_FAILURE;
+    }
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template <typename T, typename G>
+int sshex8_laplacian_diag(int L, ptrdiff_t E, const int32_t *idx, const G *fff, ptrdiff_t fstride,
+                          const float *coeff, T *d, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    switch (L) {
+#define CASE(LV)                                                                                          \
+    case LV: {                                                                                            \
+        using Sh = LapShape<LV>;                                                                          \
+        const unsigned grid = (unsigned)((E + Sh::EPB - 1) / Sh::EPB);                                    \
+        sshex8_laplacian_diag_kernel<T, G, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, fff, fstride, coeff, d); \
+        break;                                                                                            \
+    }
+        SFEM_AMD_SSHEX8_LEVELS(CASE)
+#undef CASE
+        default:
+            set_last_error("sshex8 level not supported: " + std::to_string(L));
+            return SFEM_FAILURE;
+    }
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template <typename G>
+int sshex8_elasticity_matrices(int L, ptrdiff_t E, const G *adj, const G *det, ptrdiff_t gstride, double mu,
+                               double lambda, const float *coeff, double *K, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    const ptrdiff_t n = E * 576;
+    sshex8_elasticity_matrices_kernel<G>
+            <<<(unsigned)((n + 255) / 256), 256, 0, s>>>(E, L, adj, det, gstride, mu, lambda, coeff, K);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template <typename T>
+int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const double *K, const T *x, T *y,
+                            hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    switch (L) {
+#define CASE(LV)                                                                                 \
+    case LV: {                                                                                   \
+        using Sh = ElShape<LV>;                                                                  \
+        const unsigned grid = (unsigned)((E + Sh::EPB - 1) / Sh::EPB);                           \
+        sshex8_elasticity_apply_kernel<T, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, K, x, y);       \
+        break;                                                                                   \
+    }
+        SFEM_AMD_SSHEX8_LEVELS(CASE)
+#undef CASE
+        default:
+            set_last_error("sshex8 level not supported: " + std::to_string(L));
+            return SFEM_FAILURE;
+    }
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template <typename T>
+int sshex8_elasticity_diag(int L, ptrdiff_t E, const int32_t *idx, const double *K, int block, T *d, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    const ptrdiff_t n = E * nxe_of(L);
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    switch (L) {
+#define CASE(LV)                                                                          \
+    case LV:                                                                              \
+        sshex8_elasticity_diag_kernel<T, LV><<<grid, 256, 0, s>>>(E, idx, K, block, d);   \
+        break;
+        SFEM_AMD_SSHEX8_LEVELS(CASE)
+#undef CASE
+        default:
+            set_last_error("sshex8 level not supported: " + std::to_string(L));
+            return SFEM_FAILURE;
+    }
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template int sshex8_laplacian_apply<double, float>(int, ptrdiff_t, const int32_t *, const float *, ptrdiff_t,
+                                                   const float *, const double *, double *, hipStream_t);
+template int sshex8_laplacian_apply<float, uint16_t>(int, ptrdiff_t, const int32_t *, const uint16_t *, ptrdiff_t,
+                                                     const float *, const float *, float *, hipStream_t);
+template int sshex8_laplacian_diag<double, float>(int, ptrdiff_t, const int32_t *, const float *, ptrdiff_t,
+                                                  const float *, double *, hipStream_t);
+template int sshex8_laplacian_diag<float, uint16_t>(int, ptrdiff_t, const int32_t *, const uint16_t *, ptrdiff_t,
+                                                    const float *, float *, hipStream_t);
+template int sshex8_elasticity_matrices<float>(int, ptrdiff_t, const float *, const float *, ptrdiff_t, double,
+                                               double, const float *, double *, hipStream_t);
+template int sshex8_elasticity_matrices<uint16_t>(int, ptrdiff_t, const uint16_t *, const uint16_t *, ptrdiff_t,
+                                                  double, double, const float *, double *, hipStream_t);
+template int sshex8_elasticity_apply<double>(int, ptrdiff_t, const int32_t *, const double *, const double *,
+                                             double *, hipStream_t);
+template int sshex8_elasticity_apply<float>(int, ptrdiff_t, const int32_t *, const double *, const float *, float *,
+                                            hipStream_t);
+template int sshex8_elasticity_diag<double>(int, ptrdiff_t, const int32_t *, const double *, int, double *,
+                                            hipStream_t);
+template int sshex8_elasticity_diag<float>(int, ptrdiff_t, const int32_t *, const double *, int, float *,
+                                           hipStream_t);
+
+}  // namespace sfem_amd

@@ -1,0 +1,470 @@
+// gfx950 kernels: sshex8 transfers, BLAS-1 (deterministic reductions), fused solver
+// updates and Dirichlet helpers.  T = double (plain method) or float (FP32 preconditioner).
+#include "common.hpp"
+#include "transfer_blas_kernels.hpp"
+
+namespace sfem_amd {
+
+// ------------------------------------------------------------------ transfers
+// Nested levels share node ids (hierarchical numbering): the node at coarse-local
+// (X,Y,Z) of level Lc is the fine-local node (X*s, Y*s, Z*s) of the fine level Lf = s*Lc,
+// so both directions read the fine element-major connectivity only (the reference's
+// from_level_stride, operators/sshex8/sshex8_interpolate.cpp:259-632).
+
+// Fine value = trilinear interpolant of the coarse cell corners (the reference's x/y/z
+// edge, face and volume passes compose to this).  Every fine node is written by one
+// element only (owner = lowest element id containing it), so results are deterministic.
+template <typename T>
+__global__ void sshex8_prolongate_kernel(const ptrdiff_t E, const int Lf, const int Lc, const int32_t *__restrict__ idx,
+                                         const uint8_t *__restrict__ own, const int vs, const T *__restrict__ from,
+                                         T *__restrict__ to) {
+    const int N1 = Lf + 1, nxe = N1 * N1 * N1, s = Lf / Lc;
+    const ptrdiff_t n = E * nxe;
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
+        if (!own[t]) continue;
+        const ptrdiff_t e = t / nxe;
+        const int v = (int)(t - e * nxe);
+        const int x = v % N1, y = (v / N1) % N1, z = v / (N1 * N1);
+        int cx = x / s, cy = y / s, cz = z / s;
+        if (cx == Lc) cx--;
+        if (cy == Lc) cy--;
+        if (cz == Lc) cz--;
+        const double tx = (double)(x - cx * s) / s, ty = (double)(y - cy * s) / s, tz = (double)(z - cz * s) / s;
+        const int32_t *row = idx + e * nxe;
+        const ptrdiff_t g = row[v];
+        for (int d = 0; d < vs; d++) {
+            double acc = 0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int bx = hex8_cx(k), by = hex8_cy(k), bz = hex8_cz(k);
+                const double w = (bx ? tx : 1 - tx) * (by ? ty : 1 - ty) * (bz ? tz : 1 - tz);
+                if (w == 0) continue;
+                const int32_t cg = row[lidx(Lf, (cx + bx) * s, (cy + by) * s, (cz + bz) * s)];
+                acc += w * (double)from[(ptrdiff_t)cg * vs + d];
+            }
+            to[g * vs + d] = (T)acc;
+        }
+    }
+}
+
+// Coarse += P^T fine: each coarse-local node sums the weighted fine nodes of its
+// (2s-1)^3 neighbourhood inside the element, fine values divided by their element
+// incidence count as sshex8_restrict does (operators/sshex8/sshex8_interpolate.cpp:633-767),
+// then one atomic per (element, coarse node).
+template <typename T>
+__global__ void sshex8_restrict_kernel(const ptrdiff_t E, const int Lf, const int Lc, const int32_t *__restrict__ idx,
+                                       const uint16_t *__restrict__ count, const int vs, const T *__restrict__ from,
+                                       T *__restrict__ to) {
+    const int N1 = Lf + 1, nxe = N1 * N1 * N1, s = Lf / Lc, C1 = Lc + 1, cnxe = C1 * C1 * C1;
+    const ptrdiff_t n = E * cnxe;
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const ptrdiff_t e = t / cnxe;
+        const int c = (int)(t - e * cnxe);
+        const int X = c % C1, Y = (c / C1) % C1, Z = c / (C1 * C1);
+        const int32_t *row = idx + e * nxe;
+        const int x0 = X * s - (s - 1), y0 = Y * s - (s - 1), z0 = Z * s - (s - 1);
+        for (int d = 0; d < vs; d++) {
+            double acc = 0;
+            for (int k = 0; k < 2 * s - 1; k++) {
+                const int z = z0 + k;
+                if (z < 0 || z > Lf) continue;
+                const double wz = 1.0 - (double)abs(z - Z * s) / s;
+                for (int j = 0; j < 2 * s - 1; j++) {
+                    const int y = y0 + j;
+                    if (y < 0 || y > Lf) continue;
+                    const double wy = 1.0 - (double)abs(y - Y * s) / s;
+                    for (int i = 0; i < 2 * s - 1; i++) {
+                        const int x = x0 + i;
+                        if (x < 0 || x > Lf) continue;
+                        const double wx = 1.0 - (double)abs(x - X * s) / s;
+                        const int32_t g = row[lidx(Lf, x, y, z)];
+                        acc += wx * wy * wz * ((double)from[(ptrdiff_t)g * vs + d] / count[g]);
+                    }
+                }
+            }
+            const int32_t cg = row[lidx(Lf, X * s, Y * s, Z * s)];
+            atomicAdd(&to[(ptrdiff_t)cg * vs + d], (T)acc);
+        }
+    }
+}
+
+__global__ void node_owner_kernel(const ptrdiff_t E, const int nxe, const int32_t *__restrict__ idx,
+                                  int32_t *__restrict__ owner) {
+    const ptrdiff_t n = E * nxe;
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x)
+        atomicMin(&owner[idx[t]], (int32_t)(t / nxe));
+}
+
+__global__ void owner_mask_kernel(const ptrdiff_t E, const int nxe, const int32_t *__restrict__ idx,
+                                  const int32_t *__restrict__ owner, uint8_t *__restrict__ own) {
+    const ptrdiff_t n = E * nxe;
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x)
+        own[t] = owner[idx[t]] == (int32_t)(t / nxe) ? 1 : 0;
+}
+
+__global__ void incidence_kernel(const ptrdiff_t n, const int32_t *__restrict__ idx, uint32_t *__restrict__ count) {
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x)
+        atomicAdd(&count[idx[t]], 1u);
+}
+
+__global__ void narrow_u16_kernel(const ptrdiff_t n, const uint32_t *__restrict__ a, uint16_t *__restrict__ b) {
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x)
+        b[t] = (uint16_t)a[t];
+}
+
+__global__ void fill_i32_kernel(const ptrdiff_t n, int32_t v, int32_t *__restrict__ a) {
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x)
+        a[t] = v;
+}
+
+// element-major [E][nxe_f] -> rows (X*s,Y*s,Z*s) as element-major [E][nxe_c]
+__global__ void derefine_kernel(const ptrdiff_t E, const int Lf, const int Lc, const int32_t *__restrict__ fine,
+                                int32_t *__restrict__ coarse) {
+    const int N1 = Lf + 1, nxe = N1 * N1 * N1, s = Lf / Lc, C1 = Lc + 1, cnxe = C1 * C1 * C1;
+    const ptrdiff_t n = E * cnxe;
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const ptrdiff_t e = t / cnxe;
+        const int c = (int)(t - e * cnxe);
+        const int X = c % C1, Y = (c / C1) % C1, Z = c / (C1 * C1);
+        coarse[t] = fine[e * nxe + lidx(Lf, X * s, Y * s, Z * s)];
+    }
+}
+
+// element-major sshex8 -> SoA hex8 [8][E] (standard corner order)
+__global__ void to_hex8_kernel(const ptrdiff_t E, const int L, const int32_t *__restrict__ ss, int32_t *__restrict__ h8) {
+    const int nxe = nxe_of(L);
+    const ptrdiff_t n = E * 8;
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const int v = (int)(t / E);
+        const ptrdiff_t e = t - (ptrdiff_t)v * E;
+        h8[t] = ss[e * nxe + lidx(L, hex8_cx(v) * L, hex8_cy(v) * L, hex8_cz(v) * L)];
+    }
+}
+
+template <typename T>
+int sshex8_prolongate(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint8_t *own, int vs, const T *from,
+                      T *to, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    sshex8_prolongate_kernel<T><<<grid_for(E * nxe_of(Lf)), kBlock, 0, s>>>(E, Lf, Lc, idx, own, vs, from, to);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template <typename T>
+int sshex8_restrict(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint16_t *count, int vs, const T *from,
+                    T *to, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    sshex8_restrict_kernel<T><<<grid_for(E * nxe_of(Lc)), kBlock, 0, s>>>(E, Lf, Lc, idx, count, vs, from, to);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template int sshex8_prolongate<double>(ptrdiff_t, int, int, const int32_t *, const uint8_t *, int, const double *,
+                                       double *, hipStream_t);
+template int sshex8_prolongate<float>(ptrdiff_t, int, int, const int32_t *, const uint8_t *, int, const float *,
+                                      float *, hipStream_t);
+template int sshex8_restrict<double>(ptrdiff_t, int, int, const int32_t *, const uint16_t *, int, const double *,
+                                     double *, hipStream_t);
+template int sshex8_restrict<float>(ptrdiff_t, int, int, const int32_t *, const uint16_t *, int, const float *,
+                                    float *, hipStream_t);
+
+int node_owner_mask(ptrdiff_t E, int nxe, ptrdiff_t nnodes, const int32_t *idx, uint8_t *own, hipStream_t s) {
+    int32_t *owner = nullptr;
+    SFEM_AMD_HIP_CHECK(hipMallocAsync((void **)&owner, sizeof(int32_t) * (nnodes > 0 ? nnodes : 1), s));
+    fill_i32_kernel<<<grid_for(nnodes), kBlock, 0, s>>>(nnodes, INT32_MAX, owner);
+    node_owner_kernel<<<grid_for(E * nxe), kBlock, 0, s>>>(E, nxe, idx, owner);
+    owner_mask_kernel<<<grid_for(E * nxe), kBlock, 0, s>>>(E, nxe, idx, owner, own);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    SFEM_AMD_HIP_CHECK(hipFreeAsync(owner, s));
+    return SFEM_SUCCESS;
+}
+
+int node_incidence(ptrdiff_t n_entries, ptrdiff_t nnodes, const int32_t *idx, uint16_t *count, hipStream_t s) {
+    uint32_t *c32 = nullptr;
+    SFEM_AMD_HIP_CHECK(hipMallocAsync((void **)&c32, sizeof(uint32_t) * (nnodes > 0 ? nnodes : 1), s));
+    SFEM_AMD_HIP_CHECK(hipMemsetAsync(c32, 0, sizeof(uint32_t) * nnodes, s));
+    incidence_kernel<<<grid_for(n_entries), kBlock, 0, s>>>(n_entries, idx, c32);
+    narrow_u16_kernel<<<grid_for(nnodes), kBlock, 0, s>>>(nnodes, c32, count);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    SFEM_AMD_HIP_CHECK(hipFreeAsync(c32, s));
+    return SFEM_SUCCESS;
+}
+
+int sshex8_derefine(ptrdiff_t E, int Lf, int Lc, const int32_t *fine, int32_t *coarse, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    derefine_kernel<<<grid_for(E * nxe_of(Lc)), kBlock, 0, s>>>(E, Lf, Lc, fine, coarse);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+int sshex8_to_hex8(ptrdiff_t E, int L, const int32_t *ss, int32_t *h8, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    to_hex8_kernel<<<grid_for(E * 8), kBlock, 0, s>>>(E, L, ss, h8);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+// ------------------------------------------------------------------ BLAS-1
+template <typename T>
+__global__ void axpby_kernel(const ptrdiff_t n, const T a, const T *__restrict__ x, const T b, T *__restrict__ y) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        y[i] = a * x[i] + b * y[i];
+}
+template <typename T>
+__global__ void zaxpby_kernel(const ptrdiff_t n, const T a, const T *__restrict__ x, const T b, const T *__restrict__ y,
+                              T *__restrict__ z) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        z[i] = a * x[i] + b * y[i];
+}
+template <typename T>
+__global__ void scal_kernel(const ptrdiff_t n, const T a, T *__restrict__ x) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        x[i] *= a;
+}
+template <typename T>
+__global__ void values_kernel(const ptrdiff_t n, const T a, T *__restrict__ x) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        x[i] = a;
+}
+template <typename T>
+__global__ void copy_kernel(const ptrdiff_t n, const T *__restrict__ x, T *__restrict__ y) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        y[i] = x[i];
+}
+template <typename T>
+__global__ void reciprocal_kernel(const ptrdiff_t n, const T a, T *__restrict__ x) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        x[i] = a / x[i];
+}
+// z = x * y + a * z
+template <typename T>
+__global__ void xypaz_kernel(const ptrdiff_t n, const T *__restrict__ x, const T *__restrict__ y, const T a,
+                             T *__restrict__ z) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        z[i] = x[i] * y[i] + a * z[i];
+}
+template <typename S, typename D>
+__global__ void cast_kernel(const ptrdiff_t n, const S *__restrict__ x, D *__restrict__ y) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        y[i] = (D)x[i];
+}
+__global__ void cast_accumulate_kernel(const ptrdiff_t n, const float *__restrict__ x, double *__restrict__ y) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        y[i] += (double)x[i];
+}
+// Chebyshev3 iteration_update (algebra/sfem_Chebyshev3.hpp:37-52): p = beta p - rhs + Ax; x -= alpha p
+template <typename T>
+__global__ void cheb3_update_kernel(const ptrdiff_t n, const T alpha, const T beta, const T *__restrict__ rhs,
+                                    const T *__restrict__ Ax, T *__restrict__ p, T *__restrict__ x) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const T pi = beta * p[i] - rhs[i] + Ax[i];
+        p[i] = pi;
+        x[i] -= alpha * pi;
+    }
+}
+
+// Deterministic reductions: a fixed grid (a function of n only) of 256-thread blocks,
+// wave64 shuffle trees and one pass over the partials in a fixed order.
+constexpr int kRedBlocks = 1024;
+inline int red_blocks(ptrdiff_t n) {
+    ptrdiff_t b = (n + 4 * kBlock - 1) / (4 * kBlock);
+    if (b < 1) b = 1;
+    if (b > kRedBlocks) b = kRedBlocks;
+    return (int)b;
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double part[kBlock / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) part[w] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < kBlock / 64; i++) r += part[i];
+    }
+    return r;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) dot_partial_kernel(const ptrdiff_t n, const T *__restrict__ x,
+                                                             const T *__restrict__ y, double *__restrict__ part) {
+    double acc = 0;
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        acc += (double)x[i] * (double)y[i];
+    const double r = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(kBlock) sum_partials_kernel(const int nb, const double *__restrict__ part,
+                                                              double *__restrict__ out) {
+    double acc = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += part[i];
+    const double r = block_sum(acc);
+    if (threadIdx.x == 0) out[0] = r;
+}
+
+// CG fused update (algebra/sfem_cg.hpp:26-72 host fusion): x += a p; r -= a Ap; partial (r,r)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) cg_update_kernel(const ptrdiff_t n, const T a, const T *__restrict__ p,
+                                                           const T *__restrict__ Ap, T *__restrict__ x,
+                                                           T *__restrict__ r, double *__restrict__ part) {
+    double acc = 0;
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
+        x[i] += a * p[i];
+        const T ri = r[i] - a * Ap[i];
+        r[i] = ri;
+        acc += (double)ri * (double)ri;
+    }
+    if (part) {
+        const double s = block_sum(acc);
+        if (threadIdx.x == 0) part[blockIdx.x] = s;
+    }
+}
+
+template <typename T>
+__global__ void scatter_value_kernel(const ptrdiff_t n, const int32_t *__restrict__ dofs, const T v, T *__restrict__ x) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        x[dofs[i]] = v;
+}
+template <typename T>
+__global__ void copy_dofs_kernel(const ptrdiff_t n, const int32_t *__restrict__ dofs, const T *__restrict__ src,
+                                 T *__restrict__ dst) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        dst[dofs[i]] = src[dofs[i]];
+}
+
+template <typename T>
+int blas_axpby(ptrdiff_t n, T a, const T *x, T b, T *y, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    axpby_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, a, x, b, y);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+template <typename T>
+int blas_zaxpby(ptrdiff_t n, T a, const T *x, T b, const T *y, T *z, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    zaxpby_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, a, x, b, y, z);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+template <typename T>
+int blas_scal(ptrdiff_t n, T a, T *x, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    scal_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, a, x);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+template <typename T>
+int blas_values(ptrdiff_t n, T a, T *x, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    values_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, a, x);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+template <typename T>
+int blas_copy(ptrdiff_t n, const T *x, T *y, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    copy_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, x, y);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+template <typename T>
+int blas_reciprocal(ptrdiff_t n, T a, T *x, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    reciprocal_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, a, x);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+template <typename T>
+int blas_xypaz(ptrdiff_t n, const T *x, const T *y, T a, T *z, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    xypaz_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, x, y, a, z);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+template <typename S, typename D>
+int blas_cast(ptrdiff_t n, const S *x, D *y, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    cast_kernel<S, D><<<grid_for(n), kBlock, 0, s>>>(n, x, y);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+template <typename T>
+int cheb3_update(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, T *p, T *x, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    cheb3_update_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, alpha, beta, rhs, Ax, p, x);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template <typename T>
+int blas_dot(ptrdiff_t n, const T *x, const T *y, double *ws, double *result_host, hipStream_t s) {
+    const int nb = red_blocks(n);
+    dot_partial_kernel<T><<<nb, kBlock, 0, s>>>(n, x, y, ws);
+    sum_partials_kernel<<<1, kBlock, 0, s>>>(nb, ws, ws + kRedBlocks);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    SFEM_AMD_HIP_CHECK(hipMemcpyAsync(result_host, ws + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
+    SFEM_AMD_HIP_CHECK(hipStreamSynchronize(s));
+    return SFEM_SUCCESS;
+}
+
+template <typename T>
+int cg_update(ptrdiff_t n, T a, const T *p, const T *Ap, T *x, T *r, double *ws, double *rtr_host, hipStream_t s) {
+    const int nb = red_blocks(n);
+    cg_update_kernel<T><<<nb, kBlock, 0, s>>>(n, a, p, Ap, x, r, rtr_host ? ws : nullptr);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    if (rtr_host) {
+        sum_partials_kernel<<<1, kBlock, 0, s>>>(nb, ws, ws + kRedBlocks);
+        SFEM_AMD_HIP_CHECK(hipMemcpyAsync(rtr_host, ws + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
+        SFEM_AMD_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return SFEM_SUCCESS;
+}
+
+template <typename T>
+int dofs_set_value(ptrdiff_t n, const int32_t *dofs, T v, T *x, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    scatter_value_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, dofs, v, x);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+template <typename T>
+int dofs_copy(ptrdiff_t n, const int32_t *dofs, const T *src, T *dst, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    copy_dofs_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, dofs, src, dst);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+size_t reduction_workspace_doubles() { return kRedBlocks + 2; }
+
+int cast_accumulate(ptrdiff_t n, const float *x, double *y, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    cast_accumulate_kernel<<<grid_for(n), kBlock, 0, s>>>(n, x, y);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+#define INST(T)                                                                                         \
+    template int blas_axpby<T>(ptrdiff_t, T, const T *, T, T *, hipStream_t);                           \
+    template int blas_zaxpby<T>(ptrdiff_t, T, const T *, T, const T *, T *, hipStream_t);               \
+    template int blas_scal<T>(ptrdiff_t, T, T *, hipStream_t);                                          \
+    template int blas_values<T>(ptrdiff_t, T, T *, hipStream_t);                                        \
+    template int blas_copy<T>(ptrdiff_t, const T *, T *, hipStream_t);                                  \
+    template int blas_reciprocal<T>(ptrdiff_t, T, T *, hipStream_t);                                    \
+    template int blas_xypaz<T>(ptrdiff_t, const T *, const T *, T, T *, hipStream_t);                   \
+    template int cheb3_update<T>(ptrdiff_t, T, T, const T *, const T *, T *, T *, hipStream_t);         \
+    template int blas_dot<T>(ptrdiff_t, const T *, const T *, double *, double *, hipStream_t);         \
+    template int cg_update<T>(ptrdiff_t, T, const T *, const T *, T *, T *, double *, double *, hipStream_t); \
+    template int dofs_set_value<T>(ptrdiff_t, const int32_t *, T, T *, hipStream_t);                    \
+    template int dofs_copy<T>(ptrdiff_t, const int32_t *, const T *, T *, hipStream_t);
+INST(double)
+INST(float)
+#undef INST
+template int blas_cast<double, float>(ptrdiff_t, const double *, float *, hipStream_t);
+template int blas_cast<float, double>(ptrdiff_t, const float *, double *, hipStream_t);
+
+}  // namespace sfem_amd

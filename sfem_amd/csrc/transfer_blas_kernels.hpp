@@ -1,0 +1,40 @@
+// Host-callable launchers of transfer_blas_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace sfem_amd {
+
+template <typename T>
+int sshex8_prolongate(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint8_t *own, int vs, const T *from,
+                      T *to, hipStream_t s);
+template <typename T>
+int sshex8_restrict(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint16_t *count, int vs, const T *from,
+                    T *to, hipStream_t s);
+int node_owner_mask(ptrdiff_t E, int nxe, ptrdiff_t nnodes, const int32_t *idx, uint8_t *own, hipStream_t s);
+int node_incidence(ptrdiff_t n_entries, ptrdiff_t nnodes, const int32_t *idx, uint16_t *count, hipStream_t s);
+int sshex8_derefine(ptrdiff_t E, int Lf, int Lc, const int32_t *fine, int32_t *coarse, hipStream_t s);
+int sshex8_to_hex8(ptrdiff_t E, int L, const int32_t *ss, int32_t *h8, hipStream_t s);
+
+template <typename T> int blas_axpby(ptrdiff_t n, T a, const T *x, T b, T *y, hipStream_t s);
+template <typename T> int blas_zaxpby(ptrdiff_t n, T a, const T *x, T b, const T *y, T *z, hipStream_t s);
+template <typename T> int blas_scal(ptrdiff_t n, T a, T *x, hipStream_t s);
+template <typename T> int blas_values(ptrdiff_t n, T a, T *x, hipStream_t s);
+template <typename T> int blas_copy(ptrdiff_t n, const T *x, T *y, hipStream_t s);
+template <typename T> int blas_reciprocal(ptrdiff_t n, T a, T *x, hipStream_t s);
+template <typename T> int blas_xypaz(ptrdiff_t n, const T *x, const T *y, T a, T *z, hipStream_t s);
+template <typename S, typename D> int blas_cast(ptrdiff_t n, const S *x, D *y, hipStream_t s);
+template <typename T> int cheb3_update(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, T *p, T *x, hipStream_t s);
+// ws: device workspace of reduction_workspace_doubles() doubles
+template <typename T> int blas_dot(ptrdiff_t n, const T *x, const T *y, double *ws, double *result_host, hipStream_t s);
+template <typename T>
+int cg_update(ptrdiff_t n, T a, const T *p, const T *Ap, T *x, T *r, double *ws, double *rtr_host, hipStream_t s);
+template <typename T> int dofs_set_value(ptrdiff_t n, const int32_t *dofs, T v, T *x, hipStream_t s);
+template <typename T> int dofs_copy(ptrdiff_t n, const int32_t *dofs, const T *src, T *dst, hipStream_t s);
+size_t reduction_workspace_doubles();
+// y += (double) x
+int cast_accumulate(ptrdiff_t n, const float *x, double *y, hipStream_t s);
+
+}  // namespace sfem_amd

@@ -1,0 +1,264 @@
+"""GPU parity of the HIP operators against the CPU oracle, through the C ABI.
+
+Bar: FP64 apply/diag within 1e-12 relative (2-norm) of the oracle on the same float32
+geometry words; transfers likewise; integer data (connectivity after upload/derefine)
+bit-exact.  FP32/BF16 operators state their error against the FP64 oracle (< 1e-2).
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import sfem_amd as S  # noqa: E402
+
+RTOL = 1e-12
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def skewed_box(n, tet4=False, seed=0, perturb=0.0):
+    el, pts = O.box_mesh(n + 1, n + 1, n + 1, tet4=tet4)
+    A = np.array([[1.0, 0.2, 0.1], [0.0, 1.1, 0.15], [0.05, 0.0, 0.9]])
+    pts = (A @ pts.astype(np.float64)).astype(np.float32)
+    if perturb:
+        pts = (pts + perturb * np.random.default_rng(seed).uniform(-1, 1, pts.shape)).astype(np.float32)
+    return el, pts
+
+
+def gpu_apply(op, x):
+    y = torch.zeros(op.rows, dtype=x.dtype, device="cuda")
+    op.apply(x, y)
+    torch.cuda.synchronize()
+    return y
+
+
+def test_device_present():
+    assert S.device_count() >= 1
+
+
+# ---------------------------------------------------------------- Laplacians
+def test_tet4_laplacian_parity():
+    el, pts = skewed_box(8, tet4=True, perturb=0.02)
+    fff = O.tet4_fff(el, pts)
+    n = pts.shape[1]
+    x = np.random.default_rng(0).uniform(-1, 1, n)
+    yo = np.zeros(n)
+    O.lib.oracle_tet4_laplacian_apply(el.shape[1], el, fff, x, yo)
+    m = S.Mesh(S.TET4, 1, el, n)
+    op = S.laplacian(m, fff)
+    assert relerr(host(gpu_apply(op, dev(x))), yo) < RTOL
+    do = np.zeros(n)
+    O.lib.oracle_tet4_laplacian_diag(el.shape[1], el, fff, do)
+    d = torch.zeros(n, dtype=torch.float64, device="cuda")
+    op.diag(d)
+    assert relerr(host(d), do) < RTOL
+
+
+def test_hex8_laplacian_parity_with_coefficient():
+    el, pts = skewed_box(7, perturb=0.03)
+    fff = O.hex8_fff(el, pts)
+    coeff = np.random.default_rng(1).uniform(0.1, 10, el.shape[1]).astype(np.float32)
+    n = pts.shape[1]
+    x = np.random.default_rng(0).uniform(-1, 1, n)
+    yo = np.zeros(n)
+    O.lib.oracle_hex8_laplacian_apply(el.shape[1], el, fff, coeff, x, yo)
+    op = S.laplacian(S.Mesh(S.HEX8, 1, el, n), fff, coeff)
+    assert relerr(host(gpu_apply(op, dev(x))), yo) < RTOL
+    do = np.zeros(n)
+    O.lib.oracle_hex8_laplacian_diag(el.shape[1], el, fff, coeff, do)
+    d = torch.zeros(n, dtype=torch.float64, device="cuda")
+    op.diag(d)
+    assert relerr(host(d), do) < RTOL
+
+
+@pytest.mark.parametrize("L,n", [(1, 3), (2, 3), (3, 2), (4, 3), (5, 2), (6, 2), (8, 3), (10, 1), (12, 1), (16, 2)])
+def test_sshex8_laplacian_parity(L, n):
+    mel, mpts = skewed_box(n, perturb=0.02)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    fff = O.hex8_fff(mel, mpts)
+    coeff = np.random.default_rng(L).uniform(0.1, 10, mel.shape[1]).astype(np.float32)
+    N = pts.shape[1]
+    x = np.random.default_rng(0).uniform(-1, 1, N)
+    m = S.Mesh(S.SSHEX8, L, el, N)
+    np.testing.assert_array_equal(m.download(), el)
+    for c in (None, coeff):
+        yo = np.zeros(N)
+        O.lib.oracle_sshex8_laplacian_apply(L, mel.shape[1], el, fff, c, x, yo)
+        op = S.laplacian(m, fff, c)
+        y0 = dev(np.full(N, 0.5))  # accumulate semantics: y += A x
+        op.apply(dev(x), y0)
+        assert relerr(host(y0) - 0.5, yo) < RTOL
+        do = np.zeros(N)
+        O.lib.oracle_sshex8_laplacian_diag(L, mel.shape[1], el, fff, c, do)
+        d = torch.zeros(N, dtype=torch.float64, device="cuda")
+        op.diag(d)
+        assert relerr(host(d), do) < RTOL
+
+
+def test_sshex8_laplacian_fp32_bf16_error_stated():
+    L = 8
+    mel, mpts = skewed_box(3, perturb=0.02)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    fff = O.hex8_fff(mel, mpts)
+    coeff = np.random.default_rng(2).uniform(0.1, 10, mel.shape[1]).astype(np.float32)
+    N = pts.shape[1]
+    x = np.random.default_rng(0).uniform(-1, 1, N)
+    yo = np.zeros(N)
+    O.lib.oracle_sshex8_laplacian_apply(L, mel.shape[1], el, fff, coeff, x, yo)
+    op = S.laplacian(S.Mesh(S.SSHEX8, L, el, N), fff, coeff, real_type=S.FLOAT32)
+    err = relerr(host(gpu_apply(op, dev(x, torch.float32))).astype(np.float64), yo)
+    print(f"FP32 apply with BF16 geometry: relative error vs FP64 oracle = {err:.3e}")
+    assert err < 1e-2
+
+
+def test_empty_mesh_apply_is_noop():
+    el = np.zeros((27, 0), np.int32)
+    m = S.Mesh(S.SSHEX8, 2, el, 5)
+    op = S.laplacian(m, np.zeros((0, 6), np.float32))
+    y = gpu_apply(op, dev(np.ones(5)))
+    assert np.all(host(y) == 0)
+
+
+# ---------------------------------------------------------------- elasticity
+def test_hex8_elasticity_parity():
+    el, pts = skewed_box(6, perturb=0.02)
+    adj, det = O.hex8_adj_det(el, pts)
+    coeff = np.random.default_rng(4).uniform(0.1, 10, el.shape[1]).astype(np.float32)
+    mu, lam = O.lame(1.0, 0.3)
+    n = pts.shape[1] * 3
+    x = np.random.default_rng(0).uniform(-1, 1, n)
+    op = S.linear_elasticity(S.Mesh(S.HEX8, 1, el, pts.shape[1]), adj, det, mu, lam, coeff)
+    yo = np.zeros(n)
+    O.lib.oracle_hex8_elasticity_apply(el.shape[1], el, adj, det, mu, lam, coeff, x, yo)
+    assert relerr(host(gpu_apply(op, dev(x))), yo) < RTOL
+    do = np.zeros(n)
+    O.lib.oracle_hex8_elasticity_diag(el.shape[1], el, adj, det, mu, lam, coeff, do)
+    d = torch.zeros(n, dtype=torch.float64, device="cuda")
+    op.diag(d)
+    assert relerr(host(d), do) < RTOL
+
+
+@pytest.mark.parametrize("L,n", [(1, 2), (2, 2), (4, 2), (6, 1), (8, 2), (16, 1)])
+def test_sshex8_elasticity_parity(L, n):
+    mel, mpts = skewed_box(n, perturb=0.02)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    adj, det = O.hex8_adj_det(mel, mpts)
+    coeff = np.exp(np.random.default_rng(L).uniform(np.log(0.1), np.log(10), mel.shape[1])).astype(np.float32)
+    mu, lam = O.lame(1.0, 0.3)
+    N = pts.shape[1]
+    x = np.random.default_rng(0).uniform(-1, 1, 3 * N)
+    op = S.linear_elasticity(S.Mesh(S.SSHEX8, L, el, N), adj, det, mu, lam, coeff)
+    yo = np.zeros(3 * N)
+    O.lib.oracle_sshex8_elasticity_apply(L, mel.shape[1], el, adj, det, mu, lam, coeff, x, yo)
+    assert relerr(host(gpu_apply(op, dev(x))), yo) < RTOL
+    do = np.zeros(3 * N)
+    O.lib.oracle_sshex8_elasticity_diag(L, mel.shape[1], el, adj, det, mu, lam, coeff, do)
+    d = torch.zeros(3 * N, dtype=torch.float64, device="cuda")
+    op.diag(d)
+    assert relerr(host(d), do) < RTOL
+    bo = np.zeros(6 * N)
+    O.lib.oracle_sshex8_elasticity_block_diag(L, mel.shape[1], el, adj, det, mu, lam, coeff, bo)
+    b = torch.zeros(6 * N, dtype=torch.float64, device="cuda")
+    op.block_diag_sym(b)
+    torch.cuda.synchronize()
+    assert relerr(host(b), bo) < RTOL
+    if L == 8:
+        op32 = S.linear_elasticity(S.Mesh(S.SSHEX8, L, el, N), adj, det, mu, lam, coeff, real_type=S.FLOAT32)
+        err = relerr(host(gpu_apply(op32, dev(x, torch.float32))).astype(np.float64), yo)
+        print(f"FP32 elasticity apply with BF16 geometry: relative error vs FP64 oracle = {err:.3e}")
+        assert err < 1e-2
+
+
+# ---------------------------------------------------------------- transfers
+@pytest.mark.parametrize("vs", [1, 3])
+def test_transfers_parity(vs):
+    L = 8
+    mel, mpts = skewed_box(3, perturb=0.02)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    E = mel.shape[1]
+    fine = S.Mesh(S.SSHEX8, L, el, pts.shape[1])
+    count = O.incidence(el, pts.shape[1])
+    rng = np.random.default_rng(7)
+    prev_el, prev_mesh, prev_L = el, fine, L
+    for Lc in (4, 2, 1):
+        coarse = prev_mesh.derefine(Lc)
+        ec = O.derefine(L, Lc, el)
+        if Lc == 1:
+            np.testing.assert_array_equal(coarse.download(), O.to_hex8(1, ec))
+        else:
+            np.testing.assert_array_equal(coarse.download(), ec)
+        nc, nf = coarse.nnodes, prev_mesh.nnodes
+        assert nc == ec.max() + 1
+        xc = rng.uniform(-1, 1, nc * vs)
+        yo = np.zeros(nf * vs)
+        if Lc == 1:
+            O.lib.oracle_sshex8_hierarchical_prolongation(prev_L, E, prev_el, vs, xc, yo)
+        else:
+            O.lib.oracle_sshex8_prolongate(E, Lc, ec, prev_L, prev_el, vs, xc, yo)
+        P = S.prolongation(coarse, prev_mesh, vs)
+        y = torch.zeros(nf * vs, dtype=torch.float64, device="cuda")
+        P.apply(dev(xc), y)
+        torch.cuda.synchronize()
+        assert relerr(host(y), yo) < RTOL
+        xf = rng.uniform(-1, 1, nf * vs)
+        cnt = O.incidence(prev_el, nf)
+        ro = np.zeros(nc * vs)
+        if Lc == 1:
+            O.lib.oracle_sshex8_hierarchical_restriction(prev_L, E, prev_el, cnt, vs, xf, ro)
+        else:
+            O.lib.oracle_sshex8_restrict(E, prev_L, prev_el, cnt, Lc, ec, vs, xf, ro)
+        R = S.restriction(prev_mesh, coarse, vs)
+        r = torch.zeros(nc * vs, dtype=torch.float64, device="cuda")
+        R.apply(dev(xf), r)
+        torch.cuda.synchronize()
+        assert relerr(host(r), ro) < RTOL
+        prev_el, prev_mesh, prev_L = ec, coarse, Lc
+    del count
+
+
+def test_constrained_operator_copies_dofs():
+    L = 4
+    mel, mpts = skewed_box(2)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    fff = O.hex8_fff(mel, mpts)
+    N = pts.shape[1]
+    nodes = np.where(pts[0] < 1e-6)[0].astype(np.int32)
+    op = S.laplacian(S.Mesh(S.SSHEX8, L, el, N), fff)
+    cop = S.constrained(op, nodes, 1)
+    x = np.random.default_rng(0).uniform(-1, 1, N)
+    yo = np.zeros(N)
+    O.lib.oracle_sshex8_laplacian_apply(L, mel.shape[1], el, fff, None, x, yo)
+    O.lib.oracle_constraint_nodes_copy_vec(len(nodes), nodes, 1, 0, x, yo)
+    assert relerr(host(gpu_apply(cop, dev(x))), yo) < RTOL
+    z = dev(np.ones(N))
+    S.set_constrained_value(cop, 0.0, z)
+    torch.cuda.synchronize()
+    zz = host(z)
+    assert np.all(zz[nodes] == 0) and zz.sum() == N - len(nodes)
+
+
+# ---------------------------------------------------------------- BLAS
+def test_blas_reductions_deterministic_and_accurate():
+    rng = np.random.default_rng(0)
+    for n in (0, 1, 1000, 3_000_001):
+        a, b = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+        da, db = dev(a), dev(b)
+        d1 = S.dot(da, db)
+        d2 = S.dot(da, db)
+        assert d1 == d2  # bitwise reproducible
+        assert abs(d1 - a @ b) <= 1e-12 * max(1.0, np.abs(a) @ np.abs(b))
+        if n:
+            assert abs(S.norm2(da) - np.linalg.norm(a)) < 1e-12 * np.linalg.norm(a)
