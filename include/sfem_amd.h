@@ -194,16 +194,22 @@ int sfem_amd_cheb3_update(int rt, ptrdiff_t n, double alpha, double beta, const 
 int sfem_amd_cg_create(sfem_amd_op *A, sfem_amd_op *precond, int flexible, double rtol,
                        double atol, int max_it, sfem_amd_op **out);
 /* sfem::Chebyshev3<real_t> (algebra/sfem_Chebyshev3.hpp:57-303) of degree `max_it`; eig_max
- * from PowerMethod (algebra/sfem_PowerMethod.hpp:18-55) started from ones, with
- * eigen_solver_max_it iterations (rounded up to odd, like the reference) and tolerance. */
+ * from PowerMethod (algebra/sfem_PowerMethod.hpp:18-55) with eigen_solver_max_it iterations
+ * (rounded up to odd, like the reference) and tolerance, started from `init_vector` (device;
+ * NULL = ones, the reference's init_with_ones).  With a constrained operator pass ones with the
+ * constrained dofs zeroed: the identity rows would otherwise pin the estimate at 1. */
 int sfem_amd_cheb3_create(sfem_amd_op *A, int max_it, double scale_eig_max, double scale_eig_min,
                           int eigen_solver_max_it, double eigen_solver_tol, int initial_guess_zero,
-                          sfem_amd_op **out);
+                          const void *init_vector, sfem_amd_op **out);
 double sfem_amd_cheb3_eig_max(const sfem_amd_op *cheb);
 /* StationaryIteration with (scaled) Jacobi: x += omega D^-1 (b - A x), `steps` sweeps
  * (algebra/sfem_Stationary.hpp:50-80, algebra/sfem_ShiftableJacobi.hpp).  diag: device. */
 int sfem_amd_jacobi_create(sfem_amd_op *A, const void *diag, double omega, int steps,
                            sfem_amd_op **out);
+/* y += omega * x / diag (create_inverse_diagonal_scaling, used as the coarse-solve preconditioner by
+ * drivers/mgsolve.cpp:221-226).  diag: device vector of n entries (copied). */
+int sfem_amd_inverse_diagonal_create(int real_type, ptrdiff_t n, const void *diag, double omega,
+                                     sfem_amd_op **out);
 /* sfem::Multigrid<real_t> (algebra/sfem_Multigrid.hpp:27-347).  Level 0 is the finest;
  * add_level(op, smoother_or_solver, prolongation (NULL on level 0), restriction (NULL on
  * the coarsest)). */
@@ -219,6 +225,15 @@ int sfem_amd_op_iterations(const sfem_amd_op *op);
 int sfem_amd_cg_history(const sfem_amd_op *op, double *out, int cap);
 /* Status of the last solve (SFEM_SUCCESS if converged). */
 int sfem_amd_op_last_status(const sfem_amd_op *op);
+
+/* ------------------------------------------------------------------ measurement */
+/* Times `reps` applies y += A x on `stream` with HIP events (after `warmup` untimed ones).
+ * flush_bytes > 0: before every timed apply a kernel overwrites a scratch buffer of that
+ * size (outside the timed events) so the apply starts with cold caches.  Returns the
+ * total elapsed ms of the timed region (flush_bytes == 0: one event pair around all reps;
+ * else the sum of per-apply event pairs) in *total_ms. */
+int sfem_amd_op_benchmark(sfem_amd_op *op, const void *x, void *y, int warmup, int reps,
+                          size_t flush_bytes, void *stream, double *total_ms);
 
 #ifdef __cplusplus
 }

@@ -70,19 +70,39 @@ SIGNATURES = {
     "sfem_amd_cg_update_x_r": (_d, [_i, _pd, _d, _vp, _vp, _vp, _vp, _i, _vp]),
     "sfem_amd_cheb3_update": (_i, [_i, _pd, _d, _d, _vp, _vp, _vp, _vp, _vp]),
     "sfem_amd_cg_create": (_i, [_vp, _vp, _i, _d, _d, _i, C.POINTER(_vp)]),
-    "sfem_amd_cheb3_create": (_i, [_vp, _i, _d, _d, _i, _d, _i, C.POINTER(_vp)]),
+    "sfem_amd_cheb3_create": (_i, [_vp, _i, _d, _d, _i, _d, _i, _vp, C.POINTER(_vp)]),
     "sfem_amd_cheb3_eig_max": (_d, [_vp]),
     "sfem_amd_jacobi_create": (_i, [_vp, _vp, _d, _i, C.POINTER(_vp)]),
+    "sfem_amd_inverse_diagonal_create": (_i, [_i, _pd, _vp, _d, C.POINTER(_vp)]),
     "sfem_amd_mg_create": (_i, [_i, _i, _i, _d, C.POINTER(_vp)]),
     "sfem_amd_mg_add_level": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "sfem_amd_precision_bridge_create": (_i, [_vp, C.POINTER(_vp)]),
     "sfem_amd_op_iterations": (_i, [_vp]),
     "sfem_amd_cg_history": (_i, [_vp, _vp, _i]),
     "sfem_amd_op_last_status": (_i, [_vp]),
+    "sfem_amd_op_benchmark": (_i, [_vp, _vp, _vp, _i, _i, C.c_size_t, _vp, C.POINTER(_d)]),
 }
 
 
+def _preload_shared_hip_runtime():
+    """torch-ROCm wheels bundle a HIP runtime with the system runtime's SONAME
+    (libamdhip64.so.7).  Whichever loads first serves the whole process, and torch cannot
+    initialise on a runtime another library brought in.  Loading torch's copy first (without
+    importing torch) gives the library and torch one shared runtime in every import order."""
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.submodule_search_locations:
+        return
+    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
 def _load():
+    _preload_shared_hip_runtime()
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"sfem_amd: compiled library not found at {LIB_PATH}; build it with "
@@ -160,13 +180,63 @@ def adjugate_det(element_type, level, elements, points):
     return adj, det
 
 
-# ---------------------------------------------------------------- device objects
+# ---------------------------------------------------------------- device memory
+class DeviceArray:
+    """A flat device vector owned by the library's allocator (no torch needed)."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        self.ptr = lib.sfem_amd_malloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise SfemError("sfem_amd_malloc: " + lib.sfem_amd_last_error().decode())
+        self.zero()
+
+    @classmethod
+    def from_numpy(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype or a.dtype).ravel()
+        d = cls(a.size, a.dtype)
+        if a.size:
+            _check(lib.sfem_amd_memcpy_h2d(d.ptr, a.ctypes.data_as(_vp), d.nbytes), "memcpy_h2d")
+        return d
+
+    def numpy(self):
+        out = np.empty(self.n, self.dtype)
+        if self.n:
+            _check(lib.sfem_amd_memcpy_d2h(out.ctypes.data_as(_vp), self.ptr, self.nbytes), "memcpy_d2h")
+        return out
+
+    def zero(self, stream=None):
+        _check(lib.sfem_amd_memset(self.ptr, 0, self.nbytes, _stream_ptr(stream)), "memset")
+        return self
+
+    def copy_from(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype).ravel()
+        assert a.size == self.n
+        _check(lib.sfem_amd_memcpy_h2d(self.ptr, a.ctypes.data_as(_vp), self.nbytes), "memcpy_h2d")
+        return self
+
+    def numel(self):
+        return self.n
+
+    def element_size(self):
+        return self.dtype.itemsize
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and lib is not None:
+            lib.sfem_amd_free(self.ptr)
+            self.ptr = None
+
+
 def _dptr(t):
-    """Device pointer of a torch tensor (must be contiguous and on the GPU)."""
+    """Device pointer of a DeviceArray or of a contiguous GPU torch tensor."""
     if t is None:
         return None
+    if isinstance(t, DeviceArray):
+        return C.c_void_p(t.ptr)
     if not (hasattr(t, "is_cuda") and t.is_cuda and t.is_contiguous()):
-        raise SfemError("expected a contiguous GPU tensor")
+        raise SfemError("expected a DeviceArray or a contiguous GPU tensor")
     return C.c_void_p(t.data_ptr())
 
 
@@ -233,10 +303,10 @@ class Operator:
         return lib.sfem_amd_op_real_type(self.h)
 
     def _check_vec(self, t, n):
-        import torch
-        want = torch.float64 if self.real_type == FLOAT64 else torch.float32
-        if t.dtype != want or t.numel() != n:
-            raise SfemError(f"vector must be {want} with {n} entries (got {t.dtype}, {t.numel()})")
+        size = 8 if self.real_type == FLOAT64 else 4
+        if t.element_size() != size or t.numel() != n:
+            raise SfemError(f"vector must have {size}-byte reals and {n} entries "
+                            f"(got {t.element_size()}-byte, {t.numel()})")
 
     def apply(self, x, y, stream=None):
         self._check_vec(x, self.cols)
@@ -322,9 +392,9 @@ def cg(A, precond=None, flexible=False, rtol=1e-10, atol=1e-16, max_it=10000):
 
 
 def cheb3(A, max_it=3, scale_eig_max=1.0, scale_eig_min=0.06, eigen_solver_max_it=1000, eigen_solver_tol=1e-6,
-          initial_guess_zero=False):
+          initial_guess_zero=False, init_vector=None):
     return _new_op("sfem_amd_cheb3_create", A.h, max_it, scale_eig_max, scale_eig_min, eigen_solver_max_it,
-                   eigen_solver_tol, int(initial_guess_zero), keep=[A])
+                   eigen_solver_tol, int(initial_guess_zero), _dptr(init_vector), keep=[A])
 
 
 def cheb3_eig_max(op):
@@ -333,6 +403,11 @@ def cheb3_eig_max(op):
 
 def jacobi(A, diag, omega=1.0, steps=3):
     return _new_op("sfem_amd_jacobi_create", A.h, _dptr(diag), omega, steps, keep=[A])
+
+
+def inverse_diagonal(diag, omega=1.0):
+    rt = FLOAT64 if diag.element_size() == 8 else FLOAT32
+    return _new_op("sfem_amd_inverse_diagonal_create", rt, diag.numel(), _dptr(diag), omega)
 
 
 class Multigrid(Operator):
@@ -351,6 +426,14 @@ class Multigrid(Operator):
 
 def precision_bridge(op32):
     return _new_op("sfem_amd_precision_bridge_create", op32.h, keep=[op32])
+
+
+def benchmark(op, x, y, warmup=2, reps=10, flush_bytes=0, stream=None):
+    """Total elapsed ms (HIP events) of `reps` applies; see sfem_amd_op_benchmark."""
+    ms = C.c_double()
+    _check(lib.sfem_amd_op_benchmark(op.h, _dptr(x), _dptr(y), warmup, reps, flush_bytes, _stream_ptr(stream),
+                                     C.byref(ms)), "op_benchmark")
+    return ms.value
 
 
 def dot(x, y, stream=None):

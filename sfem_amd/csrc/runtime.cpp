@@ -760,7 +760,7 @@ int sfem_amd_cg_create(sfem_amd_op *A, sfem_amd_op *P, int flexible, double rtol
 extern "C++" {
 template <typename T>
 static int cheb3_create_t(sfem_amd_op *A, int max_it, double smax, double smin, int eig_its, double eig_tol, int zero,
-                          sfem_amd_op **out) {
+                          const void *init, sfem_amd_op **out) {
     auto c = std::make_shared<sfem::Chebyshev3<T>>();
     c->default_init();
     c->set_op(view<T>(A->impl));
@@ -770,7 +770,10 @@ static int cheb3_create_t(sfem_amd_op *A, int max_it, double smax, double smin, 
     c->eigen_solver_max_it = eig_its;
     c->eigen_solver_tol = (T)eig_tol;
     c->is_initial_guess_zero = zero != 0;
-    c->init_with_ones();
+    if (init)
+        c->init((const T *)init);
+    else
+        c->init_with_ones();
     if (!std::isfinite((double)c->eig_max) || c->eig_max <= 0) {
         set_last_error("cheb3: power method failed");
         return SFEM_FAILURE;
@@ -786,10 +789,11 @@ static int cheb3_create_t(sfem_amd_op *A, int max_it, double smax, double smin, 
 }  // extern "C++"
 
 int sfem_amd_cheb3_create(sfem_amd_op *A, int max_it, double smax, double smin, int eig_its, double eig_tol, int zero,
-                          sfem_amd_op **out) {
+                          const void *init, sfem_amd_op **out) {
     CHECK_HANDLE(A);
-    if (A->impl->real_type() == SFEM_AMD_FLOAT64) return cheb3_create_t<double>(A, max_it, smax, smin, eig_its, eig_tol, zero, out);
-    return cheb3_create_t<float>(A, max_it, smax, smin, eig_its, eig_tol, zero, out);
+    if (A->impl->real_type() == SFEM_AMD_FLOAT64)
+        return cheb3_create_t<double>(A, max_it, smax, smin, eig_its, eig_tol, zero, init, out);
+    return cheb3_create_t<float>(A, max_it, smax, smin, eig_its, eig_tol, zero, init, out);
 }
 
 double sfem_amd_cheb3_eig_max(const sfem_amd_op *op) {
@@ -878,6 +882,33 @@ int sfem_amd_mg_add_level(sfem_amd_op *mgh, sfem_amd_op *op, sfem_amd_op *sm, sf
     return SFEM_FAILURE;
 }
 
+extern "C++" {
+template <typename T>
+struct InverseDiagonal final : OpBase {
+    DevArray<T> inv;
+    int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
+    ptrdiff_t rows() const override { return (ptrdiff_t)inv.n; }
+    int apply(const void *x, void *y, hipStream_t s) override {
+        return blas_xypaz<T>((ptrdiff_t)inv.n, inv.p, (const T *)x, T(1), (T *)y, s);
+    }
+};
+}  // extern "C++"
+
+int sfem_amd_inverse_diagonal_create(int rt, ptrdiff_t n, const void *diag, double omega, sfem_amd_op **out) {
+    SFEM_AMD_REQUIRE(diag && n >= 0, "inverse_diagonal: diag required");
+    auto make = [&](auto op, auto *d) -> int {
+        using T = std::remove_const_t<std::remove_pointer_t<decltype(d)>>;
+        if (op->inv.alloc(n)) return SFEM_FAILURE;
+        if (blas_copy<T>(n, d, op->inv.p, nullptr) || blas_reciprocal<T>(n, (T)omega, op->inv.p, nullptr))
+            return SFEM_FAILURE;
+        *out = wrap(op);
+        return SFEM_SUCCESS;
+    };
+    if (rt == SFEM_AMD_FLOAT64) return make(std::make_shared<InverseDiagonal<double>>(), (const double *)diag);
+    SFEM_AMD_REQUIRE(rt == SFEM_AMD_FLOAT32, "inverse_diagonal: real type");
+    return make(std::make_shared<InverseDiagonal<float>>(), (const float *)diag);
+}
+
 int sfem_amd_precision_bridge_create(sfem_amd_op *op32, sfem_amd_op **out) {
     CHECK_HANDLE(op32);
     SFEM_AMD_REQUIRE(op32->impl->real_type() == SFEM_AMD_FLOAT32, "precision bridge: FP32 operator required");
@@ -885,6 +916,42 @@ int sfem_amd_precision_bridge_create(sfem_amd_op *op32, sfem_amd_op **out) {
     b->op32 = op32->impl;
     *out = wrap(b);
     return SFEM_SUCCESS;
+}
+
+int sfem_amd_op_benchmark(sfem_amd_op *op, const void *x, void *y, int warmup, int reps, size_t flush_bytes,
+                          void *stream, double *total_ms) {
+    CHECK_HANDLE(op);
+    SFEM_AMD_REQUIRE(reps > 0 && total_ms, "benchmark: reps > 0 and an output pointer required");
+    hipStream_t s = as_stream(stream);
+    for (int i = 0; i < warmup; i++)
+        if (op->impl->apply(x, y, s)) return SFEM_FAILURE;
+    DevArray<double> scratch;
+    if (flush_bytes && scratch.alloc((flush_bytes + 7) / 8)) return SFEM_FAILURE;
+    std::vector<hipEvent_t> ev(2 * (size_t)(flush_bytes ? reps : 1));
+    for (auto &e : ev) SFEM_AMD_HIP_CHECK(hipEventCreate(&e));
+    int err = SFEM_SUCCESS;
+    if (!flush_bytes) {
+        SFEM_AMD_HIP_CHECK(hipEventRecord(ev[0], s));
+        for (int i = 0; i < reps && !err; i++) err = op->impl->apply(x, y, s);
+        SFEM_AMD_HIP_CHECK(hipEventRecord(ev[1], s));
+    } else {
+        for (int i = 0; i < reps && !err; i++) {
+            err = blas_values<double>((ptrdiff_t)scratch.n, (double)i, scratch.p, s);
+            SFEM_AMD_HIP_CHECK(hipEventRecord(ev[2 * i], s));
+            if (!err) err = op->impl->apply(x, y, s);
+            SFEM_AMD_HIP_CHECK(hipEventRecord(ev[2 * i + 1], s));
+        }
+    }
+    SFEM_AMD_HIP_CHECK(hipStreamSynchronize(s));
+    double total = 0;
+    for (size_t i = 0; i < ev.size(); i += 2) {
+        float ms = 0;
+        SFEM_AMD_HIP_CHECK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        total += ms;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    *total_ms = total;
+    return err;
 }
 
 }  // extern "C"

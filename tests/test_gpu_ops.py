@@ -9,7 +9,6 @@ import pytest
 
 import oracle_lib as O
 
-torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 import sfem_amd as S  # noqa: E402
@@ -17,12 +16,16 @@ import sfem_amd as S  # noqa: E402
 RTOL = 1e-12
 
 
-def dev(a, dtype=torch.float64):
-    return torch.as_tensor(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
+def dev(a, dtype=np.float64):
+    return S.DeviceArray.from_numpy(np.asarray(a), dtype)
 
 
 def host(t):
-    return t.detach().cpu().numpy()
+    return t.numpy()
+
+
+def zeros(n, dtype=np.float64):
+    return S.DeviceArray(n, dtype)
 
 
 def relerr(a, b):
@@ -39,9 +42,9 @@ def skewed_box(n, tet4=False, seed=0, perturb=0.0):
 
 
 def gpu_apply(op, x):
-    y = torch.zeros(op.rows, dtype=x.dtype, device="cuda")
+    y = zeros(op.rows, x.dtype)
     op.apply(x, y)
-    torch.cuda.synchronize()
+    S.synchronize()
     return y
 
 
@@ -62,7 +65,7 @@ def test_tet4_laplacian_parity():
     assert relerr(host(gpu_apply(op, dev(x))), yo) < RTOL
     do = np.zeros(n)
     O.lib.oracle_tet4_laplacian_diag(el.shape[1], el, fff, do)
-    d = torch.zeros(n, dtype=torch.float64, device="cuda")
+    d = zeros(n)
     op.diag(d)
     assert relerr(host(d), do) < RTOL
 
@@ -79,7 +82,7 @@ def test_hex8_laplacian_parity_with_coefficient():
     assert relerr(host(gpu_apply(op, dev(x))), yo) < RTOL
     do = np.zeros(n)
     O.lib.oracle_hex8_laplacian_diag(el.shape[1], el, fff, coeff, do)
-    d = torch.zeros(n, dtype=torch.float64, device="cuda")
+    d = zeros(n)
     op.diag(d)
     assert relerr(host(d), do) < RTOL
 
@@ -103,7 +106,7 @@ def test_sshex8_laplacian_parity(L, n):
         assert relerr(host(y0) - 0.5, yo) < RTOL
         do = np.zeros(N)
         O.lib.oracle_sshex8_laplacian_diag(L, mel.shape[1], el, fff, c, do)
-        d = torch.zeros(N, dtype=torch.float64, device="cuda")
+        d = zeros(N)
         op.diag(d)
         assert relerr(host(d), do) < RTOL
 
@@ -119,7 +122,7 @@ def test_sshex8_laplacian_fp32_bf16_error_stated():
     yo = np.zeros(N)
     O.lib.oracle_sshex8_laplacian_apply(L, mel.shape[1], el, fff, coeff, x, yo)
     op = S.laplacian(S.Mesh(S.SSHEX8, L, el, N), fff, coeff, real_type=S.FLOAT32)
-    err = relerr(host(gpu_apply(op, dev(x, torch.float32))).astype(np.float64), yo)
+    err = relerr(host(gpu_apply(op, dev(x, np.float32))).astype(np.float64), yo)
     print(f"FP32 apply with BF16 geometry: relative error vs FP64 oracle = {err:.3e}")
     assert err < 1e-2
 
@@ -146,7 +149,7 @@ def test_hex8_elasticity_parity():
     assert relerr(host(gpu_apply(op, dev(x))), yo) < RTOL
     do = np.zeros(n)
     O.lib.oracle_hex8_elasticity_diag(el.shape[1], el, adj, det, mu, lam, coeff, do)
-    d = torch.zeros(n, dtype=torch.float64, device="cuda")
+    d = zeros(n)
     op.diag(d)
     assert relerr(host(d), do) < RTOL
 
@@ -166,18 +169,18 @@ def test_sshex8_elasticity_parity(L, n):
     assert relerr(host(gpu_apply(op, dev(x))), yo) < RTOL
     do = np.zeros(3 * N)
     O.lib.oracle_sshex8_elasticity_diag(L, mel.shape[1], el, adj, det, mu, lam, coeff, do)
-    d = torch.zeros(3 * N, dtype=torch.float64, device="cuda")
+    d = zeros(3 * N)
     op.diag(d)
     assert relerr(host(d), do) < RTOL
     bo = np.zeros(6 * N)
     O.lib.oracle_sshex8_elasticity_block_diag(L, mel.shape[1], el, adj, det, mu, lam, coeff, bo)
-    b = torch.zeros(6 * N, dtype=torch.float64, device="cuda")
+    b = zeros(6 * N)
     op.block_diag_sym(b)
-    torch.cuda.synchronize()
+    S.synchronize()
     assert relerr(host(b), bo) < RTOL
     if L == 8:
         op32 = S.linear_elasticity(S.Mesh(S.SSHEX8, L, el, N), adj, det, mu, lam, coeff, real_type=S.FLOAT32)
-        err = relerr(host(gpu_apply(op32, dev(x, torch.float32))).astype(np.float64), yo)
+        err = relerr(host(gpu_apply(op32, dev(x, np.float32))).astype(np.float64), yo)
         print(f"FP32 elasticity apply with BF16 geometry: relative error vs FP64 oracle = {err:.3e}")
         assert err < 1e-2
 
@@ -209,9 +212,9 @@ def test_transfers_parity(vs):
         else:
             O.lib.oracle_sshex8_prolongate(E, Lc, ec, prev_L, prev_el, vs, xc, yo)
         P = S.prolongation(coarse, prev_mesh, vs)
-        y = torch.zeros(nf * vs, dtype=torch.float64, device="cuda")
+        y = zeros(nf * vs)
         P.apply(dev(xc), y)
-        torch.cuda.synchronize()
+        S.synchronize()
         assert relerr(host(y), yo) < RTOL
         xf = rng.uniform(-1, 1, nf * vs)
         cnt = O.incidence(prev_el, nf)
@@ -221,9 +224,9 @@ def test_transfers_parity(vs):
         else:
             O.lib.oracle_sshex8_restrict(E, prev_L, prev_el, cnt, Lc, ec, vs, xf, ro)
         R = S.restriction(prev_mesh, coarse, vs)
-        r = torch.zeros(nc * vs, dtype=torch.float64, device="cuda")
+        r = zeros(nc * vs)
         R.apply(dev(xf), r)
-        torch.cuda.synchronize()
+        S.synchronize()
         assert relerr(host(r), ro) < RTOL
         prev_el, prev_mesh, prev_L = ec, coarse, Lc
     del count
@@ -245,7 +248,7 @@ def test_constrained_operator_copies_dofs():
     assert relerr(host(gpu_apply(cop, dev(x))), yo) < RTOL
     z = dev(np.ones(N))
     S.set_constrained_value(cop, 0.0, z)
-    torch.cuda.synchronize()
+    S.synchronize()
     zz = host(z)
     assert np.all(zz[nodes] == 0) and zz.sum() == N - len(nodes)
 

@@ -1,0 +1,233 @@
+"""GPU solver parity (CG, Chebyshev3/PowerMethod, one V-cycle, MG-PCG, mixed-precision
+flexible CG) against the numpy solver oracle (oracle/solvers.py) driving the C oracle."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+import sfem_amd as S  # noqa: E402
+from sfem_amd import gmg  # noqa: E402
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import solvers as SO  # noqa: E402
+
+
+def dev(a, dtype=np.float64):
+    return S.DeviceArray.from_numpy(np.asarray(a), dtype)
+
+
+def host(t):
+    return t.numpy()
+
+
+def zeros(n, dtype=np.float64):
+    return S.DeviceArray(n, dtype)
+
+
+def true_residual(A, b, x):
+    r = zeros(b.n, b.dtype)
+    A.apply(x, r)
+    bb = b.numpy()
+    return float(np.linalg.norm(bb - r.numpy()) / np.linalg.norm(bb))
+
+
+def oracle_constrained(apply, nodes, bs=1):
+    dofs = (nodes[:, None].astype(np.int64) * bs + np.arange(bs)).ravel()
+
+    def f(x, y):
+        apply(x, y)
+        y[dofs] = x[dofs]
+    return f
+
+
+def test_cg_tet4_poisson_matches_oracle_iterations():
+    """Config 1 shape: 32^3 cubes x 6 tets, Dirichlet x = 0, f = 1, CG to 1e-10."""
+    el, pts = O.box_mesh(33, 33, 33, tet4=True)
+    fff = O.tet4_fff(el, pts)
+    N = pts.shape[1]
+    nodes = np.where(pts[0] == 0)[0].astype(np.int32)
+    b = np.ones(N)
+    b[nodes] = 0
+    A_o = oracle_constrained(lambda x, y: O.lib.oracle_tet4_laplacian_apply(el.shape[1], el, fff, x, y), nodes)
+    xo, it_o, st_o, _ = SO.cg(A_o, b, np.zeros(N), rtol=1e-10, atol=1e-16, max_it=10000)
+    assert st_o == 0
+    A = S.constrained(S.laplacian(S.Mesh(S.TET4, 1, el, N), fff), nodes)
+    solver = S.cg(A, rtol=1e-10, atol=1e-16, max_it=10000)
+    x = zeros(N)
+    solver.apply(dev(b), x)
+    print(f"tet4 CG: gpu {solver.iterations} its, oracle {it_o} its")
+    assert solver.last_status == 0
+    assert abs(solver.iterations - it_o) <= 1
+    assert np.linalg.norm(host(x) - xo) / np.linalg.norm(xo) < 1e-8
+    assert true_residual(A, dev(b), x) < 2e-10
+
+
+def _ss_poisson(L, n, coeff_seed=None):
+    mel, mpts = O.box_mesh(n + 1, n + 1, n + 1)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    fff = O.hex8_fff(mel, mpts)
+    N = pts.shape[1]
+    nodes = np.where((pts.min(0) < 1e-6) | (pts.max(0) > 1 - 1e-6))[0].astype(np.int32)
+    coeff = None
+    if coeff_seed is not None:
+        coeff = (10 ** np.random.default_rng(coeff_seed).uniform(0, 3, mel.shape[1])).astype(np.float32)
+    return mel, mpts, el, pts, fff, N, nodes, coeff
+
+
+def test_power_method_and_chebyshev_parity():
+    L = 4
+    mel, mpts, el, pts, fff, N, nodes, coeff = _ss_poisson(L, 3, coeff_seed=1)
+    E = mel.shape[1]
+    A_o = oracle_constrained(lambda x, y: O.lib.oracle_sshex8_laplacian_apply(L, E, el, fff, coeff, x, y), nodes)
+    v0 = np.ones(N)
+    v0[nodes] = 0
+    lam_o = SO.power_method(A_o, N, 10, 1e-5, v0=v0)
+    A = S.constrained(S.laplacian(S.Mesh(S.SSHEX8, L, el, N), fff, coeff), nodes)
+    ch = S.cheb3(A, max_it=3, scale_eig_max=1.02, eigen_solver_max_it=10, eigen_solver_tol=1e-5, init_vector=dev(v0))
+    lam = S.cheb3_eig_max(ch)
+    assert abs(lam - lam_o) < 1e-10 * lam_o
+    rng = np.random.default_rng(0)
+    b = rng.uniform(-1, 1, N)
+    b[nodes] = 0
+    x0 = rng.uniform(-1, 1, N)
+    x0[nodes] = 0
+    xo = SO.Chebyshev3(A_o, N, max_it=3, scale_eig_max=1.02, eig_max=lam_o)(b, x0.copy())
+    x = dev(x0)
+    ch.apply(dev(b), x)
+    S.synchronize()
+    assert np.linalg.norm(host(x) - xo) / np.linalg.norm(xo) < 1e-12
+
+
+def _oracle_hierarchy(L, mel, mpts, el, pts, fff, coeff, nodes, lams):
+    """Oracle V-cycle mirroring sfem_amd.gmg.SSGMG (same eig_max values)."""
+    E = mel.shape[1]
+    levels = gmg.derefinement_levels(L)
+    conns = [el] + [O.derefine(L, l, el) for l in levels[1:]]
+    out = []
+    for i, (l, c) in enumerate(zip(levels, conns)):
+        n = c.max() + 1
+        bc = nodes[nodes < n]
+        if l == 1:
+            h8 = O.to_hex8(1, c)
+            ap = (lambda h8: lambda x, y: O.lib.oracle_hex8_laplacian_apply(E, h8, fff, coeff, x, y))(h8)
+        else:
+            ap = (lambda l, c: lambda x, y: O.lib.oracle_sshex8_laplacian_apply(l, E, c, fff, coeff, x, y))(l, c)
+        A = oracle_constrained(ap, bc)
+        if i < len(levels) - 1:
+            sm = SO.Chebyshev3(A, n, max_it=3, scale_eig_max=1.02, eig_max=lams[i])
+        else:
+            d = np.zeros(n)
+            O.lib.oracle_hex8_laplacian_diag(E, O.to_hex8(1, c), fff, coeff, d)
+            d[bc] = 1
+
+            def sm(b, x, A=A, d=d):
+                x[:] = SO.cg(A, b, np.zeros_like(b), rtol=1e-12, atol=1e-30,
+                             precond=lambda r, z: z.__iadd__(r / d))[0]
+                return x
+        out.append([A, sm, None, None, n, bc, c, l])
+    for i in range(len(out) - 1):
+        f, co = out[i], out[i + 1]
+        cnt = O.incidence(f[6], f[4])
+
+        def R(r, f=f, co=co, cnt=cnt):
+            rc = np.zeros(co[4])
+            if co[7] == 1:
+                O.lib.oracle_sshex8_hierarchical_restriction(f[7], E, f[6], cnt, 1, r, rc)
+            else:
+                O.lib.oracle_sshex8_restrict(E, f[7], f[6], cnt, co[7], co[6], 1, r, rc)
+            rc[co[5]] = 0
+            return rc
+
+        def P(xc, f=f, co=co):
+            xf = np.zeros(f[4])
+            if co[7] == 1:
+                O.lib.oracle_sshex8_hierarchical_prolongation(f[7], E, f[6], 1, xc, xf)
+            else:
+                O.lib.oracle_sshex8_prolongate(E, co[7], co[6], f[7], f[6], 1, xc, xf)
+            xf[f[5]] = 0
+            return xf
+        f[3] = R
+        co[2] = P
+    return SO.Multigrid([tuple(o[:4]) for o in out], max_it=1, atol=0)
+
+
+def test_single_vcycle_matches_oracle():
+    L = 4
+    mel, mpts, el, pts, fff, N, nodes, coeff = _ss_poisson(L, 3, coeff_seed=2)
+    h = gmg.SSGMG(S.Mesh(S.SSHEX8, L, el, N), "laplacian", fff, nodes, coeff=coeff, coarse_rtol=1e-12)
+    lams = [S.cheb3_eig_max(lv.smoother) for lv in h.levels[:-1]]
+    mg_o = _oracle_hierarchy(L, mel, mpts, el, pts, fff, coeff, nodes, lams)
+    rng = np.random.default_rng(3)
+    b = rng.uniform(0, 1, N)
+    b[nodes] = 0
+    xo = mg_o(b, np.zeros(N))
+    x = zeros(N)
+    h.mg.apply(dev(b), x)
+    S.synchronize()
+    err = np.linalg.norm(host(x) - xo) / np.linalg.norm(xo)
+    print(f"one V-cycle: relative difference to the oracle {err:.2e}")
+    assert err < 1e-9
+
+
+@pytest.mark.parametrize("coeff_seed", [None, 5])
+def test_mg_pcg_poisson_converges_and_matches_oracle(coeff_seed):
+    L = 8
+    mel, mpts, el, pts, fff, N, nodes, coeff = _ss_poisson(L, 4, coeff_seed)
+    h = gmg.SSGMG(S.Mesh(S.SSHEX8, L, el, N), "laplacian", fff, nodes, coeff=coeff, coarse_rtol=1e-12)
+    lams = [S.cheb3_eig_max(lv.smoother) for lv in h.levels[:-1]]
+    b = np.random.default_rng(0).uniform(0, 1, N)
+    b[nodes] = 0
+    solver = S.cg(h.fine.A, h.mg, rtol=1e-10, max_it=200)
+    x = zeros(N)
+    solver.apply(dev(b), x)
+    mg_o = _oracle_hierarchy(L, mel, mpts, el, pts, fff, coeff, nodes, lams)
+    xo, it_o, st_o, _ = SO.cg(h_apply := mg_o.levels[0][0], b, np.zeros(N), rtol=1e-10, max_it=200,
+                              precond=lambda r, z: z.__iadd__(mg_o(r, np.zeros_like(r))))
+    print(f"MG-PCG: gpu {solver.iterations} its, oracle {it_o} its")
+    assert solver.last_status == 0 and st_o == 0
+    assert abs(solver.iterations - it_o) <= 1
+    assert true_residual(h.fine.A, dev(b), x) < 1e-8
+    assert np.linalg.norm(host(x) - xo) / np.linalg.norm(xo) < 1e-7
+
+
+def test_mixed_precision_flexible_cg_reaches_fp64_tolerance():
+    L = 8
+    mel, mpts, el, pts, fff, N, nodes, coeff = _ss_poisson(L, 4, coeff_seed=7)
+    mesh = S.Mesh(S.SSHEX8, L, el, N)
+    h64 = gmg.SSGMG(mesh, "laplacian", fff, nodes, coeff=coeff)
+    h32 = gmg.SSGMG(mesh, "laplacian", fff, nodes, coeff=coeff, real_type=S.FLOAT32)
+    P = S.precision_bridge(h32.mg)
+    solver = S.cg(h64.fine.A, P, flexible=True, rtol=1e-10, max_it=200)
+    b = dev(np.where(np.isin(np.arange(N), nodes), 0.0, np.random.default_rng(0).uniform(0, 1, N)))
+    x = zeros(N)
+    solver.apply(b, x)
+    res = true_residual(h64.fine.A, b, x)
+    print(f"mixed MG-FCG: {solver.iterations} its, true FP64 residual {res:.2e}")
+    assert solver.last_status == 0 and res < 1e-10
+
+
+def test_elasticity_mg_pcg_converges():
+    L = 4
+    mel, mpts = O.box_mesh(4, 4, 4)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    adj, det = O.hex8_adj_det(mel, mpts)
+    N = pts.shape[1]
+    nodes = np.where(pts[2] < 1e-6)[0].astype(np.int32)  # clamped at z = 0
+    mu, lam = O.lame(1.0, 0.3)
+    coeff = np.exp(np.random.default_rng(1).uniform(np.log(0.1), np.log(10), mel.shape[1])).astype(np.float32)
+    h = gmg.SSGMG(S.Mesh(S.SSHEX8, L, el, N), "elasticity", (adj, det), nodes, coeff=coeff, mu=mu, lam=lam,
+                  coarse_rtol=1e-12)
+    b = np.zeros((N, 3))
+    b[:, 2] = -1
+    b[nodes] = 0
+    solver = S.cg(h.fine.A, h.mg, rtol=1e-10, max_it=300)
+    x = zeros(3 * N)
+    solver.apply(dev(b.ravel()), x)
+    res = true_residual(h.fine.A, dev(b.ravel()), x)
+    print(f"elasticity MG-PCG: {solver.iterations} its, residual {res:.2e}")
+    assert solver.last_status == 0 and res < 1e-8
