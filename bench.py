@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Benchmark: sshex8 Laplacian L=8 on a 16^3 macro hex8 unit cube (2,146,689 DOF).
+
+One step = one FP64 matrix-free apply y += A x (x ~ U[-1,1], seed 0) on one MI355X.
+Prints ONE JSON line: throughput in DOF/s, the apply kernel's roofline (algorithmic bytes
+over its HIP-event-timed average duration), the time to relative residual 1e-10 of plain
+FP64 MG-PCG and of FP32/BF16 MG inside FP64 flexible CG (zero Dirichlet on all faces,
+RHS ~ U[0,1]), and the CPU baseline: the oracle's OpenMP apply on the host cores.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-solve] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "apply DOF/s (% HBM roofline); time-to-1e-10 MG-PCG, mixed/learned vs FP64"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+
+
+def build_problem(n_macro=16, L=8):
+    import sfem_amd as S
+    mel, mpts = S.box_mesh(n_macro + 1, n_macro + 1, n_macro + 1, element_type=S.HEX8)
+    el, pts = S.sshex8_refine(L, mel, mpts)
+    fff = S.fff(S.SSHEX8, L, el, pts)
+    return mel, mpts, el, pts, fff
+
+
+def algorithmic_bytes(N, E, L, value_bytes=8, geom_bytes=4):
+    # one read of x, one write of y, int32 connectivity, 6 FFF words per macro element
+    return 2 * value_bytes * N + 4 * (L + 1) ** 3 * E + geom_bytes * 6 * E
+
+
+def cpu_baseline(el, fff, N, L, budget_s=12.0, max_reps=20):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    E = el.shape[1]
+    x = np.random.default_rng(0).uniform(-1, 1, N)
+    y = np.zeros(N)
+    el = np.ascontiguousarray(el)
+    for _ in range(2):
+        O.lib.oracle_sshex8_laplacian_apply(L, E, el, fff, None, x, y)
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < max_reps and (len(times) < 5 or time.perf_counter() - t_start < budget_s):
+        t0 = time.perf_counter()
+        O.lib.oracle_sshex8_laplacian_apply(L, E, el, fff, None, x, y)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": N / med, "unit": "DOF/s", "cores": int(O.lib.oracle_num_threads()), "kind": "port",
+            "sample": f"full apply of this workload ({N} DOF), median of {len(times)} after 2 warm-up; "
+                      f"OpenMP restatement of affine_sshex8_laplacian_stencil_apply_fff"}
+
+
+def solve_times(S, mesh, fff, pts, rtol=1e-10):
+    from sfem_amd import gmg
+    N = pts.shape[1]
+    bc = np.where((pts.min(0) < 1e-6) | (pts.max(0) > 1 - 1e-6))[0].astype(np.int32)
+    rhs = np.random.default_rng(1).uniform(0, 1, N)
+    rhs[bc] = 0
+    b = S.DeviceArray.from_numpy(rhs)
+    out = {}
+
+    def true_res(A, x):
+        r = S.DeviceArray(N)
+        A.apply(x, r)
+        return float(np.linalg.norm(rhs - r.numpy()) / np.linalg.norm(rhs))
+
+    t0 = time.perf_counter()
+    h64 = gmg.SSGMG(mesh, "laplacian", fff, bc)
+    setup64 = time.perf_counter() - t0
+    for name in ("fp64", "mixed"):
+        if name == "fp64":
+            solver = S.cg(h64.fine.A, h64.mg, rtol=rtol, max_it=500)
+            setup = setup64
+        else:
+            t0 = time.perf_counter()
+            h32 = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32)
+            P = S.precision_bridge(h32.mg)
+            setup = time.perf_counter() - t0
+            solver = S.cg(h64.fine.A, P, flexible=True, rtol=rtol, max_it=500)
+        # one untimed solve to warm up code paths, then the timed one from x = 0
+        x = S.DeviceArray(N)
+        solver.apply(b, x)
+        x.zero()
+        S.synchronize()
+        t0 = time.perf_counter()
+        solver.apply(b, x)
+        S.synchronize()
+        t = time.perf_counter() - t0
+        out[name] = {"seconds": t, "iterations": solver.iterations, "status": solver.last_status,
+                     "true_rel_residual": true_res(h64.fine.A, x), "setup_seconds": setup,
+                     "preconditioner": ("V-cycle 8-4-2-1, Chebyshev3 FP64, FP64 coarse PCG" if name == "fp64" else
+                                        "V-cycle 8-4-2-1 in FP32 with BF16 geometry, FP64 coarse PCG; outer FP64 "
+                                        "flexible CG")}
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-flush", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (every box here has one GPU; "
+              "the multi-GPU sharding is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import sfem_amd as S
+    if S.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    L, n_macro = 8, 16
+    mel, mpts, el, pts, fff = build_problem(n_macro, L)
+    N, E = pts.shape[1], el.shape[1]
+    mesh = S.Mesh(S.SSHEX8, L, el, N)
+    op = S.laplacian(mesh, fff)
+    x = S.DeviceArray.from_numpy(np.random.default_rng(0).uniform(-1, 1, N))
+    y = S.DeviceArray(N)
+
+    total_ms = S.benchmark(op, x, y, warmup=args.warmup, reps=args.steps)
+    ms = total_ms / args.steps
+    B = algorithmic_bytes(N, E, L)
+    achieved = B / (ms * 1e-3) / 1e9
+    result = {
+        "metric": METRIC,
+        "value": N * args.steps / (total_ms * 1e-3),
+        "unit": "DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "sshex8 Laplacian apply, level 8 on a 16^3 macro hex8 unit cube "
+                               f"({N} DOF, {E} macro elements), FP64 vectors, float32 FFF, int32 ids",
+                   "dofs": N, "macro_elements": E, "level": L},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_apply": B,
+                     "note": "working set 46 MB fits the 256 MB MALL: back-to-back applies are cache-hot; "
+                             "see roofline_cold for caches overwritten before each apply"},
+    }
+    if not args.no_flush:
+        reps = 20
+        cold = S.benchmark(op, x, y, warmup=1, reps=reps, flush_bytes=1 << 30) / reps
+        ach = B / (cold * 1e-3) / 1e9
+        result["roofline_cold"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": ach / HBM_PEAK_GBS, "ms_per_apply": cold,
+                                   "note": "1 GiB written between applies (outside the timed events)"}
+    if not args.no_solve:
+        result["solve"] = solve_times(S, mesh, fff, pts)
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(el, fff, N, L)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -216,9 +216,11 @@ int sfem_amd_inverse_diagonal_create(int real_type, ptrdiff_t n, const void *dia
 int sfem_amd_mg_create(int real_type, int cycle_type, int max_it, double atol, sfem_amd_op **out);
 int sfem_amd_mg_add_level(sfem_amd_op *mg, sfem_amd_op *op, sfem_amd_op *smoother,
                           sfem_amd_op *prolongation, sfem_amd_op *restriction);
-/* An FP64 operator that runs an FP32 operator: casts x, zeroes and applies, casts back
- * (chosen form 1: the FP32 preconditioner inside FP64 flexible CG). */
-int sfem_amd_precision_bridge_create(sfem_amd_op *op32, sfem_amd_op **out);
+/* A view of an operator in the other precision: casts x, applies into a zeroed temporary of
+ * the operator's precision, and accumulates the cast result into y.  FP32 op -> FP64 view (the
+ * FP32 preconditioner inside FP64 flexible CG, chosen form 1); FP64 op -> FP32 view (the FP64
+ * coarse solve inside an FP32 multigrid). */
+int sfem_amd_precision_bridge_create(sfem_amd_op *op, sfem_amd_op **out);
 /* Iterations of the last solve of a solver op; -1 for non-solvers. */
 int sfem_amd_op_iterations(const sfem_amd_op *op);
 /* Residual history of the last CG solve (relative norms), up to `cap` entries. */

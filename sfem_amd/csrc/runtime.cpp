@@ -214,10 +214,12 @@ struct ConstrainedOp final : OpBase {
             return dofs_copy<double>((ptrdiff_t)dofs.n, dofs.p, (const double *)x, (double *)y, s);
         return dofs_copy<float>((ptrdiff_t)dofs.n, dofs.p, (const float *)x, (float *)y, s);
     }
-    int set_value(double v, void *x, hipStream_t s) const {
-        if (inner->real_type() == SFEM_AMD_FLOAT64) return dofs_set_value<double>((ptrdiff_t)dofs.n, dofs.p, v, (double *)x, s);
+    // x[c] = v for a vector of real type rt (which may differ from the operator's)
+    int set_value_rt(int rt, double v, void *x, hipStream_t s) const {
+        if (rt == SFEM_AMD_FLOAT64) return dofs_set_value<double>((ptrdiff_t)dofs.n, dofs.p, v, (double *)x, s);
         return dofs_set_value<float>((ptrdiff_t)dofs.n, dofs.p, (float)v, (float *)x, s);
     }
+    int set_value(double v, void *x, hipStream_t s) const { return set_value_rt(inner->real_type(), v, x, s); }
     ptrdiff_t rows() const override { return inner->rows(); }
     int real_type() const override { return inner->real_type(); }
     int diag(void *d, hipStream_t s) override {
@@ -249,7 +251,7 @@ struct TransferOp final : OpBase {
             err = sshex8_restrict<T>(fine->E, fine->level, Lc, fine->idx.p, fine->count.p, vs, (const T *)x, (T *)y, s);
         }
         if (err) return err;
-        if (constrained) return static_cast<ConstrainedOp *>(constrained.get())->set_value(0.0, y, s);
+        if (constrained) return static_cast<ConstrainedOp *>(constrained.get())->set_value_rt(real_type(), 0.0, y, s);
         return SFEM_SUCCESS;
     }
 };
@@ -302,6 +304,26 @@ struct PrecisionBridge final : OpBase {
         if (op32->apply(xf.p, yf.p, s)) return SFEM_FAILURE;
         return cast_accumulate(n, yf.p, (double *)y, s);  // y += (double) yf
     }
+};
+
+// FP32 -> FP64 operator -> FP32: y += (float) op64((double) x).  Used for the FP64
+// coarse solve inside an FP32 multigrid (x: coarse rhs, y: zeroed coarse solution).
+struct PrecisionBridgeDown final : OpBase {
+    std::shared_ptr<OpBase> op64;
+    DevArray<double> xd, yd;
+    int real_type() const override { return SFEM_AMD_FLOAT32; }
+    ptrdiff_t rows() const override { return op64->rows(); }
+    ptrdiff_t cols() const override { return op64->cols(); }
+    int apply(const void *x, void *y, hipStream_t s) override {
+        const ptrdiff_t n = rows(), m = cols();
+        if ((ptrdiff_t)xd.n != m && xd.alloc(m)) return SFEM_FAILURE;
+        if ((ptrdiff_t)yd.n != n && yd.alloc(n)) return SFEM_FAILURE;
+        if (blas_cast<float, double>(m, (const float *)x, xd.p, s)) return SFEM_FAILURE;
+        SFEM_AMD_HIP_CHECK(hipMemsetAsync(yd.p, 0, sizeof(double) * n, s));
+        if (op64->apply(xd.p, yd.p, s)) return SFEM_FAILURE;
+        return cast_accumulate_f32(n, yd.p, (float *)y, s);
+    }
+    int iterations() const override { return op64->iterations(); }
 };
 
 }  // namespace sfem_amd
@@ -909,11 +931,16 @@ int sfem_amd_inverse_diagonal_create(int rt, ptrdiff_t n, const void *diag, doub
     return make(std::make_shared<InverseDiagonal<float>>(), (const float *)diag);
 }
 
-int sfem_amd_precision_bridge_create(sfem_amd_op *op32, sfem_amd_op **out) {
-    CHECK_HANDLE(op32);
-    SFEM_AMD_REQUIRE(op32->impl->real_type() == SFEM_AMD_FLOAT32, "precision bridge: FP32 operator required");
-    auto b = std::make_shared<PrecisionBridge>();
-    b->op32 = op32->impl;
+int sfem_amd_precision_bridge_create(sfem_amd_op *op, sfem_amd_op **out) {
+    CHECK_HANDLE(op);
+    if (op->impl->real_type() == SFEM_AMD_FLOAT32) {
+        auto b = std::make_shared<PrecisionBridge>();
+        b->op32 = op->impl;
+        *out = wrap(b);
+        return SFEM_SUCCESS;
+    }
+    auto b = std::make_shared<PrecisionBridgeDown>();
+    b->op64 = op->impl;
     *out = wrap(b);
     return SFEM_SUCCESS;
 }

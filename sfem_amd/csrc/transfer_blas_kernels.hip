@@ -252,6 +252,10 @@ __global__ void cast_accumulate_kernel(const ptrdiff_t n, const float *__restric
     for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
         y[i] += (double)x[i];
 }
+__global__ void cast_accumulate_f32_kernel(const ptrdiff_t n, const double *__restrict__ x, float *__restrict__ y) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        y[i] += (float)x[i];
+}
 // Chebyshev3 iteration_update (algebra/sfem_Chebyshev3.hpp:37-52): p = beta p - rhs + Ax; x -= alpha p
 template <typename T>
 __global__ void cheb3_update_kernel(const ptrdiff_t n, const T alpha, const T beta, const T *__restrict__ rhs,
@@ -440,6 +444,13 @@ int dofs_copy(ptrdiff_t n, const int32_t *dofs, const T *src, T *dst, hipStream_
 }
 
 size_t reduction_workspace_doubles() { return kRedBlocks + 2; }
+
+int cast_accumulate_f32(ptrdiff_t n, const double *x, float *y, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    cast_accumulate_f32_kernel<<<grid_for(n), kBlock, 0, s>>>(n, x, y);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
 
 int cast_accumulate(ptrdiff_t n, const float *x, double *y, hipStream_t s) {
     if (n <= 0) return SFEM_SUCCESS;

@@ -36,5 +36,7 @@ template <typename T> int dofs_copy(ptrdiff_t n, const int32_t *dofs, const T *s
 size_t reduction_workspace_doubles();
 // y += (double) x
 int cast_accumulate(ptrdiff_t n, const float *x, double *y, hipStream_t s);
+// y += (float) x
+int cast_accumulate_f32(ptrdiff_t n, const double *x, float *y, hipStream_t s);
 
 }  // namespace sfem_amd

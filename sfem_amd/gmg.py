@@ -17,7 +17,7 @@ import numpy as np
 from . import FLOAT32, FLOAT64, HEX8, SSHEX8
 
 from . import (DeviceArray, Mesh, Multigrid, cg, cheb3, constrained, inverse_diagonal, jacobi, laplacian,
-               linear_elasticity, prolongation, restriction, set_constrained_value)
+               linear_elasticity, precision_bridge, prolongation, restriction, set_constrained_value)
 
 
 def derefinement_levels(L):
@@ -90,7 +90,12 @@ class SSGMG:
             co.P = prolongation(co.mesh, f.mesh, bs, rt, constraints=f.A)
         self.mg = Multigrid(real_type, 1, max_it, atol)
         for i, lv in enumerate(self.levels):
-            self.mg.add_level(lv.A, lv.smoother, getattr(lv, "P", None), getattr(lv, "R", None))
+            A, sm = lv.A, lv.smoother
+            if real_type == FLOAT32 and i == len(self.levels) - 1:
+                # the coarse solve stays FP64 inside the FP32 cycle
+                A, sm = precision_bridge(A), precision_bridge(sm)
+                lv.bridges = (A, sm)
+            self.mg.add_level(A, sm, getattr(lv, "P", None), getattr(lv, "R", None))
 
     @property
     def fine(self):
