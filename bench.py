@@ -58,7 +58,7 @@ def cpu_baseline(el, fff, N, L, budget_s=12.0, max_reps=20):
                       f"OpenMP restatement of affine_sshex8_laplacian_stencil_apply_fff"}
 
 
-def solve_times(S, mesh, fff, pts, rtol=1e-10):
+def solve_times(S, mesh, fff, pts, n_macro, rtol=1e-10):
     from sfem_amd import gmg
     N = pts.shape[1]
     bc = np.where((pts.min(0) < 1e-6) | (pts.max(0) > 1 - 1e-6))[0].astype(np.int32)
@@ -73,15 +73,15 @@ def solve_times(S, mesh, fff, pts, rtol=1e-10):
         return float(np.linalg.norm(rhs - r.numpy()) / np.linalg.norm(rhs))
 
     t0 = time.perf_counter()
-    h64 = gmg.SSGMG(mesh, "laplacian", fff, bc)
+    h64 = gmg.SSGMG(mesh, "laplacian", fff, bc, macro_box=(n_macro, 1.0))
     setup64 = time.perf_counter() - t0
     for name in ("fp64", "mixed"):
         if name == "fp64":
-            solver = S.cg(h64.fine.A, h64.mg, rtol=rtol, max_it=500)
+            solver = S.cg(h64.fine.A, h64.mg, rtol=rtol, max_it=500, stop_on_residual=True)
             setup = setup64
         else:
             t0 = time.perf_counter()
-            h32 = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32)
+            h32 = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, macro_box=(n_macro, 1.0))
             P = S.precision_bridge(h32.mg)
             setup = time.perf_counter() - t0
             solver = S.cg(h64.fine.A, P, flexible=True, rtol=rtol, max_it=500)
@@ -96,9 +96,10 @@ def solve_times(S, mesh, fff, pts, rtol=1e-10):
         t = time.perf_counter() - t0
         out[name] = {"seconds": t, "iterations": solver.iterations, "status": solver.last_status,
                      "true_rel_residual": true_res(h64.fine.A, x), "setup_seconds": setup,
-                     "preconditioner": ("V-cycle 8-4-2-1, Chebyshev3 FP64, FP64 coarse PCG" if name == "fp64" else
-                                        "V-cycle 8-4-2-1 in FP32 with BF16 geometry, FP64 coarse PCG; outer FP64 "
-                                        "flexible CG")}
+                     "preconditioner": ("V-cycle 8-4-2-1 | macro 16^3 -> 8-4-2-1 super levels, Chebyshev3, FP64; "
+                                        "FP64 coarse PCG" if name == "fp64" else
+                                        "same V-cycle in FP32 with BF16 geometry (hex8 levels bridged to FP64), "
+                                        "FP64 coarse PCG; outer FP64 flexible CG")}
     out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
     return out
 
@@ -111,6 +112,8 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flush", action="store_true")
+    ap.add_argument("--mode", choices=["apply", "copy"], default="apply",
+                    help="copy: stream-copy 256 MiB of FP64 per step (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (every box here has one GPU; "
@@ -121,6 +124,19 @@ def main():
     if S.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.mode == "copy":
+        n = (256 << 20) // 8
+        a = S.DeviceArray.from_numpy(np.ones(n))
+        b = S.DeviceArray(n)
+        import time as _t
+        S.synchronize()
+        t0 = _t.perf_counter()
+        for _ in range(args.steps):
+            S.lib.sfem_amd_blas_copy(S.FLOAT64, n, S._dptr(a), S._dptr(b), None)
+        S.synchronize()
+        dt = (_t.perf_counter() - t0) / args.steps
+        print(json.dumps({"mode": "copy", "bytes_per_step": 16 * n, "GB/s": 16 * n / dt / 1e9}))
+        return
     L, n_macro = 8, 16
     mel, mpts, el, pts, fff = build_problem(n_macro, L)
     N, E = pts.shape[1], el.shape[1]
@@ -163,7 +179,7 @@ def main():
                                    "frac": ach / HBM_PEAK_GBS, "ms_per_apply": cold,
                                    "note": "1 GiB written between applies (outside the timed events)"}
     if not args.no_solve:
-        result["solve"] = solve_times(S, mesh, fff, pts)
+        result["solve"] = solve_times(S, mesh, fff, pts, n_macro)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(el, fff, N, L)
     print(json.dumps(result))

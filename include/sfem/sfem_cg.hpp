@@ -33,6 +33,9 @@ public:
     bool verbose{false};
     bool apply_overwrites_output{false};
     bool flexible{false};
+    // fixed-preconditioner PCG that stops on ||r||/||b|| (recursive residual) instead of the
+    // reference's sqrt(r.z)/sqrt(r0.z0)
+    bool stop_on_residual_norm{false};
     int residual_replacement{50};
     int last_status{SFEM_FAILURE};
     std::vector<double> history;  // relative residual per iteration
@@ -154,6 +157,7 @@ private:
             const T ptAp = blas->dot(n, p, Ap);
             update_x_r(n, rtz / ptAp, p, Ap, x, r, false);
         }
+        const T bnorm = stop_on_residual_norm ? blas->norm2(n, b) : T(1);
         for (iterations_ = 0; iterations_ < max_it; iterations_++) {
             zero_for_apply(n, z);
             preconditioner_op->apply(r, z);
@@ -165,7 +169,13 @@ private:
             apply_op->apply(p, Ap);
             const T ptAp = blas->dot(n, p, Ap);
             const T alpha = rtz / ptAp;
-            update_x_r(n, alpha, p, Ap, x, r, false);
+            const T rtr = update_x_r(n, alpha, p, Ap, x, r, stop_on_residual_norm);
+            if (stop_on_residual_norm) {
+                const T rel = std::sqrt(rtr) / (bnorm > 0 ? bnorm : T(1));
+                history.push_back((double)rel);
+                if (rel < rtol || std::sqrt(rtr) < atol) return SFEM_SUCCESS;
+                continue;
+            }
             const T anorm = std::sqrt(std::abs(rtz));
             const T rnorm = anorm / std::sqrt(rtz0);
             history.push_back((double)rnorm);

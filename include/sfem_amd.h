@@ -159,6 +159,15 @@ int sfem_amd_restriction_create(const sfem_amd_mesh *fine, const sfem_amd_mesh *
                                 int block_size, int real_type, const sfem_amd_op *constrained,
                                 sfem_amd_op **out);
 
+/* y[perm[i]*bs + c] += x[i*bs + c] (a permutation between two numberings of the same nodes,
+ * e.g. a box mesh and the hierarchical numbering of a coarser mesh refined onto it); with
+ * transpose != 0 the inverse map y[i*bs+c] += x[perm[i]*bs+c].  perm: host int32 [n].
+ * constrained (optional): zero those dofs of the output afterwards. */
+int sfem_amd_permutation_create(ptrdiff_t n, const int32_t *perm, int bs, int real_type, int transpose,
+                                const sfem_amd_op *constrained, sfem_amd_op **out);
+/* y += B (A x) through an internal temporary (both of one real type). */
+int sfem_amd_chain_create(sfem_amd_op *A, sfem_amd_op *B, sfem_amd_op **out);
+
 /* ------------------------------------------------------------------ BLAS-1 (device) */
 /* The reference's BLAS<T> (algebra/sfem_tpl_blas.hpp:11-35) on device vectors of real type
  * `rt` (SFEM_AMD_FLOAT64 / SFEM_AMD_FLOAT32); scalars are passed as double.  Reductions are
@@ -186,12 +195,13 @@ int sfem_amd_cheb3_update(int rt, ptrdiff_t n, double alpha, double beta, const 
                           void *x, void *stream);
 
 /* ------------------------------------------------------------------ solvers as operators */
-/* sfem::ConjugateGradient<real_t> (algebra/sfem_cg.hpp:80-446): plain CG, or left-PCG with
- * a fixed preconditioner when `precond` != NULL.  flexible != 0 selects flexible PCG
- * (beta from r_new.(z_new - z_old)) with the true residual ||b - A x|| / ||b|| as stopping
- * test (the reference has no flexible CG).  Applying the returned op solves A x = b
+/* sfem::ConjugateGradient<real_t> (algebra/sfem_cg.hpp:80-446): plain CG, or PCG when
+ * `precond` != NULL.  mode 0: the reference's PCG with a fixed preconditioner (stops on
+ * sqrt(r.z)/sqrt(r0.z0), :430-437); mode 1: flexible PCG (beta from r_new.(z_new - z_old); the
+ * reference has none) stopping on the true residual ||b - A x||/||b||; mode 2: fixed-
+ * preconditioner PCG stopping on ||r||/||b||.  Applying the returned op solves A x = b
  * (x: initial guess and output).  The op keeps shared references to A and precond. */
-int sfem_amd_cg_create(sfem_amd_op *A, sfem_amd_op *precond, int flexible, double rtol,
+int sfem_amd_cg_create(sfem_amd_op *A, sfem_amd_op *precond, int mode, double rtol,
                        double atol, int max_it, sfem_amd_op **out);
 /* sfem::Chebyshev3<real_t> (algebra/sfem_Chebyshev3.hpp:57-303) of degree `max_it`; eig_max
  * from PowerMethod (algebra/sfem_PowerMethod.hpp:18-55) with eigen_solver_max_it iterations
@@ -221,6 +231,10 @@ int sfem_amd_mg_add_level(sfem_amd_op *mg, sfem_amd_op *op, sfem_amd_op *smoothe
  * FP32 preconditioner inside FP64 flexible CG, chosen form 1); FP64 op -> FP32 view (the FP64
  * coarse solve inside an FP32 multigrid). */
 int sfem_amd_precision_bridge_create(sfem_amd_op *op, sfem_amd_op **out);
+/* Solver view in the other precision with in-out x (x is the initial guess and the result):
+ * casts b and x, runs the solver/smoother, casts x back.  For an FP64 smoother inside an FP32
+ * multigrid. */
+int sfem_amd_solver_bridge_create(sfem_amd_op *solver, sfem_amd_op **out);
 /* Iterations of the last solve of a solver op; -1 for non-solvers. */
 int sfem_amd_op_iterations(const sfem_amd_op *op);
 /* Residual history of the last CG solve (relative norms), up to `cap` entries. */

@@ -74,6 +74,9 @@ SIGNATURES = {
     "sfem_amd_cheb3_eig_max": (_d, [_vp]),
     "sfem_amd_jacobi_create": (_i, [_vp, _vp, _d, _i, C.POINTER(_vp)]),
     "sfem_amd_inverse_diagonal_create": (_i, [_i, _pd, _vp, _d, C.POINTER(_vp)]),
+    "sfem_amd_permutation_create": (_i, [_pd, _vp, _i, _i, _i, _vp, C.POINTER(_vp)]),
+    "sfem_amd_chain_create": (_i, [_vp, _vp, C.POINTER(_vp)]),
+    "sfem_amd_solver_bridge_create": (_i, [_vp, C.POINTER(_vp)]),
     "sfem_amd_mg_create": (_i, [_i, _i, _i, _d, C.POINTER(_vp)]),
     "sfem_amd_mg_add_level": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "sfem_amd_precision_bridge_create": (_i, [_vp, C.POINTER(_vp)]),
@@ -386,8 +389,11 @@ def restriction(fine, coarse, block_size=1, real_type=FLOAT64, constraints=None)
                    constraints.h if constraints else None, keep=[coarse, fine, constraints])
 
 
-def cg(A, precond=None, flexible=False, rtol=1e-10, atol=1e-16, max_it=10000):
-    return _new_op("sfem_amd_cg_create", A.h, precond.h if precond else None, int(flexible), rtol, atol, max_it,
+def cg(A, precond=None, flexible=False, rtol=1e-10, atol=1e-16, max_it=10000, stop_on_residual=False):
+    """mode: flexible -> 1 (flexible PCG, true-residual stop), stop_on_residual -> 2 (fixed-preconditioner
+    PCG stopping on ||r||/||b||), else 0 (the reference's PCG)."""
+    mode = 1 if flexible else (2 if stop_on_residual else 0)
+    return _new_op("sfem_amd_cg_create", A.h, precond.h if precond else None, mode, rtol, atol, max_it,
                    keep=[A, precond])
 
 
@@ -403,6 +409,17 @@ def cheb3_eig_max(op):
 
 def jacobi(A, diag, omega=1.0, steps=3):
     return _new_op("sfem_amd_jacobi_create", A.h, _dptr(diag), omega, steps, keep=[A])
+
+
+def permutation(perm, block_size=1, real_type=FLOAT64, transpose=False, constraints=None):
+    p = _host(perm, np.int32)
+    return _new_op("sfem_amd_permutation_create", len(p), _np_ptr(p), block_size, real_type, int(transpose),
+                   constraints.h if constraints else None, keep=[constraints])
+
+
+def chain(A, B):
+    """y += B (A x)"""
+    return _new_op("sfem_amd_chain_create", A.h, B.h, keep=[A, B])
 
 
 def inverse_diagonal(diag, omega=1.0):
@@ -424,8 +441,12 @@ class Multigrid(Operator):
         self._keep += [op, smoother, prolongation, restriction]
 
 
-def precision_bridge(op32):
-    return _new_op("sfem_amd_precision_bridge_create", op32.h, keep=[op32])
+def precision_bridge(op):
+    return _new_op("sfem_amd_precision_bridge_create", op.h, keep=[op])
+
+
+def solver_bridge(solver):
+    return _new_op("sfem_amd_solver_bridge_create", solver.h, keep=[solver])
 
 
 def benchmark(op, x, y, warmup=2, reps=10, flush_bytes=0, stream=None):

@@ -256,6 +256,37 @@ struct TransferOp final : OpBase {
     }
 };
 
+// ----------------------------------------------------------------- permutations and chains
+template <typename T>
+struct PermutationOp final : OpBase {
+    DevArray<int32_t> perm;
+    ptrdiff_t n{0};
+    int bs{1}, transpose{0};
+    std::shared_ptr<OpBase> constrained;
+    int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
+    ptrdiff_t rows() const override { return n * bs; }
+    int apply(const void *x, void *y, hipStream_t s) override {
+        if (permute_add<T>(n, bs, perm.p, transpose, (const T *)x, (T *)y, s)) return SFEM_FAILURE;
+        if (constrained) return static_cast<ConstrainedOp *>(constrained.get())->set_value_rt(real_type(), 0.0, y, s);
+        return SFEM_SUCCESS;
+    }
+};
+
+struct ChainOp final : OpBase {
+    std::shared_ptr<OpBase> A, B;
+    DevArray<char> tmp;
+    int real_type() const override { return A->real_type(); }
+    ptrdiff_t rows() const override { return B->rows(); }
+    ptrdiff_t cols() const override { return A->cols(); }
+    int apply(const void *x, void *y, hipStream_t s) override {
+        const size_t bytes = (size_t)A->rows() * (real_type() == SFEM_AMD_FLOAT64 ? 8 : 4);
+        if (tmp.n != bytes && tmp.alloc(bytes)) return SFEM_FAILURE;
+        SFEM_AMD_HIP_CHECK(hipMemsetAsync(tmp.p, 0, bytes, s));
+        if (A->apply(x, tmp.p, s)) return SFEM_FAILURE;
+        return B->apply(tmp.p, y, s);
+    }
+};
+
 // ----------------------------------------------------------------- sfem:: template bridges
 template <typename T>
 struct OpView final : sfem::Operator<T> {
@@ -324,6 +355,25 @@ struct PrecisionBridgeDown final : OpBase {
         return cast_accumulate_f32(n, yd.p, (float *)y, s);
     }
     int iterations() const override { return op64->iterations(); }
+};
+
+// Solver in the other precision with x as initial guess and output.
+struct SolverBridge final : OpBase {
+    std::shared_ptr<OpBase> inner;
+    DevArray<char> bd, xd;
+    int real_type() const override { return inner->real_type() == SFEM_AMD_FLOAT64 ? SFEM_AMD_FLOAT32 : SFEM_AMD_FLOAT64; }
+    ptrdiff_t rows() const override { return inner->rows(); }
+    int apply(const void *b, void *x, hipStream_t s) override {
+        const ptrdiff_t n = rows();
+        const int rin = inner->real_type(), rout = real_type();
+        const size_t bytes = (size_t)n * (rin == SFEM_AMD_FLOAT64 ? 8 : 4);
+        if (bd.n != bytes && (bd.alloc(bytes) || xd.alloc(bytes))) return SFEM_FAILURE;
+        if (sfem_amd_blas_cast(rout, n, b, rin, bd.p, s) || sfem_amd_blas_cast(rout, n, x, rin, xd.p, s))
+            return SFEM_FAILURE;
+        if (inner->apply(bd.p, xd.p, s)) return SFEM_FAILURE;
+        return sfem_amd_blas_cast(rin, n, xd.p, rout, x, s);
+    }
+    int iterations() const override { return inner->iterations(); }
 };
 
 }  // namespace sfem_amd
@@ -755,7 +805,8 @@ static int cg_create_t(sfem_amd_op *A, sfem_amd_op *P, int flexible, double rtol
     cg->default_init();
     cg->set_op(view<T>(A->impl));
     if (P) cg->set_preconditioner_op(view<T>(P->impl));
-    cg->flexible = flexible != 0;
+    cg->flexible = flexible == 1;
+    cg->stop_on_residual_norm = flexible == 2;
     cg->rtol = (T)rtol;
     cg->atol = (T)atol;
     cg->max_it = max_it;
@@ -915,6 +966,50 @@ struct InverseDiagonal final : OpBase {
     }
 };
 }  // extern "C++"
+
+int sfem_amd_solver_bridge_create(sfem_amd_op *solver, sfem_amd_op **out) {
+    CHECK_HANDLE(solver);
+    auto b = std::make_shared<SolverBridge>();
+    b->inner = solver->impl;
+    *out = wrap(b);
+    return SFEM_SUCCESS;
+}
+
+int sfem_amd_permutation_create(ptrdiff_t n, const int32_t *perm, int bs, int rt, int transpose,
+                                const sfem_amd_op *constrained, sfem_amd_op **out) {
+    SFEM_AMD_REQUIRE(perm && n >= 0 && bs >= 1, "permutation: arguments");
+    std::vector<char> seen((size_t)n, 0);
+    for (ptrdiff_t i = 0; i < n; i++) {
+        SFEM_AMD_REQUIRE(perm[i] >= 0 && perm[i] < n && !seen[perm[i]], "permutation: not a permutation of [0, n)");
+        seen[perm[i]] = 1;
+    }
+    SFEM_AMD_REQUIRE(!constrained || dynamic_cast<ConstrainedOp *>(constrained->impl.get()),
+                     "permutation: constraints must be a constrained operator");
+    auto make = [&](auto op) -> int {
+        op->n = n;
+        op->bs = bs;
+        op->transpose = transpose != 0;
+        if (constrained) op->constrained = constrained->impl;
+        if (op->perm.upload(perm, (size_t)n)) return SFEM_FAILURE;
+        *out = wrap(op);
+        return SFEM_SUCCESS;
+    };
+    if (rt == SFEM_AMD_FLOAT64) return make(std::make_shared<PermutationOp<double>>());
+    SFEM_AMD_REQUIRE(rt == SFEM_AMD_FLOAT32, "permutation: real type");
+    return make(std::make_shared<PermutationOp<float>>());
+}
+
+int sfem_amd_chain_create(sfem_amd_op *A, sfem_amd_op *B, sfem_amd_op **out) {
+    CHECK_HANDLE(A);
+    CHECK_HANDLE(B);
+    SFEM_AMD_REQUIRE(A->impl->real_type() == B->impl->real_type(), "chain: real types differ");
+    SFEM_AMD_REQUIRE(A->impl->rows() == B->impl->cols(), "chain: sizes do not match");
+    auto c = std::make_shared<ChainOp>();
+    c->A = A->impl;
+    c->B = B->impl;
+    *out = wrap(c);
+    return SFEM_SUCCESS;
+}
 
 int sfem_amd_inverse_diagonal_create(int rt, ptrdiff_t n, const void *diag, double omega, sfem_amd_op **out) {
     SFEM_AMD_REQUIRE(diag && n >= 0, "inverse_diagonal: diag required");

@@ -443,6 +443,30 @@ int dofs_copy(ptrdiff_t n, const int32_t *dofs, const T *src, T *dst, hipStream_
     return SFEM_SUCCESS;
 }
 
+template <typename T>
+__global__ void permute_add_kernel(const ptrdiff_t n, const int bs, const int32_t *__restrict__ perm, const int transpose,
+                                   const T *__restrict__ x, T *__restrict__ y) {
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n * bs; t += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const ptrdiff_t i = t / bs;
+        const int c = (int)(t - i * bs);
+        const ptrdiff_t j = perm[i];
+        if (transpose)
+            y[i * bs + c] += x[j * bs + c];
+        else
+            y[j * bs + c] += x[i * bs + c];
+    }
+}
+
+template <typename T>
+int permute_add(ptrdiff_t n, int bs, const int32_t *perm, int transpose, const T *x, T *y, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    permute_add_kernel<T><<<grid_for(n * bs), kBlock, 0, s>>>(n, bs, perm, transpose, x, y);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+template int permute_add<double>(ptrdiff_t, int, const int32_t *, int, const double *, double *, hipStream_t);
+template int permute_add<float>(ptrdiff_t, int, const int32_t *, int, const float *, float *, hipStream_t);
+
 size_t reduction_workspace_doubles() { return kRedBlocks + 2; }
 
 int cast_accumulate_f32(ptrdiff_t n, const double *x, float *y, hipStream_t s) {

@@ -34,6 +34,9 @@ int cg_update(ptrdiff_t n, T a, const T *p, const T *Ap, T *x, T *r, double *ws,
 template <typename T> int dofs_set_value(ptrdiff_t n, const int32_t *dofs, T v, T *x, hipStream_t s);
 template <typename T> int dofs_copy(ptrdiff_t n, const int32_t *dofs, const T *src, T *dst, hipStream_t s);
 size_t reduction_workspace_doubles();
+// y[perm[i]*bs+c] += x[i*bs+c] (transpose: y[i*bs+c] += x[perm[i]*bs+c])
+template <typename T>
+int permute_add(ptrdiff_t n, int bs, const int32_t *perm, int transpose, const T *x, T *y, hipStream_t s);
 // y += (double) x
 int cast_accumulate(ptrdiff_t n, const float *x, double *y, hipStream_t s);
 // y += (float) x

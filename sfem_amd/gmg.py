@@ -7,17 +7,24 @@ algebra/mg/sfem_GeometricMultigrid.cpp:12-168) with the smoothing of drivers/mgs
   * every level rediscretises the operator with the macro geometry (dirichlet_as_zero on
     coarse levels: the coarse constrained nodes are the fine ones that exist there);
   * prolongation and restriction zero the constrained dofs of their output level;
-  * fine levels: Chebyshev3 (degree 3, power method from ones with constrained dofs zeroed,
+  * smoothing: Chebyshev3 (degree 3, power method from ones with constrained dofs zeroed,
     scale_eig_max 1.02, tolerance 1e-5, drivers/mgsolve.cpp:68-71,176-185), or the
     reference's default damped Jacobi (create_gmg_default_smoothers_and_solver);
   * coarsest: FP64 CG with inverse-diagonal preconditioning (drivers/mgsolve.cpp:214-227).
+
+Extension (not in the reference): when the macro mesh is a structured box of n^3 hex8
+elements, the hierarchy continues below it.  The macro grid is re-read as the level-S
+sshex8 refinement of a coarser "super" box of (n/S)^3 elements (a permutation maps the box
+numbering onto the super mesh's hierarchical numbering) and the levels S/2, ..., 1 of the
+super mesh are added, with per-super-element coefficients averaged from the macro ones.  The
+FP64 coarse CG then runs on a few dozen dofs instead of the whole macro grid.
 """
 import numpy as np
 
 from . import FLOAT32, FLOAT64, HEX8, SSHEX8
-
-from . import (DeviceArray, Mesh, Multigrid, cg, cheb3, constrained, inverse_diagonal, jacobi, laplacian,
-               linear_elasticity, precision_bridge, prolongation, restriction, set_constrained_value)
+from . import (DeviceArray, Mesh, Multigrid, adjugate_det, box_mesh, cg, chain, cheb3, constrained, fff,
+               inverse_diagonal, jacobi, laplacian, linear_elasticity, permutation, precision_bridge, prolongation,
+               restriction, set_constrained_value, solver_bridge, sshex8_refine)
 
 
 def derefinement_levels(L):
@@ -30,70 +37,143 @@ def derefinement_levels(L):
     return out
 
 
+def super_level(n, max_level=16):
+    """Largest power of two S <= max_level dividing n (0 if none > 1)."""
+    S = 1
+    while S * 2 <= max_level and n % (S * 2) == 0:
+        S *= 2
+    return S if S > 1 else 0
+
+
 class Level:
     pass
 
 
 class SSGMG:
-    """Holds the hierarchy; `mg` is the Multigrid operator (one V-cycle per apply when used
-    as a preconditioner).  kind: "laplacian" or "elasticity"."""
+    """Holds the hierarchy; `mg` is the Multigrid operator (one V-cycle per apply when used as a
+    preconditioner).  kind: "laplacian" (geometry = FFF [E][6]) or "elasticity" (geometry =
+    (adjugate [E][9], det [E])).  macro_box = (n, extent) enables the structured extension for a
+    macro mesh made by box_mesh(n+1, n+1, n+1, extent, extent, extent)."""
 
     def __init__(self, fine_mesh, kind, geometry, bc_nodes, coeff=None, real_type=FLOAT64, smoother="cheb3",
                  cheb_degree=3, eig_its=10, eig_tol=1e-5, scale_eig_max=1.02, coarse_rtol=1e-8,
-                 coarse_max_it=10000, mu=None, lam=None, bc_components=None, max_it=1, atol=0.0):
+                 coarse_max_it=10000, mu=None, lam=None, bc_components=None, max_it=1, atol=0.0, macro_box=None):
         self.kind = kind
-        self.block_size = 3 if kind == "elasticity" else 1
+        bs = self.block_size = 3 if kind == "elasticity" else 1
         self.real_type = real_type
-        dtype = np.float64 if real_type == FLOAT64 else np.float32
-        bs = self.block_size
+        self._smoother_cfg = dict(kind=smoother, degree=cheb_degree, eig_its=eig_its, eig_tol=eig_tol,
+                                  smax=scale_eig_max)
+        self.mu, self.lam, self.bc_components = mu, lam, bc_components
         bc_nodes = np.asarray(bc_nodes, np.int32)
         self.levels = []
         meshes = [fine_mesh]
         for l in derefinement_levels(fine_mesh.level)[1:]:
             meshes.append(meshes[-1].derefine(l))
-        for i, m in enumerate(meshes):
-            lv = Level()
-            lv.mesh = m
-            if kind == "laplacian":
-                lv.op = laplacian(m, geometry, coeff, real_type=real_type if m.element_type == SSHEX8 else FLOAT64)
+        for m in meshes:
+            self.levels.append(self._make_level(m, geometry, coeff, bc_nodes[bc_nodes < m.nnodes]))
+        # nested transfers inside the macro hierarchy
+        for i in range(len(self.levels) - 1):
+            f, co = self.levels[i], self.levels[i + 1]
+            rt = self.real_type  # all cycle vectors share the hierarchy's precision
+            f.R = restriction(f.mesh, co.mesh, bs, rt, constraints=co.A)
+            co.P = prolongation(co.mesh, f.mesh, bs, rt, constraints=f.A)
+        if macro_box is not None:
+            self._extend_below_macro(macro_box, geometry, coeff, bc_nodes)
+        self._finish(coarse_rtol, coarse_max_it, max_it, atol)
+
+    # -- helpers
+    def _make_level(self, m, geometry, coeff, bc):
+        lv = Level()
+        lv.mesh = m
+        rt = self.real_type if m.element_type == SSHEX8 else FLOAT64
+        if self.kind == "laplacian":
+            lv.op = laplacian(m, geometry, coeff, real_type=rt)
+        else:
+            adj, det = geometry
+            lv.op = linear_elasticity(m, adj, det, self.mu, self.lam, coeff, real_type=rt)
+        lv.bc = np.asarray(bc, np.int32)
+        lv.A = constrained(lv.op, lv.bc, self.block_size, self.bc_components)
+        lv.n = m.nnodes * self.block_size
+        return lv
+
+    def _make_smoother(self, lv):
+        cfg = self._smoother_cfg
+        dtype = np.float64 if lv.op.real_type == FLOAT64 else np.float32
+        if cfg["kind"] == "cheb3":
+            init = DeviceArray.from_numpy(np.ones(lv.n, dtype))
+            set_constrained_value(lv.A, 0.0, init)
+            return cheb3(lv.A, max_it=cfg["degree"], scale_eig_max=cfg["smax"], eigen_solver_max_it=cfg["eig_its"],
+                         eigen_solver_tol=cfg["eig_tol"], init_vector=init)
+        d = DeviceArray(lv.n, dtype)
+        lv.A.diag(d)
+        return jacobi(lv.A, d, omega=1.0 / self.block_size, steps=3)
+
+    def _extend_below_macro(self, macro_box, geometry, coeff, bc_nodes):
+        n, extent = macro_box
+        S = super_level(n)
+        if not S:
+            return
+        ns = n // S
+        sel, spts = box_mesh(ns + 1, ns + 1, ns + 1, extent, extent, extent, HEX8)
+        el, pts = sshex8_refine(S, sel, spts)
+        # permutation: macro box node (xi,yi,zi) -> super sshex8 node at the same grid point
+        g = np.rint(pts.astype(np.float64) * (n / extent)).astype(np.int64)
+        n1 = n + 1
+        box_id = g[1] * n1 * n1 + g[0] * n1 + g[2]
+        perm = np.empty(n1 ** 3, np.int32)
+        perm[box_id] = np.arange(n1 ** 3, dtype=np.int32)
+        top = Mesh(SSHEX8, S, el, pts.shape[1])
+        # coefficient of a super element: mean of its S^3 macro elements (box element order)
+        scoeff = None
+        if coeff is not None:
+            c = np.asarray(coeff, np.float64).reshape(n, n, n)  # [zi][yi][xi] element order
+            scoeff = c.reshape(ns, S, ns, S, ns, S).mean(axis=(1, 3, 5)).astype(np.float32).ravel()
+        if self.kind == "laplacian":
+            sgeom = fff(SSHEX8, S, el, pts)
+        else:
+            sgeom = adjugate_det(SSHEX8, S, el, pts)
+        # constrained super nodes: those at constrained macro grid points
+        mask = np.zeros(n1 ** 3, bool)
+        mask[bc_nodes[bc_nodes < n1 ** 3]] = True
+        sbc_top = np.where(mask[np.argsort(perm)])[0].astype(np.int32)
+        self._super = (top, el, pts, perm)
+        prev = None
+        link_fine = self.levels[-1]
+        for l in derefinement_levels(S)[1:]:
+            m = top.derefine(l)
+            lv = self._make_level(m, sgeom, scoeff, sbc_top[sbc_top < m.nnodes])
+            rt = self.real_type
+            bs = self.block_size
+            if prev is None:
+                # link: macro level (box numbering) <-> super level l via the top super level S
+                to_super = permutation(perm, bs, rt, transpose=False)
+                R_top = restriction(top, m, bs, rt, constraints=lv.A)
+                link_fine.R = chain(to_super, R_top)
+                P_top = prolongation(m, top, bs, rt)
+                from_super = permutation(perm, bs, rt, transpose=True, constraints=link_fine.A)
+                lv.P = chain(P_top, from_super)
             else:
-                adj, det = geometry
-                lv.op = linear_elasticity(m, adj, det, mu, lam, coeff,
-                                          real_type=real_type if m.element_type == SSHEX8 else FLOAT64)
-            lv.bc = bc_nodes[bc_nodes < m.nnodes]
-            lv.A = constrained(lv.op, lv.bc, bs, bc_components)
-            lv.n = m.nnodes * bs
+                prev.R = restriction(prev.mesh, m, bs, rt, constraints=lv.A)
+                lv.P = prolongation(m, prev.mesh, bs, rt, constraints=prev.A)
             self.levels.append(lv)
-        self.coarse_real_type = FLOAT64
-        # fine-level smoothers
-        for i, lv in enumerate(self.levels[:-1]):
-            if smoother == "cheb3":
-                init = DeviceArray.from_numpy(np.ones(lv.n, dtype))
-                set_constrained_value(lv.A, 0.0, init)
-                lv.smoother = cheb3(lv.A, max_it=cheb_degree, scale_eig_max=scale_eig_max, eigen_solver_max_it=eig_its,
-                                    eigen_solver_tol=eig_tol, init_vector=init)
-            else:  # the reference default: 3 sweeps of damped Jacobi
-                d = DeviceArray(lv.n, dtype)
-                lv.A.diag(d)
-                lv.smoother = jacobi(lv.A, d, omega=1.0 / bs, steps=3)
-        # coarsest level: FP64 PCG
+            prev = lv
+
+    def _finish(self, coarse_rtol, coarse_max_it, max_it, atol):
+        for lv in self.levels[:-1]:
+            lv.smoother = self._make_smoother(lv)
         c = self.levels[-1]
         d = DeviceArray(c.n, np.float64)
         c.A.diag(d)
         c.precond = inverse_diagonal(d, 1.0)
         c.smoother = cg(c.A, c.precond, rtol=coarse_rtol, atol=1e-30, max_it=coarse_max_it)
-        # transfers (zero constraints on the output level)
-        for i in range(len(self.levels) - 1):
-            f, co = self.levels[i], self.levels[i + 1]
-            rt = real_type
-            f.R = restriction(f.mesh, co.mesh, bs, rt, constraints=co.A)
-            co.P = prolongation(co.mesh, f.mesh, bs, rt, constraints=f.A)
-        self.mg = Multigrid(real_type, 1, max_it, atol)
+        self.mg = Multigrid(self.real_type, 1, max_it, atol)
+        last = len(self.levels) - 1
         for i, lv in enumerate(self.levels):
             A, sm = lv.A, lv.smoother
-            if real_type == FLOAT32 and i == len(self.levels) - 1:
-                # the coarse solve stays FP64 inside the FP32 cycle
-                A, sm = precision_bridge(A), precision_bridge(sm)
+            if self.real_type == FLOAT32 and lv.op.real_type == FLOAT64:
+                # FP64-only levels (HEX8) inside an FP32 cycle: operator and smoother views
+                A = precision_bridge(A)
+                sm = precision_bridge(sm) if i == last else solver_bridge(sm)
                 lv.bridges = (A, sm)
             self.mg.add_level(A, sm, getattr(lv, "P", None), getattr(lv, "R", None))
 

@@ -231,3 +231,49 @@ def test_elasticity_mg_pcg_converges():
     res = true_residual(h.fine.A, dev(b.ravel()), x)
     print(f"elasticity MG-PCG: {solver.iterations} its, residual {res:.2e}")
     assert solver.last_status == 0 and res < 1e-8
+
+
+@pytest.mark.parametrize("real_type", [S.FLOAT64, S.FLOAT32])
+def test_structured_extension_below_macro_mesh(real_type):
+    """Hierarchy continued below the macro box (super-mesh levels): still converges, and the
+    coarsest FP64 problem shrinks to the super mesh's corners."""
+    L, n = 4, 8
+    mel, mpts, el, pts, fff, N, nodes, coeff = _ss_poisson(L, n, coeff_seed=11)
+    mesh = S.Mesh(S.SSHEX8, L, el, N)
+    h64 = gmg.SSGMG(mesh, "laplacian", fff, nodes, coeff=coeff, macro_box=(n, 1.0))
+    assert [lv.mesh.level for lv in h64.levels] == [4, 2, 1, 4, 2, 1]
+    assert h64.levels[-1].mesh.nnodes == 8
+    b = np.random.default_rng(0).uniform(0, 1, N)
+    b[nodes] = 0
+    if real_type == S.FLOAT64:
+        solver = S.cg(h64.fine.A, h64.mg, rtol=1e-10, max_it=300)
+    else:
+        h32 = gmg.SSGMG(mesh, "laplacian", fff, nodes, coeff=coeff, real_type=S.FLOAT32, macro_box=(n, 1.0))
+        solver = S.cg(h64.fine.A, S.precision_bridge(h32.mg), flexible=True, rtol=1e-10, max_it=300)
+    x = zeros(N)
+    solver.apply(dev(b), x)
+    res = true_residual(h64.fine.A, dev(b), x)
+    print(f"extended hierarchy ({'FP64' if real_type == S.FLOAT64 else 'FP32'}): {solver.iterations} its, "
+          f"residual {res:.2e}")
+    assert solver.last_status == 0 and res < 1e-8
+
+
+def test_structured_extension_elasticity():
+    L, n = 4, 4
+    mel, mpts = O.box_mesh(n + 1, n + 1, n + 1)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    adj, det = O.hex8_adj_det(mel, mpts)
+    N = pts.shape[1]
+    nodes = np.where(pts[2] < 1e-6)[0].astype(np.int32)
+    mu, lam = O.lame(1.0, 0.3)
+    h = gmg.SSGMG(S.Mesh(S.SSHEX8, L, el, N), "elasticity", (adj, det), nodes, mu=mu, lam=lam,
+                  macro_box=(n, 1.0))
+    b = np.zeros((N, 3))
+    b[:, 2] = -1
+    b[nodes] = 0
+    solver = S.cg(h.fine.A, h.mg, rtol=1e-10, max_it=300)
+    x = zeros(3 * N)
+    solver.apply(dev(b.ravel()), x)
+    res = true_residual(h.fine.A, dev(b.ravel()), x)
+    print(f"extended elasticity hierarchy: {solver.iterations} its, residual {res:.2e}")
+    assert solver.last_status == 0 and res < 1e-8
