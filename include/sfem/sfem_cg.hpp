@@ -9,6 +9,7 @@
 #pragma once
 
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include "sfem_MatrixFreeLinearSolver.hpp"
@@ -39,6 +40,9 @@ public:
     int residual_replacement{50};
     int last_status{SFEM_FAILURE};
     std::vector<double> history;  // relative residual per iteration
+    // flexible PCG only: called after every iteration with (iteration, previous residual norm,
+    // new residual norm); used to switch a preconditioner component that stops converging
+    std::function<void(int, double, double)> on_iteration;
 
     ~ConjugateGradient() { release(); }
 
@@ -162,6 +166,8 @@ private:
             zero_for_apply(n, z);
             preconditioner_op->apply(r, z);
             const T rtz_new = blas->dot(n, r, z);
+            // exact convergence (e.g. a one-unknown coarse problem): the next step would be 0/0
+            if (rtz_new == 0) return SFEM_SUCCESS;
             const T beta = rtz_new / rtz;
             rtz = rtz_new;
             blas->axpby(n, T(1), z, beta, p);
@@ -210,9 +216,11 @@ private:
                 residual(n, b, x, r);
                 rtr = blas->dot(n, r, r);
             }
+            const T rprev = rnorm;
             rnorm = std::sqrt(rtr);
             history.push_back((double)(rnorm / ref));
             if (verbose) std::printf("%d: residual rel: %g\n", iterations_ + 1, (double)(rnorm / ref));
+            if (on_iteration) on_iteration(iterations_, (double)rprev, (double)rnorm);
             if (rnorm / ref < rtol || rnorm < atol) {
                 // confirm with the true residual (recomputed in the operator's precision)
                 residual(n, b, x, r);

@@ -242,6 +242,51 @@ int sfem_amd_cg_history(const sfem_amd_op *op, double *out, int cap);
 /* Status of the last solve (SFEM_SUCCESS if converged). */
 int sfem_amd_op_last_status(const sfem_amd_op *op);
 
+/* ------------------------------------------------------------------ learned smoother
+ * No counterpart in the reference (chosen form 4: a learned sshex8 smoother).  A CNN of `layers`
+ * 3x3x3 convolutions without biases (1 -> hidden -> ... -> hidden -> 1 channels, ReLU between)
+ * acts on each macro element's (L+1)^3 micro-grid with zero padding:
+ *     g = conv_n o relu o ... o relu o conv_1,   f(z) = g(z) - g(-z)
+ * Weights are float32 [layer][Cout][Cin][27], offset k = (dz+1)*9 + (dy+1)*3 + (dx+1).
+ * weights == NULL: He-normal initialisation from `seed`.  Micro-grid levels 1..8. */
+typedef struct sfem_amd_cnn sfem_amd_cnn;
+int sfem_amd_cnn_create(int hidden, int layers, const float *weights, unsigned seed, sfem_amd_cnn **out);
+int sfem_amd_cnn_num_params(const sfem_amd_cnn *cnn);
+int sfem_amd_cnn_get_weights(const sfem_amd_cnn *cnn, float *weights_host);
+int sfem_amd_cnn_set_weights(sfem_amd_cnn *cnn, const float *weights_host);
+void sfem_amd_cnn_destroy(sfem_amd_cnn *cnn);
+/* out[e][v] = f(z[e][.])[v] for E elements of level L (device, float32 [E][(L+1)^3]).
+ * mode 0: FP32 direct convolution; mode 1: fused BF16 MFMA network (hidden 16, 4 layers). */
+int sfem_amd_cnn_forward(sfem_amd_cnn *cnn, int level, ptrdiff_t nelements, const float *z, float *out,
+                         int mode, void *stream);
+/* loss = sum (f(z) - t)^2 / sum t^2 over the batch (device z, t: [E][(L+1)^3]) and its gradient
+ * w.r.t. the weights, kept for sfem_amd_cnn_adam_step and copied to grad_host when non-NULL.
+ * Synchronises. */
+int sfem_amd_cnn_loss_grad(sfem_amd_cnn *cnn, int level, ptrdiff_t nelements, const float *z,
+                           const float *target, double *loss, float *grad_host, void *stream);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradient. */
+int sfem_amd_cnn_adam_step(sfem_amd_cnn *cnn, double lr, void *stream);
+/* out[e][v] = vec[idx[e][v]*bs + comp] * (scale ? scale[idx*bs + comp] : 1) as float32, for an
+ * sshex8 mesh (device; vec/scale of real type rt). */
+int sfem_amd_gather_elements(const sfem_amd_mesh *mesh, int real_type, int bs, int comp, const void *vec,
+                             const void *scale, float *out, void *stream);
+/* Learned smoother on an sshex8 level: `steps` times
+ *     r = b - A x;  x += damping * avg_e f(D^-1 r |_e)
+ * (per component for bs > 1; shared nodes average their elements' corrections; constrained dofs
+ * untouched when A is a constrained operator).  diag: device diagonal of A (copied).  The op
+ * keeps a shared reference to A; mesh and cnn must outlive it.  mode as sfem_amd_cnn_forward. */
+int sfem_amd_learned_smoother_create(sfem_amd_op *A, const sfem_amd_mesh *mesh, sfem_amd_cnn *cnn,
+                                     const void *diag, int bs, int steps, double damping, int mode,
+                                     sfem_amd_op **out);
+/* An op that runs `primary` until switched to `fallback` (same size and real type). */
+int sfem_amd_switch_create(sfem_amd_op *primary, sfem_amd_op *fallback, sfem_amd_op **out);
+int sfem_amd_switch_select(sfem_amd_op *sw, int which);
+/* active: 0 primary, 1 fallback; events: switches made by CG fallback hooks. */
+int sfem_amd_switch_state(const sfem_amd_op *sw, int *active, int *events);
+/* Flexible CG (mode 1) only: after every iteration with ||r_k|| > max_ratio * ||r_{k-1}||, select
+ * the fallback of `sw` (once).  Several switches may be registered on one CG. */
+int sfem_amd_cg_add_fallback(sfem_amd_op *cg, sfem_amd_op *sw, double max_ratio);
+
 /* ------------------------------------------------------------------ measurement */
 /* Times `reps` applies y += A x on `stream` with HIP events (after `warmup` untimed ones).
  * flush_bytes > 0: before every timed apply a kernel overwrites a scratch buffer of that

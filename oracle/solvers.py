@@ -57,6 +57,8 @@ def cg(A, b, x, rtol=1e-10, atol=1e-16, max_it=10000, precond=None):
     for it in range(max_it):
         z = _apply0(precond, r)
         rtz_new = r @ z
+        if rtz_new == 0:  # exact convergence; the next step would be 0/0 (guard of the build)
+            return x, it, 0, hist
         beta = rtz_new / rtz
         rtz = rtz_new
         p = z + beta * p

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libsfem_amd.so")
+# SFEM_AMD_LIBRARY selects another build of the same library (e.g. a debug build)
+LIB_PATH = os.environ.get("SFEM_AMD_LIBRARY") or os.path.join(_HERE, "lib", "libsfem_amd.so")
 
 TET4, HEX8, SSHEX8 = 4, 8, 108
 FLOAT64, FLOAT32 = 0, 1
@@ -84,6 +85,20 @@ SIGNATURES = {
     "sfem_amd_cg_history": (_i, [_vp, _vp, _i]),
     "sfem_amd_op_last_status": (_i, [_vp]),
     "sfem_amd_op_benchmark": (_i, [_vp, _vp, _vp, _i, _i, C.c_size_t, _vp, C.POINTER(_d)]),
+    "sfem_amd_cnn_create": (_i, [_i, _i, _vp, C.c_uint, C.POINTER(_vp)]),
+    "sfem_amd_cnn_num_params": (_i, [_vp]),
+    "sfem_amd_cnn_get_weights": (_i, [_vp, _vp]),
+    "sfem_amd_cnn_set_weights": (_i, [_vp, _vp]),
+    "sfem_amd_cnn_destroy": (None, [_vp]),
+    "sfem_amd_cnn_forward": (_i, [_vp, _i, _pd, _vp, _vp, _i, _vp]),
+    "sfem_amd_cnn_loss_grad": (_i, [_vp, _i, _pd, _vp, _vp, C.POINTER(_d), _vp, _vp]),
+    "sfem_amd_cnn_adam_step": (_i, [_vp, _d, _vp]),
+    "sfem_amd_gather_elements": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    "sfem_amd_learned_smoother_create": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _i, C.POINTER(_vp)]),
+    "sfem_amd_switch_create": (_i, [_vp, _vp, C.POINTER(_vp)]),
+    "sfem_amd_switch_select": (_i, [_vp, _i]),
+    "sfem_amd_switch_state": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
+    "sfem_amd_cg_add_fallback": (_i, [_vp, _vp, _d]),
 }
 
 
@@ -447,6 +462,93 @@ def precision_bridge(op):
 
 def solver_bridge(solver):
     return _new_op("sfem_amd_solver_bridge_create", solver.h, keep=[solver])
+
+
+class CNN:
+    """The learned smoother's network (sfem_amd_cnn): 3x3x3 convolutions without biases,
+    1 -> hidden -> ... -> 1 channels with ReLU between, f(z) = g(z) - g(-z) per macro element."""
+
+    def __init__(self, hidden=16, layers=4, weights=None, seed=0):
+        h = _vp()
+        w = None if weights is None else _host(weights, np.float32)
+        _check(lib.sfem_amd_cnn_create(hidden, layers, None if w is None else _np_ptr(w), seed, C.byref(h)),
+               "cnn_create")
+        self.h = h
+        self.hidden, self.layers = hidden, layers
+
+    @property
+    def num_params(self):
+        return lib.sfem_amd_cnn_num_params(self.h)
+
+    def get_weights(self):
+        w = np.zeros(self.num_params, np.float32)
+        _check(lib.sfem_amd_cnn_get_weights(self.h, _np_ptr(w)), "cnn_get_weights")
+        return w
+
+    def set_weights(self, w):
+        w = _host(w, np.float32)
+        if w.size != self.num_params:
+            raise SfemError(f"cnn: {self.num_params} weights expected, got {w.size}")
+        _check(lib.sfem_amd_cnn_set_weights(self.h, _np_ptr(w)), "cnn_set_weights")
+
+    def forward(self, level, z, out, mode=0, stream=None):
+        n = (level + 1) ** 3
+        if z.numel() % n or out.numel() != z.numel() or z.element_size() != 4 or out.element_size() != 4:
+            raise SfemError("cnn_forward: float32 [E][(L+1)^3] input and output expected")
+        _check(lib.sfem_amd_cnn_forward(self.h, level, z.numel() // n, _dptr(z), _dptr(out), mode,
+                                        _stream_ptr(stream)), "cnn_forward")
+        return out
+
+    def loss_grad(self, level, z, target, want_grad=True, stream=None):
+        n = (level + 1) ** 3
+        if z.numel() % n or target.numel() != z.numel():
+            raise SfemError("cnn_loss_grad: float32 [E][(L+1)^3] input and target expected")
+        loss = C.c_double()
+        g = np.zeros(self.num_params, np.float32) if want_grad else None
+        _check(lib.sfem_amd_cnn_loss_grad(self.h, level, z.numel() // n, _dptr(z), _dptr(target), C.byref(loss),
+                                          None if g is None else _np_ptr(g), _stream_ptr(stream)), "cnn_loss_grad")
+        return loss.value, g
+
+    def adam_step(self, lr, stream=None):
+        _check(lib.sfem_amd_cnn_adam_step(self.h, lr, _stream_ptr(stream)), "cnn_adam_step")
+
+    def __del__(self):
+        if getattr(self, "h", None) and lib is not None:
+            lib.sfem_amd_cnn_destroy(self.h)
+            self.h = None
+
+
+def gather_elements(mesh, vec, out, block_size=1, component=0, scale=None, stream=None):
+    """out[e][v] = vec[idx[e][v]*bs + comp] (* scale[...]) as float32 (sshex8 meshes)."""
+    rt = FLOAT64 if vec.element_size() == 8 else FLOAT32
+    _check(lib.sfem_amd_gather_elements(mesh.h, rt, block_size, component, _dptr(vec),
+                                        None if scale is None else _dptr(scale), _dptr(out), _stream_ptr(stream)),
+           "gather_elements")
+    return out
+
+
+def learned_smoother(A, mesh, cnn, diag, block_size=1, steps=1, damping=1.0, mode=1):
+    return _new_op("sfem_amd_learned_smoother_create", A.h, mesh.h, cnn.h, _dptr(diag), block_size, steps,
+                   damping, mode, keep=[A, mesh, cnn])
+
+
+def switch(primary, fallback):
+    return _new_op("sfem_amd_switch_create", primary.h, fallback.h, keep=[primary, fallback])
+
+
+def switch_select(sw, which):
+    _check(lib.sfem_amd_switch_select(sw.h, which), "switch_select")
+
+
+def switch_state(sw):
+    a, e = C.c_int(), C.c_int()
+    _check(lib.sfem_amd_switch_state(sw.h, C.byref(a), C.byref(e)), "switch_state")
+    return a.value, e.value
+
+
+def cg_add_fallback(cg_op, sw, max_ratio=0.5):
+    _check(lib.sfem_amd_cg_add_fallback(cg_op.h, sw.h, max_ratio), "cg_add_fallback")
+    cg_op._keep.append(sw)
 
 
 def benchmark(op, x, y, warmup=2, reps=10, flush_bytes=0, stream=None):

@@ -1,0 +1,527 @@
+// Learned sshex8 smoother: a per-macro-element 3x3x3 CNN on the (L+1)^3 micro-grid (zero
+// padding outside the element).  Training kernels run in FP32 (direct convolution with the
+// element's channels staged in LDS); inference runs the fused network on MFMA with BF16
+// weights and activations and FP32 accumulation.
+//
+// Layouts: activations [E][C][nxe] float (node index = lidx), weights [Cout][Cin][27] with
+// offset k = (dz+1)*9 + (dy+1)*3 + (dx+1).
+#include "cnn_kernels.hpp"
+#include "common.hpp"
+
+namespace sfem_amd {
+
+__device__ __forceinline__ int nbr(int L, int x, int y, int z, int k) {
+    const int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+    const int xx = x + dx, yy = y + dy, zz = z + dz;
+    if (xx < 0 || xx > L || yy < 0 || yy > L || zz < 0 || zz > L) return -1;
+    return lidx(L, xx, yy, zz);
+}
+
+// out[e][co][v] = sum_ci sum_k W[co][ci][k] act(in[e][ci][v + off_k]);  act = relu if relu_in
+__global__ void __launch_bounds__(256) conv_fwd_kernel(const int L, const ptrdiff_t E, const int Cin, const int Cout,
+                                                       const float *__restrict__ in, const float *__restrict__ W,
+                                                       const int relu_in, float *__restrict__ out) {
+    extern __shared__ float smem[];
+    const int nxe = nxe_of(L);
+    float *sin = smem;  // [Cin][nxe]; the weights (<= 28 KB) are read through the caches
+    const float *sw = W;
+    const ptrdiff_t e = blockIdx.x;
+    if (e >= E) return;  // uniform per block
+    for (int i = threadIdx.x; i < Cin * nxe; i += blockDim.x) {
+        float v = in[(size_t)e * Cin * nxe + i];
+        sin[i] = relu_in ? fmaxf(v, 0.f) : v;
+    }
+    __syncthreads();
+    const int N1 = L + 1;
+    for (int t = threadIdx.x; t < Cout * nxe; t += blockDim.x) {
+        const int co = t / nxe, v = t - co * nxe;
+        const int x = v % N1, y = (v / N1) % N1, z = v / (N1 * N1);
+        int nb[27];
+#pragma unroll
+        for (int k = 0; k < 27; k++) nb[k] = nbr(L, x, y, z, k);
+        float acc = 0.f;
+        for (int ci = 0; ci < Cin; ci++) {
+            const float *w = sw + ((size_t)co * Cin + ci) * 27;
+            const float *a = sin + (size_t)ci * nxe;
+#pragma unroll
+            for (int k = 0; k < 27; k++)
+                if (nb[k] >= 0) acc += w[k] * a[nb[k]];
+        }
+        out[((size_t)e * Cout + co) * nxe + v] = acc;
+    }
+}
+
+// din[e][ci][v] = relu'(pre) * sum_co sum_k W[co][ci][k] dout[e][co][v - off_k]
+// (act_in holds the layer input before the activation when relu_in, for the mask)
+__global__ void __launch_bounds__(256) conv_bwd_data_kernel(const int L, const ptrdiff_t E, const int Cin,
+                                                            const int Cout, const float *__restrict__ dout,
+                                                            const float *__restrict__ W,
+                                                            const float *__restrict__ act_in, const int relu_in,
+                                                            float *__restrict__ din) {
+    extern __shared__ float smem[];
+    const int nxe = nxe_of(L);
+    float *sd = smem;  // [Cout][nxe]
+    const float *sw = W;
+    const ptrdiff_t e = blockIdx.x;
+    if (e >= E) return;
+    for (int i = threadIdx.x; i < Cout * nxe; i += blockDim.x) sd[i] = dout[(size_t)e * Cout * nxe + i];
+    __syncthreads();
+    const int N1 = L + 1;
+    for (int t = threadIdx.x; t < Cin * nxe; t += blockDim.x) {
+        const int ci = t / nxe, v = t - ci * nxe;
+        const int x = v % N1, y = (v / N1) % N1, z = v / (N1 * N1);
+        float acc = 0.f;
+        // out[m] += W[k] in[m + off_k]  =>  din[v] += W[k] dout[v - off_k]
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+            const int m = nbr(L, x, y, z, 26 - k);  // v - off_k
+            if (m < 0) continue;
+            for (int co = 0; co < Cout; co++) acc += sw[((size_t)co * Cin + ci) * 27 + k] * sd[(size_t)co * nxe + m];
+        }
+        if (relu_in && act_in[((size_t)e * Cin + ci) * nxe + v] <= 0.f) acc = 0.f;
+        din[((size_t)e * Cin + ci) * nxe + v] = acc;
+    }
+}
+
+// partial[b][co][ci][k] = sum over the block's elements and nodes of dout[co][v] act(in[ci][v+off_k]);
+// blockIdx.y selects a chunk of kCoChunk output channels (LDS: Cin + kCoChunk channels).
+constexpr int kCoChunk = 4;
+__global__ void __launch_bounds__(256) conv_bwd_weight_kernel(const int L, const ptrdiff_t E, const int Cin,
+                                                              const int Cout, const float *__restrict__ dout,
+                                                              const float *__restrict__ in, const int relu_in,
+                                                              float *__restrict__ partial) {
+    extern __shared__ float smem[];
+    const int nxe = nxe_of(L);
+    const int co0 = blockIdx.y * kCoChunk;
+    const int nco = min(kCoChunk, Cout - co0);
+    float *sin = smem;                     // [Cin][nxe]
+    float *sd = smem + (size_t)Cin * nxe;   // [nco][nxe]
+    const int nw_chunk = nco * Cin * 27;     // <= 4 * 16 * 27 = 1728 -> <= 7 per thread
+    float acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc[i] = 0.f;
+    for (ptrdiff_t e = blockIdx.x; e < E; e += gridDim.x) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < Cin * nxe; i += blockDim.x) {
+            const float v = in[(size_t)e * Cin * nxe + i];
+            sin[i] = relu_in ? fmaxf(v, 0.f) : v;
+        }
+        for (int i = threadIdx.x; i < nco * nxe; i += blockDim.x)
+            sd[i] = dout[((size_t)e * Cout + co0) * nxe + i];
+        __syncthreads();
+        for (int j = 0; j < 8; j++) {
+            const int t = threadIdx.x + j * blockDim.x;
+            if (t >= nw_chunk) break;
+            const int col = t / (Cin * 27), r = t - col * Cin * 27, ci = r / 27, k = r - ci * 27;
+            const int dz = k / 9 - 1, dy = (k / 3) % 3 - 1, dx = k % 3 - 1;
+            const float *d = sd + (size_t)col * nxe;
+            const float *a = sin + (size_t)ci * nxe;
+            float s = 0.f;
+            for (int z = (dz < 0 ? 1 : 0); z <= L - (dz > 0 ? 1 : 0); z++)
+                for (int y = (dy < 0 ? 1 : 0); y <= L - (dy > 0 ? 1 : 0); y++)
+                    for (int x = (dx < 0 ? 1 : 0); x <= L - (dx > 0 ? 1 : 0); x++)
+                        s += d[lidx(L, x, y, z)] * a[lidx(L, x + dx, y + dy, z + dz)];
+            acc[j] += s;
+        }
+    }
+    const int nw = Cout * Cin * 27;
+    for (int j = 0; j < 8; j++) {
+        const int t = threadIdx.x + j * blockDim.x;
+        if (t >= nw_chunk) break;
+        partial[(size_t)blockIdx.x * nw + (size_t)co0 * Cin * 27 + t] = acc[j];
+    }
+}
+
+__global__ void sum_partials_f32_kernel(const int nb, const int nw, const float *__restrict__ partial,
+                                        const float scale, float *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nw) return;
+    float s = 0.f;
+    for (int b = 0; b < nb; b++) s += partial[(size_t)b * nw + t];
+    out[t] += scale * s;
+}
+
+// out = a - b (the odd symmetrisation g(z) - g(-z))
+__global__ void sub_halves_kernel(const ptrdiff_t n, const float *__restrict__ a, const float *__restrict__ b,
+                                  float *__restrict__ out) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        out[i] = a[i] - b[i];
+}
+
+// x2 = [x; -x]
+__global__ void make_pm_kernel(const ptrdiff_t n, const float *__restrict__ x, float *__restrict__ x2) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
+        x2[i] = x[i];
+        x2[n + i] = -x[i];
+    }
+}
+
+// squared-error partials: part[b] = (sum (o - t)^2, sum t^2) over a fixed grid
+__global__ void __launch_bounds__(256) sq_err_partial_kernel(const ptrdiff_t n, const float *__restrict__ o,
+                                                             const float *__restrict__ t, double *__restrict__ part) {
+    __shared__ double s1[256], s2[256];
+    double a = 0, b = 0;
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const double d = (double)o[i] - (double)t[i];
+        a += d * d;
+        b += (double)t[i] * (double)t[i];
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            s1[threadIdx.x] += s1[threadIdx.x + st];
+            s2[threadIdx.x] += s2[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s1[0];
+        part[2 * blockIdx.x + 1] = s2[0];
+    }
+}
+
+// dpm = [2 (o - t) / norm; -2 (o - t) / norm]
+__global__ void dloss_kernel(const ptrdiff_t n, const float *__restrict__ o, const float *__restrict__ t,
+                             const float inv_norm, float *__restrict__ dpm) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const float d = 2.f * (o[i] - t[i]) * inv_norm;
+        dpm[i] = d;
+        dpm[n + i] = -d;
+    }
+}
+
+__global__ void adam_kernel(const int n, const float *__restrict__ g, float *__restrict__ w, float *__restrict__ m,
+                            float *__restrict__ v, const float lr, const float b1, const float b2, const float eps,
+                            const float c1, const float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// out[e][v] = vec[idx[e][v]*bs + comp] * (scale ? scale[...] : 1)
+template <typename T>
+__global__ void gather_elements_kernel(const ptrdiff_t E, const int nxe, const int32_t *__restrict__ idx, const int bs,
+                                       const int comp, const T *__restrict__ vec, const T *__restrict__ scale,
+                                       float *__restrict__ out) {
+    const ptrdiff_t n = E * nxe;
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const ptrdiff_t d = (ptrdiff_t)idx[t] * bs + comp;
+        out[t] = (float)(scale ? vec[d] * scale[d] : vec[d]);
+    }
+}
+
+// vec[idx[e][v]*bs + comp] += alpha * val[e][v] / count[idx]
+template <typename T>
+__global__ void scatter_elements_kernel(const ptrdiff_t E, const int nxe, const int32_t *__restrict__ idx, const int bs,
+                                        const int comp, const uint16_t *__restrict__ count, const float alpha,
+                                        const float *__restrict__ val, T *__restrict__ vec) {
+    const ptrdiff_t n = E * nxe;
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const int32_t g = idx[t];
+        atomicAdd(&vec[(ptrdiff_t)g * bs + comp], (T)(alpha * val[t] / (float)count[g]));
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+static size_t smem_fwd(int L, int Cin, int /*Cout*/) { return sizeof(float) * ((size_t)Cin * nxe_of(L)); }
+
+int cnn_conv_forward(int L, ptrdiff_t E, int Cin, int Cout, const float *in, const float *W, int relu_in, float *out,
+                     hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    conv_fwd_kernel<<<(unsigned)E, 256, smem_fwd(L, Cin, Cout), s>>>(L, E, Cin, Cout, in, W, relu_in, out);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+int cnn_conv_backward_data(int L, ptrdiff_t E, int Cin, int Cout, const float *dout, const float *W, const float *act_in,
+                           int relu_in, float *din, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    const size_t smem = sizeof(float) * ((size_t)Cout * nxe_of(L));
+    conv_bwd_data_kernel<<<(unsigned)E, 256, smem, s>>>(L, E, Cin, Cout, dout, W, act_in, relu_in, din);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+int cnn_conv_backward_weights(int L, ptrdiff_t E, int Cin, int Cout, const float *dout, const float *in, int relu_in,
+                              float *partial, int nblocks, float *dW, hipStream_t s) {
+    const int nw = Cout * Cin * 27;
+    if (kCoChunk * Cin * 27 > 8 * 256) {
+        set_last_error("cnn: layer too wide for the weight-gradient kernel");
+        return SFEM_FAILURE;
+    }
+    if (E == 0) return SFEM_SUCCESS;
+    const size_t smem = sizeof(float) * ((size_t)(Cin + kCoChunk) * nxe_of(L));
+    dim3 grid(nblocks, (Cout + kCoChunk - 1) / kCoChunk);
+    conv_bwd_weight_kernel<<<grid, 256, smem, s>>>(L, E, Cin, Cout, dout, in, relu_in, partial);
+    sum_partials_f32_kernel<<<(nw + 255) / 256, 256, 0, s>>>(nblocks, nw, partial, 1.f, dW);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+int cnn_make_pm(ptrdiff_t n, const float *x, float *x2, hipStream_t s) {
+    make_pm_kernel<<<grid_for(n), kBlock, 0, s>>>(n, x, x2);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+int cnn_sub_halves(ptrdiff_t n, const float *a, const float *b, float *out, hipStream_t s) {
+    sub_halves_kernel<<<grid_for(n), kBlock, 0, s>>>(n, a, b, out);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+int cnn_sq_err(ptrdiff_t n, const float *o, const float *t, double *ws, double *err_host, double *norm_host,
+               hipStream_t s) {
+    const int nb = 256;
+    sq_err_partial_kernel<<<nb, 256, 0, s>>>(n, o, t, ws);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    double h[2 * 256];
+    SFEM_AMD_HIP_CHECK(hipMemcpyAsync(h, ws, sizeof(h), hipMemcpyDeviceToHost, s));
+    SFEM_AMD_HIP_CHECK(hipStreamSynchronize(s));
+    double a = 0, b = 0;
+    for (int i = 0; i < nb; i++) {
+        a += h[2 * i];
+        b += h[2 * i + 1];
+    }
+    *err_host = a;
+    *norm_host = b;
+    return SFEM_SUCCESS;
+}
+
+int cnn_dloss(ptrdiff_t n, const float *o, const float *t, float inv_norm, float *dpm, hipStream_t s) {
+    dloss_kernel<<<grid_for(n), kBlock, 0, s>>>(n, o, t, inv_norm, dpm);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+int cnn_adam(int n, const float *g, float *w, float *m, float *v, float lr, float b1, float b2, float eps, int step,
+             hipStream_t s) {
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    adam_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, g, w, m, v, lr, b1, b2, eps, c1, c2);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template <typename T>
+int gather_elements(ptrdiff_t E, int nxe, const int32_t *idx, int bs, int comp, const T *vec, const T *scale,
+                    float *out, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    gather_elements_kernel<T><<<grid_for(E * nxe), kBlock, 0, s>>>(E, nxe, idx, bs, comp, vec, scale, out);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template <typename T>
+int scatter_elements(ptrdiff_t E, int nxe, const int32_t *idx, int bs, int comp, const uint16_t *count, float alpha,
+                     const float *val, T *vec, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    scatter_elements_kernel<T><<<grid_for(E * nxe), kBlock, 0, s>>>(E, nxe, idx, bs, comp, count, alpha, val, vec);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template int gather_elements<double>(ptrdiff_t, int, const int32_t *, int, int, const double *, const double *, float *,
+                                     hipStream_t);
+template int gather_elements<float>(ptrdiff_t, int, const int32_t *, int, int, const float *, const float *, float *,
+                                    hipStream_t);
+template int scatter_elements<double>(ptrdiff_t, int, const int32_t *, int, int, const uint16_t *, float, const float *,
+                                      double *, hipStream_t);
+template int scatter_elements<float>(ptrdiff_t, int, const int32_t *, int, int, const uint16_t *, float, const float *,
+                                     float *, hipStream_t);
+
+// ------------------------------------------------------------------ fused BF16 MFMA inference
+// One macro element per block (512 threads = 8 waves); the element's activations stay in LDS
+// for all four layers and both signs of f(z) = g(z) - g(-z).  Nodes are processed in tiles of
+// 16 (the MFMA N dimension); per tile and layer:
+//   layer 1 (1 -> 16):  D[co][n]  = sum_{k<32} W1[co][k] z[n + off_k]         one 16x16x32 MFMA
+//   layer 2,3 (16->16): D[co][n] += sum_{pairs p} W[co][ci,off] act[n + off]  14 16x16x32 MFMAs,
+//                       K = 32 = two offsets x 16 input channels
+//   layer 4 (16 -> 1):  P[k][n]   = sum_ci W4[ci][k] act[ci][n] (two 16x16x16 MFMAs, k < 32),
+//                       then out[n] = sum_k P[k][n + off_k] from LDS.
+// Lane layouts (64 lanes, g = lane / 16, c = lane % 16):
+//   16x16x32 A: row c, k = 8g + j (j < 8);  B: column c, k = 8g + j;  D: column c, rows 4g + r
+//   16x16x16 A: row c, k = 4g + j (j < 4);  B: column c, k = 4g + j;  D: as above
+// Activations are stored post-ReLU in BF16 as [node][16 channels]; padding nodes of the last
+// tile and neighbours outside the element read as zero.
+typedef short mf_s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mf_b8 __attribute__((ext_vector_type(8)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int mf_u4 __attribute__((ext_vector_type(4)));
+typedef unsigned int mf_u2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+    unsigned int u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+template <int L>
+struct CnnShape {
+    static constexpr int N1 = L + 1;
+    static constexpr int NXE = N1 * N1 * N1;
+    static constexpr int T = (NXE + 15) / 16;
+    static constexpr int NP = T * 16;
+    static constexpr int THREADS = 512;
+    static constexpr size_t LDS = sizeof(float) * NP + 2 * 16 * NP + (size_t)27 * NP * sizeof(float);
+};
+
+template <int L>
+__device__ __forceinline__ bool cnn_valid(int x, int y, int z, int o) {
+    const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
+    return o < 27 && (unsigned)(x + dx) <= (unsigned)L && (unsigned)(y + dy) <= (unsigned)L &&
+           (unsigned)(z + dz) <= (unsigned)L;
+}
+template <int L>
+__device__ __forceinline__ int cnn_shift(int o) {
+    constexpr int N1 = L + 1;
+    return (o / 9 - 1) * N1 * N1 + ((o / 3) % 3 - 1) * N1 + (o % 3 - 1);
+}
+
+template <int L>
+__global__ void __launch_bounds__(512) cnn_infer_kernel(const ptrdiff_t E, const unsigned short *__restrict__ W16,
+                                                        const float *__restrict__ zin, float *__restrict__ out) {
+    using S = CnnShape<L>;
+    extern __shared__ __align__(16) unsigned char cnn_smem[];
+    float *zs = (float *)cnn_smem;                                        // [NP]
+    unsigned short *actX = (unsigned short *)(cnn_smem + 4 * S::NP);      // [NP][16]
+    unsigned short *actY = actX + 16 * S::NP;                             // [NP][16]
+    float *P = (float *)(cnn_smem + 4 * S::NP + 32 * S::NP);              // [27][NP], aliases actY
+    const ptrdiff_t e = blockIdx.x;
+    if (e >= E) return;  // uniform per block
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = S::THREADS / 64;
+    const int g = lane >> 4, c = lane & 15;
+
+    for (int i = tid; i < S::NP; i += S::THREADS) zs[i] = i < S::NXE ? zin[(size_t)e * S::NXE + i] : 0.f;
+    // weights in lane order: layer 1 [64][8], layers 2-3 [14][64][8], layer 4 [2][64][4]
+    const mf_u4 w1 = *(const mf_u4 *)(W16 + lane * 8);
+    const mf_u2 w4a = *(const mf_u2 *)(W16 + 512 + 2 * 7168 + lane * 4);
+    const mf_u2 w4b = *(const mf_u2 *)(W16 + 512 + 2 * 7168 + 256 + lane * 4);
+    float gplus[(S::NXE + S::THREADS - 1) / S::THREADS];
+    __syncthreads();
+
+    for (int sign = 0; sign < 2; sign++) {
+        const float sg = sign ? -1.f : 1.f;
+        // ---- layer 1: zs -> actX
+        for (int t = wave; t < S::T; t += nw) {
+            const int n = t * 16 + c;
+            const int x = n % S::N1, y = (n / S::N1) % S::N1, z = n / (S::N1 * S::N1);
+            const bool in = n < S::NXE;
+            mf_b8 b;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int o = 8 * g + j;
+                const float v = (in && cnn_valid<L>(x, y, z, o)) ? zs[n + cnn_shift<L>(o)] : 0.f;
+                b[j] = __builtin_bit_cast(__bf16, f2bf(v));
+            }
+            mf_f4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mf_b8, w1), b, acc, 0, 0, 0);
+            mf_u2 o2;
+            o2[0] = (unsigned)f2bf(fmaxf(sg * acc[0], 0.f)) | ((unsigned)f2bf(fmaxf(sg * acc[1], 0.f)) << 16);
+            o2[1] = (unsigned)f2bf(fmaxf(sg * acc[2], 0.f)) | ((unsigned)f2bf(fmaxf(sg * acc[3], 0.f)) << 16);
+            *(mf_u2 *)(actX + n * 16 + 4 * g) = o2;
+        }
+        __syncthreads();
+        // ---- layers 2 and 3: actX -> actY -> actX
+#pragma unroll 1
+        for (int layer = 0; layer < 2; layer++) {
+            const unsigned short *ain = layer ? actY : actX;
+            unsigned short *aout = layer ? actX : actY;
+            const unsigned short *wl = W16 + 512 + layer * 7168;
+            mf_u4 w[14];
+#pragma unroll
+            for (int p = 0; p < 14; p++) w[p] = *(const mf_u4 *)(wl + (p * 64 + lane) * 8);
+            for (int t = wave; t < S::T; t += nw) {
+                const int n = t * 16 + c;
+                const int x = n % S::N1, y = (n / S::N1) % S::N1, z = n / (S::N1 * S::N1);
+                const bool in = n < S::NXE;
+                mf_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int p = 0; p < 14; p++) {
+                    const int o = 2 * p + (g >> 1);
+                    mf_u4 bv = {0u, 0u, 0u, 0u};
+                    if (in && cnn_valid<L>(x, y, z, o))
+                        bv = *(const mf_u4 *)(ain + (n + cnn_shift<L>(o)) * 16 + 8 * (g & 1));
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mf_b8, w[p]),
+                                                                   __builtin_bit_cast(mf_b8, bv), acc, 0, 0, 0);
+                }
+                mf_u2 o2;
+                o2[0] = (unsigned)f2bf(fmaxf(acc[0], 0.f)) | ((unsigned)f2bf(fmaxf(acc[1], 0.f)) << 16);
+                o2[1] = (unsigned)f2bf(fmaxf(acc[2], 0.f)) | ((unsigned)f2bf(fmaxf(acc[3], 0.f)) << 16);
+                *(mf_u2 *)(aout + n * 16 + 4 * g) = o2;
+            }
+            __syncthreads();
+        }
+        // ---- layer 4, channel mixing: actX -> P[k][n] (k < 27)
+        for (int t = wave; t < S::T; t += nw) {
+            const int n = t * 16 + c;
+            const mf_u2 bv = *(const mf_u2 *)(actX + n * 16 + 4 * g);
+            mf_f4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
+            a0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(mf_s4, w4a), __builtin_bit_cast(mf_s4, bv),
+                                                            a0, 0, 0, 0);
+            a1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(mf_s4, w4b), __builtin_bit_cast(mf_s4, bv),
+                                                            a1, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                P[(4 * g + r) * S::NP + n] = a0[r];
+                const int k = 16 + 4 * g + r;
+                if (k < 27) P[k * S::NP + n] = a1[r];
+            }
+        }
+        __syncthreads();
+        // ---- layer 4, stencil sum: g(+-z)[n] = sum_k P[k][n + off_k]
+#pragma unroll
+        for (int i = 0; i < (S::NXE + S::THREADS - 1) / S::THREADS; i++) {
+            const int n = tid + i * S::THREADS;
+            if (n < S::NXE) {
+                const int x = n % S::N1, y = (n / S::N1) % S::N1, z = n / (S::N1 * S::N1);
+                float v = 0.f;
+#pragma unroll
+                for (int k = 0; k < 27; k++)
+                    if (cnn_valid<L>(x, y, z, k)) v += P[k * S::NP + n + cnn_shift<L>(k)];
+                if (sign == 0)
+                    gplus[i] = v;
+                else
+                    out[(size_t)e * S::NXE + n] = gplus[i] - v;
+            }
+        }
+        // the next sign's layer-1 barrier orders these reads before layer 2 overwrites P
+    }
+}
+
+template <int L>
+static int launch_cnn_infer(ptrdiff_t E, const uint16_t *W16, const float *z, float *out, hipStream_t s) {
+    using S = CnnShape<L>;
+    static bool attr = false;
+    if (!attr) {
+        SFEM_AMD_HIP_CHECK(
+            hipFuncSetAttribute((const void *)cnn_infer_kernel<L>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S::LDS));
+        attr = true;
+    }
+    cnn_infer_kernel<L><<<(unsigned)E, S::THREADS, S::LDS, s>>>(E, (const unsigned short *)W16, z, out);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+int cnn_infer_mfma(int L, ptrdiff_t E, const uint16_t *W16, const float *z, float *out, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    switch (L) {
+        case 1: return launch_cnn_infer<1>(E, W16, z, out, s);
+        case 2: return launch_cnn_infer<2>(E, W16, z, out, s);
+        case 3: return launch_cnn_infer<3>(E, W16, z, out, s);
+        case 4: return launch_cnn_infer<4>(E, W16, z, out, s);
+        case 5: return launch_cnn_infer<5>(E, W16, z, out, s);
+        case 6: return launch_cnn_infer<6>(E, W16, z, out, s);
+        case 7: return launch_cnn_infer<7>(E, W16, z, out, s);
+        case 8: return launch_cnn_infer<8>(E, W16, z, out, s);
+        default: set_last_error("cnn_infer_mfma: level must be 1..8"); return SFEM_FAILURE;
+    }
+}
+
+}  // namespace sfem_amd

@@ -39,6 +39,23 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 // operators/sshex8/sshex8.hpp:18-24: local id = z*(L+1)^2 + y*(L+1) + x
 __host__ __device__ constexpr inline int lidx(int L, int x, int y, int z) { return (z * (L + 1) + y) * (L + 1) + x; }
 __host__ __device__ constexpr inline int nxe_of(int L) { return (L + 1) * (L + 1) * (L + 1); }
+// Skeleton nodes of a macro element (on its faces, edges or corners: x, y or z in {0, L}),
+// numbered in lidx order.  A line (y, z) is a skeleton line when y or z is in {0, L}: all
+// of its L+1 nodes are skeleton nodes; any other line holds two (x = 0 and x = L).
+__host__ __device__ constexpr inline int sk_count(int L) { return 2 * (L + 1) * (L + 1) + 4 * L * (L - 1); }
+__host__ __device__ constexpr inline bool sk_line(int L, int y, int z) { return y == 0 || y == L || z == 0 || z == L; }
+// slot of node (0, y, z)
+__host__ __device__ constexpr inline int sk_line_base(int L, int y, int z) {
+    const int N1 = L + 1;
+    const int plane = z == 0 ? 0 : N1 * N1 + (z - 1) * 4 * L;
+    const int in_plane = (z == 0 || z == L) ? y * N1 : (y == 0 ? 0 : N1 + (y - 1) * 2);
+    return plane + in_plane;
+}
+// slot of skeleton node (x, y, z); -1 for interior nodes
+__host__ __device__ constexpr inline int sk_slot(int L, int x, int y, int z) {
+    return sk_line(L, y, z) ? sk_line_base(L, y, z) + x
+                            : (x == 0 ? sk_line_base(L, y, z) : (x == L ? sk_line_base(L, y, z) + 1 : -1));
+}
 
 // hex8 corner order (python/sfem/mesh/box_mesh.py:520-534, sshex8.hpp:53-62):
 // bit pattern of corner v: x = (v==1||v==2||v==5||v==6), y = (v==2||v==3||v==6||v==7), z = v>=4

@@ -1,0 +1,111 @@
+// Internal types shared by the runtime translation units (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "transfer_blas_kernels.hpp"
+
+namespace sfem_amd {
+
+// ----------------------------------------------------------------- device memory helper
+template <typename T>
+struct DevArray {
+    T *p{nullptr};
+    size_t n{0};
+    DevArray() = default;
+    DevArray(const DevArray &) = delete;
+    DevArray &operator=(const DevArray &) = delete;
+    ~DevArray() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t count) {
+        reset();
+        if (count == 0) return SFEM_SUCCESS;
+        SFEM_AMD_HIP_CHECK(hipMalloc((void **)&p, sizeof(T) * count));
+        n = count;
+        return SFEM_SUCCESS;
+    }
+    int upload(const T *h, size_t count) {
+        if (alloc(count)) return SFEM_FAILURE;
+        if (count) SFEM_AMD_HIP_CHECK(hipMemcpy(p, h, sizeof(T) * count, hipMemcpyHostToDevice));
+        return SFEM_SUCCESS;
+    }
+};
+
+inline double *reduction_ws() {
+    static double *ws = nullptr;
+    if (!ws) {
+        if (hipMalloc((void **)&ws, sizeof(double) * reduction_workspace_doubles()) != hipSuccess) ws = nullptr;
+    }
+    return ws;
+}
+
+inline uint16_t to_bf16(float f) {
+    // round to nearest even
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+    const uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (uint16_t)(u >> 16);
+}
+
+// ----------------------------------------------------------------- operator base
+struct OpBase {
+    virtual ~OpBase() = default;
+    virtual int apply(const void *x, void *y, hipStream_t s) = 0;
+    virtual ptrdiff_t rows() const = 0;
+    virtual ptrdiff_t cols() const { return rows(); }
+    virtual int real_type() const = 0;
+    virtual int diag(void *, hipStream_t) {
+        set_last_error("diag not available for this operator");
+        return SFEM_FAILURE;
+    }
+    virtual int block_diag(void *, hipStream_t) {
+        set_last_error("block diagonal not available for this operator");
+        return SFEM_FAILURE;
+    }
+    virtual int iterations() const { return -1; }
+    virtual int last_status() const { return SFEM_SUCCESS; }
+    virtual const std::vector<double> *history() const { return nullptr; }
+};
+
+}  // namespace sfem_amd
+
+struct sfem_amd_op {
+    std::shared_ptr<sfem_amd::OpBase> impl;
+};
+
+struct sfem_amd_mesh {
+    int type{0};
+    int level{1};
+    ptrdiff_t E{0};
+    ptrdiff_t nnodes{0};
+    int nv{0};                             // nodes per element
+    sfem_amd::DevArray<int32_t> idx;       // SoA [nv][E] (tet4/hex8) or element-major [E][nv] (sshex8)
+    mutable sfem_amd::DevArray<uint16_t> count;  // incidence counts (lazy)
+    mutable sfem_amd::DevArray<uint8_t> own;     // owner mask (lazy)
+    const sfem_amd_mesh *parent{nullptr};        // set by sfem_amd_mesh_derefine
+    // sshex8: per-node skeleton slot lists (lazy, see mesh_skeleton)
+    mutable sfem_amd::DevArray<int32_t> sk_node, sk_off, sk_ent;
+    mutable ptrdiff_t sk_n{-1};
+};
+
+namespace sfem_amd {
+// Zeroes / sets constrained dofs; defined in runtime.cpp.
+int constrained_set_value(const OpBase *constrained, int rt, double v, void *x, hipStream_t s);
+int mesh_counts(const sfem_amd_mesh *m, hipStream_t s);
+bool is_constrained(const OpBase *op);
+// Builds (once) the skeleton slot lists of an sshex8 mesh from its connectivity.
+int mesh_skeleton(const sfem_amd_mesh *m);
+sfem_amd_op *wrap_op(std::shared_ptr<OpBase> o);
+}  // namespace sfem_amd

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "internal.hpp"
 #include "sfem/sfem_Chebyshev3.hpp"
 #include "sfem/sfem_Multigrid.hpp"
 #include "sfem/sfem_Stationary.hpp"
@@ -22,96 +23,64 @@ namespace sfem_amd {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 
-// ----------------------------------------------------------------- device memory helper
-template <typename T>
-struct DevArray {
-    T *p{nullptr};
-    size_t n{0};
-    DevArray() = default;
-    DevArray(const DevArray &) = delete;
-    DevArray &operator=(const DevArray &) = delete;
-    ~DevArray() { reset(); }
-    void reset() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    int alloc(size_t count) {
-        reset();
-        if (count == 0) return SFEM_SUCCESS;
-        SFEM_AMD_HIP_CHECK(hipMalloc((void **)&p, sizeof(T) * count));
-        n = count;
-        return SFEM_SUCCESS;
-    }
-    int upload(const T *h, size_t count) {
-        if (alloc(count)) return SFEM_FAILURE;
-        if (count) SFEM_AMD_HIP_CHECK(hipMemcpy(p, h, sizeof(T) * count, hipMemcpyHostToDevice));
-        return SFEM_SUCCESS;
-    }
-};
-
-static double *reduction_ws() {
-    static double *ws = nullptr;
-    if (!ws) {
-        if (hipMalloc((void **)&ws, sizeof(double) * reduction_workspace_doubles()) != hipSuccess) ws = nullptr;
-    }
-    return ws;
-}
-
-static uint16_t to_bf16(float f) {
-    // round to nearest even
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
-    const uint32_t lsb = (u >> 16) & 1u;
-    u += 0x7fffu + lsb;
-    return (uint16_t)(u >> 16);
-}
-
-// ----------------------------------------------------------------- operator base
-struct OpBase {
-    virtual ~OpBase() = default;
-    virtual int apply(const void *x, void *y, hipStream_t s) = 0;
-    virtual ptrdiff_t rows() const = 0;
-    virtual ptrdiff_t cols() const { return rows(); }
-    virtual int real_type() const = 0;
-    virtual int diag(void *, hipStream_t) {
-        set_last_error("diag not available for this operator");
-        return SFEM_FAILURE;
-    }
-    virtual int block_diag(void *, hipStream_t) {
-        set_last_error("block diagonal not available for this operator");
-        return SFEM_FAILURE;
-    }
-    virtual int iterations() const { return -1; }
-    virtual int last_status() const { return SFEM_SUCCESS; }
-    virtual const std::vector<double> *history() const { return nullptr; }
-};
-
 }  // namespace sfem_amd
-
-struct sfem_amd_op {
-    std::shared_ptr<sfem_amd::OpBase> impl;
-};
-
-struct sfem_amd_mesh {
-    int type{0};
-    int level{1};
-    ptrdiff_t E{0};
-    ptrdiff_t nnodes{0};
-    int nv{0};                             // nodes per element
-    sfem_amd::DevArray<int32_t> idx;       // SoA [nv][E] (tet4/hex8) or element-major [E][nv] (sshex8)
-    mutable sfem_amd::DevArray<uint16_t> count;  // incidence counts (lazy)
-    mutable sfem_amd::DevArray<uint8_t> own;     // owner mask (lazy)
-    const sfem_amd_mesh *parent{nullptr};        // set by sfem_amd_mesh_derefine
-};
 
 namespace sfem_amd {
 
-static int mesh_counts(const sfem_amd_mesh *m, hipStream_t s) {
+int mesh_counts(const sfem_amd_mesh *m, hipStream_t s) {
     if (m->count.p) return SFEM_SUCCESS;
     if (m->count.alloc(m->nnodes)) return SFEM_FAILURE;
     return node_incidence(m->E * m->nv, m->nnodes, m->idx.p, m->count.p, s);
+}
+int mesh_skeleton(const sfem_amd_mesh *m) {
+    if (m->sk_n >= 0) return SFEM_SUCCESS;
+    const int L = m->level, N1 = L + 1, nsk = sk_count(L);
+    const ptrdiff_t E = m->E;
+    if ((double)E * nsk >= 2147483647.0) {
+        set_last_error("sshex8 skeleton map: E * skeleton nodes per element exceeds int32");
+        return SFEM_FAILURE;
+    }
+    std::vector<int32_t> el((size_t)E * m->nv);
+    if (!el.empty())
+        SFEM_AMD_HIP_CHECK(hipMemcpy(el.data(), m->idx.p, sizeof(int32_t) * el.size(), hipMemcpyDeviceToHost));
+    // skeleton lidx in slot order
+    std::vector<int> sk_lidx(nsk);
+    for (int z = 0; z <= L; z++)
+        for (int y = 0; y <= L; y++)
+            for (int x = 0; x <= L; x++) {
+                const int q = sk_slot(L, x, y, z);
+                if (q >= 0) sk_lidx[q] = lidx(L, x, y, z);
+            }
+    (void)N1;
+    std::vector<int32_t> cnt(m->nnodes + 1, 0);
+    for (ptrdiff_t e = 0; e < E; e++)
+        for (int q = 0; q < nsk; q++) cnt[el[(size_t)e * m->nv + sk_lidx[q]]]++;
+    std::vector<int32_t> node, off(1, 0), pos(m->nnodes, -1);
+    for (ptrdiff_t g = 0; g < m->nnodes; g++)
+        if (cnt[g]) {
+            pos[g] = (int32_t)node.size();
+            node.push_back((int32_t)g);
+            off.push_back(off.back() + cnt[g]);
+        }
+    std::vector<int32_t> fill(node.size(), 0), ent((size_t)E * nsk);
+    for (ptrdiff_t e = 0; e < E; e++)
+        for (int q = 0; q < nsk; q++) {
+            const int32_t p = pos[el[(size_t)e * m->nv + sk_lidx[q]]];
+            ent[(size_t)off[p] + fill[p]++] = (int32_t)(e * nsk + q);
+        }
+    if (m->sk_node.upload(node.data(), node.size()) || m->sk_off.upload(off.data(), off.size()) ||
+        m->sk_ent.upload(ent.data(), ent.size()))
+        return SFEM_FAILURE;
+    m->sk_n = (ptrdiff_t)node.size();
+    return SFEM_SUCCESS;
+}
+static SkeletonMap skeleton_of(const sfem_amd_mesh *m) {
+    SkeletonMap sk;
+    sk.node = m->sk_node.p;
+    sk.off = m->sk_off.p;
+    sk.ent = m->sk_ent.p;
+    sk.n = m->sk_n;
+    return sk;
 }
 static int mesh_owner(const sfem_amd_mesh *m, hipStream_t s) {
     if (m->own.p) return SFEM_SUCCESS;
@@ -126,12 +95,16 @@ struct LaplacianOp final : OpBase {
     DevArray<float> fff32;      // SoA [6][E]
     DevArray<uint16_t> fff16;   // SoA [6][E] bf16 (FP32 path, coefficient folded in)
     DevArray<float> coeff;
+    DevArray<T> part;  // sshex8: skeleton contributions [E][sk_count(L)]
     int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
     ptrdiff_t rows() const override { return m->nnodes; }
 
     int apply(const void *xv, void *yv, hipStream_t s) override {
         const T *x = (const T *)xv;
         T *y = (T *)yv;
+        if (m->type == SFEM_AMD_SSHEX8 && !part.p && m->E > 0) {
+            if (mesh_skeleton(m) || part.alloc((size_t)m->E * sk_count(m->level))) return SFEM_FAILURE;
+        }
         if constexpr (std::is_same<T, double>::value) {
             switch (m->type) {
                 case SFEM_AMD_TET4:
@@ -140,11 +113,11 @@ struct LaplacianOp final : OpBase {
                     return hex8_laplacian_apply(m->E, m->idx.p, fff32.p, coeff.p, x, y, s);
                 default:
                     return sshex8_laplacian_apply<double, float>(m->level, m->E, m->idx.p, fff32.p, m->E, coeff.p, x,
-                                                                 y, s);
+                                                                 y, part.p, skeleton_of(m), s);
             }
         } else {
             return sshex8_laplacian_apply<float, uint16_t>(m->level, m->E, m->idx.p, fff16.p, m->E, nullptr, x, y,
-                                                           s);
+                                                           part.p, skeleton_of(m), s);
         }
     }
     int diag(void *dv, hipStream_t s) override {
@@ -385,6 +358,32 @@ static sfem_amd_op *wrap(std::shared_ptr<OpBase> o) {
     h->impl = std::move(o);
     return h;
 }
+
+namespace sfem_amd {
+sfem_amd_op *wrap_op(std::shared_ptr<OpBase> o) { return wrap(std::move(o)); }
+int constrained_set_value(const OpBase *constrained, int rt, double v, void *x, hipStream_t s) {
+    auto *c = dynamic_cast<const ConstrainedOp *>(constrained);
+    if (!c) {
+        set_last_error("not a constrained operator");
+        return SFEM_FAILURE;
+    }
+    return c->set_value_rt(rt, v, x, s);
+}
+bool is_constrained(const OpBase *op) { return dynamic_cast<const ConstrainedOp *>(op) != nullptr; }
+}  // namespace sfem_amd
+
+namespace {
+// Runs one of two operators of the same shape; the CG fallback hook selects the second one
+// when the primary (e.g. a learned smoother) stops reducing the residual.
+struct SwitchOp final : OpBase {
+    std::shared_ptr<OpBase> ops[2];
+    int active{0};
+    int events{0};
+    int real_type() const override { return ops[0]->real_type(); }
+    ptrdiff_t rows() const override { return ops[0]->rows(); }
+    int apply(const void *x, void *y, hipStream_t s) override { return ops[active]->apply(x, y, s); }
+};
+}  // namespace
 
 #define CHECK_HANDLE(h)                                  \
     do {                                                 \
@@ -1074,6 +1073,73 @@ int sfem_amd_op_benchmark(sfem_amd_op *op, const void *x, void *y, int warmup, i
     for (auto &e : ev) (void)hipEventDestroy(e);
     *total_ms = total;
     return err;
+}
+
+int sfem_amd_switch_create(sfem_amd_op *primary, sfem_amd_op *fallback, sfem_amd_op **out) {
+    CHECK_HANDLE(primary);
+    CHECK_HANDLE(fallback);
+    SFEM_AMD_REQUIRE(out, "switch_create: out");
+    SFEM_AMD_REQUIRE(primary->impl->rows() == fallback->impl->rows() &&
+                         primary->impl->real_type() == fallback->impl->real_type(),
+                     "switch_create: operators differ in size or real type");
+    auto sw = std::make_shared<SwitchOp>();
+    sw->ops[0] = primary->impl;
+    sw->ops[1] = fallback->impl;
+    *out = wrap(sw);
+    return SFEM_SUCCESS;
+}
+
+int sfem_amd_switch_select(sfem_amd_op *op, int which) {
+    CHECK_HANDLE(op);
+    auto *sw = dynamic_cast<SwitchOp *>(op->impl.get());
+    SFEM_AMD_REQUIRE(sw && (which == 0 || which == 1), "switch_select: not a switch operator or index not 0/1");
+    sw->active = which;
+    return SFEM_SUCCESS;
+}
+
+int sfem_amd_switch_state(const sfem_amd_op *op, int *active, int *events) {
+    CHECK_HANDLE(op);
+    auto *sw = dynamic_cast<const SwitchOp *>(op->impl.get());
+    SFEM_AMD_REQUIRE(sw, "switch_state: not a switch operator");
+    if (active) *active = sw->active;
+    if (events) *events = sw->events;
+    return SFEM_SUCCESS;
+}
+
+int sfem_amd_cg_add_fallback(sfem_amd_op *cg, sfem_amd_op *sw, double max_ratio) {
+    CHECK_HANDLE(cg);
+    CHECK_HANDLE(sw);
+    auto swp = std::dynamic_pointer_cast<SwitchOp>(sw->impl);
+    SFEM_AMD_REQUIRE(swp, "cg_add_fallback: not a switch operator");
+    auto hook = [swp, max_ratio](int, double prev, double now) {
+        if (swp->active == 0 && prev > 0 && now > max_ratio * prev) {
+            swp->active = 1;
+            swp->events++;
+        }
+    };
+    auto chain = [hook](std::function<void(int, double, double)> &slot) {
+        auto old = slot;
+        slot = [old, hook](int it, double a, double b) {
+            if (old) old(it, a, b);
+            hook(it, a, b);
+        };
+    };
+    if (auto *s = dynamic_cast<SolverOp<double> *>(cg->impl.get())) {
+        auto *c = dynamic_cast<sfem::ConjugateGradient<double> *>(s->op.get());
+        SFEM_AMD_REQUIRE(c && c->flexible, "cg_add_fallback: needs a flexible CG (mode 1)");
+        chain(c->on_iteration);
+        s->keep.push_back(swp);
+        return SFEM_SUCCESS;
+    }
+    if (auto *s = dynamic_cast<SolverOp<float> *>(cg->impl.get())) {
+        auto *c = dynamic_cast<sfem::ConjugateGradient<float> *>(s->op.get());
+        SFEM_AMD_REQUIRE(c && c->flexible, "cg_add_fallback: needs a flexible CG (mode 1)");
+        chain(c->on_iteration);
+        s->keep.push_back(swp);
+        return SFEM_SUCCESS;
+    }
+    set_last_error("cg_add_fallback: not a CG solver");
+    return SFEM_FAILURE;
 }
 
 }  // extern "C"

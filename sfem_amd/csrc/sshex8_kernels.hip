@@ -69,15 +69,18 @@ __device__ __forceinline__ void lap_block_matrices(ptrdiff_t E, ptrdiff_t e0, co
 // node's stencil is split into S_a (micro elements whose lower x-corner is the node) and
 // S_b (upper x-corner), each masked by which y/z micro elements exist inside the macro
 // element, so x = 0 and x = L (skeleton) need no divergent code: the whole wave is at
-// the same x.  Interior nodes are owned by one macro element (plain read-modify-write);
-// skeleton nodes are shared and use FP64/FP32 atomics.
+// the same x.  Interior nodes are owned by one macro element: their y values are loaded
+// together with x (before the barrier), so the sweep only stores.
+// Skeleton nodes are shared: their element contributions go to the element's slots of
+// `part` (plain stores, no atomics) and skeleton_gather_kernel sums them per node in a
+// fixed order, so the result is bitwise reproducible.
 template <typename T, typename G, int L>
 __global__ void __launch_bounds__(LapShape<L>::BLOCK)
     sshex8_laplacian_apply_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx, const G *__restrict__ fff,
                                   const ptrdiff_t fstride, const float *__restrict__ coeff,
-                                  const T *__restrict__ x, T *__restrict__ y) {
+                                  const T *__restrict__ x, T *__restrict__ y, T *__restrict__ part) {
     using S = LapShape<L>;
-    constexpr int N1 = S::N1, NXE = S::NXE, LINES = S::LINES, EPB = S::EPB;
+    constexpr int N1 = S::N1, NXE = S::NXE, LINES = S::LINES, EPB = S::EPB, NSK = sk_count(L);
     __shared__ T su[EPB][NXE];
     __shared__ double sK[EPB][64];
 
@@ -89,13 +92,18 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
     const ptrdiff_t e = e0 + le;
     const bool active = (le < EPB) && (e < E);
 
+    const bool skl = sk_line(L, yy, zz);
     int32_t gid[N1];
+    T old[N1];  // y of the interior nodes of an interior line, loaded early
     if (active) {
         const int32_t *row = idx + e * NXE + lidx(L, 0, yy, zz);
 #pragma unroll
-        for (int i = 0; i < N1; i++) {
-            gid[i] = row[i];
-            su[le][lidx(L, i, yy, zz)] = x[gid[i]];
+        for (int i = 0; i < N1; i++) gid[i] = row[i];
+#pragma unroll
+        for (int i = 0; i < N1; i++) su[le][lidx(L, i, yy, zz)] = x[gid[i]];
+        if (!skl) {
+#pragma unroll
+            for (int i = 1; i < L; i++) old[i] = y[gid[i]];
         }
     }
     lap_block_matrices<L, EPB, S::BLOCK, G>(E, e0, fff, fstride, coeff, sK);
@@ -149,28 +157,31 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
             w[1][a][b] = u[lidx(L, 0, ys[b], zs[a])];
             w[2][a][b] = u[lidx(L, 1, ys[b], zs[a])];
         }
-    const bool interior_line = (yy > 0 && yy < L && zz > 0 && zz < L);
 
+    T *pe = part + e * NSK + sk_line_base(L, yy, zz);
 #pragma unroll
     for (int i = 0; i <= L; i++) {
-        T acc = 0;
+        T v = 0;
         if (i < L) {
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
-                for (int b = 0; b < 3; b++) acc += Sa[a][b][0] * w[1][a][b] + Sa[a][b][1] * w[2][a][b];
+                for (int b = 0; b < 3; b++) v += Sa[a][b][0] * w[1][a][b] + Sa[a][b][1] * w[2][a][b];
         }
         if (i > 0) {
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
-                for (int b = 0; b < 3; b++) acc += Sb[a][b][0] * w[0][a][b] + Sb[a][b][1] * w[1][a][b];
+                for (int b = 0; b < 3; b++) v += Sb[a][b][0] * w[0][a][b] + Sb[a][b][1] * w[1][a][b];
         }
-        if (interior_line && i > 0 && i < L) {
-            y[gid[i]] += acc;
-        } else {
-            atomic_add(&y[gid[i]], acc);
-        }
+        if (skl)
+            pe[i] = v;
+        else if (i == 0)
+            pe[0] = v;
+        else if (i == L)
+            pe[1] = v;
+        else
+            y[gid[i]] = old[i] + v;
         // slide
         if (i < L) {
 #pragma unroll
@@ -182,6 +193,29 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
                     w[2][a][b] = (i + 2 <= L) ? u[lidx(L, i + 2 <= L ? i + 2 : L, ys[b], zs[a])] : T(0);
                 }
         }
+    }
+
+}
+
+// y[node[t]] += sum of part[ent[off[t] .. off[t+1])] * bs components: the skeleton nodes'
+// element contributions, in slot order.
+template <typename T, int BS>
+__global__ void skeleton_gather_kernel(const ptrdiff_t n, const int32_t *__restrict__ node,
+                                       const int32_t *__restrict__ off, const int32_t *__restrict__ ent,
+                                       const T *__restrict__ part, T *__restrict__ y) {
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const int j0 = off[t], j1 = off[t + 1];
+        T acc[BS];
+#pragma unroll
+        for (int c = 0; c < BS; c++) acc[c] = 0;
+        for (int j = j0; j < j1; j++) {
+            const ptrdiff_t q = (ptrdiff_t)ent[j] * BS;
+#pragma unroll
+            for (int c = 0; c < BS; c++) acc[c] += part[q + c];
+        }
+        const ptrdiff_t g = (ptrdiff_t)node[t] * BS;
+#pragma unroll
+        for (int c = 0; c < BS; c++) y[g + c] += acc[c];
     }
 }
 
@@ -378,17 +412,32 @@ bool sshex8_level_supported(int L) {
     }
 }
 
+template <typename T>
+int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStream_t s) {
+    if (sk.n == 0) return SFEM_SUCCESS;
+    if (bs == 1)
+        skeleton_gather_kernel<T, 1><<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, sk.ent, part, y);
+    else if (bs == 3)
+        skeleton_gather_kernel<T, 3><<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, sk.ent, part, y);
+    else {
+        set_last_error("skeleton_gather: block size must be 1 or 3");
+        return SFEM_FAILURE;
+    }
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
 template <typename T, typename G>
 int sshex8_laplacian_apply(int L, ptrdiff_t E, const int32_t *idx, const G *fff, ptrdiff_t fstride,
-                           const float *coeff, const T *x, T *y, hipStream_t s) {
+                           const float *coeff, const T *x, T *y, T *part, const SkeletonMap &sk, hipStream_t s) {
     if (E == 0) return SFEM_SUCCESS;
     switch (L) {
-#define CASE(LV)                                                                                            \
-    case LV: {                                                                                              \
-        using Sh = LapShape<LV>;                                                                            \
-        const unsigned grid = (unsigned)((E + Sh::EPB - 1) / Sh::EPB);                                      \
-        sshex8_laplacian_apply_kernel<T, G, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, fff, fstride, coeff, x, y); \
-        break;                                                                                              \
+#define CASE(LV)                                                                                                  \
+    case LV: {                                                                                                    \
+        using Sh = LapShape<LV>;                                                                                  \
+        const unsigned grid = (unsigned)((E + Sh::EPB - 1) / Sh::EPB);                                            \
+        sshex8_laplacian_apply_kernel<T, G, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, fff, fstride, coeff, x, y, part); \
+        break;                                                                                                    \
     }
         SFEM_AMD_SSHEX8_LEVELS(CASE)
 #undef CASE
@@ -397,7 +446,7 @@ int sshex8_laplacian_apply(int L, ptrdiff_t E, const int32_t *idx, const G *fff,
             return SFEM_FAILURE;
     }
     SFEM_AMD_HIP_CHECK(hipGetLastError());
-    return SFEM_SUCCESS;
+    return skeleton_gather<T>(sk, 1, part, y, s);
 }
 
 template <typename T, typename G>
@@ -475,10 +524,14 @@ int sshex8_elasticity_diag(int L, ptrdiff_t E, const int32_t *idx, const double 
     return SFEM_SUCCESS;
 }
 
+template int skeleton_gather<double>(const SkeletonMap &, int, const double *, double *, hipStream_t);
+template int skeleton_gather<float>(const SkeletonMap &, int, const float *, float *, hipStream_t);
 template int sshex8_laplacian_apply<double, float>(int, ptrdiff_t, const int32_t *, const float *, ptrdiff_t,
-                                                   const float *, const double *, double *, hipStream_t);
+                                                   const float *, const double *, double *, double *,
+                                                   const SkeletonMap &, hipStream_t);
 template int sshex8_laplacian_apply<float, uint16_t>(int, ptrdiff_t, const int32_t *, const uint16_t *, ptrdiff_t,
-                                                     const float *, const float *, float *, hipStream_t);
+                                                     const float *, const float *, float *, float *,
+                                                     const SkeletonMap &, hipStream_t);
 template int sshex8_laplacian_diag<double, float>(int, ptrdiff_t, const int32_t *, const float *, ptrdiff_t,
                                                   const float *, double *, hipStream_t);
 template int sshex8_laplacian_diag<float, uint16_t>(int, ptrdiff_t, const int32_t *, const uint16_t *, ptrdiff_t,

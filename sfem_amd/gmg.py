@@ -24,7 +24,7 @@ import numpy as np
 from . import FLOAT32, FLOAT64, HEX8, SSHEX8
 from . import (DeviceArray, Mesh, Multigrid, adjugate_det, box_mesh, cg, chain, cheb3, constrained, fff,
                inverse_diagonal, jacobi, laplacian, linear_elasticity, permutation, precision_bridge, prolongation,
-               restriction, set_constrained_value, solver_bridge, sshex8_refine)
+               restriction, set_constrained_value, solver_bridge, sshex8_refine, switch, switch_select)
 
 
 def derefinement_levels(L):
@@ -57,8 +57,15 @@ class SSGMG:
 
     def __init__(self, fine_mesh, kind, geometry, bc_nodes, coeff=None, real_type=FLOAT64, smoother="cheb3",
                  cheb_degree=3, eig_its=10, eig_tol=1e-5, scale_eig_max=1.02, coarse_rtol=1e-8,
-                 coarse_max_it=10000, mu=None, lam=None, bc_components=None, max_it=1, atol=0.0, macro_box=None):
+                 coarse_max_it=10000, mu=None, lam=None, bc_components=None, max_it=1, atol=0.0, macro_box=None,
+                 learned=None, learned_levels=1, learned_steps=1, learned_mode=1, learned_damping=1.0):
         self.kind = kind
+        # learned smoother (chosen form 4): a CNN replacing Chebyshev3 on the `learned_levels`
+        # finest sshex8 levels (micro-grid level <= 8), each wrapped in a switch whose fallback
+        # is the Chebyshev3 smoother (see sfem_amd.learned)
+        self._learned = dict(cnn=learned, levels=learned_levels, steps=learned_steps, mode=learned_mode,
+                             damping=learned_damping)
+        self.switches = []
         bs = self.block_size = 3 if kind == "elasticity" else 1
         self.real_type = real_type
         self._smoother_cfg = dict(kind=smoother, degree=cheb_degree, eig_its=eig_its, eig_tol=eig_tol,
@@ -159,8 +166,17 @@ class SSGMG:
             prev = lv
 
     def _finish(self, coarse_rtol, coarse_max_it, max_it, atol):
-        for lv in self.levels[:-1]:
+        cfg = self._learned
+        for i, lv in enumerate(self.levels[:-1]):
             lv.smoother = self._make_smoother(lv)
+            if (cfg["cnn"] is not None and i < cfg["levels"] and lv.mesh.element_type == SSHEX8
+                    and lv.mesh.level <= 8):
+                from .learned import make_learned_smoother
+                lv.cheb = lv.smoother
+                lv.learned = make_learned_smoother(lv, cfg["cnn"], steps=cfg["steps"], damping=cfg["damping"],
+                                                   mode=cfg["mode"], block_size=self.block_size)
+                lv.smoother = switch(lv.learned, lv.cheb)
+                self.switches.append(lv.smoother)
         c = self.levels[-1]
         d = DeviceArray(c.n, np.float64)
         c.A.diag(d)
@@ -180,3 +196,8 @@ class SSGMG:
     @property
     def fine(self):
         return self.levels[0]
+
+    def select_smoothers(self, which):
+        """0: learned smoothers (where configured), 1: their Chebyshev3 fallbacks."""
+        for sw in self.switches:
+            switch_select(sw, which)
