@@ -58,8 +58,15 @@ def cpu_baseline(el, fff, N, L, budget_s=12.0, max_reps=20):
                       f"OpenMP restatement of affine_sshex8_laplacian_stencil_apply_fff"}
 
 
-def solve_times(S, mesh, fff, pts, n_macro, rtol=1e-10):
-    from sfem_amd import gmg
+def solve_times(S, mesh, fff, pts, n_macro, rtol=1e-10, learned_steps=2):
+    """Time to ||b - A x||/||b|| <= rtol (recomputed in FP64 with the FP64 operator) of
+      fp64:          FP64 MG-PCG (Chebyshev3 V-cycle, FP64 coarse PCG);
+      mixed:         the same V-cycle in FP32 with BF16 geometry inside FP64 flexible CG;
+      mixed_learned: as mixed, with the learned CNN smoother on the finest level (trained at
+                     setup on V-cycle pairs; Chebyshev3 fallback when an iteration reduces the
+                     residual by less than 2x).
+    Every timed solve starts from x = 0 after one untimed warm-up solve."""
+    from sfem_amd import gmg, learned
     N = pts.shape[1]
     bc = np.where((pts.min(0) < 1e-6) | (pts.max(0) > 1 - 1e-6))[0].astype(np.int32)
     rhs = np.random.default_rng(1).uniform(0, 1, N)
@@ -75,19 +82,43 @@ def solve_times(S, mesh, fff, pts, n_macro, rtol=1e-10):
     t0 = time.perf_counter()
     h64 = gmg.SSGMG(mesh, "laplacian", fff, bc, macro_box=(n_macro, 1.0))
     setup64 = time.perf_counter() - t0
-    for name in ("fp64", "mixed"):
+    hierarchy = "V-cycle 8-4-2-1 | macro 16^3 -> 8-4-2-1 super levels, Chebyshev3 degree 3"
+    for name in ("fp64", "mixed", "mixed_learned"):
+        switches = []
+        extra = {}
         if name == "fp64":
             solver = S.cg(h64.fine.A, h64.mg, rtol=rtol, max_it=500, stop_on_residual=True)
             setup = setup64
-        else:
+            desc = hierarchy + ", FP64; FP64 coarse PCG; PCG stopping on ||r||/||b||"
+        elif name == "mixed":
             t0 = time.perf_counter()
             h32 = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, macro_box=(n_macro, 1.0))
             P = S.precision_bridge(h32.mg)
             setup = time.perf_counter() - t0
             solver = S.cg(h64.fine.A, P, flexible=True, rtol=rtol, max_it=500)
+            desc = hierarchy + " in FP32 with BF16 geometry (hex8 levels bridged to FP64); outer FP64 flexible CG"
+        else:
+            t0 = time.perf_counter()
+            data = learned.collect_pairs(h32, n_problems=4, cycles=2, seed=0)
+            cnn = S.CNN(16, 4, seed=0)
+            hist = learned.train(cnn, data, steps=300, lr=2e-3, batch_elements=16, seed=0)
+            t_train = time.perf_counter() - t0
+            hl = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, macro_box=(n_macro, 1.0), learned=cnn,
+                           learned_steps=learned_steps)
+            switches = hl.switches
+            P = S.precision_bridge(hl.mg)
+            setup = time.perf_counter() - t0
+            solver = learned.flexible_cg_with_fallback(h64.fine.A, P, switches, rtol=rtol, max_it=500)
+            extra = {"training_seconds": t_train, "training_loss_last20": float(np.mean(hist[-20:])),
+                     "learned_steps_per_smoothing": learned_steps}
+            desc = ("mixed V-cycle with the finest-level Chebyshev3 replaced by the learned CNN smoother "
+                    "(BF16 MFMA inference); Chebyshev3 fallback at per-iteration reduction > 0.5")
         # one untimed solve to warm up code paths, then the timed one from x = 0
         x = S.DeviceArray(N)
         solver.apply(b, x)
+        for sw in switches:
+            S.switch_select(sw, 0)
+        ev0 = sum(S.switch_state(sw)[1] for sw in switches)
         x.zero()
         S.synchronize()
         t0 = time.perf_counter()
@@ -96,11 +127,11 @@ def solve_times(S, mesh, fff, pts, n_macro, rtol=1e-10):
         t = time.perf_counter() - t0
         out[name] = {"seconds": t, "iterations": solver.iterations, "status": solver.last_status,
                      "true_rel_residual": true_res(h64.fine.A, x), "setup_seconds": setup,
-                     "preconditioner": ("V-cycle 8-4-2-1 | macro 16^3 -> 8-4-2-1 super levels, Chebyshev3, FP64; "
-                                        "FP64 coarse PCG" if name == "fp64" else
-                                        "same V-cycle in FP32 with BF16 geometry (hex8 levels bridged to FP64), "
-                                        "FP64 coarse PCG; outer FP64 flexible CG")}
+                     "preconditioner": desc, **extra}
+        if switches:
+            out[name]["fallback_events"] = int(sum(S.switch_state(sw)[1] for sw in switches) - ev0)
     out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    out["speedup_mixed_learned_vs_fp64"] = out["fp64"]["seconds"] / out["mixed_learned"]["seconds"]
     return out
 
 

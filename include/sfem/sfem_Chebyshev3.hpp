@@ -1,5 +1,8 @@
 // sfem::Chebyshev3<T> (algebra/sfem_Chebyshev3.hpp:57-303): three-term Chebyshev
 // smoother over [scale_eig_min*lmax, lmax], lmax = scale_eig_max * eig_max.
+// Extension (not in the reference): with `inv_diag` set before init(), the recurrence and the
+// power method run on D^-1 A (Jacobi-scaled Chebyshev), for coefficients that vary between
+// elements by orders of magnitude.
 #pragma once
 
 #include "sfem_Buffer.hpp"
@@ -15,6 +18,7 @@ public:
     std::shared_ptr<BLAS<T>> blas;
     std::shared_ptr<PowerMethod<T>> power_method;
     SharedBuffer<T> p_, temp_;
+    SharedBuffer<T> inv_diag;  // optional D^-1 (device)
 
     T eigen_solver_tol{1e-6};
     int eigen_solver_max_it{1000};
@@ -57,7 +61,10 @@ public:
         T *ev = blas->allocate(n_dofs);
         T *work = blas->allocate(n_dofs);
         blas->copy(n_dofs, guess, ev);
-        auto apply_fn = [this](const T *const x, T *const y) { op->apply(x, y); };
+        auto apply_fn = [this](const T *const x, T *const y) {
+            op->apply(x, y);
+            if (inv_diag) blas->xypaz(n_dofs, inv_diag->data(), y, T(0), y);
+        };
         eig_max = power_method->max_eigen_value(apply_fn, eigen_solver_max_it, eigen_solver_tol, n_dofs, ev, work);
         auto b = blas;
         p_ = Buffer<T>::own(n_dofs, work, [b](void *q) { b->destroy(q); });
@@ -84,6 +91,7 @@ public:
             op->apply(x, p);
             blas->axpy(n, T(-1), rhs, p);
         }
+        if (inv_diag) blas->xypaz(n, inv_diag->data(), p, T(0), p);
         blas->axpy(n, -alpha, p, x);
         iterations_ = 1;
         for (; iterations_ < max_it; iterations_++) {
@@ -92,8 +100,12 @@ public:
             alpha = T(1) / (avg - beta / alpha);
             if (!apply_overwrites_output) blas->zeros(n, temp);
             op->apply(x, temp);
-            // fused: p = beta p - rhs + A x; x -= alpha p
-            sfem_amd_cheb3_update(real_type_of<T>::value, n, alpha, beta, rhs, temp, p, x, stream);
+            // fused: p = beta p - rhs + A x (times D^-1 when scaled); x -= alpha p
+            if (inv_diag)
+                sfem_amd_cheb3_update_diag(real_type_of<T>::value, n, alpha, beta, rhs, temp, inv_diag->data(), p, x,
+                                           stream);
+            else
+                sfem_amd_cheb3_update(real_type_of<T>::value, n, alpha, beta, rhs, temp, p, x, stream);
         }
         return SFEM_SUCCESS;
     }

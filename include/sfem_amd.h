@@ -193,6 +193,9 @@ double sfem_amd_cg_update_x_r(int rt, ptrdiff_t n, double alpha, const void *p, 
 /* Chebyshev3 iteration_update (algebra/sfem_Chebyshev3.hpp:37-52): p = beta p - rhs + Ax; x -= alpha p */
 int sfem_amd_cheb3_update(int rt, ptrdiff_t n, double alpha, double beta, const void *rhs, const void *Ax, void *p,
                           void *x, void *stream);
+/* p = beta p + inv_diag (Ax - rhs); x -= alpha p (the Jacobi-scaled Chebyshev3 step). */
+int sfem_amd_cheb3_update_diag(int rt, ptrdiff_t n, double alpha, double beta, const void *rhs, const void *Ax,
+                               const void *inv_diag, void *p, void *x, void *stream);
 
 /* ------------------------------------------------------------------ solvers as operators */
 /* sfem::ConjugateGradient<real_t> (algebra/sfem_cg.hpp:80-446): plain CG, or PCG when
@@ -212,6 +215,12 @@ int sfem_amd_cheb3_create(sfem_amd_op *A, int max_it, double scale_eig_max, doub
                           int eigen_solver_max_it, double eigen_solver_tol, int initial_guess_zero,
                           const void *init_vector, sfem_amd_op **out);
 double sfem_amd_cheb3_eig_max(const sfem_amd_op *cheb);
+/* Chebyshev3 on D^-1 A (extension, not in the reference): as sfem_amd_cheb3_create with the
+ * residual scaled by inv_diag (device, n entries, copied) in the recurrence and in the power
+ * method.  For coefficients varying by orders of magnitude between elements. */
+int sfem_amd_cheb3_diag_create(sfem_amd_op *A, const void *inv_diag, int max_it, double scale_eig_max,
+                               double scale_eig_min, int eigen_solver_max_it, double eigen_solver_tol,
+                               int initial_guess_zero, const void *init_vector, sfem_amd_op **out);
 /* StationaryIteration with (scaled) Jacobi: x += omega D^-1 (b - A x), `steps` sweeps
  * (algebra/sfem_Stationary.hpp:50-80, algebra/sfem_ShiftableJacobi.hpp).  diag: device. */
 int sfem_amd_jacobi_create(sfem_amd_op *A, const void *diag, double omega, int steps,

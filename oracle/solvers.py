@@ -98,11 +98,17 @@ class Chebyshev3:
     """Chebyshev3::precond_apply (algebra/sfem_Chebyshev3.hpp:183-242)."""
 
     def __init__(self, A, n, max_it=3, scale_eig_max=1.0, scale_eig_min=0.06, eig_max_it=1000, eig_tol=1e-6,
-                 initial_guess_zero=False, eig_max=None):
+                 initial_guess_zero=False, eig_max=None, inv_diag=None, init=None):
+        """inv_diag: run on D^-1 A (the build's Jacobi-scaled extension; the reference has none)."""
         self.A, self.n, self.max_it = A, n, max_it
         self.smax, self.smin = scale_eig_max, scale_eig_min
         self.zero = initial_guess_zero
-        self.eig_max = power_method(A, n, eig_max_it, eig_tol) if eig_max is None else eig_max
+        self.dinv = inv_diag
+        def scaled(v, y):
+            y += inv_diag * _apply0(A, v)
+
+        Aop = A if inv_diag is None else scaled
+        self.eig_max = power_method(Aop, n, eig_max_it, eig_tol, v0=init) if eig_max is None else eig_max
 
     def __call__(self, b, x):
         """x := smooth(b, x) in place."""
@@ -110,13 +116,14 @@ class Chebyshev3:
         lmin = self.smin * lmax
         avg, diff = (lmin + lmax) / 2, (lmin - lmax) / 2
         alpha = 1.0 / avg
-        p = -b.copy() if self.zero else _apply0(self.A, x) - b
+        d = 1.0 if self.dinv is None else self.dinv
+        p = d * (-b.copy() if self.zero else _apply0(self.A, x) - b)
         x -= alpha * p
         for it in range(1, self.max_it):
             dea = diff * alpha
             beta = (0.5 if it == 1 else 0.25) * dea * dea
             alpha = 1.0 / (avg - beta / alpha)
-            p = beta * p - b + _apply0(self.A, x)
+            p = beta * p + d * (_apply0(self.A, x) - b)
             x -= alpha * p
         return x
 

@@ -70,6 +70,8 @@ SIGNATURES = {
     "sfem_amd_blas_cast": (_i, [_i, _pd, _vp, _i, _vp, _vp]),
     "sfem_amd_cg_update_x_r": (_d, [_i, _pd, _d, _vp, _vp, _vp, _vp, _i, _vp]),
     "sfem_amd_cheb3_update": (_i, [_i, _pd, _d, _d, _vp, _vp, _vp, _vp, _vp]),
+    "sfem_amd_cheb3_update_diag": (_i, [_i, _pd, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sfem_amd_cheb3_diag_create": (_i, [_vp, _vp, _i, _d, _d, _i, _d, _i, _vp, C.POINTER(_vp)]),
     "sfem_amd_cg_create": (_i, [_vp, _vp, _i, _d, _d, _i, C.POINTER(_vp)]),
     "sfem_amd_cheb3_create": (_i, [_vp, _i, _d, _d, _i, _d, _i, _vp, C.POINTER(_vp)]),
     "sfem_amd_cheb3_eig_max": (_d, [_vp]),
@@ -413,7 +415,11 @@ def cg(A, precond=None, flexible=False, rtol=1e-10, atol=1e-16, max_it=10000, st
 
 
 def cheb3(A, max_it=3, scale_eig_max=1.0, scale_eig_min=0.06, eigen_solver_max_it=1000, eigen_solver_tol=1e-6,
-          initial_guess_zero=False, init_vector=None):
+          initial_guess_zero=False, init_vector=None, inv_diag=None):
+    """Chebyshev3 smoother; with inv_diag (device D^-1) it runs on D^-1 A (an extension)."""
+    if inv_diag is not None:
+        return _new_op("sfem_amd_cheb3_diag_create", A.h, _dptr(inv_diag), max_it, scale_eig_max, scale_eig_min,
+                       eigen_solver_max_it, eigen_solver_tol, int(initial_guess_zero), _dptr(init_vector), keep=[A])
     return _new_op("sfem_amd_cheb3_create", A.h, max_it, scale_eig_max, scale_eig_min, eigen_solver_max_it,
                    eigen_solver_tol, int(initial_guess_zero), _dptr(init_vector), keep=[A])
 

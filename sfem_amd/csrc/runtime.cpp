@@ -143,6 +143,8 @@ struct ElasticityOp final : OpBase {
     const sfem_amd_mesh *m{nullptr};
     DevArray<float> adj, det, coeff;  // hex8: SoA [9][E], [E]
     DevArray<double> K;               // sshex8: per-macro 24x24 matrices
+    DevArray<float> K32;              // sshex8, FP32 operator: K rounded to float
+    DevArray<T> part;                 // sshex8: skeleton contributions [E][sk_count(L)][3]
     double mu{0}, lambda{0};
     int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
     ptrdiff_t rows() const override { return m->nnodes * 3; }
@@ -154,7 +156,20 @@ struct ElasticityOp final : OpBase {
             set_last_error("hex8 elasticity: FP64 only");
             return SFEM_FAILURE;
         }
-        return sshex8_elasticity_apply<T>(m->level, m->E, m->idx.p, K.p, (const T *)xv, (T *)yv, s);
+        if (!part.p && m->E > 0) {
+            if (mesh_skeleton(m) || part.alloc((size_t)m->E * sk_count(m->level) * 3)) return SFEM_FAILURE;
+        }
+        const T *Kt;
+        if constexpr (std::is_same<T, double>::value) {
+            Kt = K.p;
+        } else {
+            if (!K32.p && K.n) {
+                if (K32.alloc(K.n) || blas_cast<double, float>((ptrdiff_t)K.n, K.p, K32.p, s)) return SFEM_FAILURE;
+            }
+            Kt = K32.p;
+        }
+        return sshex8_elasticity_apply<T>(m->level, m->E, m->idx.p, Kt, (const T *)xv, (T *)yv, part.p,
+                                          skeleton_of(m), s);
     }
     int diag(void *d, hipStream_t s) override {
         if (m->type == SFEM_AMD_HEX8) {
@@ -795,6 +810,15 @@ int sfem_amd_cheb3_update(int rt, ptrdiff_t n, double alpha, double beta, const 
                                     (float *)x, as_stream(s)));
 }
 
+int sfem_amd_cheb3_update_diag(int rt, ptrdiff_t n, double alpha, double beta, const void *rhs, const void *Ax,
+                               const void *inv_diag, void *p, void *x, void *s) {
+    if (rt == SFEM_AMD_FLOAT64)
+        return cheb3_update_diag<double>(n, alpha, beta, (const double *)rhs, (const double *)Ax,
+                                         (const double *)inv_diag, (double *)p, (double *)x, as_stream(s));
+    return cheb3_update_diag<float>(n, (float)alpha, (float)beta, (const float *)rhs, (const float *)Ax,
+                                    (const float *)inv_diag, (float *)p, (float *)x, as_stream(s));
+}
+
 // ----------------------------------------------------------------- solvers
 extern "C++" {
 template <typename T>
@@ -832,10 +856,16 @@ int sfem_amd_cg_create(sfem_amd_op *A, sfem_amd_op *P, int flexible, double rtol
 extern "C++" {
 template <typename T>
 static int cheb3_create_t(sfem_amd_op *A, int max_it, double smax, double smin, int eig_its, double eig_tol, int zero,
-                          const void *init, sfem_amd_op **out) {
+                          const void *init, sfem_amd_op **out, const void *inv_diag = nullptr) {
     auto c = std::make_shared<sfem::Chebyshev3<T>>();
     c->default_init();
     c->set_op(view<T>(A->impl));
+    if (inv_diag) {
+        const ptrdiff_t n = A->impl->rows();
+        auto b = c->blas;
+        c->inv_diag = sfem::Buffer<T>::own(n, b->allocate(n), [b](void *q) { b->destroy(q); });
+        b->copy(n, (const T *)inv_diag, c->inv_diag->data());
+    }
     c->max_it = max_it;
     c->scale_eig_max = (T)smax;
     c->scale_eig_min = (T)smin;
@@ -866,6 +896,15 @@ int sfem_amd_cheb3_create(sfem_amd_op *A, int max_it, double smax, double smin, 
     if (A->impl->real_type() == SFEM_AMD_FLOAT64)
         return cheb3_create_t<double>(A, max_it, smax, smin, eig_its, eig_tol, zero, init, out);
     return cheb3_create_t<float>(A, max_it, smax, smin, eig_its, eig_tol, zero, init, out);
+}
+
+int sfem_amd_cheb3_diag_create(sfem_amd_op *A, const void *inv_diag, int max_it, double smax, double smin,
+                               int eig_its, double eig_tol, int zero, const void *init, sfem_amd_op **out) {
+    CHECK_HANDLE(A);
+    SFEM_AMD_REQUIRE(inv_diag, "cheb3_diag_create: inv_diag");
+    if (A->impl->real_type() == SFEM_AMD_FLOAT64)
+        return cheb3_create_t<double>(A, max_it, smax, smin, eig_its, eig_tol, zero, init, out, inv_diag);
+    return cheb3_create_t<float>(A, max_it, smax, smin, eig_its, eig_tol, zero, init, out, inv_diag);
 }
 
 double sfem_amd_cheb3_eig_max(const sfem_amd_op *op) {

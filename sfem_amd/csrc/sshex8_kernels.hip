@@ -285,16 +285,20 @@ struct ElShape {
 // element (or EPB small ones); the element is swept plane by plane in z through a ring of
 // three LDS planes (2.5-D streaming, so L = 16 fits), one thread per node of the plane.
 // Each node sums over the (up to 8) micro elements that contain it, 3 rows x 24 columns
-// of the per-macro matrix each (K rows broadcast from LDS).
+// of the per-macro matrix each (K rows broadcast from LDS).  Interior nodes update y (their
+// y is loaded with the next plane of x); skeleton nodes store their three components to the
+// element's slots of `part`, summed per node by skeleton_gather_kernel (no atomics).
 template <typename T, int L>
 __global__ void __launch_bounds__(ElShape<L>::BLOCK)
     sshex8_elasticity_apply_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
-                                   const double *__restrict__ Kall, const T *__restrict__ x,
-                                   T *__restrict__ y) {
+                                   const T *__restrict__ Kall, const T *__restrict__ x,
+                                   T *__restrict__ y, T *__restrict__ part) {
     using S = ElShape<L>;
-    constexpr int N1 = S::N1, NXE = N1 * N1 * N1, PLANE = S::PLANE, EPB = S::EPB;
+    constexpr int N1 = S::N1, NXE = N1 * N1 * N1, PLANE = S::PLANE, EPB = S::EPB, NSK = sk_count(L);
     __shared__ T su[EPB][3][PLANE][3];
-    __shared__ T sK[EPB][576];
+    // one element per block: its matrix is block-uniform and is read straight from memory into
+    // scalar registers; several elements per block: staged in LDS
+    __shared__ T sK[EPB == 1 ? 1 : EPB][EPB == 1 ? 1 : 576];
 
     const int tid = threadIdx.x;
     const int le = tid / PLANE;
@@ -303,24 +307,36 @@ __global__ void __launch_bounds__(ElShape<L>::BLOCK)
     const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
     const ptrdiff_t e = e0 + le;
     const bool active = (le < EPB) && (e < E);
+    const bool edge_xy = xx == 0 || xx == L || yy == 0 || yy == L;
 
-    for (int k = tid; k < EPB * 576; k += S::BLOCK) {
-        const int l2 = k / 576;
-        const ptrdiff_t e2 = e0 + l2;
-        sK[l2][k - l2 * 576] = (e2 < E) ? (T)Kall[e2 * 576 + (k - l2 * 576)] : T(0);
+    if constexpr (EPB > 1) {
+        for (int k = tid; k < EPB * 576; k += S::BLOCK) {
+            const int l2 = k / 576;
+            const ptrdiff_t e2 = e0 + l2;
+            sK[l2][k - l2 * 576] = (e2 < E) ? Kall[e2 * 576 + (k - l2 * 576)] : T(0);
+        }
     }
     const int32_t *row = idx + (active ? e : 0) * NXE;
+    int32_t g_next = 0;
     auto load_plane = [&](int z) {
         if (active) {
-            const int32_t g = row[lidx(L, xx, yy, z)];
+            g_next = row[lidx(L, xx, yy, z)];
 #pragma unroll
-            for (int c = 0; c < 3; c++) su[le][z % 3][p][c] = x[(ptrdiff_t)g * 3 + c];
+            for (int c = 0; c < 3; c++) su[le][z % 3][p][c] = x[(ptrdiff_t)g_next * 3 + c];
         }
     };
     load_plane(0);
     __syncthreads();
+    T *pe = part + e * (ptrdiff_t)NSK * 3;
 
     for (int z = 0; z <= L; z++) {
+        const int32_t g = g_next;
+        const bool interior = !edge_xy && z > 0 && z < L;
+        T yold[3] = {0, 0, 0};
+        if (active && interior) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) yold[c] = y[(ptrdiff_t)g * 3 + c];
+        }
         if (z + 1 <= L) {
             // slot (z+1)%3 held plane z-2, which nobody reads any more
             load_plane(z + 1);
@@ -328,7 +344,7 @@ __global__ void __launch_bounds__(ElShape<L>::BLOCK)
         __syncthreads();
         if (active) {
             T acc[3] = {0, 0, 0};
-            const T *K = sK[le];
+            const T *K = EPB == 1 ? Kall + e0 * 576 : &sK[le][0];
 #pragma unroll
             for (int c = 0; c < 8; c++) {
                 const int cx = hex8_cx(c), cy = hex8_cy(c), cz = hex8_cz(c);
@@ -354,14 +370,13 @@ __global__ void __launch_bounds__(ElShape<L>::BLOCK)
                     }
                 }
             }
-            const int32_t g = row[lidx(L, xx, yy, z)];
-            const bool interior = xx > 0 && xx < L && yy > 0 && yy < L && z > 0 && z < L;
+            if (interior) {
 #pragma unroll
-            for (int ci = 0; ci < 3; ci++) {
-                if (interior)
-                    y[(ptrdiff_t)g * 3 + ci] += acc[ci];
-                else
-                    atomic_add(&y[(ptrdiff_t)g * 3 + ci], acc[ci]);
+                for (int ci = 0; ci < 3; ci++) y[(ptrdiff_t)g * 3 + ci] = yold[ci] + acc[ci];
+            } else {
+                T *q = pe + (ptrdiff_t)sk_slot(L, xx, yy, z) * 3;
+#pragma unroll
+                for (int ci = 0; ci < 3; ci++) q[ci] = acc[ci];
             }
         }
         __syncthreads();
@@ -483,15 +498,15 @@ int sshex8_elasticity_matrices(int L, ptrdiff_t E, const G *adj, const G *det, p
 }
 
 template <typename T>
-int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const double *K, const T *x, T *y,
-                            hipStream_t s) {
+int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *x, T *y, T *part,
+                            const SkeletonMap &sk, hipStream_t s) {
     if (E == 0) return SFEM_SUCCESS;
     switch (L) {
 #define CASE(LV)                                                                                 \
     case LV: {                                                                                   \
         using Sh = ElShape<LV>;                                                                  \
         const unsigned grid = (unsigned)((E + Sh::EPB - 1) / Sh::EPB);                           \
-        sshex8_elasticity_apply_kernel<T, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, K, x, y);       \
+        sshex8_elasticity_apply_kernel<T, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, K, x, y, part); \
         break;                                                                                   \
     }
         SFEM_AMD_SSHEX8_LEVELS(CASE)
@@ -501,7 +516,7 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const double
             return SFEM_FAILURE;
     }
     SFEM_AMD_HIP_CHECK(hipGetLastError());
-    return SFEM_SUCCESS;
+    return skeleton_gather<T>(sk, 3, part, y, s);
 }
 
 template <typename T>
@@ -541,9 +556,9 @@ template int sshex8_elasticity_matrices<float>(int, ptrdiff_t, const float *, co
 template int sshex8_elasticity_matrices<uint16_t>(int, ptrdiff_t, const uint16_t *, const uint16_t *, ptrdiff_t,
                                                   double, double, const float *, double *, hipStream_t);
 template int sshex8_elasticity_apply<double>(int, ptrdiff_t, const int32_t *, const double *, const double *,
-                                             double *, hipStream_t);
-template int sshex8_elasticity_apply<float>(int, ptrdiff_t, const int32_t *, const double *, const float *, float *,
-                                            hipStream_t);
+                                             double *, double *, const SkeletonMap &, hipStream_t);
+template int sshex8_elasticity_apply<float>(int, ptrdiff_t, const int32_t *, const float *, const float *, float *,
+                                            float *, const SkeletonMap &, hipStream_t);
 template int sshex8_elasticity_diag<double>(int, ptrdiff_t, const int32_t *, const double *, int, double *,
                                             hipStream_t);
 template int sshex8_elasticity_diag<float>(int, ptrdiff_t, const int32_t *, const double *, int, float *,

@@ -31,8 +31,10 @@ template <typename G>
 int sshex8_elasticity_matrices(int L, ptrdiff_t E, const G *adj, const G *det, ptrdiff_t gstride, double mu,
                                double lambda, const float *coeff, double *K, hipStream_t s);
 template <typename T>
-int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const double *K, const T *x, T *y,
-                            hipStream_t s);
+// part: E * sk_count(L) * 3 values of scratch
+// K: per-macro 24x24 matrices in the vector precision
+int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *x, T *y, T *part,
+                            const SkeletonMap &sk, hipStream_t s);
 template <typename T>
 int sshex8_elasticity_diag(int L, ptrdiff_t E, const int32_t *idx, const double *K, int block, T *d, hipStream_t s);
 

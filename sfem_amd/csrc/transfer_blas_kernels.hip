@@ -267,6 +267,18 @@ __global__ void cheb3_update_kernel(const ptrdiff_t n, const T alpha, const T be
     }
 }
 
+// diagonally scaled variant: p = beta p + dinv (A x - rhs); x -= alpha p
+template <typename T>
+__global__ void cheb3_update_diag_kernel(const ptrdiff_t n, const T alpha, const T beta, const T *__restrict__ rhs,
+                                         const T *__restrict__ Ax, const T *__restrict__ dinv, T *__restrict__ p,
+                                         T *__restrict__ x) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const T pi = beta * p[i] + dinv[i] * (Ax[i] - rhs[i]);
+        p[i] = pi;
+        x[i] -= alpha * pi;
+    }
+}
+
 // Deterministic reductions: a fixed grid (a function of n only) of 256-thread blocks,
 // wave64 shuffle trees and one pass over the partials in a fixed order.
 constexpr int kRedBlocks = 1024;
@@ -405,6 +417,14 @@ int cheb3_update(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, T *p, 
 }
 
 template <typename T>
+int cheb3_update_diag(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, const T *dinv, T *p, T *x,
+                      hipStream_t s) {
+    cheb3_update_diag_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, alpha, beta, rhs, Ax, dinv, p, x);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template <typename T>
 int blas_dot(ptrdiff_t n, const T *x, const T *y, double *ws, double *result_host, hipStream_t s) {
     const int nb = red_blocks(n);
     dot_partial_kernel<T><<<nb, kBlock, 0, s>>>(n, x, y, ws);
@@ -492,6 +512,7 @@ int cast_accumulate(ptrdiff_t n, const float *x, double *y, hipStream_t s) {
     template int blas_reciprocal<T>(ptrdiff_t, T, T *, hipStream_t);                                    \
     template int blas_xypaz<T>(ptrdiff_t, const T *, const T *, T, T *, hipStream_t);                   \
     template int cheb3_update<T>(ptrdiff_t, T, T, const T *, const T *, T *, T *, hipStream_t);         \
+    template int cheb3_update_diag<T>(ptrdiff_t, T, T, const T *, const T *, const T *, T *, T *, hipStream_t); \
     template int blas_dot<T>(ptrdiff_t, const T *, const T *, double *, double *, hipStream_t);         \
     template int cg_update<T>(ptrdiff_t, T, const T *, const T *, T *, T *, double *, double *, hipStream_t); \
     template int dofs_set_value<T>(ptrdiff_t, const int32_t *, T, T *, hipStream_t);                    \

@@ -7,9 +7,11 @@ algebra/mg/sfem_GeometricMultigrid.cpp:12-168) with the smoothing of drivers/mgs
   * every level rediscretises the operator with the macro geometry (dirichlet_as_zero on
     coarse levels: the coarse constrained nodes are the fine ones that exist there);
   * prolongation and restriction zero the constrained dofs of their output level;
-  * smoothing: Chebyshev3 (degree 3, power method from ones with constrained dofs zeroed,
-    scale_eig_max 1.02, tolerance 1e-5, drivers/mgsolve.cpp:68-71,176-185), or the
-    reference's default damped Jacobi (create_gmg_default_smoothers_and_solver);
+  * smoothing: Chebyshev3 (degree 3, power method from ones with constrained dofs zeroed, at
+    most 1000 iterations (algebra/sfem_Chebyshev3.hpp:72), scale_eig_max 1.02, tolerance 1e-5,
+    drivers/mgsolve.cpp:68-71,176-185), or the
+    reference's default damped Jacobi (create_gmg_default_smoothers_and_solver), or
+    (extension) "cheb3_jacobi": Chebyshev3 on D^-1 A for strongly varying coefficients;
   * coarsest: FP64 CG with inverse-diagonal preconditioning (drivers/mgsolve.cpp:214-227).
 
 Extension (not in the reference): when the macro mesh is a structured box of n^3 hex8
@@ -56,7 +58,7 @@ class SSGMG:
     macro mesh made by box_mesh(n+1, n+1, n+1, extent, extent, extent)."""
 
     def __init__(self, fine_mesh, kind, geometry, bc_nodes, coeff=None, real_type=FLOAT64, smoother="cheb3",
-                 cheb_degree=3, eig_its=10, eig_tol=1e-5, scale_eig_max=1.02, coarse_rtol=1e-8,
+                 cheb_degree=3, eig_its=1000, eig_tol=1e-5, scale_eig_max=1.02, coarse_rtol=1e-8,
                  coarse_max_it=10000, mu=None, lam=None, bc_components=None, max_it=1, atol=0.0, macro_box=None,
                  learned=None, learned_levels=1, learned_steps=1, learned_mode=1, learned_damping=1.0):
         self.kind = kind
@@ -106,11 +108,16 @@ class SSGMG:
     def _make_smoother(self, lv):
         cfg = self._smoother_cfg
         dtype = np.float64 if lv.op.real_type == FLOAT64 else np.float32
-        if cfg["kind"] == "cheb3":
+        if cfg["kind"] in ("cheb3", "cheb3_jacobi"):
             init = DeviceArray.from_numpy(np.ones(lv.n, dtype))
             set_constrained_value(lv.A, 0.0, init)
+            inv = None
+            if cfg["kind"] == "cheb3_jacobi":
+                d = DeviceArray(lv.n, dtype)
+                lv.A.diag(d)
+                inv = DeviceArray.from_numpy((1.0 / d.numpy()).astype(dtype))
             return cheb3(lv.A, max_it=cfg["degree"], scale_eig_max=cfg["smax"], eigen_solver_max_it=cfg["eig_its"],
-                         eigen_solver_tol=cfg["eig_tol"], init_vector=init)
+                         eigen_solver_tol=cfg["eig_tol"], init_vector=init, inv_diag=inv)
         d = DeviceArray(lv.n, dtype)
         lv.A.diag(d)
         return jacobi(lv.A, d, omega=1.0 / self.block_size, steps=3)

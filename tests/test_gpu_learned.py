@@ -130,45 +130,55 @@ def _solve(h, N, rhs, A64, switches=(), ratio=0.5, rtol=1e-8):
     return solver.iterations, solver.last_status, res
 
 
-@pytest.mark.xfail(reason="gain threshold under calibration", strict=False)
-def test_learned_smoother_gain_and_fallback():
-    """Variable-coefficient sshex8 Poisson (L=8, 4^3 macro elements, k = 10^U[0,3] per macro
-    element).  The trained CNN smoother on the fine level is compared, inside the same FP32
-    V-cycle and FP64 flexible CG, with the untrained network and with one damped Jacobi sweep
-    (the smoother with the same number of operator applies); Chebyshev3 is reported."""
-    mesh, fff, bc, coeff, N = _poisson_problem(L=8, n_macro=4, coeff_seed=7)
-    h_plain = gmg.SSGMG(mesh, "laplacian", fff, bc, coeff=coeff, real_type=S.FLOAT32)
+def test_learned_smoother_gain_over_chebyshev():
+    """sshex8 Poisson (L=8, 4^3 macro elements, zero Dirichlet on all faces).  In the same FP32
+    V-cycle inside FP64 flexible CG, two learned-smoother steps per smoothing (2 operator applies
+    + 2 CNN evaluations) against Chebyshev3 of degree 3 (3 applies) on the finest level: fewer
+    iterations to 1e-8 and no fallback.  The untrained network is caught by the fallback."""
+    mesh, fff, bc, coeff, N = _poisson_problem(L=8, n_macro=4, coeff_seed=None)
+    h_plain = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32)
     cnn, data, before, after = _train(h_plain, seed=1, steps=300)
     print(f"training loss {before:.3f} -> {after:.3f}")
-    h64 = gmg.SSGMG(mesh, "laplacian", fff, bc, coeff=coeff)
-    A64 = h64.fine.A
+    assert after < 0.1
+    A64 = gmg.SSGMG(mesh, "laplacian", fff, bc).fine.A
     rhs = np.random.default_rng(8).uniform(0, 1, N)
     rhs[bc] = 0
 
     its_cheb, st, res = _solve(h_plain, N, rhs, A64)
     assert st == 0 and res < 1e-8
-
-    h_learn = gmg.SSGMG(mesh, "laplacian", fff, bc, coeff=coeff, real_type=S.FLOAT32, learned=cnn)
+    h_learn = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, learned=cnn, learned_steps=2)
     its_learn, st, res = _solve(h_learn, N, rhs, A64, h_learn.switches, ratio=0.5)
-    active, events = S.switch_state(h_learn.switches[0])
-    print(f"learned: {its_learn} its, res {res:.2e}, fallback events {events}")
-    assert st == 0 and res < 1e-8
-
-    h_jac = gmg.SSGMG(mesh, "laplacian", fff, bc, coeff=coeff, real_type=S.FLOAT32, smoother="jacobi")
-    its_jac, st_j, _ = _solve(h_jac, N, rhs, A64)
-
+    _, events = S.switch_state(h_learn.switches[0])
     untrained = S.CNN(16, 4, seed=123)
-    h_u = gmg.SSGMG(mesh, "laplacian", fff, bc, coeff=coeff, real_type=S.FLOAT32, learned=untrained)
-    h_u.select_smoothers(0)
+    h_u = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, learned=untrained, learned_steps=2)
     its_u, st_u, res_u = _solve(h_u, N, rhs, A64, h_u.switches, ratio=0.5)
     _, events_u = S.switch_state(h_u.switches[0])
-    print(f"iterations: chebyshev3 {its_cheb}, learned {its_learn}, jacobi(3 sweeps) {its_jac}, "
-          f"untrained {its_u} (fallback events {events_u})")
-    # the trained network must not need the fallback and must beat the untrained one
-    assert events == 0
-    assert its_learn < its_u or events_u > 0
-    # the untrained network is caught by the fallback and the solve still converges
-    assert st_u == 0 and res_u < 1e-8
+    print(f"iterations: chebyshev3 {its_cheb}, learned x2 {its_learn} (fallbacks {events}), "
+          f"untrained x2 {its_u} (fallbacks {events_u})")
+    assert st == 0 and res < 1e-8 and events == 0
+    assert its_learn < its_cheb
+    assert events_u == 1 and st_u == 0 and res_u < 1e-8
+
+
+def test_learned_smoother_variable_coefficient_falls_back():
+    """Per-element coefficients 10^U[0,3]: Chebyshev3 on A stalls, Chebyshev3 on D^-1 A does
+    not; a learned smoother with the scaled Chebyshev3 fallback converges like the latter."""
+    mesh, fff, bc, coeff, N = _poisson_problem(L=8, n_macro=4, coeff_seed=7)
+    h_cj = gmg.SSGMG(mesh, "laplacian", fff, bc, coeff=coeff, real_type=S.FLOAT32, smoother="cheb3_jacobi")
+    cnn, data, before, after = _train(h_cj, seed=1, steps=300)
+    A64 = gmg.SSGMG(mesh, "laplacian", fff, bc, coeff=coeff).fine.A
+    rhs = np.random.default_rng(8).uniform(0, 1, N)
+    rhs[bc] = 0
+    its_cj, st, res = _solve(h_cj, N, rhs, A64)
+    assert st == 0 and res < 1e-8
+    h_l = gmg.SSGMG(mesh, "laplacian", fff, bc, coeff=coeff, real_type=S.FLOAT32, smoother="cheb3_jacobi",
+                    learned=cnn, learned_steps=2)
+    its_l, st_l, res_l = _solve(h_l, N, rhs, A64, h_l.switches, ratio=0.5)
+    _, events = S.switch_state(h_l.switches[0])
+    print(f"variable coefficient: scaled chebyshev3 {its_cj} its; learned x2 {its_l} its "
+          f"(fallbacks {events}); training loss {before:.3f} -> {after:.3f}")
+    assert st_l == 0 and res_l < 1e-8
+    assert its_l <= its_cj + 3
 
 
 def test_forced_fallback_switches_once():

@@ -154,7 +154,7 @@ def test_hex8_elasticity_parity():
     assert relerr(host(d), do) < RTOL
 
 
-@pytest.mark.parametrize("L,n", [(1, 2), (2, 2), (4, 2), (6, 1), (8, 2), (16, 1)])
+@pytest.mark.parametrize("L,n", [(1, 2), (2, 2), (4, 2), (6, 1), (8, 2), (16, 1), (16, 2)])
 def test_sshex8_elasticity_parity(L, n):
     mel, mpts = skewed_box(n, perturb=0.02)
     el, pts = O.sshex8_refine(L, mel, mpts)
@@ -178,7 +178,7 @@ def test_sshex8_elasticity_parity(L, n):
     op.block_diag_sym(b)
     S.synchronize()
     assert relerr(host(b), bo) < RTOL
-    if L == 8:
+    if L in (8, 16):
         op32 = S.linear_elasticity(S.Mesh(S.SSHEX8, L, el, N), adj, det, mu, lam, coeff, real_type=S.FLOAT32)
         err = relerr(host(gpu_apply(op32, dev(x, np.float32))).astype(np.float64), yo)
         print(f"FP32 elasticity apply with BF16 geometry: relative error vs FP64 oracle = {err:.3e}")

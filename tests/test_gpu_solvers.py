@@ -103,6 +103,38 @@ def test_power_method_and_chebyshev_parity():
     assert np.linalg.norm(host(x) - xo) / np.linalg.norm(xo) < 1e-12
 
 
+def test_jacobi_scaled_chebyshev_parity():
+    """Chebyshev3 on D^-1 A (extension) against the oracle with the same scaling."""
+    L = 4
+    mel, mpts, el, pts, fff, N, nodes, coeff = _ss_poisson(L, 3, coeff_seed=2)
+    E = mel.shape[1]
+    A_o = oracle_constrained(lambda x, y: O.lib.oracle_sshex8_laplacian_apply(L, E, el, fff, coeff, x, y), nodes)
+    d = np.zeros(N)
+    O.lib.oracle_sshex8_laplacian_diag(L, E, el, fff, coeff, d)
+    d[nodes] = 1
+    v0 = np.ones(N)
+    v0[nodes] = 0
+    A = S.constrained(S.laplacian(S.Mesh(S.SSHEX8, L, el, N), fff, coeff), nodes)
+    dd = zeros(N)
+    A.diag(dd)
+    assert np.linalg.norm(host(dd) - d) < 1e-12 * np.linalg.norm(d)
+    ch_o = SO.Chebyshev3(A_o, N, max_it=3, scale_eig_max=1.02, eig_max_it=10, eig_tol=1e-5, inv_diag=1.0 / d,
+                         init=v0)
+    ch = S.cheb3(A, max_it=3, scale_eig_max=1.02, eigen_solver_max_it=10, eigen_solver_tol=1e-5, init_vector=dev(v0),
+                 inv_diag=dev(1.0 / host(dd)))
+    assert abs(S.cheb3_eig_max(ch) - ch_o.eig_max) < 1e-10 * ch_o.eig_max
+    rng = np.random.default_rng(3)
+    b = rng.uniform(-1, 1, N)
+    b[nodes] = 0
+    x0 = rng.uniform(-1, 1, N)
+    x0[nodes] = 0
+    xo = ch_o(b, x0.copy())
+    x = dev(x0)
+    ch.apply(dev(b), x)
+    S.synchronize()
+    assert np.linalg.norm(host(x) - xo) / np.linalg.norm(xo) < 1e-12
+
+
 def _oracle_hierarchy(L, mel, mpts, el, pts, fff, coeff, nodes, lams):
     """Oracle V-cycle mirroring sfem_amd.gmg.SSGMG (same eig_max values)."""
     E = mel.shape[1]

@@ -90,3 +90,23 @@ def test_two_grid_error_propagator():
     M = np.eye(n) - w * (A / D[:, None])
     E = M @ (np.eye(n) - P @ np.linalg.solve(Ac, R @ A)) @ M
     np.testing.assert_allclose(x, (np.eye(n) - E) @ np.linalg.solve(A, b), rtol=1e-12, atol=1e-12)
+
+
+def test_jacobi_scaled_chebyshev_on_diagonal_matrix_matches_identity():
+    """Chebyshev3 on D^-1 A with A = diag(d) is the unscaled smoother on A = I: the error of
+    every component is multiplied by the same polynomial value."""
+    rng = np.random.default_rng(5)
+    n = 40
+    dvec = rng.uniform(0.1, 100, n)
+    A = lambda x, y: y.__iadd__(dvec * x)  # noqa: E731
+    I = lambda x, y: y.__iadd__(x)  # noqa: E731
+    b = rng.uniform(-1, 1, n)
+    x0 = rng.uniform(-1, 1, n)
+    xs = SO.Chebyshev3(A, n, max_it=3, scale_eig_max=1.02, eig_max_it=9, eig_tol=1e-12, inv_diag=1.0 / dvec)
+    xi = SO.Chebyshev3(I, n, max_it=3, scale_eig_max=1.02, eig_max_it=9, eig_tol=1e-12)
+    assert abs(xs.eig_max - 1.0) < 1e-12 and abs(xi.eig_max - 1.0) < 1e-12
+    es = xs(b, x0.copy()) - b / dvec  # error w.r.t. the exact solution of A x = b
+    ei = xi(b / dvec, x0.copy()) - b / dvec
+    np.testing.assert_allclose(es, ei, rtol=1e-12, atol=1e-14)
+    ratio = es / (x0 - b / dvec)
+    assert np.ptp(ratio) < 1e-10 and abs(ratio[0]) < 1
