@@ -144,7 +144,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flush", action="store_true")
     ap.add_argument("--mode", choices=["apply", "copy"], default="apply",
-                    help="copy: stream-copy 256 MiB of FP64 per step (HBM counter calibration for rocprofv3 --pmc)")
+                    help="copy: stream-copy 2 GiB of FP64 per step (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (every box here has one GPU; "
@@ -156,7 +156,7 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     if args.mode == "copy":
-        n = (256 << 20) // 8
+        n = (2 << 30) // 8  # 2 GiB per array: far beyond the 256 MB MALL
         a = S.DeviceArray.from_numpy(np.ones(n))
         b = S.DeviceArray(n)
         import time as _t

@@ -41,7 +41,8 @@ public:
         return p;
     }
     void destroy(void *a) override { sfem_amd_free(a); }
-    void zeros(std::size_t n, T *x) override { sfem_amd_memset(x, 0, sizeof(T) * n, s_); }
+    // a fill kernel, not a memset: solver sequences may be captured into HIP graphs
+    void zeros(std::size_t n, T *x) override { sfem_amd_blas_values(rt, (std::ptrdiff_t)n, 0.0, x, s_); }
     void values(std::size_t n, T v, T *x) override { sfem_amd_blas_values(rt, (std::ptrdiff_t)n, (double)v, x, s_); }
     void copy(std::ptrdiff_t n, const T *src, T *dst) override { sfem_amd_blas_copy(rt, n, src, dst, s_); }
     T dot(std::ptrdiff_t n, const T *l, const T *r) override { return (T)sfem_amd_blas_dot(rt, n, l, r, s_); }

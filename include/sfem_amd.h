@@ -109,6 +109,14 @@ void sfem_amd_mesh_destroy(sfem_amd_mesh *m);
  * real_type selects FP64 or FP32 vectors; FP32 keeps geometry in BF16 (chosen form 1). */
 int sfem_amd_laplacian_create(const sfem_amd_mesh *mesh, const float *fff, const float *coeff,
                               int real_type, sfem_amd_op **out);
+/* Stored per-macro sub-element matrices ("ss:em" operators: SemiStructuredEMLaplacian /
+ * SemiStructuredEMLinearElasticity, frontend/ops/sfem_SemiStructuredEM*.cpp) applied to every
+ * micro element of an sshex8 mesh: the device counterpart of sshex8_stencil_element_matrix_apply
+ * (bs 1: matrices [E][64], row-major in hex8 corner order) and ..._apply3 (bs 3: [E][576],
+ * index component*8 + node), operators/stencil/sshex8_stencil_element_matrix_apply.cpp:10-160.
+ * matrices: host FP64 (copied).  rt: vector precision. */
+int sfem_amd_element_matrix_op_create(const sfem_amd_mesh *mesh, const double *matrices, int block_size,
+                                      int real_type, sfem_amd_op **out);
 /* Linear elasticity (3 dofs/node, interleaved): linear_elasticity_apply_adjugate_{aos,soa}
  * (operators/linear_elasticity.cpp:303-392), affine_hex8_linear_elasticity_apply_adjugate_soa
  * (operators/hex8/hex8_linear_elasticity.cpp:625-760),
@@ -287,6 +295,17 @@ int sfem_amd_gather_elements(const sfem_amd_mesh *mesh, int real_type, int bs, i
 int sfem_amd_learned_smoother_create(sfem_amd_op *A, const sfem_amd_mesh *mesh, sfem_amd_cnn *cnn,
                                      const void *diag, int bs, int steps, double damping, int mode,
                                      sfem_amd_op **out);
+/* y += M x with a dense row-major FP64 matrix (host, n x n, copied); for the coarsest level of a
+ * multigrid hierarchy as a direct solve through a precomputed inverse (no host synchronisation,
+ * unlike the reference's coarse CG, drivers/mgsolve.cpp:214-227). */
+int sfem_amd_dense_op_create(int real_type, ptrdiff_t n, const double *matrix, sfem_amd_op **out);
+/* Wraps `op` so its apply replays a captured HIP graph (no counterpart in the reference): the
+ * first apply runs eagerly, the second is captured and launched, later applies with the same
+ * (x, y, stream) relaunch the graph and others run eagerly.  For launch-bound sequences without
+ * host synchronisation, e.g. a preconditioning V-cycle (max_it 1) with a dense coarse solve. */
+int sfem_amd_graph_op_create(sfem_amd_op *op, sfem_amd_op **out);
+/* captured: 1 graph instantiated, 0 not yet, -1 capture failed (eager); replays: graph launches. */
+int sfem_amd_graph_op_stats(const sfem_amd_op *op, int *captured, int *replays);
 /* An op that runs `primary` until switched to `fallback` (same size and real type). */
 int sfem_amd_switch_create(sfem_amd_op *primary, sfem_amd_op *fallback, sfem_amd_op **out);
 int sfem_amd_switch_select(sfem_amd_op *sw, int which);

@@ -708,6 +708,65 @@ void oracle_sshex8_laplacian_apply(int L, ptrdiff_t ne, const int32_t *el, const
     }
 }
 
+/* Stored sub-element matrices applied to every micro element ("ss:em"):
+ * operators/stencil/sshex8_stencil_element_matrix_apply.cpp:10-78 (bs 1, [E][64]) and :80-160
+ * (bs 3, [E][576], index component*8 + node; vectors interleaved). */
+void oracle_sshex8_em_apply(int L, ptrdiff_t ne, const int32_t *el, int bs, const double *mats, const double *x,
+                            double *y) {
+    const int nxe = oracle_sshex8_nxe(L), nd = 8 * bs;
+#pragma omp parallel
+    {
+        double *eu = (double *)malloc(sizeof(double) * bs * nxe);
+        double *ev = (double *)malloc(sizeof(double) * bs * nxe);
+#pragma omp for
+        for (ptrdiff_t e = 0; e < ne; e++) {
+            const double *K = mats + (size_t)e * nd * nd;
+            for (int d = 0; d < nxe; d++) {
+                const int32_t n = el[(size_t)d * ne + e];
+                for (int c = 0; c < bs; c++) {
+                    eu[c * nxe + d] = x[(size_t)n * bs + c];
+                    ev[c * nxe + d] = 0;
+                }
+            }
+            for (int zi = 0; zi < L; zi++)
+                for (int yi = 0; yi < L; yi++)
+                    for (int xi = 0; xi < L; xi++) {
+                        int lev[8];
+                        double u[24];
+                        micro_corners(L, xi, yi, zi, lev);
+                        for (int c = 0; c < bs; c++)
+                            for (int v = 0; v < 8; v++) u[c * 8 + v] = eu[c * nxe + lev[v]];
+                        for (int i = 0; i < nd; i++) {
+                            double acc = 0;
+                            for (int j = 0; j < nd; j++) acc += K[i * nd + j] * u[j];
+                            ev[(i / 8) * nxe + lev[i % 8]] += acc;
+                        }
+                    }
+            for (int d = 0; d < nxe; d++) {
+                const int32_t n = el[(size_t)d * ne + e];
+                for (int c = 0; c < bs; c++) {
+#pragma omp atomic update
+                    y[(size_t)n * bs + c] += ev[c * nxe + d];
+                }
+            }
+        }
+        free(eu);
+        free(ev);
+    }
+}
+
+/* sshex8_laplacian_element_matrix (operators/sshex8/sshex8_laplacian.cpp:595-634) from the macro
+ * FFF words: sub-FFF = h * FFF (times the coefficient), then the 8x8 exact-integral matrix. */
+void oracle_sshex8_laplacian_element_matrices(int L, ptrdiff_t ne, const float *fff, const float *coeff,
+                                              double *out) {
+    for (ptrdiff_t e = 0; e < ne; e++) {
+        const double h = 1.0 / L, c = coef(coeff, e);
+        double f[6];
+        for (int k = 0; k < 6; k++) f[k] = (double)fff[e * 6 + k] * h * c;
+        oracle_hex8_laplacian_matrix(f, out + (size_t)e * 64);
+    }
+}
+
 /* operators/sshex8/sshex8_laplacian.cpp:333-435 */
 void oracle_sshex8_laplacian_diag(int L, ptrdiff_t ne, const int32_t *el, const float *fff, const float *coeff,
                                   double *d) {
@@ -745,6 +804,14 @@ static void elasticity_matrix_from_words(const float *adj, const float *det, ptr
     const double dt = (double)det[e] * (h * h * h);
     oracle_hex8_elasticity_matrix(mu * c, lambda * c, A, dt, K);
 }
+
+/* The sshex8 elasticity sub-element matrices (adjugate * h^2, det * h^3; component*8 + node). */
+void oracle_sshex8_elasticity_element_matrices(int L, ptrdiff_t ne, const float *adj, const float *det, double mu,
+                                               double lambda, const float *coeff, double *out) {
+    for (ptrdiff_t e = 0; e < ne; e++)
+        elasticity_matrix_from_words(adj, det, e, mu, lambda, coef(coeff, e), 1.0 / L, out + (size_t)e * 576);
+}
+
 
 /* operators/hex8/hex8_linear_elasticity.cpp:625-760 (affine, adjugate at centre) */
 void oracle_hex8_elasticity_apply(ptrdiff_t ne, const int32_t *el, const float *adj, const float *det, double mu,

@@ -100,6 +100,15 @@ void oracle_hex8_laplacian_diag(ptrdiff_t nelements, const int32_t *elements, co
                                 const float *coeff, double *d);
 void oracle_sshex8_laplacian_apply(int L, ptrdiff_t nelements, const int32_t *elements, const float *fff,
                                    const float *coeff, const double *x, double *y);
+/* "ss:em": stored sub-element matrices applied to every micro element (bs 1: [E][64]; bs 3:
+ * [E][576], vectors interleaved).  operators/stencil/sshex8_stencil_element_matrix_apply.cpp:10-160 */
+void oracle_sshex8_em_apply(int L, ptrdiff_t nelements, const int32_t *elements, int bs, const double *matrices,
+                            const double *x, double *y);
+/* operators/sshex8/sshex8_laplacian.cpp:595-634 (from the macro FFF words) */
+void oracle_sshex8_laplacian_element_matrices(int L, ptrdiff_t nelements, const float *fff, const float *coeff,
+                                              double *out);
+void oracle_sshex8_elasticity_element_matrices(int L, ptrdiff_t nelements, const float *adj, const float *det,
+                                               double mu, double lambda, const float *coeff, double *out);
 void oracle_sshex8_laplacian_diag(int L, ptrdiff_t nelements, const int32_t *elements, const float *fff,
                                   const float *coeff, double *d);
 /* elasticity: x,y interleaved [node*3 + c]; adjugate [E][9], det [E]. */

@@ -87,6 +87,7 @@ SIGNATURES = {
     "sfem_amd_cg_history": (_i, [_vp, _vp, _i]),
     "sfem_amd_op_last_status": (_i, [_vp]),
     "sfem_amd_op_benchmark": (_i, [_vp, _vp, _vp, _i, _i, C.c_size_t, _vp, C.POINTER(_d)]),
+    "sfem_amd_element_matrix_op_create": (_i, [_vp, _vp, _i, _i, C.POINTER(_vp)]),
     "sfem_amd_cnn_create": (_i, [_i, _i, _vp, C.c_uint, C.POINTER(_vp)]),
     "sfem_amd_cnn_num_params": (_i, [_vp]),
     "sfem_amd_cnn_get_weights": (_i, [_vp, _vp]),
@@ -97,6 +98,9 @@ SIGNATURES = {
     "sfem_amd_cnn_adam_step": (_i, [_vp, _d, _vp]),
     "sfem_amd_gather_elements": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "sfem_amd_learned_smoother_create": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _i, C.POINTER(_vp)]),
+    "sfem_amd_dense_op_create": (_i, [_i, _pd, _vp, C.POINTER(_vp)]),
+    "sfem_amd_graph_op_create": (_i, [_vp, C.POINTER(_vp)]),
+    "sfem_amd_graph_op_stats": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "sfem_amd_switch_create": (_i, [_vp, _vp, C.POINTER(_vp)]),
     "sfem_amd_switch_select": (_i, [_vp, _i]),
     "sfem_amd_switch_state": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
@@ -384,6 +388,12 @@ def linear_elasticity(mesh, adjugate, det, mu, lam, coeff=None, real_type=FLOAT6
                    real_type, keep=[mesh])
 
 
+def element_matrix_op(mesh, matrices, block_size=1, real_type=FLOAT64):
+    """ss:em operator: stored per-macro sub-element matrices ([E][64] or [E][576], FP64)."""
+    m = _host(matrices, np.float64)
+    return _new_op("sfem_amd_element_matrix_op_create", mesh.h, _np_ptr(m), block_size, real_type, keep=[mesh])
+
+
 def constrained(op, nodes, block_size=1, components=None):
     n = _host(nodes, np.int32)
     comps = None if components is None else np.ascontiguousarray(components, np.int32)
@@ -536,6 +546,25 @@ def gather_elements(mesh, vec, out, block_size=1, component=0, scale=None, strea
 def learned_smoother(A, mesh, cnn, diag, block_size=1, steps=1, damping=1.0, mode=1):
     return _new_op("sfem_amd_learned_smoother_create", A.h, mesh.h, cnn.h, _dptr(diag), block_size, steps,
                    damping, mode, keep=[A, mesh, cnn])
+
+
+def dense_op(matrix, real_type=FLOAT64):
+    """y += M x with a dense FP64 matrix (e.g. the inverse of a coarsest-level operator)."""
+    m = _host(matrix, np.float64)
+    if m.ndim != 2 or m.shape[0] != m.shape[1]:
+        raise SfemError("dense_op: square matrix expected")
+    return _new_op("sfem_amd_dense_op_create", real_type, m.shape[0], _np_ptr(m))
+
+
+def graph_op(op):
+    """Replays op's apply as a captured HIP graph (see sfem_amd_graph_op_create)."""
+    return _new_op("sfem_amd_graph_op_create", op.h, keep=[op])
+
+
+def graph_op_stats(op):
+    c, r = C.c_int(), C.c_int()
+    _check(lib.sfem_amd_graph_op_stats(op.h, C.byref(c), C.byref(r)), "graph_op_stats")
+    return c.value, r.value
 
 
 def switch(primary, fallback):

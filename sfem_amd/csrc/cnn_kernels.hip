@@ -428,7 +428,8 @@ __global__ void __launch_bounds__(512) cnn_infer_kernel(const ptrdiff_t E, const
             *(mf_u2 *)(actX + n * 16 + 4 * g) = o2;
         }
         __syncthreads();
-        // ---- layers 2 and 3: actX -> actY -> actX
+        // ---- layers 2 and 3: actX -> actY -> actX.  Two tiles per wave at a time: two independent
+        // MFMA accumulation chains hide the MFMA latency.
 #pragma unroll 1
         for (int layer = 0; layer < 2; layer++) {
             const unsigned short *ain = layer ? actY : actX;
@@ -437,24 +438,41 @@ __global__ void __launch_bounds__(512) cnn_infer_kernel(const ptrdiff_t E, const
             mf_u4 w[14];
 #pragma unroll
             for (int p = 0; p < 14; p++) w[p] = *(const mf_u4 *)(wl + (p * 64 + lane) * 8);
-            for (int t = wave; t < S::T; t += nw) {
-                const int n = t * 16 + c;
-                const int x = n % S::N1, y = (n / S::N1) % S::N1, z = n / (S::N1 * S::N1);
-                const bool in = n < S::NXE;
-                mf_f4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (int t = 2 * wave; t < S::T; t += 2 * nw) {
+                int n[2], x[2], y[2], z[2];
+                bool in[2];
+                mf_f4 acc[2];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    n[h] = (t + h) * 16 + c;
+                    x[h] = n[h] % S::N1;
+                    y[h] = (n[h] / S::N1) % S::N1;
+                    z[h] = n[h] / (S::N1 * S::N1);
+                    in[h] = n[h] < S::NXE;
+                    acc[h] = mf_f4{0.f, 0.f, 0.f, 0.f};
+                }
 #pragma unroll
                 for (int p = 0; p < 14; p++) {
                     const int o = 2 * p + (g >> 1);
-                    mf_u4 bv = {0u, 0u, 0u, 0u};
-                    if (in && cnn_valid<L>(x, y, z, o))
-                        bv = *(const mf_u4 *)(ain + (n + cnn_shift<L>(o)) * 16 + 8 * (g & 1));
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mf_b8, w[p]),
-                                                                   __builtin_bit_cast(mf_b8, bv), acc, 0, 0, 0);
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        mf_u4 bv = {0u, 0u, 0u, 0u};
+                        if (in[h] && cnn_valid<L>(x[h], y[h], z[h], o))
+                            bv = *(const mf_u4 *)(ain + (n[h] + cnn_shift<L>(o)) * 16 + 8 * (g & 1));
+                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mf_b8, w[p]),
+                                                                          __builtin_bit_cast(mf_b8, bv), acc[h], 0, 0,
+                                                                          0);
+                    }
                 }
-                mf_u2 o2;
-                o2[0] = (unsigned)f2bf(fmaxf(acc[0], 0.f)) | ((unsigned)f2bf(fmaxf(acc[1], 0.f)) << 16);
-                o2[1] = (unsigned)f2bf(fmaxf(acc[2], 0.f)) | ((unsigned)f2bf(fmaxf(acc[3], 0.f)) << 16);
-                *(mf_u2 *)(aout + n * 16 + 4 * g) = o2;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    if (t + h < S::T) {
+                        mf_u2 o2;
+                        o2[0] = (unsigned)f2bf(fmaxf(acc[h][0], 0.f)) | ((unsigned)f2bf(fmaxf(acc[h][1], 0.f)) << 16);
+                        o2[1] = (unsigned)f2bf(fmaxf(acc[h][2], 0.f)) | ((unsigned)f2bf(fmaxf(acc[h][3], 0.f)) << 16);
+                        *(mf_u2 *)(aout + n[h] * 16 + 4 * g) = o2;
+                    }
+                }
             }
             __syncthreads();
         }

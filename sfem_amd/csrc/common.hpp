@@ -33,7 +33,11 @@ void set_last_error(const std::string &msg);
         }                                                          \
     } while (0)
 
-inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+// The library's stream for calls that pass no stream (NULL): one blocking stream per device with
+// the default priority, created on first use.  Being a blocking stream it stays ordered with work on
+// the legacy default stream, and unlike that stream it can be captured into a HIP graph.
+hipStream_t library_stream();
+inline hipStream_t as_stream(void *s) { return s ? reinterpret_cast<hipStream_t>(s) : library_stream(); }
 
 // ----------------------------------------------------------------- structured helpers
 // operators/sshex8/sshex8.hpp:18-24: local id = z*(L+1)^2 + y*(L+1) + x

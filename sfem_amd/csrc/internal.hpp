@@ -49,6 +49,10 @@ inline double *reduction_ws() {
     return ws;
 }
 
+inline int fill_zero(void *p, size_t bytes, hipStream_t s) {
+    return blas_values<float>((ptrdiff_t)(bytes / sizeof(float)), 0.f, (float *)p, s);
+}
+
 inline uint16_t to_bf16(float f) {
     // round to nearest even
     uint32_t u;
@@ -96,7 +100,7 @@ struct sfem_amd_mesh {
     mutable sfem_amd::DevArray<uint8_t> own;     // owner mask (lazy)
     const sfem_amd_mesh *parent{nullptr};        // set by sfem_amd_mesh_derefine
     // sshex8: per-node skeleton slot lists (lazy, see mesh_skeleton)
-    mutable sfem_amd::DevArray<int32_t> sk_node, sk_off, sk_ent;
+    mutable sfem_amd::DevArray<int32_t> sk_node, sk_off, sk_pos;
     mutable ptrdiff_t sk_n{-1};
 };
 

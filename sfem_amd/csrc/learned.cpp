@@ -245,10 +245,10 @@ struct LearnedSmootherOp final : OpBase {
         T *x = (T *)xv;
         const int rt = real_type();
         for (int it = 0; it < steps; it++) {
-            SFEM_AMD_HIP_CHECK(hipMemsetAsync(r.p, 0, sizeof(T) * n, s));
+            if (blas_values<T>(n, T(0), r.p, s)) return SFEM_FAILURE;
             if (A->apply(x, r.p, s)) return SFEM_FAILURE;
             if (blas_axpby<T>(n, T(1), b, T(-1), r.p, s)) return SFEM_FAILURE;  // r = b - A x
-            SFEM_AMD_HIP_CHECK(hipMemsetAsync(corr.p, 0, sizeof(T) * n, s));
+            if (blas_values<T>(n, T(0), corr.p, s)) return SFEM_FAILURE;
             for (int c = 0; c < bs; c++) {
                 if (gather_elements<T>(mesh->E, mesh->nv, mesh->idx.p, bs, c, r.p, invdiag.p, z.p, s)) return SFEM_FAILURE;
                 if (sfem_amd_cnn_forward(cnn, mesh->level, mesh->E, z.p, out.p, mode, s)) return SFEM_FAILURE;

@@ -23,6 +23,19 @@ namespace sfem_amd {
 static thread_local std::string g_last_error;
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 
+hipStream_t library_stream() {
+    constexpr int kMaxDevices = 64;
+    static hipStream_t streams[kMaxDevices] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    if (!streams[dev]) {
+        hipStream_t st = nullptr;
+        if (hipStreamCreate(&st) != hipSuccess) return nullptr;
+        streams[dev] = st;
+    }
+    return streams[dev];
+}
+
 }  // namespace sfem_amd
 
 namespace sfem_amd {
@@ -62,14 +75,15 @@ int mesh_skeleton(const sfem_amd_mesh *m) {
             node.push_back((int32_t)g);
             off.push_back(off.back() + cnt[g]);
         }
-    std::vector<int32_t> fill(node.size(), 0), ent((size_t)E * nsk);
+    // slot (e, q) -> its position in the node-sorted partials (elements in ascending order)
+    std::vector<int32_t> fill(node.size(), 0), slot_pos((size_t)E * nsk);
     for (ptrdiff_t e = 0; e < E; e++)
         for (int q = 0; q < nsk; q++) {
             const int32_t p = pos[el[(size_t)e * m->nv + sk_lidx[q]]];
-            ent[(size_t)off[p] + fill[p]++] = (int32_t)(e * nsk + q);
+            slot_pos[(size_t)e * nsk + q] = off[p] + fill[p]++;
         }
     if (m->sk_node.upload(node.data(), node.size()) || m->sk_off.upload(off.data(), off.size()) ||
-        m->sk_ent.upload(ent.data(), ent.size()))
+        m->sk_pos.upload(slot_pos.data(), slot_pos.size()))
         return SFEM_FAILURE;
     m->sk_n = (ptrdiff_t)node.size();
     return SFEM_SUCCESS;
@@ -78,7 +92,7 @@ static SkeletonMap skeleton_of(const sfem_amd_mesh *m) {
     SkeletonMap sk;
     sk.node = m->sk_node.p;
     sk.off = m->sk_off.p;
-    sk.ent = m->sk_ent.p;
+    sk.pos = m->sk_pos.p;
     sk.n = m->sk_n;
     return sk;
 }
@@ -96,6 +110,7 @@ struct LaplacianOp final : OpBase {
     DevArray<uint16_t> fff16;   // SoA [6][E] bf16 (FP32 path, coefficient folded in)
     DevArray<float> coeff;
     DevArray<T> part;  // sshex8: skeleton contributions [E][sk_count(L)]
+    DevArray<double> Kem;  // sshex8 stored sub-element matrices [E][64] (instead of the FFF)
     int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
     ptrdiff_t rows() const override { return m->nnodes; }
 
@@ -112,12 +127,12 @@ struct LaplacianOp final : OpBase {
                 case SFEM_AMD_HEX8:
                     return hex8_laplacian_apply(m->E, m->idx.p, fff32.p, coeff.p, x, y, s);
                 default:
-                    return sshex8_laplacian_apply<double, float>(m->level, m->E, m->idx.p, fff32.p, m->E, coeff.p, x,
-                                                                 y, part.p, skeleton_of(m), s);
+                    return sshex8_laplacian_apply<double, float>(m->level, m->E, m->idx.p, fff32.p, m->E, coeff.p,
+                                                                 Kem.p, x, y, part.p, skeleton_of(m), s);
             }
         } else {
-            return sshex8_laplacian_apply<float, uint16_t>(m->level, m->E, m->idx.p, fff16.p, m->E, nullptr, x, y,
-                                                           part.p, skeleton_of(m), s);
+            return sshex8_laplacian_apply<float, uint16_t>(m->level, m->E, m->idx.p, fff16.p, m->E, nullptr, Kem.p,
+                                                           x, y, part.p, skeleton_of(m), s);
         }
     }
     int diag(void *dv, hipStream_t s) override {
@@ -129,10 +144,12 @@ struct LaplacianOp final : OpBase {
                 case SFEM_AMD_HEX8:
                     return hex8_laplacian_diag(m->E, m->idx.p, fff32.p, coeff.p, d, s);
                 default:
-                    return sshex8_laplacian_diag<double, float>(m->level, m->E, m->idx.p, fff32.p, m->E, coeff.p, d, s);
+                    return sshex8_laplacian_diag<double, float>(m->level, m->E, m->idx.p, fff32.p, m->E, coeff.p,
+                                                                Kem.p, d, s);
             }
         } else {
-            return sshex8_laplacian_diag<float, uint16_t>(m->level, m->E, m->idx.p, fff16.p, m->E, nullptr, d, s);
+            return sshex8_laplacian_diag<float, uint16_t>(m->level, m->E, m->idx.p, fff16.p, m->E, nullptr, Kem.p, d,
+                                                          s);
         }
     }
 };
@@ -269,7 +286,7 @@ struct ChainOp final : OpBase {
     int apply(const void *x, void *y, hipStream_t s) override {
         const size_t bytes = (size_t)A->rows() * (real_type() == SFEM_AMD_FLOAT64 ? 8 : 4);
         if (tmp.n != bytes && tmp.alloc(bytes)) return SFEM_FAILURE;
-        SFEM_AMD_HIP_CHECK(hipMemsetAsync(tmp.p, 0, bytes, s));
+        if (fill_zero(tmp.p, bytes, s)) return SFEM_FAILURE;
         if (A->apply(x, tmp.p, s)) return SFEM_FAILURE;
         return B->apply(tmp.p, y, s);
     }
@@ -281,7 +298,7 @@ struct OpView final : sfem::Operator<T> {
     std::shared_ptr<OpBase> impl;
     hipStream_t s{nullptr};
     explicit OpView(std::shared_ptr<OpBase> o) : impl(std::move(o)) {}
-    int apply(const T *const x, T *const y) override { return impl->apply(x, y, s); }
+    int apply(const T *const x, T *const y) override { return impl->apply(x, y, as_stream(s)); }
     std::ptrdiff_t rows() const override { return impl->rows(); }
     std::ptrdiff_t cols() const override { return impl->cols(); }
     sfem::ExecutionSpace execution_space() const override { return sfem::EXECUTION_SPACE_DEVICE; }
@@ -319,7 +336,7 @@ struct PrecisionBridge final : OpBase {
         const ptrdiff_t n = rows();
         if ((ptrdiff_t)xf.n != n && (xf.alloc(n) || yf.alloc(n))) return SFEM_FAILURE;
         if (blas_cast<double, float>(n, (const double *)x, xf.p, s)) return SFEM_FAILURE;
-        SFEM_AMD_HIP_CHECK(hipMemsetAsync(yf.p, 0, sizeof(float) * n, s));
+        if (blas_values<float>(n, 0.f, yf.p, s)) return SFEM_FAILURE;
         if (op32->apply(xf.p, yf.p, s)) return SFEM_FAILURE;
         return cast_accumulate(n, yf.p, (double *)y, s);  // y += (double) yf
     }
@@ -338,7 +355,7 @@ struct PrecisionBridgeDown final : OpBase {
         if ((ptrdiff_t)xd.n != m && xd.alloc(m)) return SFEM_FAILURE;
         if ((ptrdiff_t)yd.n != n && yd.alloc(n)) return SFEM_FAILURE;
         if (blas_cast<float, double>(m, (const float *)x, xd.p, s)) return SFEM_FAILURE;
-        SFEM_AMD_HIP_CHECK(hipMemsetAsync(yd.p, 0, sizeof(double) * n, s));
+        if (blas_values<double>(n, 0.0, yd.p, s)) return SFEM_FAILURE;
         if (op64->apply(xd.p, yd.p, s)) return SFEM_FAILURE;
         return cast_accumulate_f32(n, yd.p, (float *)y, s);
     }
@@ -388,6 +405,71 @@ bool is_constrained(const OpBase *op) { return dynamic_cast<const ConstrainedOp 
 }  // namespace sfem_amd
 
 namespace {
+// y += M x with a dense FP64 matrix (coarsest-level direct solve through a precomputed inverse).
+template <typename T>
+struct DenseOp final : OpBase {
+    DevArray<double> M;
+    ptrdiff_t n{0};
+    int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
+    ptrdiff_t rows() const override { return n; }
+    int apply(const void *x, void *y, hipStream_t s) override {
+        return dense_matvec<T>(n, M.p, (const T *)x, (T *)y, s);
+    }
+};
+
+// Replays the apply of `inner` as a HIP graph.  The first call runs eagerly (lazy allocations,
+// one-time kernel attributes); the second captures the launch sequence on its stream and
+// launches the graph; later calls with the same (x, y, stream) relaunch it, other arguments run
+// eagerly.  The inner apply must not synchronise with the host (e.g. a preconditioning V-cycle
+// with a direct coarse solve).  A failed capture leaves the operator eager.
+struct GraphOp final : OpBase {
+    std::shared_ptr<OpBase> inner;
+    hipGraphExec_t exec{nullptr};
+    const void *cx{nullptr};
+    void *cy{nullptr};
+    hipStream_t cs{nullptr};
+    int calls{0}, replays{0};
+    bool failed{false};
+    ~GraphOp() override {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+    int real_type() const override { return inner->real_type(); }
+    ptrdiff_t rows() const override { return inner->rows(); }
+    ptrdiff_t cols() const override { return inner->cols(); }
+    int iterations() const override { return inner->iterations(); }
+    int apply(const void *x, void *y, hipStream_t s) override {
+        if (calls++ == 0 || failed) return inner->apply(x, y, s);
+        if (!exec) {
+            if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                failed = true;
+                (void)hipGetLastError();
+                return inner->apply(x, y, s);
+            }
+            const int err = inner->apply(x, y, s);
+            hipGraph_t graph = nullptr;
+            const hipError_t e_end = hipStreamEndCapture(s, &graph);
+            if (err || e_end != hipSuccess || !graph ||
+                hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                exec = nullptr;
+                failed = true;
+                (void)hipGetLastError();
+                return inner->apply(x, y, s);  // nothing ran during the capture: run it now
+            }
+            (void)hipGraphDestroy(graph);
+            cx = x;
+            cy = y;
+            cs = s;
+        }
+        if (x == cx && y == cy && s == cs) {
+            replays++;
+            SFEM_AMD_HIP_CHECK(hipGraphLaunch(exec, s));
+            return SFEM_SUCCESS;
+        }
+        return inner->apply(x, y, s);
+    }
+};
+
 // Runs one of two operators of the same shape; the CG fallback hook selects the second one
 // when the primary (e.g. a learned smoother) stops reducing the residual.
 struct SwitchOp final : OpBase {
@@ -577,6 +659,36 @@ int sfem_amd_laplacian_create(const sfem_amd_mesh *mesh, const float *fff, const
     if (op->fff16.upload(h.data(), h.size())) return SFEM_FAILURE;
     *out = wrap(op);
     return SFEM_SUCCESS;
+}
+
+int sfem_amd_element_matrix_op_create(const sfem_amd_mesh *mesh, const double *matrices, int bs, int rt,
+                                      sfem_amd_op **out) {
+    CHECK_HANDLE(mesh);
+    SFEM_AMD_REQUIRE(matrices && out, "element_matrix_op_create: arguments");
+    SFEM_AMD_REQUIRE(mesh->type == SFEM_AMD_SSHEX8, "element_matrix_op_create: sshex8 meshes only");
+    SFEM_AMD_REQUIRE(bs == 1 || bs == 3, "element_matrix_op_create: block size 1 (8x8) or 3 (24x24)");
+    SFEM_AMD_REQUIRE(rt == SFEM_AMD_FLOAT64 || rt == SFEM_AMD_FLOAT32, "element_matrix_op_create: real type");
+    const ptrdiff_t E = mesh->E;
+    auto finish = [&](auto op, DevArray<double> &dst, size_t per) -> int {
+        op->m = mesh;
+        if (dst.upload(matrices, (size_t)E * per)) return SFEM_FAILURE;
+        *out = wrap(op);
+        return SFEM_SUCCESS;
+    };
+    if (bs == 1) {
+        if (rt == SFEM_AMD_FLOAT64) {
+            auto op = std::make_shared<LaplacianOp<double>>();
+            return finish(op, op->Kem, 64);
+        }
+        auto op = std::make_shared<LaplacianOp<float>>();
+        return finish(op, op->Kem, 64);
+    }
+    if (rt == SFEM_AMD_FLOAT64) {
+        auto op = std::make_shared<ElasticityOp<double>>();
+        return finish(op, op->K, 576);
+    }
+    auto op = std::make_shared<ElasticityOp<float>>();
+    return finish(op, op->K, 576);
 }
 
 int sfem_amd_linear_elasticity_create(const sfem_amd_mesh *mesh, const float *adj, const float *det, double mu,
@@ -1112,6 +1224,37 @@ int sfem_amd_op_benchmark(sfem_amd_op *op, const void *x, void *y, int warmup, i
     for (auto &e : ev) (void)hipEventDestroy(e);
     *total_ms = total;
     return err;
+}
+
+int sfem_amd_dense_op_create(int rt, ptrdiff_t n, const double *matrix, sfem_amd_op **out) {
+    SFEM_AMD_REQUIRE(matrix && out && n > 0, "dense_op_create: arguments");
+    SFEM_AMD_REQUIRE(rt == SFEM_AMD_FLOAT64 || rt == SFEM_AMD_FLOAT32, "dense_op_create: real type");
+    auto make = [&](auto op) -> int {
+        op->n = n;
+        if (op->M.upload(matrix, (size_t)n * n)) return SFEM_FAILURE;
+        *out = wrap(op);
+        return SFEM_SUCCESS;
+    };
+    if (rt == SFEM_AMD_FLOAT64) return make(std::make_shared<DenseOp<double>>());
+    return make(std::make_shared<DenseOp<float>>());
+}
+
+int sfem_amd_graph_op_create(sfem_amd_op *op, sfem_amd_op **out) {
+    CHECK_HANDLE(op);
+    SFEM_AMD_REQUIRE(out, "graph_op_create: out");
+    auto g = std::make_shared<GraphOp>();
+    g->inner = op->impl;
+    *out = wrap(g);
+    return SFEM_SUCCESS;
+}
+
+int sfem_amd_graph_op_stats(const sfem_amd_op *op, int *captured, int *replays) {
+    CHECK_HANDLE(op);
+    auto *g = dynamic_cast<const GraphOp *>(op->impl.get());
+    SFEM_AMD_REQUIRE(g, "graph_op_stats: not a graph operator");
+    if (captured) *captured = g->exec ? 1 : (g->failed ? -1 : 0);
+    if (replays) *replays = g->replays;
+    return SFEM_SUCCESS;
 }
 
 int sfem_amd_switch_create(sfem_amd_op *primary, sfem_amd_op *fallback, sfem_amd_op **out) {

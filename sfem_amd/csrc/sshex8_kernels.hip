@@ -47,13 +47,17 @@ struct LapShape {
 template <int L, int EPB, int BLOCK, typename G>
 __device__ __forceinline__ void lap_block_matrices(ptrdiff_t E, ptrdiff_t e0, const G *__restrict__ fff,
                                                    ptrdiff_t fstride, const float *__restrict__ coeff,
-                                                   double (*sK)[64]) {
+                                                   const double *__restrict__ Kem, double (*sK)[64]) {
     const double h = 1.0 / L;
     for (int k = threadIdx.x; k < EPB * 64; k += BLOCK) {
         const int le = k >> 6, ij = k & 63;
         const ptrdiff_t e = e0 + le;
         double v = 0;
-        if (e < E) {
+        if (e < E && Kem) {
+            // stored sub-element matrices [E][64], row-major in hex8 corner order
+            // (operators/stencil/sshex8_stencil_element_matrix_apply.cpp:10-78)
+            v = Kem[e * 64 + ij];
+        } else if (e < E) {
             double f[6];
             const double c = (coeff ? (double)coeff[e] : 1.0) * h;
 #pragma unroll
@@ -78,7 +82,8 @@ template <typename T, typename G, int L>
 __global__ void __launch_bounds__(LapShape<L>::BLOCK)
     sshex8_laplacian_apply_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx, const G *__restrict__ fff,
                                   const ptrdiff_t fstride, const float *__restrict__ coeff,
-                                  const T *__restrict__ x, T *__restrict__ y, T *__restrict__ part) {
+                                  const double *__restrict__ Kem, const T *__restrict__ x, T *__restrict__ y,
+                                  T *__restrict__ part, const int32_t *__restrict__ sk_pos) {
     using S = LapShape<L>;
     constexpr int N1 = S::N1, NXE = S::NXE, LINES = S::LINES, EPB = S::EPB, NSK = sk_count(L);
     __shared__ T su[EPB][NXE];
@@ -106,7 +111,7 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
             for (int i = 1; i < L; i++) old[i] = y[gid[i]];
         }
     }
-    lap_block_matrices<L, EPB, S::BLOCK, G>(E, e0, fff, fstride, coeff, sK);
+    lap_block_matrices<L, EPB, S::BLOCK, G>(E, e0, fff, fstride, coeff, Kem, sK);
     __syncthreads();
     if (!active) return;
 
@@ -158,7 +163,16 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
             w[2][a][b] = u[lidx(L, 1, ys[b], zs[a])];
         }
 
-    T *pe = part + e * NSK + sk_line_base(L, yy, zz);
+    // part positions of this line's skeleton nodes (all L+1 on a skeleton line, else x = 0, L)
+    const int32_t *pp = sk_pos + e * NSK + sk_line_base(L, yy, zz);
+    int32_t ppos[N1];
+    if (skl) {
+#pragma unroll
+        for (int i = 0; i <= L; i++) ppos[i] = pp[i];
+    } else {
+        ppos[0] = pp[0];
+        ppos[L] = pp[1];
+    }
 #pragma unroll
     for (int i = 0; i <= L; i++) {
         T v = 0;
@@ -174,12 +188,8 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
 #pragma unroll
                 for (int b = 0; b < 3; b++) v += Sb[a][b][0] * w[0][a][b] + Sb[a][b][1] * w[1][a][b];
         }
-        if (skl)
-            pe[i] = v;
-        else if (i == 0)
-            pe[0] = v;
-        else if (i == L)
-            pe[1] = v;
+        if (skl || i == 0 || i == L)
+            part[ppos[i]] = v;
         else
             y[gid[i]] = old[i] + v;
         // slide
@@ -201,17 +211,16 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
 // element contributions, in slot order.
 template <typename T, int BS>
 __global__ void skeleton_gather_kernel(const ptrdiff_t n, const int32_t *__restrict__ node,
-                                       const int32_t *__restrict__ off, const int32_t *__restrict__ ent,
-                                       const T *__restrict__ part, T *__restrict__ y) {
+                                       const int32_t *__restrict__ off, const T *__restrict__ part,
+                                       T *__restrict__ y) {
     for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
         const int j0 = off[t], j1 = off[t + 1];
         T acc[BS];
 #pragma unroll
         for (int c = 0; c < BS; c++) acc[c] = 0;
         for (int j = j0; j < j1; j++) {
-            const ptrdiff_t q = (ptrdiff_t)ent[j] * BS;
 #pragma unroll
-            for (int c = 0; c < BS; c++) acc[c] += part[q + c];
+            for (int c = 0; c < BS; c++) acc[c] += part[(ptrdiff_t)j * BS + c];
         }
         const ptrdiff_t g = (ptrdiff_t)node[t] * BS;
 #pragma unroll
@@ -224,7 +233,8 @@ __global__ void skeleton_gather_kernel(const ptrdiff_t n, const int32_t *__restr
 template <typename T, typename G, int L>
 __global__ void __launch_bounds__(LapShape<L>::BLOCK)
     sshex8_laplacian_diag_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx, const G *__restrict__ fff,
-                                 const ptrdiff_t fstride, const float *__restrict__ coeff, T *__restrict__ d) {
+                                 const ptrdiff_t fstride, const float *__restrict__ coeff,
+                                 const double *__restrict__ Kem, T *__restrict__ d) {
     using S = LapShape<L>;
     constexpr int N1 = S::N1, NXE = S::NXE, LINES = S::LINES, EPB = S::EPB;
     __shared__ double sK[EPB][64];
@@ -235,7 +245,7 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
     const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
     const ptrdiff_t e = e0 + le;
     const bool active = (le < EPB) && (e < E);
-    lap_block_matrices<L, EPB, S::BLOCK, G>(E, e0, fff, fstride, coeff, sK);
+    lap_block_matrices<L, EPB, S::BLOCK, G>(E, e0, fff, fstride, coeff, Kem, sK);
     __syncthreads();
     if (!active) return;
     const double *K = sK[le];
@@ -292,7 +302,7 @@ template <typename T, int L>
 __global__ void __launch_bounds__(ElShape<L>::BLOCK)
     sshex8_elasticity_apply_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
                                    const T *__restrict__ Kall, const T *__restrict__ x,
-                                   T *__restrict__ y, T *__restrict__ part) {
+                                   T *__restrict__ y, T *__restrict__ part, const int32_t *__restrict__ sk_pos) {
     using S = ElShape<L>;
     constexpr int N1 = S::N1, NXE = N1 * N1 * N1, PLANE = S::PLANE, EPB = S::EPB, NSK = sk_count(L);
     __shared__ T su[EPB][3][PLANE][3];
@@ -327,7 +337,7 @@ __global__ void __launch_bounds__(ElShape<L>::BLOCK)
     };
     load_plane(0);
     __syncthreads();
-    T *pe = part + e * (ptrdiff_t)NSK * 3;
+    const int32_t *pe = sk_pos + e * (ptrdiff_t)NSK;
 
     for (int z = 0; z <= L; z++) {
         const int32_t g = g_next;
@@ -374,7 +384,7 @@ __global__ void __launch_bounds__(ElShape<L>::BLOCK)
 #pragma unroll
                 for (int ci = 0; ci < 3; ci++) y[(ptrdiff_t)g * 3 + ci] = yold[ci] + acc[ci];
             } else {
-                T *q = pe + (ptrdiff_t)sk_slot(L, xx, yy, z) * 3;
+                T *q = part + (ptrdiff_t)pe[sk_slot(L, xx, yy, z)] * 3;
 #pragma unroll
                 for (int ci = 0; ci < 3; ci++) q[ci] = acc[ci];
             }
@@ -431,9 +441,9 @@ template <typename T>
 int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStream_t s) {
     if (sk.n == 0) return SFEM_SUCCESS;
     if (bs == 1)
-        skeleton_gather_kernel<T, 1><<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, sk.ent, part, y);
+        skeleton_gather_kernel<T, 1><<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, part, y);
     else if (bs == 3)
-        skeleton_gather_kernel<T, 3><<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, sk.ent, part, y);
+        skeleton_gather_kernel<T, 3><<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, part, y);
     else {
         set_last_error("skeleton_gather: block size must be 1 or 3");
         return SFEM_FAILURE;
@@ -444,14 +454,16 @@ int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStrea
 
 template <typename T, typename G>
 int sshex8_laplacian_apply(int L, ptrdiff_t E, const int32_t *idx, const G *fff, ptrdiff_t fstride,
-                           const float *coeff, const T *x, T *y, T *part, const SkeletonMap &sk, hipStream_t s) {
+                           const float *coeff, const double *Kem, const T *x, T *y, T *part, const SkeletonMap &sk,
+                           hipStream_t s) {
     if (E == 0) return SFEM_SUCCESS;
     switch (L) {
 #define CASE(LV)                                                                                                  \
     case LV: {                                                                                                    \
         using Sh = LapShape<LV>;                                                                                  \
         const unsigned grid = (unsigned)((E + Sh::EPB - 1) / Sh::EPB);                                            \
-        sshex8_laplacian_apply_kernel<T, G, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, fff, fstride, coeff, x, y, part); \
+        sshex8_laplacian_apply_kernel<T, G, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, fff, fstride, coeff, Kem, x, y, part, \
+                                                                             sk.pos);                                \
         break;                                                                                                    \
     }
         SFEM_AMD_SSHEX8_LEVELS(CASE)
@@ -466,14 +478,14 @@ int sshex8_laplacian_apply(int L, ptrdiff_t E, const int32_t *idx, const G *fff,
 
 template <typename T, typename G>
 int sshex8_laplacian_diag(int L, ptrdiff_t E, const int32_t *idx, const G *fff, ptrdiff_t fstride,
-                          const float *coeff, T *d, hipStream_t s) {
+                          const float *coeff, const double *Kem, T *d, hipStream_t s) {
     if (E == 0) return SFEM_SUCCESS;
     switch (L) {
 #define CASE(LV)                                                                                          \
     case LV: {                                                                                            \
         using Sh = LapShape<LV>;                                                                          \
         const unsigned grid = (unsigned)((E + Sh::EPB - 1) / Sh::EPB);                                    \
-        sshex8_laplacian_diag_kernel<T, G, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, fff, fstride, coeff, d); \
+        sshex8_laplacian_diag_kernel<T, G, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, fff, fstride, coeff, Kem, d); \
         break;                                                                                            \
     }
         SFEM_AMD_SSHEX8_LEVELS(CASE)
@@ -506,7 +518,7 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
     case LV: {                                                                                   \
         using Sh = ElShape<LV>;                                                                  \
         const unsigned grid = (unsigned)((E + Sh::EPB - 1) / Sh::EPB);                           \
-        sshex8_elasticity_apply_kernel<T, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, K, x, y, part); \
+        sshex8_elasticity_apply_kernel<T, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, K, x, y, part, sk.pos); \
         break;                                                                                   \
     }
         SFEM_AMD_SSHEX8_LEVELS(CASE)
@@ -542,15 +554,15 @@ int sshex8_elasticity_diag(int L, ptrdiff_t E, const int32_t *idx, const double 
 template int skeleton_gather<double>(const SkeletonMap &, int, const double *, double *, hipStream_t);
 template int skeleton_gather<float>(const SkeletonMap &, int, const float *, float *, hipStream_t);
 template int sshex8_laplacian_apply<double, float>(int, ptrdiff_t, const int32_t *, const float *, ptrdiff_t,
-                                                   const float *, const double *, double *, double *,
+                                                   const float *, const double *, const double *, double *, double *,
                                                    const SkeletonMap &, hipStream_t);
 template int sshex8_laplacian_apply<float, uint16_t>(int, ptrdiff_t, const int32_t *, const uint16_t *, ptrdiff_t,
-                                                     const float *, const float *, float *, float *,
+                                                     const float *, const double *, const float *, float *, float *,
                                                      const SkeletonMap &, hipStream_t);
 template int sshex8_laplacian_diag<double, float>(int, ptrdiff_t, const int32_t *, const float *, ptrdiff_t,
-                                                  const float *, double *, hipStream_t);
+                                                  const float *, const double *, double *, hipStream_t);
 template int sshex8_laplacian_diag<float, uint16_t>(int, ptrdiff_t, const int32_t *, const uint16_t *, ptrdiff_t,
-                                                    const float *, float *, hipStream_t);
+                                                    const float *, const double *, float *, hipStream_t);
 template int sshex8_elasticity_matrices<float>(int, ptrdiff_t, const float *, const float *, ptrdiff_t, double,
                                                double, const float *, double *, hipStream_t);
 template int sshex8_elasticity_matrices<uint16_t>(int, ptrdiff_t, const uint16_t *, const uint16_t *, ptrdiff_t,

@@ -279,6 +279,21 @@ __global__ void cheb3_update_diag_kernel(const ptrdiff_t n, const T alpha, const
     }
 }
 
+// y += M x with a dense row-major FP64 matrix: one wave64 per row, fixed-order shuffle tree
+// (coarsest-level direct solve with a precomputed inverse).
+template <typename T>
+__global__ void __launch_bounds__(256) dense_matvec_kernel(const ptrdiff_t n, const double *__restrict__ M,
+                                                           const T *__restrict__ x, T *__restrict__ y) {
+    const ptrdiff_t row = (ptrdiff_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;  // uniform per wave
+    double v = 0;
+    const double *Mr = M + row * n;
+    for (ptrdiff_t j = lane; j < n; j += 64) v += Mr[j] * (double)x[j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) y[row] += (T)v;
+}
+
 // Deterministic reductions: a fixed grid (a function of n only) of 256-thread blocks,
 // wave64 shuffle trees and one pass over the partials in a fixed order.
 constexpr int kRedBlocks = 1024;
@@ -425,6 +440,14 @@ int cheb3_update_diag(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, c
 }
 
 template <typename T>
+int dense_matvec(ptrdiff_t n, const double *M, const T *x, T *y, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    dense_matvec_kernel<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(n, M, x, y);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template <typename T>
 int blas_dot(ptrdiff_t n, const T *x, const T *y, double *ws, double *result_host, hipStream_t s) {
     const int nb = red_blocks(n);
     dot_partial_kernel<T><<<nb, kBlock, 0, s>>>(n, x, y, ws);
@@ -513,6 +536,7 @@ int cast_accumulate(ptrdiff_t n, const float *x, double *y, hipStream_t s) {
     template int blas_xypaz<T>(ptrdiff_t, const T *, const T *, T, T *, hipStream_t);                   \
     template int cheb3_update<T>(ptrdiff_t, T, T, const T *, const T *, T *, T *, hipStream_t);         \
     template int cheb3_update_diag<T>(ptrdiff_t, T, T, const T *, const T *, const T *, T *, T *, hipStream_t); \
+    template int dense_matvec<T>(ptrdiff_t, const double *, const T *, T *, hipStream_t);                \
     template int blas_dot<T>(ptrdiff_t, const T *, const T *, double *, double *, hipStream_t);         \
     template int cg_update<T>(ptrdiff_t, T, const T *, const T *, T *, T *, double *, double *, hipStream_t); \
     template int dofs_set_value<T>(ptrdiff_t, const int32_t *, T, T *, hipStream_t);                    \

@@ -27,6 +27,8 @@ template <typename T> int blas_reciprocal(ptrdiff_t n, T a, T *x, hipStream_t s)
 template <typename T> int blas_xypaz(ptrdiff_t n, const T *x, const T *y, T a, T *z, hipStream_t s);
 template <typename S, typename D> int blas_cast(ptrdiff_t n, const S *x, D *y, hipStream_t s);
 template <typename T> int cheb3_update(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, T *p, T *x, hipStream_t s);
+// y += M x, M dense row-major FP64 n x n
+template <typename T> int dense_matvec(ptrdiff_t n, const double *M, const T *x, T *y, hipStream_t s);
 template <typename T>
 int cheb3_update_diag(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, const T *dinv, T *p, T *x,
                       hipStream_t s);

@@ -24,7 +24,8 @@ FP64 coarse CG then runs on a few dozen dofs instead of the whole macro grid.
 import numpy as np
 
 from . import FLOAT32, FLOAT64, HEX8, SSHEX8
-from . import (DeviceArray, Mesh, Multigrid, adjugate_det, box_mesh, cg, chain, cheb3, constrained, fff,
+from . import (DeviceArray, Mesh, Multigrid, adjugate_det, box_mesh, cg, chain, cheb3, constrained, dense_op, fff,
+               graph_op,
                inverse_diagonal, jacobi, laplacian, linear_elasticity, permutation, precision_bridge, prolongation,
                restriction, set_constrained_value, solver_bridge, sshex8_refine, switch, switch_select)
 
@@ -60,8 +61,13 @@ class SSGMG:
     def __init__(self, fine_mesh, kind, geometry, bc_nodes, coeff=None, real_type=FLOAT64, smoother="cheb3",
                  cheb_degree=3, eig_its=1000, eig_tol=1e-5, scale_eig_max=1.02, coarse_rtol=1e-8,
                  coarse_max_it=10000, mu=None, lam=None, bc_components=None, max_it=1, atol=0.0, macro_box=None,
-                 learned=None, learned_levels=1, learned_steps=1, learned_mode=1, learned_damping=1.0):
+                 learned=None, learned_levels=1, learned_steps=1, learned_mode=1, learned_damping=1.0,
+                 coarse="cg"):
         self.kind = kind
+        # coarsest solve: "cg" (the reference's FP64 PCG with inverse-diagonal preconditioning) or
+        # "dense" (a precomputed inverse applied as a dense matrix: no host synchronisation, so the
+        # V-cycle can be replayed as a HIP graph, see preconditioner(graph=True))
+        self.coarse = coarse
         # learned smoother (chosen form 4): a CNN replacing Chebyshev3 on the `learned_levels`
         # finest sshex8 levels (micro-grid level <= 8), each wrapped in a switch whose fallback
         # is the Chebyshev3 smoother (see sfem_amd.learned)
@@ -185,20 +191,57 @@ class SSGMG:
                 lv.smoother = switch(lv.learned, lv.cheb)
                 self.switches.append(lv.smoother)
         c = self.levels[-1]
-        d = DeviceArray(c.n, np.float64)
-        c.A.diag(d)
-        c.precond = inverse_diagonal(d, 1.0)
-        c.smoother = cg(c.A, c.precond, rtol=coarse_rtol, atol=1e-30, max_it=coarse_max_it)
+        if self.coarse == "dense":
+            c.smoother = dense_op(self._dense_inverse(c), real_type=self.real_type)
+            c.dense = True
+        else:
+            d = DeviceArray(c.n, np.float64)
+            c.A.diag(d)
+            c.precond = inverse_diagonal(d, 1.0)
+            c.smoother = cg(c.A, c.precond, rtol=coarse_rtol, atol=1e-30, max_it=coarse_max_it)
         self.mg = Multigrid(self.real_type, 1, max_it, atol)
         last = len(self.levels) - 1
         for i, lv in enumerate(self.levels):
             A, sm = lv.A, lv.smoother
             if self.real_type == FLOAT32 and lv.op.real_type == FLOAT64:
-                # FP64-only levels (HEX8) inside an FP32 cycle: operator and smoother views
+                # FP64-only levels (HEX8) inside an FP32 cycle: operator and smoother views (the dense
+                # coarse solve is already in the cycle's precision)
                 A = precision_bridge(A)
-                sm = precision_bridge(sm) if i == last else solver_bridge(sm)
+                if not getattr(lv, "dense", False):
+                    sm = precision_bridge(sm) if i == last else solver_bridge(sm)
                 lv.bridges = (A, sm)
             self.mg.add_level(A, sm, getattr(lv, "P", None), getattr(lv, "R", None))
+
+    def _dense_inverse(self, lv, max_n=20000):
+        """Inverse of the constrained coarsest operator, assembled column by column on the device."""
+        n = lv.n
+        if n > max_n:
+            raise ValueError(f"dense coarse solve: {n} dofs on the coarsest level (limit {max_n})")
+        A = np.zeros((n, n))
+        dtype = np.float64 if lv.op.real_type == FLOAT64 else np.float32
+        e = DeviceArray(n, dtype)
+        y = DeviceArray(n, dtype)
+        col = np.zeros(n)
+        for j in range(n):
+            col[:] = 0
+            col[j] = 1
+            e.copy_from(col)
+            y.zero()
+            lv.A.apply(e, y)
+            A[:, j] = y.numpy()
+        return np.linalg.inv(A)
+
+    def preconditioner(self, graph=False, outer_fp64=True):
+        """The V-cycle as an operator for an outer CG: bridged to FP64 for an FP32 hierarchy when
+        outer_fp64, and replayed as a HIP graph when graph (needs coarse="dense")."""
+        P = self.mg
+        if self.real_type == FLOAT32 and outer_fp64:
+            P = precision_bridge(P)
+        if graph:
+            if self.coarse != "dense":
+                raise ValueError("graph replay needs the dense coarse solve (coarse='dense')")
+            P = graph_op(P)
+        return P
 
     @property
     def fine(self):

@@ -185,6 +185,27 @@ def test_sshex8_elasticity_parity(L, n):
         assert err < 1e-2
 
 
+# ---------------------------------------------------------------- stored element matrices (ss:em)
+@pytest.mark.parametrize("L,n,bs", [(2, 2, 1), (8, 2, 1), (16, 1, 1), (4, 2, 3), (16, 2, 3)])
+def test_sshex8_element_matrix_parity(L, n, bs):
+    mel, mpts = skewed_box(n, perturb=0.02)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    E = mel.shape[1]
+    rng = np.random.default_rng(L * 10 + bs)
+    K = rng.normal(size=(E, (8 * bs) ** 2))  # non-symmetric: pins the row/column convention
+    N = pts.shape[1] * bs
+    x = rng.uniform(-1, 1, N)
+    yo = np.zeros(N)
+    O.lib.oracle_sshex8_em_apply(L, E, el, bs, K, x, yo)
+    mesh = S.Mesh(S.SSHEX8, L, el, pts.shape[1])
+    op = S.element_matrix_op(mesh, K, bs)
+    assert relerr(host(gpu_apply(op, dev(x))), yo) < RTOL
+    op32 = S.element_matrix_op(mesh, K, bs, real_type=S.FLOAT32)
+    err = relerr(host(gpu_apply(op32, dev(x, np.float32))).astype(np.float64), yo)
+    print(f"L={L} bs={bs}: FP32 ss:em apply relative error vs FP64 oracle = {err:.3e}")
+    assert err < 1e-5
+
+
 # ---------------------------------------------------------------- transfers
 @pytest.mark.parametrize("vs", [1, 3])
 def test_transfers_parity(vs):

@@ -379,3 +379,62 @@ def test_dirichlet_helpers():
     y = np.zeros(12)
     O.lib.oracle_constraint_nodes_copy_vec(2, nodes, 3, 2, x, y)
     assert y[5] == 5 and y[11] == 11 and y.sum() == 16
+
+
+def _dense_micro_assembly(L, el, E, K, bs):
+    """Dense global matrix of the stored-matrix (ss:em) operator, assembled micro element by
+    micro element from the explicit corner lists (independent of the oracle's loops)."""
+    micro = _micro_hex8(L, el)  # [8][L^3 * E], macro e owns columns k*E + e
+    n = (el.max() + 1) * bs
+    A = np.zeros((n, n))
+    for col in range(micro.shape[1]):
+        e = col % E
+        nodes = micro[:, col]
+        dofs = np.array([nodes[v] * bs + c for c in range(bs) for v in range(8)])  # component*8 + node
+        A[np.ix_(dofs, dofs)] += K[e].reshape(8 * bs, 8 * bs)
+    return A
+
+
+@pytest.mark.parametrize("bs", [1, 3])
+def test_em_apply_vs_dense_assembly(bs):
+    """Random (non-symmetric) stored matrices: the oracle's ss:em apply equals the densely
+    assembled operator, so the corner order and the component*8 + node packing are pinned."""
+    L = 2
+    mel, mpts = _cube_problem("hex8", n=2)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    E = mel.shape[1]
+    K = RNG.normal(size=(E, (8 * bs) ** 2))
+    n = pts.shape[1] * bs
+    x = RNG.uniform(-1, 1, n)
+    y = np.zeros(n)
+    O.lib.oracle_sshex8_em_apply(L, E, el, bs, np.ascontiguousarray(K), x, y)
+    A = _dense_micro_assembly(L, el, E, K, bs)
+    np.testing.assert_allclose(y, A @ x, rtol=1e-12, atol=1e-12 * np.abs(A @ x).max())
+
+
+def test_em_apply_with_geometry_matrices_equals_operators():
+    """ss:em with the matrices sshex8_laplacian_element_matrix builds equals the FFF Laplacian;
+    with the elasticity sub-element matrices it equals the adjugate elasticity operator."""
+    L = 4
+    mel, mpts = _cube_problem("hex8", n=2)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    E = mel.shape[1]
+    fff = O.hex8_fff(mel, mpts)
+    coeff = RNG.uniform(0.5, 2.0, E).astype(np.float32)
+    K = np.zeros((E, 64))
+    O.lib.oracle_sshex8_laplacian_element_matrices(L, E, fff, coeff, K)
+    n = pts.shape[1]
+    x = RNG.uniform(-1, 1, n)
+    y1, y2 = np.zeros(n), np.zeros(n)
+    O.lib.oracle_sshex8_em_apply(L, E, el, 1, K, x, y1)
+    O.lib.oracle_sshex8_laplacian_apply(L, E, el, fff, coeff, x, y2)
+    np.testing.assert_allclose(y1, y2, rtol=1e-12, atol=1e-13 * np.abs(y2).max())
+    adj, det = O.hex8_adj_det(mel, mpts)
+    mu, lam = O.lame(1.0, 0.3)
+    K3 = np.zeros((E, 576))
+    O.lib.oracle_sshex8_elasticity_element_matrices(L, E, adj, det, mu, lam, coeff, K3)
+    x3 = RNG.uniform(-1, 1, 3 * n)
+    y1, y2 = np.zeros(3 * n), np.zeros(3 * n)
+    O.lib.oracle_sshex8_em_apply(L, E, el, 3, K3, x3, y1)
+    O.lib.oracle_sshex8_elasticity_apply(L, E, el, adj, det, mu, lam, coeff, x3, y2)
+    np.testing.assert_allclose(y1, y2, rtol=1e-12, atol=1e-13 * np.abs(y2).max())
