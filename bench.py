@@ -32,6 +32,11 @@ def build_problem(n_macro=16, L=8):
     return mel, mpts, el, pts, fff
 
 
+def progress(msg):
+    """Stage messages on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def algorithmic_bytes(N, E, L, value_bytes=8, geom_bytes=4):
     # one read of x, one write of y, int32 connectivity, 6 FFF words per macro element
     return 2 * value_bytes * N + 4 * (L + 1) ** 3 * E + geom_bytes * 6 * E
@@ -60,8 +65,11 @@ def cpu_baseline(el, fff, N, L, budget_s=12.0, max_reps=20):
 
 def solve_times(S, mesh, fff, pts, n_macro, rtol=1e-10, learned_steps=2):
     """Time to ||b - A x||/||b|| <= rtol (recomputed in FP64 with the FP64 operator) of
-      fp64:          FP64 MG-PCG (Chebyshev3 V-cycle, FP64 coarse PCG);
-      mixed:         the same V-cycle in FP32 with BF16 geometry inside FP64 flexible CG;
+      fp64_eager:    FP64 MG-PCG, Chebyshev3 V-cycle, the reference's FP64 coarse PCG, every
+                     kernel launched from the host;
+      fp64:          the same V-cycle with a direct (dense inverse) coarse solve, replayed as a HIP
+                     graph; PCG stopping on ||r||/||b||;
+      mixed:         that V-cycle in FP32 with BF16 geometry, graph-replayed, in FP64 flexible CG;
       mixed_learned: as mixed, with the learned CNN smoother on the finest level (trained at
                      setup on V-cycle pairs; Chebyshev3 fallback when an iteration reduces the
                      residual by less than 2x).
@@ -79,41 +87,48 @@ def solve_times(S, mesh, fff, pts, n_macro, rtol=1e-10, learned_steps=2):
         A.apply(x, r)
         return float(np.linalg.norm(rhs - r.numpy()) / np.linalg.norm(rhs))
 
-    t0 = time.perf_counter()
-    h64 = gmg.SSGMG(mesh, "laplacian", fff, bc, macro_box=(n_macro, 1.0))
-    setup64 = time.perf_counter() - t0
+    box = (n_macro, 1.0)
     hierarchy = "V-cycle 8-4-2-1 | macro 16^3 -> 8-4-2-1 super levels, Chebyshev3 degree 3"
-    for name in ("fp64", "mixed", "mixed_learned"):
+    t0 = time.perf_counter()
+    h_ref = gmg.SSGMG(mesh, "laplacian", fff, bc, macro_box=box)
+    setup_ref = time.perf_counter() - t0
+    A64 = h_ref.fine.A
+    h32 = None
+    for name in ("fp64_eager", "fp64", "mixed", "mixed_learned"):
+        progress(f"laplacian solve: {name}")
         switches = []
         extra = {}
-        if name == "fp64":
-            solver = S.cg(h64.fine.A, h64.mg, rtol=rtol, max_it=500, stop_on_residual=True)
-            setup = setup64
-            desc = hierarchy + ", FP64; FP64 coarse PCG; PCG stopping on ||r||/||b||"
-        elif name == "mixed":
-            t0 = time.perf_counter()
-            h32 = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, macro_box=(n_macro, 1.0))
-            P = S.precision_bridge(h32.mg)
+        t0 = time.perf_counter()
+        if name == "fp64_eager":
+            solver = S.cg(A64, h_ref.mg, rtol=rtol, max_it=500, stop_on_residual=True)
+            setup = setup_ref
+            desc = hierarchy + ", FP64; FP64 coarse PCG (host-synchronised); PCG stopping on ||r||/||b||"
+        elif name == "fp64":
+            h64 = gmg.SSGMG(mesh, "laplacian", fff, bc, macro_box=box, coarse="dense")
+            solver = S.cg(A64, h64.preconditioner(graph=True), rtol=rtol, max_it=500, stop_on_residual=True)
             setup = time.perf_counter() - t0
-            solver = S.cg(h64.fine.A, P, flexible=True, rtol=rtol, max_it=500)
-            desc = hierarchy + " in FP32 with BF16 geometry (hex8 levels bridged to FP64); outer FP64 flexible CG"
+            desc = hierarchy + ", FP64; dense direct coarse solve; V-cycle replayed as a HIP graph"
+        elif name == "mixed":
+            h32 = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, macro_box=box, coarse="dense")
+            solver = S.cg(A64, h32.preconditioner(graph=True), flexible=True, rtol=rtol, max_it=500)
+            setup = time.perf_counter() - t0
+            desc = hierarchy + " in FP32 with BF16 geometry, dense coarse solve, HIP graph; outer FP64 flexible CG"
         else:
-            t0 = time.perf_counter()
             data = learned.collect_pairs(h32, n_problems=4, cycles=2, seed=0)
             cnn = S.CNN(16, 4, seed=0)
             hist = learned.train(cnn, data, steps=300, lr=2e-3, batch_elements=16, seed=0)
             t_train = time.perf_counter() - t0
-            hl = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, macro_box=(n_macro, 1.0), learned=cnn,
-                           learned_steps=learned_steps)
+            hl = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, macro_box=box, coarse="dense",
+                           learned=cnn, learned_steps=learned_steps)
             switches = hl.switches
-            P = S.precision_bridge(hl.mg)
+            solver = learned.flexible_cg_with_fallback(A64, hl.preconditioner(graph=True), switches, rtol=rtol,
+                                                       max_it=500)
             setup = time.perf_counter() - t0
-            solver = learned.flexible_cg_with_fallback(h64.fine.A, P, switches, rtol=rtol, max_it=500)
             extra = {"training_seconds": t_train, "training_loss_last20": float(np.mean(hist[-20:])),
                      "learned_steps_per_smoothing": learned_steps}
-            desc = ("mixed V-cycle with the finest-level Chebyshev3 replaced by the learned CNN smoother "
-                    "(BF16 MFMA inference); Chebyshev3 fallback at per-iteration reduction > 0.5")
-        # one untimed solve to warm up code paths, then the timed one from x = 0
+            desc = ("mixed, with the finest-level Chebyshev3 replaced by the learned CNN smoother (BF16 MFMA "
+                    "inference); Chebyshev3 fallback at per-iteration reduction > 0.5")
+        # one untimed solve to warm up code paths (and capture graphs), then the timed one from x = 0
         x = S.DeviceArray(N)
         solver.apply(b, x)
         for sw in switches:
@@ -126,12 +141,94 @@ def solve_times(S, mesh, fff, pts, n_macro, rtol=1e-10, learned_steps=2):
         S.synchronize()
         t = time.perf_counter() - t0
         out[name] = {"seconds": t, "iterations": solver.iterations, "status": solver.last_status,
-                     "true_rel_residual": true_res(h64.fine.A, x), "setup_seconds": setup,
+                     "true_rel_residual": true_res(A64, x), "setup_seconds": setup,
                      "preconditioner": desc, **extra}
         if switches:
             out[name]["fallback_events"] = int(sum(S.switch_state(sw)[1] for sw in switches) - ev0)
+    for name in ("fp64", "mixed", "mixed_learned"):
+        out[f"speedup_{name}_vs_fp64_eager"] = out["fp64_eager"]["seconds"] / out[name]["seconds"]
     out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
     out["speedup_mixed_learned_vs_fp64"] = out["fp64"]["seconds"] / out["mixed_learned"]["seconds"]
+    return out
+
+
+def elasticity_solve_times(S, n_macro=32, L=8, rtol=1e-10, with_learned=False):
+    """sshex8 linear elasticity, level L on an n_macro^3 macro hex8 unit cube (3 (nL+1)^3 DOF:
+    51,066,171 at L=8, n=32), E = 1, nu = 0.3, clamped at z = 0, body force (0, 0, -1).  Time to
+    true relative residual rtol of FP64 MG-PCG (graph-replayed V-cycle, dense coarse solve) and of
+    the FP32/BF16 V-cycle in FP64 flexible CG (optionally with the learned smoother per component)."""
+    from sfem_amd import gmg, learned
+    t0 = time.perf_counter()
+    mel, mpts = S.box_mesh(n_macro + 1, n_macro + 1, n_macro + 1, element_type=S.HEX8)
+    el, pts = S.sshex8_refine(L, mel, mpts)
+    adj, det = S.adjugate_det(S.SSHEX8, L, el, pts)
+    N = pts.shape[1]
+    mesh = S.Mesh(S.SSHEX8, L, el, N)
+    bc = np.where(pts[2] < 1e-6)[0].astype(np.int32)
+    del el
+    nu = 0.3
+    mu, lam = 1.0 / (2 * (1 + nu)), nu / ((1 + nu) * (1 - 2 * nu))
+    rhs = np.zeros((N, 3))
+    rhs[:, 2] = -1
+    rhs[bc] = 0
+    rhs = rhs.ravel()
+    del pts
+    b = S.DeviceArray.from_numpy(rhs)
+    mesh_s = time.perf_counter() - t0
+    box = (n_macro, 1.0)
+    out = {"workload": f"sshex8 linear elasticity L={L} on a {n_macro}^3 macro cube, {3 * N} DOF, nu=0.3, "
+                       "clamped z=0, body force (0,0,-1)", "dofs": 3 * N, "mesh_seconds": mesh_s}
+    common = dict(mu=mu, lam=lam, macro_box=box, coarse="dense")
+    progress(f"elasticity: {3 * N} DOF mesh ready in {mesh_s:.1f}s; building FP64 hierarchy")
+    t0 = time.perf_counter()
+    h64 = gmg.SSGMG(mesh, "elasticity", (adj, det), bc, **common)
+    A64 = h64.fine.A
+    setups = {"fp64": time.perf_counter() - t0}
+    cases = [("fp64", h64, S.FLOAT64)]
+    progress("elasticity: building FP32 hierarchy")
+    t0 = time.perf_counter()
+    h32 = gmg.SSGMG(mesh, "elasticity", (adj, det), bc, real_type=S.FLOAT32, **common)
+    setups["mixed"] = time.perf_counter() - t0
+    cases.append(("mixed", h32, S.FLOAT32))
+    if with_learned:
+        progress("elasticity: training the learned smoother")
+        t0 = time.perf_counter()
+        data = learned.collect_pairs(h32, n_problems=2, cycles=2, seed=0)
+        cnn = S.CNN(16, 4, seed=0)
+        learned.train(cnn, data, steps=300, lr=2e-3, batch_elements=16, seed=0)
+        hl = gmg.SSGMG(mesh, "elasticity", (adj, det), bc, real_type=S.FLOAT32, learned=cnn, learned_steps=1,
+                       **common)
+        setups["mixed_learned"] = time.perf_counter() - t0
+        cases.append(("mixed_learned", hl, S.FLOAT32))
+
+    def true_res(x):
+        r = S.DeviceArray(3 * N)
+        A64.apply(x, r)
+        return float(np.linalg.norm(rhs - r.numpy()) / np.linalg.norm(rhs))
+
+    for name, h, rt in cases:
+        progress(f"elasticity solve: {name}")
+        P = h.preconditioner(graph=True)
+        if rt == S.FLOAT64:
+            solver = S.cg(A64, P, rtol=rtol, max_it=1000, stop_on_residual=True)
+        else:
+            solver = learned.flexible_cg_with_fallback(A64, P, h.switches, rtol=rtol, max_it=1000)
+        x = S.DeviceArray(3 * N)
+        solver.apply(b, x)  # warm-up (captures the graph)
+        h.select_smoothers(0)
+        x.zero()
+        S.synchronize()
+        t0 = time.perf_counter()
+        solver.apply(b, x)
+        S.synchronize()
+        t = time.perf_counter() - t0
+        out[name] = {"seconds": t, "iterations": solver.iterations, "status": solver.last_status,
+                     "true_rel_residual": true_res(x), "setup_seconds": setups[name]}
+        if h.switches:
+            out[name]["fallback_active"] = int(S.switch_state(h.switches[0])[0])
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
+    if with_learned:
+        out["speedup_mixed_learned_vs_fp64"] = out["fp64"]["seconds"] / out["mixed_learned"]["seconds"]
     return out
 
 
@@ -143,6 +240,10 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flush", action="store_true")
+    ap.add_argument("--no-elasticity", action="store_true", help="skip the ~51M DOF elasticity solve")
+    ap.add_argument("--only-elasticity", action="store_true", help="run only the elasticity solve section")
+    ap.add_argument("--elasticity-learned", action="store_true",
+                    help="also time the learned smoother in the elasticity solve")
     ap.add_argument("--mode", choices=["apply", "copy"], default="apply",
                     help="copy: stream-copy 2 GiB of FP64 per step (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -168,6 +269,9 @@ def main():
         dt = (_t.perf_counter() - t0) / args.steps
         print(json.dumps({"mode": "copy", "bytes_per_step": 16 * n, "GB/s": 16 * n / dt / 1e9}))
         return
+    if args.only_elasticity:
+        print(json.dumps({"solve_elasticity": elasticity_solve_times(S, with_learned=args.elasticity_learned)}))
+        return
     L, n_macro = 8, 16
     mel, mpts, el, pts, fff = build_problem(n_macro, L)
     N, E = pts.shape[1], el.shape[1]
@@ -176,6 +280,7 @@ def main():
     x = S.DeviceArray.from_numpy(np.random.default_rng(0).uniform(-1, 1, N))
     y = S.DeviceArray(N)
 
+    progress(f"apply benchmark: {args.steps} steps")
     total_ms = S.benchmark(op, x, y, warmup=args.warmup, reps=args.steps)
     ms = total_ms / args.steps
     B = algorithmic_bytes(N, E, L)
@@ -211,7 +316,10 @@ def main():
                                    "note": "1 GiB written between applies (outside the timed events)"}
     if not args.no_solve:
         result["solve"] = solve_times(S, mesh, fff, pts, n_macro)
+        if not args.no_elasticity:
+            result["solve_elasticity"] = elasticity_solve_times(S, with_learned=args.elasticity_learned)
     if not args.no_cpu:
+        progress("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(el, fff, N, L)
     print(json.dumps(result))
 

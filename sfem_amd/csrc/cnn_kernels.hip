@@ -337,19 +337,21 @@ template int scatter_elements<float>(ptrdiff_t, int, const int32_t *, int, int, 
                                      float *, hipStream_t);
 
 // ------------------------------------------------------------------ fused BF16 MFMA inference
-// One macro element per block (512 threads = 8 waves); the element's activations stay in LDS
-// for all four layers and both signs of f(z) = g(z) - g(-z).  Nodes are processed in tiles of
-// 16 (the MFMA N dimension); per tile and layer:
+// One macro element per block (512 threads = 8 waves); the element's
+// activations stay in LDS for all four layers and both signs of f(z) = g(z) - g(-z).  Nodes are
+// processed in tiles of 16 (the MFMA N dimension); per tile and layer:
 //   layer 1 (1 -> 16):  D[co][n]  = sum_{k<32} W1[co][k] z[n + off_k]         one 16x16x32 MFMA
 //   layer 2,3 (16->16): D[co][n] += sum_{pairs p} W[co][ci,off] act[n + off]  14 16x16x32 MFMAs,
-//                       K = 32 = two offsets x 16 input channels
-//   layer 4 (16 -> 1):  P[k][n]   = sum_ci W4[ci][k] act[ci][n] (two 16x16x16 MFMAs, k < 32),
-//                       then out[n] = sum_k P[k][n + off_k] from LDS.
+//                       K = 32 = two offsets x 16 input channels; each wave runs three tiles at
+//                       once (three independent accumulation chains)
+//   layer 4 (16 -> 1):  P[k][n]   = sum_ci W4[ci][k] act[ci][n] (16x16x16 MFMAs, 16 offsets k per
+//                       pass, two passes), then out[n] = sum_k P[k][n + off_k] from LDS.
 // Lane layouts (64 lanes, g = lane / 16, c = lane % 16):
 //   16x16x32 A: row c, k = 8g + j (j < 8);  B: column c, k = 8g + j;  D: column c, rows 4g + r
 //   16x16x16 A: row c, k = 4g + j (j < 4);  B: column c, k = 4g + j;  D: as above
-// Activations are stored post-ReLU in BF16 as [node][16 channels]; padding nodes of the last
-// tile and neighbours outside the element read as zero.
+// Activations are stored post-ReLU in BF16 as two channel planes [half][node][8 channels], so the
+// 16-byte B reads of the 16 lanes of a row group are consecutive (no LDS bank conflicts); padding
+// nodes of the last tile and neighbours outside the element read as zero.
 typedef short mf_s4 __attribute__((ext_vector_type(4)));
 typedef __bf16 mf_b8 __attribute__((ext_vector_type(8)));
 typedef float mf_f4 __attribute__((ext_vector_type(4)));
@@ -369,7 +371,9 @@ struct CnnShape {
     static constexpr int T = (NXE + 15) / 16;
     static constexpr int NP = T * 16;
     static constexpr int THREADS = 512;
-    static constexpr size_t LDS = sizeof(float) * NP + 2 * 16 * NP + (size_t)27 * NP * sizeof(float);
+    static constexpr int TPG = 3;  // tiles per wave iteration in the hidden layers
+    // zs [NP] float | actX [2][NP][8] bf16 | actY [2][NP][8] bf16, aliased by P [16][NP] float
+    static constexpr size_t LDS = sizeof(float) * NP + 32 * (size_t)NP + 64 * (size_t)NP;
 };
 
 template <int L>
@@ -384,26 +388,37 @@ __device__ __forceinline__ int cnn_shift(int o) {
     return (o / 9 - 1) * N1 * N1 + ((o / 3) % 3 - 1) * N1 + (o % 3 - 1);
 }
 
+// 4 channels (4g .. 4g+3) of node n, as stored from an MFMA D fragment
+__device__ __forceinline__ unsigned short *act_quad(unsigned short *a, int NP, int n, int g) {
+    return a + (g >> 1) * NP * 8 + n * 8 + 4 * (g & 1);
+}
+
+__device__ __forceinline__ mf_u2 relu_pack(const mf_f4 &acc, float sg) {
+    mf_u2 o2;
+    o2[0] = (unsigned)f2bf(fmaxf(sg * acc[0], 0.f)) | ((unsigned)f2bf(fmaxf(sg * acc[1], 0.f)) << 16);
+    o2[1] = (unsigned)f2bf(fmaxf(sg * acc[2], 0.f)) | ((unsigned)f2bf(fmaxf(sg * acc[3], 0.f)) << 16);
+    return o2;
+}
+
 template <int L>
-__global__ void __launch_bounds__(512) cnn_infer_kernel(const ptrdiff_t E, const unsigned short *__restrict__ W16,
-                                                        const float *__restrict__ zin, float *__restrict__ out) {
+__global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, const unsigned short *__restrict__ W16,
+                                                           const float *__restrict__ zin, float *__restrict__ out) {
     using S = CnnShape<L>;
+    constexpr int NP = S::NP, TPG = S::TPG, NB = (S::NXE + S::THREADS - 1) / S::THREADS;
     extern __shared__ __align__(16) unsigned char cnn_smem[];
-    float *zs = (float *)cnn_smem;                                        // [NP]
-    unsigned short *actX = (unsigned short *)(cnn_smem + 4 * S::NP);      // [NP][16]
-    unsigned short *actY = actX + 16 * S::NP;                             // [NP][16]
-    float *P = (float *)(cnn_smem + 4 * S::NP + 32 * S::NP);              // [27][NP], aliases actY
+    float *zs = (float *)cnn_smem;
+    unsigned short *actX = (unsigned short *)(cnn_smem + 4 * NP);
+    unsigned short *actY = actX + 16 * NP;
+    float *P = (float *)(cnn_smem + 4 * NP + 32 * NP);  // aliases actY (dead when P is live)
     const ptrdiff_t e = blockIdx.x;
     if (e >= E) return;  // uniform per block
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = S::THREADS / 64;
     const int g = lane >> 4, c = lane & 15;
 
-    for (int i = tid; i < S::NP; i += S::THREADS) zs[i] = i < S::NXE ? zin[(size_t)e * S::NXE + i] : 0.f;
+    for (int i = tid; i < NP; i += S::THREADS) zs[i] = i < S::NXE ? zin[(size_t)e * S::NXE + i] : 0.f;
     // weights in lane order: layer 1 [64][8], layers 2-3 [14][64][8], layer 4 [2][64][4]
     const mf_u4 w1 = *(const mf_u4 *)(W16 + lane * 8);
-    const mf_u2 w4a = *(const mf_u2 *)(W16 + 512 + 2 * 7168 + lane * 4);
-    const mf_u2 w4b = *(const mf_u2 *)(W16 + 512 + 2 * 7168 + 256 + lane * 4);
-    float gplus[(S::NXE + S::THREADS - 1) / S::THREADS];
+    float gplus[NB];
     __syncthreads();
 
     for (int sign = 0; sign < 2; sign++) {
@@ -422,14 +437,10 @@ __global__ void __launch_bounds__(512) cnn_infer_kernel(const ptrdiff_t E, const
             }
             mf_f4 acc = {0.f, 0.f, 0.f, 0.f};
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mf_b8, w1), b, acc, 0, 0, 0);
-            mf_u2 o2;
-            o2[0] = (unsigned)f2bf(fmaxf(sg * acc[0], 0.f)) | ((unsigned)f2bf(fmaxf(sg * acc[1], 0.f)) << 16);
-            o2[1] = (unsigned)f2bf(fmaxf(sg * acc[2], 0.f)) | ((unsigned)f2bf(fmaxf(sg * acc[3], 0.f)) << 16);
-            *(mf_u2 *)(actX + n * 16 + 4 * g) = o2;
+            *(mf_u2 *)act_quad(actX, NP, n, g) = relu_pack(acc, sg);
         }
         __syncthreads();
-        // ---- layers 2 and 3: actX -> actY -> actX.  Two tiles per wave at a time: two independent
-        // MFMA accumulation chains hide the MFMA latency.
+        // ---- layers 2 and 3: actX -> actY -> actX
 #pragma unroll 1
         for (int layer = 0; layer < 2; layer++) {
             const unsigned short *ain = layer ? actY : actX;
@@ -438,12 +449,13 @@ __global__ void __launch_bounds__(512) cnn_infer_kernel(const ptrdiff_t E, const
             mf_u4 w[14];
 #pragma unroll
             for (int p = 0; p < 14; p++) w[p] = *(const mf_u4 *)(wl + (p * 64 + lane) * 8);
-            for (int t = 2 * wave; t < S::T; t += 2 * nw) {
-                int n[2], x[2], y[2], z[2];
-                bool in[2];
-                mf_f4 acc[2];
+            const unsigned short *plane = ain + (g & 1) * NP * 8;
+            for (int t = TPG * wave; t < S::T; t += TPG * nw) {
+                int n[TPG], x[TPG], y[TPG], z[TPG];
+                bool in[TPG];
+                mf_f4 acc[TPG];
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
+                for (int h = 0; h < TPG; h++) {
                     n[h] = (t + h) * 16 + c;
                     x[h] = n[h] % S::N1;
                     y[h] = (n[h] / S::N1) % S::N1;
@@ -454,62 +466,67 @@ __global__ void __launch_bounds__(512) cnn_infer_kernel(const ptrdiff_t E, const
 #pragma unroll
                 for (int p = 0; p < 14; p++) {
                     const int o = 2 * p + (g >> 1);
+                    mf_u4 bv[TPG];
 #pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        mf_u4 bv = {0u, 0u, 0u, 0u};
+                    for (int h = 0; h < TPG; h++) {
+                        bv[h] = mf_u4{0u, 0u, 0u, 0u};
                         if (in[h] && cnn_valid<L>(x[h], y[h], z[h], o))
-                            bv = *(const mf_u4 *)(ain + (n[h] + cnn_shift<L>(o)) * 16 + 8 * (g & 1));
-                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mf_b8, w[p]),
-                                                                          __builtin_bit_cast(mf_b8, bv), acc[h], 0, 0,
-                                                                          0);
+                            bv[h] = *(const mf_u4 *)(plane + (n[h] + cnn_shift<L>(o)) * 8);
                     }
+#pragma unroll
+                    for (int h = 0; h < TPG; h++)
+                        acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mf_b8, w[p]),
+                                                                          __builtin_bit_cast(mf_b8, bv[h]), acc[h], 0,
+                                                                          0, 0);
                 }
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    if (t + h < S::T) {
-                        mf_u2 o2;
-                        o2[0] = (unsigned)f2bf(fmaxf(acc[h][0], 0.f)) | ((unsigned)f2bf(fmaxf(acc[h][1], 0.f)) << 16);
-                        o2[1] = (unsigned)f2bf(fmaxf(acc[h][2], 0.f)) | ((unsigned)f2bf(fmaxf(acc[h][3], 0.f)) << 16);
-                        *(mf_u2 *)(aout + n[h] * 16 + 4 * g) = o2;
-                    }
-                }
+                for (int h = 0; h < TPG; h++)
+                    if (t + h < S::T) *(mf_u2 *)act_quad(aout, NP, n[h], g) = relu_pack(acc[h], 1.f);
             }
             __syncthreads();
         }
-        // ---- layer 4, channel mixing: actX -> P[k][n] (k < 27)
-        for (int t = wave; t < S::T; t += nw) {
-            const int n = t * 16 + c;
-            const mf_u2 bv = *(const mf_u2 *)(actX + n * 16 + 4 * g);
-            mf_f4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
-            a0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(mf_s4, w4a), __builtin_bit_cast(mf_s4, bv),
-                                                            a0, 0, 0, 0);
-            a1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(mf_s4, w4b), __builtin_bit_cast(mf_s4, bv),
-                                                            a1, 0, 0, 0);
+        // ---- layer 4 in two passes of 16 offsets: P[r][n] = sum_ci W4[ci][16 pass + r] act[ci][n],
+        // then v[n] += sum_r P[r][n + off_{16 pass + r}]
+        float v[NB];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                P[(4 * g + r) * S::NP + n] = a0[r];
-                const int k = 16 + 4 * g + r;
-                if (k < 27) P[k * S::NP + n] = a1[r];
+        for (int i = 0; i < NB; i++) v[i] = 0.f;
+#pragma unroll
+        for (int pass = 0; pass < 2; pass++) {
+            const mf_u2 w4 = *(const mf_u2 *)(W16 + 512 + 2 * 7168 + pass * 256 + lane * 4);
+            for (int t = wave; t < S::T; t += nw) {
+                const int n = t * 16 + c;
+                const mf_u2 bv = *(const mf_u2 *)act_quad(actX, NP, n, g);
+                mf_f4 a = {0.f, 0.f, 0.f, 0.f};
+                a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(mf_s4, w4), __builtin_bit_cast(mf_s4, bv),
+                                                               a, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; r++) P[(4 * g + r) * NP + n] = a[r];
             }
-        }
-        __syncthreads();
-        // ---- layer 4, stencil sum: g(+-z)[n] = sum_k P[k][n + off_k]
+            __syncthreads();
 #pragma unroll
-        for (int i = 0; i < (S::NXE + S::THREADS - 1) / S::THREADS; i++) {
+            for (int i = 0; i < NB; i++) {
+                const int n = tid + i * S::THREADS;
+                if (n < S::NXE) {
+                    const int x = n % S::N1, y = (n / S::N1) % S::N1, z = n / (S::N1 * S::N1);
+#pragma unroll
+                    for (int r = 0; r < 16; r++) {
+                        const int k = 16 * pass + r;
+                        if (cnn_valid<L>(x, y, z, k)) v[i] += P[r * NP + n + cnn_shift<L>(k)];
+                    }
+                }
+            }
+            __syncthreads();  // P is rewritten by the next pass / layer 2 of the next sign
+        }
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
             const int n = tid + i * S::THREADS;
             if (n < S::NXE) {
-                const int x = n % S::N1, y = (n / S::N1) % S::N1, z = n / (S::N1 * S::N1);
-                float v = 0.f;
-#pragma unroll
-                for (int k = 0; k < 27; k++)
-                    if (cnn_valid<L>(x, y, z, k)) v += P[k * S::NP + n + cnn_shift<L>(k)];
                 if (sign == 0)
-                    gplus[i] = v;
+                    gplus[i] = v[i];
                 else
-                    out[(size_t)e * S::NXE + n] = gplus[i] - v;
+                    out[(size_t)e * S::NXE + n] = gplus[i] - v[i];
             }
         }
-        // the next sign's layer-1 barrier orders these reads before layer 2 overwrites P
     }
 }
 

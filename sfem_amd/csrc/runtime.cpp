@@ -161,6 +161,7 @@ struct ElasticityOp final : OpBase {
     DevArray<float> adj, det, coeff;  // hex8: SoA [9][E], [E]
     DevArray<double> K;               // sshex8: per-macro 24x24 matrices
     DevArray<float> K32;              // sshex8, FP32 operator: K rounded to float
+    DevArray<T> Sst;                  // sshex8 at large L: merged 27-point stencils [E][243]
     DevArray<T> part;                 // sshex8: skeleton contributions [E][sk_count(L)][3]
     double mu{0}, lambda{0};
     int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
@@ -185,7 +186,10 @@ struct ElasticityOp final : OpBase {
             }
             Kt = K32.p;
         }
-        return sshex8_elasticity_apply<T>(m->level, m->E, m->idx.p, Kt, (const T *)xv, (T *)yv, part.p,
+        if (sshex8_elasticity_uses_stencils(m->level) && !Sst.p && m->E > 0) {
+            if (Sst.alloc((size_t)m->E * 243) || sshex8_elasticity_stencils<T>(m->E, K.p, Sst.p, s)) return SFEM_FAILURE;
+        }
+        return sshex8_elasticity_apply<T>(m->level, m->E, m->idx.p, Kt, Sst.p, (const T *)xv, (T *)yv, part.p,
                                           skeleton_of(m), s);
     }
     int diag(void *d, hipStream_t s) override {

@@ -35,9 +35,13 @@ int sshex8_elasticity_matrices(int L, ptrdiff_t E, const G *adj, const G *det, p
                                double lambda, const float *coeff, double *K, hipStream_t s);
 template <typename T>
 // part: E * sk_count(L) * 3 values of scratch
-// K: per-macro 24x24 matrices in the vector precision
-int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *x, T *y, T *part,
-                            const SkeletonMap &sk, hipStream_t s);
+// K: per-macro 24x24 matrices in the vector precision; Sst: merged 27-point 3x3-block stencils
+// [E][243] (required when sshex8_elasticity_uses_stencils(L))
+int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *Sst, const T *x, T *y,
+                            T *part, const SkeletonMap &sk, hipStream_t s);
+bool sshex8_elasticity_uses_stencils(int L);
+template <typename T>
+int sshex8_elasticity_stencils(ptrdiff_t E, const double *K, T *S, hipStream_t s);
 template <typename T>
 int sshex8_elasticity_diag(int L, ptrdiff_t E, const int32_t *idx, const double *K, int block, T *d, hipStream_t s);
 

@@ -154,7 +154,7 @@ def test_hex8_elasticity_parity():
     assert relerr(host(d), do) < RTOL
 
 
-@pytest.mark.parametrize("L,n", [(1, 2), (2, 2), (4, 2), (6, 1), (8, 2), (16, 1), (16, 2)])
+@pytest.mark.parametrize("L,n", [(1, 2), (2, 2), (4, 2), (6, 1), (8, 2), (12, 1), (16, 1), (16, 2)])
 def test_sshex8_elasticity_parity(L, n):
     mel, mpts = skewed_box(n, perturb=0.02)
     el, pts = O.sshex8_refine(L, mel, mpts)
