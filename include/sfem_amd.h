@@ -233,6 +233,13 @@ int sfem_amd_cheb3_diag_create(sfem_amd_op *A, const void *inv_diag, int max_it,
  * (algebra/sfem_Stationary.hpp:50-80, algebra/sfem_ShiftableJacobi.hpp).  diag: device. */
 int sfem_amd_jacobi_create(sfem_amd_op *A, const void *diag, double omega, int steps,
                            sfem_amd_op **out);
+/* StationaryIteration with block-Jacobi on 3x3 blocks (create_gmg_default_smoothers_and_solver for
+ * block size 3, algebra/mg/sfem_GeometricMultigrid.cpp:114-131; ShiftableBlockSymJacobi,
+ * algebra/sfem_ShiftableJacobi.hpp): x += omega B^-1 (b - A x), `steps` sweeps.  block_diag_sym:
+ * device [n/3][6] in A's real type (sfem_amd_op_block_diag_sym of the unconstrained operator);
+ * dof_mask: device uint8 [n], 1 on constrained dofs (their block rows become unit rows), or NULL. */
+int sfem_amd_block_jacobi_create(sfem_amd_op *A, const void *block_diag_sym, const uint8_t *dof_mask, double omega,
+                                 int steps, sfem_amd_op **out);
 /* y += omega * x / diag (create_inverse_diagonal_scaling, used as the coarse-solve preconditioner by
  * drivers/mgsolve.cpp:221-226).  diag: device vector of n entries (copied). */
 int sfem_amd_inverse_diagonal_create(int real_type, ptrdiff_t n, const void *diag, double omega,

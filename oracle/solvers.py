@@ -140,6 +140,29 @@ class Jacobi:
         return x
 
 
+class BlockJacobi:
+    """StationaryIteration with ShiftableBlockSymJacobi (algebra/sfem_ShiftableJacobi.hpp set_diag,
+    algebra/openmp/sfem_openmp_ShiftableJacobi.hpp): symmetric blocks [n][6] -> full 3x3, rows of
+    constrained dofs replaced by unit rows, inverted, scaled by omega; x += Binv (b - A x)."""
+
+    def __init__(self, A, sym_blocks, mask=None, omega=1.0 / 3, steps=3):
+        d = np.asarray(sym_blocks, np.float64).reshape(-1, 6)
+        B = np.stack([d[:, [0, 1, 2]], d[:, [1, 3, 4]], d[:, [2, 4, 5]]], axis=1)  # [n][3][3]
+        if mask is not None:
+            m = np.asarray(mask).reshape(-1, 3).astype(bool)
+            for r in range(3):
+                B[m[:, r], r, :] = 0
+                B[m[:, r], r, r] = 1
+        self.inv = omega * np.linalg.inv(B)
+        self.A, self.steps = A, steps
+
+    def __call__(self, b, x):
+        for _ in range(self.steps):
+            r = (b - _apply0(self.A, x)).reshape(-1, 3)
+            x += np.einsum("bij,bj->bi", self.inv, r).ravel()
+        return x
+
+
 class Multigrid:
     """Multigrid::cycle (algebra/sfem_Multigrid.hpp:242-346).  levels: list of
     (A, smoother(b, x) -> x, prolongation(c) -> f or None, restriction(f) -> c or None);

@@ -99,6 +99,7 @@ SIGNATURES = {
     "sfem_amd_gather_elements": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "sfem_amd_learned_smoother_create": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _i, C.POINTER(_vp)]),
     "sfem_amd_dense_op_create": (_i, [_i, _pd, _vp, C.POINTER(_vp)]),
+    "sfem_amd_block_jacobi_create": (_i, [_vp, _vp, _vp, _d, _i, C.POINTER(_vp)]),
     "sfem_amd_graph_op_create": (_i, [_vp, C.POINTER(_vp)]),
     "sfem_amd_graph_op_stats": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "sfem_amd_switch_create": (_i, [_vp, _vp, C.POINTER(_vp)]),
@@ -440,6 +441,13 @@ def cheb3_eig_max(op):
 
 def jacobi(A, diag, omega=1.0, steps=3):
     return _new_op("sfem_amd_jacobi_create", A.h, _dptr(diag), omega, steps, keep=[A])
+
+
+def block_jacobi(A, block_diag_sym, dof_mask=None, omega=1.0 / 3, steps=3):
+    """Block-Jacobi stationary smoother on 3x3 blocks (block_diag_sym: device [n/3][6];
+    dof_mask: device uint8 [n], 1 = constrained)."""
+    return _new_op("sfem_amd_block_jacobi_create", A.h, _dptr(block_diag_sym), _dptr(dof_mask), omega, steps,
+                   keep=[A])
 
 
 def permutation(perm, block_size=1, real_type=FLOAT64, transpose=False, constraints=None):

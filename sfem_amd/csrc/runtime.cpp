@@ -1048,6 +1048,50 @@ static int jacobi_create_t(sfem_amd_op *A, const void *diag, double omega, int s
 }
 }  // extern "C++"
 
+extern "C++" {
+// y += (omega B^-1) x with B the symmetric 3x3 diagonal blocks
+template <typename T>
+struct BlockInverseOp final : OpBase {
+    DevArray<T> inv;
+    ptrdiff_t nb{0};
+    int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
+    ptrdiff_t rows() const override { return 3 * nb; }
+    int apply(const void *x, void *y, hipStream_t s) override {
+        return block3_apply<T>(nb, inv.p, (const T *)x, (T *)y, s);
+    }
+};
+
+template <typename T>
+static int block_jacobi_create_t(sfem_amd_op *A, const void *sym, const uint8_t *mask, double omega, int steps,
+                                 sfem_amd_op **out) {
+    auto P = std::make_shared<BlockInverseOp<T>>();
+    P->nb = A->impl->rows() / 3;
+    if (P->inv.alloc((size_t)P->nb * 9) ||
+        block_sym_inverse<T>(P->nb, (const T *)sym, mask, omega, P->inv.p, library_stream()))
+        return SFEM_FAILURE;
+    auto st = std::make_shared<sfem::StationaryIteration<T>>();
+    st->default_init();
+    st->set_op(view<T>(A->impl));
+    st->set_preconditioner_op(view<T>(P));
+    st->set_max_it(steps);
+    auto s = std::make_shared<SolverOp<T>>();
+    s->op = st;
+    s->n = A->impl->rows();
+    s->keep.push_back(P);
+    *out = wrap(s);
+    return SFEM_SUCCESS;
+}
+}  // extern "C++"
+
+int sfem_amd_block_jacobi_create(sfem_amd_op *A, const void *block_diag_sym, const uint8_t *dof_mask, double omega,
+                                 int steps, sfem_amd_op **out) {
+    CHECK_HANDLE(A);
+    SFEM_AMD_REQUIRE(block_diag_sym && out && A->impl->rows() % 3 == 0, "block_jacobi_create: arguments");
+    if (A->impl->real_type() == SFEM_AMD_FLOAT64)
+        return block_jacobi_create_t<double>(A, block_diag_sym, dof_mask, omega, steps, out);
+    return block_jacobi_create_t<float>(A, block_diag_sym, dof_mask, omega, steps, out);
+}
+
 int sfem_amd_jacobi_create(sfem_amd_op *A, const void *diag, double omega, int steps, sfem_amd_op **out) {
     CHECK_HANDLE(A);
     if (A->impl->real_type() == SFEM_AMD_FLOAT64) return jacobi_create_t<double>(A, diag, omega, steps, out);

@@ -294,6 +294,51 @@ __global__ void __launch_bounds__(256) dense_matvec_kernel(const ptrdiff_t n, co
     if (lane == 0) y[row] += (T)v;
 }
 
+// Block-Jacobi (algebra/sfem_ShiftableJacobi.hpp ShiftableBlockSymJacobi::set_diag and
+// algebra/openmp/sfem_openmp_ShiftableJacobi.hpp): the symmetric 3x3 blocks [n][6] (xx, xy, xz,
+// yy, yz, zz) become full blocks, rows of constrained dofs become unit rows, the blocks are
+// inverted and scaled by omega; stored [n][9] row-major.
+template <typename T>
+__global__ void block_sym_inverse_kernel(const ptrdiff_t n, const T *__restrict__ sym, const uint8_t *__restrict__ mask,
+                                         const double omega, T *__restrict__ inv) {
+    for (ptrdiff_t b = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; b < n; b += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const T *d = sym + b * 6;
+        double m[9] = {(double)d[0], (double)d[1], (double)d[2], (double)d[1], (double)d[3],
+                       (double)d[4], (double)d[2], (double)d[4], (double)d[5]};
+        if (mask) {
+            for (int r = 0; r < 3; r++)
+                if (mask[b * 3 + r])
+                    for (int c = 0; c < 3; c++) m[r * 3 + c] = (r == c) ? 1.0 : 0.0;
+        }
+        const double c00 = m[4] * m[8] - m[5] * m[7], c01 = m[5] * m[6] - m[3] * m[8], c02 = m[3] * m[7] - m[4] * m[6];
+        const double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+        const double s = omega / det;
+        T *o = inv + b * 9;
+        o[0] = (T)(c00 * s);
+        o[1] = (T)((m[2] * m[7] - m[1] * m[8]) * s);
+        o[2] = (T)((m[1] * m[5] - m[2] * m[4]) * s);
+        o[3] = (T)(c01 * s);
+        o[4] = (T)((m[0] * m[8] - m[2] * m[6]) * s);
+        o[5] = (T)((m[2] * m[3] - m[0] * m[5]) * s);
+        o[6] = (T)(c02 * s);
+        o[7] = (T)((m[1] * m[6] - m[0] * m[7]) * s);
+        o[8] = (T)((m[0] * m[4] - m[1] * m[3]) * s);
+    }
+}
+
+// y[3b + r] += sum_c inv[b][3r + c] x[3b + c]
+template <typename T>
+__global__ void block3_apply_kernel(const ptrdiff_t n, const T *__restrict__ inv, const T *__restrict__ x,
+                                    T *__restrict__ y) {
+    for (ptrdiff_t b = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; b < n; b += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const T *m = inv + b * 9;
+        const T x0 = x[3 * b], x1 = x[3 * b + 1], x2 = x[3 * b + 2];
+        y[3 * b + 0] += m[0] * x0 + m[1] * x1 + m[2] * x2;
+        y[3 * b + 1] += m[3] * x0 + m[4] * x1 + m[5] * x2;
+        y[3 * b + 2] += m[6] * x0 + m[7] * x1 + m[8] * x2;
+    }
+}
+
 // Deterministic reductions: a fixed grid (a function of n only) of 256-thread blocks,
 // wave64 shuffle trees and one pass over the partials in a fixed order.
 constexpr int kRedBlocks = 1024;
@@ -440,6 +485,21 @@ int cheb3_update_diag(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, c
 }
 
 template <typename T>
+int block_sym_inverse(ptrdiff_t n, const T *sym, const uint8_t *mask, double omega, T *inv, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    block_sym_inverse_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, sym, mask, omega, inv);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+template <typename T>
+int block3_apply(ptrdiff_t n, const T *inv, const T *x, T *y, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    block3_apply_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, inv, x, y);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template <typename T>
 int dense_matvec(ptrdiff_t n, const double *M, const T *x, T *y, hipStream_t s) {
     if (n <= 0) return SFEM_SUCCESS;
     dense_matvec_kernel<T><<<(unsigned)((n + 3) / 4), 256, 0, s>>>(n, M, x, y);
@@ -537,6 +597,8 @@ int cast_accumulate(ptrdiff_t n, const float *x, double *y, hipStream_t s) {
     template int cheb3_update<T>(ptrdiff_t, T, T, const T *, const T *, T *, T *, hipStream_t);         \
     template int cheb3_update_diag<T>(ptrdiff_t, T, T, const T *, const T *, const T *, T *, T *, hipStream_t); \
     template int dense_matvec<T>(ptrdiff_t, const double *, const T *, T *, hipStream_t);                \
+    template int block_sym_inverse<T>(ptrdiff_t, const T *, const uint8_t *, double, T *, hipStream_t);   \
+    template int block3_apply<T>(ptrdiff_t, const T *, const T *, T *, hipStream_t);                      \
     template int blas_dot<T>(ptrdiff_t, const T *, const T *, double *, double *, hipStream_t);         \
     template int cg_update<T>(ptrdiff_t, T, const T *, const T *, T *, T *, double *, double *, hipStream_t); \
     template int dofs_set_value<T>(ptrdiff_t, const int32_t *, T, T *, hipStream_t);                    \

@@ -27,6 +27,10 @@ template <typename T> int blas_reciprocal(ptrdiff_t n, T a, T *x, hipStream_t s)
 template <typename T> int blas_xypaz(ptrdiff_t n, const T *x, const T *y, T a, T *z, hipStream_t s);
 template <typename S, typename D> int blas_cast(ptrdiff_t n, const S *x, D *y, hipStream_t s);
 template <typename T> int cheb3_update(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, T *p, T *x, hipStream_t s);
+// omega * inverse of symmetric 3x3 blocks [n][6] (constrained rows -> unit rows), [n][9]
+template <typename T>
+int block_sym_inverse(ptrdiff_t n, const T *sym, const uint8_t *mask, double omega, T *inv, hipStream_t s);
+template <typename T> int block3_apply(ptrdiff_t n, const T *inv, const T *x, T *y, hipStream_t s);
 // y += M x, M dense row-major FP64 n x n
 template <typename T> int dense_matvec(ptrdiff_t n, const double *M, const T *x, T *y, hipStream_t s);
 template <typename T>

@@ -24,7 +24,8 @@ FP64 coarse CG then runs on a few dozen dofs instead of the whole macro grid.
 import numpy as np
 
 from . import FLOAT32, FLOAT64, HEX8, SSHEX8
-from . import (DeviceArray, Mesh, Multigrid, adjugate_det, box_mesh, cg, chain, cheb3, constrained, dense_op, fff,
+from . import (DeviceArray, Mesh, Multigrid, adjugate_det, block_jacobi, box_mesh, cg, chain, cheb3, constrained,
+               dense_op, fff,
                graph_op,
                inverse_diagonal, jacobi, laplacian, linear_elasticity, permutation, precision_bridge, prolongation,
                restriction, set_constrained_value, solver_bridge, sshex8_refine, switch, switch_select)
@@ -124,9 +125,21 @@ class SSGMG:
                 inv = DeviceArray.from_numpy((1.0 / d.numpy()).astype(dtype))
             return cheb3(lv.A, max_it=cfg["degree"], scale_eig_max=cfg["smax"], eigen_solver_max_it=cfg["eig_its"],
                          eigen_solver_tol=cfg["eig_tol"], init_vector=init, inv_diag=inv)
+        if self.block_size == 3:
+            # reference default for vector problems: block-Jacobi on the 3x3 nodal blocks, omega 1/3
+            # (algebra/mg/sfem_GeometricMultigrid.cpp:114-131), constrained rows as unit rows
+            d6 = DeviceArray(lv.n // 3 * 6, dtype)
+            lv.op.block_diag_sym(d6)
+            mask = np.zeros(lv.n, np.uint8)
+            comps = range(3) if self.bc_components is None else self.bc_components
+            for c in comps:
+                mask[lv.bc.astype(np.int64) * 3 + c] = 1
+            lv.mask = DeviceArray.from_numpy(mask)
+            lv.block_diag = d6
+            return block_jacobi(lv.A, d6, lv.mask, omega=1.0 / 3, steps=3)
         d = DeviceArray(lv.n, dtype)
         lv.A.diag(d)
-        return jacobi(lv.A, d, omega=1.0 / self.block_size, steps=3)
+        return jacobi(lv.A, d, omega=1.0, steps=3)
 
     def _extend_below_macro(self, macro_box, geometry, coeff, bc_nodes):
         n, extent = macro_box

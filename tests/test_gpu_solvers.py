@@ -350,3 +350,54 @@ def test_graph_replayed_vcycle_matches_eager(real_type):
         assert np.linalg.norm(b - host(rr)) / np.linalg.norm(b) < 1e-10
         its.append(solver.iterations)
     assert its[0] == its[1]
+
+
+def test_block_jacobi_smoother_parity_and_default_gmg():
+    """Block-Jacobi stationary smoother (3x3 nodal blocks, omega 1/3, constrained rows as unit
+    rows) against the oracle's, and MG-PCG with the reference's default smoothers on elasticity."""
+    L = 2
+    mel, mpts = O.box_mesh(3, 3, 3)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    adj, det = O.hex8_adj_det(mel, mpts)
+    N = pts.shape[1]
+    E = mel.shape[1]
+    nodes = np.where(pts[2] < 1e-6)[0].astype(np.int32)
+    mu, lam = O.lame(1.0, 0.3)
+    op = S.linear_elasticity(S.Mesh(S.SSHEX8, L, el, N), adj, det, mu, lam)
+    A = S.constrained(op, nodes, 3)
+    A_o = oracle_constrained(
+        lambda x, y: O.lib.oracle_sshex8_elasticity_apply(L, E, el, adj, det, mu, lam, None, x, y),
+        (nodes[:, None] * 3 + np.arange(3)).ravel().astype(np.int32))
+    sym = np.zeros(6 * N)
+    O.lib.oracle_sshex8_elasticity_block_diag(L, E, el, adj, det, mu, lam, None, sym)
+    mask = np.zeros(3 * N, np.uint8)
+    mask[(nodes[:, None] * 3 + np.arange(3)).ravel()] = 1
+    rng = np.random.default_rng(2)
+    b = rng.uniform(-1, 1, 3 * N)
+    x0 = rng.uniform(-1, 1, 3 * N)
+    b[mask == 1] = 0
+    x0[mask == 1] = 0
+    xo = SO.BlockJacobi(A_o, sym, mask, omega=1.0 / 3, steps=3)(b, x0.copy())
+    d6 = zeros(6 * N)
+    op.block_diag_sym(d6)
+    sm = S.block_jacobi(A, d6, dev(mask, np.uint8), omega=1.0 / 3, steps=3)
+    x = dev(x0)
+    sm.apply(dev(b), x)
+    S.synchronize()
+    assert np.linalg.norm(host(x) - xo) / np.linalg.norm(xo) < 1e-12
+    # reference default GMG (block-Jacobi smoothing) as a preconditioner
+    L2 = 4
+    el2, pts2 = O.sshex8_refine(L2, mel, mpts)
+    N2 = pts2.shape[1]
+    nodes2 = np.where(pts2[2] < 1e-6)[0].astype(np.int32)
+    h = gmg.SSGMG(S.Mesh(S.SSHEX8, L2, el2, N2), "elasticity", (adj, det), nodes2, mu=mu, lam=lam,
+                  smoother="jacobi", coarse_rtol=1e-12)
+    rhs = np.zeros((N2, 3))
+    rhs[:, 2] = -1
+    rhs[nodes2] = 0
+    solver = S.cg(h.fine.A, h.mg, rtol=1e-10, max_it=400)
+    xs = zeros(3 * N2)
+    solver.apply(dev(rhs.ravel()), xs)
+    res = true_residual(h.fine.A, dev(rhs.ravel()), xs)
+    print(f"elasticity MG-PCG with block-Jacobi smoothing: {solver.iterations} its, residual {res:.2e}")
+    assert solver.last_status == 0 and res < 1e-8

@@ -110,3 +110,29 @@ def test_jacobi_scaled_chebyshev_on_diagonal_matrix_matches_identity():
     np.testing.assert_allclose(es, ei, rtol=1e-12, atol=1e-14)
     ratio = es / (x0 - b / dvec)
     assert np.ptp(ratio) < 1e-10 and abs(ratio[0]) < 1
+
+
+def test_block_jacobi_on_block_diagonal_matrix_is_exact_with_omega_one():
+    """For a block-diagonal SPD matrix, one block-Jacobi sweep with omega = 1 solves exactly; with
+    a constrained dof the row is the identity (x_c += b_c - x_c)."""
+    rng = np.random.default_rng(3)
+    n = 7
+    blocks = []
+    for _ in range(n):
+        M = rng.normal(size=(3, 3))
+        blocks.append(M @ M.T + 3 * np.eye(3))
+    Bd = np.zeros((3 * n, 3 * n))
+    for i, M in enumerate(blocks):
+        Bd[3 * i:3 * i + 3, 3 * i:3 * i + 3] = M
+    A = lambda x, y: y.__iadd__(Bd @ x)  # noqa: E731
+    sym = np.array([[M[0, 0], M[0, 1], M[0, 2], M[1, 1], M[1, 2], M[2, 2]] for M in blocks])
+    b = rng.uniform(-1, 1, 3 * n)
+    x = SO.BlockJacobi(A, sym, omega=1.0, steps=1)(b, np.zeros(3 * n))
+    np.testing.assert_allclose(Bd @ x, b, rtol=1e-12)
+    # masked: dof 4 (block 1, row 1) becomes an identity row in the block used by the smoother
+    mask = np.zeros(3 * n, np.uint8)
+    mask[4] = 1
+    bj = SO.BlockJacobi(A, sym, mask=mask, omega=1.0, steps=1)
+    Bm = blocks[1].copy()
+    Bm[1, :] = [0, 1, 0]
+    np.testing.assert_allclose(bj.inv[1], np.linalg.inv(Bm), rtol=1e-12)
