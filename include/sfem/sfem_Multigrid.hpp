@@ -67,6 +67,11 @@ private:
     };
     std::vector<SharedOperator<T>> operator_, smoother_, prolongation_, restriction_;
     std::vector<Memory> memory_;
+public:
+    // the level operators overwrite their output (as ConjugateGradient::apply_overwrites_output)
+    bool apply_overwrites_output{false};
+
+private:
     int max_it_{10};
     int iterations_{0};
     int cycle_type_{V_CYCLE};
@@ -102,7 +107,7 @@ private:
         for (int k = 0; k < cycle_type_; k++) {
             smoother->apply(mem.rhs->data(), mem.solution->data());
             // residual
-            blas_->zeros(mem.size(), mem.work->data());
+            if (!apply_overwrites_output) blas_->zeros(mem.size(), mem.work->data());
             op->apply(mem.solution->data(), mem.work->data());
             blas_->axpby(mem.size(), T(1), mem.rhs->data(), T(-1), mem.work->data());
             // the finest residual norm (a host synchronisation) is only needed to stop or report;

@@ -149,7 +149,7 @@ private:
         const T rtr0 = blas->dot(n, r, r);
         iterations_ = 0;
         if (rtr0 == 0) return SFEM_SUCCESS;
-        zero_for_apply(n, z);
+        blas->zeros(n, z);  // the preconditioner starts from z = 0 whatever the operator does
         preconditioner_op->apply(r, z);
         blas->copy(n, z, p);
         zero_for_apply(n, Ap);
@@ -163,7 +163,7 @@ private:
         }
         const T bnorm = stop_on_residual_norm ? blas->norm2(n, b) : T(1);
         for (iterations_ = 0; iterations_ < max_it; iterations_++) {
-            zero_for_apply(n, z);
+            blas->zeros(n, z);
             preconditioner_op->apply(r, z);
             const T rtz_new = blas->dot(n, r, z);
             // exact convergence (e.g. a one-unknown coarse problem): the next step would be 0/0
@@ -201,7 +201,7 @@ private:
         iterations_ = 0;
         const T ref = bnorm > 0 ? bnorm : T(1);
         if (rnorm == 0 || rnorm / ref < rtol || rnorm < atol) return SFEM_SUCCESS;
-        zero_for_apply(n, z);
+        blas->zeros(n, z);  // the preconditioner starts from z = 0 whatever the operator does
         preconditioner_op->apply(r, z);
         blas->copy(n, z, p);
         T rtz = blas->dot(n, r, z);
@@ -233,7 +233,7 @@ private:
                 rnorm = true_norm;
             }
             blas->copy(n, z, zold);
-            zero_for_apply(n, z);
+            blas->zeros(n, z);
             preconditioner_op->apply(r, z);
             const T rtz_new = blas->dot(n, r, z);
             const T rtzold = blas->dot(n, r, zold);

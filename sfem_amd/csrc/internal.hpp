@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -67,6 +68,12 @@ inline uint16_t to_bf16(float f) {
 struct OpBase {
     virtual ~OpBase() = default;
     virtual int apply(const void *x, void *y, hipStream_t s) = 0;
+    // y = A x (overwriting y).  Default: zero y, then accumulate; operators whose kernels own
+    // every output entry override it and skip both the fill and the read of y.
+    virtual int apply_overwrite(const void *x, void *y, hipStream_t s) {
+        if (fill_zero(y, (size_t)rows() * (real_type() == SFEM_AMD_FLOAT64 ? 8 : 4), s)) return SFEM_FAILURE;
+        return apply(x, y, s);
+    }
     virtual ptrdiff_t rows() const = 0;
     virtual ptrdiff_t cols() const { return rows(); }
     virtual int real_type() const = 0;
@@ -85,6 +92,14 @@ struct OpBase {
 
 }  // namespace sfem_amd
 
+namespace sfem_amd {
+struct SkeletonStore {
+    DevArray<int32_t> node, off, pos, ent;
+    ptrdiff_t n{0};
+    bool covers{false};  // every mesh node lies in some element (overwriting applies allowed)
+};
+}  // namespace sfem_amd
+
 struct sfem_amd_op {
     std::shared_ptr<sfem_amd::OpBase> impl;
 };
@@ -99,9 +114,8 @@ struct sfem_amd_mesh {
     mutable sfem_amd::DevArray<uint16_t> count;  // incidence counts (lazy)
     mutable sfem_amd::DevArray<uint8_t> own;     // owner mask (lazy)
     const sfem_amd_mesh *parent{nullptr};        // set by sfem_amd_mesh_derefine
-    // sshex8: per-node skeleton slot lists (lazy, see mesh_skeleton)
-    mutable sfem_amd::DevArray<int32_t> sk_node, sk_off, sk_pos;
-    mutable ptrdiff_t sk_n{-1};
+    // sshex8: per-node skeleton slot lists of the mesh viewed at level Lc (lazy, mesh_skeleton)
+    mutable std::map<int, std::unique_ptr<sfem_amd::SkeletonStore>> skel;
 };
 
 namespace sfem_amd {
@@ -110,6 +124,6 @@ int constrained_set_value(const OpBase *constrained, int rt, double v, void *x, 
 int mesh_counts(const sfem_amd_mesh *m, hipStream_t s);
 bool is_constrained(const OpBase *op);
 // Builds (once) the skeleton slot lists of an sshex8 mesh from its connectivity.
-int mesh_skeleton(const sfem_amd_mesh *m);
+int mesh_skeleton(const sfem_amd_mesh *m, int Lc);
 sfem_amd_op *wrap_op(std::shared_ptr<OpBase> o);
 }  // namespace sfem_amd

@@ -245,8 +245,7 @@ struct LearnedSmootherOp final : OpBase {
         T *x = (T *)xv;
         const int rt = real_type();
         for (int it = 0; it < steps; it++) {
-            if (blas_values<T>(n, T(0), r.p, s)) return SFEM_FAILURE;
-            if (A->apply(x, r.p, s)) return SFEM_FAILURE;
+            if (A->apply_overwrite(x, r.p, s)) return SFEM_FAILURE;
             if (blas_axpby<T>(n, T(1), b, T(-1), r.p, s)) return SFEM_FAILURE;  // r = b - A x
             if (blas_values<T>(n, T(0), corr.p, s)) return SFEM_FAILURE;
             for (int c = 0; c < bs; c++) {

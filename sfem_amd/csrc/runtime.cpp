@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -45,9 +46,18 @@ int mesh_counts(const sfem_amd_mesh *m, hipStream_t s) {
     if (m->count.alloc(m->nnodes)) return SFEM_FAILURE;
     return node_incidence(m->E * m->nv, m->nnodes, m->idx.p, m->count.p, s);
 }
-int mesh_skeleton(const sfem_amd_mesh *m) {
-    if (m->sk_n >= 0) return SFEM_SUCCESS;
-    const int L = m->level, N1 = L + 1, nsk = sk_count(L);
+// Skeleton slot lists of an sshex8 mesh viewed at level Lc (Lc divides the mesh level L): the
+// coarse-local node (X, Y, Z) of element e is the mesh node at lidx(L, X*s, Y*s, Z*s), s = L/Lc.
+// Lc == L is the mesh itself; smaller Lc serve the restriction onto derefined levels.
+int mesh_skeleton(const sfem_amd_mesh *m, int Lc) {
+    if (Lc <= 0) Lc = m->level;
+    auto &slot = m->skel[Lc];
+    if (slot) return SFEM_SUCCESS;
+    const int L = m->level, nsk = sk_count(Lc), st = L / Lc;
+    if (L % Lc) {
+        set_last_error("sshex8 skeleton map: level does not divide the mesh level");
+        return SFEM_FAILURE;
+    }
     const ptrdiff_t E = m->E;
     if ((double)E * nsk >= 2147483647.0) {
         set_last_error("sshex8 skeleton map: E * skeleton nodes per element exceeds int32");
@@ -56,15 +66,44 @@ int mesh_skeleton(const sfem_amd_mesh *m) {
     std::vector<int32_t> el((size_t)E * m->nv);
     if (!el.empty())
         SFEM_AMD_HIP_CHECK(hipMemcpy(el.data(), m->idx.p, sizeof(int32_t) * el.size(), hipMemcpyDeviceToHost));
-    // skeleton lidx in slot order
+    // mesh-local index of every skeleton slot of the level-Lc view, in slot order
     std::vector<int> sk_lidx(nsk);
-    for (int z = 0; z <= L; z++)
-        for (int y = 0; y <= L; y++)
-            for (int x = 0; x <= L; x++) {
-                const int q = sk_slot(L, x, y, z);
-                if (q >= 0) sk_lidx[q] = lidx(L, x, y, z);
+    for (int z = 0; z <= Lc; z++)
+        for (int y = 0; y <= Lc; y++)
+            for (int x = 0; x <= Lc; x++) {
+                const int q = sk_slot(Lc, x, y, z);
+                if (q >= 0) sk_lidx[q] = lidx(L, x * st, y * st, z * st);
             }
-    (void)N1;
+    auto store = std::make_unique<SkeletonStore>();
+    if (Lc == L) {
+        // the apply kernels update a macro element's interior nodes without atomics: each must
+        // belong to that element alone (and to no skeleton); count every node's occurrences
+        const int N1 = L + 1;
+        std::vector<uint8_t> seen(m->nnodes, 0);
+        bool exclusive = true;
+        for (ptrdiff_t e = 0; e < E && exclusive; e++)
+            for (int v = 0; v < m->nv; v++) {
+                const int32_t g = el[(size_t)e * m->nv + v];
+                const int x = v % N1, y = (v / N1) % N1, z = v / (N1 * N1);
+                if (sk_slot(L, x, y, z) >= 0) {
+                    seen[g] |= 1;
+                } else {
+                    if (seen[g] & 2) exclusive = false;
+                    seen[g] |= 2;
+                }
+            }
+        bool covers = exclusive;
+        for (ptrdiff_t g = 0; g < m->nnodes && exclusive; g++) {
+            if (!seen[g]) covers = false;
+            if ((seen[g] & 3) == 3) exclusive = false;  // interior here, skeleton elsewhere
+        }
+        if (!exclusive) {
+            set_last_error("sshex8 mesh: an element-interior node is shared with another element (non-conforming "
+                           "connectivity)");
+            return SFEM_FAILURE;
+        }
+        store->covers = covers;
+    }
     std::vector<int32_t> cnt(m->nnodes + 1, 0);
     for (ptrdiff_t e = 0; e < E; e++)
         for (int q = 0; q < nsk; q++) cnt[el[(size_t)e * m->nv + sk_lidx[q]]]++;
@@ -75,27 +114,36 @@ int mesh_skeleton(const sfem_amd_mesh *m) {
             node.push_back((int32_t)g);
             off.push_back(off.back() + cnt[g]);
         }
-    // slot (e, q) -> its position in the node-sorted partials (elements in ascending order)
-    std::vector<int32_t> fill(node.size(), 0), slot_pos((size_t)E * nsk);
+    // slot (e, q) -> its position in the node-sorted list (elements in ascending order), and back
+    std::vector<int32_t> fill(node.size(), 0), slot_pos((size_t)E * nsk), ent((size_t)E * nsk);
     for (ptrdiff_t e = 0; e < E; e++)
         for (int q = 0; q < nsk; q++) {
             const int32_t p = pos[el[(size_t)e * m->nv + sk_lidx[q]]];
-            slot_pos[(size_t)e * nsk + q] = off[p] + fill[p]++;
+            const int32_t j = off[p] + fill[p]++;
+            slot_pos[(size_t)e * nsk + q] = j;
+            ent[j] = (int32_t)(e * nsk + q);
         }
-    if (m->sk_node.upload(node.data(), node.size()) || m->sk_off.upload(off.data(), off.size()) ||
-        m->sk_pos.upload(slot_pos.data(), slot_pos.size()))
+    if (store->node.upload(node.data(), node.size()) || store->off.upload(off.data(), off.size()) ||
+        store->pos.upload(slot_pos.data(), slot_pos.size()) || store->ent.upload(ent.data(), ent.size()))
         return SFEM_FAILURE;
-    m->sk_n = (ptrdiff_t)node.size();
+    store->n = (ptrdiff_t)node.size();
+    slot = std::move(store);
     return SFEM_SUCCESS;
 }
-static SkeletonMap skeleton_of(const sfem_amd_mesh *m) {
-    SkeletonMap sk;
-    sk.node = m->sk_node.p;
-    sk.off = m->sk_off.p;
-    sk.pos = m->sk_pos.p;
-    sk.n = m->sk_n;
+// an empty map when the level has none (E == 0: the kernels then return before launching)
+static SkeletonMap skeleton_of(const sfem_amd_mesh *m, int Lc = 0) {
+    SkeletonMap sk{};
+    const auto it = m->skel.find(Lc > 0 ? Lc : m->level);
+    if (it == m->skel.end() || !it->second) return sk;
+    const SkeletonStore &st = *it->second;
+    sk.node = st.node.p;
+    sk.off = st.off.p;
+    sk.pos = st.pos.p;
+    sk.ent = st.ent.p;
+    sk.n = st.n;
     return sk;
 }
+static bool skeleton_covers(const sfem_amd_mesh *m) { return m->skel.at(m->level)->covers; }
 static int mesh_owner(const sfem_amd_mesh *m, hipStream_t s) {
     if (m->own.p) return SFEM_SUCCESS;
     if (m->own.alloc((size_t)m->E * m->nv)) return SFEM_FAILURE;
@@ -114,11 +162,18 @@ struct LaplacianOp final : OpBase {
     int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
     ptrdiff_t rows() const override { return m->nnodes; }
 
-    int apply(const void *xv, void *yv, hipStream_t s) override {
+    int apply(const void *xv, void *yv, hipStream_t s) override { return apply_impl(xv, yv, s, 0); }
+    int apply_overwrite(const void *xv, void *yv, hipStream_t s) override {
+        if (m->type != SFEM_AMD_SSHEX8) return OpBase::apply_overwrite(xv, yv, s);
+        if (mesh_skeleton(m, 0)) return SFEM_FAILURE;
+        if (!skeleton_covers(m)) return OpBase::apply_overwrite(xv, yv, s);
+        return apply_impl(xv, yv, s, 1);
+    }
+    int apply_impl(const void *xv, void *yv, hipStream_t s, int ow) {
         const T *x = (const T *)xv;
         T *y = (T *)yv;
         if (m->type == SFEM_AMD_SSHEX8 && !part.p && m->E > 0) {
-            if (mesh_skeleton(m) || part.alloc((size_t)m->E * sk_count(m->level))) return SFEM_FAILURE;
+            if (mesh_skeleton(m, 0) || part.alloc((size_t)m->E * sk_count(m->level))) return SFEM_FAILURE;
         }
         if constexpr (std::is_same<T, double>::value) {
             switch (m->type) {
@@ -128,11 +183,11 @@ struct LaplacianOp final : OpBase {
                     return hex8_laplacian_apply(m->E, m->idx.p, fff32.p, coeff.p, x, y, s);
                 default:
                     return sshex8_laplacian_apply<double, float>(m->level, m->E, m->idx.p, fff32.p, m->E, coeff.p,
-                                                                 Kem.p, x, y, part.p, skeleton_of(m), s);
+                                                                 Kem.p, x, y, part.p, skeleton_of(m), s, ow);
             }
         } else {
             return sshex8_laplacian_apply<float, uint16_t>(m->level, m->E, m->idx.p, fff16.p, m->E, nullptr, Kem.p,
-                                                           x, y, part.p, skeleton_of(m), s);
+                                                           x, y, part.p, skeleton_of(m), s, ow);
         }
     }
     int diag(void *dv, hipStream_t s) override {
@@ -161,12 +216,18 @@ struct ElasticityOp final : OpBase {
     DevArray<float> adj, det, coeff;  // hex8: SoA [9][E], [E]
     DevArray<double> K;               // sshex8: per-macro 24x24 matrices
     DevArray<float> K32;              // sshex8, FP32 operator: K rounded to float
-    DevArray<T> Sst;                  // sshex8 at large L: merged 27-point stencils [E][243]
     DevArray<T> part;                 // sshex8: skeleton contributions [E][sk_count(L)][3]
     double mu{0}, lambda{0};
     int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
     ptrdiff_t rows() const override { return m->nnodes * 3; }
-    int apply(const void *xv, void *yv, hipStream_t s) override {
+    int apply(const void *xv, void *yv, hipStream_t s) override { return apply_impl(xv, yv, s, 0); }
+    int apply_overwrite(const void *xv, void *yv, hipStream_t s) override {
+        if (m->type != SFEM_AMD_SSHEX8) return OpBase::apply_overwrite(xv, yv, s);
+        if (mesh_skeleton(m, 0)) return SFEM_FAILURE;
+        if (!skeleton_covers(m)) return OpBase::apply_overwrite(xv, yv, s);
+        return apply_impl(xv, yv, s, 1);
+    }
+    int apply_impl(const void *xv, void *yv, hipStream_t s, int ow) {
         if (m->type == SFEM_AMD_HEX8) {
             if constexpr (std::is_same<T, double>::value)
                 return hex8_elasticity_apply(m->E, m->idx.p, adj.p, det.p, mu, lambda, coeff.p, (const double *)xv,
@@ -175,7 +236,7 @@ struct ElasticityOp final : OpBase {
             return SFEM_FAILURE;
         }
         if (!part.p && m->E > 0) {
-            if (mesh_skeleton(m) || part.alloc((size_t)m->E * sk_count(m->level) * 3)) return SFEM_FAILURE;
+            if (mesh_skeleton(m, 0) || part.alloc((size_t)m->E * sk_count(m->level) * 3)) return SFEM_FAILURE;
         }
         const T *Kt;
         if constexpr (std::is_same<T, double>::value) {
@@ -186,11 +247,8 @@ struct ElasticityOp final : OpBase {
             }
             Kt = K32.p;
         }
-        if (sshex8_elasticity_uses_stencils(m->level) && !Sst.p && m->E > 0) {
-            if (Sst.alloc((size_t)m->E * 243) || sshex8_elasticity_stencils<T>(m->E, K.p, Sst.p, s)) return SFEM_FAILURE;
-        }
-        return sshex8_elasticity_apply<T>(m->level, m->E, m->idx.p, Kt, Sst.p, (const T *)xv, (T *)yv, part.p,
-                                          skeleton_of(m), s);
+        return sshex8_elasticity_apply<T>(m->level, m->E, m->idx.p, Kt, (const T *)xv, (T *)yv, part.p,
+                                          skeleton_of(m), s, ow);
     }
     int diag(void *d, hipStream_t s) override {
         if (m->type == SFEM_AMD_HEX8) {
@@ -216,6 +274,10 @@ struct ConstrainedOp final : OpBase {
     DevArray<int32_t> dofs;
     int apply(const void *x, void *y, hipStream_t s) override {
         if (inner->apply(x, y, s)) return SFEM_FAILURE;
+        return copy_constrained(x, y, s);
+    }
+    int apply_overwrite(const void *x, void *y, hipStream_t s) override {
+        if (inner->apply_overwrite(x, y, s)) return SFEM_FAILURE;
         return copy_constrained(x, y, s);
     }
     int copy_constrained(const void *x, void *y, hipStream_t s) const {
@@ -247,6 +309,11 @@ struct TransferOp final : OpBase {
     int vs{1};
     bool prolong{true};
     std::shared_ptr<OpBase> constrained;  // ConstrainedOp on the output level (or null)
+    DevArray<T> part;                     // restriction: coarse skeleton contributions
+    // the element tile fits the default LDS budget: use the LDS-staged kernels
+    bool staged() const {
+        return (vs == 1 || vs == 3) && sshex8_transfer_lds_bytes(prolong ? Lc : fine->level, vs) <= 48 * 1024;
+    }
     int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
     ptrdiff_t rows() const override { return prolong ? fine->nnodes * vs : n_coarse * vs; }
     ptrdiff_t cols() const override { return prolong ? n_coarse * vs : fine->nnodes * vs; }
@@ -254,10 +321,25 @@ struct TransferOp final : OpBase {
         int err;
         if (prolong) {
             if (mesh_owner(fine, s)) return SFEM_FAILURE;
-            err = sshex8_prolongate<T>(fine->E, fine->level, Lc, fine->idx.p, fine->own.p, vs, (const T *)x, (T *)y, s);
+            if (staged())
+                err = sshex8_prolongate_lds<T>(fine->E, fine->level, Lc, fine->idx.p, fine->own.p, vs, (const T *)x,
+                                               (T *)y, s);
+            else
+                err = sshex8_prolongate<T>(fine->E, fine->level, Lc, fine->idx.p, fine->own.p, vs, (const T *)x,
+                                           (T *)y, s);
         } else {
             if (mesh_counts(fine, s)) return SFEM_FAILURE;
-            err = sshex8_restrict<T>(fine->E, fine->level, Lc, fine->idx.p, fine->count.p, vs, (const T *)x, (T *)y, s);
+            if (staged()) {
+                if (mesh_skeleton(fine, Lc)) return SFEM_FAILURE;
+                if (!part.p && part.alloc((size_t)fine->E * sk_count(Lc) * vs)) return SFEM_FAILURE;
+                const SkeletonMap sk = skeleton_of(fine, Lc);
+                err = sshex8_restrict_lds<T>(fine->E, fine->level, Lc, fine->idx.p, fine->count.p, vs, (const T *)x,
+                                             (T *)y, part.p, sk.pos, s);
+                if (!err) err = skeleton_gather<T>(sk, vs, part.p, (T *)y, s, 0);
+            } else {
+                err = sshex8_restrict<T>(fine->E, fine->level, Lc, fine->idx.p, fine->count.p, vs, (const T *)x,
+                                         (T *)y, s);
+            }
         }
         if (err) return err;
         if (constrained) return static_cast<ConstrainedOp *>(constrained.get())->set_value_rt(real_type(), 0.0, y, s);
@@ -301,17 +383,20 @@ template <typename T>
 struct OpView final : sfem::Operator<T> {
     std::shared_ptr<OpBase> impl;
     hipStream_t s{nullptr};
-    explicit OpView(std::shared_ptr<OpBase> o) : impl(std::move(o)) {}
-    int apply(const T *const x, T *const y) override { return impl->apply(x, y, as_stream(s)); }
+    bool overwrite{false};  // apply() overwrites y (the owning solver skips its zero fill)
+    explicit OpView(std::shared_ptr<OpBase> o, bool ow = false) : impl(std::move(o)), overwrite(ow) {}
+    int apply(const T *const x, T *const y) override {
+        return overwrite ? impl->apply_overwrite(x, y, as_stream(s)) : impl->apply(x, y, as_stream(s));
+    }
     std::ptrdiff_t rows() const override { return impl->rows(); }
     std::ptrdiff_t cols() const override { return impl->cols(); }
     sfem::ExecutionSpace execution_space() const override { return sfem::EXECUTION_SPACE_DEVICE; }
 };
 
 template <typename T>
-static sfem::SharedOperator<T> view(const std::shared_ptr<OpBase> &o) {
+static sfem::SharedOperator<T> view(const std::shared_ptr<OpBase> &o, bool overwrite = false) {
     if (!o) return nullptr;
-    return std::make_shared<OpView<T>>(o);
+    return std::make_shared<OpView<T>>(o, overwrite);
 }
 
 template <typename T>
@@ -748,17 +833,28 @@ int sfem_amd_linear_elasticity_create(const sfem_amd_mesh *mesh, const float *ad
     return SFEM_SUCCESS;
 }
 
+// host-side set-up inside an apply (maps, staging vectors) may throw: no exception crosses the C ABI
+static int guarded(const std::function<int()> &f) {
+    try {
+        return f();
+    } catch (const std::exception &e) {
+        set_last_error(std::string("host exception: ") + e.what());
+    } catch (...) {
+        set_last_error("host exception");
+    }
+    return SFEM_FAILURE;
+}
 int sfem_amd_op_apply(sfem_amd_op *op, const void *x, void *y, void *stream) {
     CHECK_HANDLE(op);
-    return op->impl->apply(x, y, as_stream(stream));
+    return guarded([&] { return op->impl->apply(x, y, as_stream(stream)); });
 }
 int sfem_amd_op_diag(sfem_amd_op *op, void *d, void *stream) {
     CHECK_HANDLE(op);
-    return op->impl->diag(d, as_stream(stream));
+    return guarded([&] { return op->impl->diag(d, as_stream(stream)); });
 }
 int sfem_amd_op_block_diag_sym(sfem_amd_op *op, void *d, void *stream) {
     CHECK_HANDLE(op);
-    return op->impl->block_diag(d, as_stream(stream));
+    return guarded([&] { return op->impl->block_diag(d, as_stream(stream)); });
 }
 ptrdiff_t sfem_amd_op_rows(const sfem_amd_op *op) { return op ? op->impl->rows() : -1; }
 ptrdiff_t sfem_amd_op_cols(const sfem_amd_op *op) { return op ? op->impl->cols() : -1; }
@@ -942,7 +1038,8 @@ static int cg_create_t(sfem_amd_op *A, sfem_amd_op *P, int flexible, double rtol
                        sfem_amd_op **out) {
     auto cg = std::make_shared<sfem::ConjugateGradient<T>>();
     cg->default_init();
-    cg->set_op(view<T>(A->impl));
+    cg->set_op(view<T>(A->impl, true));
+    cg->apply_overwrites_output = true;
     if (P) cg->set_preconditioner_op(view<T>(P->impl));
     cg->flexible = flexible == 1;
     cg->stop_on_residual_norm = flexible == 2;
@@ -975,7 +1072,8 @@ static int cheb3_create_t(sfem_amd_op *A, int max_it, double smax, double smin, 
                           const void *init, sfem_amd_op **out, const void *inv_diag = nullptr) {
     auto c = std::make_shared<sfem::Chebyshev3<T>>();
     c->default_init();
-    c->set_op(view<T>(A->impl));
+    c->set_op(view<T>(A->impl, true));
+    c->apply_overwrites_output = true;
     if (inv_diag) {
         const ptrdiff_t n = A->impl->rows();
         auto b = c->blas;
@@ -1127,7 +1225,9 @@ template <typename T>
 static int mg_add_level_t(SolverOp<T> *s, sfem_amd_op *op, sfem_amd_op *sm, sfem_amd_op *P, sfem_amd_op *R) {
     auto *mg = dynamic_cast<sfem::Multigrid<T> *>(s->op.get());
     SFEM_AMD_REQUIRE(mg, "mg_add_level: not a multigrid handle");
-    mg->add_level(view<T>(op->impl), view<T>(sm->impl), P ? view<T>(P->impl) : nullptr, R ? view<T>(R->impl) : nullptr);
+    mg->add_level(view<T>(op->impl, true), view<T>(sm->impl), P ? view<T>(P->impl) : nullptr,
+                  R ? view<T>(R->impl) : nullptr);
+    mg->apply_overwrites_output = true;
     if (s->n == 0) s->n = op->impl->rows();
     return SFEM_SUCCESS;
 }

@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
     sshex8_laplacian_apply_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx, const G *__restrict__ fff,
                                   const ptrdiff_t fstride, const float *__restrict__ coeff,
                                   const double *__restrict__ Kem, const T *__restrict__ x, T *__restrict__ y,
-                                  T *__restrict__ part, const int32_t *__restrict__ sk_pos) {
+                                  T *__restrict__ part, const int32_t *__restrict__ sk_pos, const int overwrite) {
     using S = LapShape<L>;
     constexpr int N1 = S::N1, NXE = S::NXE, LINES = S::LINES, EPB = S::EPB, NSK = sk_count(L);
     __shared__ T su[EPB][NXE];
@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
         for (int i = 0; i < N1; i++) su[le][lidx(L, i, yy, zz)] = x[gid[i]];
         if (!skl) {
 #pragma unroll
-            for (int i = 1; i < L; i++) old[i] = y[gid[i]];
+            for (int i = 1; i < L; i++) old[i] = overwrite ? T(0) : y[gid[i]];
         }
     }
     lap_block_matrices<L, EPB, S::BLOCK, G>(E, e0, fff, fstride, coeff, Kem, sK);
@@ -211,20 +211,21 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
 // element contributions, in slot order.
 template <typename T, int BS>
 __global__ void skeleton_gather_kernel(const ptrdiff_t n, const int32_t *__restrict__ node,
-                                       const int32_t *__restrict__ off, const T *__restrict__ part,
-                                       T *__restrict__ y) {
+                                       const int32_t *__restrict__ off, const int32_t *__restrict__ ent,
+                                       const T *__restrict__ part, T *__restrict__ y, const int overwrite) {
     for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
         const int j0 = off[t], j1 = off[t + 1];
         T acc[BS];
 #pragma unroll
         for (int c = 0; c < BS; c++) acc[c] = 0;
         for (int j = j0; j < j1; j++) {
+            const ptrdiff_t q = (ent ? (ptrdiff_t)ent[j] : (ptrdiff_t)j) * BS;
 #pragma unroll
-            for (int c = 0; c < BS; c++) acc[c] += part[(ptrdiff_t)j * BS + c];
+            for (int c = 0; c < BS; c++) acc[c] += part[q + c];
         }
         const ptrdiff_t g = (ptrdiff_t)node[t] * BS;
 #pragma unroll
-        for (int c = 0; c < BS; c++) y[g + c] += acc[c];
+        for (int c = 0; c < BS; c++) y[g + c] = overwrite ? acc[c] : y[g + c] + acc[c];
     }
 }
 
@@ -297,12 +298,13 @@ struct ElShape {
 // Each node sums over the (up to 8) micro elements that contain it, 3 rows x 24 columns
 // of the per-macro matrix each (K rows broadcast from LDS).  Interior nodes update y (their
 // y is loaded with the next plane of x); skeleton nodes store their three components to the
-// element's slots of `part`, summed per node by skeleton_gather_kernel (no atomics).
+// element's own slots of `part` (element-major: 24-byte runs), summed per node by
+// skeleton_gather_kernel through the node-sorted slot list (no atomics).
 template <typename T, int L>
 __global__ void __launch_bounds__(ElShape<L>::BLOCK)
     sshex8_elasticity_apply_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
                                    const T *__restrict__ Kall, const T *__restrict__ x,
-                                   T *__restrict__ y, T *__restrict__ part, const int32_t *__restrict__ sk_pos) {
+                                   T *__restrict__ y, T *__restrict__ part, const int overwrite) {
     using S = ElShape<L>;
     constexpr int N1 = S::N1, NXE = N1 * N1 * N1, PLANE = S::PLANE, EPB = S::EPB, NSK = sk_count(L);
     __shared__ T su[EPB][3][PLANE][3];
@@ -337,13 +339,13 @@ __global__ void __launch_bounds__(ElShape<L>::BLOCK)
     };
     load_plane(0);
     __syncthreads();
-    const int32_t *pe = sk_pos + e * (ptrdiff_t)NSK;
+    T *pe = part + e * (ptrdiff_t)NSK * 3;  // the element's skeleton slots (element-major)
 
     for (int z = 0; z <= L; z++) {
         const int32_t g = g_next;
         const bool interior = !edge_xy && z > 0 && z < L;
         T yold[3] = {0, 0, 0};
-        if (active && interior) {
+        if (active && interior && !overwrite) {
 #pragma unroll
             for (int c = 0; c < 3; c++) yold[c] = y[(ptrdiff_t)g * 3 + c];
         }
@@ -384,149 +386,7 @@ __global__ void __launch_bounds__(ElShape<L>::BLOCK)
 #pragma unroll
                 for (int ci = 0; ci < 3; ci++) y[(ptrdiff_t)g * 3 + ci] = yold[ci] + acc[ci];
             } else {
-                T *q = part + (ptrdiff_t)pe[sk_slot(L, xx, yy, z)] * 3;
-#pragma unroll
-                for (int ci = 0; ci < 3; ci++) q[ci] = acc[ci];
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// Merged 3x3-block 27-point stencil of a node with all 8 micro elements inside the macro
-// element: S[off][ci][cj] = sum over the node's corner c of each micro element of
-// K[(ci*8 + c)*24 + cj*8 + j], j the corner at offset off from c.  [E][27][3][3], T precision.
-template <typename T>
-__global__ void sshex8_elasticity_stencil_kernel(const ptrdiff_t E, const double *__restrict__ Kall,
-                                                 T *__restrict__ Sall) {
-    const ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= E * 243) return;
-    const ptrdiff_t e = t / 243;
-    const int r = (int)(t - e * 243);
-    const int off = r / 9, ci = (r / 3) % 3, cj = r % 3;
-    const int dz = off / 9 - 1, dy = (off / 3) % 3 - 1, dx = off % 3 - 1;
-    const double *K = Kall + e * 576;
-    double v = 0;
-    for (int c = 0; c < 8; c++) {
-        const int jx = hex8_cx(c) + dx, jy = hex8_cy(c) + dy, jz = hex8_cz(c) + dz;
-        if (jx < 0 || jx > 1 || jy < 0 || jy > 1 || jz < 0 || jz > 1) continue;
-        v += K[(ci * 8 + c) * 24 + cj * 8 + hex8_corner(jx, jy, jz)];
-    }
-    Sall[t] = (T)v;
-}
-
-// One macro element per block, planes in z as in sshex8_elasticity_apply_kernel, but the threads
-// of a plane are split so that every wave takes one path: the (L-1)^2 nodes inside the plane
-// first (waves of interior nodes), then the 4L nodes of the plane's boundary ring (their own
-// waves).  On the planes 0 < z < L the interior threads use the merged stencil (243 FMA per node,
-// coefficients read into scalar registers); ring threads and the planes z = 0, L sum their micro
-// elements (576 FMA per node).
-template <int L>
-struct ElShape1 {
-    static constexpr int N1 = L + 1;
-    static constexpr int PLANE = N1 * N1;
-    static constexpr int NI = (L - 1) * (L - 1);
-    static constexpr int RING = 4 * L;
-    static constexpr int TI = ((NI + 63) / 64) * 64;
-    static constexpr int BLOCK = TI + ((RING + 63) / 64) * 64;
-};
-
-template <typename T, int L>
-__global__ void __launch_bounds__(ElShape1<L>::BLOCK)
-    sshex8_elasticity_apply1_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx, const T *__restrict__ Kall,
-                                    const T *__restrict__ Sall, const T *__restrict__ x, T *__restrict__ y,
-                                    T *__restrict__ part, const int32_t *__restrict__ sk_pos) {
-    using S = ElShape1<L>;
-    constexpr int N1 = S::N1, NXE = N1 * N1 * N1, PLANE = S::PLANE, NSK = sk_count(L);
-    __shared__ T su[3][PLANE][3];
-    const int tid = threadIdx.x;
-    const ptrdiff_t e = blockIdx.x;
-    if (e >= E) return;  // uniform per block
-    // thread -> node of the plane
-    bool active, inner;
-    int xx, yy;
-    if (tid < S::TI) {
-        active = tid < S::NI;
-        inner = true;
-        xx = 1 + tid % (L - 1);
-        yy = 1 + tid / (L - 1);
-    } else {
-        const int k = tid - S::TI;
-        active = k < S::RING;
-        inner = false;
-        const int side = k / L, q = k % L;
-        xx = side == 0 ? q : side == 1 ? L : side == 2 ? L - q : 0;
-        yy = side == 0 ? 0 : side == 1 ? q : side == 2 ? L : L - q;
-    }
-    if (!active) xx = yy = 0;
-    const int p = xx + yy * N1;
-    const int32_t *row = idx + e * NXE;
-    const T *K = Kall + e * 576;
-    const T *Se = Sall + e * 243;
-    const int32_t *pe = sk_pos + e * (ptrdiff_t)NSK;
-    int32_t g_next = 0;
-    auto load_plane = [&](int z) {
-        if (active) {
-            g_next = row[lidx(L, xx, yy, z)];
-#pragma unroll
-            for (int c = 0; c < 3; c++) su[z % 3][p][c] = x[(ptrdiff_t)g_next * 3 + c];
-        }
-    };
-    load_plane(0);
-    __syncthreads();
-    for (int z = 0; z <= L; z++) {
-        const int32_t g = g_next;
-        const bool merged = inner && z > 0 && z < L;  // uniform per wave
-        T yold[3] = {0, 0, 0};
-        if (active && merged) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) yold[c] = y[(ptrdiff_t)g * 3 + c];
-        }
-        if (z + 1 <= L) load_plane(z + 1);
-        __syncthreads();
-        if (active) {
-            T acc[3] = {0, 0, 0};
-            if (merged) {
-#pragma unroll
-                for (int off = 0; off < 27; off++) {
-                    const int dz = off / 9 - 1, dy = (off / 3) % 3 - 1, dx = off % 3 - 1;
-                    const T *src = su[(z + dz + 3) % 3][(xx + dx) + (yy + dy) * N1];
-                    const T u0 = src[0], u1 = src[1], u2 = src[2];
-                    const T *Sk = Se + off * 9;
-#pragma unroll
-                    for (int ci = 0; ci < 3; ci++) acc[ci] += Sk[ci * 3 + 0] * u0 + Sk[ci * 3 + 1] * u1 + Sk[ci * 3 + 2] * u2;
-                }
-            } else {
-#pragma unroll
-                for (int c = 0; c < 8; c++) {
-                    const int cx = hex8_cx(c), cy = hex8_cy(c), cz = hex8_cz(c);
-                    const bool ok = (cx ? xx > 0 : xx < L) && (cy ? yy > 0 : yy < L) && (cz ? z > 0 : z < L);
-                    if (ok) {
-                        T ue[24];
-#pragma unroll
-                        for (int j = 0; j < 8; j++) {
-                            const int nx = xx - cx + hex8_cx(j), ny = yy - cy + hex8_cy(j), nz = z - cz + hex8_cz(j);
-                            const T *src = su[nz % 3][nx + ny * N1];
-                            ue[0 * 8 + j] = src[0];
-                            ue[1 * 8 + j] = src[1];
-                            ue[2 * 8 + j] = src[2];
-                        }
-#pragma unroll
-                        for (int ci = 0; ci < 3; ci++) {
-                            const T *Kr = K + (ci * 8 + c) * 24;
-                            T sacc = 0;
-#pragma unroll
-                            for (int j = 0; j < 24; j++) sacc += Kr[j] * ue[j];
-                            acc[ci] += sacc;
-                        }
-                    }
-                }
-            }
-            if (merged) {
-#pragma unroll
-                for (int ci = 0; ci < 3; ci++) y[(ptrdiff_t)g * 3 + ci] = yold[ci] + acc[ci];
-            } else {
-                T *q = part + (ptrdiff_t)pe[sk_slot(L, xx, yy, z)] * 3;
+                T *q = pe + (ptrdiff_t)sk_slot(L, xx, yy, z) * 3;
 #pragma unroll
                 for (int ci = 0; ci < 3; ci++) q[ci] = acc[ci];
             }
@@ -580,12 +440,16 @@ bool sshex8_level_supported(int L) {
 }
 
 template <typename T>
-int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStream_t s) {
+int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStream_t s, int overwrite) {
     if (sk.n == 0) return SFEM_SUCCESS;
+    // scalar partials are node-sorted (read in place); vector partials are element-major (read
+    // through ent)
     if (bs == 1)
-        skeleton_gather_kernel<T, 1><<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, part, y);
+        skeleton_gather_kernel<T, 1>
+                <<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, nullptr, part, y, overwrite);
     else if (bs == 3)
-        skeleton_gather_kernel<T, 3><<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, part, y);
+        skeleton_gather_kernel<T, 3>
+                <<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, sk.ent, part, y, overwrite);
     else {
         set_last_error("skeleton_gather: block size must be 1 or 3");
         return SFEM_FAILURE;
@@ -597,7 +461,7 @@ int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStrea
 template <typename T, typename G>
 int sshex8_laplacian_apply(int L, ptrdiff_t E, const int32_t *idx, const G *fff, ptrdiff_t fstride,
                            const float *coeff, const double *Kem, const T *x, T *y, T *part, const SkeletonMap &sk,
-                           hipStream_t s) {
+                           hipStream_t s, int overwrite) {
     if (E == 0) return SFEM_SUCCESS;
     switch (L) {
 #define CASE(LV)                                                                                                  \
@@ -605,7 +469,7 @@ int sshex8_laplacian_apply(int L, ptrdiff_t E, const int32_t *idx, const G *fff,
         using Sh = LapShape<LV>;                                                                                  \
         const unsigned grid = (unsigned)((E + Sh::EPB - 1) / Sh::EPB);                                            \
         sshex8_laplacian_apply_kernel<T, G, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, fff, fstride, coeff, Kem, x, y, part, \
-                                                                             sk.pos);                                \
+                                                                             sk.pos, overwrite);                     \
         break;                                                                                                    \
     }
         SFEM_AMD_SSHEX8_LEVELS(CASE)
@@ -615,7 +479,7 @@ int sshex8_laplacian_apply(int L, ptrdiff_t E, const int32_t *idx, const G *fff,
             return SFEM_FAILURE;
     }
     SFEM_AMD_HIP_CHECK(hipGetLastError());
-    return skeleton_gather<T>(sk, 1, part, y, s);
+    return skeleton_gather<T>(sk, 1, part, y, s, overwrite);
 }
 
 template <typename T, typename G>
@@ -652,36 +516,16 @@ int sshex8_elasticity_matrices(int L, ptrdiff_t E, const G *adj, const G *det, p
 }
 
 template <typename T>
-int sshex8_elasticity_stencils(ptrdiff_t E, const double *K, T *S, hipStream_t s) {
+int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *x, T *y, T *part,
+                            const SkeletonMap &sk, hipStream_t s, int overwrite) {
     if (E == 0) return SFEM_SUCCESS;
-    const ptrdiff_t n = E * 243;
-    sshex8_elasticity_stencil_kernel<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(E, K, S);
-    SFEM_AMD_HIP_CHECK(hipGetLastError());
-    return SFEM_SUCCESS;
-}
-
-bool sshex8_elasticity_uses_stencils(int L) { return L >= 12; }
-
-template <typename T>
-int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *Sst, const T *x, T *y,
-                            T *part, const SkeletonMap &sk, hipStream_t s) {
-    if (E == 0) return SFEM_SUCCESS;
-    if (L >= 12 && !Sst) {
-        set_last_error("sshex8 elasticity: merged stencils required at this level");
-        return SFEM_FAILURE;
-    }
     switch (L) {
-#define CASE(LV)                                                                                           \
-    case LV: {                                                                                             \
-        if constexpr (LV >= 12) {                                                                          \
-            sshex8_elasticity_apply1_kernel<T, LV>                                                         \
-                    <<<(unsigned)E, ElShape1<LV>::BLOCK, 0, s>>>(E, idx, K, Sst, x, y, part, sk.pos);      \
-        } else {                                                                                           \
-            using Sh = ElShape<LV>;                                                                        \
-            const unsigned grid = (unsigned)((E + Sh::EPB - 1) / Sh::EPB);                                 \
-            sshex8_elasticity_apply_kernel<T, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, K, x, y, part, sk.pos); \
-        }                                                                                                  \
-        break;                                                                                             \
+#define CASE(LV)                                                                                        \
+    case LV: {                                                                                          \
+        using Sh = ElShape<LV>;                                                                         \
+        const unsigned grid = (unsigned)((E + Sh::EPB - 1) / Sh::EPB);                                  \
+        sshex8_elasticity_apply_kernel<T, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, K, x, y, part, overwrite); \
+        break;                                                                                          \
     }
         SFEM_AMD_SSHEX8_LEVELS(CASE)
 #undef CASE
@@ -690,7 +534,7 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
             return SFEM_FAILURE;
     }
     SFEM_AMD_HIP_CHECK(hipGetLastError());
-    return skeleton_gather<T>(sk, 3, part, y, s);
+    return skeleton_gather<T>(sk, 3, part, y, s, overwrite);
 }
 
 template <typename T>
@@ -713,14 +557,14 @@ int sshex8_elasticity_diag(int L, ptrdiff_t E, const int32_t *idx, const double 
     return SFEM_SUCCESS;
 }
 
-template int skeleton_gather<double>(const SkeletonMap &, int, const double *, double *, hipStream_t);
-template int skeleton_gather<float>(const SkeletonMap &, int, const float *, float *, hipStream_t);
+template int skeleton_gather<double>(const SkeletonMap &, int, const double *, double *, hipStream_t, int);
+template int skeleton_gather<float>(const SkeletonMap &, int, const float *, float *, hipStream_t, int);
 template int sshex8_laplacian_apply<double, float>(int, ptrdiff_t, const int32_t *, const float *, ptrdiff_t,
                                                    const float *, const double *, const double *, double *, double *,
-                                                   const SkeletonMap &, hipStream_t);
+                                                   const SkeletonMap &, hipStream_t, int);
 template int sshex8_laplacian_apply<float, uint16_t>(int, ptrdiff_t, const int32_t *, const uint16_t *, ptrdiff_t,
                                                      const float *, const double *, const float *, float *, float *,
-                                                     const SkeletonMap &, hipStream_t);
+                                                     const SkeletonMap &, hipStream_t, int);
 template int sshex8_laplacian_diag<double, float>(int, ptrdiff_t, const int32_t *, const float *, ptrdiff_t,
                                                   const float *, const double *, double *, hipStream_t);
 template int sshex8_laplacian_diag<float, uint16_t>(int, ptrdiff_t, const int32_t *, const uint16_t *, ptrdiff_t,
@@ -730,11 +574,9 @@ template int sshex8_elasticity_matrices<float>(int, ptrdiff_t, const float *, co
 template int sshex8_elasticity_matrices<uint16_t>(int, ptrdiff_t, const uint16_t *, const uint16_t *, ptrdiff_t,
                                                   double, double, const float *, double *, hipStream_t);
 template int sshex8_elasticity_apply<double>(int, ptrdiff_t, const int32_t *, const double *, const double *,
-                                             const double *, double *, double *, const SkeletonMap &, hipStream_t);
-template int sshex8_elasticity_apply<float>(int, ptrdiff_t, const int32_t *, const float *, const float *,
-                                            const float *, float *, float *, const SkeletonMap &, hipStream_t);
-template int sshex8_elasticity_stencils<double>(ptrdiff_t, const double *, double *, hipStream_t);
-template int sshex8_elasticity_stencils<float>(ptrdiff_t, const double *, float *, hipStream_t);
+                                             double *, double *, const SkeletonMap &, hipStream_t, int);
+template int sshex8_elasticity_apply<float>(int, ptrdiff_t, const int32_t *, const float *, const float *, float *,
+                                            float *, const SkeletonMap &, hipStream_t, int);
 template int sshex8_elasticity_diag<double>(int, ptrdiff_t, const int32_t *, const double *, int, double *,
                                             hipStream_t);
 template int sshex8_elasticity_diag<float>(int, ptrdiff_t, const int32_t *, const double *, int, float *,

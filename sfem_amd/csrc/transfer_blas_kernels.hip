@@ -88,6 +88,115 @@ __global__ void sshex8_restrict_kernel(const ptrdiff_t E, const int Lf, const in
     }
 }
 
+// LDS-staged variants, one macro element per block.
+// Prolongation: the element's coarse values are staged in LDS once, then every owned fine node
+// interpolates from LDS (the per-node variant above re-reads up to 8 coarse values and their ids
+// from memory for every fine node).
+template <typename T, int VS>
+__global__ void __launch_bounds__(256) sshex8_prolongate_lds_kernel(const ptrdiff_t E, const int Lf, const int Lc,
+                                                                    const int32_t *__restrict__ idx,
+                                                                    const uint8_t *__restrict__ own,
+                                                                    const T *__restrict__ from, T *__restrict__ to) {
+    extern __shared__ double tr_smem[];
+    const ptrdiff_t e = blockIdx.x;
+    if (e >= E) return;  // uniform per block
+    const int N1 = Lf + 1, nxe = N1 * N1 * N1, s = Lf / Lc, C1 = Lc + 1, cnxe = C1 * C1 * C1;
+    const int32_t *row = idx + e * nxe;
+    for (int c = threadIdx.x; c < cnxe; c += blockDim.x) {
+        const int X = c % C1, Y = (c / C1) % C1, Z = c / (C1 * C1);
+        const ptrdiff_t cg = row[lidx(Lf, X * s, Y * s, Z * s)];
+#pragma unroll
+        for (int d = 0; d < VS; d++) tr_smem[c * VS + d] = (double)from[cg * VS + d];
+    }
+    __syncthreads();
+    const uint8_t *ow = own + e * nxe;
+    for (int v = threadIdx.x; v < nxe; v += blockDim.x) {
+        if (!ow[v]) continue;
+        const int x = v % N1, y = (v / N1) % N1, z = v / (N1 * N1);
+        int cx = x / s, cy = y / s, cz = z / s;
+        if (cx == Lc) cx--;
+        if (cy == Lc) cy--;
+        if (cz == Lc) cz--;
+        const double tx = (double)(x - cx * s) / s, ty = (double)(y - cy * s) / s, tz = (double)(z - cz * s) / s;
+        double acc[VS];
+#pragma unroll
+        for (int d = 0; d < VS; d++) acc[d] = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int bx = hex8_cx(k), by = hex8_cy(k), bz = hex8_cz(k);
+            const double w = (bx ? tx : 1 - tx) * (by ? ty : 1 - ty) * (bz ? tz : 1 - tz);
+            if (w == 0) continue;
+            const double *src = tr_smem + lidx(Lc, cx + bx, cy + by, cz + bz) * VS;
+#pragma unroll
+            for (int d = 0; d < VS; d++) acc[d] += w * src[d];
+        }
+        const ptrdiff_t g = row[v];
+#pragma unroll
+        for (int d = 0; d < VS; d++) to[g * VS + d] = (T)acc[d];
+    }
+}
+
+// Restriction: the element's fine values divided by their incidence counts are staged in LDS;
+// each coarse node of the element sums its (2s-1)^3 neighbourhood from LDS.  Coarse nodes inside
+// the element (owned by it alone) are updated in place; coarse skeleton nodes store their
+// element contribution (scalar: node-sorted position pos[e][slot]; vector: element-major slot)
+// for the skeleton gather.  No atomics, deterministic.
+template <typename T, int VS>
+__global__ void __launch_bounds__(256) sshex8_restrict_lds_kernel(const ptrdiff_t E, const int Lf, const int Lc,
+                                                                  const int32_t *__restrict__ idx,
+                                                                  const uint16_t *__restrict__ count,
+                                                                  const T *__restrict__ from, T *__restrict__ to,
+                                                                  T *__restrict__ part,
+                                                                  const int32_t *__restrict__ cpos) {
+    extern __shared__ double tr_smem[];
+    const ptrdiff_t e = blockIdx.x;
+    if (e >= E) return;  // uniform per block
+    const int N1 = Lf + 1, nxe = N1 * N1 * N1, s = Lf / Lc, C1 = Lc + 1, cnxe = C1 * C1 * C1;
+    const int nskc = sk_count(Lc);
+    const int32_t *row = idx + e * nxe;
+    for (int v = threadIdx.x; v < nxe; v += blockDim.x) {
+        const ptrdiff_t g = row[v];
+        const double inv = 1.0 / (double)count[g];
+#pragma unroll
+        for (int d = 0; d < VS; d++) tr_smem[v * VS + d] = (double)from[g * VS + d] * inv;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < cnxe; c += blockDim.x) {
+        const int X = c % C1, Y = (c / C1) % C1, Z = c / (C1 * C1);
+        double acc[VS];
+#pragma unroll
+        for (int d = 0; d < VS; d++) acc[d] = 0;
+        for (int k = -(s - 1); k <= s - 1; k++) {
+            const int z = Z * s + k;
+            if (z < 0 || z > Lf) continue;
+            const double wz = 1.0 - (double)abs(k) / s;
+            for (int j = -(s - 1); j <= s - 1; j++) {
+                const int y = Y * s + j;
+                if (y < 0 || y > Lf) continue;
+                const double wy = 1.0 - (double)abs(j) / s;
+                for (int i = -(s - 1); i <= s - 1; i++) {
+                    const int x = X * s + i;
+                    if (x < 0 || x > Lf) continue;
+                    const double w = (1.0 - (double)abs(i) / s) * wy * wz;
+                    const double *src = tr_smem + lidx(Lf, x, y, z) * VS;
+#pragma unroll
+                    for (int d = 0; d < VS; d++) acc[d] += w * src[d];
+                }
+            }
+        }
+        const int q = sk_slot(Lc, X, Y, Z);
+        if (q < 0) {
+            const ptrdiff_t cg = row[lidx(Lf, X * s, Y * s, Z * s)];
+#pragma unroll
+            for (int d = 0; d < VS; d++) to[cg * VS + d] += (T)acc[d];
+        } else {
+            const ptrdiff_t slot = VS == 1 ? (ptrdiff_t)cpos[e * nskc + q] : (e * nskc + q);
+#pragma unroll
+            for (int d = 0; d < VS; d++) part[slot * VS + d] = (T)acc[d];
+        }
+    }
+}
+
 __global__ void node_owner_kernel(const ptrdiff_t E, const int nxe, const int32_t *__restrict__ idx,
                                   int32_t *__restrict__ owner) {
     const ptrdiff_t n = E * nxe;
@@ -159,6 +268,40 @@ int sshex8_restrict(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint1
     return SFEM_SUCCESS;
 }
 
+size_t sshex8_transfer_lds_bytes(int L, int vs) { return sizeof(double) * (size_t)nxe_of(L) * vs; }
+
+template <typename T>
+int sshex8_prolongate_lds(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint8_t *own, int vs, const T *from,
+                          T *to, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    const size_t lds = sshex8_transfer_lds_bytes(Lc, vs);
+    if (vs == 1)
+        sshex8_prolongate_lds_kernel<T, 1><<<(unsigned)E, 256, lds, s>>>(E, Lf, Lc, idx, own, from, to);
+    else if (vs == 3)
+        sshex8_prolongate_lds_kernel<T, 3><<<(unsigned)E, 256, lds, s>>>(E, Lf, Lc, idx, own, from, to);
+    else
+        return sshex8_prolongate<T>(E, Lf, Lc, idx, own, vs, from, to, s);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
+template <typename T>
+int sshex8_restrict_lds(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint16_t *count, int vs, const T *from,
+                        T *to, T *part, const int32_t *cpos, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    const size_t lds = sshex8_transfer_lds_bytes(Lf, vs);
+    if (vs == 1)
+        sshex8_restrict_lds_kernel<T, 1><<<(unsigned)E, 256, lds, s>>>(E, Lf, Lc, idx, count, from, to, part, cpos);
+    else if (vs == 3)
+        sshex8_restrict_lds_kernel<T, 3><<<(unsigned)E, 256, lds, s>>>(E, Lf, Lc, idx, count, from, to, part, cpos);
+    else {
+        set_last_error("sshex8_restrict_lds: vector size 1 or 3");
+        return SFEM_FAILURE;
+    }
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
 template int sshex8_prolongate<double>(ptrdiff_t, int, int, const int32_t *, const uint8_t *, int, const double *,
                                        double *, hipStream_t);
 template int sshex8_prolongate<float>(ptrdiff_t, int, int, const int32_t *, const uint8_t *, int, const float *,
@@ -167,6 +310,14 @@ template int sshex8_restrict<double>(ptrdiff_t, int, int, const int32_t *, const
                                      double *, hipStream_t);
 template int sshex8_restrict<float>(ptrdiff_t, int, int, const int32_t *, const uint16_t *, int, const float *,
                                     float *, hipStream_t);
+template int sshex8_prolongate_lds<double>(ptrdiff_t, int, int, const int32_t *, const uint8_t *, int, const double *,
+                                           double *, hipStream_t);
+template int sshex8_prolongate_lds<float>(ptrdiff_t, int, int, const int32_t *, const uint8_t *, int, const float *,
+                                          float *, hipStream_t);
+template int sshex8_restrict_lds<double>(ptrdiff_t, int, int, const int32_t *, const uint16_t *, int, const double *,
+                                         double *, double *, const int32_t *, hipStream_t);
+template int sshex8_restrict_lds<float>(ptrdiff_t, int, int, const int32_t *, const uint16_t *, int, const float *,
+                                        float *, float *, const int32_t *, hipStream_t);
 
 int node_owner_mask(ptrdiff_t E, int nxe, ptrdiff_t nnodes, const int32_t *idx, uint8_t *own, hipStream_t s) {
     int32_t *owner = nullptr;

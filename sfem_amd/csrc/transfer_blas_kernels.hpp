@@ -13,6 +13,17 @@ int sshex8_prolongate(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uin
 template <typename T>
 int sshex8_restrict(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint16_t *count, int vs, const T *from,
                     T *to, hipStream_t s);
+// LDS-staged variants (one macro element per block): the prolongation stages the coarse values
+// (nxe(Lc) * vs doubles), the restriction the count-scaled fine values (nxe(Lf) * vs doubles) and
+// writes coarse skeleton contributions to `part` for skeleton_gather (scalar: node-sorted
+// positions cpos; vector: element-major slots), so it adds no atomics.
+size_t sshex8_transfer_lds_bytes(int L, int vs);
+template <typename T>
+int sshex8_prolongate_lds(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint8_t *own, int vs, const T *from,
+                          T *to, hipStream_t s);
+template <typename T>
+int sshex8_restrict_lds(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint16_t *count, int vs, const T *from,
+                        T *to, T *part, const int32_t *cpos, hipStream_t s);
 int node_owner_mask(ptrdiff_t E, int nxe, ptrdiff_t nnodes, const int32_t *idx, uint8_t *own, hipStream_t s);
 int node_incidence(ptrdiff_t n_entries, ptrdiff_t nnodes, const int32_t *idx, uint16_t *count, hipStream_t s);
 int sshex8_derefine(ptrdiff_t E, int Lf, int Lc, const int32_t *fine, int32_t *coarse, hipStream_t s);
