@@ -12,6 +12,8 @@
 
 #include <hip/hip_bf16.h>
 
+#include <type_traits>
+
 namespace sfem_amd {
 
 __device__ __forceinline__ float bf16_to_float(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
@@ -180,13 +182,19 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
-                for (int b = 0; b < 3; b++) v += Sa[a][b][0] * w[1][a][b] + Sa[a][b][1] * w[2][a][b];
+                for (int b = 0; b < 3; b++) {
+                    v = fma(Sa[a][b][0], w[1][a][b], v);
+                    v = fma(Sa[a][b][1], w[2][a][b], v);
+                }
         }
         if (i > 0) {
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
-                for (int b = 0; b < 3; b++) v += Sb[a][b][0] * w[0][a][b] + Sb[a][b][1] * w[1][a][b];
+                for (int b = 0; b < 3; b++) {
+                    v = fma(Sb[a][b][0], w[0][a][b], v);
+                    v = fma(Sb[a][b][1], w[1][a][b], v);
+                }
         }
         if (skl || i == 0 || i == L)
             part[ppos[i]] = v;
@@ -395,6 +403,151 @@ __global__ void __launch_bounds__(ElShape<L>::BLOCK)
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// FP32 sshex8 elasticity on the matrix cores (the operator of the FP32 multigrid preconditioner).
+// Element-by-element: all L^3 micro elements of a macro element share its 24x24 matrix K, so
+// Y = K X with X = the micro elements' gathered corner values (24 x L^3) is one batched GEMM.
+// It runs as v_mfma_f32_16x16x32_bf16 tiles: M = the 24 rows of K (two 16-row tiles), K = its 24
+// columns (8 corners x 3 components, padded to 32), N = 16 micro elements.  FP32 operands are
+// split into bf16 hi + lo parts and multiplied as hi*hi + hi*lo + lo*hi (~2^-17 relative per
+// product, FP32 accumulation), far below the BF16-geometry error of this operator.  The micro
+// elements are visited in 8 parity colours (lower corner parity in x, y, z): two micro elements
+// of one colour share no node, so each colour's contributions are added to the LDS node
+// accumulators without atomics, colour by colour (a fixed order: deterministic).  Interior nodes
+// then update y; skeleton nodes store to the element's slots of `part` (as the vector kernel).
+template <int L>
+struct ElMfma {
+    static constexpr int N1 = L + 1, NXE = N1 * N1 * N1, H = L / 2;
+    static constexpr int NC = H * H * H;                 // micro elements per colour
+    static constexpr int TPC = (NC + 15) / 16;           // 16-column tiles per colour
+    static constexpr int EPB = TPC >= 4 ? 1 : 4 / TPC;   // macro elements per 256-thread block
+    static constexpr int WPE = 4 / EPB;                  // waves per macro element
+};
+
+typedef __bf16 el_b8 __attribute__((ext_vector_type(8)));
+typedef float el_f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t el_f2bf(float f) {  // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+// v = hi + lo (+ ~2^-17 |v|)
+__device__ __forceinline__ void el_split(float v, el_b8 &hi, el_b8 &lo, int j) {
+    const uint16_t h = el_f2bf(v);
+    const float r = v - __uint_as_float((uint32_t)h << 16);
+    hi[j] = __builtin_bit_cast(__bf16, h);
+    lo[j] = __builtin_bit_cast(__bf16, el_f2bf(r));
+}
+
+template <int L>
+__global__ void __launch_bounds__(256)
+    sshex8_elasticity_apply_mfma_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
+                                        const float *__restrict__ Kall, const float *__restrict__ x,
+                                        float *__restrict__ y, float *__restrict__ part, const int overwrite) {
+    using S = ElMfma<L>;
+    constexpr int N1 = S::N1, NXE = S::NXE, H = S::H, NC = S::NC, TPC = S::TPC, EPB = S::EPB, WPE = S::WPE;
+    constexpr int NSK = sk_count(L);
+    __shared__ float sx[EPB][3][NXE];  // x, one plane per component
+    __shared__ float sy[EPB][3][NXE];  // node accumulators
+    __shared__ int32_t sg[EPB][NXE];   // node ids
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int c = lane & 15, g = lane >> 4;
+    const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
+
+    for (int k = tid; k < EPB * NXE; k += 256) {
+        const int le = k / NXE, v = k - le * NXE;
+        const ptrdiff_t e = e0 + le;
+        float a0 = 0, a1 = 0, a2 = 0;
+        int32_t gid = 0;
+        if (e < E) {
+            gid = idx[e * NXE + v];
+            const float *xp = x + (ptrdiff_t)gid * 3;
+            a0 = xp[0];
+            a1 = xp[1];
+            a2 = xp[2];
+        }
+        sg[le][v] = gid;
+        sx[le][0][v] = a0;
+        sx[le][1][v] = a1;
+        sx[le][2][v] = a2;
+        sy[le][0][v] = 0;
+        sy[le][1][v] = 0;
+        sy[le][2][v] = 0;
+    }
+
+    // A operands (this wave's macro element): tile tt holds K rows 16tt + c, columns 8g + j
+    const int le = wave / WPE, wsub = wave - le * WPE;
+    const ptrdiff_t e = e0 + le;
+    const bool active = e < E;
+    el_b8 ahi[2], alo[2];
+#pragma unroll
+    for (int tt = 0; tt < 2; tt++) {
+        const int row = 16 * tt + c;
+        const bool ok = active && row < 24 && g < 3;
+        const float *kr = Kall + (ok ? e * 576 + row * 24 + 8 * g : 0);
+#pragma unroll
+        for (int j = 0; j < 8; j++) el_split(ok ? kr[j] : 0.f, ahi[tt], alo[tt], j);
+    }
+    __syncthreads();
+
+    const int gc = g < 3 ? g : 0;
+#pragma unroll 1
+    for (int colour = 0; colour < 8; colour++) {
+        const int px = colour & 1, py = (colour >> 1) & 1, pz = colour >> 2;
+        for (int t = wsub; t < TPC; t += WPE) {
+            const int q = 16 * t + c;
+            const bool valid = active && q < NC;
+            const int qq = valid ? q : 0;
+            const int mx = 2 * (qq % H) + px, my = 2 * ((qq / H) % H) + py, mz = 2 * (qq / (H * H)) + pz;
+            el_b8 bhi, blo;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int node = lidx(L, mx + hex8_cx(j), my + hex8_cy(j), mz + hex8_cz(j));
+                const float v = (valid && g < 3) ? sx[le][gc][node] : 0.f;
+                el_split(v, bhi, blo, j);
+            }
+#pragma unroll
+            for (int tt = 0; tt < 2; tt++) {
+                el_f4 d = {0.f, 0.f, 0.f, 0.f};
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[tt], bhi, d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[tt], blo, d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[tt], bhi, d, 0, 0, 0);
+                // lane holds rows R = 16tt + 4g + r of micro element q: component R / 8, corner R % 8
+                const int R0 = 16 * tt + 4 * g;
+                if (valid && R0 < 24) {
+                    const int ci = R0 >> 3;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int k = (R0 & 7) + r;
+                        const int node = lidx(L, mx + hex8_cx(k), my + hex8_cy(k), mz + hex8_cz(k));
+                        sy[le][ci][node] += d[r];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    for (int k = tid; k < EPB * NXE; k += 256) {
+        const int le2 = k / NXE, v = k - le2 * NXE;
+        const ptrdiff_t e2 = e0 + le2;
+        if (e2 >= E) continue;
+        const int xx = v % N1, yy = (v / N1) % N1, zz = v / (N1 * N1);
+        const int slot = sk_slot(L, xx, yy, zz);
+        if (slot < 0) {
+            float *yp = y + (ptrdiff_t)sg[le2][v] * 3;
+#pragma unroll
+            for (int ci = 0; ci < 3; ci++) yp[ci] = (overwrite ? 0.f : yp[ci]) + sy[le2][ci][v];
+        } else {
+            float *q = part + (e2 * (ptrdiff_t)NSK + slot) * 3;
+#pragma unroll
+            for (int ci = 0; ci < 3; ci++) q[ci] = sy[le2][ci][v];
+        }
+    }
+}
+
 // d += diagonal (block = 0) or symmetric 3x3 block diagonal [node][6] (block = 1).
 template <typename T, int L>
 __global__ void sshex8_elasticity_diag_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
@@ -517,8 +670,23 @@ int sshex8_elasticity_matrices(int L, ptrdiff_t E, const G *adj, const G *det, p
 
 template <typename T>
 int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *x, T *y, T *part,
-                            const SkeletonMap &sk, hipStream_t s, int overwrite) {
+                            const SkeletonMap &sk, hipStream_t s, int overwrite, int matrix_cores) {
     if (E == 0) return SFEM_SUCCESS;
+    if constexpr (std::is_same<T, float>::value) {
+        if (matrix_cores && (L == 2 || L == 4 || L == 6 || L == 8 || L == 10 || L == 12)) {
+            switch (L) {
+#define MCASE(LV)                                                                                               \
+    case LV:                                                                                                    \
+        sshex8_elasticity_apply_mfma_kernel<LV><<<(unsigned)((E + ElMfma<LV>::EPB - 1) / ElMfma<LV>::EPB), 256, 0, \
+                                                  s>>>(E, idx, K, x, y, part, overwrite);                       \
+        break;
+                MCASE(2) MCASE(4) MCASE(6) MCASE(8) MCASE(10) MCASE(12)
+#undef MCASE
+            }
+            SFEM_AMD_HIP_CHECK(hipGetLastError());
+            return skeleton_gather<T>(sk, 3, part, y, s, overwrite);
+        }
+    }
     switch (L) {
 #define CASE(LV)                                                                                        \
     case LV: {                                                                                          \
@@ -574,9 +742,9 @@ template int sshex8_elasticity_matrices<float>(int, ptrdiff_t, const float *, co
 template int sshex8_elasticity_matrices<uint16_t>(int, ptrdiff_t, const uint16_t *, const uint16_t *, ptrdiff_t,
                                                   double, double, const float *, double *, hipStream_t);
 template int sshex8_elasticity_apply<double>(int, ptrdiff_t, const int32_t *, const double *, const double *,
-                                             double *, double *, const SkeletonMap &, hipStream_t, int);
+                                             double *, double *, const SkeletonMap &, hipStream_t, int, int);
 template int sshex8_elasticity_apply<float>(int, ptrdiff_t, const int32_t *, const float *, const float *, float *,
-                                            float *, const SkeletonMap &, hipStream_t, int);
+                                            float *, const SkeletonMap &, hipStream_t, int, int);
 template int sshex8_elasticity_diag<double>(int, ptrdiff_t, const int32_t *, const double *, int, double *,
                                             hipStream_t);
 template int sshex8_elasticity_diag<float>(int, ptrdiff_t, const int32_t *, const double *, int, float *,

@@ -185,6 +185,38 @@ def test_sshex8_elasticity_parity(L, n):
         assert err < 1e-2
 
 
+def bf16_round(a):
+    """float32 -> nearest-even bfloat16, returned as float32 (the FP32 operator's geometry words)."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize("L,n", [(2, 3), (4, 2), (6, 2), (8, 2), (12, 1)])
+def test_sshex8_elasticity_fp32_matrix_cores(L, n):
+    """The FP32 sshex8 elasticity operator runs on the matrix cores (bf16 hi/lo split products,
+    FP32 accumulation).  On the same BF16-rounded geometry as the FP64 oracle it must agree to the
+    splitting error (~1e-5), and repeated applies must be bitwise identical."""
+    mel, mpts = skewed_box(n, perturb=0.02)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    adj, det = O.hex8_adj_det(mel, mpts)
+    E = mel.shape[1]
+    coeff = np.exp(np.random.default_rng(L + 1).uniform(np.log(0.1), np.log(10), E)).astype(np.float32)
+    mu, lam = O.lame(1.0, 0.3)
+    N = pts.shape[1]
+    x = np.random.default_rng(3).uniform(-1, 1, 3 * N).astype(np.float32)
+    op32 = S.linear_elasticity(S.Mesh(S.SSHEX8, L, el, N), adj, det, mu, lam, coeff, real_type=S.FLOAT32)
+    yo = np.zeros(3 * N)
+    O.lib.oracle_sshex8_elasticity_apply(L, E, el, bf16_round(adj), bf16_round(det), mu, lam, coeff,
+                                         x.astype(np.float64), yo)
+    y1 = host(gpu_apply(op32, dev(x, np.float32)))
+    y2 = host(gpu_apply(op32, dev(x, np.float32)))
+    assert np.array_equal(y1, y2)
+    err = relerr(y1.astype(np.float64), yo)
+    print(f"L={L}: FP32 matrix-core elasticity apply vs FP64 oracle (same geometry) = {err:.3e}")
+    assert err < 1e-4
+
+
 # ---------------------------------------------------------------- stored element matrices (ss:em)
 @pytest.mark.parametrize("L,n,bs", [(2, 2, 1), (8, 2, 1), (16, 1, 1), (4, 2, 3), (16, 2, 3)])
 def test_sshex8_element_matrix_parity(L, n, bs):
