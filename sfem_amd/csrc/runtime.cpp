@@ -216,7 +216,8 @@ struct ElasticityOp final : OpBase {
     DevArray<float> adj, det, coeff;  // hex8: SoA [9][E], [E]
     DevArray<double> K;               // sshex8: per-macro 24x24 matrices
     DevArray<float> K32;              // sshex8, FP32 operator: K rounded to float
-    bool matrix_cores{false};         // sshex8 FP32 from geometry: MFMA kernel (bf16x3 products)
+    // FP32 from geometry: the bf16x3 MFMA kernel (FP64 always runs on the matrix cores: exact)
+    bool matrix_cores{false};
     DevArray<T> part;                 // sshex8: skeleton contributions [E][sk_count(L)][3]
     double mu{0}, lambda{0};
     int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
@@ -249,7 +250,7 @@ struct ElasticityOp final : OpBase {
             Kt = K32.p;
         }
         return sshex8_elasticity_apply<T>(m->level, m->E, m->idx.p, Kt, (const T *)xv, (T *)yv, part.p,
-                                          skeleton_of(m), s, ow, matrix_cores ? 1 : 0);
+                                          skeleton_of(m), s, ow, (std::is_same<T, double>::value || matrix_cores) ? 1 : 0);
     }
     int diag(void *d, hipStream_t s) override {
         if (m->type == SFEM_AMD_HEX8) {

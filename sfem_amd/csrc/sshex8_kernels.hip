@@ -215,6 +215,27 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
 
 }
 
+template <typename T>
+struct MfmaAcc;
+// Operands: lane (c, g) = (lane % 16, lane / 16) holds A[c][g] and B[g][c].  Result element r
+// of the lane is D[row(g, r)][c]: row 4g + r for FP32, 4r + g for FP64 (measured on gfx950).
+template <>
+struct MfmaAcc<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 step(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int g, int r) { return 4 * r + g; }
+};
+template <>
+struct MfmaAcc<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 step(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
+};
+
 // y[node[t]] += sum of part[ent[off[t] .. off[t+1])] * bs components: the skeleton nodes'
 // element contributions, in slot order.
 template <typename T, int BS>
@@ -427,17 +448,24 @@ struct ElMfma {
 typedef __bf16 el_b8 __attribute__((ext_vector_type(8)));
 typedef float el_f4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t el_f2bf(float f) {  // round to nearest even
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+// v = hi + lo (+ ~2^-17 |v|), both bf16 (round to nearest even, v_cvt_pk_bf16_f32), packed as
+// hi | lo << 16
+__device__ __forceinline__ uint32_t el_split_pack(float v) {
+    const __bf16 h = (__bf16)v;
+    const __bf16 l = (__bf16)(v - (float)h);
+    return (uint32_t)__builtin_bit_cast(uint16_t, h) | ((uint32_t)__builtin_bit_cast(uint16_t, l) << 16);
 }
-// v = hi + lo (+ ~2^-17 |v|)
-__device__ __forceinline__ void el_split(float v, el_b8 &hi, el_b8 &lo, int j) {
-    const uint16_t h = el_f2bf(v);
-    const float r = v - __uint_as_float((uint32_t)h << 16);
-    hi[j] = __builtin_bit_cast(__bf16, h);
-    lo[j] = __builtin_bit_cast(__bf16, el_f2bf(r));
+// eight packed pairs -> the hi and lo bf16x8 operands
+__device__ __forceinline__ void el_unpack8(const uint32_t (&p)[8], el_b8 &hi, el_b8 &lo) {
+    typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+    u4 h, l;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        h[i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x05040100u);
+        l[i] = __builtin_amdgcn_perm(p[2 * i + 1], p[2 * i], 0x07060302u);
+    }
+    hi = __builtin_bit_cast(el_b8, h);
+    lo = __builtin_bit_cast(el_b8, l);
 }
 
 template <int L>
@@ -448,33 +476,43 @@ __global__ void __launch_bounds__(256)
     using S = ElMfma<L>;
     constexpr int N1 = S::N1, NXE = S::NXE, H = S::H, NC = S::NC, TPC = S::TPC, EPB = S::EPB, WPE = S::WPE;
     constexpr int NSK = sk_count(L);
-    __shared__ float sx[EPB][3][NXE];  // x, one plane per component
-    __shared__ float sy[EPB][3][NXE];  // node accumulators
-    __shared__ int32_t sg[EPB][NXE];   // node ids
+    constexpr int NPT = (EPB * NXE + 255) / 256;  // nodes per thread in the load and store phases
+    __shared__ uint32_t sx[EPB][3][NXE];  // x split into bf16 hi/lo pairs, one plane per component
+    __shared__ float sy[EPB][3][NXE];     // node accumulators
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int c = lane & 15, g = lane >> 4;
     const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
 
-    for (int k = tid; k < EPB * NXE; k += 256) {
-        const int le = k / NXE, v = k - le * NXE;
-        const ptrdiff_t e = e0 + le;
-        float a0 = 0, a1 = 0, a2 = 0;
-        int32_t gid = 0;
-        if (e < E) {
-            gid = idx[e * NXE + v];
-            const float *xp = x + (ptrdiff_t)gid * 3;
-            a0 = xp[0];
-            a1 = xp[1];
-            a2 = xp[2];
+    // node k = tid + 256 i of the block: ids first, then all values at once; interior nodes
+    // start from their y (accumulate), skeleton nodes from 0
+    int32_t gid[NPT];
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int k = tid + 256 * i, le = k / NXE;
+        gid[i] = (k < EPB * NXE && e0 + le < E) ? idx[e0 * NXE + k] : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int k = tid + 256 * i, le = k / NXE, v = k - le * NXE;
+        const bool interior = sk_slot(L, v % N1, (v / N1) % N1, v / (N1 * N1)) < 0;
+        float xv[3] = {0, 0, 0}, yv[3] = {0, 0, 0};
+        if (gid[i] >= 0) {
+            const float *xp = x + (ptrdiff_t)gid[i] * 3;
+            const float *yp = y + (ptrdiff_t)gid[i] * 3;
+#pragma unroll
+            for (int ci = 0; ci < 3; ci++) {
+                xv[ci] = xp[ci];
+                if (interior && !overwrite) yv[ci] = yp[ci];
+            }
         }
-        sg[le][v] = gid;
-        sx[le][0][v] = a0;
-        sx[le][1][v] = a1;
-        sx[le][2][v] = a2;
-        sy[le][0][v] = 0;
-        sy[le][1][v] = 0;
-        sy[le][2][v] = 0;
+        if (k < EPB * NXE) {
+#pragma unroll
+            for (int ci = 0; ci < 3; ci++) {
+                sx[le][ci][v] = el_split_pack(xv[ci]);
+                sy[le][ci][v] = yv[ci];
+            }
+        }
     }
 
     // A operands (this wave's macro element): tile tt holds K rows 16tt + c, columns 8g + j
@@ -487,8 +525,10 @@ __global__ void __launch_bounds__(256)
         const int row = 16 * tt + c;
         const bool ok = active && row < 24 && g < 3;
         const float *kr = Kall + (ok ? e * 576 + row * 24 + 8 * g : 0);
+        uint32_t pk[8];
 #pragma unroll
-        for (int j = 0; j < 8; j++) el_split(ok ? kr[j] : 0.f, ahi[tt], alo[tt], j);
+        for (int j = 0; j < 8; j++) pk[j] = el_split_pack(ok ? kr[j] : 0.f);
+        el_unpack8(pk, ahi[tt], alo[tt]);
     }
     __syncthreads();
 
@@ -497,17 +537,21 @@ __global__ void __launch_bounds__(256)
     for (int colour = 0; colour < 8; colour++) {
         const int px = colour & 1, py = (colour >> 1) & 1, pz = colour >> 2;
         for (int t = wsub; t < TPC; t += WPE) {
-            const int q = 16 * t + c;
-            const bool valid = active && q < NC;
-            const int qq = valid ? q : 0;
+            const unsigned q = 16 * t + c;
+            const bool valid = active && q < (unsigned)NC;
+            const unsigned qq = valid ? q : 0;
             const int mx = 2 * (qq % H) + px, my = 2 * ((qq / H) % H) + py, mz = 2 * (qq / (H * H)) + pz;
-            el_b8 bhi, blo;
+            const int base = lidx(L, mx, my, mz);
+            uint32_t pk[8];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                const int node = lidx(L, mx + hex8_cx(j), my + hex8_cy(j), mz + hex8_cz(j));
-                const float v = (valid && g < 3) ? sx[le][gc][node] : 0.f;
-                el_split(v, bhi, blo, j);
+                const int node = base + lidx(L, hex8_cx(j), hex8_cy(j), hex8_cz(j));
+                // in range for every lane (qq, gc clamped); rows k >= 24 meet zero columns of A and
+                // invalid columns are never scattered, so no masking is needed
+                pk[j] = sx[le][gc][node];
             }
+            el_b8 bhi, blo;
+            el_unpack8(pk, bhi, blo);
 #pragma unroll
             for (int tt = 0; tt < 2; tt++) {
                 el_f4 d = {0.f, 0.f, 0.f, 0.f};
@@ -521,8 +565,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         const int k = (R0 & 7) + r;
-                        const int node = lidx(L, mx + hex8_cx(k), my + hex8_cy(k), mz + hex8_cz(k));
-                        sy[le][ci][node] += d[r];
+                        sy[le][ci][base + lidx(L, hex8_cx(k), hex8_cy(k), hex8_cz(k))] += d[r];
                     }
                 }
             }
@@ -530,21 +573,125 @@ __global__ void __launch_bounds__(256)
         __syncthreads();
     }
 
-    for (int k = tid; k < EPB * NXE; k += 256) {
-        const int le2 = k / NXE, v = k - le2 * NXE;
-        const ptrdiff_t e2 = e0 + le2;
-        if (e2 >= E) continue;
-        const int xx = v % N1, yy = (v / N1) % N1, zz = v / (N1 * N1);
-        const int slot = sk_slot(L, xx, yy, zz);
-        if (slot < 0) {
-            float *yp = y + (ptrdiff_t)sg[le2][v] * 3;
 #pragma unroll
-            for (int ci = 0; ci < 3; ci++) yp[ci] = (overwrite ? 0.f : yp[ci]) + sy[le2][ci][v];
-        } else {
-            float *q = part + (e2 * (ptrdiff_t)NSK + slot) * 3;
+    for (int i = 0; i < NPT; i++) {
+        const int k = tid + 256 * i, le2 = k / NXE, v = k - le2 * NXE;
+        if (gid[i] < 0) continue;
+        const int slot = sk_slot(L, v % N1, (v / N1) % N1, v / (N1 * N1));
+        float *q = slot < 0 ? y + (ptrdiff_t)gid[i] * 3 : part + ((e0 + le2) * (ptrdiff_t)NSK + slot) * 3;
 #pragma unroll
-            for (int ci = 0; ci < 3; ci++) q[ci] = sy[le2][ci][v];
+        for (int ci = 0; ci < 3; ci++) q[ci] = sy[le2][ci][v];
+    }
+}
+
+// FP64 sshex8 elasticity on the matrix cores: the same element-by-element GEMM as the FP32 kernel
+// above, in exact FP64 with v_mfma_f64_16x16x4f64: M = the 24 rows of K (two 16-row tiles, rows
+// 24..31 zero), K = its 24 columns (six steps of 4), N = 16 micro elements of one parity colour.
+// The node sums run colour by colour (deterministic); they differ from the node-wise vector kernel
+// only in the order of the FP64 additions.
+template <int L>
+__global__ void __launch_bounds__(256)
+    sshex8_elasticity_apply_mfma64_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
+                                          const double *__restrict__ Kall, const double *__restrict__ x,
+                                          double *__restrict__ y, double *__restrict__ part, const int overwrite) {
+    using S = ElMfma<L>;
+    using M = MfmaAcc<double>;
+    constexpr int N1 = S::N1, NXE = S::NXE, H = S::H, NC = S::NC, TPC = S::TPC, EPB = S::EPB, WPE = S::WPE;
+    constexpr int NSK = sk_count(L);
+    constexpr int NPT = (EPB * NXE + 255) / 256;
+    __shared__ double sx[EPB][3][NXE];
+    __shared__ double sy[EPB][3][NXE];
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int c = lane & 15, g = lane >> 4;
+    const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
+
+    int32_t gid[NPT];
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int k = tid + 256 * i, le = k / NXE;
+        gid[i] = (k < EPB * NXE && e0 + le < E) ? idx[e0 * NXE + k] : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int k = tid + 256 * i, le = k / NXE, v = k - le * NXE;
+        const bool interior = sk_slot(L, v % N1, (v / N1) % N1, v / (N1 * N1)) < 0;
+        double xv[3] = {0, 0, 0}, yv[3] = {0, 0, 0};
+        if (gid[i] >= 0) {
+            const double *xp = x + (ptrdiff_t)gid[i] * 3;
+            const double *yp = y + (ptrdiff_t)gid[i] * 3;
+#pragma unroll
+            for (int ci = 0; ci < 3; ci++) {
+                xv[ci] = xp[ci];
+                if (interior && !overwrite) yv[ci] = yp[ci];
+            }
         }
+        if (k < EPB * NXE) {
+#pragma unroll
+            for (int ci = 0; ci < 3; ci++) {
+                sx[le][ci][v] = xv[ci];
+                sy[le][ci][v] = yv[ci];
+            }
+        }
+    }
+
+    // A: tile tt, step s holds K[16tt + c][4s + g]
+    const int le = wave / WPE, wsub = wave - le * WPE;
+    const ptrdiff_t e = e0 + le;
+    const bool active = e < E;
+    double a[2][6];
+#pragma unroll
+    for (int tt = 0; tt < 2; tt++)
+#pragma unroll
+        for (int s = 0; s < 6; s++) {
+            const int row = 16 * tt + c;
+            a[tt][s] = (active && row < 24) ? Kall[e * 576 + row * 24 + 4 * s + g] : 0.0;
+        }
+    __syncthreads();
+
+#pragma unroll 1
+    for (int colour = 0; colour < 8; colour++) {
+        const int px = colour & 1, py = (colour >> 1) & 1, pz = colour >> 2;
+        for (int t = wsub; t < TPC; t += WPE) {
+            const int q = 16 * t + c;
+            const bool valid = active && q < NC;
+            const int qq = valid ? q : 0;
+            const int mx = 2 * (qq % H) + px, my = 2 * ((qq / H) % H) + py, mz = 2 * (qq / (H * H)) + pz;
+            typename M::v4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
+#pragma unroll
+            for (int s = 0; s < 6; s++) {
+                const int kk = 4 * s + g, j = kk & 7;  // column: component kk / 8, corner kk % 8
+                const int node = lidx(L, mx + hex8_cx(j), my + hex8_cy(j), mz + hex8_cz(j));
+                const double b = sx[le][kk >> 3][node];  // in range (qq clamped); invalid columns unused
+                d0 = M::step(a[0][s], b, d0);
+                d1 = M::step(a[1][s], b, d1);
+            }
+            if (valid) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+#pragma unroll
+                    for (int tt = 0; tt < 2; tt++) {
+                        const int R = 16 * tt + M::row(g, r);  // component R / 8, corner R % 8
+                        if (R < 24) {
+                            const int k = R & 7;
+                            sy[le][R >> 3][lidx(L, mx + hex8_cx(k), my + hex8_cy(k), mz + hex8_cz(k))] +=
+                                    tt ? d1[r] : d0[r];
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int k = tid + 256 * i, le2 = k / NXE, v = k - le2 * NXE;
+        if (gid[i] < 0) continue;
+        const int slot = sk_slot(L, v % N1, (v / N1) % N1, v / (N1 * N1));
+        double *q = slot < 0 ? y + (ptrdiff_t)gid[i] * 3 : part + ((e0 + le2) * (ptrdiff_t)NSK + slot) * 3;
+#pragma unroll
+        for (int ci = 0; ci < 3; ci++) q[ci] = sy[le2][ci][v];
     }
 }
 
@@ -672,6 +819,22 @@ template <typename T>
 int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *x, T *y, T *part,
                             const SkeletonMap &sk, hipStream_t s, int overwrite, int matrix_cores) {
     if (E == 0) return SFEM_SUCCESS;
+    if constexpr (std::is_same<T, double>::value) {
+        if (matrix_cores && (L == 2 || L == 4 || L == 6 || L == 8)) {
+            switch (L) {
+#define MCASE(LV)                                                                                           \
+    case LV:                                                                                                \
+        sshex8_elasticity_apply_mfma64_kernel<LV>                                                           \
+                <<<(unsigned)((E + ElMfma<LV>::EPB - 1) / ElMfma<LV>::EPB), 256, 0, s>>>(E, idx, K, x, y, part, \
+                                                                                         overwrite);        \
+        break;
+                MCASE(2) MCASE(4) MCASE(6) MCASE(8)
+#undef MCASE
+            }
+            SFEM_AMD_HIP_CHECK(hipGetLastError());
+            return skeleton_gather<T>(sk, 3, part, y, s, overwrite);
+        }
+    }
     if constexpr (std::is_same<T, float>::value) {
         if (matrix_cores && (L == 2 || L == 4 || L == 6 || L == 8 || L == 10 || L == 12)) {
             switch (L) {

@@ -37,8 +37,9 @@ int sshex8_elasticity_matrices(int L, ptrdiff_t E, const G *adj, const G *det, p
                                double lambda, const float *coeff, double *K, hipStream_t s);
 template <typename T>
 // K: per-macro 24x24 matrices in the vector precision; part: E * sk_count(L) * 3 values
-// matrix_cores (FP32 only, even L <= 12): the bf16x3 MFMA kernel, ~1e-5 relative per product;
-// otherwise the exact vector kernel
+// matrix_cores, even L: FP64 (L <= 8) the FP64 MFMA kernel (exact FP64, element-by-element
+// summation order); FP32 (L <= 12) the bf16x3 MFMA kernel, ~1e-5 relative per product; otherwise
+// the node-wise vector kernel
 int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *x, T *y, T *part,
                             const SkeletonMap &sk, hipStream_t s, int overwrite = 0, int matrix_cores = 0);
 template <typename T>
