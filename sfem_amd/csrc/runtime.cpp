@@ -219,14 +219,53 @@ struct ElasticityOp final : OpBase {
     // FP32 from geometry: the bf16x3 MFMA kernel (FP64 always runs on the matrix cores: exact)
     bool matrix_cores{false};
     DevArray<T> part;                 // sshex8: skeleton contributions [E][sk_count(L)][3]
+    // levels beyond the matrix-core kernels' LDS tile (L = 2 Ls): the operator applied as 8 sub-blocks
+    // of level Ls per macro element (sharing its matrix), with their own skeleton slots
+    std::unique_ptr<sfem_amd_mesh> sub;
     double mu{0}, lambda{0};
     int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
     ptrdiff_t rows() const override { return m->nnodes * 3; }
+    bool uses_matrix_cores() const { return std::is_same<T, double>::value || matrix_cores; }
+    // the sub-block level, or 0 when the macro elements are applied directly
+    int subblock_level() const {
+        if (m->type != SFEM_AMD_SSHEX8 || !uses_matrix_cores()) return 0;
+        const int L = m->level;
+        if constexpr (std::is_same<T, double>::value) {
+            if (L % 2 == 0 && L <= 8) return 0;
+            return (L == 12 || L == 16) ? L / 2 : 0;
+        } else {
+            if (L % 2 == 0 && L <= 12) return 0;
+            return L == 16 ? 8 : 0;
+        }
+    }
+    // the mesh the kernels run on (built on first use)
+    const sfem_amd_mesh *work_mesh(hipStream_t s) {
+        const int Ls = subblock_level();
+        if (!Ls) return m;
+        if (!sub) {
+            auto sm = std::make_unique<sfem_amd_mesh>();
+            sm->type = SFEM_AMD_SSHEX8;
+            sm->level = Ls;
+            sm->E = m->E * 8;
+            sm->nnodes = m->nnodes;
+            sm->nv = nxe_of(Ls);
+            if (sm->idx.alloc((size_t)sm->E * sm->nv) ||
+                sshex8_subblock_idx(m->E, m->level, Ls, m->idx.p, sm->idx.p, s))
+                return nullptr;
+            if (hipStreamSynchronize(s) != hipSuccess) {
+                set_last_error("elasticity sub-blocks: stream synchronisation failed");
+                return nullptr;
+            }
+            sub = std::move(sm);
+        }
+        return sub.get();
+    }
     int apply(const void *xv, void *yv, hipStream_t s) override { return apply_impl(xv, yv, s, 0); }
     int apply_overwrite(const void *xv, void *yv, hipStream_t s) override {
         if (m->type != SFEM_AMD_SSHEX8) return OpBase::apply_overwrite(xv, yv, s);
-        if (mesh_skeleton(m, 0)) return SFEM_FAILURE;
-        if (!skeleton_covers(m)) return OpBase::apply_overwrite(xv, yv, s);
+        const sfem_amd_mesh *wm = work_mesh(s);
+        if (!wm || mesh_skeleton(wm, 0)) return SFEM_FAILURE;
+        if (!skeleton_covers(wm)) return OpBase::apply_overwrite(xv, yv, s);
         return apply_impl(xv, yv, s, 1);
     }
     int apply_impl(const void *xv, void *yv, hipStream_t s, int ow) {
@@ -237,8 +276,10 @@ struct ElasticityOp final : OpBase {
             set_last_error("hex8 elasticity: FP64 only");
             return SFEM_FAILURE;
         }
-        if (!part.p && m->E > 0) {
-            if (mesh_skeleton(m, 0) || part.alloc((size_t)m->E * sk_count(m->level) * 3)) return SFEM_FAILURE;
+        const sfem_amd_mesh *wm = work_mesh(s);
+        if (!wm) return SFEM_FAILURE;
+        if (!part.p && wm->E > 0) {
+            if (mesh_skeleton(wm, 0) || part.alloc((size_t)wm->E * sk_count(wm->level) * 3)) return SFEM_FAILURE;
         }
         const T *Kt;
         if constexpr (std::is_same<T, double>::value) {
@@ -249,8 +290,8 @@ struct ElasticityOp final : OpBase {
             }
             Kt = K32.p;
         }
-        return sshex8_elasticity_apply<T>(m->level, m->E, m->idx.p, Kt, (const T *)xv, (T *)yv, part.p,
-                                          skeleton_of(m), s, ow, (std::is_same<T, double>::value || matrix_cores) ? 1 : 0);
+        return sshex8_elasticity_apply<T>(wm->level, wm->E, wm->idx.p, Kt, (const T *)xv, (T *)yv, part.p,
+                                          skeleton_of(wm), s, ow, uses_matrix_cores() ? 1 : 0, wm == m ? 0 : 3);
     }
     int diag(void *d, hipStream_t s) override {
         if (m->type == SFEM_AMD_HEX8) {

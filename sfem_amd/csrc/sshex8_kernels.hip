@@ -468,11 +468,14 @@ __device__ __forceinline__ void el_unpack8(const uint32_t (&p)[8], el_b8 &hi, el
     lo = __builtin_bit_cast(el_b8, l);
 }
 
+// Consecutive groups of 2^kshift elements share one matrix (kshift > 0: the sub-blocks of a macro
+// element of a higher level, see sshex8_subblock_idx).
 template <int L>
 __global__ void __launch_bounds__(256)
     sshex8_elasticity_apply_mfma_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
                                         const float *__restrict__ Kall, const float *__restrict__ x,
-                                        float *__restrict__ y, float *__restrict__ part, const int overwrite) {
+                                        float *__restrict__ y, float *__restrict__ part, const int overwrite,
+                                        const int kshift) {
     using S = ElMfma<L>;
     constexpr int N1 = S::N1, NXE = S::NXE, H = S::H, NC = S::NC, TPC = S::TPC, EPB = S::EPB, WPE = S::WPE;
     constexpr int NSK = sk_count(L);
@@ -524,7 +527,7 @@ __global__ void __launch_bounds__(256)
     for (int tt = 0; tt < 2; tt++) {
         const int row = 16 * tt + c;
         const bool ok = active && row < 24 && g < 3;
-        const float *kr = Kall + (ok ? e * 576 + row * 24 + 8 * g : 0);
+        const float *kr = Kall + (ok ? (e >> kshift) * 576 + row * 24 + 8 * g : 0);
         uint32_t pk[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) pk[j] = el_split_pack(ok ? kr[j] : 0.f);
@@ -593,7 +596,8 @@ template <int L>
 __global__ void __launch_bounds__(256)
     sshex8_elasticity_apply_mfma64_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
                                           const double *__restrict__ Kall, const double *__restrict__ x,
-                                          double *__restrict__ y, double *__restrict__ part, const int overwrite) {
+                                          double *__restrict__ y, double *__restrict__ part, const int overwrite,
+                                          const int kshift) {
     using S = ElMfma<L>;
     using M = MfmaAcc<double>;
     constexpr int N1 = S::N1, NXE = S::NXE, H = S::H, NC = S::NC, TPC = S::TPC, EPB = S::EPB, WPE = S::WPE;
@@ -645,7 +649,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int s = 0; s < 6; s++) {
             const int row = 16 * tt + c;
-            a[tt][s] = (active && row < 24) ? Kall[e * 576 + row * 24 + 4 * s + g] : 0.0;
+            a[tt][s] = (active && row < 24) ? Kall[(e >> kshift) * 576 + row * 24 + 4 * s + g] : 0.0;
         }
     __syncthreads();
 
@@ -815,10 +819,44 @@ int sshex8_elasticity_matrices(int L, ptrdiff_t E, const G *adj, const G *det, p
     return SFEM_SUCCESS;
 }
 
+// connectivity of the sub-blocks: macro element e of level L = s * Ls as s^3 elements of level Ls,
+// sub-element e * s^3 + (oz * s + oy) * s + ox
+__global__ void sshex8_subblock_idx_kernel(const ptrdiff_t E, const int L, const int Ls,
+                                           const int32_t *__restrict__ idx, int32_t *__restrict__ out) {
+    const int sb = L / Ls, n1 = Ls + 1, nxs = n1 * n1 * n1, nsub = sb * sb * sb;
+    const ptrdiff_t n = E * nsub * nxs;
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const ptrdiff_t es = t / nxs;
+        const int v = (int)(t - es * nxs);
+        const ptrdiff_t e = es / nsub;
+        const int o = (int)(es - e * nsub);
+        const int ox = o % sb, oy = (o / sb) % sb, oz = o / (sb * sb);
+        const int x = v % n1, y = (v / n1) % n1, z = v / (n1 * n1);
+        out[t] = idx[e * nxe_of(L) + lidx(L, Ls * ox + x, Ls * oy + y, Ls * oz + z)];
+    }
+}
+
+int sshex8_subblock_idx(ptrdiff_t E, int L, int Ls, const int32_t *idx, int32_t *out, hipStream_t s) {
+    if (E == 0) return SFEM_SUCCESS;
+    if (Ls <= 0 || L % Ls) {
+        set_last_error("sshex8_subblock_idx: the sub-level must divide the level");
+        return SFEM_FAILURE;
+    }
+    const int sb = L / Ls;
+    sshex8_subblock_idx_kernel<<<grid_for(E * sb * sb * sb * nxe_of(Ls)), kBlock, 0, s>>>(E, L, Ls, idx, out);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
 template <typename T>
 int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *x, T *y, T *part,
-                            const SkeletonMap &sk, hipStream_t s, int overwrite, int matrix_cores) {
+                            const SkeletonMap &sk, hipStream_t s, int overwrite, int matrix_cores, int kshift) {
     if (E == 0) return SFEM_SUCCESS;
+    if (kshift && !(matrix_cores && ((std::is_same<T, double>::value && (L == 2 || L == 4 || L == 6 || L == 8)) ||
+                                     (std::is_same<T, float>::value && L % 2 == 0 && L <= 12)))) {
+        set_last_error("sshex8_elasticity_apply: shared matrices need the matrix-core kernels");
+        return SFEM_FAILURE;
+    }
     if constexpr (std::is_same<T, double>::value) {
         if (matrix_cores && (L == 2 || L == 4 || L == 6 || L == 8)) {
             switch (L) {
@@ -826,7 +864,7 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
     case LV:                                                                                                \
         sshex8_elasticity_apply_mfma64_kernel<LV>                                                           \
                 <<<(unsigned)((E + ElMfma<LV>::EPB - 1) / ElMfma<LV>::EPB), 256, 0, s>>>(E, idx, K, x, y, part, \
-                                                                                         overwrite);        \
+                                                                                         overwrite, kshift); \
         break;
                 MCASE(2) MCASE(4) MCASE(6) MCASE(8)
 #undef MCASE
@@ -841,7 +879,7 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
 #define MCASE(LV)                                                                                               \
     case LV:                                                                                                    \
         sshex8_elasticity_apply_mfma_kernel<LV><<<(unsigned)((E + ElMfma<LV>::EPB - 1) / ElMfma<LV>::EPB), 256, 0, \
-                                                  s>>>(E, idx, K, x, y, part, overwrite);                       \
+                                                  s>>>(E, idx, K, x, y, part, overwrite, kshift);               \
         break;
                 MCASE(2) MCASE(4) MCASE(6) MCASE(8) MCASE(10) MCASE(12)
 #undef MCASE
@@ -905,9 +943,9 @@ template int sshex8_elasticity_matrices<float>(int, ptrdiff_t, const float *, co
 template int sshex8_elasticity_matrices<uint16_t>(int, ptrdiff_t, const uint16_t *, const uint16_t *, ptrdiff_t,
                                                   double, double, const float *, double *, hipStream_t);
 template int sshex8_elasticity_apply<double>(int, ptrdiff_t, const int32_t *, const double *, const double *,
-                                             double *, double *, const SkeletonMap &, hipStream_t, int, int);
+                                             double *, double *, const SkeletonMap &, hipStream_t, int, int, int);
 template int sshex8_elasticity_apply<float>(int, ptrdiff_t, const int32_t *, const float *, const float *, float *,
-                                            float *, const SkeletonMap &, hipStream_t, int, int);
+                                            float *, const SkeletonMap &, hipStream_t, int, int, int);
 template int sshex8_elasticity_diag<double>(int, ptrdiff_t, const int32_t *, const double *, int, double *,
                                             hipStream_t);
 template int sshex8_elasticity_diag<float>(int, ptrdiff_t, const int32_t *, const double *, int, float *,

@@ -40,8 +40,13 @@ template <typename T>
 // matrix_cores, even L: FP64 (L <= 8) the FP64 MFMA kernel (exact FP64, element-by-element
 // summation order); FP32 (L <= 12) the bf16x3 MFMA kernel, ~1e-5 relative per product; otherwise
 // the node-wise vector kernel
+// kshift > 0 (matrix-core kernels only): elements e share the matrix K[e >> kshift] (sub-blocks)
 int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *x, T *y, T *part,
-                            const SkeletonMap &sk, hipStream_t s, int overwrite = 0, int matrix_cores = 0);
+                            const SkeletonMap &sk, hipStream_t s, int overwrite = 0, int matrix_cores = 0,
+                            int kshift = 0);
+// connectivity [E * (L/Ls)^3][nxe(Ls)] of the macro elements cut into (L/Ls)^3 sub-blocks of level
+// Ls (sub-element e * (L/Ls)^3 + (oz * s + oy) * s + ox): the level-L operator as a level-Ls one
+int sshex8_subblock_idx(ptrdiff_t E, int L, int Ls, const int32_t *idx, int32_t *out, hipStream_t s);
 template <typename T>
 int sshex8_elasticity_diag(int L, ptrdiff_t E, const int32_t *idx, const double *K, int block, T *d, hipStream_t s);
 
