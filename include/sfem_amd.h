@@ -88,6 +88,23 @@ int sfem_amd_adjugate_det(int element_type, int level, ptrdiff_t nelements, ptrd
 /* Upload connectivity (host, reference layout) and keep it in the device layout. */
 int sfem_amd_mesh_create(int element_type, int level, ptrdiff_t nelements, ptrdiff_t stride,
                          const int32_t *elements, ptrdiff_t nnodes, sfem_amd_mesh **out);
+/* Mesh setup on the device (the reference leaves it to the host smesh library: to_semistructured,
+ * ssmg/tests/sfem_SSGMGTest.cpp:77-88; FFF / JacobianAdjugateAndDeterminant,
+ * frontend/ops/sfem_Laplacian.cpp:88, frontend/ops/sfem_LinearElasticity.cpp:276-278).
+ * sfem_amd_sshex8_refine_device: hex8 macro elements (device, [8][nelements]) -> the sshex8 level-L
+ * device mesh with exactly the ids of sfem_amd_sshex8_refine.  _refine_points_device: the refined
+ * points (device [3][nnodes of fine]) from the macro points (device [3][nnodes_macro]).
+ * sfem_amd_mesh_fff / _adjugate_det: the geometry caches of a device mesh from device points
+ * ([3][nnodes of the mesh]) into device arrays [E][6], [E][9] and [E] (as sfem_amd_fff /
+ * sfem_amd_adjugate_det).  The operator constructors accept these device arrays directly. */
+int sfem_amd_sshex8_refine_device(int level, ptrdiff_t nelements, ptrdiff_t nnodes, const int32_t *macro_elements,
+                                  sfem_amd_mesh **out, void *stream);
+int sfem_amd_sshex8_refine_points_device(const sfem_amd_mesh *fine, ptrdiff_t nnodes_macro,
+                                         const int32_t *macro_elements, const float *macro_points, float *points,
+                                         void *stream);
+int sfem_amd_mesh_fff(const sfem_amd_mesh *m, const float *points, float *fff, void *stream);
+int sfem_amd_mesh_adjugate_det(const sfem_amd_mesh *m, const float *points, float *adjugate, float *det,
+                               void *stream);
 /* frontend/sfem_API.hpp:1412-1475 sshex8_derefine_element_connectivity, on device.
  * level 1 yields a HEX8 mesh in the standard corner order. */
 int sfem_amd_mesh_derefine(const sfem_amd_mesh *fine, int level, sfem_amd_mesh **out);

@@ -39,6 +39,10 @@ SIGNATURES = {
     "sfem_amd_adjugate_det": (_i, [_i, _i, _pd, _pd, _vp, _pd, _vp, _vp, _vp]),
     "sfem_amd_mesh_create": (_i, [_i, _i, _pd, _pd, _vp, _pd, C.POINTER(_vp)]),
     "sfem_amd_mesh_derefine": (_i, [_vp, _i, C.POINTER(_vp)]),
+    "sfem_amd_sshex8_refine_device": (_i, [_i, _pd, _pd, _vp, C.POINTER(_vp), _vp]),
+    "sfem_amd_sshex8_refine_points_device": (_i, [_vp, _pd, _vp, _vp, _vp, _vp]),
+    "sfem_amd_mesh_fff": (_i, [_vp, _vp, _vp, _vp]),
+    "sfem_amd_mesh_adjugate_det": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "sfem_amd_mesh_nnodes": (_pd, [_vp]),
     "sfem_amd_mesh_nelements": (_pd, [_vp]),
     "sfem_amd_mesh_level": (_i, [_vp]),
@@ -375,18 +379,71 @@ def _new_op(fn, *args, keep=()):
     return Operator(h, keep)
 
 
+def _geom(a):
+    """(pointer, keep-alive) of a geometry array: a DeviceArray is passed as device memory."""
+    if isinstance(a, DeviceArray):
+        return _vp(a.ptr), a
+    h = _host(a, np.float32)
+    return _np_ptr(h), h
+
+
 def laplacian(mesh, fff_aos, coeff=None, real_type=FLOAT64):
-    f = _host(fff_aos, np.float32)
-    c = _host(coeff, np.float32)
-    return _new_op("sfem_amd_laplacian_create", mesh.h, _np_ptr(f), _np_ptr(c), real_type, keep=[mesh])
+    """fff_aos [E][6] (numpy, host) or a DeviceArray (e.g. from mesh_fff); coeff likewise."""
+    f, fk = _geom(fff_aos)
+    c, ck = _geom(coeff)
+    return _new_op("sfem_amd_laplacian_create", mesh.h, f, c, real_type, keep=[mesh])
 
 
 def linear_elasticity(mesh, adjugate, det, mu, lam, coeff=None, real_type=FLOAT64):
-    a = _host(adjugate, np.float32)
-    d = _host(det, np.float32)
-    c = _host(coeff, np.float32)
-    return _new_op("sfem_amd_linear_elasticity_create", mesh.h, _np_ptr(a), _np_ptr(d), mu, lam, _np_ptr(c),
-                   real_type, keep=[mesh])
+    a, ak = _geom(adjugate)
+    d, dk = _geom(det)
+    c, ck = _geom(coeff)
+    return _new_op("sfem_amd_linear_elasticity_create", mesh.h, a, d, mu, lam, c, real_type, keep=[mesh])
+
+
+def sshex8_refine_device(level, macro_elements, macro_points=None, nnodes=None, stream=None):
+    """Refinement on the device: macro_elements [8][E] (numpy or DeviceArray int32, SoA) and
+    optionally macro_points [3][N] (numpy or DeviceArray float32).  Returns (Mesh, points) with
+    points a DeviceArray [3 * nnodes] (or None); ids identical to sshex8_refine."""
+    if isinstance(macro_elements, DeviceArray):
+        me = macro_elements
+        ne = me.n // 8
+    else:
+        h = np.ascontiguousarray(macro_elements, np.int32)
+        ne = h.shape[1]
+        me = DeviceArray.from_numpy(h)
+    mp = None
+    if macro_points is not None:
+        mp = macro_points if isinstance(macro_points, DeviceArray) else \
+            DeviceArray.from_numpy(np.ascontiguousarray(macro_points, np.float32))
+        nnodes = mp.n // 3
+    if nnodes is None:
+        raise ValueError("sshex8_refine_device: nnodes (or macro_points) required")
+    h = _vp()
+    _check(lib.sfem_amd_sshex8_refine_device(level, ne, nnodes, me.ptr, C.byref(h), _stream_ptr(stream)),
+           "sshex8_refine_device")
+    mesh = Mesh(None, None, None, None, _handle=h)
+    pts = None
+    if mp is not None:
+        pts = DeviceArray(3 * mesh.nnodes, np.float32)
+        _check(lib.sfem_amd_sshex8_refine_points_device(mesh.h, nnodes, me.ptr, mp.ptr, pts.ptr,
+                                                         _stream_ptr(stream)), "sshex8_refine_points_device")
+    return mesh, pts
+
+
+def mesh_fff(mesh, points, stream=None):
+    """FFF [E][6] of a device mesh from device points [3][nnodes] -> DeviceArray float32."""
+    out = DeviceArray(6 * mesh.nelements, np.float32)
+    _check(lib.sfem_amd_mesh_fff(mesh.h, points.ptr, out.ptr, _stream_ptr(stream)), "mesh_fff")
+    return out
+
+
+def mesh_adjugate_det(mesh, points, stream=None):
+    adj = DeviceArray(9 * mesh.nelements, np.float32)
+    det = DeviceArray(mesh.nelements, np.float32)
+    _check(lib.sfem_amd_mesh_adjugate_det(mesh.h, points.ptr, adj.ptr, det.ptr, _stream_ptr(stream)),
+           "mesh_adjugate_det")
+    return adj, det
 
 
 def element_matrix_op(mesh, matrices, block_size=1, real_type=FLOAT64):

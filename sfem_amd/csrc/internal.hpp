@@ -37,7 +37,8 @@ struct DevArray {
     }
     int upload(const T *h, size_t count) {
         if (alloc(count)) return SFEM_FAILURE;
-        if (count) SFEM_AMD_HIP_CHECK(hipMemcpy(p, h, sizeof(T) * count, hipMemcpyHostToDevice));
+        // host or device source (unified addressing)
+        if (count) SFEM_AMD_HIP_CHECK(hipMemcpy(p, h, sizeof(T) * count, hipMemcpyDefault));
         return SFEM_SUCCESS;
     }
 };

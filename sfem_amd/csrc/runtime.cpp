@@ -11,6 +11,7 @@
 
 #include "common.hpp"
 #include "internal.hpp"
+#include "mesh_device.hpp"
 #include "sfem/sfem_Chebyshev3.hpp"
 #include "sfem/sfem_Multigrid.hpp"
 #include "sfem/sfem_Stationary.hpp"
@@ -622,6 +623,17 @@ struct SwitchOp final : OpBase {
         }                                                \
     } while (0)
 
+// geometry arrays may live in host or device memory (device: transformed in place on the device)
+static bool device_pointer(const void *p) {
+    if (!p) return false;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // plain host memory: not an error for this query
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice;
+}
+
 extern "C" {
 
 const char *sfem_amd_last_error(void) { return g_last_error.c_str(); }
@@ -706,6 +718,52 @@ int sfem_amd_mesh_create(int type, int level, ptrdiff_t ne, ptrdiff_t stride, co
     return SFEM_SUCCESS;
 }
 
+int sfem_amd_sshex8_refine_device(int level, ptrdiff_t nelements, ptrdiff_t nnodes, const int32_t *macro_elements,
+                                  sfem_amd_mesh **out, void *stream) {
+    SFEM_AMD_REQUIRE(out, "sshex8_refine_device: output handle");
+    SFEM_AMD_REQUIRE(sshex8_level_supported(level), "sshex8_refine_device: unsupported sshex8 level");
+    SFEM_AMD_REQUIRE(nelements == 0 || (macro_elements && device_pointer(macro_elements)),
+                     "sshex8_refine_device: macro elements must be device memory");
+    auto m = std::make_unique<sfem_amd_mesh>();
+    m->type = SFEM_AMD_SSHEX8;
+    m->level = level;
+    m->E = nelements;
+    m->nv = nxe_of(level);
+    ptrdiff_t nt = 0;
+    if (sshex8_refine_device(level, nelements, nnodes, macro_elements, m->idx, nt, as_stream(stream)))
+        return SFEM_FAILURE;
+    m->nnodes = nt;
+    *out = m.release();
+    return SFEM_SUCCESS;
+}
+
+int sfem_amd_sshex8_refine_points_device(const sfem_amd_mesh *fine, ptrdiff_t nnodes_macro,
+                                         const int32_t *macro_elements, const float *macro_points, float *points,
+                                         void *stream) {
+    CHECK_HANDLE(fine);
+    SFEM_AMD_REQUIRE(fine->type == SFEM_AMD_SSHEX8, "sshex8_refine_points_device: sshex8 mesh required");
+    SFEM_AMD_REQUIRE(fine->E == 0 || (device_pointer(macro_elements) && device_pointer(macro_points) &&
+                                      device_pointer(points)),
+                     "sshex8_refine_points_device: device memory required");
+    return sshex8_refine_points_device(fine->level, fine->E, nnodes_macro, fine->nnodes, macro_elements,
+                                       macro_points, fine->idx.p, points, as_stream(stream));
+}
+
+int sfem_amd_mesh_fff(const sfem_amd_mesh *m, const float *points, float *fff, void *stream) {
+    CHECK_HANDLE(m);
+    SFEM_AMD_REQUIRE(m->E == 0 || (device_pointer(points) && device_pointer(fff)), "mesh_fff: device memory required");
+    return mesh_geometry_device(m->type, m->level, m->E, m->idx.p, m->nnodes, points, 0, fff, nullptr,
+                                as_stream(stream));
+}
+
+int sfem_amd_mesh_adjugate_det(const sfem_amd_mesh *m, const float *points, float *adj, float *det, void *stream) {
+    CHECK_HANDLE(m);
+    SFEM_AMD_REQUIRE(m->type != SFEM_AMD_TET4, "mesh_adjugate_det: HEX8 or SSHEX8");
+    SFEM_AMD_REQUIRE(m->E == 0 || (device_pointer(points) && device_pointer(adj) && device_pointer(det)),
+                     "mesh_adjugate_det: device memory required");
+    return mesh_geometry_device(m->type, m->level, m->E, m->idx.p, m->nnodes, points, 1, adj, det, as_stream(stream));
+}
+
 int sfem_amd_mesh_derefine(const sfem_amd_mesh *fine, int level, sfem_amd_mesh **out) {
     CHECK_HANDLE(fine);
     SFEM_AMD_REQUIRE(fine->type == SFEM_AMD_SSHEX8, "derefine: sshex8 mesh required");
@@ -759,6 +817,11 @@ void sfem_amd_mesh_destroy(sfem_amd_mesh *m) { delete m; }
 
 // ----------------------------------------------------------------- operators
 static int soa_upload(DevArray<float> &dst, const float *aos, ptrdiff_t E, int k) {
+    if (device_pointer(aos)) {
+        if (dst.alloc((size_t)k * E) || aos_to_soa_device(aos, E, k, dst.p, nullptr)) return SFEM_FAILURE;
+        SFEM_AMD_HIP_CHECK(hipDeviceSynchronize());
+        return SFEM_SUCCESS;
+    }
     std::vector<float> h((size_t)k * E);
     for (ptrdiff_t e = 0; e < E; e++)
         for (int q = 0; q < k; q++) h[(size_t)q * E + e] = aos[e * k + q];
@@ -783,6 +846,14 @@ int sfem_amd_laplacian_create(const sfem_amd_mesh *mesh, const float *fff, const
     SFEM_AMD_REQUIRE(mesh->type == SFEM_AMD_SSHEX8, "FP32 Laplacian: sshex8 meshes only");
     auto op = std::make_shared<LaplacianOp<float>>();
     op->m = mesh;
+    if (device_pointer(fff)) {
+        SFEM_AMD_REQUIRE(!coeff || device_pointer(coeff), "laplacian_create: fff and coeff in the same memory");
+        if (op->fff16.alloc((size_t)6 * E) || aos_to_bf16_soa_device(fff, coeff, E, 6, op->fff16.p, nullptr))
+            return SFEM_FAILURE;
+        SFEM_AMD_HIP_CHECK(hipDeviceSynchronize());
+        *out = wrap(op);
+        return SFEM_SUCCESS;
+    }
     std::vector<uint16_t> h((size_t)6 * E);
     for (ptrdiff_t e = 0; e < E; e++) {
         const double c = coeff ? (double)coeff[e] : 1.0;
@@ -849,13 +920,21 @@ int sfem_amd_linear_elasticity_create(const sfem_amd_mesh *mesh, const float *ad
             if (sshex8_elasticity_matrices<float>(mesh->level, E, a.p, d.p, E, mu, lambda, dcoeff.p, op->K.p, nullptr))
                 return SFEM_FAILURE;
         } else {
-            std::vector<uint16_t> ha((size_t)9 * E), hd(E);
-            for (ptrdiff_t e = 0; e < E; e++) {
-                for (int q = 0; q < 9; q++) ha[(size_t)q * E + e] = to_bf16(adj[e * 9 + q]);
-                hd[e] = to_bf16(det[e]);
-            }
             DevArray<uint16_t> a, d;
-            if (a.upload(ha.data(), ha.size()) || d.upload(hd.data(), hd.size())) return SFEM_FAILURE;
+            if (device_pointer(adj)) {
+                SFEM_AMD_REQUIRE(device_pointer(det), "elasticity_create: adjugate and det in the same memory");
+                if (a.alloc((size_t)9 * E) || d.alloc((size_t)E) ||
+                    aos_to_bf16_soa_device(adj, nullptr, E, 9, a.p, nullptr) ||
+                    aos_to_bf16_soa_device(det, nullptr, E, 1, d.p, nullptr))
+                    return SFEM_FAILURE;
+            } else {
+                std::vector<uint16_t> ha((size_t)9 * E), hd(E);
+                for (ptrdiff_t e = 0; e < E; e++) {
+                    for (int q = 0; q < 9; q++) ha[(size_t)q * E + e] = to_bf16(adj[e * 9 + q]);
+                    hd[e] = to_bf16(det[e]);
+                }
+                if (a.upload(ha.data(), ha.size()) || d.upload(hd.data(), hd.size())) return SFEM_FAILURE;
+            }
             if (sshex8_elasticity_matrices<uint16_t>(mesh->level, E, a.p, d.p, E, mu, lambda, dcoeff.p, op->K.p,
                                                      nullptr))
                 return SFEM_FAILURE;
