@@ -205,6 +205,91 @@ __global__ void hex8_elasticity_apply_kernel(const ptrdiff_t E, const int32_t *_
     }
 }
 
+// The same operator with eight lanes per element, one per Gauss point (small meshes, e.g. the hex8
+// levels of a multigrid hierarchy, where one lane per element leaves most of the GPU idle): each
+// lane evaluates its point's contribution to all 24 element rows, a butterfly over the eight lanes
+// sums them, and lane n adds the rows of corner n.
+__global__ void __launch_bounds__(256)
+    hex8_elasticity_apply_qp_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
+                                    const float *__restrict__ adjg, const float *__restrict__ detg,
+                                    const double mu0, const double lambda0, const float *__restrict__ coeff,
+                                    const double *__restrict__ x, double *__restrict__ y) {
+    const double gp[2] = {0.5 - 0.28867513459481288225, 0.5 + 0.28867513459481288225};
+    const int q = threadIdx.x & 7;
+    // every lane of a group of eight takes part in the butterfly, also past the last element
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t - q < E * 8;
+         t += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const ptrdiff_t e = t >> 3;
+        const bool valid = e < E;
+        const ptrdiff_t ee = valid ? e : 0;
+        int32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = idx[k * E + ee];
+        double adj[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) adj[k] = (double)adjg[k * E + ee];
+        const double det = (double)detg[ee];
+        const double c = coeff ? (double)coeff[ee] : 1.0;
+        const double mu = mu0 * c, lambda = lambda0 * c;
+        const double idet = 1.0 / det;
+        const double w = det * 0.125;
+        const double qx = gp[q & 1], qy = gp[(q >> 1) & 1], qz = gp[q >> 2];
+        double Gh[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            double g[3];
+            q1_ref_grad(n, qx, qy, qz, g);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double u = x[(ptrdiff_t)v[n] * 3 + d];
+                Gh[d][0] += u * g[0];
+                Gh[d][1] += u * g[1];
+                Gh[d][2] += u * g[2];
+            }
+        }
+        double F[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                F[d][k] = (Gh[d][0] * adj[0 * 3 + k] + Gh[d][1] * adj[1 * 3 + k] + Gh[d][2] * adj[2 * 3 + k]) * idet;
+        const double tr = F[0][0] + F[1][1] + F[2][2];
+        double Sg[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) Sg[d][k] = mu * (F[d][k] + F[k][d]) + (d == k ? lambda * tr : 0.0);
+        double R[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                R[d][a] = w * idet * (Sg[d][0] * adj[a * 3 + 0] + Sg[d][1] * adj[a * 3 + 1] + Sg[d][2] * adj[a * 3 + 2]);
+        double out[3][8];
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            double g[3];
+            q1_ref_grad(n, qx, qy, qz, g);
+#pragma unroll
+            for (int d = 0; d < 3; d++) out[d][n] = R[d][0] * g[0] + R[d][1] * g[1] + R[d][2] * g[2];
+        }
+#pragma unroll
+        for (int m = 1; m < 8; m <<= 1)
+#pragma unroll
+            for (int n = 0; n < 8; n++)
+#pragma unroll
+                for (int d = 0; d < 3; d++) out[d][n] += __shfl_xor(out[d][n], m, 8);
+        if (valid) {
+#pragma unroll
+            for (int n = 0; n < 8; n++)
+                if (n == q) {
+#pragma unroll
+                    for (int d = 0; d < 3; d++) atomicAdd(&y[(ptrdiff_t)v[n] * 3 + d], out[d][n]);
+                }
+        }
+    }
+}
+
 __global__ void hex8_elasticity_diag_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
                                             const float *__restrict__ adjg, const float *__restrict__ detg,
                                             const double mu0, const double lambda0, const float *__restrict__ coeff,
@@ -260,7 +345,11 @@ int hex8_laplacian_diag(ptrdiff_t E, const int32_t *idx, const float *fff, const
 int hex8_elasticity_apply(ptrdiff_t E, const int32_t *idx, const float *adj, const float *det, double mu,
                           double lambda, const float *coeff, const double *x, double *y, hipStream_t s) {
     if (E == 0) return SFEM_SUCCESS;
-    hex8_elasticity_apply_kernel<<<grid_for(E), kBlock, 0, s>>>(E, idx, adj, det, mu, lambda, coeff, x, y);
+    // one lane per element fills the GPU only from ~2^18 elements on; below, a lane per Gauss point
+    if (E < (ptrdiff_t)1 << 18)
+        hex8_elasticity_apply_qp_kernel<<<grid_for(E * 8), kBlock, 0, s>>>(E, idx, adj, det, mu, lambda, coeff, x, y);
+    else
+        hex8_elasticity_apply_kernel<<<grid_for(E), kBlock, 0, s>>>(E, idx, adj, det, mu, lambda, coeff, x, y);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
 }

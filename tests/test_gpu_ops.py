@@ -136,8 +136,9 @@ def test_empty_mesh_apply_is_noop():
 
 
 # ---------------------------------------------------------------- elasticity
-def test_hex8_elasticity_parity():
-    el, pts = skewed_box(6, perturb=0.02)
+@pytest.mark.parametrize("n", [6, 65])  # one lane per Gauss point (small) / per element (>= 2^18 elements)
+def test_hex8_elasticity_parity(n):
+    el, pts = skewed_box(n, perturb=0.02 * 6 / n)
     adj, det = O.hex8_adj_det(el, pts)
     coeff = np.random.default_rng(4).uniform(0.1, 10, el.shape[1]).astype(np.float32)
     mu, lam = O.lame(1.0, 0.3)
@@ -192,7 +193,7 @@ def bf16_round(a):
     return u.astype(np.uint32).view(np.float32)
 
 
-@pytest.mark.parametrize("L,n", [(2, 3), (4, 2), (6, 2), (8, 2), (12, 1)])
+@pytest.mark.parametrize("L,n", [(4, 3), (4, 2), (6, 2), (8, 2), (12, 1), (16, 1)])
 def test_sshex8_elasticity_fp32_matrix_cores(L, n):
     """The FP32 sshex8 elasticity operator runs on the matrix cores (bf16 hi/lo split products,
     FP32 accumulation).  On the same BF16-rounded geometry as the FP64 oracle it must agree to the
