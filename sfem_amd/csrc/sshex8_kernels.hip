@@ -475,7 +475,7 @@ __global__ void __launch_bounds__(256)
     sshex8_elasticity_apply_mfma_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
                                         const float *__restrict__ Kall, const float *__restrict__ x,
                                         float *__restrict__ y, float *__restrict__ part, const int overwrite,
-                                        const int kshift) {
+                                        const int kshift, const int32_t *__restrict__ sk_pos) {
     using S = ElMfma<L>;
     constexpr int N1 = S::N1, NXE = S::NXE, H = S::H, NC = S::NC, TPC = S::TPC, EPB = S::EPB, WPE = S::WPE;
     constexpr int NSK = sk_count(L);
@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(256)
 
     // node k = tid + 256 i of the block: ids first, then all values at once; interior nodes
     // start from their y (accumulate), skeleton nodes from 0
-    int32_t gid[NPT];
+    int32_t gid[NPT], pos[NPT];  // node ids; node-sorted skeleton slots (-1: interior)
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         const int k = tid + 256 * i, le = k / NXE;
@@ -498,7 +498,9 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         const int k = tid + 256 * i, le = k / NXE, v = k - le * NXE;
-        const bool interior = sk_slot(L, v % N1, (v / N1) % N1, v / (N1 * N1)) < 0;
+        const int slot = sk_slot(L, v % N1, (v / N1) % N1, v / (N1 * N1));
+        const bool interior = slot < 0;
+        pos[i] = (gid[i] >= 0 && !interior) ? sk_pos[(e0 + le) * NSK + slot] : -1;
         float xv[3] = {0, 0, 0}, yv[3] = {0, 0, 0};
         if (gid[i] >= 0) {
             const float *xp = x + (ptrdiff_t)gid[i] * 3;
@@ -580,8 +582,7 @@ __global__ void __launch_bounds__(256)
     for (int i = 0; i < NPT; i++) {
         const int k = tid + 256 * i, le2 = k / NXE, v = k - le2 * NXE;
         if (gid[i] < 0) continue;
-        const int slot = sk_slot(L, v % N1, (v / N1) % N1, v / (N1 * N1));
-        float *q = slot < 0 ? y + (ptrdiff_t)gid[i] * 3 : part + ((e0 + le2) * (ptrdiff_t)NSK + slot) * 3;
+        float *q = pos[i] < 0 ? y + (ptrdiff_t)gid[i] * 3 : part + (ptrdiff_t)pos[i] * 3;
 #pragma unroll
         for (int ci = 0; ci < 3; ci++) q[ci] = sy[le2][ci][v];
     }
@@ -597,7 +598,7 @@ __global__ void __launch_bounds__(256)
     sshex8_elasticity_apply_mfma64_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
                                           const double *__restrict__ Kall, const double *__restrict__ x,
                                           double *__restrict__ y, double *__restrict__ part, const int overwrite,
-                                          const int kshift) {
+                                          const int kshift, const int32_t *__restrict__ sk_pos) {
     using S = ElMfma<L>;
     using M = MfmaAcc<double>;
     constexpr int N1 = S::N1, NXE = S::NXE, H = S::H, NC = S::NC, TPC = S::TPC, EPB = S::EPB, WPE = S::WPE;
@@ -610,7 +611,7 @@ __global__ void __launch_bounds__(256)
     const int c = lane & 15, g = lane >> 4;
     const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
 
-    int32_t gid[NPT];
+    int32_t gid[NPT], pos[NPT];  // node ids; node-sorted skeleton slots (-1: interior)
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         const int k = tid + 256 * i, le = k / NXE;
@@ -619,7 +620,9 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         const int k = tid + 256 * i, le = k / NXE, v = k - le * NXE;
-        const bool interior = sk_slot(L, v % N1, (v / N1) % N1, v / (N1 * N1)) < 0;
+        const int slot = sk_slot(L, v % N1, (v / N1) % N1, v / (N1 * N1));
+        const bool interior = slot < 0;
+        pos[i] = (gid[i] >= 0 && !interior) ? sk_pos[(e0 + le) * NSK + slot] : -1;
         double xv[3] = {0, 0, 0}, yv[3] = {0, 0, 0};
         if (gid[i] >= 0) {
             const double *xp = x + (ptrdiff_t)gid[i] * 3;
@@ -692,8 +695,7 @@ __global__ void __launch_bounds__(256)
     for (int i = 0; i < NPT; i++) {
         const int k = tid + 256 * i, le2 = k / NXE, v = k - le2 * NXE;
         if (gid[i] < 0) continue;
-        const int slot = sk_slot(L, v % N1, (v / N1) % N1, v / (N1 * N1));
-        double *q = slot < 0 ? y + (ptrdiff_t)gid[i] * 3 : part + ((e0 + le2) * (ptrdiff_t)NSK + slot) * 3;
+        double *q = pos[i] < 0 ? y + (ptrdiff_t)gid[i] * 3 : part + (ptrdiff_t)pos[i] * 3;
 #pragma unroll
         for (int ci = 0; ci < 3; ci++) q[ci] = sy[le2][ci][v];
     }
@@ -744,16 +746,17 @@ bool sshex8_level_supported(int L) {
 }
 
 template <typename T>
-int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStream_t s, int overwrite) {
+int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStream_t s, int overwrite,
+                    int node_sorted) {
     if (sk.n == 0) return SFEM_SUCCESS;
-    // scalar partials are node-sorted (read in place); vector partials are element-major (read
-    // through ent)
+    // node-sorted partials are read in place; element-major ones through ent.  Default: scalar
+    // partials node-sorted, vector partials element-major
+    if (node_sorted < 0) node_sorted = bs == 1;
+    const int32_t *ent = node_sorted ? nullptr : sk.ent;
     if (bs == 1)
-        skeleton_gather_kernel<T, 1>
-                <<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, nullptr, part, y, overwrite);
+        skeleton_gather_kernel<T, 1><<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, ent, part, y, overwrite);
     else if (bs == 3)
-        skeleton_gather_kernel<T, 3>
-                <<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, sk.ent, part, y, overwrite);
+        skeleton_gather_kernel<T, 3><<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, ent, part, y, overwrite);
     else {
         set_last_error("skeleton_gather: block size must be 1 or 3");
         return SFEM_FAILURE;
@@ -864,13 +867,13 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
     case LV:                                                                                                \
         sshex8_elasticity_apply_mfma64_kernel<LV>                                                           \
                 <<<(unsigned)((E + ElMfma<LV>::EPB - 1) / ElMfma<LV>::EPB), 256, 0, s>>>(E, idx, K, x, y, part, \
-                                                                                         overwrite, kshift); \
+                                                                                         overwrite, kshift, sk.pos); \
         break;
                 MCASE(2) MCASE(4) MCASE(6) MCASE(8)
 #undef MCASE
             }
             SFEM_AMD_HIP_CHECK(hipGetLastError());
-            return skeleton_gather<T>(sk, 3, part, y, s, overwrite);
+            return skeleton_gather<T>(sk, 3, part, y, s, overwrite, 1);  // node-sorted partials
         }
     }
     if constexpr (std::is_same<T, float>::value) {
@@ -879,13 +882,13 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
 #define MCASE(LV)                                                                                               \
     case LV:                                                                                                    \
         sshex8_elasticity_apply_mfma_kernel<LV><<<(unsigned)((E + ElMfma<LV>::EPB - 1) / ElMfma<LV>::EPB), 256, 0, \
-                                                  s>>>(E, idx, K, x, y, part, overwrite, kshift);               \
+                                                  s>>>(E, idx, K, x, y, part, overwrite, kshift, sk.pos);       \
         break;
                 MCASE(2) MCASE(4) MCASE(6) MCASE(8) MCASE(10) MCASE(12)
 #undef MCASE
             }
             SFEM_AMD_HIP_CHECK(hipGetLastError());
-            return skeleton_gather<T>(sk, 3, part, y, s, overwrite);
+            return skeleton_gather<T>(sk, 3, part, y, s, overwrite, 1);  // node-sorted partials
         }
     }
     switch (L) {
@@ -926,8 +929,8 @@ int sshex8_elasticity_diag(int L, ptrdiff_t E, const int32_t *idx, const double 
     return SFEM_SUCCESS;
 }
 
-template int skeleton_gather<double>(const SkeletonMap &, int, const double *, double *, hipStream_t, int);
-template int skeleton_gather<float>(const SkeletonMap &, int, const float *, float *, hipStream_t, int);
+template int skeleton_gather<double>(const SkeletonMap &, int, const double *, double *, hipStream_t, int, int);
+template int skeleton_gather<float>(const SkeletonMap &, int, const float *, float *, hipStream_t, int, int);
 template int sshex8_laplacian_apply<double, float>(int, ptrdiff_t, const int32_t *, const float *, ptrdiff_t,
                                                    const float *, const double *, const double *, double *, double *,
                                                    const SkeletonMap &, hipStream_t, int);

@@ -20,7 +20,10 @@ struct SkeletonMap {
 };
 // y[node[t]*bs + c] (+)= sum_{j in [off[t], off[t+1])} part[(bs == 1 ? j : ent[j])*bs + c]
 template <typename T>
-int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStream_t s, int overwrite = 0);
+// node_sorted: partial j of the node-sorted list at part[j * bs] (1) or at part[ent[j] * bs] (0);
+// -1: scalar partials node-sorted, vector partials element-major
+int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStream_t s, int overwrite = 0,
+                    int node_sorted = -1);
 
 // G = float (FP64 path) or uint16_t holding bf16 (FP32 path).
 template <typename T, typename G>
