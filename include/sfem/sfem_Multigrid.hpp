@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "sfem_Buffer.hpp"
+#include "sfem_MatrixFreeLinearSolver.hpp"
 #include "sfem_Operator.hpp"
 #include "sfem_tpl_blas.hpp"
 
@@ -38,6 +39,9 @@ public:
         memory_.clear();
     }
     void set_cycle_type(int v) { cycle_type_ = v; }
+    // x is zero on entry to apply() (e.g. a preconditioner whose caller zeroes its output): the
+    // first finest-level pre-smoothing then skips its product with the zero iterate.
+    void set_initial_guess_zero(bool v) { is_initial_guess_zero_ = v; }
     void set_max_it(int v) { max_it_ = v; }
     void set_atol(T v) { atol_ = v; }
     int n_levels() const { return (int)smoother_.size(); }
@@ -74,6 +78,7 @@ public:
 private:
     int max_it_{10};
     int iterations_{0};
+    bool is_initial_guess_zero_{false};
     int cycle_type_{V_CYCLE};
     T atol_{1e-10};
 
@@ -91,6 +96,15 @@ private:
         }
     }
 
+    // Runs a smoother that may be a MatrixFreeLinearSolver in its zero-initial-guess mode; smoothers
+    // inside a multigrid otherwise run with a general initial guess (post-smoothing needs it).
+    static void pre_smooth(const SharedOperator<T> &smoother, bool zero_x, const T *rhs, T *x) {
+        auto *mf = zero_x ? dynamic_cast<MatrixFreeLinearSolver<T> *>(smoother.get()) : nullptr;
+        if (mf) mf->set_initial_guess_zero(true);
+        smoother->apply(rhs, x);
+        if (mf) mf->set_initial_guess_zero(false);
+    }
+
     // algebra/sfem_Multigrid.hpp:242-346
     CycleReturnCode cycle(int level) {
         Memory &mem = memory_[level];
@@ -105,7 +119,10 @@ private:
         auto &prolongation = prolongation_[level + 1];
         Memory &coarse = memory_[level + 1];
         for (int k = 0; k < cycle_type_; k++) {
-            smoother->apply(mem.rhs->data(), mem.solution->data());
+            // coarse solutions arrive zeroed (below), the finest one when the caller says so: the
+            // smoother's first residual is then rhs itself, and A 0 is not computed (same result)
+            const bool zero_x = k == 0 && (level > 0 || (is_initial_guess_zero_ && iterations_ == 0));
+            pre_smooth(smoother, zero_x, mem.rhs->data(), mem.solution->data());
             // residual
             if (!apply_overwrites_output) blas_->zeros(mem.size(), mem.work->data());
             op->apply(mem.solution->data(), mem.work->data());

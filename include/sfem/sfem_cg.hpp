@@ -106,6 +106,15 @@ private:
     T update_x_r(std::ptrdiff_t n, T alpha, const T *p, const T *Ap, T *x, T *r, bool with_rtr) {
         return (T)sfem_amd_cg_update_x_r(real_type_of<T>::value, n, alpha, p, Ap, x, r, with_rtr ? 1 : 0, stream);
     }
+    // z = P r from z = 0 whatever the operator does; a solver or multigrid preconditioner is told
+    // that its initial guess is zero for this apply (it may skip products with it)
+    void precondition(std::ptrdiff_t n, const T *r, T *z) {
+        blas->zeros(n, z);
+        auto *mf = dynamic_cast<MatrixFreeLinearSolver<T> *>(preconditioner_op.get());
+        if (mf) mf->set_initial_guess_zero(true);
+        preconditioner_op->apply(r, z);
+        if (mf) mf->set_initial_guess_zero(false);
+    }
     void residual(std::ptrdiff_t n, const T *b, const T *x, T *r) {
         zero_for_apply(n, r);
         apply_op->apply(x, r);
@@ -149,8 +158,7 @@ private:
         const T rtr0 = blas->dot(n, r, r);
         iterations_ = 0;
         if (rtr0 == 0) return SFEM_SUCCESS;
-        blas->zeros(n, z);  // the preconditioner starts from z = 0 whatever the operator does
-        preconditioner_op->apply(r, z);
+        precondition(n, r, z);
         blas->copy(n, z, p);
         zero_for_apply(n, Ap);
         apply_op->apply(p, Ap);
@@ -163,8 +171,7 @@ private:
         }
         const T bnorm = stop_on_residual_norm ? blas->norm2(n, b) : T(1);
         for (iterations_ = 0; iterations_ < max_it; iterations_++) {
-            blas->zeros(n, z);
-            preconditioner_op->apply(r, z);
+            precondition(n, r, z);
             const T rtz_new = blas->dot(n, r, z);
             // exact convergence (e.g. a one-unknown coarse problem): the next step would be 0/0
             if (rtz_new == 0) return SFEM_SUCCESS;
@@ -201,8 +208,7 @@ private:
         iterations_ = 0;
         const T ref = bnorm > 0 ? bnorm : T(1);
         if (rnorm == 0 || rnorm / ref < rtol || rnorm < atol) return SFEM_SUCCESS;
-        blas->zeros(n, z);  // the preconditioner starts from z = 0 whatever the operator does
-        preconditioner_op->apply(r, z);
+        precondition(n, r, z);
         blas->copy(n, z, p);
         T rtz = blas->dot(n, r, z);
         for (iterations_ = 0; iterations_ < max_it; iterations_++) {
@@ -233,8 +239,7 @@ private:
                 rnorm = true_norm;
             }
             blas->copy(n, z, zold);
-            blas->zeros(n, z);
-            preconditioner_op->apply(r, z);
+            precondition(n, r, z);
             const T rtz_new = blas->dot(n, r, z);
             const T rtzold = blas->dot(n, r, zold);
             const T beta = (rtz_new - rtzold) / rtz;

@@ -86,6 +86,9 @@ struct OpBase {
         set_last_error("block diagonal not available for this operator");
         return SFEM_FAILURE;
     }
+    // Solvers and smoothers: the output holds zeros on entry to the next applies (work that
+    // multiplies the initial guess may be skipped).  Other operators ignore it.
+    virtual void set_initial_guess_zero(bool) {}
     virtual int iterations() const { return -1; }
     virtual int last_status() const { return SFEM_SUCCESS; }
     virtual const std::vector<double> *history() const { return nullptr; }

@@ -217,6 +217,7 @@ struct ElasticityOp final : OpBase {
     DevArray<float> adj, det, coeff;  // hex8: SoA [9][E], [E]
     DevArray<double> K;               // sshex8: per-macro 24x24 matrices
     DevArray<float> K32;              // sshex8, FP32 operator: K rounded to float
+    DevArray<uint32_t> Apk;           // FP32 matrix cores: K32 split into bf16 A operands
     // FP32 from geometry: the bf16x3 MFMA kernel (FP64 always runs on the matrix cores: exact)
     bool matrix_cores{false};
     DevArray<T> part;                 // sshex8: skeleton contributions [E][sk_count(L)][3]
@@ -290,9 +291,15 @@ struct ElasticityOp final : OpBase {
                 if (K32.alloc(K.n) || blas_cast<double, float>((ptrdiff_t)K.n, K.p, K32.p, s)) return SFEM_FAILURE;
             }
             Kt = K32.p;
+            if (matrix_cores && !Apk.p && K32.n) {
+                const ptrdiff_t nK = (ptrdiff_t)K32.n / 576;
+                if (Apk.alloc((size_t)nK * 1024) || sshex8_elasticity_pack_bf16x2(nK, K32.p, Apk.p, s))
+                    return SFEM_FAILURE;
+            }
         }
         return sshex8_elasticity_apply<T>(wm->level, wm->E, wm->idx.p, Kt, (const T *)xv, (T *)yv, part.p,
-                                          skeleton_of(wm), s, ow, uses_matrix_cores() ? 1 : 0, wm == m ? 0 : 3);
+                                          skeleton_of(wm), s, ow, uses_matrix_cores() ? 1 : 0, wm == m ? 0 : 3,
+                                          Apk.p);
     }
     int diag(void *d, hipStream_t s) override {
         if (m->type == SFEM_AMD_HEX8) {
@@ -423,8 +430,11 @@ struct ChainOp final : OpBase {
 };
 
 // ----------------------------------------------------------------- sfem:: template bridges
+// A runtime operator seen through the sfem:: interfaces.  It is a MatrixFreeLinearSolver so that
+// a multigrid or CG can pass set_initial_guess_zero on to a wrapped smoother or preconditioner;
+// the solver set-up calls are not forwarded (the wrapped object is configured at creation).
 template <typename T>
-struct OpView final : sfem::Operator<T> {
+struct OpView final : sfem::MatrixFreeLinearSolver<T> {
     std::shared_ptr<OpBase> impl;
     hipStream_t s{nullptr};
     bool overwrite{false};  // apply() overwrites y (the owning solver skips its zero fill)
@@ -432,6 +442,12 @@ struct OpView final : sfem::Operator<T> {
     int apply(const T *const x, T *const y) override {
         return overwrite ? impl->apply_overwrite(x, y, as_stream(s)) : impl->apply(x, y, as_stream(s));
     }
+    void set_op(const sfem::SharedOperator<T> &) override {}
+    void set_preconditioner_op(const sfem::SharedOperator<T> &) override {}
+    void set_max_it(int) override {}
+    void set_n_dofs(std::ptrdiff_t) override {}
+    void set_initial_guess_zero(bool v) override { impl->set_initial_guess_zero(v); }
+    int iterations() const override { return impl->iterations(); }
     std::ptrdiff_t rows() const override { return impl->rows(); }
     std::ptrdiff_t cols() const override { return impl->cols(); }
     sfem::ExecutionSpace execution_space() const override { return sfem::EXECUTION_SPACE_DEVICE; }
@@ -454,6 +470,12 @@ struct SolverOp final : OpBase {
     int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
     ptrdiff_t rows() const override { return n; }
     int apply(const void *x, void *y, hipStream_t) override { return op->apply((const T *)x, (T *)y); }
+    void set_initial_guess_zero(bool v) override {
+        if (auto *m = dynamic_cast<sfem::MatrixFreeLinearSolver<T> *>(op.get()))
+            m->set_initial_guess_zero(v);
+        else if (auto *mg = dynamic_cast<sfem::Multigrid<T> *>(op.get()))
+            mg->set_initial_guess_zero(v);
+    }
     int iterations() const override { return its ? its() : -1; }
     int last_status() const override { return status ? status() : SFEM_SUCCESS; }
     const std::vector<double> *history() const override { return hist ? hist() : nullptr; }
@@ -470,7 +492,10 @@ struct PrecisionBridge final : OpBase {
         if ((ptrdiff_t)xf.n != n && (xf.alloc(n) || yf.alloc(n))) return SFEM_FAILURE;
         if (blas_cast<double, float>(n, (const double *)x, xf.p, s)) return SFEM_FAILURE;
         if (blas_values<float>(n, 0.f, yf.p, s)) return SFEM_FAILURE;
-        if (op32->apply(xf.p, yf.p, s)) return SFEM_FAILURE;
+        op32->set_initial_guess_zero(true);  // yf was just zeroed
+        const int err = op32->apply(xf.p, yf.p, s);
+        op32->set_initial_guess_zero(false);
+        if (err) return SFEM_FAILURE;
         return cast_accumulate(n, yf.p, (double *)y, s);  // y += (double) yf
     }
 };
@@ -563,6 +588,7 @@ struct GraphOp final : OpBase {
     hipStream_t cs{nullptr};
     int calls{0}, replays{0};
     bool failed{false};
+    bool zero_guess{false}, czero{false};  // current / captured set_initial_guess_zero
     ~GraphOp() override {
         if (exec) (void)hipGraphExecDestroy(exec);
     }
@@ -570,6 +596,10 @@ struct GraphOp final : OpBase {
     ptrdiff_t rows() const override { return inner->rows(); }
     ptrdiff_t cols() const override { return inner->cols(); }
     int iterations() const override { return inner->iterations(); }
+    void set_initial_guess_zero(bool v) override {
+        zero_guess = v;
+        inner->set_initial_guess_zero(v);
+    }
     int apply(const void *x, void *y, hipStream_t s) override {
         if (calls++ == 0 || failed) return inner->apply(x, y, s);
         if (!exec) {
@@ -593,8 +623,9 @@ struct GraphOp final : OpBase {
             cx = x;
             cy = y;
             cs = s;
+            czero = zero_guess;
         }
-        if (x == cx && y == cy && s == cs) {
+        if (x == cx && y == cy && s == cs && zero_guess == czero) {
             replays++;
             SFEM_AMD_HIP_CHECK(hipGraphLaunch(exec, s));
             return SFEM_SUCCESS;
@@ -612,6 +643,10 @@ struct SwitchOp final : OpBase {
     int real_type() const override { return ops[0]->real_type(); }
     ptrdiff_t rows() const override { return ops[0]->rows(); }
     int apply(const void *x, void *y, hipStream_t s) override { return ops[active]->apply(x, y, s); }
+    void set_initial_guess_zero(bool v) override {
+        ops[0]->set_initial_guess_zero(v);
+        ops[1]->set_initial_guess_zero(v);
+    }
 };
 }  // namespace
 

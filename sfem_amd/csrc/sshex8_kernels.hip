@@ -436,6 +436,38 @@ __global__ void __launch_bounds__(ElShape<L>::BLOCK)
 // of one colour share no node, so each colour's contributions are added to the LDS node
 // accumulators without atomics, colour by colour (a fixed order: deterministic).  Interior nodes
 // then update y; skeleton nodes store to the element's slots of `part` (as the vector kernel).
+// Skeleton slot (sk_slot) of each local node id, -1 for interior nodes: one table load in place
+// of the coordinate division and the face/edge tests per node.
+// LDS layout of the FP32 matrix-core kernel's node accumulators: node (x, y, z) of component
+// plane c at c * PS + z * PL + y * RS + x.  LDS serves a half-wave (32 lanes) per pass and dword
+// slot from 32 banks; with the padded L = 8 strides the read-modify-writes of one parity colour
+// (the corners of 16 micro elements in two component planes per half-wave) meet each bank once
+// per pass, against twice with the dense strides (bank model checked against SQ_LDS_BANK_CONFLICT).
+template <int L>
+struct ElLds {
+    static constexpr int RS = L + 1, PL = (L + 1) * (L + 1), PS = (L + 1) * (L + 1) * (L + 1);
+};
+template <>
+struct ElLds<8> {
+    static constexpr int RS = 12, PL = 109, PS = 977;
+};
+
+template <int L>
+struct SkSlotTable {
+    int16_t v[(L + 1) * (L + 1) * (L + 1)];    // skeleton slot
+    int16_t lds[(L + 1) * (L + 1) * (L + 1)];  // ElLds position in a component plane
+    constexpr SkSlotTable() : v(), lds() {
+        for (int z = 0; z <= L; z++)
+            for (int y = 0; y <= L; y++)
+                for (int x = 0; x <= L; x++) {
+                    v[lidx(L, x, y, z)] = (int16_t)sk_slot(L, x, y, z);
+                    lds[lidx(L, x, y, z)] = (int16_t)(z * ElLds<L>::PL + y * ElLds<L>::RS + x);
+                }
+    }
+};
+template <int L>
+__constant__ SkSlotTable<L> sk_slot_table = SkSlotTable<L>();
+
 template <int L>
 struct ElMfma {
     static constexpr int N1 = L + 1, NXE = N1 * N1 * N1, H = L / 2;
@@ -468,20 +500,52 @@ __device__ __forceinline__ void el_unpack8(const uint32_t (&p)[8], el_b8 &hi, el
     lo = __builtin_bit_cast(el_b8, l);
 }
 
+typedef uint32_t el_u4 __attribute__((ext_vector_type(4)));
+
+// The A operands of the FP32 kernel below, split once per matrix: for matrix e, 16-row tile tt and
+// lane (c, g) = (lane % 16, lane / 16), the bf16 hi and lo parts of K rows 16 tt + c, columns
+// 8 g .. 8 g + 7 (zero outside the 24 x 24 matrix), at Apk[((e * 2 + tt) * 2 + part) * 64 + lane].
+__global__ void sshex8_elasticity_pack_bf16x2_kernel(const ptrdiff_t nK, const float *__restrict__ K,
+                                                     el_u4 *__restrict__ Apk) {
+    const ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nK * 128) return;
+    const ptrdiff_t e = t >> 7;
+    const int tt = (int)(t >> 6) & 1, lane = (int)t & 63, c = lane & 15, g = lane >> 4;
+    const int row = 16 * tt + c;
+    const bool ok = row < 24 && g < 3;
+    const float *kr = K + e * 576 + (ok ? row * 24 + 8 * g : 0);
+    uint32_t pk[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) pk[j] = el_split_pack(ok ? kr[j] : 0.f);
+    el_b8 hi, lo;
+    el_unpack8(pk, hi, lo);
+    Apk[((e * 2 + tt) * 2 + 0) * 64 + lane] = __builtin_bit_cast(el_u4, hi);
+    Apk[((e * 2 + tt) * 2 + 1) * 64 + lane] = __builtin_bit_cast(el_u4, lo);
+}
+
+int sshex8_elasticity_pack_bf16x2(ptrdiff_t nK, const float *K, uint32_t *Apk, hipStream_t s) {
+    if (nK == 0) return SFEM_SUCCESS;
+    sshex8_elasticity_pack_bf16x2_kernel<<<(unsigned)((nK * 128 + 255) / 256), 256, 0, s>>>(nK, K, (el_u4 *)Apk);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
 // Consecutive groups of 2^kshift elements share one matrix (kshift > 0: the sub-blocks of a macro
-// element of a higher level, see sshex8_subblock_idx).
+// element of a higher level, see sshex8_subblock_idx).  Apk: the matrices' packed A operands
+// (sshex8_elasticity_pack_bf16x2).
 template <int L>
 __global__ void __launch_bounds__(256)
     sshex8_elasticity_apply_mfma_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
-                                        const float *__restrict__ Kall, const float *__restrict__ x,
+                                        const el_u4 *__restrict__ Apk, const float *__restrict__ x,
                                         float *__restrict__ y, float *__restrict__ part, const int overwrite,
                                         const int kshift, const int32_t *__restrict__ sk_pos) {
     using S = ElMfma<L>;
     constexpr int N1 = S::N1, NXE = S::NXE, H = S::H, NC = S::NC, TPC = S::TPC, EPB = S::EPB, WPE = S::WPE;
     constexpr int NSK = sk_count(L);
     constexpr int NPT = (EPB * NXE + 255) / 256;  // nodes per thread in the load and store phases
+    using Y = ElLds<L>;
     __shared__ uint32_t sx[EPB][3][NXE];  // x split into bf16 hi/lo pairs, one plane per component
-    __shared__ float sy[EPB][3][NXE];     // node accumulators
+    __shared__ float sy[EPB][3][Y::PS];   // node accumulators
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int c = lane & 15, g = lane >> 4;
@@ -490,6 +554,7 @@ __global__ void __launch_bounds__(256)
     // node k = tid + 256 i of the block: ids first, then all values at once; interior nodes
     // start from their y (accumulate), skeleton nodes from 0
     int32_t gid[NPT], pos[NPT];  // node ids; node-sorted skeleton slots (-1: interior)
+    int lds[NPT];                // LDS positions
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         const int k = tid + 256 * i, le = k / NXE;
@@ -498,7 +563,8 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         const int k = tid + 256 * i, le = k / NXE, v = k - le * NXE;
-        const int slot = sk_slot(L, v % N1, (v / N1) % N1, v / (N1 * N1));
+        const int slot = sk_slot_table<L>.v[v];
+        lds[i] = sk_slot_table<L>.lds[v];
         const bool interior = slot < 0;
         pos[i] = (gid[i] >= 0 && !interior) ? sk_pos[(e0 + le) * NSK + slot] : -1;
         float xv[3] = {0, 0, 0}, yv[3] = {0, 0, 0};
@@ -515,7 +581,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int ci = 0; ci < 3; ci++) {
                 sx[le][ci][v] = el_split_pack(xv[ci]);
-                sy[le][ci][v] = yv[ci];
+                sy[le][ci][lds[i]] = yv[ci];
             }
         }
     }
@@ -525,36 +591,56 @@ __global__ void __launch_bounds__(256)
     const ptrdiff_t e = e0 + le;
     const bool active = e < E;
     el_b8 ahi[2], alo[2];
+    {
+        const el_u4 *ak = Apk + (active ? (e >> kshift) : 0) * 256 + lane;
 #pragma unroll
-    for (int tt = 0; tt < 2; tt++) {
-        const int row = 16 * tt + c;
-        const bool ok = active && row < 24 && g < 3;
-        const float *kr = Kall + (ok ? (e >> kshift) * 576 + row * 24 + 8 * g : 0);
-        uint32_t pk[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) pk[j] = el_split_pack(ok ? kr[j] : 0.f);
-        el_unpack8(pk, ahi[tt], alo[tt]);
+        for (int tt = 0; tt < 2; tt++) {
+            ahi[tt] = __builtin_bit_cast(el_b8, ak[(2 * tt) * 64]);
+            alo[tt] = __builtin_bit_cast(el_b8, ak[(2 * tt + 1) * 64]);
+        }
     }
     __syncthreads();
 
-    const int gc = g < 3 ? g : 0;
+    // Tiles of 16 micro elements of one colour: wave wsub takes tiles wsub, wsub + WPE, ...  Lane
+    // (c, g) feeds B rows 8g .. 8g + 7 of column c (component g, corners 0..7; lanes g = 3 meet zero
+    // columns of A and read plane 2 with the g = 2 lanes of their half-wave, adding no bank traffic)
+    // and receives D rows 16tt + 4g + r: component (16tt + 4g) / 8, corner r (g even) or r + 4 (g
+    // odd, one plane up in z).  Addresses: per lane and tile for colour 0, plus a wave-uniform
+    // colour shift.
+    constexpr int TPW = (TPC + WPE - 1) / WPE;
+    const int gc = g < 3 ? g : 2;
+    const uint32_t *sxl = &sx[0][0][0] + (le * 3 + gc) * NXE;
+    float *syl[2];
+#pragma unroll
+    for (int tt = 0; tt < 2; tt++) {
+        const int ci = (16 * tt + 4 * g) >> 3;
+        syl[tt] = &sy[0][0][0] + (le * 3 + (ci < 3 ? ci : 0)) * Y::PS + ((g & 1) ? Y::PL : 0);
+    }
+    const bool row_ok[2] = {true, g < 2};
+    int xb[TPW], yb[TPW];
+    bool valid[TPW];
+#pragma unroll
+    for (int u = 0; u < TPW; u++) {
+        const int t = wsub + u * WPE;
+        const unsigned q = 16 * t + c;
+        valid[u] = active && t < TPC && q < (unsigned)NC;
+        const unsigned qq = valid[u] ? q : 0;
+        const int mx = 2 * (qq % H), my = 2 * ((qq / H) % H), mz = 2 * (qq / (H * H));
+        xb[u] = lidx(L, mx, my, mz);
+        yb[u] = mz * Y::PL + my * Y::RS + mx;
+    }
 #pragma unroll 1
     for (int colour = 0; colour < 8; colour++) {
         const int px = colour & 1, py = (colour >> 1) & 1, pz = colour >> 2;
-        for (int t = wsub; t < TPC; t += WPE) {
-            const unsigned q = 16 * t + c;
-            const bool valid = active && q < (unsigned)NC;
-            const unsigned qq = valid ? q : 0;
-            const int mx = 2 * (qq % H) + px, my = 2 * ((qq / H) % H) + py, mz = 2 * (qq / (H * H)) + pz;
-            const int base = lidx(L, mx, my, mz);
+        const int xc = lidx(L, px, py, pz), yc = pz * Y::PL + py * Y::RS + px;
+#pragma unroll
+        for (int u = 0; u < TPW; u++) {
+            if (wsub + u * WPE >= TPC) break;  // wave-uniform
+            // in range for every lane (qq, gc clamped); invalid columns are never scattered
+            const uint32_t *xp = sxl + xb[u] + xc;
             uint32_t pk[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int node = base + lidx(L, hex8_cx(j), hex8_cy(j), hex8_cz(j));
-                // in range for every lane (qq, gc clamped); rows k >= 24 meet zero columns of A and
-                // invalid columns are never scattered, so no masking is needed
-                pk[j] = sx[le][gc][node];
-            }
+            for (int j = 0; j < 8; j++) pk[j] = xp[lidx(L, hex8_cx(j), hex8_cy(j), hex8_cz(j))];
             el_b8 bhi, blo;
             el_unpack8(pk, bhi, blo);
 #pragma unroll
@@ -563,15 +649,12 @@ __global__ void __launch_bounds__(256)
                 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[tt], bhi, d, 0, 0, 0);
                 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[tt], blo, d, 0, 0, 0);
                 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[tt], bhi, d, 0, 0, 0);
-                // lane holds rows R = 16tt + 4g + r of micro element q: component R / 8, corner R % 8
-                const int R0 = 16 * tt + 4 * g;
-                if (valid && R0 < 24) {
-                    const int ci = R0 >> 3;
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int k = (R0 & 7) + r;
-                        sy[le][ci][base + lidx(L, hex8_cx(k), hex8_cy(k), hex8_cz(k))] += d[r];
-                    }
+                if (valid[u] && row_ok[tt]) {
+                    float *yp = syl[tt] + yb[u] + yc;
+                    yp[0] += d[0];
+                    yp[1] += d[1];
+                    yp[Y::RS + 1] += d[2];
+                    yp[Y::RS] += d[3];
                 }
             }
         }
@@ -584,7 +667,7 @@ __global__ void __launch_bounds__(256)
         if (gid[i] < 0) continue;
         float *q = pos[i] < 0 ? y + (ptrdiff_t)gid[i] * 3 : part + (ptrdiff_t)pos[i] * 3;
 #pragma unroll
-        for (int ci = 0; ci < 3; ci++) q[ci] = sy[le2][ci][v];
+        for (int ci = 0; ci < 3; ci++) q[ci] = sy[le2][ci][lds[i]];
     }
 }
 
@@ -620,7 +703,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         const int k = tid + 256 * i, le = k / NXE, v = k - le * NXE;
-        const int slot = sk_slot(L, v % N1, (v / N1) % N1, v / (N1 * N1));
+        const int slot = sk_slot_table<L>.v[v];
         const bool interior = slot < 0;
         pos[i] = (gid[i] >= 0 && !interior) ? sk_pos[(e0 + le) * NSK + slot] : -1;
         double xv[3] = {0, 0, 0}, yv[3] = {0, 0, 0};
@@ -853,7 +936,8 @@ int sshex8_subblock_idx(ptrdiff_t E, int L, int Ls, const int32_t *idx, int32_t 
 
 template <typename T>
 int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *x, T *y, T *part,
-                            const SkeletonMap &sk, hipStream_t s, int overwrite, int matrix_cores, int kshift) {
+                            const SkeletonMap &sk, hipStream_t s, int overwrite, int matrix_cores, int kshift,
+                            const uint32_t *Apk) {
     if (E == 0) return SFEM_SUCCESS;
     if (kshift && !(matrix_cores && ((std::is_same<T, double>::value && (L == 2 || L == 4 || L == 6 || L == 8)) ||
                                      (std::is_same<T, float>::value && L % 2 == 0 && L <= 12)))) {
@@ -878,11 +962,16 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
     }
     if constexpr (std::is_same<T, float>::value) {
         if (matrix_cores && (L == 2 || L == 4 || L == 6 || L == 8 || L == 10 || L == 12)) {
+            if (!Apk) {
+                set_last_error("sshex8_elasticity_apply: the FP32 matrix-core kernel needs packed matrices");
+                return SFEM_FAILURE;
+            }
             switch (L) {
 #define MCASE(LV)                                                                                               \
     case LV:                                                                                                    \
         sshex8_elasticity_apply_mfma_kernel<LV><<<(unsigned)((E + ElMfma<LV>::EPB - 1) / ElMfma<LV>::EPB), 256, 0, \
-                                                  s>>>(E, idx, K, x, y, part, overwrite, kshift, sk.pos);       \
+                                                  s>>>(E, idx, (const el_u4 *)Apk, x, y, part, overwrite, kshift, \
+                                                       sk.pos);                                                 \
         break;
                 MCASE(2) MCASE(4) MCASE(6) MCASE(8) MCASE(10) MCASE(12)
 #undef MCASE
@@ -946,9 +1035,11 @@ template int sshex8_elasticity_matrices<float>(int, ptrdiff_t, const float *, co
 template int sshex8_elasticity_matrices<uint16_t>(int, ptrdiff_t, const uint16_t *, const uint16_t *, ptrdiff_t,
                                                   double, double, const float *, double *, hipStream_t);
 template int sshex8_elasticity_apply<double>(int, ptrdiff_t, const int32_t *, const double *, const double *,
-                                             double *, double *, const SkeletonMap &, hipStream_t, int, int, int);
+                                             double *, double *, const SkeletonMap &, hipStream_t, int, int, int,
+                                             const uint32_t *);
 template int sshex8_elasticity_apply<float>(int, ptrdiff_t, const int32_t *, const float *, const float *, float *,
-                                            float *, const SkeletonMap &, hipStream_t, int, int, int);
+                                            float *, const SkeletonMap &, hipStream_t, int, int, int,
+                                            const uint32_t *);
 template int sshex8_elasticity_diag<double>(int, ptrdiff_t, const int32_t *, const double *, int, double *,
                                             hipStream_t);
 template int sshex8_elasticity_diag<float>(int, ptrdiff_t, const int32_t *, const double *, int, float *,

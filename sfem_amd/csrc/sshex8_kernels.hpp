@@ -44,9 +44,13 @@ template <typename T>
 // summation order); FP32 (L <= 12) the bf16x3 MFMA kernel, ~1e-5 relative per product; otherwise
 // the node-wise vector kernel
 // kshift > 0 (matrix-core kernels only): elements e share the matrix K[e >> kshift] (sub-blocks)
+// Apk: the FP32 matrix-core kernel's operands, sshex8_elasticity_pack_bf16x2 of K (required there)
 int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *x, T *y, T *part,
                             const SkeletonMap &sk, hipStream_t s, int overwrite = 0, int matrix_cores = 0,
-                            int kshift = 0);
+                            int kshift = 0, const uint32_t *Apk = nullptr);
+// The bf16 hi/lo split of nK float 24 x 24 matrices in the FP32 matrix-core kernel's A-operand
+// layout: nK * 1024 words.
+int sshex8_elasticity_pack_bf16x2(ptrdiff_t nK, const float *K, uint32_t *Apk, hipStream_t s);
 // connectivity [E * (L/Ls)^3][nxe(Ls)] of the macro elements cut into (L/Ls)^3 sub-blocks of level
 // Ls (sub-element e * (L/Ls)^3 + (oz * s + oy) * s + ox): the level-L operator as a level-Ls one
 int sshex8_subblock_idx(ptrdiff_t E, int L, int Ls, const int32_t *idx, int32_t *out, hipStream_t s);
