@@ -152,33 +152,41 @@ def solve_times(S, mesh, fff, pts, n_macro, rtol=1e-10, learned_steps=2):
     return out
 
 
-def elasticity_solve_times(S, n_macro=32, L=8, rtol=1e-10, with_learned=False):
+def elasticity_solve_times(S, n_macro=32, L=8, rtol=1e-10, with_learned=False, coeff_seed=None):
     """sshex8 linear elasticity, level L on an n_macro^3 macro hex8 unit cube (3 (nL+1)^3 DOF:
-    51,066,171 at L=8, n=32), E = 1, nu = 0.3, clamped at z = 0, body force (0, 0, -1).  Time to
-    true relative residual rtol of FP64 MG-PCG (graph-replayed V-cycle, dense coarse solve) and of
-    the FP32/BF16 V-cycle in FP64 flexible CG (optionally with the learned smoother per component)."""
+    50,923,779 at L=8, n=32; 171,199,875 at L=16, n=24), nu = 0.3, clamped at z = 0, body force
+    (0, 0, -1); E = 1, or with coeff_seed a per-macro-element Young's modulus log-uniform in
+    [0.1, 10].  Mesh, refined points and adjugates are built on the device.  Time to true relative
+    residual rtol of FP64 MG-PCG (graph-replayed V-cycle, dense coarse solve) and of the FP32/BF16
+    V-cycle in FP64 flexible CG (optionally with the learned smoother per component)."""
     from sfem_amd import gmg, learned
     t0 = time.perf_counter()
     mel, mpts = S.box_mesh(n_macro + 1, n_macro + 1, n_macro + 1, element_type=S.HEX8)
-    el, pts = S.sshex8_refine(L, mel, mpts)
-    adj, det = S.adjugate_det(S.SSHEX8, L, el, pts)
-    N = pts.shape[1]
-    mesh = S.Mesh(S.SSHEX8, L, el, N)
-    bc = np.where(pts[2] < 1e-6)[0].astype(np.int32)
-    del el
+    mesh, pts_d = S.sshex8_refine_device(L, mel, mpts)
+    adj, det = S.mesh_adjugate_det(mesh, pts_d)
+    N = mesh.nnodes
+    bc = np.where(pts_d.numpy().reshape(3, N)[2] < 1e-6)[0].astype(np.int32)
+    del pts_d
+    coeff = None
+    if coeff_seed is not None:
+        coeff = (10.0 ** np.random.default_rng(coeff_seed).uniform(-1, 1, n_macro ** 3)).astype(np.float32)
     nu = 0.3
     mu, lam = 1.0 / (2 * (1 + nu)), nu / ((1 + nu) * (1 - 2 * nu))
     rhs = np.zeros((N, 3))
     rhs[:, 2] = -1
     rhs[bc] = 0
     rhs = rhs.ravel()
-    del pts
     b = S.DeviceArray.from_numpy(rhs)
     mesh_s = time.perf_counter() - t0
     box = (n_macro, 1.0)
     out = {"workload": f"sshex8 linear elasticity L={L} on a {n_macro}^3 macro cube, {3 * N} DOF, nu=0.3, "
-                       "clamped z=0, body force (0,0,-1)", "dofs": 3 * N, "mesh_seconds": mesh_s}
-    common = dict(mu=mu, lam=lam, macro_box=box, coarse="dense")
+                       + ("E log-uniform in [0.1,10] per macro element (seed %d), " % coeff_seed
+                          if coeff_seed is not None else "E=1, ")
+                       + "clamped z=0, body force (0,0,-1)", "dofs": 3 * N, "mesh_seconds": mesh_s}
+    # jumping coefficients: Chebyshev3 on D^-1 A (111 against 23 FP64 iterations at L=16, n=24)
+    smoother = "cheb3" if coeff is None else "cheb3_jacobi"
+    out["smoother"] = smoother
+    common = dict(mu=mu, lam=lam, macro_box=box, coarse="dense", coeff=coeff, smoother=smoother)
     progress(f"elasticity: {3 * N} DOF mesh ready in {mesh_s:.1f}s; building FP64 hierarchy")
     t0 = time.perf_counter()
     h64 = gmg.SSGMG(mesh, "elasticity", (adj, det), bc, **common)
@@ -232,6 +240,16 @@ def elasticity_solve_times(S, n_macro=32, L=8, rtol=1e-10, with_learned=False):
     return out
 
 
+def elasticity_cases(S, args):
+    out = {}
+    if args.elasticity_case in ("l8", "both"):
+        out["solve_elasticity"] = elasticity_solve_times(S, with_learned=args.elasticity_learned)
+    if args.elasticity_case in ("l16", "both"):
+        out["solve_elasticity_l16"] = elasticity_solve_times(S, n_macro=24, L=16, coeff_seed=0,
+                                                             with_learned=args.elasticity_learned)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,6 +262,9 @@ def main():
     ap.add_argument("--only-elasticity", action="store_true", help="run only the elasticity solve section")
     ap.add_argument("--elasticity-learned", action="store_true",
                     help="also time the learned smoother in the elasticity solve")
+    ap.add_argument("--elasticity-case", choices=["l8", "l16", "both"], default="both",
+                    help="l8: L=8 on 32^3 macro elements, E=1 (50.9M DOF); l16: L=16 on 24^3 macro elements, "
+                         "E log-uniform per macro element (171.2M DOF)")
     ap.add_argument("--mode", choices=["apply", "copy"], default="apply",
                     help="copy: stream-copy 2 GiB of FP64 per step (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -270,7 +291,7 @@ def main():
         print(json.dumps({"mode": "copy", "bytes_per_step": 16 * n, "GB/s": 16 * n / dt / 1e9}))
         return
     if args.only_elasticity:
-        print(json.dumps({"solve_elasticity": elasticity_solve_times(S, with_learned=args.elasticity_learned)}))
+        print(json.dumps(elasticity_cases(S, args)))
         return
     L, n_macro = 8, 16
     mel, mpts, el, pts, fff = build_problem(n_macro, L)
@@ -317,7 +338,7 @@ def main():
     if not args.no_solve:
         result["solve"] = solve_times(S, mesh, fff, pts, n_macro)
         if not args.no_elasticity:
-            result["solve_elasticity"] = elasticity_solve_times(S, with_learned=args.elasticity_learned)
+            result.update(elasticity_cases(S, args))
     if not args.no_cpu:
         progress("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(el, fff, N, L)
