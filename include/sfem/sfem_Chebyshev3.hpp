@@ -83,16 +83,14 @@ public:
         const T lmin = scale_eig_min * lmax;
         const T avg = (lmin + lmax) / 2, diff = (lmin - lmax) / 2;
         T alpha = T(1) / avg, beta = 0, dea = 0;
-        // iteration 0: p = A x - rhs (or -rhs when x == 0); x -= alpha p
-        if (is_initial_guess_zero) {
-            blas->zaxpby(n, T(-1), rhs, T(0), rhs, p);
-        } else {
+        // iteration 0: p = A x - rhs (or -rhs when x == 0), scaled by D^-1 when given; x -= alpha p
+        // (one fused pass after the product)
+        if (!is_initial_guess_zero) {
             if (!apply_overwrites_output) blas->zeros(n, p);
             op->apply(x, p);
-            blas->axpy(n, T(-1), rhs, p);
         }
-        if (inv_diag) blas->xypaz(n, inv_diag->data(), p, T(0), p);
-        blas->axpy(n, -alpha, p, x);
+        sfem_amd_cheb3_start(real_type_of<T>::value, n, (double)alpha, rhs, inv_diag ? inv_diag->data() : nullptr,
+                             is_initial_guess_zero ? 0 : 1, p, x, stream);
         iterations_ = 1;
         for (; iterations_ < max_it; iterations_++) {
             dea = diff * alpha;

@@ -80,11 +80,11 @@ public:
     }
 
 private:
-    T *r_{nullptr}, *z_{nullptr}, *p_{nullptr}, *Ap_{nullptr}, *zold_{nullptr};
+    T *r_{nullptr}, *z_{nullptr}, *p_{nullptr}, *Ap_{nullptr};
     std::ptrdiff_t wn_{-1};
 
     void release() {
-        for (T **q : {&r_, &z_, &p_, &Ap_, &zold_}) {
+        for (T **q : {&r_, &z_, &p_, &Ap_}) {
             if (*q && blas) blas->destroy(*q);
             *q = nullptr;
         }
@@ -97,7 +97,6 @@ private:
         z_ = blas->allocate(n);
         p_ = blas->allocate(n);
         Ap_ = blas->allocate(n);
-        zold_ = blas->allocate(n);
         wn_ = n;
     }
     void zero_for_apply(std::ptrdiff_t n, T *y) {
@@ -105,6 +104,13 @@ private:
     }
     T update_x_r(std::ptrdiff_t n, T alpha, const T *p, const T *Ap, T *x, T *r, bool with_rtr) {
         return (T)sfem_amd_cg_update_x_r(real_type_of<T>::value, n, alpha, p, Ap, x, r, with_rtr ? 1 : 0, stream);
+    }
+    // returns (r, r) of the updated r and its (r, z) in rz
+    T update_x_r_dot(std::ptrdiff_t n, T alpha, const T *p, const T *Ap, T *x, T *r, const T *z, T &rz) {
+        double d = 0;
+        const T rtr = (T)sfem_amd_cg_update_x_r_dot(real_type_of<T>::value, n, alpha, p, Ap, x, r, z, &d, stream);
+        rz = (T)d;
+        return rtr;
     }
     // z = P r from z = 0 whatever the operator does; a solver or multigrid preconditioner is told
     // that its initial guess is zero for this apply (it may skip products with it)
@@ -201,7 +207,7 @@ private:
     // Flexible PCG with true-residual stopping (extension)
     int flexible_pcg(std::ptrdiff_t n, const T *b, T *x) {
         ensure_workspace(n);
-        T *r = r_, *z = z_, *p = p_, *Ap = Ap_, *zold = zold_;
+        T *r = r_, *z = z_, *p = p_, *Ap = Ap_;
         const T bnorm = blas->norm2(n, b);
         residual(n, b, x, r);
         T rnorm = blas->norm2(n, r);
@@ -217,10 +223,13 @@ private:
             const T ptAp = blas->dot(n, p, Ap);
             const T alpha = rtz / ptAp;
             const bool replace = residual_replacement > 0 && (iterations_ + 1) % residual_replacement == 0;
-            T rtr = update_x_r(n, alpha, p, Ap, x, r, !replace);
+            // r_new . z_old for beta, taken while z still holds z_old
+            T rtzold = 0;
+            T rtr = replace ? update_x_r(n, alpha, p, Ap, x, r, false) : update_x_r_dot(n, alpha, p, Ap, x, r, z, rtzold);
             if (replace) {
                 residual(n, b, x, r);
                 rtr = blas->dot(n, r, r);
+                rtzold = blas->dot(n, r, z);
             }
             const T rprev = rnorm;
             rnorm = std::sqrt(rtr);
@@ -237,11 +246,10 @@ private:
                     return SFEM_SUCCESS;
                 }
                 rnorm = true_norm;
+                rtzold = blas->dot(n, r, z);  // r was replaced by the true residual
             }
-            blas->copy(n, z, zold);
             precondition(n, r, z);
             const T rtz_new = blas->dot(n, r, z);
-            const T rtzold = blas->dot(n, r, zold);
             const T beta = (rtz_new - rtzold) / rtz;
             rtz = rtz_new;
             blas->axpby(n, T(1), z, beta, p);

@@ -215,12 +215,20 @@ int sfem_amd_blas_cast(int rt_from, ptrdiff_t n, const void *x, int rt_to, void 
  * when with_rtr != 0 (synchronising), else 0. */
 double sfem_amd_cg_update_x_r(int rt, ptrdiff_t n, double alpha, const void *p, const void *Ap, void *x, void *r,
                               int with_rtr, void *stream);
+/* The same update, also returning (r, z) of the updated r in *rz (synchronising): the r_new . z_old
+ * term of the flexible PCG's beta, read before the preconditioner overwrites z (no copy of z). */
+double sfem_amd_cg_update_x_r_dot(int rt, ptrdiff_t n, double alpha, const void *p, const void *Ap, void *x, void *r,
+                                  const void *z, double *rz, void *stream);
 /* Chebyshev3 iteration_update (algebra/sfem_Chebyshev3.hpp:37-52): p = beta p - rhs + Ax; x -= alpha p */
 int sfem_amd_cheb3_update(int rt, ptrdiff_t n, double alpha, double beta, const void *rhs, const void *Ax, void *p,
                           void *x, void *stream);
 /* p = beta p + inv_diag (Ax - rhs); x -= alpha p (the Jacobi-scaled Chebyshev3 step). */
 int sfem_amd_cheb3_update_diag(int rt, ptrdiff_t n, double alpha, double beta, const void *rhs, const void *Ax,
                                const void *inv_diag, void *p, void *x, void *stream);
+/* Chebyshev3 iteration 0 (algebra/sfem_Chebyshev3.hpp:208-217) in one pass: p = p - rhs when p holds
+ * A x (have_ax), else p = -rhs (zero initial guess); p *= inv_diag when inv_diag != NULL; x -= alpha p. */
+int sfem_amd_cheb3_start(int rt, ptrdiff_t n, double alpha, const void *rhs, const void *inv_diag, int have_ax,
+                         void *p, void *x, void *stream);
 
 /* ------------------------------------------------------------------ solvers as operators */
 /* sfem::ConjugateGradient<real_t> (algebra/sfem_cg.hpp:80-446): plain CG, or PCG when

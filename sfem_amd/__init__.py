@@ -73,8 +73,10 @@ SIGNATURES = {
     "sfem_amd_blas_norm2": (_d, [_i, _pd, _vp, _vp]),
     "sfem_amd_blas_cast": (_i, [_i, _pd, _vp, _i, _vp, _vp]),
     "sfem_amd_cg_update_x_r": (_d, [_i, _pd, _d, _vp, _vp, _vp, _vp, _i, _vp]),
+    "sfem_amd_cg_update_x_r_dot": (_d, [_i, _pd, _d, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double), _vp]),
     "sfem_amd_cheb3_update": (_i, [_i, _pd, _d, _d, _vp, _vp, _vp, _vp, _vp]),
     "sfem_amd_cheb3_update_diag": (_i, [_i, _pd, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sfem_amd_cheb3_start": (_i, [_i, _pd, _d, _vp, _vp, _i, _vp, _vp, _vp]),
     "sfem_amd_cheb3_diag_create": (_i, [_vp, _vp, _i, _d, _d, _i, _d, _i, _vp, C.POINTER(_vp)]),
     "sfem_amd_cg_create": (_i, [_vp, _vp, _i, _d, _d, _i, C.POINTER(_vp)]),
     "sfem_amd_cheb3_create": (_i, [_vp, _i, _d, _d, _i, _d, _i, _vp, C.POINTER(_vp)]),

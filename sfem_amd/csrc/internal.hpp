@@ -120,6 +120,8 @@ struct sfem_amd_mesh {
     const sfem_amd_mesh *parent{nullptr};        // set by sfem_amd_mesh_derefine
     // sshex8: per-node skeleton slot lists of the mesh viewed at level Lc (lazy, mesh_skeleton)
     mutable std::map<int, std::unique_ptr<sfem_amd::SkeletonStore>> skel;
+    // sshex8: the macro elements cut into sub-blocks of level Ls (lazy, mesh_subblocks)
+    mutable std::map<int, std::unique_ptr<sfem_amd_mesh>> sub;
 };
 
 namespace sfem_amd {
@@ -129,5 +131,8 @@ int mesh_counts(const sfem_amd_mesh *m, hipStream_t s);
 bool is_constrained(const OpBase *op);
 // Builds (once) the skeleton slot lists of an sshex8 mesh from its connectivity.
 int mesh_skeleton(const sfem_amd_mesh *m, int Lc);
+// The sshex8 mesh's macro elements as (L/Ls)^3 sub-blocks of level Ls (sshex8_subblock_idx), built
+// once per level and shared; null on failure.
+const sfem_amd_mesh *mesh_subblocks(const sfem_amd_mesh *m, int Ls, hipStream_t s);
 sfem_amd_op *wrap_op(std::shared_ptr<OpBase> o);
 }  // namespace sfem_amd

@@ -150,6 +150,29 @@ static int mesh_owner(const sfem_amd_mesh *m, hipStream_t s) {
     if (m->own.alloc((size_t)m->E * m->nv)) return SFEM_FAILURE;
     return node_owner_mask(m->E, m->nv, m->nnodes, m->idx.p, m->own.p, s);
 }
+const sfem_amd_mesh *mesh_subblocks(const sfem_amd_mesh *m, int Ls, hipStream_t s) {
+    auto &slot = m->sub[Ls];
+    if (slot) return slot.get();
+    if (m->type != SFEM_AMD_SSHEX8 || Ls <= 0 || m->level % Ls) {
+        set_last_error("sshex8 sub-blocks: the sub-level must divide the mesh level");
+        return nullptr;
+    }
+    const int f = m->level / Ls;
+    auto sm = std::make_unique<sfem_amd_mesh>();
+    sm->type = SFEM_AMD_SSHEX8;
+    sm->level = Ls;
+    sm->E = m->E * f * f * f;
+    sm->nnodes = m->nnodes;
+    sm->nv = nxe_of(Ls);
+    if (sm->idx.alloc((size_t)sm->E * sm->nv) || sshex8_subblock_idx(m->E, m->level, Ls, m->idx.p, sm->idx.p, s))
+        return nullptr;
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        set_last_error("sshex8 sub-blocks: stream synchronisation failed");
+        return nullptr;
+    }
+    slot = std::move(sm);
+    return slot.get();
+}
 
 // ----------------------------------------------------------------- Laplacian
 template <typename T>
@@ -223,7 +246,6 @@ struct ElasticityOp final : OpBase {
     DevArray<T> part;                 // sshex8: skeleton contributions [E][sk_count(L)][3]
     // levels beyond the matrix-core kernels' LDS tile (L = 2 Ls): the operator applied as 8 sub-blocks
     // of level Ls per macro element (sharing its matrix), with their own skeleton slots
-    std::unique_ptr<sfem_amd_mesh> sub;
     double mu{0}, lambda{0};
     int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
     ptrdiff_t rows() const override { return m->nnodes * 3; }
@@ -243,24 +265,7 @@ struct ElasticityOp final : OpBase {
     // the mesh the kernels run on (built on first use)
     const sfem_amd_mesh *work_mesh(hipStream_t s) {
         const int Ls = subblock_level();
-        if (!Ls) return m;
-        if (!sub) {
-            auto sm = std::make_unique<sfem_amd_mesh>();
-            sm->type = SFEM_AMD_SSHEX8;
-            sm->level = Ls;
-            sm->E = m->E * 8;
-            sm->nnodes = m->nnodes;
-            sm->nv = nxe_of(Ls);
-            if (sm->idx.alloc((size_t)sm->E * sm->nv) ||
-                sshex8_subblock_idx(m->E, m->level, Ls, m->idx.p, sm->idx.p, s))
-                return nullptr;
-            if (hipStreamSynchronize(s) != hipSuccess) {
-                set_last_error("elasticity sub-blocks: stream synchronisation failed");
-                return nullptr;
-            }
-            sub = std::move(sm);
-        }
-        return sub.get();
+        return Ls ? mesh_subblocks(m, Ls, s) : m;
     }
     int apply(const void *xv, void *yv, hipStream_t s) override { return apply_impl(xv, yv, s, 0); }
     int apply_overwrite(const void *xv, void *yv, hipStream_t s) override {
@@ -379,13 +384,23 @@ struct TransferOp final : OpBase {
                 err = sshex8_prolongate<T>(fine->E, fine->level, Lc, fine->idx.p, fine->own.p, vs, (const T *)x,
                                            (T *)y, s);
         } else {
-            if (mesh_counts(fine, s)) return SFEM_FAILURE;
-            if (staged()) {
-                if (mesh_skeleton(fine, Lc)) return SFEM_FAILURE;
-                if (!part.p && part.alloc((size_t)fine->E * sk_count(Lc) * vs)) return SFEM_FAILURE;
-                const SkeletonMap sk = skeleton_of(fine, Lc);
-                err = sshex8_restrict_lds<T>(fine->E, fine->level, Lc, fine->idx.p, fine->count.p, vs, (const T *)x,
-                                             (T *)y, part.p, sk.pos, s);
+            // a fine element tile beyond the LDS budget: restrict its 2^3 sub-blocks of level L/2 onto
+            // level Lc/2 instead (the same operator: the sub-blocks align with coarse micro elements)
+            const sfem_amd_mesh *rf = fine;
+            int rlc = Lc;
+            if (!staged() && (vs == 1 || vs == 3) && fine->level % 2 == 0 && Lc % 2 == 0 &&
+                sshex8_transfer_lds_bytes(fine->level / 2, vs) <= 48 * 1024) {
+                rf = mesh_subblocks(fine, fine->level / 2, s);
+                if (!rf) return SFEM_FAILURE;
+                rlc = Lc / 2;
+            }
+            if (mesh_counts(rf, s)) return SFEM_FAILURE;
+            if (rf != fine || staged()) {
+                if (mesh_skeleton(rf, rlc)) return SFEM_FAILURE;
+                if (!part.p && part.alloc((size_t)rf->E * sk_count(rlc) * vs)) return SFEM_FAILURE;
+                const SkeletonMap sk = skeleton_of(rf, rlc);
+                err = sshex8_restrict_lds<T>(rf->E, rf->level, rlc, rf->idx.p, rf->count.p, vs, (const T *)x, (T *)y,
+                                             part.p, sk.pos, s);
                 if (!err) err = skeleton_gather<T>(sk, vs, part.p, (T *)y, s, 0);
             } else {
                 err = sshex8_restrict<T>(fine->E, fine->level, Lc, fine->idx.p, fine->count.p, vs, (const T *)x,
@@ -1171,6 +1186,21 @@ double sfem_amd_cg_update_x_r(int rt, ptrdiff_t n, double a, const void *p, cons
                                          with_rtr ? &rtr : nullptr, as_stream(s));
     return err ? NAN : rtr;
 }
+double sfem_amd_cg_update_x_r_dot(int rt, ptrdiff_t n, double a, const void *p, const void *Ap, void *x, void *r,
+                                  const void *z, double *rz, void *s) {
+    double *ws = reduction_ws();
+    double rtr = 0;
+    if (!ws || !z || !rz) {
+        set_last_error("cg_update_x_r_dot: arguments");
+        return NAN;
+    }
+    int err = rt == SFEM_AMD_FLOAT64
+                      ? cg_update<double>(n, a, (const double *)p, (const double *)Ap, (double *)x, (double *)r, ws,
+                                          &rtr, as_stream(s), (const double *)z, rz)
+                      : cg_update<float>(n, (float)a, (const float *)p, (const float *)Ap, (float *)x, (float *)r, ws,
+                                         &rtr, as_stream(s), (const float *)z, rz);
+    return err ? NAN : rtr;
+}
 int sfem_amd_cheb3_update(int rt, ptrdiff_t n, double alpha, double beta, const void *rhs, const void *Ax, void *p,
                           void *x, void *s) {
     DISPATCH_RT(rt,
@@ -1187,6 +1217,15 @@ int sfem_amd_cheb3_update_diag(int rt, ptrdiff_t n, double alpha, double beta, c
                                          (const double *)inv_diag, (double *)p, (double *)x, as_stream(s));
     return cheb3_update_diag<float>(n, (float)alpha, (float)beta, (const float *)rhs, (const float *)Ax,
                                     (const float *)inv_diag, (float *)p, (float *)x, as_stream(s));
+}
+
+int sfem_amd_cheb3_start(int rt, ptrdiff_t n, double alpha, const void *rhs, const void *inv_diag, int have_ax,
+                         void *p, void *x, void *s) {
+    if (rt == SFEM_AMD_FLOAT64)
+        return cheb3_start<double>(n, alpha, (const double *)rhs, (const double *)inv_diag, have_ax, (double *)p,
+                                   (double *)x, as_stream(s));
+    return cheb3_start<float>(n, (float)alpha, (const float *)rhs, (const float *)inv_diag, have_ax, (float *)p,
+                              (float *)x, as_stream(s));
 }
 
 // ----------------------------------------------------------------- solvers

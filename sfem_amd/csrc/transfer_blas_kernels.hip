@@ -419,6 +419,19 @@ __global__ void cheb3_update_kernel(const ptrdiff_t n, const T alpha, const T be
 }
 
 // diagonally scaled variant: p = beta p + dinv (A x - rhs); x -= alpha p
+// Chebyshev3 iteration 0: p = (p holds A x ? p - rhs : -rhs), times dinv when given; x -= alpha p
+template <typename T>
+__global__ void cheb3_start_kernel(const ptrdiff_t n, const T alpha, const T *__restrict__ rhs,
+                                   const T *__restrict__ dinv, const int have_ax, T *__restrict__ p,
+                                   T *__restrict__ x) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
+        T pi = have_ax ? p[i] - rhs[i] : -rhs[i];
+        if (dinv) pi = dinv[i] * pi;
+        p[i] = pi;
+        x[i] -= alpha * pi;
+    }
+}
+
 template <typename T>
 __global__ void cheb3_update_diag_kernel(const ptrdiff_t n, const T alpha, const T beta, const T *__restrict__ rhs,
                                          const T *__restrict__ Ax, const T *__restrict__ dinv, T *__restrict__ p,
@@ -533,22 +546,41 @@ __global__ void __launch_bounds__(kBlock) sum_partials_kernel(const int nb, cons
     if (threadIdx.x == 0) out[0] = r;
 }
 
-// CG fused update (algebra/sfem_cg.hpp:26-72 host fusion): x += a p; r -= a Ap; partial (r,r)
+// CG fused update (algebra/sfem_cg.hpp:26-72 host fusion): x += a p; r -= a Ap; partial (r,r) and,
+// with z, partial (r,z) of the updated r at part[kRedBlocks + block]
 template <typename T>
 __global__ void __launch_bounds__(kBlock) cg_update_kernel(const ptrdiff_t n, const T a, const T *__restrict__ p,
                                                            const T *__restrict__ Ap, T *__restrict__ x,
-                                                           T *__restrict__ r, double *__restrict__ part) {
-    double acc = 0;
+                                                           T *__restrict__ r, const T *__restrict__ z,
+                                                           double *__restrict__ part) {
+    double acc = 0, acz = 0;
     for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
         x[i] += a * p[i];
         const T ri = r[i] - a * Ap[i];
         r[i] = ri;
         acc += (double)ri * (double)ri;
+        if (z) acz += (double)ri * (double)z[i];
     }
     if (part) {
         const double s = block_sum(acc);
         if (threadIdx.x == 0) part[blockIdx.x] = s;
+        if (z) {
+            __syncthreads();  // block_sum's shared scratch is reused
+            const double t = block_sum(acz);
+            if (threadIdx.x == 0) part[kRedBlocks + blockIdx.x] = t;
+        }
     }
+}
+
+// Pinned host words the final reductions write into directly (no copy kernel): host-synchronising
+// reductions only, one at a time per process.
+static double *pinned_results() {
+    static double *h = nullptr;
+    if (!h && hipHostMalloc((void **)&h, 8 * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        h = nullptr;
+    }
+    return h;
 }
 
 template <typename T>
@@ -628,6 +660,13 @@ int cheb3_update(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, T *p, 
 }
 
 template <typename T>
+int cheb3_start(ptrdiff_t n, T alpha, const T *rhs, const T *dinv, int have_ax, T *p, T *x, hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    cheb3_start_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, alpha, rhs, dinv, have_ax, p, x);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+template <typename T>
 int cheb3_update_diag(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, const T *dinv, T *p, T *x,
                       hipStream_t s) {
     cheb3_update_diag_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, alpha, beta, rhs, Ax, dinv, p, x);
@@ -658,27 +697,38 @@ int dense_matvec(ptrdiff_t n, const double *M, const T *x, T *y, hipStream_t s) 
     return SFEM_SUCCESS;
 }
 
-template <typename T>
-int blas_dot(ptrdiff_t n, const T *x, const T *y, double *ws, double *result_host, hipStream_t s) {
-    const int nb = red_blocks(n);
-    dot_partial_kernel<T><<<nb, kBlock, 0, s>>>(n, x, y, ws);
-    sum_partials_kernel<<<1, kBlock, 0, s>>>(nb, ws, ws + kRedBlocks);
+// the k sums of partials part[j * kRedBlocks + block] -> host values out[j]
+static int finish_sums(int nb, int k, double *ws, double *out, hipStream_t s) {
+    double *h = pinned_results();
+    for (int j = 0; j < k; j++)
+        sum_partials_kernel<<<1, kBlock, 0, s>>>(nb, ws + j * kRedBlocks, h ? h + j : ws + 2 * kRedBlocks + j);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
-    SFEM_AMD_HIP_CHECK(hipMemcpyAsync(result_host, ws + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
+    if (!h)
+        SFEM_AMD_HIP_CHECK(hipMemcpyAsync(out, ws + 2 * kRedBlocks, k * sizeof(double), hipMemcpyDeviceToHost, s));
     SFEM_AMD_HIP_CHECK(hipStreamSynchronize(s));
+    if (h)
+        for (int j = 0; j < k; j++) out[j] = h[j];
     return SFEM_SUCCESS;
 }
 
 template <typename T>
-int cg_update(ptrdiff_t n, T a, const T *p, const T *Ap, T *x, T *r, double *ws, double *rtr_host, hipStream_t s) {
+int blas_dot(ptrdiff_t n, const T *x, const T *y, double *ws, double *result_host, hipStream_t s) {
     const int nb = red_blocks(n);
-    cg_update_kernel<T><<<nb, kBlock, 0, s>>>(n, a, p, Ap, x, r, rtr_host ? ws : nullptr);
+    dot_partial_kernel<T><<<nb, kBlock, 0, s>>>(n, x, y, ws);
+    return finish_sums(nb, 1, ws, result_host, s);
+}
+
+template <typename T>
+int cg_update(ptrdiff_t n, T a, const T *p, const T *Ap, T *x, T *r, double *ws, double *rtr_host, hipStream_t s,
+              const T *z, double *rz_host) {
+    const int nb = red_blocks(n);
+    cg_update_kernel<T><<<nb, kBlock, 0, s>>>(n, a, p, Ap, x, r, rtr_host ? z : nullptr, rtr_host ? ws : nullptr);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
-    if (rtr_host) {
-        sum_partials_kernel<<<1, kBlock, 0, s>>>(nb, ws, ws + kRedBlocks);
-        SFEM_AMD_HIP_CHECK(hipMemcpyAsync(rtr_host, ws + kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
-        SFEM_AMD_HIP_CHECK(hipStreamSynchronize(s));
-    }
+    if (!rtr_host) return SFEM_SUCCESS;
+    double out[2] = {0, 0};
+    if (finish_sums(nb, z ? 2 : 1, ws, out, s)) return SFEM_FAILURE;
+    *rtr_host = out[0];
+    if (z && rz_host) *rz_host = out[1];
     return SFEM_SUCCESS;
 }
 
@@ -721,7 +771,7 @@ int permute_add(ptrdiff_t n, int bs, const int32_t *perm, int transpose, const T
 template int permute_add<double>(ptrdiff_t, int, const int32_t *, int, const double *, double *, hipStream_t);
 template int permute_add<float>(ptrdiff_t, int, const int32_t *, int, const float *, float *, hipStream_t);
 
-size_t reduction_workspace_doubles() { return kRedBlocks + 2; }
+size_t reduction_workspace_doubles() { return 2 * kRedBlocks + 4; }
 
 int cast_accumulate_f32(ptrdiff_t n, const double *x, float *y, hipStream_t s) {
     if (n <= 0) return SFEM_SUCCESS;
@@ -747,11 +797,13 @@ int cast_accumulate(ptrdiff_t n, const float *x, double *y, hipStream_t s) {
     template int blas_xypaz<T>(ptrdiff_t, const T *, const T *, T, T *, hipStream_t);                   \
     template int cheb3_update<T>(ptrdiff_t, T, T, const T *, const T *, T *, T *, hipStream_t);         \
     template int cheb3_update_diag<T>(ptrdiff_t, T, T, const T *, const T *, const T *, T *, T *, hipStream_t); \
+    template int cheb3_start<T>(ptrdiff_t, T, const T *, const T *, int, T *, T *, hipStream_t);          \
     template int dense_matvec<T>(ptrdiff_t, const double *, const T *, T *, hipStream_t);                \
     template int block_sym_inverse<T>(ptrdiff_t, const T *, const uint8_t *, double, T *, hipStream_t);   \
     template int block3_apply<T>(ptrdiff_t, const T *, const T *, T *, hipStream_t);                      \
     template int blas_dot<T>(ptrdiff_t, const T *, const T *, double *, double *, hipStream_t);         \
-    template int cg_update<T>(ptrdiff_t, T, const T *, const T *, T *, T *, double *, double *, hipStream_t); \
+    template int cg_update<T>(ptrdiff_t, T, const T *, const T *, T *, T *, double *, double *, hipStream_t,  \
+                              const T *, double *);                                                       \
     template int dofs_set_value<T>(ptrdiff_t, const int32_t *, T, T *, hipStream_t);                    \
     template int dofs_copy<T>(ptrdiff_t, const int32_t *, const T *, T *, hipStream_t);
 INST(double)

@@ -47,10 +47,15 @@ template <typename T> int dense_matvec(ptrdiff_t n, const double *M, const T *x,
 template <typename T>
 int cheb3_update_diag(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, const T *dinv, T *p, T *x,
                       hipStream_t s);
+// Chebyshev3 iteration 0: p = (have_ax ? p - rhs : -rhs) [* dinv]; x -= alpha p
+template <typename T>
+int cheb3_start(ptrdiff_t n, T alpha, const T *rhs, const T *dinv, int have_ax, T *p, T *x, hipStream_t s);
 // ws: device workspace of reduction_workspace_doubles() doubles
 template <typename T> int blas_dot(ptrdiff_t n, const T *x, const T *y, double *ws, double *result_host, hipStream_t s);
 template <typename T>
-int cg_update(ptrdiff_t n, T a, const T *p, const T *Ap, T *x, T *r, double *ws, double *rtr_host, hipStream_t s);
+// with z (and rtr_host): also (r, z) of the updated r into *rz_host
+int cg_update(ptrdiff_t n, T a, const T *p, const T *Ap, T *x, T *r, double *ws, double *rtr_host, hipStream_t s,
+              const T *z = nullptr, double *rz_host = nullptr);
 template <typename T> int dofs_set_value(ptrdiff_t n, const int32_t *dofs, T v, T *x, hipStream_t s);
 template <typename T> int dofs_copy(ptrdiff_t n, const int32_t *dofs, const T *src, T *dst, hipStream_t s);
 size_t reduction_workspace_doubles();

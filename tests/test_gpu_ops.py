@@ -286,6 +286,38 @@ def test_transfers_parity(vs):
     del count
 
 
+@pytest.mark.parametrize("vs", [1, 3])
+def test_transfers_level16_parity(vs):
+    """Level 16 -> 8: the restriction's element tile exceeds the LDS budget, so it runs on the 2^3
+    sub-blocks of level 8 (onto level 4); the prolongation stages the coarse level-8 tile."""
+    L, Lc = 16, 8
+    mel, mpts = skewed_box(2, perturb=0.02)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    E, nf = mel.shape[1], pts.shape[1]
+    fine = S.Mesh(S.SSHEX8, L, el, nf)
+    coarse = fine.derefine(Lc)
+    ec = O.derefine(L, Lc, el)
+    np.testing.assert_array_equal(coarse.download(), ec)
+    nc = coarse.nnodes
+    rng = np.random.default_rng(11)
+    xc = rng.uniform(-1, 1, nc * vs)
+    yo = np.zeros(nf * vs)
+    O.lib.oracle_sshex8_prolongate(E, Lc, ec, L, el, vs, xc, yo)
+    xf = rng.uniform(-1, 1, nf * vs)
+    ro = np.zeros(nc * vs)
+    O.lib.oracle_sshex8_restrict(E, L, el, O.incidence(el, nf), Lc, ec, vs, xf, ro)
+    for rt, dt, tol in ((S.FLOAT64, np.float64, RTOL), (S.FLOAT32, np.float32, 1e-6)):
+        P = S.prolongation(coarse, fine, vs, real_type=rt)
+        y = zeros(nf * vs, dt)
+        P.apply(dev(xc, dt), y)
+        R = S.restriction(fine, coarse, vs, real_type=rt)
+        r = zeros(nc * vs, dt)
+        R.apply(dev(xf, dt), r)
+        S.synchronize()
+        assert relerr(host(y).astype(np.float64), yo) < tol
+        assert relerr(host(r).astype(np.float64), ro) < tol
+
+
 def test_constrained_operator_copies_dofs():
     L = 4
     mel, mpts = skewed_box(2)
