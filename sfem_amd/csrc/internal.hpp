@@ -86,6 +86,16 @@ struct OpBase {
         set_last_error("block diagonal not available for this operator");
         return SFEM_FAILURE;
     }
+    // sshex8 operators: the mesh whose skeleton gather writes their shared nodes (Dirichlet rows on
+    // it can be set there, see apply_constrained); null otherwise
+    virtual const sfem_amd_mesh *gather_mesh(hipStream_t) { return nullptr; }
+    virtual int block_size() const { return 1; }
+    // y (+)= A x with the rows flagged in cmask (per entry of gather_mesh's skeleton, bit per
+    // component) set to x instead; overwrite: y is output only
+    virtual int apply_constrained(const void *, void *, const uint8_t *, int, hipStream_t) {
+        set_last_error("constrained gather not available for this operator");
+        return SFEM_FAILURE;
+    }
     // Solvers and smoothers: the output holds zeros on entry to the next applies (work that
     // multiplies the initial guess may be skipped).  Other operators ignore it.
     virtual void set_initial_guess_zero(bool) {}

@@ -2,6 +2,7 @@
 // handles over the sfem:: templates (include/sfem/).  See include/sfem_amd.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -193,12 +194,23 @@ struct LaplacianOp final : OpBase {
         if (!skeleton_covers(m)) return OpBase::apply_overwrite(xv, yv, s);
         return apply_impl(xv, yv, s, 1);
     }
-    int apply_impl(const void *xv, void *yv, hipStream_t s, int ow) {
+    const sfem_amd_mesh *gather_mesh(hipStream_t) override { return m->type == SFEM_AMD_SSHEX8 ? m : nullptr; }
+    int apply_constrained(const void *xv, void *yv, const uint8_t *cmask, int ow, hipStream_t s) override {
+        if (m->type != SFEM_AMD_SSHEX8 || mesh_skeleton(m, 0)) return SFEM_FAILURE;
+        if (ow && !skeleton_covers(m)) {
+            if (fill_zero(yv, (size_t)rows() * sizeof(T), s)) return SFEM_FAILURE;
+            ow = 0;
+        }
+        return apply_impl(xv, yv, s, ow, cmask);
+    }
+    int apply_impl(const void *xv, void *yv, hipStream_t s, int ow, const uint8_t *cmask = nullptr) {
         const T *x = (const T *)xv;
         T *y = (T *)yv;
         if (m->type == SFEM_AMD_SSHEX8 && !part.p && m->E > 0) {
             if (mesh_skeleton(m, 0) || part.alloc((size_t)m->E * sk_count(m->level))) return SFEM_FAILURE;
         }
+        SkeletonMap sk = skeleton_of(m);
+        sk.cmask = cmask;
         if constexpr (std::is_same<T, double>::value) {
             switch (m->type) {
                 case SFEM_AMD_TET4:
@@ -207,11 +219,11 @@ struct LaplacianOp final : OpBase {
                     return hex8_laplacian_apply(m->E, m->idx.p, fff32.p, coeff.p, x, y, s);
                 default:
                     return sshex8_laplacian_apply<double, float>(m->level, m->E, m->idx.p, fff32.p, m->E, coeff.p,
-                                                                 Kem.p, x, y, part.p, skeleton_of(m), s, ow);
+                                                                 Kem.p, x, y, part.p, sk, s, ow);
             }
         } else {
             return sshex8_laplacian_apply<float, uint16_t>(m->level, m->E, m->idx.p, fff16.p, m->E, nullptr, Kem.p,
-                                                           x, y, part.p, skeleton_of(m), s, ow);
+                                                           x, y, part.p, sk, s, ow);
         }
     }
     int diag(void *dv, hipStream_t s) override {
@@ -275,7 +287,20 @@ struct ElasticityOp final : OpBase {
         if (!skeleton_covers(wm)) return OpBase::apply_overwrite(xv, yv, s);
         return apply_impl(xv, yv, s, 1);
     }
-    int apply_impl(const void *xv, void *yv, hipStream_t s, int ow) {
+    int block_size() const override { return 3; }
+    const sfem_amd_mesh *gather_mesh(hipStream_t s) override {
+        return m->type == SFEM_AMD_SSHEX8 ? work_mesh(s) : nullptr;
+    }
+    int apply_constrained(const void *xv, void *yv, const uint8_t *cmask, int ow, hipStream_t s) override {
+        const sfem_amd_mesh *wm = m->type == SFEM_AMD_SSHEX8 ? work_mesh(s) : nullptr;
+        if (!wm || mesh_skeleton(wm, 0)) return SFEM_FAILURE;
+        if (ow && !skeleton_covers(wm)) {
+            if (fill_zero(yv, (size_t)rows() * sizeof(T), s)) return SFEM_FAILURE;
+            ow = 0;
+        }
+        return apply_impl(xv, yv, s, ow, cmask);
+    }
+    int apply_impl(const void *xv, void *yv, hipStream_t s, int ow, const uint8_t *cmask = nullptr) {
         if (m->type == SFEM_AMD_HEX8) {
             if constexpr (std::is_same<T, double>::value)
                 return hex8_elasticity_apply(m->E, m->idx.p, adj.p, det.p, mu, lambda, coeff.p, (const double *)xv,
@@ -302,9 +327,10 @@ struct ElasticityOp final : OpBase {
                     return SFEM_FAILURE;
             }
         }
-        return sshex8_elasticity_apply<T>(wm->level, wm->E, wm->idx.p, Kt, (const T *)xv, (T *)yv, part.p,
-                                          skeleton_of(wm), s, ow, uses_matrix_cores() ? 1 : 0, wm == m ? 0 : 3,
-                                          Apk.p);
+        SkeletonMap sk = skeleton_of(wm);
+        sk.cmask = cmask;
+        return sshex8_elasticity_apply<T>(wm->level, wm->E, wm->idx.p, Kt, (const T *)xv, (T *)yv, part.p, sk, s, ow,
+                                          uses_matrix_cores() ? 1 : 0, wm == m ? 0 : 3, Apk.p);
     }
     int diag(void *d, hipStream_t s) override {
         if (m->type == SFEM_AMD_HEX8) {
@@ -328,11 +354,45 @@ struct ElasticityOp final : OpBase {
 struct ConstrainedOp final : OpBase {
     std::shared_ptr<OpBase> inner;
     DevArray<int32_t> dofs;
+    // the constrained dofs as flags on the inner sshex8 operator's skeleton entries, when every
+    // one is a skeleton (macro-element boundary) dof: its gather then writes y = x there
+    DevArray<uint8_t> cmask;
+    int cmask_state{0};  // 0: not yet built, 1: in use, -1: not applicable (separate copy)
+    bool fused(hipStream_t s) {
+        if (cmask_state == 0) {
+            cmask_state = -1;
+            if (!build_cmask(s)) cmask_state = 1;
+            (void)hipGetLastError();
+        }
+        return cmask_state > 0;
+    }
+    int build_cmask(hipStream_t s) {
+        const sfem_amd_mesh *gm = inner->gather_mesh(s);
+        if (!gm || mesh_skeleton(gm, 0)) return SFEM_FAILURE;
+        const SkeletonMap sk = skeleton_of(gm);
+        const int bs = inner->block_size();
+        if (bs > 8 || sk.n == 0) return SFEM_FAILURE;
+        std::vector<int32_t> node((size_t)sk.n), d(dofs.n);
+        if (hipStreamSynchronize(s) != hipSuccess ||
+            hipMemcpy(node.data(), sk.node, sizeof(int32_t) * node.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+            (!d.empty() && hipMemcpy(d.data(), dofs.p, sizeof(int32_t) * d.size(), hipMemcpyDeviceToHost) != hipSuccess))
+            return SFEM_FAILURE;
+        std::vector<uint8_t> mask((size_t)sk.n, 0);
+        for (const int32_t q : d) {
+            const int32_t g = q / bs;
+            const auto it = std::lower_bound(node.begin(), node.end(), g);  // skeleton nodes ascend
+            if (it == node.end() || *it != g) return SFEM_FAILURE;         // an interior dof
+            mask[(size_t)(it - node.begin())] |= (uint8_t)(1u << (q % bs));
+        }
+        return cmask.upload(mask.data(), mask.size());
+    }
     int apply(const void *x, void *y, hipStream_t s) override {
+        if (fused(s)) return inner->apply_constrained(x, y, cmask.p, 0, s);
         if (inner->apply(x, y, s)) return SFEM_FAILURE;
         return copy_constrained(x, y, s);
     }
     int apply_overwrite(const void *x, void *y, hipStream_t s) override {
+        if (fused(s)) return inner->apply_constrained(x, y, cmask.p, 1, s);
         if (inner->apply_overwrite(x, y, s)) return SFEM_FAILURE;
         return copy_constrained(x, y, s);
     }

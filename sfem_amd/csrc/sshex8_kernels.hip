@@ -238,10 +238,13 @@ struct MfmaAcc<float> {
 
 // y[node[t]] += sum of part[ent[off[t] .. off[t+1])] * bs components: the skeleton nodes'
 // element contributions, in slot order.
+// With cmask: components flagged in cmask[t] are Dirichlet rows, set to cx (the constrained
+// operator's y = x there) instead.
 template <typename T, int BS>
 __global__ void skeleton_gather_kernel(const ptrdiff_t n, const int32_t *__restrict__ node,
                                        const int32_t *__restrict__ off, const int32_t *__restrict__ ent,
-                                       const T *__restrict__ part, T *__restrict__ y, const int overwrite) {
+                                       const T *__restrict__ part, T *__restrict__ y, const int overwrite,
+                                       const uint8_t *__restrict__ cmask, const T *__restrict__ cx) {
     for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
         const int j0 = off[t], j1 = off[t + 1];
         T acc[BS];
@@ -253,8 +256,10 @@ __global__ void skeleton_gather_kernel(const ptrdiff_t n, const int32_t *__restr
             for (int c = 0; c < BS; c++) acc[c] += part[q + c];
         }
         const ptrdiff_t g = (ptrdiff_t)node[t] * BS;
+        const unsigned cm = cmask ? cmask[t] : 0u;
 #pragma unroll
-        for (int c = 0; c < BS; c++) y[g + c] = overwrite ? acc[c] : y[g + c] + acc[c];
+        for (int c = 0; c < BS; c++)
+            y[g + c] = ((cm >> c) & 1u) ? cx[g + c] : (overwrite ? acc[c] : y[g + c] + acc[c]);
     }
 }
 
@@ -830,16 +835,19 @@ bool sshex8_level_supported(int L) {
 
 template <typename T>
 int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStream_t s, int overwrite,
-                    int node_sorted) {
+                    int node_sorted, const T *cx) {
     if (sk.n == 0) return SFEM_SUCCESS;
     // node-sorted partials are read in place; element-major ones through ent.  Default: scalar
     // partials node-sorted, vector partials element-major
     if (node_sorted < 0) node_sorted = bs == 1;
     const int32_t *ent = node_sorted ? nullptr : sk.ent;
+    const uint8_t *cm = cx ? sk.cmask : nullptr;
     if (bs == 1)
-        skeleton_gather_kernel<T, 1><<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, ent, part, y, overwrite);
+        skeleton_gather_kernel<T, 1>
+                <<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, ent, part, y, overwrite, cm, cx);
     else if (bs == 3)
-        skeleton_gather_kernel<T, 3><<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, ent, part, y, overwrite);
+        skeleton_gather_kernel<T, 3>
+                <<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, ent, part, y, overwrite, cm, cx);
     else {
         set_last_error("skeleton_gather: block size must be 1 or 3");
         return SFEM_FAILURE;
@@ -869,7 +877,7 @@ int sshex8_laplacian_apply(int L, ptrdiff_t E, const int32_t *idx, const G *fff,
             return SFEM_FAILURE;
     }
     SFEM_AMD_HIP_CHECK(hipGetLastError());
-    return skeleton_gather<T>(sk, 1, part, y, s, overwrite);
+    return skeleton_gather<T>(sk, 1, part, y, s, overwrite, -1, x);
 }
 
 template <typename T, typename G>
@@ -957,7 +965,7 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
 #undef MCASE
             }
             SFEM_AMD_HIP_CHECK(hipGetLastError());
-            return skeleton_gather<T>(sk, 3, part, y, s, overwrite, 1);  // node-sorted partials
+            return skeleton_gather<T>(sk, 3, part, y, s, overwrite, 1, x);  // node-sorted partials
         }
     }
     if constexpr (std::is_same<T, float>::value) {
@@ -977,7 +985,7 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
 #undef MCASE
             }
             SFEM_AMD_HIP_CHECK(hipGetLastError());
-            return skeleton_gather<T>(sk, 3, part, y, s, overwrite, 1);  // node-sorted partials
+            return skeleton_gather<T>(sk, 3, part, y, s, overwrite, 1, x);  // node-sorted partials
         }
     }
     switch (L) {
@@ -995,7 +1003,7 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
             return SFEM_FAILURE;
     }
     SFEM_AMD_HIP_CHECK(hipGetLastError());
-    return skeleton_gather<T>(sk, 3, part, y, s, overwrite);
+    return skeleton_gather<T>(sk, 3, part, y, s, overwrite, -1, x);
 }
 
 template <typename T>
@@ -1018,8 +1026,10 @@ int sshex8_elasticity_diag(int L, ptrdiff_t E, const int32_t *idx, const double 
     return SFEM_SUCCESS;
 }
 
-template int skeleton_gather<double>(const SkeletonMap &, int, const double *, double *, hipStream_t, int, int);
-template int skeleton_gather<float>(const SkeletonMap &, int, const float *, float *, hipStream_t, int, int);
+template int skeleton_gather<double>(const SkeletonMap &, int, const double *, double *, hipStream_t, int, int,
+                                     const double *);
+template int skeleton_gather<float>(const SkeletonMap &, int, const float *, float *, hipStream_t, int, int,
+                                    const float *);
 template int sshex8_laplacian_apply<double, float>(int, ptrdiff_t, const int32_t *, const float *, ptrdiff_t,
                                                    const float *, const double *, const double *, double *, double *,
                                                    const SkeletonMap &, hipStream_t, int);

@@ -17,13 +17,17 @@ bool sshex8_level_supported(int L);
 struct SkeletonMap {
     const int32_t *node{nullptr}, *off{nullptr}, *pos{nullptr}, *ent{nullptr};
     ptrdiff_t n{0};
+    // optional Dirichlet rows: bit c of cmask[t] marks component c of node[t] as a constrained dof,
+    // whose output the gather sets to the operator's input there (y = x) in place of the sum
+    const uint8_t *cmask{nullptr};
 };
 // y[node[t]*bs + c] (+)= sum_{j in [off[t], off[t+1])} part[(bs == 1 ? j : ent[j])*bs + c]
 template <typename T>
 // node_sorted: partial j of the node-sorted list at part[j * bs] (1) or at part[ent[j] * bs] (0);
 // -1: scalar partials node-sorted, vector partials element-major
+// cx: the operator's input, read at the constrained rows of sk.cmask (unused without it)
 int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStream_t s, int overwrite = 0,
-                    int node_sorted = -1);
+                    int node_sorted = -1, const T *cx = nullptr);
 
 // G = float (FP64 path) or uint16_t holding bf16 (FP32 path).
 template <typename T, typename G>

@@ -339,6 +339,43 @@ def test_constrained_operator_copies_dofs():
     assert np.all(zz[nodes] == 0) and zz.sum() == N - len(nodes)
 
 
+@pytest.mark.parametrize("kind,L", [("laplacian", 4), ("elasticity", 4), ("elasticity", 16)])
+@pytest.mark.parametrize("interior", [False, True])
+def test_constrained_gather_matches_separate_copy(kind, L, interior):
+    """Dirichlet rows on the macro-element skeleton are written by the operator's skeleton gather
+    (y = x there); with a constrained interior dof the operator falls back to apply + copy.  Both
+    must equal the unconstrained apply followed by the copy, bitwise (accumulating into y)."""
+    mel, mpts = skewed_box(2, perturb=0.02)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    N = pts.shape[1]
+    mesh = S.Mesh(S.SSHEX8, L, el, N)
+    nodes = np.where(pts[0] < 1e-6)[0].astype(np.int32)
+    if interior:
+        nodes = np.append(nodes, el[(L // 2 * (L + 1) + L // 2) * (L + 1) + L // 2, 0]).astype(np.int32)
+    rng = np.random.default_rng(L)
+    for rt, dt in ((S.FLOAT64, np.float64), (S.FLOAT32, np.float32)):
+        if kind == "laplacian":
+            bs, comps = 1, None
+            op = S.laplacian(mesh, O.hex8_fff(mel, mpts), real_type=rt)
+        else:
+            bs, comps = 3, [0, 2]
+            adj, det = O.hex8_adj_det(mel, mpts)
+            mu, lam = O.lame(1.0, 0.3)
+            op = S.linear_elasticity(mesh, adj, det, mu, lam, real_type=rt)
+        cop = S.constrained(op, nodes, bs, comps)
+        x = rng.uniform(-1, 1, bs * N).astype(dt)
+        y0 = rng.uniform(-1, 1, bs * N).astype(dt)
+        yr = dev(y0, dt)
+        op.apply(dev(x, dt), yr)
+        ref = host(yr)
+        dofs = (nodes[:, None] * bs + np.asarray(comps if comps else [0])[None, :]).ravel()
+        ref[dofs] = x[dofs]
+        y = dev(y0, dt)
+        cop.apply(dev(x, dt), y)
+        S.synchronize()
+        np.testing.assert_array_equal(host(y), ref)
+
+
 # ---------------------------------------------------------------- BLAS
 def test_blas_reductions_deterministic_and_accurate():
     rng = np.random.default_rng(0)
