@@ -240,6 +240,34 @@ def elasticity_solve_times(S, n_macro=32, L=8, rtol=1e-10, with_learned=False, c
     return out
 
 
+def elasticity_apply_times(S, n_macro, L, steps, coeff_seed=None):
+    """Matrix-free sshex8 elasticity apply (y = A x, FP64 and the FP32/BF16 matrix-core operator),
+    HIP-event timed over `steps` applies: DOF/s and algorithmic GB/s, B = 2 s_v N + 4 nxe E + 4 * 10 E
+    (x read, y written, int32 connectivity, adjugate + det; BF16 geometry: 2 bytes per word)."""
+    mel, mpts = S.box_mesh(n_macro + 1, n_macro + 1, n_macro + 1, element_type=S.HEX8)
+    mesh, pts_d = S.sshex8_refine_device(L, mel, mpts)
+    adj, det = S.mesh_adjugate_det(mesh, pts_d)
+    del pts_d
+    N, E = 3 * mesh.nnodes, n_macro ** 3
+    coeff = None
+    if coeff_seed is not None:
+        coeff = (10.0 ** np.random.default_rng(coeff_seed).uniform(-1, 1, E)).astype(np.float32)
+    nu = 0.3
+    mu, lam = 1.0 / (2 * (1 + nu)), nu / ((1 + nu) * (1 - 2 * nu))
+    out = {"workload": f"sshex8 linear elasticity apply, L={L} on a {n_macro}^3 macro cube, {N} DOF", "dofs": N,
+           "macro_elements": E, "steps": steps}
+    for name, rt, dt, vb, gb in (("fp64", S.FLOAT64, np.float64, 8, 4), ("fp32", S.FLOAT32, np.float32, 4, 2)):
+        op = S.linear_elasticity(mesh, adj, det, mu, lam, coeff, real_type=rt)
+        x = S.DeviceArray.from_numpy(np.random.default_rng(0).uniform(-1, 1, N).astype(dt), dt)
+        y = S.DeviceArray(N, dt)
+        ms = S.benchmark(op, x, y, warmup=2, reps=steps) / steps
+        B = 2 * vb * N + 4 * (L + 1) ** 3 * E + gb * 10 * E
+        out[name] = {"ms_per_apply": ms, "dof_per_s": N / (ms * 1e-3), "algorithmic_bytes": B,
+                     "GB_per_s": B / (ms * 1e-3) / 1e9, "hbm_frac": B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        del op, x, y
+    return out
+
+
 def elasticity_cases(S, args):
     out = {}
     if args.elasticity_case in ("l8", "both"):
@@ -265,8 +293,9 @@ def main():
     ap.add_argument("--elasticity-case", choices=["l8", "l16", "both"], default="both",
                     help="l8: L=8 on 32^3 macro elements, E=1 (50.9M DOF); l16: L=16 on 24^3 macro elements, "
                          "E log-uniform per macro element (171.2M DOF)")
-    ap.add_argument("--mode", choices=["apply", "copy"], default="apply",
-                    help="copy: stream-copy 2 GiB of FP64 per step (HBM counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--mode", choices=["apply", "copy", "elasticity-apply"], default="apply",
+                    help="copy: stream-copy 2 GiB of FP64 per step (HBM counter calibration for rocprofv3 --pmc); "
+                         "elasticity-apply: time the sshex8 elasticity operators of --elasticity-case")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (every box here has one GPU; "
@@ -289,6 +318,14 @@ def main():
         S.synchronize()
         dt = (_t.perf_counter() - t0) / args.steps
         print(json.dumps({"mode": "copy", "bytes_per_step": 16 * n, "GB/s": 16 * n / dt / 1e9}))
+        return
+    if args.mode == "elasticity-apply":
+        out = {}
+        if args.elasticity_case in ("l8", "both"):
+            out["elasticity_apply_l8"] = elasticity_apply_times(S, 32, 8, args.steps)
+        if args.elasticity_case in ("l16", "both"):
+            out["elasticity_apply_l16"] = elasticity_apply_times(S, 24, 16, args.steps, coeff_seed=0)
+        print(json.dumps(out))
         return
     if args.only_elasticity:
         print(json.dumps(elasticity_cases(S, args)))
