@@ -457,21 +457,33 @@ struct ElLds<8> {
     static constexpr int RS = 12, PL = 109, PS = 977;
 };
 
+// The FP64 kernel's node planes (x and accumulators, 8-byte words).  Same bank model: with the
+// padded L = 8 strides the B-operand reads and accumulator updates of a parity colour need half
+// the passes of the dense layout.
 template <int L>
+struct ElLds64 {
+    static constexpr int RS = L + 1, PL = (L + 1) * (L + 1), PS = (L + 1) * (L + 1) * (L + 1);
+};
+template <>
+struct ElLds64<8> {
+    static constexpr int RS = 10, PL = 90, PS = 809;
+};
+
+template <int L, typename Lay>
 struct SkSlotTable {
     int16_t v[(L + 1) * (L + 1) * (L + 1)];    // skeleton slot
-    int16_t lds[(L + 1) * (L + 1) * (L + 1)];  // ElLds position in a component plane
+    int16_t lds[(L + 1) * (L + 1) * (L + 1)];  // Lay position in a component plane
     constexpr SkSlotTable() : v(), lds() {
         for (int z = 0; z <= L; z++)
             for (int y = 0; y <= L; y++)
                 for (int x = 0; x <= L; x++) {
                     v[lidx(L, x, y, z)] = (int16_t)sk_slot(L, x, y, z);
-                    lds[lidx(L, x, y, z)] = (int16_t)(z * ElLds<L>::PL + y * ElLds<L>::RS + x);
+                    lds[lidx(L, x, y, z)] = (int16_t)(z * Lay::PL + y * Lay::RS + x);
                 }
     }
 };
-template <int L>
-__constant__ SkSlotTable<L> sk_slot_table = SkSlotTable<L>();
+template <int L, typename Lay = ElLds<L>>
+__constant__ SkSlotTable<L, Lay> sk_slot_table = SkSlotTable<L, Lay>();
 
 template <int L>
 struct ElMfma {
@@ -689,11 +701,12 @@ __global__ void __launch_bounds__(256)
                                           const int kshift, const int32_t *__restrict__ sk_pos) {
     using S = ElMfma<L>;
     using M = MfmaAcc<double>;
+    using Y = ElLds64<L>;
     constexpr int N1 = S::N1, NXE = S::NXE, H = S::H, NC = S::NC, TPC = S::TPC, EPB = S::EPB, WPE = S::WPE;
     constexpr int NSK = sk_count(L);
     constexpr int NPT = (EPB * NXE + 255) / 256;
-    __shared__ double sx[EPB][3][NXE];
-    __shared__ double sy[EPB][3][NXE];
+    __shared__ double sx[EPB][3][Y::PS];
+    __shared__ double sy[EPB][3][Y::PS];
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int c = lane & 15, g = lane >> 4;
@@ -705,10 +718,12 @@ __global__ void __launch_bounds__(256)
         const int k = tid + 256 * i, le = k / NXE;
         gid[i] = (k < EPB * NXE && e0 + le < E) ? idx[e0 * NXE + k] : -1;
     }
+    int lds[NPT];  // LDS positions
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         const int k = tid + 256 * i, le = k / NXE, v = k - le * NXE;
-        const int slot = sk_slot_table<L>.v[v];
+        const int slot = sk_slot_table<L, Y>.v[v];
+        lds[i] = sk_slot_table<L, Y>.lds[v];
         const bool interior = slot < 0;
         pos[i] = (gid[i] >= 0 && !interior) ? sk_pos[(e0 + le) * NSK + slot] : -1;
         double xv[3] = {0, 0, 0}, yv[3] = {0, 0, 0};
@@ -724,8 +739,8 @@ __global__ void __launch_bounds__(256)
         if (k < EPB * NXE) {
 #pragma unroll
             for (int ci = 0; ci < 3; ci++) {
-                sx[le][ci][v] = xv[ci];
-                sy[le][ci][v] = yv[ci];
+                sx[le][ci][lds[i]] = xv[ci];
+                sy[le][ci][lds[i]] = yv[ci];
             }
         }
     }
@@ -756,7 +771,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int s = 0; s < 6; s++) {
                 const int kk = 4 * s + g, j = kk & 7;  // column: component kk / 8, corner kk % 8
-                const int node = lidx(L, mx + hex8_cx(j), my + hex8_cy(j), mz + hex8_cz(j));
+                const int node = (mz + hex8_cz(j)) * Y::PL + (my + hex8_cy(j)) * Y::RS + mx + hex8_cx(j);
                 const double b = sx[le][kk >> 3][node];  // in range (qq clamped); invalid columns unused
                 d0 = M::step(a[0][s], b, d0);
                 d1 = M::step(a[1][s], b, d1);
@@ -769,7 +784,7 @@ __global__ void __launch_bounds__(256)
                         const int R = 16 * tt + M::row(g, r);  // component R / 8, corner R % 8
                         if (R < 24) {
                             const int k = R & 7;
-                            sy[le][R >> 3][lidx(L, mx + hex8_cx(k), my + hex8_cy(k), mz + hex8_cz(k))] +=
+                            sy[le][R >> 3][(mz + hex8_cz(k)) * Y::PL + (my + hex8_cy(k)) * Y::RS + mx + hex8_cx(k)] +=
                                     tt ? d1[r] : d0[r];
                         }
                     }
@@ -785,7 +800,7 @@ __global__ void __launch_bounds__(256)
         if (gid[i] < 0) continue;
         double *q = pos[i] < 0 ? y + (ptrdiff_t)gid[i] * 3 : part + (ptrdiff_t)pos[i] * 3;
 #pragma unroll
-        for (int ci = 0; ci < 3; ci++) q[ci] = sy[le2][ci][v];
+        for (int ci = 0; ci < 3; ci++) q[ci] = sy[le2][ci][lds[i]];
     }
 }
 
