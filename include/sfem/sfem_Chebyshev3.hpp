@@ -11,6 +11,17 @@
 
 namespace sfem {
 
+// Extension: an operator that can run a Chebyshev3 step inside its apply.  With t = A x:
+// p = D^-1 (t - rhs) (first step) or beta p + D^-1 (t - rhs), then x -= alpha p, without storing t
+// (D^-1 = inv_diag, or the identity when null).  The smoother uses it when cheb3_fusable().
+template <typename T>
+class FusedChebyshevStep {
+public:
+    virtual ~FusedChebyshevStep() = default;
+    virtual bool cheb3_fusable() = 0;
+    virtual int cheb3_step(T *x, const T *rhs, const T *inv_diag, T *p, T alpha, T beta, int first) = 0;
+};
+
 template <typename T>
 class Chebyshev3 final : public MatrixFreeLinearSolver<T> {
 public:
@@ -83,19 +94,31 @@ public:
         const T lmin = scale_eig_min * lmax;
         const T avg = (lmin + lmax) / 2, diff = (lmin - lmax) / 2;
         T alpha = T(1) / avg, beta = 0, dea = 0;
+        // operators that run the step inside their apply (same words, no product vector)
+        auto *fz = dynamic_cast<FusedChebyshevStep<T> *>(op.get());
+        const bool fused = fz && apply_overwrites_output && fz->cheb3_fusable();
+        const T *dinv = inv_diag ? inv_diag->data() : nullptr;
         // iteration 0: p = A x - rhs (or -rhs when x == 0), scaled by D^-1 when given; x -= alpha p
         // (one fused pass after the product)
-        if (!is_initial_guess_zero) {
-            if (!apply_overwrites_output) blas->zeros(n, p);
-            op->apply(x, p);
+        if (!is_initial_guess_zero && fused) {
+            if (fz->cheb3_step(x, rhs, dinv, p, alpha, T(0), 1)) return SFEM_FAILURE;
+        } else {
+            if (!is_initial_guess_zero) {
+                if (!apply_overwrites_output) blas->zeros(n, p);
+                op->apply(x, p);
+            }
+            sfem_amd_cheb3_start(real_type_of<T>::value, n, (double)alpha, rhs, dinv, is_initial_guess_zero ? 0 : 1,
+                                 p, x, stream);
         }
-        sfem_amd_cheb3_start(real_type_of<T>::value, n, (double)alpha, rhs, inv_diag ? inv_diag->data() : nullptr,
-                             is_initial_guess_zero ? 0 : 1, p, x, stream);
         iterations_ = 1;
         for (; iterations_ < max_it; iterations_++) {
             dea = diff * alpha;
             beta = (iterations_ == 1 ? T(0.5) : T(0.25)) * dea * dea;
             alpha = T(1) / (avg - beta / alpha);
+            if (fused) {
+                if (fz->cheb3_step(x, rhs, dinv, p, alpha, beta, 0)) return SFEM_FAILURE;
+                continue;
+            }
             if (!apply_overwrites_output) blas->zeros(n, temp);
             op->apply(x, temp);
             // fused: p = beta p - rhs + A x (times D^-1 when scaled); x -= alpha p

@@ -49,6 +49,12 @@ typedef struct sfem_amd_op sfem_amd_op;
 /* ------------------------------------------------------------------ library */
 /* Last error message of this thread ("" if none). */
 const char *sfem_amd_last_error(void);
+
+/* Library options (no counterpart in the reference).  "fused_chebyshev" (default 0): 1 makes
+ * Chebyshev3 smoothers on sshex8 matrix-core elasticity operators run each step inside the
+ * operator's apply kernels (same results, no product vector; slower on MI355X than the separate
+ * streaming update, whose accesses are contiguous); 0 runs the separate product and update. */
+int sfem_amd_set_option(const char *name, int value);
 /* Number of visible GPUs; 0 when no HIP device is present. */
 int sfem_amd_device_count(void);
 int sfem_amd_device_synchronize(void);

@@ -77,6 +77,7 @@ SIGNATURES = {
     "sfem_amd_cheb3_update": (_i, [_i, _pd, _d, _d, _vp, _vp, _vp, _vp, _vp]),
     "sfem_amd_cheb3_update_diag": (_i, [_i, _pd, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sfem_amd_cheb3_start": (_i, [_i, _pd, _d, _vp, _vp, _i, _vp, _vp, _vp]),
+    "sfem_amd_set_option": (_i, [C.c_char_p, _i]),
     "sfem_amd_cheb3_diag_create": (_i, [_vp, _vp, _i, _d, _d, _i, _d, _i, _vp, C.POINTER(_vp)]),
     "sfem_amd_cg_create": (_i, [_vp, _vp, _i, _d, _d, _i, C.POINTER(_vp)]),
     "sfem_amd_cheb3_create": (_i, [_vp, _i, _d, _d, _i, _d, _i, _vp, C.POINTER(_vp)]),
@@ -669,6 +670,11 @@ def dot(x, y, stream=None):
 def norm2(x, stream=None):
     rt = FLOAT64 if x.element_size() == 8 else FLOAT32
     return lib.sfem_amd_blas_norm2(rt, x.numel(), _dptr(x), _stream_ptr(stream))
+
+
+def set_option(name, value):
+    """Library options, see sfem_amd_set_option ("fused_chebyshev": 1 or 0, default 0)."""
+    _check(lib.sfem_amd_set_option(name.encode(), int(value)), "set_option")
 
 
 def device_count():

@@ -128,6 +128,28 @@ __host__ __device__ inline double hex8_elasticity_entry(const double *adj, doubl
     return v * det;
 }
 
+// ----------------------------------------------------------------- Chebyshev3 step
+// One Chebyshev3 step on entry i given t = (A x)_i (algebra/sfem_Chebyshev3.hpp:37-52, 208-217):
+// first step p = t - rhs, later ones p = beta p + (t - rhs); with a Jacobi scaling the
+// (t - rhs) term is multiplied by dinv; then x -= alpha p.  Shared by the vector kernels and the
+// apply epilogues so that fused and separate steps give the same words.
+template <typename T>
+struct ChebStep {
+    const T *rhs{nullptr};
+    const T *dinv{nullptr};  // optional D^-1
+    T *p{nullptr};           // null: no step (a plain apply)
+    T *x{nullptr};           // the operator input, updated in place
+    T alpha{0}, beta{0};
+    int first{0};  // p holds nothing yet (beta unused)
+};
+template <typename T>
+__host__ __device__ __forceinline__ T cheb3_direction(T p, T rhs, T t, T beta, const T *dinv, ptrdiff_t i,
+                                                      int first) {
+    T r = t - rhs;
+    if (dinv) r = dinv[i] * r;
+    return first ? r : __builtin_fma(beta, p, r);
+}
+
 // ----------------------------------------------------------------- launch helpers
 constexpr int kBlock = 256;
 inline unsigned grid_for(ptrdiff_t n, int block = kBlock, unsigned cap = 1u << 20) {

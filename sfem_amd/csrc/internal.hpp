@@ -96,6 +96,14 @@ struct OpBase {
         set_last_error("constrained gather not available for this operator");
         return SFEM_FAILURE;
     }
+    // A Chebyshev3 step run inside the apply (ChebStep: with t = A x, p and x updated in place, t
+    // not stored); cmask as in apply_constrained.  Only where cheb3_fusable says so.
+    virtual bool cheb3_fusable(hipStream_t) { return false; }
+    virtual int cheb3_step(void *, const void *, const void *, void *, double, double, int, const uint8_t *,
+                           hipStream_t) {
+        set_last_error("fused Chebyshev step not available for this operator");
+        return SFEM_FAILURE;
+    }
     // Solvers and smoothers: the output holds zeros on entry to the next applies (work that
     // multiplies the initial guess may be skipped).  Other operators ignore it.
     virtual void set_initial_guess_zero(bool) {}
@@ -135,6 +143,9 @@ struct sfem_amd_mesh {
 };
 
 namespace sfem_amd {
+// Library options (sfem_amd_set_option): Chebyshev3 steps fused into the sshex8 matrix-core
+// elasticity applies (default off: measured slower than the streaming update kernels).
+extern bool g_fused_chebyshev;
 // Zeroes / sets constrained dofs; defined in runtime.cpp.
 int constrained_set_value(const OpBase *constrained, int rt, double v, void *x, hipStream_t s);
 int mesh_counts(const sfem_amd_mesh *m, hipStream_t s);

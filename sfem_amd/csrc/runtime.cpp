@@ -151,6 +151,8 @@ static int mesh_owner(const sfem_amd_mesh *m, hipStream_t s) {
     if (m->own.alloc((size_t)m->E * m->nv)) return SFEM_FAILURE;
     return node_owner_mask(m->E, m->nv, m->nnodes, m->idx.p, m->own.p, s);
 }
+bool g_fused_chebyshev = false;
+
 const sfem_amd_mesh *mesh_subblocks(const sfem_amd_mesh *m, int Ls, hipStream_t s) {
     auto &slot = m->sub[Ls];
     if (slot) return slot.get();
@@ -300,7 +302,26 @@ struct ElasticityOp final : OpBase {
         }
         return apply_impl(xv, yv, s, ow, cmask);
     }
-    int apply_impl(const void *xv, void *yv, hipStream_t s, int ow, const uint8_t *cmask = nullptr) {
+    bool cheb3_fusable(hipStream_t s) override {
+        if (!g_fused_chebyshev || m->type != SFEM_AMD_SSHEX8) return false;
+        const sfem_amd_mesh *wm = work_mesh(s);
+        if (!wm || mesh_skeleton(wm, 0) || !skeleton_covers(wm)) return false;
+        return sshex8_elasticity_cheb3_fusable(wm->level, std::is_same<T, double>::value, uses_matrix_cores());
+    }
+    int cheb3_step(void *x, const void *rhs, const void *dinv, void *p, double alpha, double beta, int first,
+                   const uint8_t *cmask, hipStream_t s) override {
+        ChebStep<T> cs;
+        cs.rhs = (const T *)rhs;
+        cs.dinv = (const T *)dinv;
+        cs.p = (T *)p;
+        cs.x = (T *)x;
+        cs.alpha = (T)alpha;
+        cs.beta = (T)beta;
+        cs.first = first;
+        return apply_impl(x, p, s, 1, cmask, &cs);  // y is not touched by the fused step
+    }
+    int apply_impl(const void *xv, void *yv, hipStream_t s, int ow, const uint8_t *cmask = nullptr,
+                   const ChebStep<T> *cs = nullptr) {
         if (m->type == SFEM_AMD_HEX8) {
             if constexpr (std::is_same<T, double>::value)
                 return hex8_elasticity_apply(m->E, m->idx.p, adj.p, det.p, mu, lambda, coeff.p, (const double *)xv,
@@ -330,7 +351,8 @@ struct ElasticityOp final : OpBase {
         SkeletonMap sk = skeleton_of(wm);
         sk.cmask = cmask;
         return sshex8_elasticity_apply<T>(wm->level, wm->E, wm->idx.p, Kt, (const T *)xv, (T *)yv, part.p, sk, s, ow,
-                                          uses_matrix_cores() ? 1 : 0, wm == m ? 0 : 3, Apk.p);
+                                          uses_matrix_cores() ? 1 : 0, wm == m ? 0 : 3, Apk.p,
+                                          cs ? *cs : ChebStep<T>());
     }
     int diag(void *d, hipStream_t s) override {
         if (m->type == SFEM_AMD_HEX8) {
@@ -395,6 +417,11 @@ struct ConstrainedOp final : OpBase {
         if (fused(s)) return inner->apply_constrained(x, y, cmask.p, 1, s);
         if (inner->apply_overwrite(x, y, s)) return SFEM_FAILURE;
         return copy_constrained(x, y, s);
+    }
+    bool cheb3_fusable(hipStream_t s) override { return fused(s) && inner->cheb3_fusable(s); }
+    int cheb3_step(void *x, const void *rhs, const void *dinv, void *p, double alpha, double beta, int first,
+                   const uint8_t *, hipStream_t s) override {
+        return inner->cheb3_step(x, rhs, dinv, p, alpha, beta, first, cmask.p, s);
     }
     int copy_constrained(const void *x, void *y, hipStream_t s) const {
         if (inner->real_type() == SFEM_AMD_FLOAT64)
@@ -509,7 +536,7 @@ struct ChainOp final : OpBase {
 // a multigrid or CG can pass set_initial_guess_zero on to a wrapped smoother or preconditioner;
 // the solver set-up calls are not forwarded (the wrapped object is configured at creation).
 template <typename T>
-struct OpView final : sfem::MatrixFreeLinearSolver<T> {
+struct OpView final : sfem::MatrixFreeLinearSolver<T>, sfem::FusedChebyshevStep<T> {
     std::shared_ptr<OpBase> impl;
     hipStream_t s{nullptr};
     bool overwrite{false};  // apply() overwrites y (the owning solver skips its zero fill)
@@ -523,6 +550,10 @@ struct OpView final : sfem::MatrixFreeLinearSolver<T> {
     void set_n_dofs(std::ptrdiff_t) override {}
     void set_initial_guess_zero(bool v) override { impl->set_initial_guess_zero(v); }
     int iterations() const override { return impl->iterations(); }
+    bool cheb3_fusable() override { return impl->cheb3_fusable(as_stream(s)); }
+    int cheb3_step(T *x, const T *rhs, const T *inv_diag, T *p, T alpha, T beta, int first) override {
+        return impl->cheb3_step(x, rhs, inv_diag, p, alpha, beta, first, nullptr, as_stream(s));
+    }
     std::ptrdiff_t rows() const override { return impl->rows(); }
     std::ptrdiff_t cols() const override { return impl->cols(); }
     sfem::ExecutionSpace execution_space() const override { return sfem::EXECUTION_SPACE_DEVICE; }
@@ -747,6 +778,16 @@ static bool device_pointer(const void *p) {
 extern "C" {
 
 const char *sfem_amd_last_error(void) { return g_last_error.c_str(); }
+
+int sfem_amd_set_option(const char *name, int value) {
+    SFEM_AMD_REQUIRE(name, "set_option: name");
+    if (std::strcmp(name, "fused_chebyshev") == 0) {
+        g_fused_chebyshev = value != 0;
+        return SFEM_SUCCESS;
+    }
+    set_last_error(std::string("set_option: unknown option ") + name);
+    return SFEM_FAILURE;
+}
 
 int sfem_amd_device_count(void) {
     int n = 0;

@@ -240,11 +240,13 @@ struct MfmaAcc<float> {
 // element contributions, in slot order.
 // With cmask: components flagged in cmask[t] are Dirichlet rows, set to cx (the constrained
 // operator's y = x there) instead.
+// With a Chebyshev step (cs.p != null, overwriting apply): the node's product t feeds the step
+// (p and x updated there, y not written).
 template <typename T, int BS>
 __global__ void skeleton_gather_kernel(const ptrdiff_t n, const int32_t *__restrict__ node,
                                        const int32_t *__restrict__ off, const int32_t *__restrict__ ent,
                                        const T *__restrict__ part, T *__restrict__ y, const int overwrite,
-                                       const uint8_t *__restrict__ cmask, const T *__restrict__ cx) {
+                                       const uint8_t *__restrict__ cmask, const T *cx, const ChebStep<T> cs) {
     for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
         const int j0 = off[t], j1 = off[t + 1];
         T acc[BS];
@@ -258,8 +260,17 @@ __global__ void skeleton_gather_kernel(const ptrdiff_t n, const int32_t *__restr
         const ptrdiff_t g = (ptrdiff_t)node[t] * BS;
         const unsigned cm = cmask ? cmask[t] : 0u;
 #pragma unroll
-        for (int c = 0; c < BS; c++)
-            y[g + c] = ((cm >> c) & 1u) ? cx[g + c] : (overwrite ? acc[c] : y[g + c] + acc[c]);
+        for (int c = 0; c < BS; c++) {
+            const T v = ((cm >> c) & 1u) ? cx[g + c] : (overwrite ? acc[c] : y[g + c] + acc[c]);
+            if (cs.p) {
+                const T pi = cheb3_direction<T>(cs.first ? T(0) : cs.p[g + c], cs.rhs[g + c], v, cs.beta, cs.dinv, g + c,
+                                                cs.first);
+                cs.p[g + c] = pi;
+                cs.x[g + c] = __builtin_fma(-cs.alpha, pi, cs.x[g + c]);
+            } else {
+                y[g + c] = v;
+            }
+        }
     }
 }
 
@@ -553,9 +564,9 @@ int sshex8_elasticity_pack_bf16x2(ptrdiff_t nK, const float *K, uint32_t *Apk, h
 template <int L>
 __global__ void __launch_bounds__(256)
     sshex8_elasticity_apply_mfma_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
-                                        const el_u4 *__restrict__ Apk, const float *__restrict__ x,
-                                        float *__restrict__ y, float *__restrict__ part, const int overwrite,
-                                        const int kshift, const int32_t *__restrict__ sk_pos) {
+                                        const el_u4 *__restrict__ Apk, const float *x, float *__restrict__ y,
+                                        float *__restrict__ part, const int overwrite, const int kshift,
+                                        const int32_t *__restrict__ sk_pos, const ChebStep<float> cs) {
     using S = ElMfma<L>;
     constexpr int N1 = S::N1, NXE = S::NXE, H = S::H, NC = S::NC, TPC = S::TPC, EPB = S::EPB, WPE = S::WPE;
     constexpr int NSK = sk_count(L);
@@ -572,6 +583,7 @@ __global__ void __launch_bounds__(256)
     // start from their y (accumulate), skeleton nodes from 0
     int32_t gid[NPT], pos[NPT];  // node ids; node-sorted skeleton slots (-1: interior)
     int lds[NPT];                // LDS positions
+    float xk[NPT][3];            // x of the node (the Chebyshev step updates interior ones)
 #pragma unroll
     for (int i = 0; i < NPT; i++) {
         const int k = tid + 256 * i, le = k / NXE;
@@ -601,6 +613,8 @@ __global__ void __launch_bounds__(256)
                 sy[le][ci][lds[i]] = yv[ci];
             }
         }
+#pragma unroll
+        for (int ci = 0; ci < 3; ci++) xk[i][ci] = xv[ci];
     }
 
     // A operands (this wave's macro element): tile tt holds K rows 16tt + c, columns 8g + j
@@ -682,6 +696,17 @@ __global__ void __launch_bounds__(256)
     for (int i = 0; i < NPT; i++) {
         const int k = tid + 256 * i, le2 = k / NXE, v = k - le2 * NXE;
         if (gid[i] < 0) continue;
+        if (pos[i] < 0 && cs.p) {  // interior node: its product is complete here
+#pragma unroll
+            for (int ci = 0; ci < 3; ci++) {
+                const ptrdiff_t q = (ptrdiff_t)gid[i] * 3 + ci;
+                const float pi = cheb3_direction<float>(cs.first ? 0.f : cs.p[q], cs.rhs[q], sy[le2][ci][lds[i]],
+                                                        cs.beta, cs.dinv, q, cs.first);
+                cs.p[q] = pi;
+                cs.x[q] = __builtin_fma(-cs.alpha, pi, xk[i][ci]);
+            }
+            continue;
+        }
         float *q = pos[i] < 0 ? y + (ptrdiff_t)gid[i] * 3 : part + (ptrdiff_t)pos[i] * 3;
 #pragma unroll
         for (int ci = 0; ci < 3; ci++) q[ci] = sy[le2][ci][lds[i]];
@@ -696,9 +721,9 @@ __global__ void __launch_bounds__(256)
 template <int L>
 __global__ void __launch_bounds__(256)
     sshex8_elasticity_apply_mfma64_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
-                                          const double *__restrict__ Kall, const double *__restrict__ x,
-                                          double *__restrict__ y, double *__restrict__ part, const int overwrite,
-                                          const int kshift, const int32_t *__restrict__ sk_pos) {
+                                          const double *__restrict__ Kall, const double *x, double *__restrict__ y,
+                                          double *__restrict__ part, const int overwrite, const int kshift,
+                                          const int32_t *__restrict__ sk_pos, const ChebStep<double> cs) {
     using S = ElMfma<L>;
     using M = MfmaAcc<double>;
     using Y = ElLds64<L>;
@@ -798,6 +823,17 @@ __global__ void __launch_bounds__(256)
     for (int i = 0; i < NPT; i++) {
         const int k = tid + 256 * i, le2 = k / NXE, v = k - le2 * NXE;
         if (gid[i] < 0) continue;
+        if (pos[i] < 0 && cs.p) {  // interior node: its product is complete here
+#pragma unroll
+            for (int ci = 0; ci < 3; ci++) {
+                const ptrdiff_t q = (ptrdiff_t)gid[i] * 3 + ci;
+                const double pi = cheb3_direction<double>(cs.first ? 0.0 : cs.p[q], cs.rhs[q], sy[le2][ci][lds[i]],
+                                                          cs.beta, cs.dinv, q, cs.first);
+                cs.p[q] = pi;
+                cs.x[q] = __builtin_fma(-cs.alpha, pi, sx[le2][ci][lds[i]]);
+            }
+            continue;
+        }
         double *q = pos[i] < 0 ? y + (ptrdiff_t)gid[i] * 3 : part + (ptrdiff_t)pos[i] * 3;
 #pragma unroll
         for (int ci = 0; ci < 3; ci++) q[ci] = sy[le2][ci][lds[i]];
@@ -850,7 +886,7 @@ bool sshex8_level_supported(int L) {
 
 template <typename T>
 int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStream_t s, int overwrite,
-                    int node_sorted, const T *cx) {
+                    int node_sorted, const T *cx, const ChebStep<T> &cs) {
     if (sk.n == 0) return SFEM_SUCCESS;
     // node-sorted partials are read in place; element-major ones through ent.  Default: scalar
     // partials node-sorted, vector partials element-major
@@ -859,10 +895,10 @@ int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStrea
     const uint8_t *cm = cx ? sk.cmask : nullptr;
     if (bs == 1)
         skeleton_gather_kernel<T, 1>
-                <<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, ent, part, y, overwrite, cm, cx);
+                <<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, ent, part, y, overwrite, cm, cx, cs);
     else if (bs == 3)
         skeleton_gather_kernel<T, 3>
-                <<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, ent, part, y, overwrite, cm, cx);
+                <<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, ent, part, y, overwrite, cm, cx, cs);
     else {
         set_last_error("skeleton_gather: block size must be 1 or 3");
         return SFEM_FAILURE;
@@ -957,10 +993,19 @@ int sshex8_subblock_idx(ptrdiff_t E, int L, int Ls, const int32_t *idx, int32_t 
     return SFEM_SUCCESS;
 }
 
+bool sshex8_elasticity_cheb3_fusable(int L, bool fp64, bool matrix_cores) {
+    if (!matrix_cores) return false;
+    return fp64 ? (L == 2 || L == 4 || L == 6 || L == 8) : (L % 2 == 0 && L >= 2 && L <= 12);
+}
+
 template <typename T>
 int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *x, T *y, T *part,
                             const SkeletonMap &sk, hipStream_t s, int overwrite, int matrix_cores, int kshift,
-                            const uint32_t *Apk) {
+                            const uint32_t *Apk, const ChebStep<T> &cs) {
+    if (cs.p && (!overwrite || !sshex8_elasticity_cheb3_fusable(L, std::is_same<T, double>::value, matrix_cores))) {
+        set_last_error("sshex8_elasticity_apply: the Chebyshev epilogue needs an overwriting matrix-core apply");
+        return SFEM_FAILURE;
+    }
     if (E == 0) return SFEM_SUCCESS;
     if (kshift && !(matrix_cores && ((std::is_same<T, double>::value && (L == 2 || L == 4 || L == 6 || L == 8)) ||
                                      (std::is_same<T, float>::value && L % 2 == 0 && L <= 12)))) {
@@ -974,13 +1019,13 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
     case LV:                                                                                                \
         sshex8_elasticity_apply_mfma64_kernel<LV>                                                           \
                 <<<(unsigned)((E + ElMfma<LV>::EPB - 1) / ElMfma<LV>::EPB), 256, 0, s>>>(E, idx, K, x, y, part, \
-                                                                                         overwrite, kshift, sk.pos); \
+                                                                                         overwrite, kshift, sk.pos, cs); \
         break;
                 MCASE(2) MCASE(4) MCASE(6) MCASE(8)
 #undef MCASE
             }
             SFEM_AMD_HIP_CHECK(hipGetLastError());
-            return skeleton_gather<T>(sk, 3, part, y, s, overwrite, 1, x);  // node-sorted partials
+            return skeleton_gather<T>(sk, 3, part, y, s, overwrite, 1, x, cs);  // node-sorted partials
         }
     }
     if constexpr (std::is_same<T, float>::value) {
@@ -994,13 +1039,13 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
     case LV:                                                                                                    \
         sshex8_elasticity_apply_mfma_kernel<LV><<<(unsigned)((E + ElMfma<LV>::EPB - 1) / ElMfma<LV>::EPB), 256, 0, \
                                                   s>>>(E, idx, (const el_u4 *)Apk, x, y, part, overwrite, kshift, \
-                                                       sk.pos);                                                 \
+                                                       sk.pos, cs);                                             \
         break;
                 MCASE(2) MCASE(4) MCASE(6) MCASE(8) MCASE(10) MCASE(12)
 #undef MCASE
             }
             SFEM_AMD_HIP_CHECK(hipGetLastError());
-            return skeleton_gather<T>(sk, 3, part, y, s, overwrite, 1, x);  // node-sorted partials
+            return skeleton_gather<T>(sk, 3, part, y, s, overwrite, 1, x, cs);  // node-sorted partials
         }
     }
     switch (L) {
@@ -1042,9 +1087,9 @@ int sshex8_elasticity_diag(int L, ptrdiff_t E, const int32_t *idx, const double 
 }
 
 template int skeleton_gather<double>(const SkeletonMap &, int, const double *, double *, hipStream_t, int, int,
-                                     const double *);
+                                     const double *, const ChebStep<double> &);
 template int skeleton_gather<float>(const SkeletonMap &, int, const float *, float *, hipStream_t, int, int,
-                                    const float *);
+                                    const float *, const ChebStep<float> &);
 template int sshex8_laplacian_apply<double, float>(int, ptrdiff_t, const int32_t *, const float *, ptrdiff_t,
                                                    const float *, const double *, const double *, double *, double *,
                                                    const SkeletonMap &, hipStream_t, int);
@@ -1061,10 +1106,10 @@ template int sshex8_elasticity_matrices<uint16_t>(int, ptrdiff_t, const uint16_t
                                                   double, double, const float *, double *, hipStream_t);
 template int sshex8_elasticity_apply<double>(int, ptrdiff_t, const int32_t *, const double *, const double *,
                                              double *, double *, const SkeletonMap &, hipStream_t, int, int, int,
-                                             const uint32_t *);
+                                             const uint32_t *, const ChebStep<double> &);
 template int sshex8_elasticity_apply<float>(int, ptrdiff_t, const int32_t *, const float *, const float *, float *,
                                             float *, const SkeletonMap &, hipStream_t, int, int, int,
-                                            const uint32_t *);
+                                            const uint32_t *, const ChebStep<float> &);
 template int sshex8_elasticity_diag<double>(int, ptrdiff_t, const int32_t *, const double *, int, double *,
                                             hipStream_t);
 template int sshex8_elasticity_diag<float>(int, ptrdiff_t, const int32_t *, const double *, int, float *,

@@ -5,6 +5,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "common.hpp"
+
 namespace sfem_amd {
 
 bool sshex8_level_supported(int L);
@@ -25,9 +27,10 @@ struct SkeletonMap {
 template <typename T>
 // node_sorted: partial j of the node-sorted list at part[j * bs] (1) or at part[ent[j] * bs] (0);
 // -1: scalar partials node-sorted, vector partials element-major
-// cx: the operator's input, read at the constrained rows of sk.cmask (unused without it)
+// cx: the operator's input, read at the constrained rows of sk.cmask (unused without it); cs: a
+// Chebyshev3 step fed by the (overwritten) product instead of storing y
 int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStream_t s, int overwrite = 0,
-                    int node_sorted = -1, const T *cx = nullptr);
+                    int node_sorted = -1, const T *cx = nullptr, const ChebStep<T> &cs = ChebStep<T>());
 
 // G = float (FP64 path) or uint16_t holding bf16 (FP32 path).
 template <typename T, typename G>
@@ -49,9 +52,13 @@ template <typename T>
 // the node-wise vector kernel
 // kshift > 0 (matrix-core kernels only): elements e share the matrix K[e >> kshift] (sub-blocks)
 // Apk: the FP32 matrix-core kernel's operands, sshex8_elasticity_pack_bf16x2 of K (required there)
+// cs (matrix-core kernels, overwrite = 1): the product feeds a Chebyshev3 step (x, p updated in
+// place; y not written), see sshex8_elasticity_cheb3_fusable
 int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, const T *x, T *y, T *part,
                             const SkeletonMap &sk, hipStream_t s, int overwrite = 0, int matrix_cores = 0,
-                            int kshift = 0, const uint32_t *Apk = nullptr);
+                            int kshift = 0, const uint32_t *Apk = nullptr, const ChebStep<T> &cs = ChebStep<T>());
+// the level and precision run on the matrix-core kernels (which carry the Chebyshev epilogue)
+bool sshex8_elasticity_cheb3_fusable(int L, bool fp64, bool matrix_cores);
 // The bf16 hi/lo split of nK float 24 x 24 matrices in the FP32 matrix-core kernel's A-operand
 // layout: nK * 1024 words.
 int sshex8_elasticity_pack_bf16x2(ptrdiff_t nK, const float *K, uint32_t *Apk, hipStream_t s);

@@ -412,9 +412,9 @@ template <typename T>
 __global__ void cheb3_update_kernel(const ptrdiff_t n, const T alpha, const T beta, const T *__restrict__ rhs,
                                     const T *__restrict__ Ax, T *__restrict__ p, T *__restrict__ x) {
     for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
-        const T pi = beta * p[i] - rhs[i] + Ax[i];
+        const T pi = cheb3_direction<T>(p[i], rhs[i], Ax[i], beta, nullptr, i, 0);
         p[i] = pi;
-        x[i] -= alpha * pi;
+        x[i] = __builtin_fma(-alpha, pi, x[i]);
     }
 }
 
@@ -425,10 +425,15 @@ __global__ void cheb3_start_kernel(const ptrdiff_t n, const T alpha, const T *__
                                    const T *__restrict__ dinv, const int have_ax, T *__restrict__ p,
                                    T *__restrict__ x) {
     for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
-        T pi = have_ax ? p[i] - rhs[i] : -rhs[i];
-        if (dinv) pi = dinv[i] * pi;
+        T pi;
+        if (have_ax) {
+            pi = cheb3_direction<T>(T(0), rhs[i], p[i], T(0), dinv, i, 1);
+        } else {
+            pi = -rhs[i];
+            if (dinv) pi = dinv[i] * pi;
+        }
         p[i] = pi;
-        x[i] -= alpha * pi;
+        x[i] = __builtin_fma(-alpha, pi, x[i]);
     }
 }
 
@@ -437,9 +442,9 @@ __global__ void cheb3_update_diag_kernel(const ptrdiff_t n, const T alpha, const
                                          const T *__restrict__ Ax, const T *__restrict__ dinv, T *__restrict__ p,
                                          T *__restrict__ x) {
     for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
-        const T pi = beta * p[i] + dinv[i] * (Ax[i] - rhs[i]);
+        const T pi = cheb3_direction<T>(p[i], rhs[i], Ax[i], beta, dinv, i, 0);
         p[i] = pi;
-        x[i] -= alpha * pi;
+        x[i] = __builtin_fma(-alpha, pi, x[i]);
     }
 }
 

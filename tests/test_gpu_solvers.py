@@ -401,3 +401,46 @@ def test_block_jacobi_smoother_parity_and_default_gmg():
     res = true_residual(h.fine.A, dev(rhs.ravel()), xs)
     print(f"elasticity MG-PCG with block-Jacobi smoothing: {solver.iterations} its, residual {res:.2e}")
     assert solver.last_status == 0 and res < 1e-8
+
+
+@pytest.mark.parametrize("L,rt", [(4, S.FLOAT32), (8, S.FLOAT64), (8, S.FLOAT32), (16, S.FLOAT64), (16, S.FLOAT32)])
+@pytest.mark.parametrize("jacobi", [False, True])
+def test_fused_chebyshev_steps_match_separate_kernels(L, rt, jacobi):
+    """Chebyshev3 on a constrained sshex8 elasticity operator: the steps run inside the matrix-core
+    apply (interior nodes) and the skeleton gather (skeleton nodes, Dirichlet rows) must give the
+    words of the separate product + update kernels, for the general-guess start and the updates."""
+    dt = np.float64 if rt == S.FLOAT64 else np.float32
+    el_m, pts_m = O.box_mesh(3, 3, 3)
+    el, pts = O.sshex8_refine(L, el_m, pts_m)
+    N = pts.shape[1]
+    mesh = S.Mesh(S.SSHEX8, L, el, N)
+    adj, det = O.hex8_adj_det(el_m, pts_m)
+    mu, lam = O.lame(1.0, 0.3)
+    coeff = np.exp(np.random.default_rng(L).uniform(-1, 1, el_m.shape[1])).astype(np.float32)
+    op = S.linear_elasticity(mesh, adj, det, mu, lam, coeff, real_type=rt)
+    nodes = np.where(pts[2] < 1e-6)[0].astype(np.int32)
+    A = S.constrained(op, nodes, 3)
+    inv = None
+    if jacobi:
+        d = zeros(3 * N, dt)
+        A.diag(d)
+        inv = dev((1.0 / host(d)).astype(dt), dt)
+    init = np.ones(3 * N, dt)
+    init[(nodes[:, None] * 3 + np.arange(3)).ravel()] = 0
+    sm = S.cheb3(A, max_it=3, scale_eig_max=1.02, init_vector=dev(init, dt), inv_diag=inv)
+    rng = np.random.default_rng(1)
+    rhs = rng.uniform(-1, 1, 3 * N).astype(dt)
+    rhs[(nodes[:, None] * 3 + np.arange(3)).ravel()] = 0
+    x0 = rng.uniform(-1, 1, 3 * N).astype(dt)
+    out = {}
+    try:
+        for fused in (1, 0):
+            S.set_option("fused_chebyshev", fused)
+            x = dev(x0, dt)
+            sm.apply(dev(rhs, dt), x)
+            S.synchronize()
+            out[fused] = host(x)
+    finally:
+        S.set_option("fused_chebyshev", 0)
+    np.testing.assert_array_equal(out[1], out[0])
+    assert np.linalg.norm(out[1] - x0) > 0
