@@ -14,6 +14,16 @@
 
 namespace sfem {
 
+// Extension: a prolongation that adds its result into the fine iterate (y += P x, constrained fine
+// rows unchanged) — the coarse-grid correction in one pass instead of zero + apply + axpby.
+template <typename T>
+class AddingProlongation {
+public:
+    virtual ~AddingProlongation() = default;
+    virtual bool adds_correction() = 0;
+    virtual int apply_add(const T *coarse, T *fine) = 0;
+};
+
 template <typename T>
 class Multigrid final : public Operator<T> {
 public:
@@ -74,6 +84,8 @@ private:
 public:
     // the level operators overwrite their output (as ConjugateGradient::apply_overwrites_output)
     bool apply_overwrites_output{false};
+    // the restrictions overwrite their output (no zero fill of the coarse right-hand side)
+    bool restriction_overwrites_output{false};
 
 private:
     int max_it_{10};
@@ -137,14 +149,19 @@ private:
                 if (nr < (double)atol_) return CYCLE_CONVERGED;
             }
             // restriction
-            blas_->zeros(coarse.rhs->size(), coarse.rhs->data());
+            if (!restriction_overwrites_output) blas_->zeros(coarse.rhs->size(), coarse.rhs->data());
             restriction->apply(mem.work->data(), coarse.rhs->data());
             blas_->zeros(coarse.solution->size(), coarse.solution->data());
             if (cycle(level + 1) == CYCLE_FAILURE) return CYCLE_FAILURE;
             // prolongation and correction
-            blas_->zeros(mem.size(), mem.work->data());
-            prolongation->apply(coarse.solution->data(), mem.work->data());
-            blas_->axpby(mem.size(), T(1), mem.work->data(), T(1), mem.solution->data());
+            auto *ap = dynamic_cast<AddingProlongation<T> *>(prolongation.get());
+            if (ap && ap->adds_correction()) {
+                if (ap->apply_add(coarse.solution->data(), mem.solution->data())) return CYCLE_FAILURE;
+            } else {
+                blas_->zeros(mem.size(), mem.work->data());
+                prolongation->apply(coarse.solution->data(), mem.work->data());
+                blas_->axpby(mem.size(), T(1), mem.work->data(), T(1), mem.solution->data());
+            }
             smoother->apply(mem.rhs->data(), mem.solution->data());
         }
         return CYCLE_CONTINUE;

@@ -96,6 +96,13 @@ struct OpBase {
         set_last_error("constrained gather not available for this operator");
         return SFEM_FAILURE;
     }
+    // Prolongations: y += P x with constrained fine rows unchanged (the multigrid's coarse
+    // correction in one pass), where adds_correction says so.
+    virtual bool adds_correction(hipStream_t) { return false; }
+    virtual int apply_add(const void *, void *, hipStream_t) {
+        set_last_error("adding prolongation not available for this operator");
+        return SFEM_FAILURE;
+    }
     // A Chebyshev3 step run inside the apply (ChebStep: with t = A x, p and x updated in place, t
     // not stored); cmask as in apply_constrained.  Only where cheb3_fusable says so.
     virtual bool cheb3_fusable(hipStream_t) { return false; }

@@ -453,6 +453,8 @@ struct TransferOp final : OpBase {
     bool prolong{true};
     std::shared_ptr<OpBase> constrained;  // ConstrainedOp on the output level (or null)
     DevArray<T> part;                     // restriction: coarse skeleton contributions
+    DevArray<uint8_t> keep;               // adding prolongation: constrained fine dofs (bit per component)
+    bool keep_built{false};
     // the element tile fits the default LDS budget: use the LDS-staged kernels
     bool staged() const {
         return (vs == 1 || vs == 3) && sshex8_transfer_lds_bytes(prolong ? Lc : fine->level, vs) <= 48 * 1024;
@@ -460,9 +462,42 @@ struct TransferOp final : OpBase {
     int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
     ptrdiff_t rows() const override { return prolong ? fine->nnodes * vs : n_coarse * vs; }
     ptrdiff_t cols() const override { return prolong ? n_coarse * vs : fine->nnodes * vs; }
-    int apply(const void *x, void *y, hipStream_t s) override {
-        int err;
+    bool adds_correction(hipStream_t) override { return prolong && staged(); }
+    // y += P x, constrained fine dofs unchanged: the same words as zero + apply (overwriting) +
+    // zeroing the constrained rows + y += that
+    int apply_add(const void *x, void *y, hipStream_t s) override {
+        if (!prolong || !staged()) return OpBase::apply_add(x, y, s);
+        if (mesh_owner(fine, s)) return SFEM_FAILURE;
+        if (!keep_built) {
+            if (constrained) {
+                auto *c = static_cast<ConstrainedOp *>(constrained.get());
+                std::vector<int32_t> d(c->dofs.n);
+                if (!d.empty() && hipMemcpy(d.data(), c->dofs.p, sizeof(int32_t) * d.size(), hipMemcpyDeviceToHost) !=
+                                          hipSuccess) {
+                    set_last_error("prolongation: reading the constrained dofs failed");
+                    return SFEM_FAILURE;
+                }
+                std::vector<uint8_t> k((size_t)fine->nnodes, 0);
+                for (const int32_t q : d)
+                    if (q >= 0 && q / vs < fine->nnodes) k[(size_t)(q / vs)] |= (uint8_t)(1u << (q % vs));
+                if (keep.upload(k.data(), k.size())) return SFEM_FAILURE;
+            }
+            keep_built = true;
+        }
+        return sshex8_prolongate_lds<T>(fine->E, fine->level, Lc, fine->idx.p, fine->own.p, vs, (const T *)x, (T *)y,
+                                        s, keep.p, 1);
+    }
+    int apply_overwrite(const void *x, void *y, hipStream_t s) override {
         if (prolong) {
+            if (!staged()) return OpBase::apply_overwrite(x, y, s);
+            return apply(x, y, s);  // every fine node is written by its owner element
+        }
+        return restrict_impl(x, y, s, 1);
+    }
+    int apply(const void *x, void *y, hipStream_t s) override {
+        if (!prolong) return restrict_impl(x, y, s, 0);
+        int err;
+        {
             if (mesh_owner(fine, s)) return SFEM_FAILURE;
             if (staged())
                 err = sshex8_prolongate_lds<T>(fine->E, fine->level, Lc, fine->idx.p, fine->own.p, vs, (const T *)x,
@@ -470,7 +505,14 @@ struct TransferOp final : OpBase {
             else
                 err = sshex8_prolongate<T>(fine->E, fine->level, Lc, fine->idx.p, fine->own.p, vs, (const T *)x,
                                            (T *)y, s);
-        } else {
+        }
+        if (err) return err;
+        if (constrained) return static_cast<ConstrainedOp *>(constrained.get())->set_value_rt(real_type(), 0.0, y, s);
+        return SFEM_SUCCESS;
+    }
+    int restrict_impl(const void *x, void *y, hipStream_t s, int ow) {
+        int err;
+        {
             // a fine element tile beyond the LDS budget: restrict its 2^3 sub-blocks of level L/2 onto
             // level Lc/2 instead (the same operator: the sub-blocks align with coarse micro elements)
             const sfem_amd_mesh *rf = fine;
@@ -487,9 +529,10 @@ struct TransferOp final : OpBase {
                 if (!part.p && part.alloc((size_t)rf->E * sk_count(rlc) * vs)) return SFEM_FAILURE;
                 const SkeletonMap sk = skeleton_of(rf, rlc);
                 err = sshex8_restrict_lds<T>(rf->E, rf->level, rlc, rf->idx.p, rf->count.p, vs, (const T *)x, (T *)y,
-                                             part.p, sk.pos, s);
-                if (!err) err = skeleton_gather<T>(sk, vs, part.p, (T *)y, s, 0);
+                                             part.p, sk.pos, s, ow);
+                if (!err) err = skeleton_gather<T>(sk, vs, part.p, (T *)y, s, ow);
             } else {
+                if (ow && fill_zero(y, (size_t)rows() * sizeof(T), s)) return SFEM_FAILURE;
                 err = sshex8_restrict<T>(fine->E, fine->level, Lc, fine->idx.p, fine->count.p, vs, (const T *)x,
                                          (T *)y, s);
             }
@@ -536,7 +579,7 @@ struct ChainOp final : OpBase {
 // a multigrid or CG can pass set_initial_guess_zero on to a wrapped smoother or preconditioner;
 // the solver set-up calls are not forwarded (the wrapped object is configured at creation).
 template <typename T>
-struct OpView final : sfem::MatrixFreeLinearSolver<T>, sfem::FusedChebyshevStep<T> {
+struct OpView final : sfem::MatrixFreeLinearSolver<T>, sfem::FusedChebyshevStep<T>, sfem::AddingProlongation<T> {
     std::shared_ptr<OpBase> impl;
     hipStream_t s{nullptr};
     bool overwrite{false};  // apply() overwrites y (the owning solver skips its zero fill)
@@ -551,6 +594,8 @@ struct OpView final : sfem::MatrixFreeLinearSolver<T>, sfem::FusedChebyshevStep<
     void set_initial_guess_zero(bool v) override { impl->set_initial_guess_zero(v); }
     int iterations() const override { return impl->iterations(); }
     bool cheb3_fusable() override { return impl->cheb3_fusable(as_stream(s)); }
+    bool adds_correction() override { return impl->adds_correction(as_stream(s)); }
+    int apply_add(const T *coarse, T *fine) override { return impl->apply_add(coarse, fine, as_stream(s)); }
     int cheb3_step(T *x, const T *rhs, const T *inv_diag, T *p, T alpha, T beta, int first) override {
         return impl->cheb3_step(x, rhs, inv_diag, p, alpha, beta, first, nullptr, as_stream(s));
     }
@@ -1524,8 +1569,9 @@ static int mg_add_level_t(SolverOp<T> *s, sfem_amd_op *op, sfem_amd_op *sm, sfem
     auto *mg = dynamic_cast<sfem::Multigrid<T> *>(s->op.get());
     SFEM_AMD_REQUIRE(mg, "mg_add_level: not a multigrid handle");
     mg->add_level(view<T>(op->impl, true), view<T>(sm->impl), P ? view<T>(P->impl) : nullptr,
-                  R ? view<T>(R->impl) : nullptr);
+                  R ? view<T>(R->impl, true) : nullptr);
     mg->apply_overwrites_output = true;
+    mg->restriction_overwrites_output = true;  // the views call apply_overwrite
     if (s->n == 0) s->n = op->impl->rows();
     return SFEM_SUCCESS;
 }

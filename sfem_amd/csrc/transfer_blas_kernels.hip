@@ -96,7 +96,8 @@ template <typename T, int VS>
 __global__ void __launch_bounds__(256) sshex8_prolongate_lds_kernel(const ptrdiff_t E, const int Lf, const int Lc,
                                                                     const int32_t *__restrict__ idx,
                                                                     const uint8_t *__restrict__ own,
-                                                                    const T *__restrict__ from, T *__restrict__ to) {
+                                                                    const T *__restrict__ from, T *__restrict__ to,
+                                                                    const uint8_t *__restrict__ keep, const int add) {
     extern __shared__ double tr_smem[];
     const ptrdiff_t e = blockIdx.x;
     if (e >= E) return;  // uniform per block
@@ -131,8 +132,14 @@ __global__ void __launch_bounds__(256) sshex8_prolongate_lds_kernel(const ptrdif
             for (int d = 0; d < VS; d++) acc[d] += w * src[d];
         }
         const ptrdiff_t g = row[v];
+        const unsigned km = keep ? keep[g] : 0u;
 #pragma unroll
-        for (int d = 0; d < VS; d++) to[g * VS + d] = (T)acc[d];
+        for (int d = 0; d < VS; d++) {
+            if (!add)
+                to[g * VS + d] = (T)acc[d];
+            else if (!((km >> d) & 1u))
+                to[g * VS + d] = to[g * VS + d] + (T)acc[d];
+        }
     }
 }
 
@@ -147,7 +154,8 @@ __global__ void __launch_bounds__(256) sshex8_restrict_lds_kernel(const ptrdiff_
                                                                   const uint16_t *__restrict__ count,
                                                                   const T *__restrict__ from, T *__restrict__ to,
                                                                   T *__restrict__ part,
-                                                                  const int32_t *__restrict__ cpos) {
+                                                                  const int32_t *__restrict__ cpos,
+                                                                  const int overwrite) {
     extern __shared__ double tr_smem[];
     const ptrdiff_t e = blockIdx.x;
     if (e >= E) return;  // uniform per block
@@ -188,7 +196,7 @@ __global__ void __launch_bounds__(256) sshex8_restrict_lds_kernel(const ptrdiff_
         if (q < 0) {
             const ptrdiff_t cg = row[lidx(Lf, X * s, Y * s, Z * s)];
 #pragma unroll
-            for (int d = 0; d < VS; d++) to[cg * VS + d] += (T)acc[d];
+            for (int d = 0; d < VS; d++) to[cg * VS + d] = overwrite ? (T)acc[d] : to[cg * VS + d] + (T)acc[d];
         } else {
             const ptrdiff_t slot = VS == 1 ? (ptrdiff_t)cpos[e * nskc + q] : (e * nskc + q);
 #pragma unroll
@@ -272,28 +280,34 @@ size_t sshex8_transfer_lds_bytes(int L, int vs) { return sizeof(double) * (size_
 
 template <typename T>
 int sshex8_prolongate_lds(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint8_t *own, int vs, const T *from,
-                          T *to, hipStream_t s) {
+                          T *to, hipStream_t s, const uint8_t *keep, int add) {
     if (E == 0) return SFEM_SUCCESS;
     const size_t lds = sshex8_transfer_lds_bytes(Lc, vs);
     if (vs == 1)
-        sshex8_prolongate_lds_kernel<T, 1><<<(unsigned)E, 256, lds, s>>>(E, Lf, Lc, idx, own, from, to);
+        sshex8_prolongate_lds_kernel<T, 1><<<(unsigned)E, 256, lds, s>>>(E, Lf, Lc, idx, own, from, to, keep, add);
     else if (vs == 3)
-        sshex8_prolongate_lds_kernel<T, 3><<<(unsigned)E, 256, lds, s>>>(E, Lf, Lc, idx, own, from, to);
-    else
+        sshex8_prolongate_lds_kernel<T, 3><<<(unsigned)E, 256, lds, s>>>(E, Lf, Lc, idx, own, from, to, keep, add);
+    else if (!add)
         return sshex8_prolongate<T>(E, Lf, Lc, idx, own, vs, from, to, s);
+    else {
+        set_last_error("sshex8_prolongate_lds: adding form needs vector size 1 or 3");
+        return SFEM_FAILURE;
+    }
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
 }
 
 template <typename T>
 int sshex8_restrict_lds(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint16_t *count, int vs, const T *from,
-                        T *to, T *part, const int32_t *cpos, hipStream_t s) {
+                        T *to, T *part, const int32_t *cpos, hipStream_t s, int overwrite) {
     if (E == 0) return SFEM_SUCCESS;
     const size_t lds = sshex8_transfer_lds_bytes(Lf, vs);
     if (vs == 1)
-        sshex8_restrict_lds_kernel<T, 1><<<(unsigned)E, 256, lds, s>>>(E, Lf, Lc, idx, count, from, to, part, cpos);
+        sshex8_restrict_lds_kernel<T, 1>
+                <<<(unsigned)E, 256, lds, s>>>(E, Lf, Lc, idx, count, from, to, part, cpos, overwrite);
     else if (vs == 3)
-        sshex8_restrict_lds_kernel<T, 3><<<(unsigned)E, 256, lds, s>>>(E, Lf, Lc, idx, count, from, to, part, cpos);
+        sshex8_restrict_lds_kernel<T, 3>
+                <<<(unsigned)E, 256, lds, s>>>(E, Lf, Lc, idx, count, from, to, part, cpos, overwrite);
     else {
         set_last_error("sshex8_restrict_lds: vector size 1 or 3");
         return SFEM_FAILURE;
@@ -311,13 +325,13 @@ template int sshex8_restrict<double>(ptrdiff_t, int, int, const int32_t *, const
 template int sshex8_restrict<float>(ptrdiff_t, int, int, const int32_t *, const uint16_t *, int, const float *,
                                     float *, hipStream_t);
 template int sshex8_prolongate_lds<double>(ptrdiff_t, int, int, const int32_t *, const uint8_t *, int, const double *,
-                                           double *, hipStream_t);
+                                           double *, hipStream_t, const uint8_t *, int);
 template int sshex8_prolongate_lds<float>(ptrdiff_t, int, int, const int32_t *, const uint8_t *, int, const float *,
-                                          float *, hipStream_t);
+                                          float *, hipStream_t, const uint8_t *, int);
 template int sshex8_restrict_lds<double>(ptrdiff_t, int, int, const int32_t *, const uint16_t *, int, const double *,
-                                         double *, double *, const int32_t *, hipStream_t);
+                                         double *, double *, const int32_t *, hipStream_t, int);
 template int sshex8_restrict_lds<float>(ptrdiff_t, int, int, const int32_t *, const uint16_t *, int, const float *,
-                                        float *, float *, const int32_t *, hipStream_t);
+                                        float *, float *, const int32_t *, hipStream_t, int);
 
 int node_owner_mask(ptrdiff_t E, int nxe, ptrdiff_t nnodes, const int32_t *idx, uint8_t *own, hipStream_t s) {
     int32_t *owner = nullptr;

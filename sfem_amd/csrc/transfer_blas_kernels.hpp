@@ -19,11 +19,14 @@ int sshex8_restrict(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint1
 // positions cpos; vector: element-major slots), so it adds no atomics.
 size_t sshex8_transfer_lds_bytes(int L, int vs);
 template <typename T>
+// add: to += P from instead of to = P from, leaving the components flagged in keep[node] (bit per
+// component, optional) unchanged (a coarse correction added to a fine iterate)
 int sshex8_prolongate_lds(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint8_t *own, int vs, const T *from,
-                          T *to, hipStream_t s);
+                          T *to, hipStream_t s, const uint8_t *keep = nullptr, int add = 0);
 template <typename T>
+// overwrite: the coarse interior nodes are set, not accumulated (gather the skeleton with overwrite too)
 int sshex8_restrict_lds(ptrdiff_t E, int Lf, int Lc, const int32_t *idx, const uint16_t *count, int vs, const T *from,
-                        T *to, T *part, const int32_t *cpos, hipStream_t s);
+                        T *to, T *part, const int32_t *cpos, hipStream_t s, int overwrite = 0);
 int node_owner_mask(ptrdiff_t E, int nxe, ptrdiff_t nnodes, const int32_t *idx, uint8_t *own, hipStream_t s);
 int node_incidence(ptrdiff_t n_entries, ptrdiff_t nnodes, const int32_t *idx, uint16_t *count, hipStream_t s);
 int sshex8_derefine(ptrdiff_t E, int Lf, int Lc, const int32_t *fine, int32_t *coarse, hipStream_t s);
