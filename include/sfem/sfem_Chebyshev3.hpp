@@ -51,6 +51,7 @@ public:
     }
     int iterations() const override { return iterations_; }
     void set_initial_guess_zero(bool v) override { is_initial_guess_zero = v; }
+    bool zero_guess_ignores_x() const override { return true; }  // the first step writes x
     void set_op(const SharedOperator<T> &o) override {
         op = o;
         n_dofs = o->rows();

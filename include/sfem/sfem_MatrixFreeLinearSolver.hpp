@@ -14,6 +14,9 @@ public:
     virtual void set_max_it(int it) = 0;
     virtual void set_n_dofs(std::ptrdiff_t n) = 0;
     virtual void set_initial_guess_zero(bool) {}
+    // Extension: with set_initial_guess_zero(true) the solver does not read x (the caller may skip
+    // zeroing it).
+    virtual bool zero_guess_ignores_x() const { return false; }
     virtual int iterations() const = 0;
 };
 

@@ -151,7 +151,12 @@ private:
             // restriction
             if (!restriction_overwrites_output) blas_->zeros(coarse.rhs->size(), coarse.rhs->data());
             restriction->apply(mem.work->data(), coarse.rhs->data());
-            blas_->zeros(coarse.solution->size(), coarse.solution->data());
+            // the coarse solution starts at zero: the coarsest level zeroes it itself, and a smoother
+            // that does not read x in its zero-guess mode needs no fill either
+            const bool coarsest_next = level + 1 == n_levels() - 1;
+            auto *csm = coarsest_next ? nullptr : dynamic_cast<MatrixFreeLinearSolver<T> *>(smoother_[level + 1].get());
+            if (!coarsest_next && !(csm && csm->zero_guess_ignores_x()))
+                blas_->zeros(coarse.solution->size(), coarse.solution->data());
             if (cycle(level + 1) == CYCLE_FAILURE) return CYCLE_FAILURE;
             // prolongation and correction
             auto *ap = dynamic_cast<AddingProlongation<T> *>(prolongation.get());

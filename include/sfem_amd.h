@@ -232,7 +232,8 @@ int sfem_amd_cheb3_update(int rt, ptrdiff_t n, double alpha, double beta, const 
 int sfem_amd_cheb3_update_diag(int rt, ptrdiff_t n, double alpha, double beta, const void *rhs, const void *Ax,
                                const void *inv_diag, void *p, void *x, void *stream);
 /* Chebyshev3 iteration 0 (algebra/sfem_Chebyshev3.hpp:208-217) in one pass: p = p - rhs when p holds
- * A x (have_ax), else p = -rhs (zero initial guess); p *= inv_diag when inv_diag != NULL; x -= alpha p. */
+ * A x (have_ax), else p = -rhs (zero initial guess); p *= inv_diag when inv_diag != NULL; x -= alpha p
+ * (with a zero initial guess x = -alpha p: x is output only). */
 int sfem_amd_cheb3_start(int rt, ptrdiff_t n, double alpha, const void *rhs, const void *inv_diag, int have_ax,
                          void *p, void *x, void *stream);
 

@@ -114,6 +114,8 @@ struct OpBase {
     // Solvers and smoothers: the output holds zeros on entry to the next applies (work that
     // multiplies the initial guess may be skipped).  Other operators ignore it.
     virtual void set_initial_guess_zero(bool) {}
+    // with set_initial_guess_zero(true) the output's contents are not read
+    virtual bool zero_guess_ignores_x() const { return false; }
     virtual int iterations() const { return -1; }
     virtual int last_status() const { return SFEM_SUCCESS; }
     virtual const std::vector<double> *history() const { return nullptr; }

@@ -373,6 +373,26 @@ struct ElasticityOp final : OpBase {
 };
 
 // ----------------------------------------------------------------- constraints
+// The dofs (node * bs + component) as bits on the skeleton entries of sk; fails when a dof's node
+// is not a skeleton node (or bs > 8).
+static int skeleton_dof_mask(const SkeletonMap &sk, int bs, const DevArray<int32_t> &dofs, DevArray<uint8_t> &mask_out,
+                             hipStream_t s) {
+    if (bs > 8 || sk.n == 0) return SFEM_FAILURE;
+    std::vector<int32_t> node((size_t)sk.n), d(dofs.n);
+    if (hipStreamSynchronize(s) != hipSuccess ||
+        hipMemcpy(node.data(), sk.node, sizeof(int32_t) * node.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+        (!d.empty() && hipMemcpy(d.data(), dofs.p, sizeof(int32_t) * d.size(), hipMemcpyDeviceToHost) != hipSuccess))
+        return SFEM_FAILURE;
+    std::vector<uint8_t> mask((size_t)sk.n, 0);
+    for (const int32_t q : d) {
+        const int32_t g = q / bs;
+        const auto it = std::lower_bound(node.begin(), node.end(), g);  // skeleton nodes ascend
+        if (it == node.end() || *it != g) return SFEM_FAILURE;         // an interior dof
+        mask[(size_t)(it - node.begin())] |= (uint8_t)(1u << (q % bs));
+    }
+    return mask_out.upload(mask.data(), mask.size());
+}
+
 struct ConstrainedOp final : OpBase {
     std::shared_ptr<OpBase> inner;
     DevArray<int32_t> dofs;
@@ -391,22 +411,7 @@ struct ConstrainedOp final : OpBase {
     int build_cmask(hipStream_t s) {
         const sfem_amd_mesh *gm = inner->gather_mesh(s);
         if (!gm || mesh_skeleton(gm, 0)) return SFEM_FAILURE;
-        const SkeletonMap sk = skeleton_of(gm);
-        const int bs = inner->block_size();
-        if (bs > 8 || sk.n == 0) return SFEM_FAILURE;
-        std::vector<int32_t> node((size_t)sk.n), d(dofs.n);
-        if (hipStreamSynchronize(s) != hipSuccess ||
-            hipMemcpy(node.data(), sk.node, sizeof(int32_t) * node.size(), hipMemcpyDeviceToHost) != hipSuccess ||
-            (!d.empty() && hipMemcpy(d.data(), dofs.p, sizeof(int32_t) * d.size(), hipMemcpyDeviceToHost) != hipSuccess))
-            return SFEM_FAILURE;
-        std::vector<uint8_t> mask((size_t)sk.n, 0);
-        for (const int32_t q : d) {
-            const int32_t g = q / bs;
-            const auto it = std::lower_bound(node.begin(), node.end(), g);  // skeleton nodes ascend
-            if (it == node.end() || *it != g) return SFEM_FAILURE;         // an interior dof
-            mask[(size_t)(it - node.begin())] |= (uint8_t)(1u << (q % bs));
-        }
-        return cmask.upload(mask.data(), mask.size());
+        return skeleton_dof_mask(skeleton_of(gm), inner->block_size(), dofs, cmask, s);
     }
     int apply(const void *x, void *y, hipStream_t s) override {
         if (fused(s)) return inner->apply_constrained(x, y, cmask.p, 0, s);
@@ -455,6 +460,8 @@ struct TransferOp final : OpBase {
     DevArray<T> part;                     // restriction: coarse skeleton contributions
     DevArray<uint8_t> keep;               // adding prolongation: constrained fine dofs (bit per component)
     bool keep_built{false};
+    DevArray<uint8_t> rmask;              // restriction: constrained coarse dofs on the coarse skeleton
+    int rmask_state{0};                   // 0 not built, 1 the gather zeroes them, -1 separate zeroing
     // the element tile fits the default LDS budget: use the LDS-staged kernels
     bool staged() const {
         return (vs == 1 || vs == 3) && sshex8_transfer_lds_bytes(prolong ? Lc : fine->level, vs) <= 48 * 1024;
@@ -527,10 +534,19 @@ struct TransferOp final : OpBase {
             if (rf != fine || staged()) {
                 if (mesh_skeleton(rf, rlc)) return SFEM_FAILURE;
                 if (!part.p && part.alloc((size_t)rf->E * sk_count(rlc) * vs)) return SFEM_FAILURE;
-                const SkeletonMap sk = skeleton_of(rf, rlc);
+                SkeletonMap sk = skeleton_of(rf, rlc);
+                if (constrained && rmask_state == 0) {
+                    rmask_state = skeleton_dof_mask(sk, vs, static_cast<ConstrainedOp *>(constrained.get())->dofs,
+                                                    rmask, s)
+                                          ? -1
+                                          : 1;
+                    (void)hipGetLastError();
+                }
+                if (rmask_state > 0) sk.cmask = rmask.p;  // the gather writes 0 at constrained coarse dofs
                 err = sshex8_restrict_lds<T>(rf->E, rf->level, rlc, rf->idx.p, rf->count.p, vs, (const T *)x, (T *)y,
                                              part.p, sk.pos, s, ow);
                 if (!err) err = skeleton_gather<T>(sk, vs, part.p, (T *)y, s, ow);
+                if (!err && rmask_state > 0) return SFEM_SUCCESS;
             } else {
                 if (ow && fill_zero(y, (size_t)rows() * sizeof(T), s)) return SFEM_FAILURE;
                 err = sshex8_restrict<T>(fine->E, fine->level, Lc, fine->idx.p, fine->count.p, vs, (const T *)x,
@@ -592,6 +608,7 @@ struct OpView final : sfem::MatrixFreeLinearSolver<T>, sfem::FusedChebyshevStep<
     void set_max_it(int) override {}
     void set_n_dofs(std::ptrdiff_t) override {}
     void set_initial_guess_zero(bool v) override { impl->set_initial_guess_zero(v); }
+    bool zero_guess_ignores_x() const override { return impl->zero_guess_ignores_x(); }
     int iterations() const override { return impl->iterations(); }
     bool cheb3_fusable() override { return impl->cheb3_fusable(as_stream(s)); }
     bool adds_correction() override { return impl->adds_correction(as_stream(s)); }
@@ -626,6 +643,10 @@ struct SolverOp final : OpBase {
             m->set_initial_guess_zero(v);
         else if (auto *mg = dynamic_cast<sfem::Multigrid<T> *>(op.get()))
             mg->set_initial_guess_zero(v);
+    }
+    bool zero_guess_ignores_x() const override {
+        auto *m = dynamic_cast<const sfem::MatrixFreeLinearSolver<T> *>(op.get());
+        return m && m->zero_guess_ignores_x();
     }
     int iterations() const override { return its ? its() : -1; }
     int last_status() const override { return status ? status() : SFEM_SUCCESS; }
@@ -797,6 +818,9 @@ struct SwitchOp final : OpBase {
     void set_initial_guess_zero(bool v) override {
         ops[0]->set_initial_guess_zero(v);
         ops[1]->set_initial_guess_zero(v);
+    }
+    bool zero_guess_ignores_x() const override {
+        return ops[0]->zero_guess_ignores_x() && ops[1]->zero_guess_ignores_x();
     }
 };
 }  // namespace

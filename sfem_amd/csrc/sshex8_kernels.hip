@@ -239,7 +239,7 @@ struct MfmaAcc<float> {
 // y[node[t]] += sum of part[ent[off[t] .. off[t+1])] * bs components: the skeleton nodes'
 // element contributions, in slot order.
 // With cmask: components flagged in cmask[t] are Dirichlet rows, set to cx (the constrained
-// operator's y = x there) instead.
+// operator's y = x there), or to zero without cx (a restriction onto constrained dofs), instead.
 // With a Chebyshev step (cs.p != null, overwriting apply): the node's product t feeds the step
 // (p and x updated there, y not written).
 template <typename T, int BS>
@@ -261,7 +261,7 @@ __global__ void skeleton_gather_kernel(const ptrdiff_t n, const int32_t *__restr
         const unsigned cm = cmask ? cmask[t] : 0u;
 #pragma unroll
         for (int c = 0; c < BS; c++) {
-            const T v = ((cm >> c) & 1u) ? cx[g + c] : (overwrite ? acc[c] : y[g + c] + acc[c]);
+            const T v = ((cm >> c) & 1u) ? (cx ? cx[g + c] : T(0)) : (overwrite ? acc[c] : y[g + c] + acc[c]);
             if (cs.p) {
                 const T pi = cheb3_direction<T>(cs.first ? T(0) : cs.p[g + c], cs.rhs[g + c], v, cs.beta, cs.dinv, g + c,
                                                 cs.first);
@@ -892,7 +892,7 @@ int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStrea
     // partials node-sorted, vector partials element-major
     if (node_sorted < 0) node_sorted = bs == 1;
     const int32_t *ent = node_sorted ? nullptr : sk.ent;
-    const uint8_t *cm = cx ? sk.cmask : nullptr;
+    const uint8_t *cm = sk.cmask;
     if (bs == 1)
         skeleton_gather_kernel<T, 1>
                 <<<grid_for(sk.n), kBlock, 0, s>>>(sk.n, sk.node, sk.off, ent, part, y, overwrite, cm, cx, cs);

@@ -27,7 +27,8 @@ struct SkeletonMap {
 template <typename T>
 // node_sorted: partial j of the node-sorted list at part[j * bs] (1) or at part[ent[j] * bs] (0);
 // -1: scalar partials node-sorted, vector partials element-major
-// cx: the operator's input, read at the constrained rows of sk.cmask (unused without it); cs: a
+// cx: the operator's input, read at the constrained rows of sk.cmask (null: those rows are set to
+// zero); cs: a
 // Chebyshev3 step fed by the (overwritten) product instead of storing y
 int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStream_t s, int overwrite = 0,
                     int node_sorted = -1, const T *cx = nullptr, const ChebStep<T> &cs = ChebStep<T>());

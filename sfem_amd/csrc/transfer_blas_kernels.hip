@@ -434,20 +434,22 @@ __global__ void cheb3_update_kernel(const ptrdiff_t n, const T alpha, const T be
 
 // diagonally scaled variant: p = beta p + dinv (A x - rhs); x -= alpha p
 // Chebyshev3 iteration 0: p = (p holds A x ? p - rhs : -rhs), times dinv when given; x -= alpha p
+// (x = -alpha p for a zero initial guess, whatever x holds)
 template <typename T>
 __global__ void cheb3_start_kernel(const ptrdiff_t n, const T alpha, const T *__restrict__ rhs,
                                    const T *__restrict__ dinv, const int have_ax, T *__restrict__ p,
                                    T *__restrict__ x) {
     for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
-        T pi;
         if (have_ax) {
-            pi = cheb3_direction<T>(T(0), rhs[i], p[i], T(0), dinv, i, 1);
-        } else {
-            pi = -rhs[i];
+            const T pi = cheb3_direction<T>(T(0), rhs[i], p[i], T(0), dinv, i, 1);
+            p[i] = pi;
+            x[i] = __builtin_fma(-alpha, pi, x[i]);
+        } else {  // zero initial guess: x's contents are not read
+            T pi = -rhs[i];
             if (dinv) pi = dinv[i] * pi;
+            p[i] = pi;
+            x[i] = -(alpha * pi);
         }
-        p[i] = pi;
-        x[i] = __builtin_fma(-alpha, pi, x[i]);
     }
 }
 
