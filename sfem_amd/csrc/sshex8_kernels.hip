@@ -840,6 +840,187 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Small levels (L = 2: 8 micro elements): the colour-by-colour scatter above would fill 1/16 of
+// each 16-column tile and pay a barrier per colour.  Here one wave takes one macro
+// element and runs all of its micro elements through the GEMM at once (one or four tiles), the 24
+// results of each micro element go to an LDS table Y[micro][row], and each node then sums its
+// (at most 8) entries in a fixed order (z, y, x of the micro element ascending): deterministic, no
+// atomics.  FP32: bf16x3 products as in the colour kernel (same packed A operands); FP64:
+// v_mfma_f64_16x16x4f64.  Interior nodes are stored (or fed to a Chebyshev step), skeleton nodes go
+// to their node-sorted partial slots.
+template <typename T, int L>
+struct ElSmall {
+    static constexpr int N1 = L + 1, NXE = N1 * N1 * N1, NM = L * L * L, TILES = (NM + 15) / 16;
+    static constexpr int EPB = 4;                                  // one wave per macro element
+    static constexpr int NPT = (EPB * NXE + 255) / 256;            // nodes per thread
+    static constexpr int XW = std::is_same<T, double>::value ? 2 : 1;  // x words per value in LDS
+};
+
+template <typename T, int L>
+__global__ void __launch_bounds__(256)
+    sshex8_elasticity_apply_small_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx, const T *__restrict__ Kall,
+                                         const el_u4 *__restrict__ Apk, const T *x, T *__restrict__ y,
+                                         T *__restrict__ part, const int overwrite, const int kshift,
+                                         const int32_t *__restrict__ sk_pos, const ChebStep<T> cs) {
+    using S = ElSmall<T, L>;
+    constexpr int N1 = S::N1, NXE = S::NXE, NM = S::NM, EPB = S::EPB, NPT = S::NPT;
+    constexpr int NSK = sk_count(L);
+    constexpr bool F64 = std::is_same<T, double>::value;
+    typedef typename std::conditional<F64, double, uint32_t>::type XWord;  // FP32: bf16 hi | lo
+    __shared__ XWord sx[EPB][3][NXE];
+    __shared__ T sY[EPB][NM][24];
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int c = lane & 15, g = lane >> 4;
+    const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
+
+    int32_t gid[NPT], pos[NPT];
+    T xk[NPT][3], yk[NPT][3];
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int k = tid + 256 * i, le = k / NXE;
+        gid[i] = (k < EPB * NXE && e0 + le < E) ? idx[e0 * NXE + k] : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int k = tid + 256 * i, le = k / NXE, v = k - le * NXE;
+        const int slot = sk_slot_table<L>.v[v];
+        const bool interior = slot < 0;
+        pos[i] = (gid[i] >= 0 && !interior) ? sk_pos[(e0 + le) * NSK + slot] : -1;
+#pragma unroll
+        for (int ci = 0; ci < 3; ci++) {
+            xk[i][ci] = gid[i] >= 0 ? x[(ptrdiff_t)gid[i] * 3 + ci] : T(0);
+            yk[i][ci] = (gid[i] >= 0 && interior && !overwrite && !cs.p) ? y[(ptrdiff_t)gid[i] * 3 + ci] : T(0);
+        }
+        if (k < EPB * NXE) {
+#pragma unroll
+            for (int ci = 0; ci < 3; ci++) {
+                if constexpr (F64)
+                    sx[le][ci][v] = xk[i][ci];
+                else
+                    sx[le][ci][v] = el_split_pack(xk[i][ci]);
+            }
+        }
+    }
+
+    const int le = wave;
+    const ptrdiff_t e = e0 + le;
+    const bool active = e < E;
+    const ptrdiff_t ek = active ? (e >> kshift) : 0;
+    // A operands of the element's matrix
+    el_b8 ahi[2], alo[2];
+    double a64[2][6];
+    if constexpr (F64) {
+#pragma unroll
+        for (int tt = 0; tt < 2; tt++)
+#pragma unroll
+            for (int st = 0; st < 6; st++) {
+                const int row = 16 * tt + c;
+                a64[tt][st] = (active && row < 24) ? (double)Kall[ek * 576 + row * 24 + 4 * st + g] : 0.0;
+            }
+    } else {
+        const el_u4 *ak = Apk + ek * 256 + lane;
+#pragma unroll
+        for (int tt = 0; tt < 2; tt++) {
+            ahi[tt] = __builtin_bit_cast(el_b8, ak[(2 * tt) * 64]);
+            alo[tt] = __builtin_bit_cast(el_b8, ak[(2 * tt + 1) * 64]);
+        }
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int t = 0; t < S::TILES; t++) {
+        const int q = 16 * t + c;
+        const bool valid = active && q < NM;
+        const int qq = valid ? q : 0;
+        const int mx = qq % L, my = (qq / L) % L, mz = qq / (L * L);
+        const int base = lidx(L, mx, my, mz);
+        if constexpr (F64) {
+            using M = MfmaAcc<double>;
+            typename M::v4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
+#pragma unroll
+            for (int st = 0; st < 6; st++) {
+                const int kk = 4 * st + g, j = kk & 7;  // column: component kk / 8, corner kk % 8
+                const double b = sx[le][kk >> 3][base + lidx(L, hex8_cx(j), hex8_cy(j), hex8_cz(j))];
+                d0 = M::step(a64[0][st], b, d0);
+                d1 = M::step(a64[1][st], b, d1);
+            }
+            if (valid) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int R0 = M::row(g, r), R1 = 16 + M::row(g, r);
+                    sY[le][q][R0] = d0[r];
+                    if (R1 < 24) sY[le][q][R1] = d1[r];
+                }
+            }
+        } else {
+            const int gc = g < 3 ? g : 2;  // lanes g = 3 meet zero columns of A
+            uint32_t pk[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) pk[j] = sx[le][gc][base + lidx(L, hex8_cx(j), hex8_cy(j), hex8_cz(j))];
+            el_b8 bhi, blo;
+            el_unpack8(pk, bhi, blo);
+#pragma unroll
+            for (int tt = 0; tt < 2; tt++) {
+                el_f4 d = {0.f, 0.f, 0.f, 0.f};
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[tt], bhi, d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[tt], blo, d, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[tt], bhi, d, 0, 0, 0);
+                const int R0 = 16 * tt + 4 * g;  // rows R0 .. R0 + 3
+                if (valid && R0 < 24) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) sY[le][q][R0 + r] = d[r];
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int i = 0; i < NPT; i++) {
+        const int k = tid + 256 * i, le2 = k / NXE, v = k - le2 * NXE;
+        if (gid[i] < 0) continue;
+        const int xx = v % N1, yy = (v / N1) % N1, zz = v / (N1 * N1);
+        T acc[3] = {0, 0, 0};
+#pragma unroll
+        for (int bz = 0; bz < 2; bz++) {
+            const int mz = zz - 1 + bz;
+            if (mz < 0 || mz >= L) continue;
+#pragma unroll
+            for (int by = 0; by < 2; by++) {
+                const int my = yy - 1 + by;
+                if (my < 0 || my >= L) continue;
+#pragma unroll
+                for (int bx = 0; bx < 2; bx++) {
+                    const int mx = xx - 1 + bx;
+                    if (mx < 0 || mx >= L) continue;
+                    const int m = (mz * L + my) * L + mx;
+                    const int j = hex8_corner(1 - bx, 1 - by, 1 - bz);  // the node's corner in micro element m
+#pragma unroll
+                    for (int ci = 0; ci < 3; ci++) acc[ci] += sY[le2][m][ci * 8 + j];
+                }
+            }
+        }
+        if (pos[i] < 0) {
+#pragma unroll
+            for (int ci = 0; ci < 3; ci++) {
+                const ptrdiff_t qd = (ptrdiff_t)gid[i] * 3 + ci;
+                if (cs.p) {
+                    const T pi = cheb3_direction<T>(cs.first ? T(0) : cs.p[qd], cs.rhs[qd], acc[ci], cs.beta, cs.dinv,
+                                                    qd, cs.first);
+                    cs.p[qd] = pi;
+                    cs.x[qd] = __builtin_fma(-cs.alpha, pi, xk[i][ci]);
+                } else {
+                    y[qd] = overwrite ? acc[ci] : yk[i][ci] + acc[ci];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int ci = 0; ci < 3; ci++) part[(ptrdiff_t)pos[i] * 3 + ci] = acc[ci];
+        }
+    }
+}
+
 // d += diagonal (block = 0) or symmetric 3x3 block diagonal [node][6] (block = 1).
 template <typename T, int L>
 __global__ void sshex8_elasticity_diag_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
@@ -1011,6 +1192,19 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
                                      (std::is_same<T, float>::value && L % 2 == 0 && L <= 12)))) {
         set_last_error("sshex8_elasticity_apply: shared matrices need the matrix-core kernels");
         return SFEM_FAILURE;
+    }
+    // L = 2 on the all-at-once kernel (37.8 vs 110 us FP64, 24 vs 47 us FP32 for 32^3 macro elements);
+    // at L = 4 its 64-entry result table costs more LDS occupancy than the colours save (176 vs
+    // 135 us FP64), so L >= 4 keeps the colour kernels
+    if (matrix_cores && L == 2) {
+        if (!std::is_same<T, double>::value && !Apk) {
+            set_last_error("sshex8_elasticity_apply: the FP32 matrix-core kernel needs packed matrices");
+            return SFEM_FAILURE;
+        }
+        sshex8_elasticity_apply_small_kernel<T, 2><<<(unsigned)((E + 3) / 4), 256, 0, s>>>(
+                E, idx, K, (const el_u4 *)Apk, x, y, part, overwrite, kshift, sk.pos, cs);
+        SFEM_AMD_HIP_CHECK(hipGetLastError());
+        return skeleton_gather<T>(sk, 3, part, y, s, overwrite, 1, x, cs);  // node-sorted partials
     }
     if constexpr (std::is_same<T, double>::value) {
         if (matrix_cores && (L == 2 || L == 4 || L == 6 || L == 8)) {

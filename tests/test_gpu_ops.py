@@ -193,7 +193,7 @@ def bf16_round(a):
     return u.astype(np.uint32).view(np.float32)
 
 
-@pytest.mark.parametrize("L,n", [(4, 3), (4, 2), (6, 2), (8, 2), (12, 1), (16, 1)])
+@pytest.mark.parametrize("L,n", [(2, 3), (4, 3), (4, 2), (6, 2), (8, 2), (12, 1), (16, 1)])
 def test_sshex8_elasticity_fp32_matrix_cores(L, n):
     """The FP32 sshex8 elasticity operator runs on the matrix cores (bf16 hi/lo split products,
     FP32 accumulation).  On the same BF16-rounded geometry as the FP64 oracle it must agree to the
