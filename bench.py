@@ -12,6 +12,7 @@ RHS ~ U[0,1]), and the CPU baseline: the oracle's OpenMP apply on the host cores
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -35,6 +36,21 @@ def build_problem(n_macro=16, L=8):
 def progress(msg):
     """Stage messages on stderr (the JSON result is the only stdout line)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+PROFILE = os.path.join(ROOT, "profiles", "sshex8_laplacian_apply_L8_16cube.md")
+
+
+def profiled_traffic():
+    """HBM bytes per apply of this workload measured by rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE,
+    calibrated with a copy kernel of known size), read from the committed profile summary that
+    states the command; None without it."""
+    try:
+        with open(PROFILE) as f:
+            m = re.search(r"Total per apply: ([0-9.]+) MB", f.read())
+        return float(m.group(1)) * 1e6 if m else None
+    except OSError:
+        return None
 
 
 def algorithmic_bytes(N, E, L, value_bytes=8, geom_bytes=4):
@@ -360,7 +376,9 @@ def main():
                                f"({N} DOF, {E} macro elements), FP64 vectors, float32 FFF, int32 ids",
                    "dofs": N, "macro_elements": E, "level": L},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
+                     "traffic_source": "profiles/sshex8_laplacian_apply_L8_16cube.md (rocprofv3 --pmc FETCH_SIZE, "
+                                       "WRITE_SIZE per dispatch, both kernels of one apply)",
                      "algorithmic_bytes_per_apply": B,
                      "note": "working set 46 MB fits the 256 MB MALL: back-to-back applies are cache-hot; "
                              "see roofline_cold for caches overwritten before each apply"},
