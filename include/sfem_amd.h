@@ -343,7 +343,9 @@ int sfem_amd_dense_op_create(int real_type, ptrdiff_t n, const double *matrix, s
  * (x, y, stream) relaunch the graph and others run eagerly.  For launch-bound sequences without
  * host synchronisation, e.g. a preconditioning V-cycle (max_it 1) with a dense coarse solve. */
 int sfem_amd_graph_op_create(sfem_amd_op *op, sfem_amd_op **out);
-/* captured: 1 graph instantiated, 0 not yet, -1 capture failed (eager); replays: graph launches. */
+/* captured: graphs instantiated (one per initial-guess mode: the zero-guess apply a CG makes of its
+ * preconditioner records a different sequence), 0 not yet, -1 capture failed (eager); replays:
+ * graph launches. */
 int sfem_amd_graph_op_stats(const sfem_amd_op *op, int *captured, int *replays);
 /* An op that runs `primary` until switched to `fallback` (same size and real type). */
 int sfem_amd_switch_create(sfem_amd_op *primary, sfem_amd_op *fallback, sfem_amd_op **out);

@@ -747,23 +747,29 @@ struct DenseOp final : OpBase {
     }
 };
 
-// Replays the apply of `inner` as a HIP graph.  The first call runs eagerly (lazy allocations,
-// one-time kernel attributes); the second captures the launch sequence on its stream and
-// launches the graph; later calls with the same (x, y, stream) relaunch it, other arguments run
+// Replays the apply of `inner` as a HIP graph.  Per initial-guess mode (set_initial_guess_zero,
+// which changes the recorded sequence) the first call runs eagerly (lazy allocations, one-time
+// kernel attributes), the second captures the launch sequence on its stream and launches the
+// graph; later calls with the same (x, y, stream) and mode relaunch it, other arguments run
 // eagerly.  The inner apply must not synchronise with the host (e.g. a preconditioning V-cycle
 // with a direct coarse solve).  A failed capture leaves the operator eager.
 struct GraphOp final : OpBase {
     std::shared_ptr<OpBase> inner;
-    hipGraphExec_t exec{nullptr};
-    const void *cx{nullptr};
-    void *cy{nullptr};
-    hipStream_t cs{nullptr};
-    int calls{0}, replays{0};
+    struct Capture {
+        hipGraphExec_t exec{nullptr};
+        const void *cx{nullptr};
+        void *cy{nullptr};
+        hipStream_t cs{nullptr};
+        int calls{0};
+    } cap[2];  // [mode]: general / zero initial guess
+    int replays{0};
     bool failed{false};
-    bool zero_guess{false}, czero{false};  // current / captured set_initial_guess_zero
+    bool zero_guess{false};
     ~GraphOp() override {
-        if (exec) (void)hipGraphExecDestroy(exec);
+        for (auto &c : cap)
+            if (c.exec) (void)hipGraphExecDestroy(c.exec);
     }
+    int captured() const { return (cap[0].exec ? 1 : 0) + (cap[1].exec ? 1 : 0); }
     int real_type() const override { return inner->real_type(); }
     ptrdiff_t rows() const override { return inner->rows(); }
     ptrdiff_t cols() const override { return inner->cols(); }
@@ -773,8 +779,9 @@ struct GraphOp final : OpBase {
         inner->set_initial_guess_zero(v);
     }
     int apply(const void *x, void *y, hipStream_t s) override {
-        if (calls++ == 0 || failed) return inner->apply(x, y, s);
-        if (!exec) {
+        Capture &c = cap[zero_guess ? 1 : 0];
+        if (c.calls++ == 0 || failed) return inner->apply(x, y, s);
+        if (!c.exec) {
             if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
                 failed = true;
                 (void)hipGetLastError();
@@ -784,22 +791,21 @@ struct GraphOp final : OpBase {
             hipGraph_t graph = nullptr;
             const hipError_t e_end = hipStreamEndCapture(s, &graph);
             if (err || e_end != hipSuccess || !graph ||
-                hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+                hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0) != hipSuccess) {
                 if (graph) (void)hipGraphDestroy(graph);
-                exec = nullptr;
+                c.exec = nullptr;
                 failed = true;
                 (void)hipGetLastError();
                 return inner->apply(x, y, s);  // nothing ran during the capture: run it now
             }
             (void)hipGraphDestroy(graph);
-            cx = x;
-            cy = y;
-            cs = s;
-            czero = zero_guess;
+            c.cx = x;
+            c.cy = y;
+            c.cs = s;
         }
-        if (x == cx && y == cy && s == cs && zero_guess == czero) {
+        if (x == c.cx && y == c.cy && s == c.cs) {
             replays++;
-            SFEM_AMD_HIP_CHECK(hipGraphLaunch(exec, s));
+            SFEM_AMD_HIP_CHECK(hipGraphLaunch(c.exec, s));
             return SFEM_SUCCESS;
         }
         return inner->apply(x, y, s);
@@ -1768,7 +1774,7 @@ int sfem_amd_graph_op_stats(const sfem_amd_op *op, int *captured, int *replays) 
     CHECK_HANDLE(op);
     auto *g = dynamic_cast<const GraphOp *>(op->impl.get());
     SFEM_AMD_REQUIRE(g, "graph_op_stats: not a graph operator");
-    if (captured) *captured = g->exec ? 1 : (g->failed ? -1 : 0);
+    if (captured) *captured = g->failed ? -1 : g->captured();
     if (replays) *replays = g->replays;
     return SFEM_SUCCESS;
 }

@@ -350,6 +350,9 @@ def test_graph_replayed_vcycle_matches_eager(real_type):
         assert np.linalg.norm(b - host(rr)) / np.linalg.norm(b) < 1e-10
         its.append(solver.iterations)
     assert its[0] == its[1]
+    # the CG applies its preconditioner in zero-initial-guess mode: a second graph, replayed
+    captured, replays2 = S.graph_op_stats(P_graph)
+    assert captured == 2 and replays2 >= replays + its[1] - 2
 
 
 def test_block_jacobi_smoother_parity_and_default_gmg():
