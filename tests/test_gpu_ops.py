@@ -4,6 +4,8 @@ Bar: FP64 apply/diag within 1e-12 relative (2-norm) of the oracle on the same fl
 geometry words; transfers likewise; integer data (connectivity after upload/derefine)
 bit-exact.  FP32/BF16 operators state their error against the FP64 oracle (< 1e-2).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -377,6 +379,43 @@ def test_constrained_gather_matches_separate_copy(kind, L, interior):
 
 
 # ---------------------------------------------------------------- BLAS
+@pytest.mark.parametrize("rt", [S.FLOAT64, S.FLOAT32])
+def test_fused_vector_steps(rt):
+    """sfem_amd_cheb3_start (Chebyshev3's first step, with and without A x and D^-1) and
+    sfem_amd_cg_update_x_r_dot (CG's x/r update returning (r,r) and (r,z)) against numpy."""
+    dt = np.float64 if rt == S.FLOAT64 else np.float32
+    tol = 1e-13 if rt == S.FLOAT64 else 1e-6
+    rng = np.random.default_rng(4)
+    n = 100_003
+    rhs, ax, dinv, x0 = (rng.uniform(-1, 1, n).astype(dt) for _ in range(4))
+    alpha = 0.37
+    for have_ax in (0, 1):
+        for use_d in (False, True):
+            p = dev(ax, dt)
+            x = dev(x0, dt)
+            d = dev(dinv, dt) if use_d else None
+            b = dev(rhs, dt)  # device arrays stay referenced until the kernel has run
+            assert S.lib.sfem_amd_cheb3_start(rt, n, alpha, S._dptr(b), S._dptr(d), have_ax, S._dptr(p), S._dptr(x),
+                                              None) == 0
+            S.synchronize()
+            po = (ax.astype(np.float64) - rhs) if have_ax else -rhs.astype(np.float64)
+            if use_d:
+                po = dinv * po
+            xo = (x0 if have_ax else 0) - alpha * po
+            assert relerr(host(p).astype(np.float64), po) < tol
+            assert relerr(host(x).astype(np.float64), xo) < tol
+    pv, apv, xv, rv, zv = (rng.uniform(-1, 1, n).astype(dt) for _ in range(5))
+    x, r, p, ap, z = dev(xv, dt), dev(rv, dt), dev(pv, dt), dev(apv, dt), dev(zv, dt)
+    rz = C.c_double()
+    rr = S.lib.sfem_amd_cg_update_x_r_dot(rt, n, alpha, S._dptr(p), S._dptr(ap), S._dptr(x), S._dptr(r), S._dptr(z),
+                                          C.byref(rz), None)
+    xo = xv.astype(np.float64) + alpha * pv
+    ro = rv.astype(np.float64) - alpha * apv
+    assert relerr(host(x).astype(np.float64), xo) < tol and relerr(host(r).astype(np.float64), ro) < tol
+    assert abs(rr - ro @ ro) < tol * (ro @ ro)
+    assert abs(rz.value - ro @ zv) < tol * (np.abs(ro) @ np.abs(zv))
+
+
 def test_blas_reductions_deterministic_and_accurate():
     rng = np.random.default_rng(0)
     for n in (0, 1, 1000, 3_000_001):
