@@ -368,7 +368,9 @@ template <int L>
 struct CnnShape {
     static constexpr int N1 = L + 1;
     static constexpr int NXE = N1 * N1 * N1;
-    static constexpr int T = (NXE + 15) / 16;
+    // node tiles of 16 with at least one padding node: node NXE holds zero in every plane and
+    // stands in for missing neighbours
+    static constexpr int T = NXE / 16 + 1;
     static constexpr int NP = T * 16;
     static constexpr int THREADS = 512;
     static constexpr int TPG = 3;  // tiles per wave iteration in the hidden layers
@@ -376,12 +378,21 @@ struct CnnShape {
     static constexpr size_t LDS = sizeof(float) * NP + 32 * (size_t)NP + 64 * (size_t)NP;
 };
 
+// Neighbour validity as bit tests.  cnn_avail: the directions in which node (x, y, z) has a
+// neighbour (bits x-, x+, y-, y+, z-, z+) and bit 6 for a real node (in); cnn_need(o): the bits offset
+// o requires (bit 7, never available, for the padding offsets o >= 27).  Offset o of the node exists
+// iff (cnn_need(o) & ~cnn_avail) == 0.
 template <int L>
-__device__ __forceinline__ bool cnn_valid(int x, int y, int z, int o) {
-    const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1;
-    return o < 27 && (unsigned)(x + dx) <= (unsigned)L && (unsigned)(y + dy) <= (unsigned)L &&
-           (unsigned)(z + dz) <= (unsigned)L;
+__device__ __forceinline__ unsigned cnn_avail(int x, int y, int z, bool in) {
+    return (unsigned)(x > 0) | (unsigned)(x < L) << 1 | (unsigned)(y > 0) << 2 | (unsigned)(y < L) << 3 |
+           (unsigned)(z > 0) << 4 | (unsigned)(z < L) << 5 | (unsigned)in << 6;
 }
+__host__ __device__ constexpr unsigned cnn_need(int o) {
+    return o >= 27 ? 128u
+                   : 64u | (o % 3 == 0 ? 1u : 0u) | (o % 3 == 2 ? 2u : 0u) | ((o / 3) % 3 == 0 ? 4u : 0u) |
+                         ((o / 3) % 3 == 2 ? 8u : 0u) | (o / 9 == 0 ? 16u : 0u) | (o / 9 == 2 ? 32u : 0u);
+}
+
 template <int L>
 __device__ __forceinline__ int cnn_shift(int o) {
     constexpr int N1 = L + 1;
@@ -418,6 +429,15 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
     for (int i = tid; i < NP; i += S::THREADS) zs[i] = i < S::NXE ? zin[(size_t)e * S::NXE + i] : 0.f;
     // weights in lane order: layer 1 [64][8], layers 2-3 [14][64][8], layer 4 [2][64][4]
     const mf_u4 w1 = *(const mf_u4 *)(W16 + lane * 8);
+    // layer 1: lane group g gathers offsets 8g .. 8g+7
+    unsigned need1[8];
+    int sh1[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int o = 8 * g + j;
+        need1[j] = g == 0 ? cnn_need(j) : g == 1 ? cnn_need(8 + j) : g == 2 ? cnn_need(16 + j) : cnn_need(24 + j);
+        sh1[j] = o < 27 ? cnn_shift<L>(o) : 0;
+    }
     float gplus[NB];
     __syncthreads();
 
@@ -426,15 +446,11 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
         // ---- layer 1: zs -> actX
         for (int t = wave; t < S::T; t += nw) {
             const int n = t * 16 + c;
-            const int x = n % S::N1, y = (n / S::N1) % S::N1, z = n / (S::N1 * S::N1);
-            const bool in = n < S::NXE;
+            const unsigned na = ~cnn_avail<L>(n % S::N1, (n / S::N1) % S::N1, n / (S::N1 * S::N1), n < S::NXE);
             mf_b8 b;
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int o = 8 * g + j;
-                const float v = (in && cnn_valid<L>(x, y, z, o)) ? zs[n + cnn_shift<L>(o)] : 0.f;
-                b[j] = __builtin_bit_cast(__bf16, f2bf(v));
-            }
+            for (int j = 0; j < 8; j++)  // offset 8g + j; missing neighbours read zs[NXE] == 0
+                b[j] = __builtin_bit_cast(__bf16, f2bf(zs[(need1[j] & na) == 0 ? n + sh1[j] : S::NXE]));
             mf_f4 acc = {0.f, 0.f, 0.f, 0.f};
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mf_b8, w1), b, acc, 0, 0, 0);
             *(mf_u2 *)act_quad(actX, NP, n, g) = relu_pack(acc, sg);
@@ -451,28 +467,26 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
             for (int p = 0; p < 14; p++) w[p] = *(const mf_u4 *)(wl + (p * 64 + lane) * 8);
             const unsigned short *plane = ain + (g & 1) * NP * 8;
             for (int t = TPG * wave; t < S::T; t += TPG * nw) {
-                int n[TPG], x[TPG], y[TPG], z[TPG];
-                bool in[TPG];
+                int n[TPG];
+                unsigned na[TPG];
                 mf_f4 acc[TPG];
 #pragma unroll
                 for (int h = 0; h < TPG; h++) {
                     n[h] = (t + h) * 16 + c;
-                    x[h] = n[h] % S::N1;
-                    y[h] = (n[h] / S::N1) % S::N1;
-                    z[h] = n[h] / (S::N1 * S::N1);
-                    in[h] = n[h] < S::NXE;
+                    na[h] = ~cnn_avail<L>(n[h] % S::N1, (n[h] / S::N1) % S::N1, n[h] / (S::N1 * S::N1), n[h] < S::NXE);
                     acc[h] = mf_f4{0.f, 0.f, 0.f, 0.f};
                 }
+                const bool hi = (g >> 1) != 0;  // lane groups 2, 3 take the pair's second offset
 #pragma unroll
                 for (int p = 0; p < 14; p++) {
-                    const int o = 2 * p + (g >> 1);
+                    const int sh = hi ? cnn_shift<L>(2 * p + 1) : cnn_shift<L>(2 * p);
+                    const unsigned need = hi ? cnn_need(2 * p + 1) : cnn_need(2 * p);
                     mf_u4 bv[TPG];
+                    // missing neighbours read node NXE's row, which is zero: the layers have no bias,
+                    // so a node with no valid input (n >= NXE) stores ReLU(0) = 0
 #pragma unroll
-                    for (int h = 0; h < TPG; h++) {
-                        bv[h] = mf_u4{0u, 0u, 0u, 0u};
-                        if (in[h] && cnn_valid<L>(x[h], y[h], z[h], o))
-                            bv[h] = *(const mf_u4 *)(plane + (n[h] + cnn_shift<L>(o)) * 8);
-                    }
+                    for (int h = 0; h < TPG; h++)
+                        bv[h] = *(const mf_u4 *)(plane + ((need & na[h]) == 0 ? n[h] + sh : S::NXE) * 8);
 #pragma unroll
                     for (int h = 0; h < TPG; h++)
                         acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mf_b8, w[p]),
@@ -507,11 +521,12 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
             for (int i = 0; i < NB; i++) {
                 const int n = tid + i * S::THREADS;
                 if (n < S::NXE) {
-                    const int x = n % S::N1, y = (n / S::N1) % S::N1, z = n / (S::N1 * S::N1);
+                    const unsigned na = ~cnn_avail<L>(n % S::N1, (n / S::N1) % S::N1, n / (S::N1 * S::N1), true);
+                    // missing neighbours add P[r][NXE] == 0 (computed from the zero activation row)
 #pragma unroll
                     for (int r = 0; r < 16; r++) {
                         const int k = 16 * pass + r;
-                        if (cnn_valid<L>(x, y, z, k)) v[i] += P[r * NP + n + cnn_shift<L>(k)];
+                        if (k < 27) v[i] += P[r * NP + ((cnn_need(k) & na) == 0 ? n + cnn_shift<L>(k) : S::NXE)];
                     }
                 }
             }
