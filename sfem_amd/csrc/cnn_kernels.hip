@@ -358,10 +358,14 @@ typedef float mf_f4 __attribute__((ext_vector_type(4)));
 typedef unsigned int mf_u4 __attribute__((ext_vector_type(4)));
 typedef unsigned int mf_u2 __attribute__((ext_vector_type(2)));
 
-__device__ __forceinline__ unsigned short f2bf(float f) {
-    unsigned int u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
+typedef __bf16 mf_b2 __attribute__((ext_vector_type(2)));
+typedef float mf_f2 __attribute__((ext_vector_type(2)));
+
+// float -> bf16, round to nearest even: the conversions lower to gfx950's v_cvt_pk_bf16_f32
+// (bf_pack converts two values with one instruction)
+__device__ __forceinline__ __bf16 f2bf(float f) { return (__bf16)f; }
+__device__ __forceinline__ unsigned bf_pack(float lo, float hi) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(mf_f2{lo, hi}, mf_b2));
 }
 
 template <int L>
@@ -406,8 +410,8 @@ __device__ __forceinline__ unsigned short *act_quad(unsigned short *a, int NP, i
 
 __device__ __forceinline__ mf_u2 relu_pack(const mf_f4 &acc, float sg) {
     mf_u2 o2;
-    o2[0] = (unsigned)f2bf(fmaxf(sg * acc[0], 0.f)) | ((unsigned)f2bf(fmaxf(sg * acc[1], 0.f)) << 16);
-    o2[1] = (unsigned)f2bf(fmaxf(sg * acc[2], 0.f)) | ((unsigned)f2bf(fmaxf(sg * acc[3], 0.f)) << 16);
+    o2[0] = bf_pack(fmaxf(sg * acc[0], 0.f), fmaxf(sg * acc[1], 0.f));
+    o2[1] = bf_pack(fmaxf(sg * acc[2], 0.f), fmaxf(sg * acc[3], 0.f));
     return o2;
 }
 
@@ -450,7 +454,7 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
             mf_b8 b;
 #pragma unroll
             for (int j = 0; j < 8; j++)  // offset 8g + j; missing neighbours read zs[NXE] == 0
-                b[j] = __builtin_bit_cast(__bf16, f2bf(zs[(need1[j] & na) == 0 ? n + sh1[j] : S::NXE]));
+                b[j] = f2bf(zs[(need1[j] & na) == 0 ? n + sh1[j] : S::NXE]);
             mf_f4 acc = {0.f, 0.f, 0.f, 0.f};
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mf_b8, w1), b, acc, 0, 0, 0);
             *(mf_u2 *)act_quad(actX, NP, n, g) = relu_pack(acc, sg);
