@@ -79,7 +79,7 @@ def cpu_baseline(el, fff, N, L, budget_s=12.0, max_reps=20):
                       f"OpenMP restatement of affine_sshex8_laplacian_stencil_apply_fff"}
 
 
-def solve_times(S, mesh, fff, pts, n_macro, rtol=1e-10, learned_steps=2):
+def solve_times(S, mesh, fff, pts, n_macro, rtol=1e-10, learned_steps=1):
     """Time to ||b - A x||/||b|| <= rtol (recomputed in FP64 with the FP64 operator) of
       fp64_eager:    FP64 MG-PCG, Chebyshev3 V-cycle, the reference's FP64 coarse PCG, every
                      kernel launched from the host;
@@ -88,7 +88,9 @@ def solve_times(S, mesh, fff, pts, n_macro, rtol=1e-10, learned_steps=2):
       mixed:         that V-cycle in FP32 with BF16 geometry, graph-replayed, in FP64 flexible CG;
       mixed_learned: as mixed, with the learned CNN smoother on the finest level (trained at
                      setup on V-cycle pairs; Chebyshev3 fallback when an iteration reduces the
-                     residual by less than 2x).
+                     residual by less than 2x); `learned_steps` CNN steps per smoothing (one is
+                     faster to 1e-10 than two on the 16^3, L=8 problem: 7 iterations at half the
+                     inference cost against 6).
     Every timed solve starts from x = 0 after one untimed warm-up solve."""
     from sfem_amd import gmg, learned
     N = pts.shape[1]
