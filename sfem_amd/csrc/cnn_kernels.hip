@@ -349,9 +349,11 @@ template int scatter_elements<float>(ptrdiff_t, int, const int32_t *, int, int, 
 // Lane layouts (64 lanes, g = lane / 16, c = lane % 16):
 //   16x16x32 A: row c, k = 8g + j (j < 8);  B: column c, k = 8g + j;  D: column c, rows 4g + r
 //   16x16x16 A: row c, k = 4g + j (j < 4);  B: column c, k = 4g + j;  D: as above
-// Activations are stored post-ReLU in BF16 as two channel planes [half][node][8 channels], so the
-// 16-byte B reads of the 16 lanes of a row group are consecutive (no LDS bank conflicts); padding
-// nodes of the last tile and neighbours outside the element read as zero.
+// The input and the activations of layers 1-3 live on the element's grid with a one-point zero
+// halo ((L+3)^3 points), so neighbour reads need no validity tests: one address per tile and a
+// per-pair offset.  Activations are stored post-ReLU in BF16 as two channel planes
+// [half][point][8 channels] (16-byte B reads).  Layer 4's stencil sum reads the unpadded P, with
+// the padding node NXE (P == 0) standing in for missing neighbours.
 typedef short mf_s4 __attribute__((ext_vector_type(4)));
 typedef __bf16 mf_b8 __attribute__((ext_vector_type(8)));
 typedef float mf_f4 __attribute__((ext_vector_type(4)));
@@ -372,14 +374,19 @@ template <int L>
 struct CnnShape {
     static constexpr int N1 = L + 1;
     static constexpr int NXE = N1 * N1 * N1;
-    // node tiles of 16 with at least one padding node: node NXE holds zero in every plane and
-    // stands in for missing neighbours
+    // node tiles of 16 with at least one padding node (node NXE: P[k][NXE] == 0 stands in for the
+    // missing neighbours of the layer-4 stencil sum)
     static constexpr int T = NXE / 16 + 1;
     static constexpr int NP = T * 16;
+    // layers 1-3 hold the element on an (L+3)^3 grid with a zero halo, so every node's 27
+    // neighbours are in range and read without tests
+    static constexpr int P1 = L + 3;
+    static constexpr int NH = P1 * P1 * P1;
     static constexpr int THREADS = 512;
     static constexpr int TPG = 3;  // tiles per wave iteration in the hidden layers
-    // zs [NP] float | actX [2][NP][8] bf16 | actY [2][NP][8] bf16, aliased by P [16][NP] float
-    static constexpr size_t LDS = sizeof(float) * NP + 32 * (size_t)NP + 64 * (size_t)NP;
+    // zs [NH] float | actX [2][NH][8] bf16 | actY [2][NH][8] bf16 | P [16][NP] float
+    static constexpr size_t ZS = (sizeof(float) * NH + 15) / 16 * 16;
+    static constexpr size_t LDS = ZS + 64 * (size_t)NH + 64 * (size_t)NP;
 };
 
 // Neighbour validity as bit tests.  cnn_avail: the directions in which node (x, y, z) has a
@@ -403,9 +410,23 @@ __device__ __forceinline__ int cnn_shift(int o) {
     return (o / 9 - 1) * N1 * N1 + ((o / 3) % 3 - 1) * N1 + (o % 3 - 1);
 }
 
-// 4 channels (4g .. 4g+3) of node n, as stored from an MFMA D fragment
-__device__ __forceinline__ unsigned short *act_quad(unsigned short *a, int NP, int n, int g) {
-    return a + (g >> 1) * NP * 8 + n * 8 + 4 * (g & 1);
+// node n = lidx(x, y, z) on the haloed grid; cnn_pshift(o): offset o there (0 for the
+// zero-weight padding offsets o >= 27, which then read the node itself)
+template <int L>
+__device__ __forceinline__ int cnn_pad(int n) {
+    constexpr int N1 = L + 1, P1 = L + 3;
+    const int x = n % N1, y = (n / N1) % N1, z = n / (N1 * N1);
+    return (x + 1) + (y + 1) * P1 + (z + 1) * P1 * P1;
+}
+template <int L>
+__device__ __forceinline__ int cnn_pshift(int o) {
+    constexpr int P1 = L + 3;
+    return o >= 27 ? 0 : (o / 9 - 1) * P1 * P1 + ((o / 3) % 3 - 1) * P1 + (o % 3 - 1);
+}
+
+// 4 channels (4g .. 4g+3) of grid point q, as stored from an MFMA D fragment
+__device__ __forceinline__ unsigned short *act_quad(unsigned short *a, int NH, int q, int g) {
+    return a + (g >> 1) * NH * 8 + q * 8 + 4 * (g & 1);
 }
 
 __device__ __forceinline__ mf_u2 relu_pack(const mf_f4 &acc, float sg) {
@@ -419,29 +440,31 @@ template <int L>
 __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, const unsigned short *__restrict__ W16,
                                                            const float *__restrict__ zin, float *__restrict__ out) {
     using S = CnnShape<L>;
-    constexpr int NP = S::NP, TPG = S::TPG, NB = (S::NXE + S::THREADS - 1) / S::THREADS;
+    constexpr int NP = S::NP, NH = S::NH, P1 = S::P1, TPG = S::TPG, NB = (S::NXE + S::THREADS - 1) / S::THREADS;
     extern __shared__ __align__(16) unsigned char cnn_smem[];
     float *zs = (float *)cnn_smem;
-    unsigned short *actX = (unsigned short *)(cnn_smem + 4 * NP);
-    unsigned short *actY = actX + 16 * NP;
-    float *P = (float *)(cnn_smem + 4 * NP + 32 * NP);  // aliases actY (dead when P is live)
+    unsigned short *actX = (unsigned short *)(cnn_smem + S::ZS);
+    unsigned short *actY = actX + 16 * NH;
+    float *P = (float *)(cnn_smem + S::ZS + 64 * (size_t)NH);
     const ptrdiff_t e = blockIdx.x;
     if (e >= E) return;  // uniform per block
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = S::THREADS / 64;
     const int g = lane >> 4, c = lane & 15;
 
-    for (int i = tid; i < NP; i += S::THREADS) zs[i] = i < S::NXE ? zin[(size_t)e * S::NXE + i] : 0.f;
+    // the input on the haloed grid; both activation buffers zeroed (the layers write only the
+    // interior points, so the halo stays zero for both signs)
+    for (int q = tid; q < NH; q += S::THREADS) {
+        const int px = q % P1 - 1, py = (q / P1) % P1 - 1, pz = q / (P1 * P1) - 1;
+        const bool in = (unsigned)px <= (unsigned)L && (unsigned)py <= (unsigned)L && (unsigned)pz <= (unsigned)L;
+        zs[q] = in ? zin[(size_t)e * S::NXE + (pz * S::N1 + py) * S::N1 + px] : 0.f;
+    }
+    for (int i = tid; i < 4 * NH; i += S::THREADS) ((mf_u4 *)actX)[i] = mf_u4{0u, 0u, 0u, 0u};
     // weights in lane order: layer 1 [64][8], layers 2-3 [14][64][8], layer 4 [2][64][4]
     const mf_u4 w1 = *(const mf_u4 *)(W16 + lane * 8);
-    // layer 1: lane group g gathers offsets 8g .. 8g+7
-    unsigned need1[8];
-    int sh1[8];
+    int sh1[8];  // layer 1: lane group g gathers offsets 8g .. 8g+7
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const int o = 8 * g + j;
-        need1[j] = g == 0 ? cnn_need(j) : g == 1 ? cnn_need(8 + j) : g == 2 ? cnn_need(16 + j) : cnn_need(24 + j);
-        sh1[j] = o < 27 ? cnn_shift<L>(o) : 0;
-    }
+    for (int j = 0; j < 8; j++) sh1[j] = cnn_pshift<L>(8 * g + j);
+    const bool hi = (g >> 1) != 0;  // hidden layers: lane groups 2, 3 take the pair's second offset
     float gplus[NB];
     __syncthreads();
 
@@ -450,14 +473,14 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
         // ---- layer 1: zs -> actX
         for (int t = wave; t < S::T; t += nw) {
             const int n = t * 16 + c;
-            const unsigned na = ~cnn_avail<L>(n % S::N1, (n / S::N1) % S::N1, n / (S::N1 * S::N1), n < S::NXE);
+            const bool real = n < S::NXE;
+            const int q = cnn_pad<L>(real ? n : 0);
             mf_b8 b;
 #pragma unroll
-            for (int j = 0; j < 8; j++)  // offset 8g + j; missing neighbours read zs[NXE] == 0
-                b[j] = f2bf(zs[(need1[j] & na) == 0 ? n + sh1[j] : S::NXE]);
+            for (int j = 0; j < 8; j++) b[j] = f2bf(zs[q + sh1[j]]);
             mf_f4 acc = {0.f, 0.f, 0.f, 0.f};
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mf_b8, w1), b, acc, 0, 0, 0);
-            *(mf_u2 *)act_quad(actX, NP, n, g) = relu_pack(acc, sg);
+            if (real) *(mf_u2 *)act_quad(actX, NH, q, g) = relu_pack(acc, sg);
         }
         __syncthreads();
         // ---- layers 2 and 3: actX -> actY -> actX
@@ -469,28 +492,26 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
             mf_u4 w[14];
 #pragma unroll
             for (int p = 0; p < 14; p++) w[p] = *(const mf_u4 *)(wl + (p * 64 + lane) * 8);
-            const unsigned short *plane = ain + (g & 1) * NP * 8;
+            // byte addresses: the lane's channel plane plus 16 bytes per grid point
+            const unsigned char *plane = (const unsigned char *)(ain + (g & 1) * NH * 8);
             for (int t = TPG * wave; t < S::T; t += TPG * nw) {
-                int n[TPG];
-                unsigned na[TPG];
+                int q[TPG], qb[TPG];
+                bool real[TPG];
                 mf_f4 acc[TPG];
 #pragma unroll
                 for (int h = 0; h < TPG; h++) {
-                    n[h] = (t + h) * 16 + c;
-                    na[h] = ~cnn_avail<L>(n[h] % S::N1, (n[h] / S::N1) % S::N1, n[h] / (S::N1 * S::N1), n[h] < S::NXE);
+                    const int n = (t + h) * 16 + c;
+                    real[h] = n < S::NXE;
+                    q[h] = cnn_pad<L>(real[h] ? n : 0);
+                    qb[h] = q[h] * 16;
                     acc[h] = mf_f4{0.f, 0.f, 0.f, 0.f};
                 }
-                const bool hi = (g >> 1) != 0;  // lane groups 2, 3 take the pair's second offset
 #pragma unroll
                 for (int p = 0; p < 14; p++) {
-                    const int sh = hi ? cnn_shift<L>(2 * p + 1) : cnn_shift<L>(2 * p);
-                    const unsigned need = hi ? cnn_need(2 * p + 1) : cnn_need(2 * p);
+                    const int shb = 16 * (hi ? cnn_pshift<L>(2 * p + 1) : cnn_pshift<L>(2 * p));
                     mf_u4 bv[TPG];
-                    // missing neighbours read node NXE's row, which is zero: the layers have no bias,
-                    // so a node with no valid input (n >= NXE) stores ReLU(0) = 0
 #pragma unroll
-                    for (int h = 0; h < TPG; h++)
-                        bv[h] = *(const mf_u4 *)(plane + ((need & na[h]) == 0 ? n[h] + sh : S::NXE) * 8);
+                    for (int h = 0; h < TPG; h++) bv[h] = *(const mf_u4 *)(plane + qb[h] + shb);
 #pragma unroll
                     for (int h = 0; h < TPG; h++)
                         acc[h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mf_b8, w[p]),
@@ -499,7 +520,7 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
                 }
 #pragma unroll
                 for (int h = 0; h < TPG; h++)
-                    if (t + h < S::T) *(mf_u2 *)act_quad(aout, NP, n[h], g) = relu_pack(acc[h], 1.f);
+                    if (real[h]) *(mf_u2 *)act_quad(aout, NH, q[h], g) = relu_pack(acc[h], 1.f);
             }
             __syncthreads();
         }
@@ -513,7 +534,9 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
             const mf_u2 w4 = *(const mf_u2 *)(W16 + 512 + 2 * 7168 + pass * 256 + lane * 4);
             for (int t = wave; t < S::T; t += nw) {
                 const int n = t * 16 + c;
-                const mf_u2 bv = *(const mf_u2 *)act_quad(actX, NP, n, g);
+                const bool real = n < S::NXE;
+                const mf_u2 ld = *(const mf_u2 *)act_quad(actX, NH, cnn_pad<L>(real ? n : 0), g);
+                const mf_u2 bv = real ? ld : mf_u2{0u, 0u};  // padding nodes: P[r][n] = 0
                 mf_f4 a = {0.f, 0.f, 0.f, 0.f};
                 a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(mf_s4, w4), __builtin_bit_cast(mf_s4, bv),
                                                                a, 0, 0, 0);
@@ -526,7 +549,7 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
                 const int n = tid + i * S::THREADS;
                 if (n < S::NXE) {
                     const unsigned na = ~cnn_avail<L>(n % S::N1, (n / S::N1) % S::N1, n / (S::N1 * S::N1), true);
-                    // missing neighbours add P[r][NXE] == 0 (computed from the zero activation row)
+                    // missing neighbours add P[r][NXE] == 0 (the padding node's product)
 #pragma unroll
                     for (int r = 0; r < 16; r++) {
                         const int k = 16 * pass + r;
@@ -534,7 +557,7 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
                     }
                 }
             }
-            __syncthreads();  // P is rewritten by the next pass / layer 2 of the next sign
+            __syncthreads();  // P is rewritten by the next pass / sign
         }
 #pragma unroll
         for (int i = 0; i < NB; i++) {
