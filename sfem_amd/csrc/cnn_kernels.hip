@@ -378,15 +378,22 @@ struct CnnShape {
     // missing neighbours of the layer-4 stencil sum)
     static constexpr int T = NXE / 16 + 1;
     static constexpr int NP = T * 16;
-    // layers 1-3 hold the element on an (L+3)^3 grid with a zero halo, so every node's 27
-    // neighbours are in range and read without tests
+    // layers 1-3 hold the element on grids with a zero halo, so every node's 27 neighbours are in
+    // range and read without tests: the input on the (L+3)^3 grid, the activations on a grid with
+    // line stride SY and plane stride SZ chosen so that consecutive nodes, also across line and
+    // plane ends, sit one 16-byte row apart mod 8 (eight lanes read all 32 LDS banks)
     static constexpr int P1 = L + 3;
     static constexpr int NH = P1 * P1 * P1;
+    static constexpr int SY = L + 9;  // >= L + 3, = L + 1 (mod 8)
+    static constexpr int SZ = P1 * SY + ((((L * SY + L + 1) - P1 * SY) % 8) + 8) % 8;  // = L SY + L + 1 (mod 8)
+    static constexpr int NA = P1 * SZ;
     static constexpr int THREADS = 512;
     static constexpr int TPG = 3;  // tiles per wave iteration in the hidden layers
-    // zs [NH] float | actX [2][NH][8] bf16 | actY [2][NH][8] bf16 | P [16][NP] float
+    // zs [NH] float | actX [2][NA][8] bf16 | actY [2][NA][8] bf16, aliased by P [16][NP] float
+    // (actY is cleared again at the start of each sign)
     static constexpr size_t ZS = (sizeof(float) * NH + 15) / 16 * 16;
-    static constexpr size_t LDS = ZS + 64 * (size_t)NH + 64 * (size_t)NP;
+    static constexpr size_t ACT = 32 * (size_t)NA;
+    static constexpr size_t LDS = ZS + ACT + (ACT > 64 * (size_t)NP ? ACT : 64 * (size_t)NP);
 };
 
 // Neighbour validity as bit tests.  cnn_avail: the directions in which node (x, y, z) has a
@@ -410,23 +417,23 @@ __device__ __forceinline__ int cnn_shift(int o) {
     return (o / 9 - 1) * N1 * N1 + ((o / 3) % 3 - 1) * N1 + (o % 3 - 1);
 }
 
-// node n = lidx(x, y, z) on the haloed grid; cnn_pshift(o): offset o there (0 for the
-// zero-weight padding offsets o >= 27, which then read the node itself)
-template <int L>
+// node n = lidx(x, y, z) on a haloed grid with line stride SY and plane stride SZ;
+// cnn_pshift(o): offset o there (0 for the zero-weight padding offsets o >= 27, which then read
+// the node itself)
+template <int L, int SY, int SZ>
 __device__ __forceinline__ int cnn_pad(int n) {
-    constexpr int N1 = L + 1, P1 = L + 3;
+    constexpr int N1 = L + 1;
     const int x = n % N1, y = (n / N1) % N1, z = n / (N1 * N1);
-    return (x + 1) + (y + 1) * P1 + (z + 1) * P1 * P1;
+    return (x + 1) + (y + 1) * SY + (z + 1) * SZ;
 }
-template <int L>
+template <int SY, int SZ>
 __device__ __forceinline__ int cnn_pshift(int o) {
-    constexpr int P1 = L + 3;
-    return o >= 27 ? 0 : (o / 9 - 1) * P1 * P1 + ((o / 3) % 3 - 1) * P1 + (o % 3 - 1);
+    return o >= 27 ? 0 : (o / 9 - 1) * SZ + ((o / 3) % 3 - 1) * SY + (o % 3 - 1);
 }
 
 // 4 channels (4g .. 4g+3) of grid point q, as stored from an MFMA D fragment
-__device__ __forceinline__ unsigned short *act_quad(unsigned short *a, int NH, int q, int g) {
-    return a + (g >> 1) * NH * 8 + q * 8 + 4 * (g & 1);
+__device__ __forceinline__ unsigned short *act_quad(unsigned short *a, int NA, int q, int g) {
+    return a + (g >> 1) * NA * 8 + q * 8 + 4 * (g & 1);
 }
 
 __device__ __forceinline__ mf_u2 relu_pack(const mf_f4 &acc, float sg) {
@@ -440,47 +447,49 @@ template <int L>
 __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, const unsigned short *__restrict__ W16,
                                                            const float *__restrict__ zin, float *__restrict__ out) {
     using S = CnnShape<L>;
-    constexpr int NP = S::NP, NH = S::NH, P1 = S::P1, TPG = S::TPG, NB = (S::NXE + S::THREADS - 1) / S::THREADS;
+    constexpr int NP = S::NP, NH = S::NH, NA = S::NA, P1 = S::P1, SY = S::SY, SZ = S::SZ, TPG = S::TPG,
+                  NB = (S::NXE + S::THREADS - 1) / S::THREADS;
     extern __shared__ __align__(16) unsigned char cnn_smem[];
     float *zs = (float *)cnn_smem;
     unsigned short *actX = (unsigned short *)(cnn_smem + S::ZS);
-    unsigned short *actY = actX + 16 * NH;
-    float *P = (float *)(cnn_smem + S::ZS + 64 * (size_t)NH);
+    unsigned short *actY = actX + 16 * NA;
+    float *P = (float *)actY;
     const ptrdiff_t e = blockIdx.x;
     if (e >= E) return;  // uniform per block
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = S::THREADS / 64;
     const int g = lane >> 4, c = lane & 15;
 
-    // the input on the haloed grid; both activation buffers zeroed (the layers write only the
-    // interior points, so the halo stays zero for both signs)
+    // the input on its haloed grid; actX zeroed (the layers write only the interior points, so its
+    // halo stays zero for both signs)
     for (int q = tid; q < NH; q += S::THREADS) {
         const int px = q % P1 - 1, py = (q / P1) % P1 - 1, pz = q / (P1 * P1) - 1;
         const bool in = (unsigned)px <= (unsigned)L && (unsigned)py <= (unsigned)L && (unsigned)pz <= (unsigned)L;
         zs[q] = in ? zin[(size_t)e * S::NXE + (pz * S::N1 + py) * S::N1 + px] : 0.f;
     }
-    for (int i = tid; i < 4 * NH; i += S::THREADS) ((mf_u4 *)actX)[i] = mf_u4{0u, 0u, 0u, 0u};
+    for (int i = tid; i < 2 * NA; i += S::THREADS) ((mf_u4 *)actX)[i] = mf_u4{0u, 0u, 0u, 0u};
     // weights in lane order: layer 1 [64][8], layers 2-3 [14][64][8], layer 4 [2][64][4]
     const mf_u4 w1 = *(const mf_u4 *)(W16 + lane * 8);
     int sh1[8];  // layer 1: lane group g gathers offsets 8g .. 8g+7
 #pragma unroll
-    for (int j = 0; j < 8; j++) sh1[j] = cnn_pshift<L>(8 * g + j);
+    for (int j = 0; j < 8; j++) sh1[j] = cnn_pshift<P1, P1 * P1>(8 * g + j);
     const bool hi = (g >> 1) != 0;  // hidden layers: lane groups 2, 3 take the pair's second offset
     float gplus[NB];
     __syncthreads();
 
     for (int sign = 0; sign < 2; sign++) {
         const float sg = sign ? -1.f : 1.f;
-        // ---- layer 1: zs -> actX
+        // ---- layer 1: zs -> actX; actY (P of the previous sign) cleared for layer 2
+        for (int i = tid; i < 2 * NA; i += S::THREADS) ((mf_u4 *)actY)[i] = mf_u4{0u, 0u, 0u, 0u};
         for (int t = wave; t < S::T; t += nw) {
             const int n = t * 16 + c;
             const bool real = n < S::NXE;
-            const int q = cnn_pad<L>(real ? n : 0);
+            const int qz = cnn_pad<L, P1, P1 * P1>(real ? n : 0);
             mf_b8 b;
 #pragma unroll
-            for (int j = 0; j < 8; j++) b[j] = f2bf(zs[q + sh1[j]]);
+            for (int j = 0; j < 8; j++) b[j] = f2bf(zs[qz + sh1[j]]);
             mf_f4 acc = {0.f, 0.f, 0.f, 0.f};
             acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mf_b8, w1), b, acc, 0, 0, 0);
-            if (real) *(mf_u2 *)act_quad(actX, NH, q, g) = relu_pack(acc, sg);
+            if (real) *(mf_u2 *)act_quad(actX, NA, cnn_pad<L, SY, SZ>(n), g) = relu_pack(acc, sg);
         }
         __syncthreads();
         // ---- layers 2 and 3: actX -> actY -> actX
@@ -493,7 +502,7 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
 #pragma unroll
             for (int p = 0; p < 14; p++) w[p] = *(const mf_u4 *)(wl + (p * 64 + lane) * 8);
             // byte addresses: the lane's channel plane plus 16 bytes per grid point
-            const unsigned char *plane = (const unsigned char *)(ain + (g & 1) * NH * 8);
+            const unsigned char *plane = (const unsigned char *)(ain + (g & 1) * NA * 8);
             for (int t = TPG * wave; t < S::T; t += TPG * nw) {
                 int q[TPG], qb[TPG];
                 bool real[TPG];
@@ -502,13 +511,13 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
                 for (int h = 0; h < TPG; h++) {
                     const int n = (t + h) * 16 + c;
                     real[h] = n < S::NXE;
-                    q[h] = cnn_pad<L>(real[h] ? n : 0);
+                    q[h] = cnn_pad<L, SY, SZ>(real[h] ? n : 0);
                     qb[h] = q[h] * 16;
                     acc[h] = mf_f4{0.f, 0.f, 0.f, 0.f};
                 }
 #pragma unroll
                 for (int p = 0; p < 14; p++) {
-                    const int shb = 16 * (hi ? cnn_pshift<L>(2 * p + 1) : cnn_pshift<L>(2 * p));
+                    const int shb = 16 * (hi ? cnn_pshift<SY, SZ>(2 * p + 1) : cnn_pshift<SY, SZ>(2 * p));
                     mf_u4 bv[TPG];
 #pragma unroll
                     for (int h = 0; h < TPG; h++) bv[h] = *(const mf_u4 *)(plane + qb[h] + shb);
@@ -520,7 +529,7 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
                 }
 #pragma unroll
                 for (int h = 0; h < TPG; h++)
-                    if (real[h]) *(mf_u2 *)act_quad(aout, NH, q[h], g) = relu_pack(acc[h], 1.f);
+                    if (real[h]) *(mf_u2 *)act_quad(aout, NA, q[h], g) = relu_pack(acc[h], 1.f);
             }
             __syncthreads();
         }
@@ -535,7 +544,7 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
             for (int t = wave; t < S::T; t += nw) {
                 const int n = t * 16 + c;
                 const bool real = n < S::NXE;
-                const mf_u2 ld = *(const mf_u2 *)act_quad(actX, NH, cnn_pad<L>(real ? n : 0), g);
+                const mf_u2 ld = *(const mf_u2 *)act_quad(actX, NA, cnn_pad<L, SY, SZ>(real ? n : 0), g);
                 const mf_u2 bv = real ? ld : mf_u2{0u, 0u};  // padding nodes: P[r][n] = 0
                 mf_f4 a = {0.f, 0.f, 0.f, 0.f};
                 a = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(mf_s4, w4), __builtin_bit_cast(mf_s4, bv),
