@@ -65,7 +65,7 @@ def test_cnn_loss_and_gradient_match_oracle():
         assert err < 1e-4
 
 
-@pytest.mark.parametrize("L", [1, 2, 3, 5, 8])
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_cnn_mfma_inference_matches_fp32(L):
     n = (L + 1) ** 3
     E = 7
