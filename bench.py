@@ -316,8 +316,8 @@ def main():
                          "elasticity-apply: time the sshex8 elasticity operators of --elasticity-case")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (every box here has one GPU; "
-              "the multi-GPU sharding is described in DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (this build runs on a single "
+              "GPU; the workload measured is one GPU's shard, and no multi-GPU path is built).", file=sys.stderr)
         sys.exit(2)
 
     import sfem_amd as S
