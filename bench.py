@@ -392,6 +392,16 @@ def main():
         result["roofline_cold"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                    "frac": ach / HBM_PEAK_GBS, "ms_per_apply": cold,
                                    "note": "1 GiB written between applies (outside the timed events)"}
+    S.set_option("atomic_skeleton", 1)
+    try:
+        ta = S.benchmark(op, x, y, warmup=args.warmup, reps=args.steps) / args.steps
+    finally:
+        S.set_option("atomic_skeleton", 0)
+    ach = B / (ta * 1e-3) / 1e9
+    result["apply_atomic_skeleton"] = {
+        "ms_per_apply": ta, "value": N / (ta * 1e-3), "unit": "DOF/s", "achieved": ach, "frac": ach / HBM_PEAK_GBS,
+        "note": "option atomic_skeleton: macro-element boundary sums with FP64 atomics inside the line-sweep "
+                "kernel (one launch, no partials); not bitwise reproducible, so not the headline value"}
     if not args.no_solve:
         result["solve"] = solve_times(S, mesh, fff, pts, n_macro)
         if not args.no_elasticity:

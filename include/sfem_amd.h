@@ -53,7 +53,11 @@ const char *sfem_amd_last_error(void);
 /* Library options (no counterpart in the reference).  "fused_chebyshev" (default 0): 1 makes
  * Chebyshev3 smoothers on sshex8 matrix-core elasticity operators run each step inside the
  * operator's apply kernels (same results, no product vector; slower on MI355X than the separate
- * streaming update, whose accesses are contiguous); 0 runs the separate product and update. */
+ * streaming update, whose accesses are contiguous); 0 runs the separate product and update.
+ * "atomic_skeleton" (default 0): 1 makes accumulating (y += A x), unconstrained sshex8 Laplacian
+ * applies add the macro-element boundary contributions to y with atomics, one launch and no
+ * partials (results differ from the default in the last bits and are not reproducible run to
+ * run); 0 keeps the fixed-order two-phase sum. */
 int sfem_amd_set_option(const char *name, int value);
 /* Number of visible GPUs; 0 when no HIP device is present. */
 int sfem_amd_device_count(void);

@@ -673,7 +673,8 @@ def norm2(x, stream=None):
 
 
 def set_option(name, value):
-    """Library options, see sfem_amd_set_option ("fused_chebyshev": 1 or 0, default 0)."""
+    """Library options, see sfem_amd_set_option ("fused_chebyshev", "atomic_skeleton": 1 or 0,
+    default 0)."""
     _check(lib.sfem_amd_set_option(name.encode(), int(value)), "set_option")
 
 

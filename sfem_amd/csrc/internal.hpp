@@ -155,6 +155,8 @@ namespace sfem_amd {
 // Library options (sfem_amd_set_option): Chebyshev3 steps fused into the sshex8 matrix-core
 // elasticity applies (default off: measured slower than the streaming update kernels).
 extern bool g_fused_chebyshev;
+// sshex8 Laplacian accumulating applies sum skeleton nodes with atomics (option "atomic_skeleton")
+extern bool g_atomic_skeleton;
 // Zeroes / sets constrained dofs; defined in runtime.cpp.
 int constrained_set_value(const OpBase *constrained, int rt, double v, void *x, hipStream_t s);
 int mesh_counts(const sfem_amd_mesh *m, hipStream_t s);

@@ -152,6 +152,7 @@ static int mesh_owner(const sfem_amd_mesh *m, hipStream_t s) {
     return node_owner_mask(m->E, m->nv, m->nnodes, m->idx.p, m->own.p, s);
 }
 bool g_fused_chebyshev = false;
+bool g_atomic_skeleton = false;
 
 const sfem_amd_mesh *mesh_subblocks(const sfem_amd_mesh *m, int Ls, hipStream_t s) {
     auto &slot = m->sub[Ls];
@@ -221,11 +222,11 @@ struct LaplacianOp final : OpBase {
                     return hex8_laplacian_apply(m->E, m->idx.p, fff32.p, coeff.p, x, y, s);
                 default:
                     return sshex8_laplacian_apply<double, float>(m->level, m->E, m->idx.p, fff32.p, m->E, coeff.p,
-                                                                 Kem.p, x, y, part.p, sk, s, ow);
+                                                                 Kem.p, x, y, part.p, sk, s, ow, g_atomic_skeleton);
             }
         } else {
             return sshex8_laplacian_apply<float, uint16_t>(m->level, m->E, m->idx.p, fff16.p, m->E, nullptr, Kem.p,
-                                                           x, y, part.p, sk, s, ow);
+                                                           x, y, part.p, sk, s, ow, g_atomic_skeleton);
         }
     }
     int diag(void *dv, hipStream_t s) override {
@@ -858,6 +859,10 @@ int sfem_amd_set_option(const char *name, int value) {
     SFEM_AMD_REQUIRE(name, "set_option: name");
     if (std::strcmp(name, "fused_chebyshev") == 0) {
         g_fused_chebyshev = value != 0;
+        return SFEM_SUCCESS;
+    }
+    if (std::strcmp(name, "atomic_skeleton") == 0) {
+        g_atomic_skeleton = value != 0;
         return SFEM_SUCCESS;
     }
     set_last_error(std::string("set_option: unknown option ") + name);

@@ -165,15 +165,18 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
             w[2][a][b] = u[lidx(L, 1, ys[b], zs[a])];
         }
 
-    // part positions of this line's skeleton nodes (all L+1 on a skeleton line, else x = 0, L)
+    // part positions of this line's skeleton nodes (all L+1 on a skeleton line, else x = 0, L);
+    // part == null: skeleton contributions are added to y atomically instead (accumulating apply)
     const int32_t *pp = sk_pos + e * NSK + sk_line_base(L, yy, zz);
     int32_t ppos[N1];
-    if (skl) {
+    if (part) {
+        if (skl) {
 #pragma unroll
-        for (int i = 0; i <= L; i++) ppos[i] = pp[i];
-    } else {
-        ppos[0] = pp[0];
-        ppos[L] = pp[1];
+            for (int i = 0; i <= L; i++) ppos[i] = pp[i];
+        } else {
+            ppos[0] = pp[0];
+            ppos[L] = pp[1];
+        }
     }
 #pragma unroll
     for (int i = 0; i <= L; i++) {
@@ -196,10 +199,14 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
                     v = fma(Sb[a][b][1], w[1][a][b], v);
                 }
         }
-        if (skl || i == 0 || i == L)
-            part[ppos[i]] = v;
-        else
+        if (skl || i == 0 || i == L) {
+            if (part)
+                part[ppos[i]] = v;
+            else
+                atomic_add(&y[gid[i]], v);
+        } else {
             y[gid[i]] = old[i] + v;
+        }
         // slide
         if (i < L) {
 #pragma unroll
@@ -1091,8 +1098,11 @@ int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStrea
 template <typename T, typename G>
 int sshex8_laplacian_apply(int L, ptrdiff_t E, const int32_t *idx, const G *fff, ptrdiff_t fstride,
                            const float *coeff, const double *Kem, const T *x, T *y, T *part, const SkeletonMap &sk,
-                           hipStream_t s, int overwrite) {
+                           hipStream_t s, int overwrite, bool atomic_skeleton) {
     if (E == 0) return SFEM_SUCCESS;
+    // atomic skeleton sums: accumulating, unconstrained applies only
+    const bool atomic = atomic_skeleton && !overwrite && !sk.cmask;
+    if (atomic) part = nullptr;
     switch (L) {
 #define CASE(LV)                                                                                                  \
     case LV: {                                                                                                    \
@@ -1109,6 +1119,7 @@ int sshex8_laplacian_apply(int L, ptrdiff_t E, const int32_t *idx, const G *fff,
             return SFEM_FAILURE;
     }
     SFEM_AMD_HIP_CHECK(hipGetLastError());
+    if (atomic) return SFEM_SUCCESS;
     return skeleton_gather<T>(sk, 1, part, y, s, overwrite, -1, x);
 }
 
@@ -1286,10 +1297,10 @@ template int skeleton_gather<float>(const SkeletonMap &, int, const float *, flo
                                     const float *, const ChebStep<float> &);
 template int sshex8_laplacian_apply<double, float>(int, ptrdiff_t, const int32_t *, const float *, ptrdiff_t,
                                                    const float *, const double *, const double *, double *, double *,
-                                                   const SkeletonMap &, hipStream_t, int);
+                                                   const SkeletonMap &, hipStream_t, int, bool);
 template int sshex8_laplacian_apply<float, uint16_t>(int, ptrdiff_t, const int32_t *, const uint16_t *, ptrdiff_t,
                                                      const float *, const double *, const float *, float *, float *,
-                                                     const SkeletonMap &, hipStream_t, int);
+                                                     const SkeletonMap &, hipStream_t, int, bool);
 template int sshex8_laplacian_diag<double, float>(int, ptrdiff_t, const int32_t *, const float *, ptrdiff_t,
                                                   const float *, const double *, double *, hipStream_t);
 template int sshex8_laplacian_diag<float, uint16_t>(int, ptrdiff_t, const int32_t *, const uint16_t *, ptrdiff_t,

@@ -37,9 +37,11 @@ int skeleton_gather(const SkeletonMap &sk, int bs, const T *part, T *y, hipStrea
 template <typename T, typename G>
 // part: E * sk_count(L) values of scratch (the skeleton contributions)
 // Kem != NULL: stored sub-element matrices [E][64] replace the FFF (fff unused)
+// atomic_skeleton (accumulating applies without constrained rows): skeleton contributions are
+// added to y with atomics (no partials, no gather; summation order not reproducible)
 int sshex8_laplacian_apply(int L, ptrdiff_t E, const int32_t *idx, const G *fff, ptrdiff_t fstride,
                            const float *coeff, const double *Kem, const T *x, T *y, T *part, const SkeletonMap &sk,
-                           hipStream_t s, int overwrite = 0);
+                           hipStream_t s, int overwrite = 0, bool atomic_skeleton = false);
 template <typename T, typename G>
 int sshex8_laplacian_diag(int L, ptrdiff_t E, const int32_t *idx, const G *fff, ptrdiff_t fstride,
                           const float *coeff, const double *Kem, T *d, hipStream_t s);

@@ -129,6 +129,35 @@ def test_sshex8_laplacian_fp32_bf16_error_stated():
     assert err < 1e-2
 
 
+@pytest.mark.parametrize("L,n", [(2, 3), (4, 2), (8, 3)])
+def test_sshex8_laplacian_atomic_skeleton(L, n):
+    """Option "atomic_skeleton": accumulating applies add the skeleton contributions with atomics and
+    still match the oracle (FP64) and the two-phase sum (FP32, up to summation order)."""
+    mel, mpts = skewed_box(n, perturb=0.02)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    fff = O.hex8_fff(mel, mpts)
+    N = pts.shape[1]
+    x = np.random.default_rng(3).uniform(-1, 1, N)
+    yo = np.zeros(N)
+    O.lib.oracle_sshex8_laplacian_apply(L, mel.shape[1], el, fff, None, x, yo)
+    m = S.Mesh(S.SSHEX8, L, el, N)
+    op = S.laplacian(m, fff)
+    op32 = S.laplacian(m, fff, real_type=S.FLOAT32)
+    xd, x32 = dev(x), dev(x, np.float32)
+    ref32 = host(gpu_apply(op32, x32)).astype(np.float64)
+    y64 = dev(np.full(N, 0.5))  # accumulate semantics: y += A x
+    y32 = zeros(N, np.float32)
+    S.set_option("atomic_skeleton", 1)
+    try:
+        op.apply(xd, y64)
+        op32.apply(x32, y32)
+        S.synchronize()
+    finally:
+        S.set_option("atomic_skeleton", 0)
+    assert relerr(host(y64) - 0.5, yo) < RTOL
+    assert relerr(host(y32).astype(np.float64), ref32) < 1e-5
+
+
 def test_empty_mesh_apply_is_noop():
     el = np.zeros((27, 0), np.int32)
     m = S.Mesh(S.SSHEX8, 2, el, 5)
