@@ -33,7 +33,8 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(const int L, const ptrdif
     }
     __syncthreads();
     const int N1 = L + 1;
-    for (int t = threadIdx.x; t < Cout * nxe; t += blockDim.x) {
+    // blockIdx.y splits the element's outputs over gridDim.y blocks (each stages the whole input)
+    for (int t = blockIdx.y * blockDim.x + threadIdx.x; t < Cout * nxe; t += blockDim.x * gridDim.y) {
         const int co = t / nxe, v = t - co * nxe;
         const int x = v % N1, y = (v / N1) % N1, z = v / (N1 * N1);
         int nb[27];
@@ -67,7 +68,7 @@ __global__ void __launch_bounds__(256) conv_bwd_data_kernel(const int L, const p
     for (int i = threadIdx.x; i < Cout * nxe; i += blockDim.x) sd[i] = dout[(size_t)e * Cout * nxe + i];
     __syncthreads();
     const int N1 = L + 1;
-    for (int t = threadIdx.x; t < Cin * nxe; t += blockDim.x) {
+    for (int t = blockIdx.y * blockDim.x + threadIdx.x; t < Cin * nxe; t += blockDim.x * gridDim.y) {
         const int ci = t / nxe, v = t - ci * nxe;
         const int x = v % N1, y = (v / N1) % N1, z = v / (N1 * N1);
         float acc = 0.f;
@@ -232,10 +233,17 @@ __global__ void scatter_elements_kernel(const ptrdiff_t E, const int nxe, const 
 // ------------------------------------------------------------------ launchers
 static size_t smem_fwd(int L, int Cin, int /*Cout*/) { return sizeof(float) * ((size_t)Cin * nxe_of(L)); }
 
+// blocks per element for n outputs of 256-thread blocks (training batches hold few elements)
+static unsigned conv_split(int n) {
+    const int b = (n + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > 16 ? 16 : b));
+}
+
 int cnn_conv_forward(int L, ptrdiff_t E, int Cin, int Cout, const float *in, const float *W, int relu_in, float *out,
                      hipStream_t s) {
     if (E == 0) return SFEM_SUCCESS;
-    conv_fwd_kernel<<<(unsigned)E, 256, smem_fwd(L, Cin, Cout), s>>>(L, E, Cin, Cout, in, W, relu_in, out);
+    conv_fwd_kernel<<<dim3((unsigned)E, conv_split(Cout * nxe_of(L))), 256, smem_fwd(L, Cin, Cout), s>>>(
+        L, E, Cin, Cout, in, W, relu_in, out);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
 }
@@ -244,7 +252,8 @@ int cnn_conv_backward_data(int L, ptrdiff_t E, int Cin, int Cout, const float *d
                            int relu_in, float *din, hipStream_t s) {
     if (E == 0) return SFEM_SUCCESS;
     const size_t smem = sizeof(float) * ((size_t)Cout * nxe_of(L));
-    conv_bwd_data_kernel<<<(unsigned)E, 256, smem, s>>>(L, E, Cin, Cout, dout, W, act_in, relu_in, din);
+    conv_bwd_data_kernel<<<dim3((unsigned)E, conv_split(Cin * nxe_of(L))), 256, smem, s>>>(L, E, Cin, Cout, dout, W,
+                                                                                         act_in, relu_in, din);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
 }
