@@ -77,9 +77,20 @@ ptrdiff_t sfem_amd_box_nnodes(int nx, int ny, int nz);
 ptrdiff_t sfem_amd_box_nelements(int nx, int ny, int nz, int element_type);
 int sfem_amd_box_mesh(int nx, int ny, int nz, double w, double h, double t, int element_type,
                       int32_t *elements, float *points);
-/* Hex8 -> sshex8 level L with hierarchical numbering (derefined levels are prefixes);
- * replaces smesh::to_semistructured (ssmg/tests/sfem_SSGMGTest.cpp:77-88).
- * With elements == NULL returns the node count only. */
+/* Hex8 -> sshex8 level L; replaces smesh::to_semistructured(L, mesh, hierarchical, false)
+ * (ssmg/tests/sfem_SSGMGTest.cpp:77-88) with the numbering of the in-tree text
+ * (mesh/sfem_hex8_mesh_graph.cpp:448-735 sshex8_generate_elements, :994-1060
+ * sshex8_hierarchical_renumbering).  Raw ids: corners keep the macro ids, then edge-interior nodes
+ * (edges in CRS (i<j) order, running from the smaller corner id), side-interior nodes (sides in
+ * first-processed (element, side) order, walked from the side's smallest corner id), volume nodes.
+ * hierarchical != 0: corners keep their ids, the other nodes are renumbered by first encounter over
+ * (level 1 < ... < L/2 < L, element, z, y, x), so every derefined level is a prefix of the
+ * numbering.  Levels: 1, then L, L/2, ... while even (8: 1 2 4 8; 12: 1 6 12); for odd L > 1,
+ * where the reference's renumbering aborts, 1 and L.  With elements == NULL returns the node
+ * count only.  sfem_amd_sshex8_refine = the hierarchical numbering. */
+ptrdiff_t sfem_amd_sshex8_refine_ex(int level, int hierarchical, ptrdiff_t nelements, ptrdiff_t nnodes,
+                                    const int32_t *macro_elements, const float *macro_points,
+                                    int32_t *elements, float *points);
 ptrdiff_t sfem_amd_sshex8_refine(int level, ptrdiff_t nelements, ptrdiff_t nnodes,
                                  const int32_t *macro_elements, const float *macro_points,
                                  int32_t *elements, float *points);

@@ -91,251 +91,393 @@ int oracle_sshex8_nxe(int L) { return (L + 1) * (L + 1) * (L + 1); }
 
 static inline int lidx(int L, int x, int y, int z) { return (z * (L + 1) + y) * (L + 1) + x; }
 
-/* Standard hex8 corner (bits: x=c&1?...) -> structured coordinate. Corner order of
- * box_mesh.py:520-534 / operators/sshex8/sshex8.hpp:53-62. */
+/* Standard hex8 corner -> structured coordinate: the corner order of box_mesh.py:520-534 and
+ * operators/sshex8/sshex8.hpp:53-62 (lagr_to_proteus_corners, mesh/sfem_hex8_mesh_graph.cpp:451-459). */
 static const int hex8_cx[8] = {0, 1, 1, 0, 0, 1, 1, 0};
 static const int hex8_cy[8] = {0, 0, 1, 1, 0, 0, 1, 1};
 static const int hex8_cz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
 
-/* Local edges (pairs of corners) and faces (4 corners, cyclic) of a hex8. */
-static const int hex8_edges[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {4, 5}, {5, 6},
-                                      {7, 6}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
-static const int hex8_faces[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4},
-                                     {3, 2, 6, 7}, {0, 3, 7, 4}, {1, 2, 6, 5}};
+/* hex8_edge_connectivity, LAGRANGE order (mesh/sfem_hex8_mesh_graph.cpp:37-47 and the local
+ * copy lagr_connectivity at :534-543): the three corners joined to each corner by an edge. */
+static const int hex8_edge_conn[8][3] = {{1, 3, 4}, {0, 2, 5}, {1, 3, 6}, {0, 2, 7},
+                                         {0, 5, 7}, {1, 4, 6}, {2, 5, 7}, {3, 4, 6}};
 
-typedef struct {
-    int32_t k[4];
-    ptrdiff_t id;
-} key4_t;
+/* HEX8 local side table (4 corners per side, cyclic).  The reference fills it with smesh's
+ * fill_local_side_table (mesh/sfem_hex8_mesh_graph.cpp:652), which is absent; this is the
+ * in-tree HEX8 side list of python/codegen/hex8.py:44-52 (exodus side order). */
+static const int hex8_sides[6][4] = {{0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6},
+                                     {3, 0, 4, 7}, {3, 2, 1, 0}, {4, 5, 6, 7}};
 
-static int cmp_key4(const void *a, const void *b) {
-    const key4_t *p = (const key4_t *)a, *q = (const key4_t *)b;
-    for (int i = 0; i < 4; i++) {
-        if (p->k[i] != q->k[i]) return p->k[i] < q->k[i] ? -1 : 1;
-    }
-    return p->id < q->id ? -1 : (p->id > q->id);
+static int cmp_i32(const void *a, const void *b) {
+    const int32_t p = *(const int32_t *)a, q = *(const int32_t *)b;
+    return (p > q) - (p < q);
 }
 
-static void sort4(int32_t *v) {
-    for (int i = 1; i < 4; i++)
-        for (int j = i; j > 0 && v[j - 1] > v[j]; j--) {
-            int32_t t = v[j];
-            v[j] = v[j - 1];
-            v[j - 1] = t;
+/* sortreduce: sort ascending and drop duplicates; returns the new length. */
+static int sortreduce_i32(int32_t *v, int n) {
+    if (n == 0) return 0;
+    qsort(v, (size_t)n, sizeof(int32_t), cmp_i32);
+    int m = 1;
+    for (int i = 1; i < n; i++)
+        if (v[i] != v[m - 1]) v[m++] = v[i];
+    return m;
+}
+
+/* node -> elements (CSR), elements in ascending order */
+static void build_n2e(ptrdiff_t ne, ptrdiff_t nn, const int32_t *me, ptrdiff_t **ptr_out, ptrdiff_t **idx_out) {
+    ptrdiff_t *ptr = (ptrdiff_t *)calloc((size_t)nn + 1, sizeof(ptrdiff_t));
+    for (int v = 0; v < 8; v++)
+        for (ptrdiff_t e = 0; e < ne; e++) ptr[me[v * ne + e] + 1]++;
+    for (ptrdiff_t i = 0; i < nn; i++) ptr[i + 1] += ptr[i];
+    ptrdiff_t *fill = (ptrdiff_t *)malloc(sizeof(ptrdiff_t) * ((size_t)nn + 1));
+    memcpy(fill, ptr, sizeof(ptrdiff_t) * ((size_t)nn + 1));
+    ptrdiff_t *idx = (ptrdiff_t *)malloc(sizeof(ptrdiff_t) * (size_t)(ptr[nn] + 1));
+    for (ptrdiff_t e = 0; e < ne; e++)
+        for (int v = 0; v < 8; v++) {
+            const int32_t n = me[v * ne + e];
+            /* an element listing a node twice is degenerate; keep one entry */
+            int dup = 0;
+            for (int w = 0; w < v; w++) dup |= me[w * ne + e] == n;
+            if (!dup) idx[fill[n]++] = e;
         }
+    /* entries were appended in element order: already ascending; fix the pointers for dups */
+    ptrdiff_t w = 0;
+    for (ptrdiff_t i = 0; i < nn; i++) {
+        const ptrdiff_t b = ptr[i], end = fill[i];
+        ptr[i] = w;
+        for (ptrdiff_t k = b; k < end; k++) idx[w++] = idx[k];
+    }
+    ptr[nn] = w;
+    free(fill);
+    *ptr_out = ptr;
+    *idx_out = idx;
+}
+
+/* hex8_build_edge_graph_from_n2e (mesh/sfem_hex8_mesh_graph.cpp:52-135): row i holds the
+ * sorted unique corners joined to node i by an element edge. */
+static void hex8_edge_graph(ptrdiff_t ne, ptrdiff_t nn, const int32_t *me, const ptrdiff_t *n2eptr,
+                            const ptrdiff_t *n2e, ptrdiff_t **rowptr_out, int32_t **colidx_out) {
+    ptrdiff_t *rowptr = (ptrdiff_t *)calloc((size_t)nn + 1, sizeof(ptrdiff_t));
+    ptrdiff_t maxdeg = 0;
+    for (ptrdiff_t i = 0; i < nn; i++)
+        if (n2eptr[i + 1] - n2eptr[i] > maxdeg) maxdeg = n2eptr[i + 1] - n2eptr[i];
+    int32_t *buf = (int32_t *)malloc(sizeof(int32_t) * (size_t)(3 * maxdeg + 1));
+    for (int pass = 0; pass < 2; pass++) {
+        int32_t *colidx = pass ? *colidx_out : NULL;
+        for (ptrdiff_t node = 0; node < nn; node++) {
+            int nb = 0;
+            for (ptrdiff_t k = n2eptr[node]; k < n2eptr[node + 1]; k++) {
+                const ptrdiff_t e = n2e[k];
+                int l = 0;
+                for (int v = 0; v < 8; v++)
+                    if (me[v * ne + e] == node) {
+                        l = v;
+                        break;
+                    }
+                for (int d = 0; d < 3; d++) buf[nb++] = me[hex8_edge_conn[l][d] * ne + e];
+            }
+            nb = sortreduce_i32(buf, nb);
+            if (!pass)
+                rowptr[node + 1] = nb;
+            else
+                memcpy(&colidx[rowptr[node]], buf, sizeof(int32_t) * (size_t)nb);
+        }
+        if (!pass) {
+            for (ptrdiff_t i = 0; i < nn; i++) rowptr[i + 1] += rowptr[i];
+            *colidx_out = (int32_t *)malloc(sizeof(int32_t) * (size_t)(rowptr[nn] + 1));
+        }
+    }
+    free(buf);
+    *rowptr_out = rowptr;
+}
+
+/* element adjacency through sides (the role of smesh's create_element_adj_table,
+ * mesh/sfem_hex8_mesh_graph.cpp:655): adj[e*6+f] = the other element holding the four corners of
+ * side f of e, or -1.  On a conforming hex8 mesh a side has at most two elements. */
+static void hex8_adj_table(ptrdiff_t ne, const int32_t *me, const ptrdiff_t *n2eptr, const ptrdiff_t *n2e,
+                           ptrdiff_t *adj) {
+    for (ptrdiff_t e = 0; e < ne; e++)
+        for (int f = 0; f < 6; f++) {
+            int32_t s[4];
+            for (int i = 0; i < 4; i++) s[i] = me[hex8_sides[f][i] * ne + e];
+            adj[e * 6 + f] = -1;
+            for (ptrdiff_t k = n2eptr[s[0]]; k < n2eptr[s[0] + 1]; k++) {
+                const ptrdiff_t o = n2e[k];
+                if (o == e) continue;
+                int hits = 0;
+                for (int i = 1; i < 4; i++)
+                    for (int v = 0; v < 8; v++)
+                        if (me[v * ne + o] == s[i]) {
+                            hits++;
+                            break;
+                        }
+                if (hits == 3) {
+                    adj[e * 6 + f] = o;
+                    break;
+                }
+            }
+        }
+}
+
+/* index_face (mesh/sfem_hex8_mesh_graph.cpp:346-446): numbers the interior nodes of side f of
+ * element e.  o = the side corner with the smallest global id, u and v its two neighbours on the
+ * side ordered by global id; nodes run u fastest, then v, starting next to o. */
+static void index_face(int L, ptrdiff_t ne, const int32_t *me, ptrdiff_t face_offset, ptrdiff_t e, int f,
+                       int32_t *elements) {
+    int argmin = 0;
+    int32_t valmin = me[hex8_sides[f][0] * ne + e];
+    for (int i = 0; i < 4; i++) {
+        const int32_t t = me[hex8_sides[f][i] * ne + e];
+        if (t < valmin) {
+            argmin = i;
+            valmin = t;
+        }
+    }
+    const int lst_o = argmin;
+    int lst_u = (lst_o + 1) % 4, lst_v = (lst_o + 3) % 4;
+    if (me[hex8_sides[f][lst_u] * ne + e] > me[hex8_sides[f][lst_v] * ne + e]) {
+        const int t = lst_v;
+        lst_v = lst_u;
+        lst_u = t;
+    }
+    const int co = hex8_sides[f][lst_o], cu = hex8_sides[f][lst_u], cv = hex8_sides[f][lst_v];
+    const int oc[3] = {hex8_cx[co] * L, hex8_cy[co] * L, hex8_cz[co] * L};
+    const int uc[3] = {hex8_cx[cu] * L, hex8_cy[cu] * L, hex8_cz[cu] * L};
+    const int vc[3] = {hex8_cx[cv] * L, hex8_cy[cv] * L, hex8_cz[cv] * L};
+    int o_start[3], u_len[3], u_dir[3], v_len[3], v_dir[3];
+    for (int d = 0; d < 3; d++) o_start[d] = oc[d];
+    for (int d = 0; d < 3; d++) {
+        int x = uc[d] - oc[d];
+        u_dir[d] = 1;
+        u_len[d] = 1;
+        if (x > 0) {
+            u_len[d] = x - 1;
+            o_start[d] = 1;
+        } else if (x < 0) {
+            u_len[d] = x + 1;
+            u_dir[d] = -1;
+            o_start[d] = L - 1;
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        int x = vc[d] - oc[d];
+        v_dir[d] = 1;
+        v_len[d] = 1;
+        if (x > 0) {
+            v_len[d] = x - 1;
+            o_start[d] = 1;
+        } else if (x < 0) {
+            v_len[d] = x + 1;
+            v_dir[d] = -1;
+            o_start[d] = L - 1;
+        }
+    }
+    ptrdiff_t local = 0;
+    for (int vz = 0; vz != v_len[2]; vz += v_dir[2])
+        for (int vy = 0; vy != v_len[1]; vy += v_dir[1])
+            for (int vx = 0; vx != v_len[0]; vx += v_dir[0])
+                for (int uz = 0; uz != u_len[2]; uz += u_dir[2])
+                    for (int uy = 0; uy != u_len[1]; uy += u_dir[1])
+                        for (int ux = 0; ux != u_len[0]; ux += u_dir[0]) {
+                            const int p = lidx(L, ux + vx + o_start[0], uy + vy + o_start[1], uz + vz + o_start[2]);
+                            elements[(size_t)p * ne + e] = (int32_t)(face_offset + local++);
+                        }
 }
 
 /*
- * sshex8 refinement with hierarchical numbering.  Restates the in-tree
- * historical algorithm (mesh/sfem_hex8_mesh_graph.cpp:346-735, commented out; the
- * live one is smesh::to_semistructured, absent from the reference checkout):
- *   - corners keep their macro ids;
- *   - edge-interior nodes are numbered per unique edge (edges ordered by first
- *     appearance), running from the lower to the higher global corner id;
- *   - face-interior nodes are numbered per unique face (first appearance), with
- *     the origin at the face's min-id corner and u towards its lower-id neighbour;
- *   - volume-interior nodes are base + e*(L-1)^3 + local (x fastest);
- * then the hierarchical renumbering (:994-1080) stably moves nodes that exist at
- * coarser levels (L, L/2, ... down to the odd part, then 1) to the front, so each
- * derefined level is a prefix of the fine numbering.
+ * sshex8_generate_elements (mesh/sfem_hex8_mesh_graph.cpp:448-735, commented out in the
+ * checkout; the live one is smesh::to_semistructured, absent).  Raw numbering:
+ *   1) corners keep their macro ids;
+ *   2) edge-interior nodes: index_base + edge_id*(L-1) + en, where edge ids enumerate the pairs
+ *      (i, j), i < j, of the hex8 edge graph in CRS order (rows ascending, sorted columns), and
+ *      en runs from the lower-id corner towards the higher-id one (:510-633);
+ *   3) face-interior nodes: sides visited element by element, side by side; a side already
+ *      reached from a lower-numbered neighbour is skipped; both elements of a side are filled
+ *      with index_face (:636-697);
+ *   4) volume-interior nodes: base + e*(L-1)^3 + local, x fastest (:702-722).
+ * elements: SoA [(L+1)^3][ne].  Returns the node count (interior_start + ne*(L-1)^3).
  */
-static int level_rank(int L, int x, int y, int z) {
-    /* levels: 1, then the odd part m of L, 2m, 4m, ..., L.  Rank 0 = coarsest. */
-    if (x % L == 0 && y % L == 0 && z % L == 0) return 0;
-    int m = L;
-    while (m % 2 == 0) m /= 2;
-    /* level l = m * 2^k has grid step L/l */
-    int rank = 1;
-    for (int l = m; l <= L; l *= 2, rank++) {
-        if (l == 1) continue;
-        int s = L / l;
-        if (x % s == 0 && y % s == 0 && z % s == 0) return rank;
-    }
-    return rank;
-}
+static ptrdiff_t sshex8_generate_elements(int L, ptrdiff_t ne, ptrdiff_t nnodes, const int32_t *me,
+                                          int32_t *elements) {
+    ptrdiff_t *n2eptr, *n2e;
+    build_n2e(ne, nnodes, me, &n2eptr, &n2e);
 
-ptrdiff_t oracle_sshex8_refine(int L, ptrdiff_t ne, ptrdiff_t nnodes, const int32_t *me,
-                               const float *mp, int32_t *elements, float *points) {
-    const int nxe = oracle_sshex8_nxe(L);
-    const int Lm1 = L - 1;
-
-    /* unique edges */
-    key4_t *ek = (key4_t *)malloc(sizeof(key4_t) * 12 * (size_t)ne);
-    for (ptrdiff_t e = 0; e < ne; e++)
-        for (int k = 0; k < 12; k++) {
-            int32_t a = me[hex8_edges[k][0] * ne + e], b = me[hex8_edges[k][1] * ne + e];
-            key4_t *q = &ek[e * 12 + k];
-            q->k[0] = a < b ? a : b;
-            q->k[1] = a < b ? b : a;
-            q->k[2] = q->k[3] = 0;
-            q->id = e * 12 + k;
-        }
-    qsort(ek, 12 * (size_t)ne, sizeof(key4_t), cmp_key4);
-    /* edge id by first appearance: first the minimum occurrence per key */
-    ptrdiff_t *occ_edge = (ptrdiff_t *)malloc(sizeof(ptrdiff_t) * 12 * (size_t)ne); /* occurrence -> unique slot */
-    ptrdiff_t nedges = 0;
-    ptrdiff_t *first_occ = (ptrdiff_t *)malloc(sizeof(ptrdiff_t) * 12 * (size_t)ne);
-    for (ptrdiff_t i = 0; i < 12 * ne; i++) {
-        if (i == 0 || ek[i].k[0] != ek[i - 1].k[0] || ek[i].k[1] != ek[i - 1].k[1]) {
-            first_occ[nedges] = ek[i].id;
-            nedges++;
-        }
-        occ_edge[ek[i].id] = nedges - 1;
+    for (int d = 0; d < 8; d++) {
+        const int p = lidx(L, hex8_cx[d] * L, hex8_cy[d] * L, hex8_cz[d] * L);
+        for (ptrdiff_t e = 0; e < ne; e++) elements[(size_t)p * ne + e] = me[d * ne + e];
     }
-    /* rank unique edges by first occurrence */
-    ptrdiff_t *edge_rank = (ptrdiff_t *)malloc(sizeof(ptrdiff_t) * (size_t)(nedges + 1));
-    {
-        key4_t *tmp = (key4_t *)malloc(sizeof(key4_t) * (size_t)(nedges + 1));
-        for (ptrdiff_t i = 0; i < nedges; i++) {
-            tmp[i].k[2] = tmp[i].k[3] = 0;
-            tmp[i].id = i;
-            tmp[i].k[0] = (int32_t)(first_occ[i] >> 31);
-            tmp[i].k[1] = (int32_t)(first_occ[i] & 0x7fffffff);
-        }
-        qsort(tmp, nedges, sizeof(key4_t), cmp_key4);
-        for (ptrdiff_t i = 0; i < nedges; i++) edge_rank[tmp[i].id] = i;
-        free(tmp);
-    }
+    ptrdiff_t index_base = nnodes;
 
-    /* unique faces */
-    key4_t *fk = (key4_t *)malloc(sizeof(key4_t) * 6 * (size_t)ne);
-    for (ptrdiff_t e = 0; e < ne; e++)
-        for (int f = 0; f < 6; f++) {
-            key4_t *q = &fk[e * 6 + f];
-            for (int i = 0; i < 4; i++) q->k[i] = me[hex8_faces[f][i] * ne + e];
-            sort4(q->k);
-            q->id = e * 6 + f;
-        }
-    qsort(fk, 6 * (size_t)ne, sizeof(key4_t), cmp_key4);
-    ptrdiff_t *occ_face = (ptrdiff_t *)malloc(sizeof(ptrdiff_t) * 6 * (size_t)ne);
-    ptrdiff_t nfaces = 0;
-    ptrdiff_t *ffirst = (ptrdiff_t *)malloc(sizeof(ptrdiff_t) * 6 * (size_t)ne);
-    for (ptrdiff_t i = 0; i < 6 * ne; i++) {
-        if (i == 0 || memcmp(fk[i].k, fk[i - 1].k, sizeof(int32_t) * 4) != 0) {
-            ffirst[nfaces] = fk[i].id;
-            nfaces++;
-        }
-        occ_face[fk[i].id] = nfaces - 1;
-    }
-    ptrdiff_t *face_rank = (ptrdiff_t *)malloc(sizeof(ptrdiff_t) * (size_t)(nfaces + 1));
-    {
-        key4_t *tmp = (key4_t *)malloc(sizeof(key4_t) * (size_t)(nfaces + 1));
-        for (ptrdiff_t i = 0; i < nfaces; i++) {
-            tmp[i].k[0] = (int32_t)(ffirst[i] >> 31);
-            tmp[i].k[1] = (int32_t)(ffirst[i] & 0x7fffffff);
-            tmp[i].k[2] = tmp[i].k[3] = 0;
-            tmp[i].id = i;
-        }
-        qsort(tmp, nfaces, sizeof(key4_t), cmp_key4);
-        for (ptrdiff_t i = 0; i < nfaces; i++) face_rank[tmp[i].id] = i;
-        free(tmp);
-    }
+    const int nxedge = L - 1;
+    if (nxedge > 0) {
+        ptrdiff_t *rowptr;
+        int32_t *colidx = NULL;
+        hex8_edge_graph(ne, nnodes, me, n2eptr, n2e, &rowptr, &colidx);
+        const ptrdiff_t nnz = rowptr[nnodes];
+        int32_t *edge_idx = (int32_t *)calloc((size_t)nnz + 1, sizeof(int32_t));
+        int32_t next_id = 0;
+        for (ptrdiff_t i = 0; i < nnodes; i++)
+            for (ptrdiff_t k = rowptr[i]; k < rowptr[i + 1]; k++)
+                if (i < colidx[k]) edge_idx[k] = next_id++;
+        const ptrdiff_t nedges = next_id;
 
-    const ptrdiff_t base_edge = nnodes;
-    const ptrdiff_t base_face = base_edge + nedges * Lm1;
-    const ptrdiff_t base_vol = base_face + nfaces * (ptrdiff_t)Lm1 * Lm1;
-    const ptrdiff_t ntotal = base_vol + ne * (ptrdiff_t)Lm1 * Lm1 * Lm1;
-
-    if (!elements) {
-        free(ek); free(occ_edge); free(first_occ); free(edge_rank);
-        free(fk); free(occ_face); free(ffirst); free(face_rank);
-        return ntotal;
-    }
-
-    int32_t *tmp_el = (int32_t *)malloc(sizeof(int32_t) * (size_t)nxe * ne);
-    for (ptrdiff_t e = 0; e < ne; e++) {
-        int32_t c[8];
-        for (int v = 0; v < 8; v++) c[v] = me[v * ne + e];
-        for (int z = 0; z <= L; z++)
-            for (int y = 0; y <= L; y++)
-                for (int x = 0; x <= L; x++) {
-                    const int bx = (x == 0 || x == L), by = (y == 0 || y == L), bz = (z == 0 || z == L);
-                    const int nb = bx + by + bz;
-                    ptrdiff_t id = -1;
-                    if (nb == 3) {
-                        for (int v = 0; v < 8; v++)
-                            if (hex8_cx[v] * L == x && hex8_cy[v] * L == y && hex8_cz[v] * L == z) id = c[v];
-                    } else if (nb == 2) {
-                        /* edge: find local edge whose two corners bracket this node */
-                        for (int k = 0; k < 12; k++) {
-                            int a = hex8_edges[k][0], b = hex8_edges[k][1];
-                            int ax = hex8_cx[a] * L, ay = hex8_cy[a] * L, az = hex8_cz[a] * L;
-                            int bxx = hex8_cx[b] * L, byy = hex8_cy[b] * L;
-                            /* node lies on segment a-b? */
-                            int t;
-                            if (ax != bxx) {
-                                if (y != ay || z != az) continue;
-                                t = x;
-                            } else if (ay != byy) {
-                                if (x != ax || z != az) continue;
-                                t = y;
-                            } else {
-                                if (x != ax || y != ay) continue;
-                                t = z;
-                            }
-                            /* t measured from corner a (a has the 0 coordinate on this axis) */
-                            int ga = c[a], gb = c[b];
-                            int tt = ga < gb ? t : L - t;
-                            id = base_edge + edge_rank[occ_edge[e * 12 + k]] * Lm1 + (tt - 1);
-                            break;
+        for (ptrdiff_t e = 0; e < ne; e++) {
+            int32_t nodes[8];
+            for (int d = 0; d < 8; d++) nodes[d] = me[d * ne + e];
+            for (int d1 = 0; d1 < 8; d1++) {
+                const int32_t node1 = nodes[d1];
+                for (int d2 = 0; d2 < 3; d2++) {
+                    const int c2 = hex8_edge_conn[d1][d2];
+                    const int32_t node2 = nodes[c2];
+                    if (node1 > node2) continue; /* edges run from the smaller id */
+                    ptrdiff_t k = rowptr[node1];
+                    while (colidx[k] != node2) k++;
+                    const ptrdiff_t g_edge = edge_idx[k];
+                    const int c1[3] = {hex8_cx[d1] * L, hex8_cy[d1] * L, hex8_cz[d1] * L};
+                    const int cc[3] = {hex8_cx[c2] * L, hex8_cy[c2] * L, hex8_cz[c2] * L};
+                    int start[3], len[3], dir[3];
+                    for (int d = 0; d < 3; d++) {
+                        start[d] = c1[d];
+                        int x = cc[d] - c1[d];
+                        dir[d] = 1;
+                        len[d] = 1;
+                        if (x > 0) {
+                            len[d] = x - 1;
+                            start[d] = 1;
+                        } else if (x < 0) {
+                            len[d] = x + 1;
+                            dir[d] = -1;
+                            start[d] = L - 1;
                         }
-                    } else if (nb == 1) {
-                        int f;
-                        /* face of the node */
-                        if (z == 0) f = 0;
-                        else if (z == L) f = 1;
-                        else if (y == 0) f = 2;
-                        else if (y == L) f = 3;
-                        else if (x == 0) f = 4;
-                        else f = 5;
-                        /* origin = min-id corner of the face; u = lower-id neighbour */
-                        int fc[4];
-                        for (int i = 0; i < 4; i++) fc[i] = hex8_faces[f][i];
-                        int am = 0;
-                        for (int i = 1; i < 4; i++)
-                            if (c[fc[i]] < c[fc[am]]) am = i;
-                        int o = fc[am], nu = fc[(am + 1) % 4], nv = fc[(am + 3) % 4];
-                        if (c[nu] > c[nv]) {
-                            int tmp = nu;
-                            nu = nv;
-                            nv = tmp;
-                        }
-                        /* coordinates along u and v, measured from o */
-                        int oc[3] = {hex8_cx[o] * L, hex8_cy[o] * L, hex8_cz[o] * L};
-                        int uc[3] = {hex8_cx[nu] * L, hex8_cy[nu] * L, hex8_cz[nu] * L};
-                        int vc[3] = {hex8_cx[nv] * L, hex8_cy[nv] * L, hex8_cz[nv] * L};
-                        int p[3] = {x, y, z};
-                        int iu = 0, iv = 0;
-                        for (int d = 0; d < 3; d++) {
-                            if (uc[d] != oc[d]) iu = (uc[d] > oc[d]) ? p[d] - oc[d] : oc[d] - p[d];
-                            if (vc[d] != oc[d]) iv = (vc[d] > oc[d]) ? p[d] - oc[d] : oc[d] - p[d];
-                        }
-                        id = base_face + face_rank[occ_face[e * 6 + f]] * (ptrdiff_t)Lm1 * Lm1 + (iv - 1) * Lm1 +
-                             (iu - 1);
-                    } else {
-                        id = base_vol + e * (ptrdiff_t)Lm1 * Lm1 * Lm1 + ((z - 1) * Lm1 + (y - 1)) * Lm1 + (x - 1);
                     }
-                    tmp_el[lidx(L, x, y, z) * ne + e] = (int32_t)id;
+                    const ptrdiff_t edge_start = index_base + g_edge * nxedge;
+                    int en = 0;
+                    for (int zi = 0; zi != len[2]; zi += dir[2])
+                        for (int yi = 0; yi != len[1]; yi += dir[1])
+                            for (int xi = 0; xi != len[0]; xi += dir[0]) {
+                                const int p = lidx(L, start[0] + xi, start[1] + yi, start[2] + zi);
+                                elements[(size_t)p * ne + e] = (int32_t)(edge_start + en++);
+                            }
+                }
+            }
+        }
+        free(edge_idx);
+        free(rowptr);
+        free(colidx);
+        index_base += nedges * nxedge;
+    }
+
+    const ptrdiff_t nxf = (ptrdiff_t)(L - 1) * (L - 1);
+    if (nxf > 0) {
+        ptrdiff_t *adj = (ptrdiff_t *)malloc(sizeof(ptrdiff_t) * 6 * (size_t)ne + 1);
+        hex8_adj_table(ne, me, n2eptr, n2e, adj);
+        ptrdiff_t n_unique_faces = 0;
+        for (ptrdiff_t e = 0; e < ne; e++)
+            for (int f = 0; f < 6; f++) {
+                const ptrdiff_t nb = adj[e * 6 + f];
+                if (nb >= 0 && nb < e) continue;
+                const ptrdiff_t offset = index_base + n_unique_faces * nxf;
+                index_face(L, ne, me, offset, e, f, elements);
+                if (nb >= 0) {
+                    int nf = 0;
+                    for (; nf < 6; nf++)
+                        if (adj[nb * 6 + nf] == e) break;
+                    index_face(L, ne, me, offset, nb, nf, elements);
+                }
+                n_unique_faces++;
+            }
+        free(adj);
+        index_base += n_unique_faces * nxf;
+    }
+
+    const ptrdiff_t nxelement = (ptrdiff_t)(L - 1) * (L - 1) * (L - 1);
+    if (nxelement > 0) {
+        const int Lm1 = L - 1;
+        for (int zi = 1; zi < L; zi++)
+            for (int yi = 1; yi < L; yi++)
+                for (int xi = 1; xi < L; xi++) {
+                    const int p = lidx(L, xi, yi, zi);
+                    const ptrdiff_t en = ((ptrdiff_t)(zi - 1) * Lm1 + (yi - 1)) * Lm1 + xi - 1;
+                    for (ptrdiff_t e = 0; e < ne; e++)
+                        elements[(size_t)p * ne + e] = (int32_t)(index_base + e * nxelement + en);
                 }
     }
+    free(n2eptr);
+    free(n2e);
+    return index_base + ne * nxelement;
+}
 
-    /* hierarchical renumbering: rank of first level, stable on the id */
-    int *rank = (int *)malloc(sizeof(int) * (size_t)ntotal);
-    for (ptrdiff_t i = 0; i < ntotal; i++) rank[i] = -1;
-    for (ptrdiff_t e = 0; e < ne; e++)
-        for (int z = 0; z <= L; z++)
-            for (int y = 0; y <= L; y++)
-                for (int x = 0; x <= L; x++) rank[tmp_el[lidx(L, x, y, z) * ne + e]] = level_rank(L, x, y, z);
-    int maxrank = 0;
-    for (ptrdiff_t i = 0; i < ntotal; i++)
-        if (rank[i] > maxrank) maxrank = rank[i];
-    ptrdiff_t *perm = (ptrdiff_t *)malloc(sizeof(ptrdiff_t) * (size_t)ntotal);
-    ptrdiff_t next = 0;
-    for (int r = 0; r <= maxrank; r++)
-        for (ptrdiff_t i = 0; i < ntotal; i++)
-            if (rank[i] == r) perm[i] = next++;
-    for (ptrdiff_t i = 0; i < (ptrdiff_t)nxe * ne; i++) elements[i] = (int32_t)perm[tmp_el[i]];
+/* sshex8_hierarchical_n_levels / sshex8_hierarchical_mesh_levels
+ * (mesh/sfem_hex8_mesh_graph.cpp:964-992): 1, then the even chain ... L/4, L/2, L that ends at L
+ * (8 -> [1, 2, 4, 8]; 12 -> [1, 6, 12]; 6 -> [1, 6]).  For odd L > 1 the reference list is [1]
+ * and its renumbering stops with "Uninitialized node mapping"; this restatement appends L
+ * (extension: [1, L]), which changes nothing where the reference succeeds. */
+int oracle_sshex8_hierarchical_levels(int L, int *levels) {
+    int count = 0;
+    int l = L;
+    for (; l > 1 && l % 2 == 0; l /= 2) count++;
+    if (l >= 1) count++;
+    int n = count;
+    int c = 0;
+    l = L;
+    for (; l > 1 && l % 2 == 0; l /= 2) {
+        if (levels) levels[n - 1 - c] = l;
+        c++;
+    }
+    if (levels) levels[0] = 1;
+    if (n == 1 && L > 1) {
+        if (levels) levels[1] = L;
+        n = 2;
+    }
+    return n;
+}
 
+/* sshex8_hierarchical_renumbering (mesh/sfem_hex8_mesh_graph.cpp:994-1060): macro corners keep
+ * their ids; then, level by level from the coarsest (levels[1]) to L, elements in order and the
+ * level's nodes of each element in (z, y, x) order, every node not yet numbered takes the next id.
+ * Every derefined level is therefore a prefix of the numbering.  In place on elements [nxe][ne];
+ * returns the node count. */
+static ptrdiff_t sshex8_hierarchical_renumbering(int L, ptrdiff_t ne, ptrdiff_t nnodes, int32_t *elements) {
+    int levels[32];
+    const int nlevels = oracle_sshex8_hierarchical_levels(L, levels);
+    int32_t *map = (int32_t *)malloc(sizeof(int32_t) * ((size_t)nnodes + 1));
+    for (ptrdiff_t i = 0; i < nnodes; i++) map[i] = -1;
+    int32_t next_id = 0;
+    for (int zi = 0; zi <= 1; zi++)
+        for (int yi = 0; yi <= 1; yi++)
+            for (int xi = 0; xi <= 1; xi++)
+                for (ptrdiff_t e = 0; e < ne; e++) {
+                    const int32_t idx = elements[(size_t)lidx(L, xi * L, yi * L, zi * L) * ne + e];
+                    map[idx] = idx;
+                    if (idx > next_id) next_id = idx;
+                }
+    next_id++;
+    for (int k = 1; k < nlevels; k++) {
+        const int l = levels[k], sf = L / l;
+        for (ptrdiff_t e = 0; e < ne; e++)
+            for (int zi = 0; zi <= l; zi++)
+                for (int yi = 0; yi <= l; yi++)
+                    for (int xi = 0; xi <= l; xi++) {
+                        const int32_t idx = elements[(size_t)lidx(L, xi * sf, yi * sf, zi * sf) * ne + e];
+                        if (map[idx] == -1) map[idx] = next_id++;
+                    }
+    }
+    const ptrdiff_t nxe = oracle_sshex8_nxe(L);
+    for (ptrdiff_t i = 0; i < nxe * ne; i++) elements[i] = map[elements[i]];
+    free(map);
+    return next_id;
+}
+
+ptrdiff_t oracle_sshex8_refine_ex(int L, int hierarchical, ptrdiff_t ne, ptrdiff_t nnodes, const int32_t *me,
+                                  const float *mp, int32_t *elements, float *points) {
+    const int nxe = oracle_sshex8_nxe(L);
+    int32_t *el = elements ? elements : (int32_t *)malloc(sizeof(int32_t) * (size_t)nxe * ne + 4);
+    ptrdiff_t ntotal = sshex8_generate_elements(L, ne, nnodes, me, el);
+    if (hierarchical) ntotal = sshex8_hierarchical_renumbering(L, ne, ntotal, el);
+    if (!elements) {
+        free(el);
+        return ntotal;
+    }
     if (points) {
         for (ptrdiff_t e = 0; e < ne; e++) {
             float cx[8], cy[8], cz[8];
@@ -365,11 +507,12 @@ ptrdiff_t oracle_sshex8_refine(int L, ptrdiff_t ne, ptrdiff_t nnodes, const int3
                     }
         }
     }
-
-    free(perm); free(rank); free(tmp_el);
-    free(ek); free(occ_edge); free(first_occ); free(edge_rank);
-    free(fk); free(occ_face); free(ffirst); free(face_rank);
     return ntotal;
+}
+
+ptrdiff_t oracle_sshex8_refine(int L, ptrdiff_t ne, ptrdiff_t nnodes, const int32_t *me, const float *mp,
+                               int32_t *elements, float *points) {
+    return oracle_sshex8_refine_ex(L, 1, ne, nnodes, me, mp, elements, points);
 }
 
 void oracle_sshex8_derefine(int from_level, int to_level, ptrdiff_t ne, const int32_t *fine, int32_t *coarse) {

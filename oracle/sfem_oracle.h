@@ -47,12 +47,19 @@ void oracle_box_mesh(int nx, int ny, int nz, double w, double h, double t, int t
 
 /* Number of nodes of the sshex8 refinement of a hex8 mesh (no numbering). */
 int  oracle_sshex8_nxe(int L);
-/* Refine a hex8 mesh to sshex8 level L with hierarchical numbering.
+/* Refine a hex8 mesh to sshex8 level L: sshex8_generate_elements, then (hierarchical != 0)
+ * sshex8_hierarchical_renumbering (mesh/sfem_hex8_mesh_graph.cpp:448-735,994-1060).
  * Out: elements [nxe][nelements], returns node count; points [3][nnodes_out]
- * (pass NULL to query the count only).  See sfem_oracle.c for the numbering. */
+ * (elements == NULL: count only).  oracle_sshex8_refine = hierarchical numbering. */
+ptrdiff_t oracle_sshex8_refine_ex(int L, int hierarchical, ptrdiff_t nelements, ptrdiff_t nnodes,
+                                  const int32_t *macro_elements, const float *macro_points,
+                                  int32_t *elements, float *points);
 ptrdiff_t oracle_sshex8_refine(int L, ptrdiff_t nelements, ptrdiff_t nnodes,
                                const int32_t *macro_elements, const float *macro_points,
                                int32_t *elements, float *points);
+/* sshex8_hierarchical_mesh_levels (mesh/sfem_hex8_mesh_graph.cpp:964-992), coarsest first;
+ * returns the count (levels may be NULL).  Odd L > 1: [1, L] (extension). */
+int oracle_sshex8_hierarchical_levels(int L, int *levels);
 /* frontend/sfem_API.hpp:1412-1475: coarse row lidx(to,x,y,z) = fine row lidx(from, x*s, y*s, z*s). */
 void oracle_sshex8_derefine(int from_level, int to_level, ptrdiff_t nelements,
                             const int32_t *fine, int32_t *coarse);

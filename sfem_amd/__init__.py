@@ -35,6 +35,7 @@ SIGNATURES = {
     "sfem_amd_box_nelements": (_pd, [_i, _i, _i, _i]),
     "sfem_amd_box_mesh": (_i, [_i, _i, _i, _d, _d, _d, _i, _vp, _vp]),
     "sfem_amd_sshex8_refine": (_pd, [_i, _pd, _pd, _vp, _vp, _vp, _vp]),
+    "sfem_amd_sshex8_refine_ex": (_pd, [_i, _i, _pd, _pd, _vp, _vp, _vp, _vp]),
     "sfem_amd_fff": (_i, [_i, _i, _pd, _pd, _vp, _pd, _vp, _vp]),
     "sfem_amd_adjugate_det": (_i, [_i, _i, _pd, _pd, _vp, _pd, _vp, _vp, _vp]),
     "sfem_amd_mesh_create": (_i, [_i, _i, _pd, _pd, _vp, _pd, C.POINTER(_vp)]),
@@ -178,16 +179,19 @@ def box_mesh(nx, ny, nz, w=1.0, h=1.0, t=1.0, element_type=HEX8):
     return el, pts
 
 
-def sshex8_refine(level, macro_elements, macro_points):
+def sshex8_refine(level, macro_elements, macro_points, hierarchical=True):
+    """smesh::to_semistructured(level, mesh, hierarchical, false): connectivity [(L+1)^3][E] and
+    points [3][N] (hierarchical numbering by default: every derefined level is a prefix)."""
     me = _host(macro_elements, np.int32)
     mp = _host(macro_points, np.float32)
-    n = lib.sfem_amd_sshex8_refine(level, me.shape[1], mp.shape[1], _np_ptr(me), _np_ptr(mp), None, None)
+    h = int(bool(hierarchical))
+    n = lib.sfem_amd_sshex8_refine_ex(level, h, me.shape[1], mp.shape[1], _np_ptr(me), _np_ptr(mp), None, None)
     if n < 0:
         raise SfemError(lib.sfem_amd_last_error().decode())
     el = np.zeros(((level + 1) ** 3, me.shape[1]), np.int32)
     pts = np.zeros((3, n), np.float32)
-    n2 = lib.sfem_amd_sshex8_refine(level, me.shape[1], mp.shape[1], _np_ptr(me), _np_ptr(mp), _np_ptr(el),
-                                    _np_ptr(pts))
+    n2 = lib.sfem_amd_sshex8_refine_ex(level, h, me.shape[1], mp.shape[1], _np_ptr(me), _np_ptr(mp), _np_ptr(el),
+                                       _np_ptr(pts))
     if n2 != n:
         raise SfemError(lib.sfem_amd_last_error().decode())
     return el, pts

@@ -71,6 +71,44 @@ __host__ __device__ constexpr inline int hex8_corner(int x, int y, int z) {
     return z * 4 + (y ? (x ? 2 : 3) : (x ? 1 : 0));
 }
 
+// HEX8 sides, 4 corners each in cyclic order, in the side order of python/codegen/hex8.py:44-52
+// (y=0, x=1, y=1, x=0, z=0, z=1): the local side table the reference's sshex8 numbering walks
+// (mesh/sfem_hex8_mesh_graph.cpp:652, filled by the absent smesh fill_local_side_table).
+__host__ __device__ constexpr inline int hex8_side(int f, int i) {
+    constexpr int t[6][4] = {{0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}, {3, 2, 1, 0}, {4, 5, 6, 7}};
+    return t[f][i];
+}
+// side of a face-interior local node (exactly one coordinate in {0, L})
+__host__ __device__ constexpr inline int hex8_side_of(int L, int x, int y, int z) {
+    return y == 0 ? 0 : x == L ? 1 : y == L ? 2 : x == 0 ? 3 : z == 0 ? 4 : 5;
+}
+
+// Levels of the hierarchical sshex8 numbering, coarsest first (sshex8_hierarchical_mesh_levels,
+// mesh/sfem_hex8_mesh_graph.cpp:964-992): 1, then the chain ..., L/4, L/2, L while halving stays
+// even (8 -> 1 2 4 8; 12 -> 1 6 12; 6 -> 1 6).  For odd L > 1 the reference lists [1] only and its
+// renumbering aborts; this library uses [1, L].  Returns the count (levels may be null).
+__host__ __device__ inline int hierarchical_levels(int L, int *levels) {
+    int n = 1;
+    for (int l = L; l > 1 && l % 2 == 0; l /= 2) n++;
+    if (n == 1 && L > 1) n = 2;
+    if (levels) {
+        levels[0] = 1;
+        int k = n - 1;
+        levels[k] = L;
+        for (int l = L; l > 1 && l % 2 == 0 && k > 1; l /= 2) levels[--k] = l / 2;
+    }
+    return n;
+}
+// index k >= 1 of the coarsest level whose grid holds local node (x, y, z); 0 for macro corners
+__host__ __device__ inline int hierarchical_level_index(int L, const int *levels, int nlevels, int x, int y, int z) {
+    if (x % L == 0 && y % L == 0 && z % L == 0) return 0;
+    for (int k = 1; k < nlevels; k++) {
+        const int s = L / levels[k];
+        if (x % s == 0 && y % s == 0 && z % s == 0) return k;
+    }
+    return nlevels - 1;
+}
+
 // 1-D Q1 integrals on [0,1]: mass, stiffness, and C[i][j] = int psi_i' psi_j.
 __host__ __device__ inline double q1_mass(int i, int j) { return i == j ? 1.0 / 3.0 : 1.0 / 6.0; }
 __host__ __device__ inline double q1_stiff(int i, int j) { return i == j ? 1.0 : -1.0; }

@@ -20,13 +20,14 @@ float linspace_f32(int i, int n, double w) {
     return (float)((double)i * (w / (double)(n - 1)));
 }
 
+// the 12 hex8 edges as corner pairs
 constexpr int kEdges[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {4, 5}, {5, 6},
                                {7, 6}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
-constexpr int kFaces[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4}, {3, 2, 6, 7}, {0, 3, 7, 4}, {1, 2, 6, 5}};
 
-// unique keys ranked by first occurrence (occurrence = element*slots + slot)
+// occurrence -> rank of its key among the unique keys, ordered by key (by_key) or by the first
+// occurrence of each key (occurrence = element * slots + slot)
 template <typename Key>
-std::vector<int64_t> rank_by_first_occurrence(const std::vector<Key> &keys, int64_t &n_unique) {
+std::vector<int64_t> rank_keys(const std::vector<Key> &keys, bool by_key, int64_t &n_unique) {
     const int64_t n = (int64_t)keys.size();
     std::vector<int64_t> order(n);
     std::iota(order.begin(), order.end(), 0);
@@ -43,29 +44,14 @@ std::vector<int64_t> rank_by_first_occurrence(const std::vector<Key> &keys, int6
         }
         slot[order[i]] = n_unique - 1;
     }
-    // unique slot -> rank by first occurrence
+    if (by_key) return slot;
     std::vector<int64_t> by_first(n_unique);
     std::iota(by_first.begin(), by_first.end(), 0);
     std::sort(by_first.begin(), by_first.end(), [&](int64_t a, int64_t b) { return first[a] < first[b]; });
     std::vector<int64_t> rank(n_unique);
     for (int64_t r = 0; r < n_unique; r++) rank[by_first[r]] = r;
     for (int64_t i = 0; i < n; i++) slot[i] = rank[slot[i]];
-    return slot;  // occurrence -> rank of its unique key
-}
-
-// rank of the coarsest level at which local node (x,y,z) exists: 0 for corners, then
-// the odd part m of L, 2m, ... L (levels 1, m, 2m, ..., L; a level l has grid step L/l)
-int level_rank(int L, int x, int y, int z) {
-    if (x % L == 0 && y % L == 0 && z % L == 0) return 0;
-    int m = L;
-    while (m % 2 == 0) m /= 2;
-    int rank = 1;
-    for (int l = m; l <= L; l *= 2, rank++) {
-        if (l == 1) continue;
-        const int s = L / l;
-        if (x % s == 0 && y % s == 0 && z % s == 0) return rank;
-    }
-    return rank;
+    return slot;
 }
 
 void center_jacobian(const double *x, const double *y, const double *z, double *J) {
@@ -159,13 +145,23 @@ int sfem_amd_box_mesh(int nx, int ny, int nz, double w, double h, double t, int 
     return SFEM_SUCCESS;
 }
 
-ptrdiff_t sfem_amd_sshex8_refine(int L, ptrdiff_t ne, ptrdiff_t nnodes, const int32_t *me, const float *mp,
-                                 int32_t *elements, float *points) {
+// Hex8 -> sshex8 refinement (the role of smesh::to_semistructured, ssmg/tests/sfem_SSGMGTest.cpp:77-88;
+// numbering of the in-tree sshex8_generate_elements + sshex8_hierarchical_renumbering,
+// mesh/sfem_hex8_mesh_graph.cpp:448-735,994-1060):
+//   raw ids: corners keep the macro ids; edge-interior nodes base + edge*(L-1) + step from the
+//   smaller corner id, edges ranked by their (smaller, larger) corner ids (the CRS (i<j) order of
+//   the edge graph); side-interior nodes per side ranked by first occurrence (element, side) in the
+//   HEX8 side order, walked from the side's smallest-id corner o, u fastest (u, v: o's neighbours on
+//   the side, u the one with the smaller id); volume nodes base + e*(L-1)^3 + local, x fastest.
+//   hierarchical: corners keep their ids; the other nodes are numbered by first encounter over
+//   (level, element, z, y, x) of the level list 1, ..., L/2, L.
+ptrdiff_t sfem_amd_sshex8_refine_ex(int L, int hierarchical, ptrdiff_t ne, ptrdiff_t nnodes, const int32_t *me,
+                                    const float *mp, int32_t *elements, float *points) {
     if (L < 1) {
         set_last_error("sshex8 refine: level must be >= 1");
         return -1;
     }
-    const int Lm1 = L - 1, N1 = L + 1, nxe = nxe_of(L);
+    const int Lm1 = L - 1, nxe = nxe_of(L);
 
     std::vector<uint64_t> ekeys(12 * (size_t)ne);
     for (ptrdiff_t e = 0; e < ne; e++)
@@ -174,32 +170,30 @@ ptrdiff_t sfem_amd_sshex8_refine(int L, ptrdiff_t ne, ptrdiff_t nnodes, const in
             ekeys[e * 12 + k] = ((uint64_t)std::min(a, b) << 32) | std::max(a, b);
         }
     int64_t nedges = 0;
-    const std::vector<int64_t> edge_of = rank_by_first_occurrence(ekeys, nedges);
+    const std::vector<int64_t> edge_of = rank_keys(ekeys, /*by_key=*/true, nedges);
     ekeys.clear();
 
     std::vector<std::array<int32_t, 4>> fkeys(6 * (size_t)ne);
     for (ptrdiff_t e = 0; e < ne; e++)
         for (int f = 0; f < 6; f++) {
             std::array<int32_t, 4> k;
-            for (int i = 0; i < 4; i++) k[i] = me[kFaces[f][i] * ne + e];
+            for (int i = 0; i < 4; i++) k[i] = me[hex8_side(f, i) * ne + e];
             std::sort(k.begin(), k.end());
             fkeys[e * 6 + f] = k;
         }
     int64_t nfaces = 0;
-    const std::vector<int64_t> face_of = rank_by_first_occurrence(fkeys, nfaces);
+    const std::vector<int64_t> face_of = rank_keys(fkeys, /*by_key=*/false, nfaces);
     fkeys.clear();
 
     const int64_t base_edge = nnodes;
     const int64_t base_face = base_edge + nedges * Lm1;
     const int64_t base_vol = base_face + nfaces * (int64_t)Lm1 * Lm1;
-    const int64_t ntotal = base_vol + (int64_t)ne * Lm1 * Lm1 * Lm1;
-    if (!elements) return (ptrdiff_t)ntotal;
-    if (ntotal >= INT32_MAX) {
+    const int64_t nraw = base_vol + (int64_t)ne * Lm1 * Lm1 * Lm1;
+    if (nraw >= INT32_MAX) {
         set_last_error("sshex8 refine: node count exceeds int32");
         return -1;
     }
 
-    // local classification tables
     std::vector<int> vx(nxe), vy(nxe), vz(nxe);
     for (int z = 0; z <= L; z++)
         for (int y = 0; y <= L; y++)
@@ -241,12 +235,12 @@ ptrdiff_t sfem_amd_sshex8_refine(int L, ptrdiff_t ne, ptrdiff_t nnodes, const in
                     break;
                 }
                 case 1: {
-                    const int f = z == 0 ? 0 : z == L ? 1 : y == 0 ? 2 : y == L ? 3 : x == 0 ? 4 : 5;
+                    const int f = hex8_side_of(L, x, y, z);
                     int am = 0;
                     for (int i = 1; i < 4; i++)
-                        if (c[kFaces[f][i]] < c[kFaces[f][am]]) am = i;
-                    const int o = kFaces[f][am];
-                    int nu = kFaces[f][(am + 1) % 4], nv = kFaces[f][(am + 3) % 4];
+                        if (c[hex8_side(f, i)] < c[hex8_side(f, am)]) am = i;
+                    const int o = hex8_side(f, am);
+                    int nu = hex8_side(f, (am + 1) % 4), nv = hex8_side(f, (am + 3) % 4);
                     if (c[nu] > c[nv]) std::swap(nu, nv);
                     const int oc[3] = {hex8_cx(o) * L, hex8_cy(o) * L, hex8_cz(o) * L};
                     const int uc[3] = {hex8_cx(nu) * L, hex8_cy(nu) * L, hex8_cz(nu) * L};
@@ -267,24 +261,44 @@ ptrdiff_t sfem_amd_sshex8_refine(int L, ptrdiff_t ne, ptrdiff_t nnodes, const in
         }
     }
 
-    // hierarchical renumbering: stable counting sort of ids by level rank
-    std::vector<int8_t> rank((size_t)ntotal, -1);
-    int maxrank = 0;
-    std::vector<int> vrank(nxe);
-    for (int v = 0; v < nxe; v++) {
-        vrank[v] = level_rank(L, vx[v], vy[v], vz[v]);
-        maxrank = std::max(maxrank, vrank[v]);
+    int64_t ntotal = nraw;
+    std::vector<int32_t> map;  // raw id -> hierarchical id
+    if (hierarchical) {
+        int levels[32];
+        const int nlev = hierarchical_levels(L, levels);
+        // a node's first encounter is in the lowest element holding it, at the coarsest level whose
+        // grid holds it; the element loop below runs levels, then elements, then z, y, x
+        std::vector<int32_t> first((size_t)nraw, INT32_MAX);
+        for (ptrdiff_t e = ne - 1; e >= 0; e--)
+            for (int v = 0; v < nxe; v++) first[pre[(size_t)v * ne + e]] = (int32_t)e;
+        map.assign((size_t)nraw, -1);
+        int64_t next = 0;
+        for (int cv = 0; cv < 8; cv++)
+            for (ptrdiff_t e = 0; e < ne; e++) {
+                const int32_t id = me[cv * ne + e];
+                map[id] = id;
+                next = std::max<int64_t>(next, id);
+            }
+        next++;
+        for (int k = 1; k < nlev; k++) {
+            const int l = levels[k], s = L / l;
+            std::vector<int> loc;  // local ids of the level-k nodes (not on a coarser grid), (z, y, x) order
+            for (int z = 0; z <= l; z++)
+                for (int y = 0; y <= l; y++)
+                    for (int x = 0; x <= l; x++)
+                        if (hierarchical_level_index(L, levels, nlev, x * s, y * s, z * s) == k)
+                            loc.push_back(lidx(L, x * s, y * s, z * s));
+            for (ptrdiff_t e = 0; e < ne; e++)
+                for (const int v : loc) {
+                    const int32_t id = pre[(size_t)v * ne + e];
+                    if (first[id] == (int32_t)e && map[id] < 0) map[id] = (int32_t)next++;
+                }
+        }
+        ntotal = next;
     }
+    if (!elements) return (ptrdiff_t)ntotal;
 #pragma omp parallel for schedule(static)
-    for (ptrdiff_t e = 0; e < ne; e++)
-        for (int v = 0; v < nxe; v++) rank[pre[(size_t)v * ne + e]] = (int8_t)vrank[v];
-    std::vector<int64_t> start(maxrank + 2, 0);
-    for (int64_t i = 0; i < ntotal; i++) start[rank[i] + 1]++;
-    for (int r = 0; r <= maxrank; r++) start[r + 1] += start[r];
-    std::vector<int32_t> perm((size_t)ntotal);
-    for (int64_t i = 0; i < ntotal; i++) perm[i] = (int32_t)(start[rank[i]]++);
-#pragma omp parallel for schedule(static)
-    for (ptrdiff_t i = 0; i < (ptrdiff_t)nxe * ne; i++) elements[i] = perm[pre[i]];
+    for (ptrdiff_t i = 0; i < (ptrdiff_t)nxe * ne; i++) elements[i] = hierarchical ? map[pre[i]] : pre[i];
 
     if (points) {
 #pragma omp parallel for schedule(static)
@@ -312,8 +326,12 @@ ptrdiff_t sfem_amd_sshex8_refine(int L, ptrdiff_t ne, ptrdiff_t nnodes, const in
             }
         }
     }
-    (void)N1;
     return (ptrdiff_t)ntotal;
+}
+
+ptrdiff_t sfem_amd_sshex8_refine(int L, ptrdiff_t ne, ptrdiff_t nnodes, const int32_t *me, const float *mp,
+                                 int32_t *elements, float *points) {
+    return sfem_amd_sshex8_refine_ex(L, 1, ne, nnodes, me, mp, elements, points);
 }
 
 int sfem_amd_fff(int type, int level, ptrdiff_t ne, ptrdiff_t stride, const int32_t *el, ptrdiff_t nn,

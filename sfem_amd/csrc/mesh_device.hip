@@ -1,8 +1,9 @@
 // Mesh and geometry setup on the device: hex8 -> sshex8 refinement with the hierarchical
 // numbering of the host path (mesh.cpp, identical ids), the refined points, and the FFF /
-// adjugate caches of a device mesh.  Integer work: radix sorts (hipCUB) rank the unique edges and
-// faces by first occurrence, one thread per (element, local node) assigns the ids, prefix sums
-// renumber them level by level.  Setup only, deterministic.
+// adjugate caches of a device mesh.  Integer work: radix sorts (hipCUB) rank the unique edges (by
+// their corner ids) and sides (by first occurrence), one thread per (element, local node) assigns
+// the raw ids, and the hierarchical renumbering sorts the nodes by their first encounter
+// (level, element, local node), found with 64-bit atomic minima.  Setup only, deterministic.
 #include <hipcub/hipcub.hpp>
 
 #include <climits>
@@ -15,29 +16,10 @@
 namespace sfem_amd {
 namespace {
 
-// hex8 edges and faces in corner numbering (as mesh.cpp)
+// the 12 hex8 edges as corner pairs (as mesh.cpp)
 __device__ __forceinline__ int edge_corner(int k, int end) {
     constexpr int t[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {4, 5}, {5, 6}, {7, 6}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
     return t[k][end];
-}
-__device__ __forceinline__ int face_corner(int f, int i) {
-    constexpr int t[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4}, {3, 2, 6, 7}, {0, 3, 7, 4}, {1, 2, 6, 5}};
-    return t[f][i];
-}
-
-// rank of the coarsest level holding local node (x, y, z): 0 for corners, then levels m, 2m, ... L
-// (m the odd part of L); the same sequence as level_rank in mesh.cpp
-__host__ __device__ int node_level_rank(int L, int x, int y, int z) {
-    if (x % L == 0 && y % L == 0 && z % L == 0) return 0;
-    int m = L;
-    while (m % 2 == 0) m /= 2;
-    int rank = 1;
-    for (int l = m; l <= L; l *= 2, rank++) {
-        if (l == 1) continue;
-        const int s = L / l;
-        if (x % s == 0 && y % s == 0 && z % s == 0) return rank;
-    }
-    return rank;
 }
 
 __global__ void edge_keys_kernel(const ptrdiff_t ne, const int32_t *__restrict__ me, uint64_t *__restrict__ key,
@@ -60,7 +42,7 @@ __global__ void face_keys_kernel(const ptrdiff_t ne, const int32_t *__restrict__
         const ptrdiff_t e = t / 6;
         const int f = (int)(t - e * 6);
         uint32_t k[4];
-        for (int i = 0; i < 4; i++) k[i] = (uint32_t)me[face_corner(f, i) * ne + e];
+        for (int i = 0; i < 4; i++) k[i] = (uint32_t)me[hex8_side(f, i) * ne + e];
         for (int i = 1; i < 4; i++)
             for (int j = i; j > 0 && k[j - 1] > k[j]; j--) {
                 const uint32_t s = k[j];
@@ -91,34 +73,35 @@ __global__ void gather_u64_kernel(const ptrdiff_t n, const uint64_t *__restrict_
         dst[t] = src[at[t]];
 }
 
-// sorted position i starts a run of equal keys: head[i] = i (else 0), first[pos[i]] = 1 at run starts
+// sorted position i starts a run of equal keys: head[i] = i (else 0); first = 1 at run starts, stored
+// at the occurrence (pos[i]) or, by_key, at the sorted position i
 __global__ void run_heads_kernel(const ptrdiff_t n, const uint64_t *__restrict__ k1, const uint64_t *__restrict__ k2,
-                                 const int32_t *__restrict__ pos, int32_t *__restrict__ head,
+                                 const int32_t *__restrict__ pos, const int by_key, int32_t *__restrict__ head,
                                  int32_t *__restrict__ first) {
     for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
         const bool h = i == 0 || k1[i] != k1[i - 1] || (k2 && k2[i] != k2[i - 1]);
         head[i] = h ? (int32_t)i : 0;
-        first[pos[i]] = h ? 1 : 0;
+        first[by_key ? i : pos[i]] = h ? 1 : 0;
     }
 }
 
-// occurrence pos[i] -> rank (by first occurrence) of its key: the rank of its run's first occurrence
+// occurrence pos[i] -> rank of its key: the rank at its run's first entry (by occurrence of that
+// entry, or by_key by its sorted position)
 __global__ void run_rank_kernel(const ptrdiff_t n, const int32_t *__restrict__ pos, const int32_t *__restrict__ head,
-                                const int32_t *__restrict__ rank_of_first, int32_t *__restrict__ out) {
+                                const int by_key, const int32_t *__restrict__ rank, int32_t *__restrict__ out) {
     for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
-        out[pos[i]] = rank_of_first[pos[head[i]]];
+        out[pos[i]] = rank[by_key ? head[i] : pos[head[i]]];
 }
 
 struct MaxOp {
     __host__ __device__ int32_t operator()(int32_t a, int32_t b) const { return a > b ? a : b; }
 };
 
-// per (element, local node): the pre-renumbering id (host path: the OpenMP loop of
-// sfem_amd_sshex8_refine) and its level rank
+// per (element, local node): the raw id (host path: the OpenMP loop of sfem_amd_sshex8_refine_ex)
 __global__ void refine_ids_kernel(const ptrdiff_t ne, const int L, const int32_t *__restrict__ me,
                                   const int32_t *__restrict__ edge_of, const int32_t *__restrict__ face_of,
                                   const int64_t base_edge, const int64_t base_face, const int64_t base_vol,
-                                  int32_t *__restrict__ pre, int8_t *__restrict__ rank) {
+                                  int32_t *__restrict__ pre) {
     const int Lm1 = L - 1, N1 = L + 1, nxe = N1 * N1 * N1;
     const ptrdiff_t n = ne * nxe;
     for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
@@ -152,12 +135,12 @@ __global__ void refine_ids_kernel(const ptrdiff_t ne, const int L, const int32_t
                 break;
             }
             case 1: {
-                const int f = z == 0 ? 0 : z == L ? 1 : y == 0 ? 2 : y == L ? 3 : x == 0 ? 4 : 5;
+                const int f = hex8_side_of(L, x, y, z);
                 int am = 0;
                 for (int i = 1; i < 4; i++)
-                    if (c[face_corner(f, i)] < c[face_corner(f, am)]) am = i;
-                const int o = face_corner(f, am);
-                int nu = face_corner(f, (am + 1) % 4), nv = face_corner(f, (am + 3) % 4);
+                    if (c[hex8_side(f, i)] < c[hex8_side(f, am)]) am = i;
+                const int o = hex8_side(f, am);
+                int nu = hex8_side(f, (am + 1) % 4), nv = hex8_side(f, (am + 3) % 4);
                 if (c[nu] > c[nv]) {
                     const int s = nu;
                     nu = nv;
@@ -179,20 +162,55 @@ __global__ void refine_ids_kernel(const ptrdiff_t ne, const int L, const int32_t
                 id = base_vol + (int64_t)e * Lm1 * Lm1 * Lm1 + ((int64_t)(z - 1) * Lm1 + (y - 1)) * Lm1 + (x - 1);
         }
         pre[t] = (int32_t)id;
-        rank[id] = (int8_t)node_level_rank(L, x, y, z);  // every writer of a node writes the same rank
     }
 }
 
-__global__ void rank_flags_kernel(const ptrdiff_t n, const int8_t *__restrict__ rank, const int r,
-                                  int32_t *__restrict__ flag) {
-    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
-        flag[i] = rank[i] == r ? 1 : 0;
+// First encounter of each non-corner node in the hierarchical renumbering's traversal (levels
+// coarse to fine, then elements, then the level's local nodes in (z, y, x) order): the minimum of
+// key = (level index, element, local id on the level's grid) over the node's occurrences.
+__global__ void first_encounter_kernel(const ptrdiff_t ne, const int L, const int32_t *__restrict__ pre,
+                                       unsigned long long *__restrict__ key) {
+    int levels[32];
+    const int nlev = hierarchical_levels(L, levels);
+    const int N1 = L + 1, nxe = N1 * N1 * N1;
+    const ptrdiff_t n = ne * nxe;
+    for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
+        const ptrdiff_t e = t / nxe;
+        const int v = (int)(t - e * nxe);
+        const int x = v % N1, y = (v / N1) % N1, z = v / (N1 * N1);
+        const int k = hierarchical_level_index(L, levels, nlev, x, y, z);
+        if (k == 0) continue;  // macro corner: keeps its id
+        const int l = levels[k], st = L / l;
+        const unsigned long long kk = ((unsigned long long)k << 58) | ((unsigned long long)e << 24) |
+                                      (unsigned long long)lidx(l, x / st, y / st, z / st);
+        atomicMin(&key[pre[t]], kk);
+    }
 }
 
-__global__ void rank_place_kernel(const ptrdiff_t n, const int8_t *__restrict__ rank, const int r, const int32_t start,
-                                  const int32_t *__restrict__ scan, int32_t *__restrict__ perm) {
+// raw ids [base, n) with their keys, for the sort; corners [0, base) map to themselves
+__global__ void renumber_pairs_kernel(const ptrdiff_t n, const ptrdiff_t base, const unsigned long long *__restrict__ key,
+                                      uint64_t *__restrict__ k_out, int32_t *__restrict__ id_out,
+                                      int32_t *__restrict__ perm) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
+        if (i < base) {
+            perm[i] = (int32_t)i;
+        } else {
+            k_out[i - base] = (uint64_t)key[i];
+            id_out[i - base] = (int32_t)i;
+        }
+    }
+}
+
+// sorted position i -> new id next + i
+__global__ void renumber_place_kernel(const ptrdiff_t n, const int32_t next, const int32_t *__restrict__ sorted_id,
+                                      int32_t *__restrict__ perm) {
     for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
-        if (rank[i] == r) perm[i] = start + scan[i];
+        perm[sorted_id[i]] = next + (int32_t)i;
+}
+
+__global__ void fill_u64_kernel(const ptrdiff_t n, const unsigned long long v, unsigned long long *__restrict__ a) {
+    for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
+        a[i] = v;
 }
 
 // element-major connectivity of the refined mesh, and the lowest element holding each node
@@ -305,12 +323,12 @@ __global__ void geometry_kernel(const ptrdiff_t E, const int type, const int L, 
     }
 }
 
-// ranks of unique keys by first occurrence: occurrence t (= element * slots + slot) -> rank of its
-// key among the unique keys ordered by first occurrence (mesh.cpp: rank_by_first_occurrence).
-// Keys are (k1, k2) (k2 = nullptr: k1 alone); radix sorts are stable, so after sorting equal keys
-// keep occurrence order and a run's first entry is the key's first occurrence.
-int rank_first_occurrence(ptrdiff_t n, const uint64_t *k1, const uint64_t *k2, int32_t *out, int64_t &n_unique,
-                          hipStream_t s) {
+// ranks of unique keys: occurrence t (= element * slots + slot) -> rank of its key among the unique
+// keys ordered by first occurrence, or (by_key) by key (mesh.cpp: rank_keys).  Keys are (k1, k2)
+// (k2 = nullptr: k1 alone); radix sorts are stable, so after sorting equal keys keep occurrence
+// order and a run's first entry is the key's first occurrence.
+int rank_keys(ptrdiff_t n, const uint64_t *k1, const uint64_t *k2, bool by_key, int32_t *out, int64_t &n_unique,
+              hipStream_t s) {
     n_unique = 0;
     if (n == 0) return SFEM_SUCCESS;
     if (n >= INT_MAX) {
@@ -350,7 +368,8 @@ int rank_first_occurrence(ptrdiff_t n, const uint64_t *k1, const uint64_t *k2, i
         SFEM_AMD_HIP_CHECK(hipMemcpyAsync(pos.p, spos.p, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
     }
     // pos: occurrence of sorted entry i
-    run_heads_kernel<<<grid_for(n), kBlock, 0, s>>>(n, sk1.p, k2 ? sk2.p : nullptr, pos.p, head.p, first.p);
+    run_heads_kernel<<<grid_for(n), kBlock, 0, s>>>(n, sk1.p, k2 ? sk2.p : nullptr, pos.p, by_key ? 1 : 0, head.p,
+                                                    first.p);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     size_t need = 0;
     SFEM_AMD_HIP_CHECK(hipcub::DeviceScan::InclusiveScan(nullptr, need, head.p, headmax.p, MaxOp(), ni, s));
@@ -366,7 +385,7 @@ int rank_first_occurrence(ptrdiff_t n, const uint64_t *k1, const uint64_t *k2, i
         tb = need;
     }
     SFEM_AMD_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, first.p, rank.p, ni, s));
-    run_rank_kernel<<<grid_for(n), kBlock, 0, s>>>(n, pos.p, headmax.p, rank.p, out);
+    run_rank_kernel<<<grid_for(n), kBlock, 0, s>>>(n, pos.p, headmax.p, by_key ? 1 : 0, rank.p, out);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     int32_t last[2];
     SFEM_AMD_HIP_CHECK(hipMemcpyAsync(&last[0], rank.p + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -394,63 +413,73 @@ int sshex8_refine_device(int L, ptrdiff_t ne, ptrdiff_t nnodes, const int32_t *m
             return SFEM_FAILURE;
         edge_keys_kernel<<<grid_for(12 * ne), kBlock, 0, s>>>(ne, me, ek.p, dummy.p);
         SFEM_AMD_HIP_CHECK(hipGetLastError());
-        if (rank_first_occurrence(12 * ne, ek.p, nullptr, edge_of.p, nedges, s)) return SFEM_FAILURE;
+        if (rank_keys(12 * ne, ek.p, nullptr, /*by_key=*/true, edge_of.p, nedges, s)) return SFEM_FAILURE;
         ek.reset();
         if (fk1.alloc((size_t)6 * ne) || fk2.alloc((size_t)6 * ne) || face_of.alloc((size_t)6 * ne))
             return SFEM_FAILURE;
         face_keys_kernel<<<grid_for(6 * ne), kBlock, 0, s>>>(ne, me, fk1.p, fk2.p, dummy.p);
         SFEM_AMD_HIP_CHECK(hipGetLastError());
-        if (rank_first_occurrence(6 * ne, fk1.p, fk2.p, face_of.p, nfaces, s)) return SFEM_FAILURE;
+        if (rank_keys(6 * ne, fk1.p, fk2.p, /*by_key=*/false, face_of.p, nfaces, s)) return SFEM_FAILURE;
     }
     const int64_t base_edge = nnodes;
     const int64_t base_face = base_edge + nedges * Lm1;
     const int64_t base_vol = base_face + nfaces * (int64_t)Lm1 * Lm1;
-    const int64_t ntotal = base_vol + (int64_t)ne * Lm1 * Lm1 * Lm1;
-    if (ntotal >= INT32_MAX || (int64_t)ne * nxe >= INT32_MAX) {
+    const int64_t nraw = base_vol + (int64_t)ne * Lm1 * Lm1 * Lm1;
+    if (nraw >= INT32_MAX || (int64_t)ne * nxe >= INT32_MAX || ne >= ((ptrdiff_t)1 << 34)) {
         set_last_error("sshex8 refine: node count exceeds int32");
         return SFEM_FAILURE;
     }
-    ntotal_out = (ptrdiff_t)ntotal;
-    DevArray<int32_t> pre, perm, flag, scan;
-    DevArray<int8_t> rank;
-    if (pre.alloc((size_t)nxe * ne) || rank.alloc((size_t)ntotal) || perm.alloc((size_t)ntotal) ||
-        flag.alloc((size_t)ntotal) || scan.alloc((size_t)ntotal) || idx.alloc((size_t)nxe * ne))
-        return SFEM_FAILURE;
+    DevArray<int32_t> pre, perm;
+    if (pre.alloc((size_t)nxe * ne) || perm.alloc((size_t)nraw) || idx.alloc((size_t)nxe * ne)) return SFEM_FAILURE;
     if (ne > 0) {
         refine_ids_kernel<<<grid_for((ptrdiff_t)nxe * ne), kBlock, 0, s>>>(ne, L, me, edge_of.p, face_of.p, base_edge,
-                                                                           base_face, base_vol, pre.p, rank.p);
+                                                                           base_face, base_vol, pre.p);
         SFEM_AMD_HIP_CHECK(hipGetLastError());
     }
-    // hierarchical renumbering: a stable counting sort of the ids by level rank
-    int maxrank = 0;
-    for (int z = 0; z <= L; z++)
-        for (int y = 0; y <= L; y++)
-            for (int x = 0; x <= L; x++) maxrank = std::max(maxrank, node_level_rank(L, x, y, z));
-    DevArray<unsigned char> tmp;
-    size_t tb = 0;
-    int32_t start = 0;
-    for (int r = 0; r <= maxrank && ntotal > 0; r++) {
-        rank_flags_kernel<<<grid_for(ntotal), kBlock, 0, s>>>(ntotal, rank.p, r, flag.p);
+    // hierarchical renumbering: corners (raw ids below nnodes, the macro ids) keep their ids; the other
+    // nodes take next, next+1, ... in the order of their first-encounter keys (next = largest corner
+    // id + 1, as sshex8_hierarchical_renumbering, mesh/sfem_hex8_mesh_graph.cpp:1006-1025)
+    int32_t next = 0;
+    if (ne > 0) {
+        DevArray<int32_t> mx;
+        DevArray<unsigned char> tmp;
+        if (mx.alloc(1)) return SFEM_FAILURE;
+        size_t need = 0;
+        SFEM_AMD_HIP_CHECK(hipcub::DeviceReduce::Max(nullptr, need, me, mx.p, (int)(8 * ne), s));
+        if (tmp.alloc(need)) return SFEM_FAILURE;
+        SFEM_AMD_HIP_CHECK(hipcub::DeviceReduce::Max(tmp.p, need, me, mx.p, (int)(8 * ne), s));
+        SFEM_AMD_HIP_CHECK(hipMemcpyAsync(&next, mx.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        SFEM_AMD_HIP_CHECK(hipStreamSynchronize(s));
+        next += 1;
+    }
+    const ptrdiff_t nrest = (ptrdiff_t)(nraw - nnodes);
+    if (nrest > 0) {
+        DevArray<unsigned long long> key;
+        DevArray<uint64_t> k_in, k_out;
+        DevArray<int32_t> id_in, id_out;
+        DevArray<unsigned char> tmp;
+        if (key.alloc((size_t)nraw) || k_in.alloc((size_t)nrest) || k_out.alloc((size_t)nrest) ||
+            id_in.alloc((size_t)nrest) || id_out.alloc((size_t)nrest))
+            return SFEM_FAILURE;
+        fill_u64_kernel<<<grid_for(nraw), kBlock, 0, s>>>(nraw, ~0ull, key.p);
+        SFEM_AMD_HIP_CHECK(hipGetLastError());
+        first_encounter_kernel<<<grid_for((ptrdiff_t)nxe * ne), kBlock, 0, s>>>(ne, L, pre.p, key.p);
+        SFEM_AMD_HIP_CHECK(hipGetLastError());
+        renumber_pairs_kernel<<<grid_for(nraw), kBlock, 0, s>>>(nraw, nnodes, key.p, k_in.p, id_in.p, perm.p);
         SFEM_AMD_HIP_CHECK(hipGetLastError());
         size_t need = 0;
-        SFEM_AMD_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, flag.p, scan.p, (int)ntotal, s));
-        if (need > tb) {
-            if (tmp.alloc(need)) return SFEM_FAILURE;
-            tb = need;
-        }
-        SFEM_AMD_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.p, tb, flag.p, scan.p, (int)ntotal, s));
-        rank_place_kernel<<<grid_for(ntotal), kBlock, 0, s>>>(ntotal, rank.p, r, start, scan.p, perm.p);
+        SFEM_AMD_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, k_in.p, k_out.p, id_in.p, id_out.p,
+                                                              (int)nrest, 0, 64, s));
+        if (tmp.alloc(need)) return SFEM_FAILURE;
+        SFEM_AMD_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp.p, need, k_in.p, k_out.p, id_in.p, id_out.p,
+                                                              (int)nrest, 0, 64, s));
+        renumber_place_kernel<<<grid_for(nrest), kBlock, 0, s>>>(nrest, next, id_out.p, perm.p);
         SFEM_AMD_HIP_CHECK(hipGetLastError());
-        int32_t cnt[2];
-        SFEM_AMD_HIP_CHECK(hipMemcpyAsync(&cnt[0], scan.p + ntotal - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        SFEM_AMD_HIP_CHECK(hipMemcpyAsync(&cnt[1], flag.p + ntotal - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        SFEM_AMD_HIP_CHECK(hipStreamSynchronize(s));
-        start += cnt[0] + cnt[1];
+    } else if (nnodes > 0) {
+        renumber_pairs_kernel<<<grid_for(nraw), kBlock, 0, s>>>(nraw, nnodes, nullptr, nullptr, nullptr, perm.p);
+        SFEM_AMD_HIP_CHECK(hipGetLastError());
     }
-    if (start != ntotal) {
-        set_last_error("sshex8 refine on device: renumbering does not cover every node");
-        return SFEM_FAILURE;
-    }
+    ntotal_out = (ptrdiff_t)next + nrest;
     if (ne > 0) {
         refine_connect_kernel<<<grid_for((ptrdiff_t)nxe * ne), kBlock, 0, s>>>(ne, nxe, pre.p, perm.p, idx.p, nullptr);
         SFEM_AMD_HIP_CHECK(hipGetLastError());

@@ -2,8 +2,9 @@
 
 Mirrors create_ssgmg / create_gmg_data (ssmg/sfem_ssgmg.cpp:9-42,
 algebra/mg/sfem_GeometricMultigrid.cpp:12-168) with the smoothing of drivers/mgsolve.cpp:
-  * levels = derefinement levels of L (L, L/2, ... down to the odd part, then 1; 8 -> 8,4,2,1);
-    level 1 is the HEX8 macro mesh;
+  * levels = derefinement levels of L, the levels of the hierarchical numbering finest first
+    (sshex8_hierarchical_mesh_levels, mesh/sfem_hex8_mesh_graph.cpp:964-992: 8 -> 8,4,2,1;
+    12 -> 12,6,1); level 1 is the HEX8 macro mesh;
   * every level rediscretises the operator with the macro geometry (dirichlet_as_zero on
     coarse levels: the coarse constrained nodes are the fine ones that exist there);
   * prolongation and restriction zero the constrained dofs of their output level;
@@ -32,9 +33,12 @@ from . import (DeviceArray, Mesh, Multigrid, adjugate_det, block_jacobi, box_mes
 
 
 def derefinement_levels(L):
-    """[L, L/2, ..., m, 1] with m the odd part of L (frontend/tests/sfem_SemiStructuredMeshTest.cpp:18-22)."""
+    """Levels of the hierarchical numbering, finest first: L, L/2, ... while the quotient stays even,
+    then 1 (8 -> [8, 4, 2, 1], frontend/tests/sfem_SemiStructuredMeshTest.cpp:18-22; 12 -> [12, 6, 1],
+    sshex8_hierarchical_mesh_levels, mesh/sfem_hex8_mesh_graph.cpp:964-992).  Only these levels are
+    prefixes of the fine numbering."""
     out = [L]
-    while out[-1] % 2 == 0 and out[-1] > 1:
+    while out[-1] % 2 == 0 and (out[-1] // 2) % 2 == 0:
         out.append(out[-1] // 2)
     if out[-1] != 1:
         out.append(1)

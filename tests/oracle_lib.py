@@ -49,6 +49,8 @@ def _load():
         "oracle_box_mesh": (None, [C.c_int] * 3 + [C.c_double] * 3 + [C.c_int, _i32p, _f32p]),
         "oracle_sshex8_nxe": (C.c_int, [C.c_int]),
         "oracle_sshex8_refine": (_pd, [C.c_int, _pd, _pd, _i32p, _f32p, C.c_void_p, C.c_void_p]),
+        "oracle_sshex8_refine_ex": (_pd, [C.c_int, C.c_int, _pd, _pd, _i32p, _f32p, C.c_void_p, C.c_void_p]),
+        "oracle_sshex8_hierarchical_levels": (C.c_int, [C.c_int, C.c_void_p]),
         "oracle_sshex8_derefine": (None, [C.c_int, C.c_int, _pd, _i32p, _i32p]),
         "oracle_sshex8_to_hex8": (None, [C.c_int, _pd, _i32p, _i32p]),
         "oracle_node_incidence": (None, [C.c_int, _pd, _pd, _i32p, _u16p]),
@@ -116,16 +118,24 @@ def box_mesh(nx, ny, nz, w=1.0, h=1.0, t=1.0, tet4=False):
     return el, pts
 
 
-def sshex8_refine(L, macro_el, macro_pts):
+def sshex8_refine(L, macro_el, macro_pts, hierarchical=True):
     ne = macro_el.shape[1]
     nn = macro_pts.shape[1]
     macro_el = np.ascontiguousarray(macro_el, np.int32)
     macro_pts = np.ascontiguousarray(macro_pts, np.float32)
-    ntot = lib.oracle_sshex8_refine(L, ne, nn, macro_el, macro_pts, None, None)
+    h = int(bool(hierarchical))
+    ntot = lib.oracle_sshex8_refine_ex(L, h, ne, nn, macro_el, macro_pts, None, None)
     el = np.zeros(((L + 1) ** 3, ne), np.int32)
     pts = np.zeros((3, ntot), np.float32)
-    lib.oracle_sshex8_refine(L, ne, nn, macro_el, macro_pts, el.ctypes.data, pts.ctypes.data)
+    lib.oracle_sshex8_refine_ex(L, h, ne, nn, macro_el, macro_pts, el.ctypes.data, pts.ctypes.data)
     return el, pts
+
+
+def hierarchical_levels(L):
+    """sshex8_hierarchical_mesh_levels, coarsest first ([1, 2, 4, 8] for 8)."""
+    buf = (C.c_int * 32)()
+    n = lib.oracle_sshex8_hierarchical_levels(L, buf)
+    return [buf[i] for i in range(n)]
 
 
 def derefine(Lf, Lc, el):

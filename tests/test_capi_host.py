@@ -39,24 +39,30 @@ def test_box_mesh_matches_oracle(tet4):
         np.testing.assert_array_equal(a[1], b[1])
 
 
-@pytest.mark.parametrize("n,L", [(2, 2), (3, 4), (2, 8), (2, 6), (1, 16)])
-def test_sshex8_refine_matches_oracle_bitexact(n, L):
+@pytest.mark.parametrize("hierarchical", [True, False])
+@pytest.mark.parametrize("n,L", [(2, 2), (3, 4), (2, 8), (2, 6), (1, 16), (2, 12), (2, 3)])
+def test_sshex8_refine_matches_oracle_bitexact(n, L, hierarchical):
+    """Product host refinement (sort-based edge/side ranks, first-element table) against the oracle's
+    line-by-line restatement of sshex8_generate_elements + sshex8_hierarchical_renumbering, which the
+    hand-derived tables of test_oracle_mesh.py pin."""
     mel, mpts = O.box_mesh(n + 1, n + 1, n + 1)
-    e1, p1 = S.sshex8_refine(L, mel, mpts)
-    e2, p2 = O.sshex8_refine(L, mel, mpts)
+    e1, p1 = S.sshex8_refine(L, mel, mpts, hierarchical=hierarchical)
+    e2, p2 = O.sshex8_refine(L, mel, mpts, hierarchical=hierarchical)
     np.testing.assert_array_equal(e1, e2)
     np.testing.assert_allclose(p1, p2, rtol=0, atol=1e-6)
 
 
-def test_sshex8_refine_permuted_elements_matches_oracle():
+@pytest.mark.parametrize("hierarchical", [True, False])
+def test_sshex8_refine_permuted_elements_matches_oracle(hierarchical):
     mel, mpts = O.box_mesh(3, 4, 3)
     rng = np.random.default_rng(5)
     mel = mel[:, rng.permutation(mel.shape[1])]
     perm = [1, 2, 3, 0, 5, 6, 7, 4]
     mel[:, ::3] = mel[perm][:, ::3]
-    e1, _ = S.sshex8_refine(4, mel, mpts)
-    e2, _ = O.sshex8_refine(4, mel, mpts)
-    np.testing.assert_array_equal(e1, e2)
+    for L in (3, 4):
+        e1, _ = S.sshex8_refine(L, mel, mpts, hierarchical=hierarchical)
+        e2, _ = O.sshex8_refine(L, mel, mpts, hierarchical=hierarchical)
+        np.testing.assert_array_equal(e1, e2)
 
 
 def _ulp_close(a, b, ulps=4):
