@@ -578,17 +578,51 @@ static void fff_from_jacobian(const double *J, double scale, float *fff) {
     for (int i = 0; i < 6; i++) fff[i] = (float)g[i];
 }
 
-/* operators/tet4/tet4_fff.cpp:5-28 + tet4_inline_cpu.hpp:224-262.  The reference
- * evaluates in float32; here float64 then one rounding (differs by a few ulp). */
+/* operators/tet4/tet4_fff.cpp:5-28 + tet4_fff (operators/tet4/tet4_inline_cpu.hpp:224-278).
+ * The reference keeps every temporary in float32 (geom_t) with this evaluation order; two steps
+ * are evaluated in double and rounded to float: the volume det/6 (its 1/6 constants are double)
+ * and 1/det^2 (the numerator 1. is double).  Reproducing the reference's float32 words needs the
+ * same order and no FMA contraction (the oracle is built with -ffp-contract=off).
+ * e1, e2, e3: the tet's edge vectors from vertex 0 (the columns of J). */
+static void tet4_fff_f32(const float *px, const float *py, const float *pz, float *fff) {
+    const float e1x = px[1] - px[0], e2x = px[2] - px[0], e3x = px[3] - px[0];
+    const float e1y = py[1] - py[0], e2y = py[2] - py[0], e3y = py[3] - py[0];
+    const float e1z = pz[1] - pz[0], e2z = pz[2] - pz[0], e3z = pz[3] - pz[0];
+    /* the six products of det(J) */
+    const float t_yz = e2y * e3z, p0 = e1x * t_yz;
+    const float t_zy = e3y * e2z, p1 = e1x * t_zy;
+    const float t_xz = e2x * e3z, p2 = t_xz * e1y;
+    const float t_xy = e2x * e3y, p3 = e1z * t_xy;
+    const float p4 = e3x * e2z * e1y;
+    const float t_yx = e2y * e3x, p5 = e1z * t_yx;
+    const double sixth = 1.0 / 6.0;
+    const float vol = (float)(-sixth * p2 + sixth * p3 + sixth * p4 - sixth * p5 + sixth * p0 - sixth * p1);
+    const float det = -p2 + p3 + p4 - p5 + p0 - p1;
+    const float inv_det2 = (float)(1. / (det * det));
+    /* cofactors (adjugate entries) of J, row by row */
+    const float a00 = t_xy - t_yx, a01 = e3x * e2z - t_xz, a02 = t_yz - t_zy;
+    const float a10 = e3x * e1y - e1x * e3y, a11 = e1x * e3z - e1z * e3x, a12 = e1z * e3y - e3z * e1y;
+    const float a20 = e1x * e2y - e2x * e1y, a21 = e2x * e1z - e1x * e2z, a22 = e2z * e1y - e2y * e1z;
+    const float a00s = a00 * inv_det2, a01s = inv_det2 * a01, a02s = inv_det2 * a02;
+    fff[0] = vol * (a00 * a00 * inv_det2 + inv_det2 * (a01 * a01) + inv_det2 * (a02 * a02));
+    fff[1] = vol * (a10 * a00s + a11 * a01s + a12 * a02s);
+    fff[2] = vol * (a00s * a20 + a01s * a21 + a02s * a22);
+    fff[3] = vol * (inv_det2 * (a10 * a10) + inv_det2 * (a11 * a11) + inv_det2 * (a12 * a12));
+    fff[4] = vol * (inv_det2 * a10 * a20 + inv_det2 * a11 * a21 + inv_det2 * a12 * a22);
+    fff[5] = vol * (inv_det2 * (a20 * a20) + inv_det2 * (a21 * a21) + inv_det2 * (a22 * a22));
+}
+
 void oracle_tet4_fff(ptrdiff_t ne, const int32_t *el, ptrdiff_t nn, const float *p, float *fff) {
 #pragma omp parallel for
     for (ptrdiff_t e = 0; e < ne; e++) {
-        int32_t v[4];
-        for (int i = 0; i < 4; i++) v[i] = el[(size_t)i * ne + e];
-        double J[9];
-        for (int d = 0; d < 3; d++)
-            for (int j = 0; j < 3; j++) J[d * 3 + j] = (double)p[d * nn + v[j + 1]] - (double)p[d * nn + v[0]];
-        fff_from_jacobian(J, 1.0 / 6.0, &fff[e * 6]);
+        float px[4], py[4], pz[4];
+        for (int i = 0; i < 4; i++) {
+            const int32_t v = el[(size_t)i * ne + e];
+            px[i] = p[0 * nn + v];
+            py[i] = p[1 * nn + v];
+            pz[i] = p[2 * nn + v];
+        }
+        tet4_fff_f32(px, py, pz, &fff[e * 6]);
     }
 }
 

@@ -83,6 +83,37 @@ __host__ __device__ constexpr inline int hex8_side_of(int L, int x, int y, int z
     return y == 0 ? 0 : x == L ? 1 : y == L ? 2 : x == 0 ? 3 : z == 0 ? 4 : 5;
 }
 
+// tet4 FFF as the reference evaluates it (tet4_fff, operators/tet4/tet4_inline_cpu.hpp:224-278):
+// every temporary is float32, except the volume det/6 (double constants, rounded once) and
+// 1/det^2 (a double quotient, rounded once); contraction into FMA is off so that the float32
+// words equal the reference's.  e1, e2, e3: edge vectors from vertex 0; aij: cofactors of J.
+__host__ __device__ inline void tet4_fff_f32(const float *px, const float *py, const float *pz, float *fff) {
+#pragma clang fp contract(off)
+    const float e1x = px[1] - px[0], e2x = px[2] - px[0], e3x = px[3] - px[0];
+    const float e1y = py[1] - py[0], e2y = py[2] - py[0], e3y = py[3] - py[0];
+    const float e1z = pz[1] - pz[0], e2z = pz[2] - pz[0], e3z = pz[3] - pz[0];
+    const float t_yz = e2y * e3z, p0 = e1x * t_yz;
+    const float t_zy = e3y * e2z, p1 = e1x * t_zy;
+    const float t_xz = e2x * e3z, p2 = t_xz * e1y;
+    const float t_xy = e2x * e3y, p3 = e1z * t_xy;
+    const float p4 = e3x * e2z * e1y;
+    const float t_yx = e2y * e3x, p5 = e1z * t_yx;
+    const double sixth = 1.0 / 6.0;
+    const float vol = (float)(-sixth * p2 + sixth * p3 + sixth * p4 - sixth * p5 + sixth * p0 - sixth * p1);
+    const float det = -p2 + p3 + p4 - p5 + p0 - p1;
+    const float inv_det2 = (float)(1. / (det * det));
+    const float a00 = t_xy - t_yx, a01 = e3x * e2z - t_xz, a02 = t_yz - t_zy;
+    const float a10 = e3x * e1y - e1x * e3y, a11 = e1x * e3z - e1z * e3x, a12 = e1z * e3y - e3z * e1y;
+    const float a20 = e1x * e2y - e2x * e1y, a21 = e2x * e1z - e1x * e2z, a22 = e2z * e1y - e2y * e1z;
+    const float a00s = a00 * inv_det2, a01s = inv_det2 * a01, a02s = inv_det2 * a02;
+    fff[0] = vol * (a00 * a00 * inv_det2 + inv_det2 * (a01 * a01) + inv_det2 * (a02 * a02));
+    fff[1] = vol * (a10 * a00s + a11 * a01s + a12 * a02s);
+    fff[2] = vol * (a00s * a20 + a01s * a21 + a02s * a22);
+    fff[3] = vol * (inv_det2 * (a10 * a10) + inv_det2 * (a11 * a11) + inv_det2 * (a12 * a12));
+    fff[4] = vol * (inv_det2 * a10 * a20 + inv_det2 * a11 * a21 + inv_det2 * a12 * a22);
+    fff[5] = vol * (inv_det2 * (a20 * a20) + inv_det2 * (a21 * a21) + inv_det2 * (a22 * a22));
+}
+
 // Levels of the hierarchical sshex8 numbering, coarsest first (sshex8_hierarchical_mesh_levels,
 // mesh/sfem_hex8_mesh_graph.cpp:964-992): 1, then the chain ..., L/4, L/2, L while halving stays
 // even (8 -> 1 2 4 8; 12 -> 1 6 12; 6 -> 1 6).  For odd L > 1 the reference lists [1] only and its

@@ -55,6 +55,7 @@ std::vector<int64_t> rank_keys(const std::vector<Key> &keys, bool by_key, int64_
 }
 
 void center_jacobian(const double *x, const double *y, const double *z, double *J) {
+#pragma clang fp contract(off)
     const double *c[3] = {x, y, z};
     for (int i = 0; i < 9; i++) J[i] = 0;
     for (int v = 0; v < 8; v++) {
@@ -65,6 +66,7 @@ void center_jacobian(const double *x, const double *y, const double *z, double *
 }
 
 void adjugate3(const double *J, double *A, double *det) {
+#pragma clang fp contract(off)
     A[0] = J[4] * J[8] - J[5] * J[7];
     A[1] = J[2] * J[7] - J[1] * J[8];
     A[2] = J[1] * J[5] - J[2] * J[4];
@@ -78,6 +80,7 @@ void adjugate3(const double *J, double *A, double *det) {
 }
 
 void fff_of(const double *J, double scale, float *out) {
+#pragma clang fp contract(off)
     double A[9], det;
     adjugate3(J, A, &det);
     int k = 0;
@@ -301,8 +304,13 @@ ptrdiff_t sfem_amd_sshex8_refine_ex(int L, int hierarchical, ptrdiff_t ne, ptrdi
     for (ptrdiff_t i = 0; i < (ptrdiff_t)nxe * ne; i++) elements[i] = hierarchical ? map[pre[i]] : pre[i];
 
     if (points) {
+        // each node written by the lowest element holding it (deterministic; as the device path)
+        std::vector<int32_t> owner((size_t)ntotal, INT32_MAX);
+        for (ptrdiff_t e = ne - 1; e >= 0; e--)
+            for (int v = 0; v < nxe; v++) owner[elements[(size_t)v * ne + e]] = (int32_t)e;
 #pragma omp parallel for schedule(static)
         for (ptrdiff_t e = 0; e < ne; e++) {
+#pragma clang fp contract(off)
             double cx[8], cy[8], cz[8];
             for (int v = 0; v < 8; v++) {
                 const int32_t n = me[v * ne + e];
@@ -320,6 +328,7 @@ ptrdiff_t sfem_amd_sshex8_refine_ex(int L, int hierarchical, ptrdiff_t ne, ptrdi
                     pz += f * cz[k];
                 }
                 const int32_t n = elements[(size_t)v * ne + e];
+                if (owner[n] != (int32_t)e) continue;
                 points[n] = (float)px;
                 points[ntotal + n] = (float)py;
                 points[2 * ntotal + n] = (float)pz;
@@ -343,10 +352,14 @@ int sfem_amd_fff(int type, int level, ptrdiff_t ne, ptrdiff_t stride, const int3
     for (ptrdiff_t e = 0; e < ne; e++) {
         double J[9];
         if (type == SFEM_AMD_TET4) {
-            const int32_t v0 = el[e];
-            for (int d = 0; d < 3; d++)
-                for (int j = 0; j < 3; j++) J[d * 3 + j] = (double)p[d * nn + el[(j + 1) * stride + e]] - (double)p[d * nn + v0];
-            fff_of(J, 1.0 / 6.0, &fff[e * 6]);
+            float px[4], py[4], pz[4];
+            for (int i = 0; i < 4; i++) {
+                const int32_t v = el[i * stride + e];
+                px[i] = p[v];
+                py[i] = p[nn + v];
+                pz[i] = p[2 * nn + v];
+            }
+            tet4_fff_f32(px, py, pz, &fff[e * 6]);
         } else {
             double x[8], y[8], z[8];
             for (int v = 0; v < 8; v++) {

@@ -231,6 +231,7 @@ __global__ void refine_points_kernel(const ptrdiff_t ne, const int L, const ptrd
                                      const int32_t *__restrict__ me, const float *__restrict__ mp,
                                      const int32_t *__restrict__ idx, const int32_t *__restrict__ owner,
                                      float *__restrict__ points) {
+#pragma clang fp contract(off)
     const int N1 = L + 1, nxe = N1 * N1 * N1;
     const ptrdiff_t n = ne * nxe;
     for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
@@ -260,6 +261,7 @@ __global__ void fill_i32_kernel(const ptrdiff_t n, const int32_t v, int32_t *__r
 
 // Jacobian at the element centre from its 8 corners (host: center_jacobian), FFF / adjugate
 __device__ void centre_jacobian(const double *x, const double *y, const double *z, double *J) {
+#pragma clang fp contract(off)
     const double *c[3] = {x, y, z};
     for (int i = 0; i < 9; i++) J[i] = 0;
     for (int v = 0; v < 8; v++) {
@@ -269,6 +271,7 @@ __device__ void centre_jacobian(const double *x, const double *y, const double *
     }
 }
 __device__ void adjugate_of(const double *J, double *A, double *det) {
+#pragma clang fp contract(off)
     A[0] = J[4] * J[8] - J[5] * J[7];
     A[1] = J[2] * J[7] - J[1] * J[8];
     A[2] = J[1] * J[5] - J[2] * J[4];
@@ -286,14 +289,22 @@ __device__ void adjugate_of(const double *J, double *A, double *det) {
 __global__ void geometry_kernel(const ptrdiff_t E, const int type, const int L, const int32_t *__restrict__ idx,
                                 const ptrdiff_t nn, const float *__restrict__ p, const int kind,
                                 float *__restrict__ out, float *__restrict__ det_out) {
+    // no contraction into FMA: the words equal the host path's (mesh.cpp)
+#pragma clang fp contract(off)
     for (ptrdiff_t e = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (ptrdiff_t)gridDim.x * blockDim.x) {
         double J[9];
-        double scale = 1.0;
+        const double scale = 1.0;
         if (type == SFEM_AMD_TET4) {
-            const int32_t v0 = idx[e];
-            for (int d = 0; d < 3; d++)
-                for (int j = 0; j < 3; j++) J[d * 3 + j] = (double)p[d * nn + idx[(j + 1) * E + e]] - (double)p[d * nn + v0];
-            scale = 1.0 / 6.0;
+            // the reference's float32 evaluation (as the host path)
+            float px[4], py[4], pz[4];
+            for (int i = 0; i < 4; i++) {
+                const int32_t v = idx[i * E + e];
+                px[i] = p[v];
+                py[i] = p[nn + v];
+                pz[i] = p[2 * nn + v];
+            }
+            if (kind == 0) tet4_fff_f32(px, py, pz, &out[e * 6]);
+            continue;
         } else {
             double x[8], y[8], z[8];
             for (int v = 0; v < 8; v++) {

@@ -1,7 +1,7 @@
 """Mesh and geometry setup on the device against the host path (which the oracle pins bit-exactly,
-tests/test_capi_host.py): refined connectivity and node count bit-exact, refined points and
-geometry caches to float32 rounding, operators built from device geometry equal to the host-built
-ones."""
+tests/test_capi_host.py): refined connectivity, node count, refined points and geometry caches
+bit-exact (neither side contracts into FMA; shared nodes are written by their lowest element), and
+operators built from device geometry equal to the host-built ones."""
 import time
 
 import numpy as np
@@ -34,8 +34,24 @@ def test_device_refine_matches_host(L, n, seed):
     mesh, pts_d = S.sshex8_refine_device(L, mel, mpts)
     assert mesh.nnodes == pts_h.shape[1]
     np.testing.assert_array_equal(mesh.download(), el_h)
-    pd = pts_d.numpy().reshape(3, -1)
-    assert np.max(np.abs(pd - pts_h)) <= 1e-6 * max(1.0, np.max(np.abs(pts_h)))
+    np.testing.assert_array_equal(pts_d.numpy().reshape(3, -1), pts_h)
+
+
+def test_device_tet4_hex8_geometry_match_host():
+    """tet4 FFF on the device follows the reference's float32 evaluation as the host path does."""
+    rng = np.random.default_rng(21)
+    for tet4 in (True, False):
+        el, pts = O.box_mesh(6, 5, 7, tet4=tet4)
+        pts = (pts + 0.03 * rng.uniform(-1, 1, pts.shape)).astype(np.float32)
+        typ = S.TET4 if tet4 else S.HEX8
+        mesh = S.Mesh(typ, 1, el, pts.shape[1])
+        fff_d = S.mesh_fff(mesh, S.DeviceArray.from_numpy(pts.ravel()))
+        np.testing.assert_array_equal(fff_d.numpy().reshape(-1, 6), S.fff(typ, 1, el, pts))
+        if not tet4:
+            adj_d, det_d = S.mesh_adjugate_det(mesh, S.DeviceArray.from_numpy(pts.ravel()))
+            adj_h, det_h = S.adjugate_det(typ, 1, el, pts)
+            np.testing.assert_array_equal(adj_d.numpy().reshape(-1, 9), adj_h)
+            np.testing.assert_array_equal(det_d.numpy(), det_h)
 
 
 @pytest.mark.parametrize("L", [2, 8])
@@ -46,11 +62,11 @@ def test_device_geometry_and_operators_match_host(L):
     N, E = pts_h.shape[1], mel.shape[1]
     fff_h = S.fff(S.SSHEX8, L, el_h, pts_h)
     fff_d = S.mesh_fff(mesh, pts_d)
-    np.testing.assert_allclose(fff_d.numpy().reshape(E, 6), fff_h, rtol=2e-6, atol=1e-7)
+    np.testing.assert_array_equal(fff_d.numpy().reshape(E, 6), fff_h)
     adj_h, det_h = S.adjugate_det(S.SSHEX8, L, el_h, pts_h)
     adj_d, det_d = S.mesh_adjugate_det(mesh, pts_d)
-    np.testing.assert_allclose(adj_d.numpy().reshape(E, 9), adj_h, rtol=2e-6, atol=1e-7)
-    np.testing.assert_allclose(det_d.numpy(), det_h, rtol=2e-6, atol=1e-9)
+    np.testing.assert_array_equal(adj_d.numpy().reshape(E, 9), adj_h)
+    np.testing.assert_array_equal(det_d.numpy(), det_h)
     # operators from device geometry vs the same words from the host
     x = S.DeviceArray.from_numpy(np.random.default_rng(L).uniform(-1, 1, N))
     lap_d = S.laplacian(mesh, fff_d)

@@ -99,6 +99,25 @@ def test_tet4_fff_general():
     np.testing.assert_allclose(O.tet4_fff(el, pts)[0], sym6(Ji @ Ji.T * np.linalg.det(Jf) / 6), rtol=2e-6)
 
 
+def test_tet4_fff_is_float32_evaluation():
+    """tet4_fff (operators/tet4/tet4_inline_cpu.hpp:224-278) keeps its temporaries in float32: the
+    words agree with J^-1 J^-T det/6 evaluated in float64 to float32 rounding error, and differ
+    from that value rounded once to float32 in some words (a float64 restatement would match it)."""
+    rng = np.random.default_rng(12)
+    el, pts = O.box_mesh(7, 7, 7, tet4=True)
+    pts = (pts + 0.04 * rng.uniform(-1, 1, pts.shape)).astype(np.float32)
+    got = O.tet4_fff(el, pts)
+    want = np.empty_like(got, dtype=np.float64)
+    for e in range(el.shape[1]):
+        v = pts[:, el[:, e]].astype(np.float64)
+        J = v[:, 1:] - v[:, :1]
+        Ji = np.linalg.inv(J)
+        want[e] = sym6(Ji @ Ji.T * np.linalg.det(J) / 6)
+    scale = np.abs(want).max(axis=1, keepdims=True)
+    assert np.max(np.abs(got - want) / scale) < 2e-5
+    assert np.count_nonzero(got != want.astype(np.float32)) > 0.05 * got.size
+
+
 def test_hex8_fff_box_and_parallelepiped():
     el = np.arange(8, dtype=np.int32).reshape(8, 1)
     a, b, c = 2.0, 0.5, 4.0
