@@ -230,6 +230,15 @@ int sfem_amd_blas_xypaz(int rt, ptrdiff_t n, const void *x, const void *y, doubl
                         void *stream);                                                   /* z = x*y + alpha*z */
 double sfem_amd_blas_dot(int rt, ptrdiff_t n, const void *x, const void *y, void *stream);
 double sfem_amd_blas_norm2(int rt, ptrdiff_t n, const void *x, void *stream);
+/* The dot of sfem_amd_blas_dot written to *result, a DEVICE double, without allocating: no
+ * hipMalloc, no pinned host memory, no host synchronisation (so it can run under HIP graph
+ * capture or in a loop that must not allocate).  No counterpart in the reference: it exists for
+ * callers that cannot allocate.  Same kernels, grid and summation order as sfem_amd_blas_dot, so
+ * the two results are the same word.  The partials live in a static per-GPU device array of
+ * 1024 doubles (8 KiB, held until the process ends); grids never exceed it, so n is limited only
+ * by n >= 0.  Every call on one GPU shares that array: calls that overlap on one GPU (concurrent
+ * streams or threads) race. */
+int sfem_amd_blas_dot_noalloc(int rt, ptrdiff_t n, const void *x, const void *y, double *result, void *stream);
 /* y = (real type rt_to) x */
 int sfem_amd_blas_cast(int rt_from, ptrdiff_t n, const void *x, int rt_to, void *y, void *stream);
 /* Fused CG step (algebra/sfem_cg.hpp:26-72): x += alpha p, r -= alpha Ap; returns (r,r)

@@ -72,6 +72,7 @@ SIGNATURES = {
     "sfem_amd_blas_xypaz": (_i, [_i, _pd, _vp, _vp, _d, _vp, _vp]),
     "sfem_amd_blas_dot": (_d, [_i, _pd, _vp, _vp, _vp]),
     "sfem_amd_blas_norm2": (_d, [_i, _pd, _vp, _vp]),
+    "sfem_amd_blas_dot_noalloc": (_i, [_i, _pd, _vp, _vp, _vp, _vp]),
     "sfem_amd_blas_cast": (_i, [_i, _pd, _vp, _i, _vp, _vp]),
     "sfem_amd_cg_update_x_r": (_d, [_i, _pd, _d, _vp, _vp, _vp, _vp, _i, _vp]),
     "sfem_amd_cg_update_x_r_dot": (_d, [_i, _pd, _d, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_double), _vp]),

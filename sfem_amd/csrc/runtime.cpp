@@ -1349,6 +1349,10 @@ double sfem_amd_blas_dot(int rt, ptrdiff_t n, const void *x, const void *y, void
     return err ? NAN : r;
 }
 double sfem_amd_blas_norm2(int rt, ptrdiff_t n, const void *x, void *s) { return std::sqrt(sfem_amd_blas_dot(rt, n, x, x, s)); }
+int sfem_amd_blas_dot_noalloc(int rt, ptrdiff_t n, const void *x, const void *y, double *result, void *s) {
+    DISPATCH_RT(rt, blas_dot_noalloc<double>(n, (const double *)x, (const double *)y, result, as_stream(s)),
+                blas_dot_noalloc<float>(n, (const float *)x, (const float *)y, result, as_stream(s)));
+}
 int sfem_amd_blas_cast(int rt_from, ptrdiff_t n, const void *x, int rt_to, void *y, void *s) {
     if (rt_from == SFEM_AMD_FLOAT64 && rt_to == SFEM_AMD_FLOAT32) return blas_cast<double, float>(n, (const double *)x, (float *)y, as_stream(s));
     if (rt_from == SFEM_AMD_FLOAT32 && rt_to == SFEM_AMD_FLOAT64) return blas_cast<float, double>(n, (const float *)x, (double *)y, as_stream(s));

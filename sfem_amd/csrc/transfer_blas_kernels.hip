@@ -1,5 +1,8 @@
 // gfx950 kernels: sshex8 transfers, BLAS-1 (deterministic reductions), fused solver
 // updates and Dirichlet helpers.  T = double (plain method) or float (FP32 preconditioner).
+#include <mutex>
+#include <string>
+
 #include "common.hpp"
 #include "transfer_blas_kernels.hpp"
 
@@ -739,6 +742,58 @@ int blas_dot(ptrdiff_t n, const T *x, const T *y, double *ws, double *result_hos
     return finish_sums(nb, 1, ws, result_host, s);
 }
 
+// Allocation-free dot: the per-block partials live in `scratch`, a file-scope device array that
+// every GPU holds once for the life of the process (8 KiB: one double per block of the capped
+// reduction grid).  Host code never passes the array's name to a kernel: on the host it is a
+// stand-in, not a device address.  The address is looked up once per device with that device
+// current and kept in scratch_dev[device].  The partials are written in full before they are read,
+// so nothing needs clearing; calls that overlap on one GPU share the array and race.
+constexpr int kScratchDoubles = kRedBlocks;
+constexpr int kScratchMaxDevices = 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+static double *scratch_ptr() {
+    static std::mutex mu;
+    static double *scratch_dev[kScratchMaxDevices] = {};
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchMaxDevices) {
+        (void)hipGetLastError();
+        set_last_error("dot_noalloc: no current HIP device (or device index beyond 63)");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !p) {
+            (void)hipGetLastError();
+            set_last_error(std::string("dot_noalloc: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(err));
+            return nullptr;
+        }
+        scratch_dev[dev] = (double *)p;
+    }
+    return scratch_dev[dev];
+}
+
+template <typename T>
+int blas_dot_noalloc(ptrdiff_t n, const T *x, const T *y, double *result, hipStream_t s) {
+    if (n < 0 || !result || (n > 0 && (!x || !y))) {
+        set_last_error("dot_noalloc: n must be >= 0 and x, y, result non-null");
+        return SFEM_FAILURE;
+    }
+    const int nb = red_blocks(n);  // same grid as blas_dot: the same partials, summed in the same order
+    if (nb > kScratchDoubles) {
+        set_last_error("dot_noalloc: reduction grid exceeds the 1024-partial scratch array");
+        return SFEM_FAILURE;
+    }
+    double *ws = scratch_ptr();
+    if (!ws) return SFEM_FAILURE;
+    dot_partial_kernel<T><<<nb, kBlock, 0, s>>>(n, x, y, ws);
+    sum_partials_kernel<<<1, kBlock, 0, s>>>(nb, ws, result);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+
 template <typename T>
 int cg_update(ptrdiff_t n, T a, const T *p, const T *Ap, T *x, T *r, double *ws, double *rtr_host, hipStream_t s,
               const T *z, double *rz_host) {
@@ -823,6 +878,7 @@ int cast_accumulate(ptrdiff_t n, const float *x, double *y, hipStream_t s) {
     template int block_sym_inverse<T>(ptrdiff_t, const T *, const uint8_t *, double, T *, hipStream_t);   \
     template int block3_apply<T>(ptrdiff_t, const T *, const T *, T *, hipStream_t);                      \
     template int blas_dot<T>(ptrdiff_t, const T *, const T *, double *, double *, hipStream_t);         \
+    template int blas_dot_noalloc<T>(ptrdiff_t, const T *, const T *, double *, hipStream_t);           \
     template int cg_update<T>(ptrdiff_t, T, const T *, const T *, T *, T *, double *, double *, hipStream_t,  \
                               const T *, double *);                                                       \
     template int dofs_set_value<T>(ptrdiff_t, const int32_t *, T, T *, hipStream_t);                    \

@@ -55,6 +55,9 @@ template <typename T>
 int cheb3_start(ptrdiff_t n, T alpha, const T *rhs, const T *dinv, int have_ax, T *p, T *x, hipStream_t s);
 // ws: device workspace of reduction_workspace_doubles() doubles
 template <typename T> int blas_dot(ptrdiff_t n, const T *x, const T *y, double *ws, double *result_host, hipStream_t s);
+// the same dot with its partials in the library's static device scratch: *result (device) = (x, y),
+// no allocation and no host synchronisation
+template <typename T> int blas_dot_noalloc(ptrdiff_t n, const T *x, const T *y, double *result, hipStream_t s);
 template <typename T>
 // with z (and rtr_host): also (r, z) of the updated r into *rz_host
 int cg_update(ptrdiff_t n, T a, const T *p, const T *Ap, T *x, T *r, double *ws, double *rtr_host, hipStream_t s,
