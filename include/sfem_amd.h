@@ -57,7 +57,12 @@ const char *sfem_amd_last_error(void);
  * "atomic_skeleton" (default 0): 1 makes accumulating (y += A x), unconstrained sshex8 Laplacian
  * applies add the macro-element boundary contributions to y with atomics, one launch and no
  * partials (results differ from the default in the last bits and are not reproducible run to
- * run); 0 keeps the fixed-order two-phase sum. */
+ * run); 0 keeps the fixed-order two-phase sum.  "elasticity_form" (default 0): the sshex8 linear
+ * elasticity apply in stencil form (a 27-point stencil of 3x3 blocks per macro element, FP64
+ * arithmetic) or in element form (the shared 24x24 micro-element matrix, on the matrix cores for even
+ * levels); 0 picks the stencil form for FP64 operators at levels >= 10, 1 forces the element form,
+ * 2 the stencil form (levels 2-6, 8, 10, 12, 16).  Both give the reference's operator; the
+ * summation order differs. */
 int sfem_amd_set_option(const char *name, int value);
 /* Number of visible GPUs; 0 when no HIP device is present. */
 int sfem_amd_device_count(void);

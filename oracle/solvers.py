@@ -73,6 +73,50 @@ def cg(A, b, x, rtol=1e-10, atol=1e-16, max_it=10000, precond=None):
     return x, max_it, 1, hist
 
 
+def flexible_pcg(A, b, x, precond, rtol=1e-10, atol=1e-16, max_it=10000, residual_replacement=50):
+    """The build's flexible PCG (the reference has none; its PCG is algebra/sfem_cg.hpp:365-445):
+    beta = r_new.(z_new - z_old) / (r_old.z_old) with r_new.z_old taken before z is replaced; stops
+    on the true relative residual ||b - A x|| / ||b|| (confirmed by recomputation), and replaces the
+    recursive residual by b - A x every `residual_replacement` iterations.  Returns
+    (x, iterations, status, history of relative residuals), iterations counted as completed steps.
+    With a fixed SPD preconditioner r_new.z_old = 0 in exact arithmetic, so the iterates are those
+    of standard PCG (tests/test_oracle_solvers.py pins it there and against dense solves)."""
+    x = x.copy()
+    bnorm = np.linalg.norm(b)
+    ref = bnorm if bnorm > 0 else 1.0
+    r = b - _apply0(A, x)
+    rnorm = np.linalg.norm(r)
+    hist = []
+    if rnorm == 0 or rnorm / ref < rtol or rnorm < atol:
+        return x, 0, 0, hist
+    z = _apply0(precond, r)
+    p = z.copy()
+    rtz = r @ z
+    for it in range(max_it):
+        Ap = _apply0(A, p)
+        alpha = rtz / (p @ Ap)
+        x += alpha * p
+        r -= alpha * Ap
+        if residual_replacement > 0 and (it + 1) % residual_replacement == 0:
+            r = b - _apply0(A, x)
+        rtzold = r @ z
+        rnorm = np.linalg.norm(r)
+        hist.append(rnorm / ref)
+        if rnorm / ref < rtol or rnorm < atol:
+            r = b - _apply0(A, x)
+            true_norm = np.linalg.norm(r)
+            if true_norm / ref < rtol or true_norm < atol:
+                hist[-1] = true_norm / ref
+                return x, it + 1, 0, hist
+            rtzold = r @ z
+        z = _apply0(precond, r)
+        rtz_new = r @ z
+        beta = (rtz_new - rtzold) / rtz
+        rtz = rtz_new
+        p = z + beta * p
+    return x, max_it, 1, hist
+
+
 def power_method(A, n, max_it, tol, v0=None):
     """PowerMethod::max_eigen_value (algebra/sfem_PowerMethod.hpp:18-55): odd iteration
     count, stops when |lambda - lambda_old| <= tol on an even step."""

@@ -219,6 +219,17 @@ __host__ __device__ __forceinline__ T cheb3_direction(T p, T rhs, T t, T beta, c
     return first ? r : __builtin_fma(beta, p, r);
 }
 
+// ----------------------------------------------------------------- workgroup barrier for LDS
+// Orders LDS accesses only: __syncthreads() is a workgroup fence over all address spaces, which on
+// this chip waits for every outstanding global load and store (vmcnt(0)) before the barrier.
+// Kernels whose barriers only separate LDS writes from LDS reads use this one, so global stores
+// issued before it stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ----------------------------------------------------------------- launch helpers
 constexpr int kBlock = 256;
 inline unsigned grid_for(ptrdiff_t n, int block = kBlock, unsigned cap = 1u << 20) {

@@ -157,6 +157,10 @@ namespace sfem_amd {
 extern bool g_fused_chebyshev;
 // sshex8 Laplacian accumulating applies sum skeleton nodes with atomics (option "atomic_skeleton")
 extern bool g_atomic_skeleton;
+// sshex8 elasticity operators: 0 stencil form for FP64 at levels >= 10, element form otherwise; 1
+// element form (matrix-core / node-wise kernels) everywhere; 2 stencil form wherever supported
+// (option "elasticity_form")
+extern int g_elasticity_form;
 // Zeroes / sets constrained dofs; defined in runtime.cpp.
 int constrained_set_value(const OpBase *constrained, int rt, double v, void *x, hipStream_t s);
 int mesh_counts(const sfem_amd_mesh *m, hipStream_t s);

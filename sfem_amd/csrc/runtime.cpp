@@ -153,6 +153,7 @@ static int mesh_owner(const sfem_amd_mesh *m, hipStream_t s) {
 }
 bool g_fused_chebyshev = false;
 bool g_atomic_skeleton = false;
+int g_elasticity_form = 0;
 
 const sfem_amd_mesh *mesh_subblocks(const sfem_amd_mesh *m, int Ls, hipStream_t s) {
     auto &slot = m->sub[Ls];
@@ -259,15 +260,26 @@ struct ElasticityOp final : OpBase {
     // FP32 from geometry: the bf16x3 MFMA kernel (FP64 always runs on the matrix cores: exact)
     bool matrix_cores{false};
     DevArray<T> part;                 // sshex8: skeleton contributions [E][sk_count(L)][3]
+    const sfem_amd_mesh *part_mesh{nullptr};  // the mesh `part` was sized for
     // levels beyond the matrix-core kernels' LDS tile (L = 2 Ls): the operator applied as 8 sub-blocks
     // of level Ls per macro element (sharing its matrix), with their own skeleton slots
     double mu{0}, lambda{0};
     int real_type() const override { return std::is_same<T, double>::value ? SFEM_AMD_FLOAT64 : SFEM_AMD_FLOAT32; }
     ptrdiff_t rows() const override { return m->nnodes * 3; }
     bool uses_matrix_cores() const { return std::is_same<T, double>::value || matrix_cores; }
+    // the stencil form (27-point 3x3-block stencil per macro element, sshex8_elasticity_apply_stencil):
+    // option elasticity_form 2; by default (0) the FP64 operators at levels >= 10, where it beats the
+    // element form (L = 16 on a 24^3 macro cube: 2.5 vs 3.0 ms; the FP32 matrix-core element form
+    // and the FP64 one at L <= 8 stay faster); 1 keeps the element form
+    bool uses_stencil() const {
+        if (m->type != SFEM_AMD_SSHEX8 || !sshex8_elasticity_stencil_supported(m->level)) return false;
+        if (g_elasticity_form == 1) return false;
+        if (g_elasticity_form == 2) return true;
+        return std::is_same<T, double>::value && m->level >= 10;
+    }
     // the sub-block level, or 0 when the macro elements are applied directly
     int subblock_level() const {
-        if (m->type != SFEM_AMD_SSHEX8 || !uses_matrix_cores()) return 0;
+        if (m->type != SFEM_AMD_SSHEX8 || !uses_matrix_cores() || uses_stencil()) return 0;
         const int L = m->level;
         if constexpr (std::is_same<T, double>::value) {
             if (L % 2 == 0 && L <= 8) return 0;
@@ -304,7 +316,7 @@ struct ElasticityOp final : OpBase {
         return apply_impl(xv, yv, s, ow, cmask);
     }
     bool cheb3_fusable(hipStream_t s) override {
-        if (!g_fused_chebyshev || m->type != SFEM_AMD_SSHEX8) return false;
+        if (!g_fused_chebyshev || m->type != SFEM_AMD_SSHEX8 || uses_stencil()) return false;
         const sfem_amd_mesh *wm = work_mesh(s);
         if (!wm || mesh_skeleton(wm, 0) || !skeleton_covers(wm)) return false;
         return sshex8_elasticity_cheb3_fusable(wm->level, std::is_same<T, double>::value, uses_matrix_cores());
@@ -332,9 +344,13 @@ struct ElasticityOp final : OpBase {
         }
         const sfem_amd_mesh *wm = work_mesh(s);
         if (!wm) return SFEM_FAILURE;
-        if (!part.p && wm->E > 0) {
-            if (mesh_skeleton(wm, 0) || part.alloc((size_t)wm->E * sk_count(wm->level) * 3)) return SFEM_FAILURE;
+        // the partials are sized for the mesh the kernels run on (the form may change with the option)
+        if ((!part.p || part_mesh != wm) && wm->E > 0) {
+            // (+3: the stencil kernel's sink for lanes without a store)
+            if (mesh_skeleton(wm, 0) || part.alloc((size_t)wm->E * sk_count(wm->level) * 3 + 3)) return SFEM_FAILURE;
+            part_mesh = wm;
         }
+        const bool stencil = uses_stencil() && !cs;
         const T *Kt;
         if constexpr (std::is_same<T, double>::value) {
             Kt = K.p;
@@ -343,7 +359,7 @@ struct ElasticityOp final : OpBase {
                 if (K32.alloc(K.n) || blas_cast<double, float>((ptrdiff_t)K.n, K.p, K32.p, s)) return SFEM_FAILURE;
             }
             Kt = K32.p;
-            if (matrix_cores && !Apk.p && K32.n) {
+            if (matrix_cores && !stencil && !Apk.p && K32.n) {
                 const ptrdiff_t nK = (ptrdiff_t)K32.n / 576;
                 if (Apk.alloc((size_t)nK * 1024) || sshex8_elasticity_pack_bf16x2(nK, K32.p, Apk.p, s))
                     return SFEM_FAILURE;
@@ -351,6 +367,9 @@ struct ElasticityOp final : OpBase {
         }
         SkeletonMap sk = skeleton_of(wm);
         sk.cmask = cmask;
+        if (stencil)
+            return sshex8_elasticity_apply_stencil<T>(m->level, m->E, m->idx.p, K.p, (const T *)xv, (T *)yv, part.p,
+                                                      sk, s, ow);
         return sshex8_elasticity_apply<T>(wm->level, wm->E, wm->idx.p, Kt, (const T *)xv, (T *)yv, part.p, sk, s, ow,
                                           uses_matrix_cores() ? 1 : 0, wm == m ? 0 : 3, Apk.p,
                                           cs ? *cs : ChebStep<T>());
@@ -863,6 +882,11 @@ int sfem_amd_set_option(const char *name, int value) {
     }
     if (std::strcmp(name, "atomic_skeleton") == 0) {
         g_atomic_skeleton = value != 0;
+        return SFEM_SUCCESS;
+    }
+    if (std::strcmp(name, "elasticity_form") == 0) {
+        SFEM_AMD_REQUIRE(value >= 0 && value <= 2, "set_option: elasticity_form is 0 (auto), 1 (element) or 2 (stencil)");
+        g_elasticity_form = value;
         return SFEM_SUCCESS;
     }
     set_last_error(std::string("set_option: unknown option ") + name);

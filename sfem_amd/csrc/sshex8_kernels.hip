@@ -13,6 +13,7 @@
 #include <hip/hip_bf16.h>
 
 #include <type_traits>
+#include <utility>
 
 namespace sfem_amd {
 
@@ -446,6 +447,16 @@ __global__ void __launch_bounds__(ElShape<L>::BLOCK)
         __syncthreads();
     }
 }
+
+// ------------------------------------------------------------------------------------------
+// sshex8 elasticity in stencil form.  All micro elements of an affine macro element share one
+// 24x24 matrix K, so at a node whose 8 surrounding micro elements lie in the element the operator
+// is one 27-point stencil of 3x3 blocks (243 coefficients).  Split by the z offset d of the
+// neighbour: S_m (d = -1), S_p (d = +1) and the in-plane part S_0 = S_0lo + S_0up, where S_0lo
+// collects the micro elements above the node and S_0up those below (a node on the z = 0 face sees
+// only S_0lo, on z = L only S_0up).  Coefficient order within a block: [nbr 9][i][j],
+// nbr = (dy+1)*3 + dx+1, i the output component, j the input component.
+__host__ __device__ constexpr inline int st_nbr(int dx, int dy) { return (dy + 1) * 3 + dx + 1; }
 
 // ------------------------------------------------------------------------------------------
 // FP32 sshex8 elasticity on the matrix cores (the operator of the FP32 multigrid preconditioner).
@@ -1271,6 +1282,310 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
     return skeleton_gather<T>(sk, 3, part, y, s, overwrite, -1, x);
 }
 
+// ------------------------------------------------------------------------------------------
+// Stencil form with the coefficients in VGPRs.  Every 16-lane row of a wave holds one stencil
+// table: lane k keeps coefficients k, k + 16, ... (21 doubles), and each FMA takes its coefficient
+// from lane n % 16 of the row through the DPP row broadcast (v_fmac_f64 with row_newbcast), so the
+// 243 FMAs of a node need no scalar loads and no per-lane masks.  A row's columns share one kind
+// of (x, y) position, whose micro elements (xy quadrants, bit qy*2 + qx) define its table: interior
+// columns (all four), the four faces y = 0, x = L, y = L, x = 0 (two quadrants each, L - 1 columns)
+// and the four corner columns (one quadrant, one column).  Register table blocks: S_m, S_0, S_p;
+// the z-face block S_0lo (a node on z = 0 sees S_0lo, on z = L S_0 - S_0lo) stays in LDS, read on
+// those two planes only.  Arithmetic is FP64 for both vector precisions (FP32 vectors
+// are converted on load and store: on this chip FP32 FMA without packing runs at the FP64 rate).
+template <int K>
+__device__ __forceinline__ void fmac_row_bcast(double &acc, double coef, double v) {
+    asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" : "+v"(acc) : "v"(coef), "v"(v), "i"(K));
+}
+
+template <int L>
+struct StDpp {
+    static constexpr int N1 = L + 1, PLANE = N1 * N1, H = N1 + 2, P2 = H * H;  // halo-padded plane
+    static constexpr int NI = (L - 1) * (L - 1);
+    static constexpr int RI = (NI + 15) / 16;  // rows of interior columns
+    static constexpr int ROWS = RI + 8;        // + 4 face rows + 4 corner rows
+    static constexpr int BLOCK = 64 * ((ROWS + 3) / 4);
+    static constexpr int NC = 4 * 81;           // coefficients per kind (S_m, S_0, S_p, S_0lo)
+    static constexpr int NT = (3 * 81 + 15) / 16;  // register table per lane (S_m, S_0, S_p)
+    static constexpr int NTYPES = 9;
+};
+
+// quadrant mask of row kind t: 0 interior, 1-4 faces y = 0, x = L, y = L, x = 0, 5-8 corners
+// (0, 0), (L, 0), (L, L), (0, L)
+__host__ __device__ constexpr inline int st_kind_qmask(int t) {
+    constexpr int q[9] = {0b1111, 0b1100, 0b0101, 0b0011, 0b1010, 0b1000, 0b0100, 0b0001, 0b0010};
+    return q[t];
+}
+
+// coefficient of block b at neighbour (dx, dy), rows i, cols j, for the quadrants in qmask
+__device__ inline double st_coefficient_q(const double *K, int b, int dx, int dy, int i, int j, int qmask) {
+    const int dz = b == 0 ? -1 : (b == 2 ? 1 : 0);
+    const bool below = b == 0 || b == 1, above = b == 1 || b == 2 || b == 3;
+    double v = 0;
+    for (int oz = 0; oz < 2; oz++) {
+        if (!(oz ? above : below)) continue;
+        const int cz = 1 - oz, nz = cz + dz;
+        if (nz < 0 || nz > 1) continue;
+        for (int qy = 0; qy < 2; qy++)
+            for (int qx = 0; qx < 2; qx++) {
+                if (!((qmask >> (qy * 2 + qx)) & 1)) continue;
+                const int cx = 1 - qx, cy = 1 - qy, nx = cx + dx, ny = cy + dy;
+                if (nx < 0 || nx > 1 || ny < 0 || ny > 1) continue;
+                v += K[(i * 8 + hex8_corner(cx, cy, cz)) * 24 + j * 8 + hex8_corner(nx, ny, nz)];
+            }
+    }
+    return v;
+}
+
+// The FMAs of neighbour NB (values v of its three components) into the outputs at z = p + 1
+// (block S_m), p (block S_0, unless MID is false) and p - 1 (block S_p), output component fastest
+// within each input component, so consecutive FMAs update different accumulators.
+template <int NB, int NT, bool MID>
+__device__ __forceinline__ void st_dpp_nbr(double (&ah)[3], double (&am)[3], double (&al)[3], const double (&tab)[NT],
+                                           const double v[3]) {
+    fmac_row_bcast<(0 + NB * 9 + 0) % 16>(ah[0], tab[(0 + NB * 9 + 0) / 16], v[0]);
+    if (MID) fmac_row_bcast<(81 + NB * 9 + 0) % 16>(am[0], tab[(81 + NB * 9 + 0) / 16], v[0]);
+    fmac_row_bcast<(162 + NB * 9 + 0) % 16>(al[0], tab[(162 + NB * 9 + 0) / 16], v[0]);
+    fmac_row_bcast<(0 + NB * 9 + 3) % 16>(ah[1], tab[(0 + NB * 9 + 3) / 16], v[0]);
+    if (MID) fmac_row_bcast<(81 + NB * 9 + 3) % 16>(am[1], tab[(81 + NB * 9 + 3) / 16], v[0]);
+    fmac_row_bcast<(162 + NB * 9 + 3) % 16>(al[1], tab[(162 + NB * 9 + 3) / 16], v[0]);
+    fmac_row_bcast<(0 + NB * 9 + 6) % 16>(ah[2], tab[(0 + NB * 9 + 6) / 16], v[0]);
+    if (MID) fmac_row_bcast<(81 + NB * 9 + 6) % 16>(am[2], tab[(81 + NB * 9 + 6) / 16], v[0]);
+    fmac_row_bcast<(162 + NB * 9 + 6) % 16>(al[2], tab[(162 + NB * 9 + 6) / 16], v[0]);
+    fmac_row_bcast<(0 + NB * 9 + 1) % 16>(ah[0], tab[(0 + NB * 9 + 1) / 16], v[1]);
+    if (MID) fmac_row_bcast<(81 + NB * 9 + 1) % 16>(am[0], tab[(81 + NB * 9 + 1) / 16], v[1]);
+    fmac_row_bcast<(162 + NB * 9 + 1) % 16>(al[0], tab[(162 + NB * 9 + 1) / 16], v[1]);
+    fmac_row_bcast<(0 + NB * 9 + 4) % 16>(ah[1], tab[(0 + NB * 9 + 4) / 16], v[1]);
+    if (MID) fmac_row_bcast<(81 + NB * 9 + 4) % 16>(am[1], tab[(81 + NB * 9 + 4) / 16], v[1]);
+    fmac_row_bcast<(162 + NB * 9 + 4) % 16>(al[1], tab[(162 + NB * 9 + 4) / 16], v[1]);
+    fmac_row_bcast<(0 + NB * 9 + 7) % 16>(ah[2], tab[(0 + NB * 9 + 7) / 16], v[1]);
+    if (MID) fmac_row_bcast<(81 + NB * 9 + 7) % 16>(am[2], tab[(81 + NB * 9 + 7) / 16], v[1]);
+    fmac_row_bcast<(162 + NB * 9 + 7) % 16>(al[2], tab[(162 + NB * 9 + 7) / 16], v[1]);
+    fmac_row_bcast<(0 + NB * 9 + 2) % 16>(ah[0], tab[(0 + NB * 9 + 2) / 16], v[2]);
+    if (MID) fmac_row_bcast<(81 + NB * 9 + 2) % 16>(am[0], tab[(81 + NB * 9 + 2) / 16], v[2]);
+    fmac_row_bcast<(162 + NB * 9 + 2) % 16>(al[0], tab[(162 + NB * 9 + 2) / 16], v[2]);
+    fmac_row_bcast<(0 + NB * 9 + 5) % 16>(ah[1], tab[(0 + NB * 9 + 5) / 16], v[2]);
+    if (MID) fmac_row_bcast<(81 + NB * 9 + 5) % 16>(am[1], tab[(81 + NB * 9 + 5) / 16], v[2]);
+    fmac_row_bcast<(162 + NB * 9 + 5) % 16>(al[1], tab[(162 + NB * 9 + 5) / 16], v[2]);
+    fmac_row_bcast<(0 + NB * 9 + 8) % 16>(ah[2], tab[(0 + NB * 9 + 8) / 16], v[2]);
+    if (MID) fmac_row_bcast<(81 + NB * 9 + 8) % 16>(am[2], tab[(81 + NB * 9 + 8) / 16], v[2]);
+    fmac_row_bcast<(162 + NB * 9 + 8) % 16>(al[2], tab[(162 + NB * 9 + 8) / 16], v[2]);
+}
+
+template <typename TIO, int L, int G>
+__global__ void __launch_bounds__(StDpp<L>::BLOCK)
+    sshex8_elasticity_stencil_dpp_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx,
+                                         const double *__restrict__ Kall, const TIO *__restrict__ x,
+                                         TIO *__restrict__ y, TIO *__restrict__ part,
+                                         const int32_t *__restrict__ sk_pos, const int overwrite) {
+    using S = StDpp<L>;
+    constexpr int N1 = S::N1, PLANE = S::PLANE, H = S::H, P2 = S::P2, NXE = N1 * PLANE, NSK = sk_count(L),
+                  NT = S::NT, NC = S::NC;
+    constexpr int XR = 2 * G;      // x planes in LDS: G being read, G arriving
+    constexpr int IR = 3 * G + 1;  // node-id ring: planes z - 1 .. z + 3G - 1 of a step starting at z
+    __shared__ double stab[S::NTYPES][NC];  // the nine kinds' tables (S_0lo read from here)
+    __shared__ double sx[XR][3][P2 + 1];    // halo-padded planes (+ a spare cell for idle lanes)
+    __shared__ double sK[576];
+    __shared__ int32_t sg[IR][S::BLOCK];    // this thread's node ids of planes z, by z % IR
+
+    const ptrdiff_t e = blockIdx.x;
+    const int tid = threadIdx.x, r = tid >> 4, k = tid & 15;
+    for (int i = tid; i < 576; i += S::BLOCK) sK[i] = Kall[e * 576 + i];
+    for (int i = tid; i < XR * 3 * (P2 + 1); i += S::BLOCK) (&sx[0][0][0])[i] = 0;  // the halo stays zero
+
+    // this lane's column and its row's kind
+    int cx = 1, cy = 1, kind = -1;
+    bool active = false;
+    if (r < S::RI) {
+        const int ci = r * 16 + k;
+        kind = 0;
+        if (ci < S::NI) {
+            active = true;
+            cx = 1 + ci % (L - 1);
+            cy = 1 + ci / (L - 1);
+        }
+    } else if (r < S::RI + 4) {
+        const int f = r - S::RI, t = 1 + min(k, L - 2);
+        kind = 1 + f;
+        active = k < L - 1;
+        if (f == 0) { cx = t; cy = 0; }
+        else if (f == 1) { cx = L; cy = t; }
+        else if (f == 2) { cx = t; cy = L; }
+        else { cx = 0; cy = t; }
+    } else if (r < S::RI + 8) {
+        const int c = r - S::RI - 4;
+        kind = 5 + c;
+        active = k == 0;
+        if (c == 0) { cx = 0; cy = 0; }
+        else if (c == 1) { cx = L; cy = 0; }
+        else if (c == 2) { cx = L; cy = L; }
+        else { cx = 0; cy = L; }
+    }
+    const bool interior = kind == 0;
+    const int32_t *row = idx + e * NXE + cy * N1 + cx;  // node (cx, cy, z) at row[z * PLANE]
+    const int me = (cy + 1) * H + cx + 1;               // its cell in the halo-padded plane
+    // Every lane's column lies in the element (idle lanes sit on a real node too), so no load needs
+    // a condition.  Loads and stores share one wait counter on this chip, and a wait while both are
+    // pending covers all of them: so each step issues all its loads first, consumes every loaded
+    // value (into LDS or the stored data) before it issues its stores, and carries no loaded
+    // register into the next step (node ids travel through the thread's own LDS ring).
+    TIO x0[G][3];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const int32_t ga = row[(g <= L ? g : L) * PLANE], gb = row[(G + g <= L ? G + g : L) * PLANE];
+        sg[g % IR][tid] = ga;
+        sg[(G + g) % IR][tid] = gb;
+#pragma unroll
+        for (int c = 0; c < 3; c++) x0[g][c] = x[(ptrdiff_t)ga * 3 + c];
+    }
+    __syncthreads();  // K staged, planes zeroed
+    for (int t = tid; t < S::NTYPES * NC; t += S::BLOCK) {
+        const int kd = t / NC, n = t - kd * NC, b = n / 81, q = n - b * 81, nb = q / 9;
+        stab[kd][n] = st_coefficient_q(sK, b, nb % 3 - 1, nb / 3 - 1, (q % 9) / 3, q % 3, st_kind_qmask(kd));
+    }
+    __syncthreads();  // tables ready
+    double tab[NT];
+    const double *lo = stab[kind >= 0 ? kind : 0] + 3 * 81;  // S_0lo of this row's kind
+#pragma unroll
+    for (int m = 0; m < NT; m++) {
+        const int n = m * 16 + k;
+        tab[m] = (kind >= 0 && n < 3 * 81) ? stab[kind][n] : 0.0;
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) sx[g][c][active ? me : P2] = (double)x0[g][c];
+
+    double acc_lo[3] = {0, 0, 0}, acc_mid[3] = {0, 0, 0}, acc_hi[3] = {0, 0, 0};
+    for (int p0 = 0; p0 <= L; p0 += G) {
+        lds_barrier();  // planes p0 .. p0 + G - 1 staged; the other G buffers are free
+        // loads of this step: for the nodes at z = p - 1 completed by planes p = p0 .. p0 + G - 1,
+        // their old y (interior nodes, accumulating) and skeleton slot positions; the ids of planes
+        // p0 + 2G ..; x of planes p0 + G .. (all clamped at z = L)
+        TIO yold[G][3], xn[G][3];
+        int32_t spos[G], gid2[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int q = p0 + g > 0 ? p0 + g - 1 : 0;
+            const int32_t gq = sg[q % IR][tid];
+            const int slot = sk_slot(L, cx, cy, q);
+            spos[g] = sk_pos[e * NSK + (slot >= 0 ? slot : 0)];
+#pragma unroll
+            for (int c = 0; c < 3; c++) yold[g][c] = overwrite ? TIO(0) : y[(ptrdiff_t)gq * 3 + c];
+        }
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int z2 = p0 + 2 * G + g, z1 = p0 + G + g;
+            gid2[g] = row[(z2 <= L ? z2 : L) * PLANE];
+            const int32_t g1 = sg[(z1 <= L ? z1 : L) % IR][tid];
+#pragma unroll
+            for (int c = 0; c < 3; c++) xn[g][c] = x[(ptrdiff_t)g1 * 3 + c];
+        }
+        double res[G][3];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int p = p0 + g;
+            const int buf = p % XR;
+            const double *pl0 = &sx[buf][0][0], *pl1 = &sx[buf][1][0], *pl2 = &sx[buf][2][0];
+            // all lanes run the FMAs (the table lanes of every row must be active for the broadcast);
+            // the outputs of the planes outside [0, L] are never stored.  FACE: 0 inner plane, 1 z = 0
+            // (in-plane block S_0lo from LDS instead of S_0), 2 z = L (S_0 - S_0lo)
+            auto sweep = [&](auto face) {
+                constexpr int FACE = decltype(face)::value;
+#define ST_NB(NB)                                                                                       \
+    {                                                                                                   \
+        constexpr int o = (NB / 3 - 1) * H + (NB % 3 - 1);                                              \
+        const double v[3] = {pl0[me + o], pl1[me + o], pl2[me + o]};                                    \
+        st_dpp_nbr<NB, NT, FACE != 1>(acc_hi, acc_mid, acc_lo, tab, v);                                 \
+        if (FACE) {                                                                                     \
+            const double sgn = FACE == 1 ? 1.0 : -1.0;                                                  \
+            _Pragma("unroll") for (int i = 0; i < 3; i++)                                               \
+                _Pragma("unroll") for (int j = 0; j < 3; j++)                                           \
+                    acc_mid[i] = fma(sgn * lo[NB * 9 + i * 3 + j], v[j], acc_mid[i]);                   \
+        }                                                                                               \
+    }
+                ST_NB(0) ST_NB(1) ST_NB(2) ST_NB(3) ST_NB(4) ST_NB(5) ST_NB(6) ST_NB(7) ST_NB(8)
+#undef ST_NB
+            };
+            if (p <= L) {
+                if (p == 0)
+                    sweep(std::integral_constant<int, 1>());
+                else if (p == L)
+                    sweep(std::integral_constant<int, 2>());
+                else
+                    sweep(std::integral_constant<int, 0>());
+            }
+            // plane p - 1 is complete
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                res[g][c] = acc_lo[c];
+                acc_lo[c] = acc_mid[c];
+                acc_mid[c] = acc_hi[c];
+                acc_hi[c] = 0;
+            }
+        }
+        // consume the loads: x of the next G planes into the free buffers (idle lanes: the spare
+        // cell), the ids into the ring
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int z1 = p0 + G + g;
+#pragma unroll
+            for (int c = 0; c < 3; c++) sx[z1 % XR][c][active ? me : P2] = (double)xn[g][c];
+            sg[(p0 + 2 * G + g) % IR][tid] = gid2[g];
+        }
+        // the stores of the completed planes z = p - 1 (lanes with nothing to store write the
+        // 3-entry sink after the partials, which nothing reads: no branch)
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            // (a step slot p = L + 1 computes nothing and completes z = L)
+            const int p = p0 + g, q = p - 1;
+            const bool ok = active && p >= 1 && p <= L + 1;
+            const bool to_y = interior && q > 0 && q < L;
+            const int32_t gq = sg[(q >= 0 ? q : 0) % IR][tid];
+            TIO *dst = ok ? (to_y ? y + (ptrdiff_t)gq * 3 : part + (ptrdiff_t)spos[g] * 3) : part + E * NSK * 3;
+#pragma unroll
+            for (int c = 0; c < 3; c++) dst[c] = (TIO)((to_y ? (double)yold[g][c] : 0.0) + res[g][c]);
+        }
+    }
+    // z = L (a skeleton plane) when the last step ended with it (L + 1 a multiple of G); otherwise the
+    // step slot p = L + 1 stored it
+    if ((L + 1) % G == 0 && active) {
+        const ptrdiff_t ps = (ptrdiff_t)sk_pos[e * NSK + sk_slot(L, cx, cy, L)] * 3;
+#pragma unroll
+        for (int c = 0; c < 3; c++) part[ps + c] = (TIO)acc_lo[c];
+    }
+}
+
+bool sshex8_elasticity_stencil_supported(int L) {
+    return L == 2 || L == 3 || L == 4 || L == 5 || L == 6 || L == 8 || L == 10 || L == 12 || L == 16;
+}
+
+template <typename T>
+int sshex8_elasticity_apply_stencil(int L, ptrdiff_t E, const int32_t *idx, const double *K64, const T *x, T *y,
+                                    T *part, const SkeletonMap &sk, hipStream_t s, int overwrite) {
+    if (E == 0) return SFEM_SUCCESS;
+    {
+        if (!K64 || !part || !sk.pos) {
+            set_last_error("sshex8_elasticity_apply_stencil: matrices, partials and slot positions required");
+            return SFEM_FAILURE;
+        }
+        switch (L) {
+#define DCASE(LV)                                                                                              \
+    case LV:                                                                                                   \
+        sshex8_elasticity_stencil_dpp_kernel<T, LV, 1><<<(unsigned)E, StDpp<LV>::BLOCK, 0, s>>>(E, idx, K64, x, y, part, \
+                                                                                            sk.pos, overwrite); \
+        break;
+            DCASE(2) DCASE(3) DCASE(4) DCASE(5) DCASE(6) DCASE(8) DCASE(10) DCASE(12) DCASE(16)
+#undef DCASE
+            default:
+                set_last_error("sshex8 elasticity stencil form: level not supported: " + std::to_string(L));
+                return SFEM_FAILURE;
+        }
+        SFEM_AMD_HIP_CHECK(hipGetLastError());
+        return skeleton_gather<T>(sk, 3, part, y, s, overwrite, 1, x);
+    }
+}
+
 template <typename T>
 int sshex8_elasticity_diag(int L, ptrdiff_t E, const int32_t *idx, const double *K, int block, T *d, hipStream_t s) {
     if (E == 0) return SFEM_SUCCESS;
@@ -1315,6 +1630,10 @@ template int sshex8_elasticity_apply<double>(int, ptrdiff_t, const int32_t *, co
 template int sshex8_elasticity_apply<float>(int, ptrdiff_t, const int32_t *, const float *, const float *, float *,
                                             float *, const SkeletonMap &, hipStream_t, int, int, int,
                                             const uint32_t *, const ChebStep<float> &);
+template int sshex8_elasticity_apply_stencil<double>(int, ptrdiff_t, const int32_t *, const double *, const double *,
+                                                     double *, double *, const SkeletonMap &, hipStream_t, int);
+template int sshex8_elasticity_apply_stencil<float>(int, ptrdiff_t, const int32_t *, const double *, const float *,
+                                                    float *, float *, const SkeletonMap &, hipStream_t, int);
 template int sshex8_elasticity_diag<double>(int, ptrdiff_t, const int32_t *, const double *, int, double *,
                                             hipStream_t);
 template int sshex8_elasticity_diag<float>(int, ptrdiff_t, const int32_t *, const double *, int, float *,

@@ -68,6 +68,15 @@ int sshex8_elasticity_pack_bf16x2(ptrdiff_t nK, const float *K, uint32_t *Apk, h
 // connectivity [E * (L/Ls)^3][nxe(Ls)] of the macro elements cut into (L/Ls)^3 sub-blocks of level
 // Ls (sub-element e * (L/Ls)^3 + (oz * s + oy) * s + ox): the level-L operator as a level-Ls one
 int sshex8_subblock_idx(ptrdiff_t E, int L, int Ls, const int32_t *idx, int32_t *out, hipStream_t s);
+// Stencil form of the sshex8 elasticity apply (levels 2-6, 8, 10, 12, 16): a 27-point stencil of
+// 3x3 blocks per macro element, derived in the kernel from its FP64 24x24 matrix K64 and applied in
+// FP64 for either vector precision; part: node-sorted skeleton partials (sk.pos), summed by
+// skeleton_gather, followed by 3 spare entries (E * sk_count(L) * 3 + 3 values) that the kernel
+// writes and nothing reads.
+bool sshex8_elasticity_stencil_supported(int L);
+template <typename T>
+int sshex8_elasticity_apply_stencil(int L, ptrdiff_t E, const int32_t *idx, const double *K64, const T *x, T *y,
+                                    T *part, const SkeletonMap &sk, hipStream_t s, int overwrite = 0);
 template <typename T>
 int sshex8_elasticity_diag(int L, ptrdiff_t E, const int32_t *idx, const double *K, int block, T *d, hipStream_t s);
 

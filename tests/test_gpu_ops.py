@@ -217,6 +217,54 @@ def test_sshex8_elasticity_parity(L, n):
         assert err < 1e-2
 
 
+@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("L,n", [(2, 2), (3, 2), (4, 2), (5, 1), (6, 2), (8, 2), (10, 1), (12, 1), (16, 2)])
+def test_sshex8_elasticity_stencil_and_element_forms(L, n, form):
+    """Both forms of the sshex8 elasticity apply (1: element form on the matrix cores / node-wise
+    kernel, 2: the 27-point 3x3-block stencil form) against the FP64 oracle, accumulating into a
+    nonzero y, with per-element coefficients; FP32 stencil form on the same BF16 geometry words."""
+    mel, mpts = skewed_box(n, perturb=0.02)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    adj, det = O.hex8_adj_det(mel, mpts)
+    E = mel.shape[1]
+    coeff = np.exp(np.random.default_rng(L + 7).uniform(np.log(0.1), np.log(10), E)).astype(np.float32)
+    mu, lam = O.lame(1.0, 0.3)
+    N = pts.shape[1]
+    rng = np.random.default_rng(L)
+    x = rng.uniform(-1, 1, 3 * N)
+    y0 = rng.uniform(-1, 1, 3 * N)
+    S.set_option("elasticity_form", form)
+    try:
+        mesh = S.Mesh(S.SSHEX8, L, el, N)
+        op = S.linear_elasticity(mesh, adj, det, mu, lam, coeff)
+        yo = y0.copy()
+        O.lib.oracle_sshex8_elasticity_apply(L, E, el, adj, det, mu, lam, coeff, x, yo)
+        y = dev(y0)
+        op.apply(dev(x), y)
+        S.synchronize()
+        err = relerr(host(y), yo)
+        print(f"L={L} form={form}: FP64 accumulate vs oracle {err:.2e}")
+        assert err < RTOL
+        if form == 2 and sshex8_stencil(L):
+            x32 = x.astype(np.float32)
+            op32 = S.linear_elasticity(mesh, adj, det, mu, lam, coeff, real_type=S.FLOAT32)
+            yb = np.zeros(3 * N)
+            O.lib.oracle_sshex8_elasticity_apply(L, E, el, bf16_round(adj), bf16_round(det), mu, lam, coeff,
+                                                 x32.astype(np.float64), yb)
+            y1 = host(gpu_apply(op32, dev(x32, np.float32)))
+            y2 = host(gpu_apply(op32, dev(x32, np.float32)))
+            assert np.array_equal(y1, y2)
+            err32 = relerr(y1.astype(np.float64), yb)
+            print(f"L={L}: FP32 stencil form vs FP64 oracle on the same BF16 geometry {err32:.2e}")
+            assert err32 < 1e-5
+    finally:
+        S.set_option("elasticity_form", 0)
+
+
+def sshex8_stencil(L):
+    return L in (2, 3, 4, 5, 6, 8, 10, 12, 16)
+
+
 def bf16_round(a):
     """float32 -> nearest-even bfloat16, returned as float32 (the FP32 operator's geometry words)."""
     u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)

@@ -265,6 +265,50 @@ def test_elasticity_mg_pcg_converges():
     assert solver.last_status == 0 and res < 1e-8
 
 
+def test_elasticity_mg_pcg_stencil_form_matches_element_form():
+    """The same L=8 elasticity MG-PCG (clamped z = 0, body force) with the operators in element form
+    and in stencil form: identical FP64 iteration counts and residual histories to rounding, and the
+    FP32 stencil preconditioner inside FP64 flexible CG reaches the FP64 tolerance."""
+    L = 8
+    mel, mpts = O.box_mesh(3, 3, 3)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    adj, det = O.hex8_adj_det(mel, mpts)
+    N = pts.shape[1]
+    nodes = np.where(pts[2] < 1e-6)[0].astype(np.int32)
+    mu, lam = O.lame(1.0, 0.3)
+    coeff = np.exp(np.random.default_rng(2).uniform(np.log(0.1), np.log(10), mel.shape[1])).astype(np.float32)
+    b = np.zeros((N, 3))
+    b[:, 2] = -1
+    b[nodes] = 0
+    runs = {}
+    for form in (1, 2):
+        S.set_option("elasticity_form", form)
+        try:
+            mesh = S.Mesh(S.SSHEX8, L, el, N)
+            h = gmg.SSGMG(mesh, "elasticity", (adj, det), nodes, coeff=coeff, mu=mu, lam=lam, coarse_rtol=1e-12)
+            solver = S.cg(h.fine.A, h.mg, rtol=1e-10, max_it=300)
+            x = zeros(3 * N)
+            solver.apply(dev(b.ravel()), x)
+            res = true_residual(h.fine.A, dev(b.ravel()), x)
+            h32 = gmg.SSGMG(mesh, "elasticity", (adj, det), nodes, coeff=coeff, mu=mu, lam=lam,
+                            real_type=S.FLOAT32)
+            fs = S.cg(h.fine.A, S.precision_bridge(h32.mg), flexible=True, rtol=1e-10, max_it=300)
+            x32 = zeros(3 * N)
+            fs.apply(dev(b.ravel()), x32)
+            res32 = true_residual(h.fine.A, dev(b.ravel()), x32)
+            runs[form] = (solver.iterations, res, host(x), fs.iterations, res32)
+            print(f"form {form}: FP64 {solver.iterations} its (res {res:.2e}), mixed {fs.iterations} its "
+                  f"(res {res32:.2e})")
+            # the reference's PCG stops on the preconditioned residual: the true one lands near 1e-9
+            assert solver.last_status == 0 and res < 1e-8
+            assert fs.last_status == 0 and res32 < 1e-9
+        finally:
+            S.set_option("elasticity_form", 0)
+    assert runs[1][0] == runs[2][0]
+    assert np.linalg.norm(runs[2][2] - runs[1][2]) / np.linalg.norm(runs[1][2]) < 1e-9
+    assert abs(runs[1][3] - runs[2][3]) <= 1
+
+
 @pytest.mark.parametrize("real_type", [S.FLOAT64, S.FLOAT32])
 def test_structured_extension_below_macro_mesh(real_type):
     """Hierarchy continued below the macro box (super-mesh levels): still converges, and the
@@ -447,3 +491,63 @@ def test_fused_chebyshev_steps_match_separate_kernels(L, rt, jacobi):
         S.set_option("fused_chebyshev", 0)
     np.testing.assert_array_equal(out[1], out[0])
     assert np.linalg.norm(out[1] - x0) > 0
+
+
+def test_flexible_cg_matches_oracle_with_fixed_jacobi_preconditioner():
+    """The product's flexible PCG (mode 1) against the oracle's flexible_pcg on a variable-
+    coefficient sshex8 Poisson problem with a fixed SPD (Jacobi) preconditioner: same iteration
+    count, the same residual history to rounding, the same solution."""
+    L = 4
+    mel, mpts, el, pts, fff, N, nodes, coeff = _ss_poisson(L, 3, coeff_seed=5)
+    E = mel.shape[1]
+    A_o = oracle_constrained(lambda x, y: O.lib.oracle_sshex8_laplacian_apply(L, E, el, fff, coeff, x, y), nodes)
+    d = np.zeros(N)
+    O.lib.oracle_sshex8_laplacian_diag(L, E, el, fff, coeff, d)
+    d[nodes] = 1
+
+    def jac(r, z):
+        z += r / d
+    b = np.random.default_rng(7).uniform(0, 1, N)
+    b[nodes] = 0
+    xo, it_o, st_o, hist_o = SO.flexible_pcg(A_o, b, np.zeros(N), jac, rtol=1e-10)
+    assert st_o == 0
+    A = S.constrained(S.laplacian(S.Mesh(S.SSHEX8, L, el, N), fff, coeff), nodes)
+    M = S.inverse_diagonal(dev(d))
+    solver = S.cg(A, M, flexible=True, rtol=1e-10)
+    x = zeros(N)
+    solver.apply(dev(b), x)
+    hist = solver.history()
+    print(f"flexible CG: gpu {solver.iterations} its, oracle {it_o} its")
+    assert solver.last_status == 0 and solver.iterations == it_o
+    m = min(len(hist), len(hist_o))
+    # relative residuals agree to 1e-7 of their value, or to 1e-13 absolute near the 1e-10 stop
+    np.testing.assert_allclose(hist[:m], hist_o[:m], rtol=1e-7, atol=1e-13)
+    assert np.linalg.norm(host(x) - xo) / np.linalg.norm(xo) < 1e-9
+    assert true_residual(A, dev(b), x) < 1e-10
+
+
+@pytest.mark.parametrize("omega,steps", [(1.0, 3), (2.0 / 3.0, 5)])
+def test_scalar_jacobi_stationary_iteration_matches_oracle(omega, steps):
+    """StationaryIteration with scaled Jacobi (the reference's Laplacian GMG smoother, omega = 1,
+    three sweeps) against oracle/solvers.Jacobi on the constrained sshex8 Poisson operator."""
+    L = 4
+    mel, mpts, el, pts, fff, N, nodes, coeff = _ss_poisson(L, 3, coeff_seed=6)
+    E = mel.shape[1]
+    A_o = oracle_constrained(lambda x, y: O.lib.oracle_sshex8_laplacian_apply(L, E, el, fff, coeff, x, y), nodes)
+    A = S.constrained(S.laplacian(S.Mesh(S.SSHEX8, L, el, N), fff, coeff), nodes)
+    dd = zeros(N)
+    A.diag(dd)
+    d = host(dd)
+    rng = np.random.default_rng(8)
+    b = rng.uniform(-1, 1, N)
+    b[nodes] = 0
+    x0 = rng.uniform(-1, 1, N)
+    x0[nodes] = 0
+    xo = SO.Jacobi(A_o, d, omega=omega, steps=steps)(b, x0.copy())
+    sm = S.jacobi(A, dd, omega=omega, steps=steps)
+    x = dev(x0)
+    sm.apply(dev(b), x)
+    S.synchronize()
+    err = np.linalg.norm(host(x) - xo) / np.linalg.norm(xo)
+    print(f"Jacobi omega={omega:.3f} steps={steps}: relative difference {err:.2e}")
+    assert err < 1e-12

@@ -136,3 +136,85 @@ def test_block_jacobi_on_block_diagonal_matrix_is_exact_with_omega_one():
     Bm = blocks[1].copy()
     Bm[1, :] = [0, 1, 0]
     np.testing.assert_allclose(bj.inv[1], np.linalg.inv(Bm), rtol=1e-12)
+
+
+def _textbook_pcg(A, b, precond, rtol, max_it):
+    """Fletcher-Reeves PCG (beta = r_new.z_new / r_old.z_old): ||r_k|| / ||b|| per iteration."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = np.zeros_like(b)
+    precond(r, z)
+    p = z.copy()
+    rtz = r @ z
+    hist = []
+    for k in range(max_it):
+        Ap = A @ p
+        alpha = rtz / (p @ Ap)
+        x += alpha * p
+        r -= alpha * Ap
+        hist.append(np.linalg.norm(r) / np.linalg.norm(b))
+        if hist[-1] < rtol:
+            return x, k + 1, hist
+        z = np.zeros_like(b)
+        precond(r, z)
+        rtz_new = r @ z
+        p = z + (rtz_new / rtz) * p
+        rtz = rtz_new
+    return x, max_it, hist
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_flexible_pcg_equals_pcg_for_fixed_spd_preconditioner(seed):
+    """With a fixed SPD preconditioner r_new.z_old = 0 in exact arithmetic: the flexible PCG's
+    iterates are standard PCG's (residual histories to rounding, same count, same solution)."""
+    n = 60
+    A = spd_with_eigs(np.linspace(1, 100, n), seed=seed)
+    b = np.random.default_rng(seed + 10).normal(size=n)
+    d = np.diag(A) * np.random.default_rng(seed + 20).uniform(0.5, 2, n)
+
+    def jac(r, z):
+        z += r / d
+    x, it, status, hist = SO.flexible_pcg(dense(A), b, np.zeros(n), jac, rtol=1e-10)
+    xt, itt, ht = _textbook_pcg(A, b, jac, 1e-10, 1000)
+    assert status == 0 and abs(it - itt) <= 1
+    m = min(25, len(hist), len(ht))
+    np.testing.assert_allclose(hist[:m], ht[:m], rtol=1e-8)
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-8)
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10  # the true residual is the stop rule
+
+
+def test_flexible_pcg_converges_under_a_varying_preconditioner():
+    """Jacobi with a fresh random scaling (1 +- 0.6) on every application is not a fixed map:
+    the flexible beta still converges to the true tolerance, while the fixed-preconditioner beta
+    (r_new.z_new / r_old.z_old, the same algorithm without the r_new.z_old term) stagnates."""
+    n = 60
+    A = spd_with_eigs(np.linspace(1, 100, n), seed=0)
+    b = np.random.default_rng(10).normal(size=n)
+    d = np.diag(A) * np.random.default_rng(20).uniform(0.5, 2, n)
+
+    def varying(seed):
+        rng = np.random.default_rng(seed)
+
+        def P(r, z):
+            z += r / d * (1 + 0.6 * rng.uniform(-1, 1, n))
+        return P
+    x, it, status, _ = SO.flexible_pcg(dense(A), b, np.zeros(n), varying(30), rtol=1e-10, max_it=1000)
+    assert status == 0 and it < 400
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10
+    _, itt, _ = _textbook_pcg(A, b, varying(30), 1e-10, 1000)
+    assert itt == 1000  # without the flexible term: no convergence within 1000 iterations
+
+
+def test_flexible_pcg_exact_preconditioner_and_residual_replacement():
+    A = spd_with_eigs(np.linspace(1, 100, 30), seed=4)
+    b = np.random.default_rng(5).normal(size=30)
+    x, it, status, _ = SO.flexible_pcg(dense(A), b, np.zeros(30), dense(np.linalg.inv(A)), rtol=1e-10)
+    assert status == 0 and it == 1
+    d = np.diag(A)
+
+    def jac(r, z):
+        z += r / d
+    x1, _, s1, _ = SO.flexible_pcg(dense(A), b, np.zeros(30), jac, rtol=1e-12)
+    x2, _, s2, _ = SO.flexible_pcg(dense(A), b, np.zeros(30), jac, rtol=1e-12, residual_replacement=3)
+    assert s1 == 0 and s2 == 0
+    np.testing.assert_allclose(x2, x1, rtol=1e-9)
