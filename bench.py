@@ -1,13 +1,16 @@
 #!/usr/bin/env python3
-"""Benchmark: sshex8 Laplacian L=8 on a 16^3 macro hex8 unit cube (2,146,689 DOF).
+"""Benchmark: sshex8 linear elasticity, level 16 on a 24^3 macro hex8 unit cube (171,199,875 DOF,
+57,066,625 nodes; Young's modulus log-uniform in [0.1, 10] per macro element, nu = 0.3).
 
-One step = one FP64 matrix-free apply y += A x (x ~ U[-1,1], seed 0) on one MI355X.
-Prints ONE JSON line: throughput in DOF/s, the apply kernel's roofline (algorithmic bytes
-over its HIP-event-timed average duration), the time to relative residual 1e-10 of plain
-FP64 MG-PCG and of FP32/BF16 MG inside FP64 flexible CG (zero Dirichlet on all faces,
-RHS ~ U[0,1]), and the CPU baseline: the oracle's OpenMP apply on the host cores.
+One step = one FP64 matrix-free apply y += A x (x ~ U[-1,1], seed 0) on one MI355X.  Prints ONE JSON
+line: throughput in DOF/s, the apply's roofline (algorithmic bytes over its HIP-event-timed average
+duration), the FP32 apply beside it, time to true relative residual 1e-10 of FP64 MG-PCG and of the
+FP32/BF16 V-cycle in FP64 flexible CG on the same workload, the same for the ~50M DOF elasticity
+problem (L=8 on 32^3) and for the sshex8 Laplacian L=8 on 16^3 (its apply as well), and the CPU
+baseline: the oracle's OpenMP restatement of the reference's loop, built on this host with
+-O3 -fopenmp -march=native and timed on a bounded slab of the workload.
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-solve] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-solve] [--no-cpu] [--no-extra]
 """
 import argparse
 import json
@@ -38,15 +41,16 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-PROFILE = os.path.join(ROOT, "profiles", "sshex8_laplacian_apply_L8_16cube.md")
+PROFILE_LAPLACIAN = os.path.join(ROOT, "profiles", "sshex8_laplacian_apply_L8_16cube.md")
+PROFILE_ELASTICITY = os.path.join(ROOT, "profiles", "sshex8_elasticity_apply_L16_24cube.md")
 
 
-def profiled_traffic():
-    """HBM bytes per apply of this workload measured by rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE,
+def profiled_traffic(path):
+    """HBM bytes per apply of a workload measured by rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE,
     calibrated with a copy kernel of known size), read from the committed profile summary that
-    states the command; None without it."""
+    states the commands; None without it."""
     try:
-        with open(PROFILE) as f:
+        with open(path) as f:
             m = re.search(r"Total per apply: ([0-9.]+) MB", f.read())
         return float(m.group(1)) * 1e6 if m else None
     except OSError:
@@ -58,25 +62,37 @@ def algorithmic_bytes(N, E, L, value_bytes=8, geom_bytes=4):
     return 2 * value_bytes * N + 4 * (L + 1) ** 3 * E + geom_bytes * 6 * E
 
 
-def cpu_baseline(el, fff, N, L, budget_s=12.0, max_reps=20):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as O
-    E = el.shape[1]
-    x = np.random.default_rng(0).uniform(-1, 1, N)
-    y = np.zeros(N)
-    el = np.ascontiguousarray(el)
-    for _ in range(2):
-        O.lib.oracle_sshex8_laplacian_apply(L, E, el, fff, None, x, y)
-    times = []
-    t_start = time.perf_counter()
-    while len(times) < max_reps and (len(times) < 5 or time.perf_counter() - t_start < budget_s):
-        t0 = time.perf_counter()
-        O.lib.oracle_sshex8_laplacian_apply(L, E, el, fff, None, x, y)
-        times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
-    return {"value": N / med, "unit": "DOF/s", "cores": int(O.lib.oracle_num_threads()), "kind": "port",
-            "sample": f"full apply of this workload ({N} DOF), median of {len(times)} after 2 warm-up; "
-                      f"OpenMP restatement of affine_sshex8_laplacian_stencil_apply_fff"}
+def elasticity_algorithmic_bytes(N, E, L, value_bytes=8, geom_bytes=4):
+    # x read + y written, int32 connectivity, 9 adjugate + 1 determinant words and 1 coefficient per
+    # macro element (config 5: 2,739,198,000 + 271,669,248 + 552,960 + 55,296 = 3,011,475,504 B)
+    return 2 * value_bytes * N + 4 * (L + 1) ** 3 * E + geom_bytes * 10 * E + 4 * E
+
+
+def cpu_baseline(workload, size, budget_s):
+    """The oracle's OpenMP restatement of the reference's CPU loop (oracle/cpu_bench.c), compiled on
+    this host with the reference's release flags (-O3 -fopenmp -march=native) and run in a child
+    process with OMP_PROC_BIND=close, OMP_PLACES=cores and this process's OMP_NUM_THREADS (the
+    host's CPU share for one GPU, else all cores)."""
+    import subprocess
+    import tempfile
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    src = [os.path.join(ROOT, "oracle", "cpu_bench.c"), os.path.join(ROOT, "oracle", "sfem_oracle.c")]
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "cpu_bench")
+        flags = ["-O3", "-march=native", "-fopenmp", "-std=gnu11"]
+        r = subprocess.run(["gcc", *flags, "-I", os.path.join(ROOT, "oracle"), *src, "-lm", "-o", exe],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            return {"error": "gcc failed: " + r.stderr[-400:]}
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        r = subprocess.run([exe, workload, str(size), str(budget_s)], capture_output=True, text=True, env=env,
+                           timeout=600)
+        if r.returncode != 0:
+            return {"error": "cpu_bench failed: " + r.stderr[-400:]}
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["build"] = "gcc " + " ".join(flags)
+    out["omp"] = f"OMP_NUM_THREADS={threads} OMP_PROC_BIND=close OMP_PLACES=cores"
+    return out
 
 
 def solve_times(S, mesh, fff, pts, n_macro, rtol=1e-10, learned_steps=1):
@@ -258,62 +274,102 @@ def elasticity_solve_times(S, n_macro=32, L=8, rtol=1e-10, with_learned=False, c
     return out
 
 
-def elasticity_apply_times(S, n_macro, L, steps, coeff_seed=None):
-    """Matrix-free sshex8 elasticity apply (y = A x, FP64 and the FP32/BF16 matrix-core operator),
-    HIP-event timed over `steps` applies: DOF/s and algorithmic GB/s, B = 2 s_v N + 4 nxe E + 4 * 10 E
-    (x read, y written, int32 connectivity, adjugate + det; BF16 geometry: 2 bytes per word)."""
+def config5_apply(S, steps, warmup):
+    """The headline workload: FP64 (and, beside it, FP32) sshex8 elasticity apply, L=16 on 24^3."""
+    n_macro, L = 24, 16
+    progress("config 5: refining the 24^3 macro cube to level 16 on the device")
     mel, mpts = S.box_mesh(n_macro + 1, n_macro + 1, n_macro + 1, element_type=S.HEX8)
     mesh, pts_d = S.sshex8_refine_device(L, mel, mpts)
     adj, det = S.mesh_adjugate_det(mesh, pts_d)
     del pts_d
     N, E = 3 * mesh.nnodes, n_macro ** 3
-    coeff = None
-    if coeff_seed is not None:
-        coeff = (10.0 ** np.random.default_rng(coeff_seed).uniform(-1, 1, E)).astype(np.float32)
+    coeff = (10.0 ** np.random.default_rng(0).uniform(-1, 1, E)).astype(np.float32)
     nu = 0.3
     mu, lam = 1.0 / (2 * (1 + nu)), nu / ((1 + nu) * (1 - 2 * nu))
-    out = {"workload": f"sshex8 linear elasticity apply, L={L} on a {n_macro}^3 macro cube, {N} DOF", "dofs": N,
-           "macro_elements": E, "steps": steps}
-    for name, rt, dt, vb, gb in (("fp64", S.FLOAT64, np.float64, 8, 4), ("fp32", S.FLOAT32, np.float32, 4, 2)):
-        op = S.linear_elasticity(mesh, adj, det, mu, lam, coeff, real_type=rt)
-        x = S.DeviceArray.from_numpy(np.random.default_rng(0).uniform(-1, 1, N).astype(dt), dt)
-        y = S.DeviceArray(N, dt)
-        ms = S.benchmark(op, x, y, warmup=2, reps=steps) / steps
-        B = 2 * vb * N + 4 * (L + 1) ** 3 * E + gb * 10 * E
-        out[name] = {"ms_per_apply": ms, "dof_per_s": N / (ms * 1e-3), "algorithmic_bytes": B,
-                     "GB_per_s": B / (ms * 1e-3) / 1e9, "hbm_frac": B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
-        del op, x, y
+    op = S.linear_elasticity(mesh, adj, det, mu, lam, coeff)
+    x = S.DeviceArray.from_numpy(np.random.default_rng(0).uniform(-1, 1, N))
+    y = S.DeviceArray(N)
+    progress(f"config 5 apply benchmark: {steps} steps")
+    total_ms = S.benchmark(op, x, y, warmup=warmup, reps=steps)
+    out = {"N": N, "E": E, "L": L, "nodes": mesh.nnodes, "total_ms": total_ms}
+    del op, x, y
+    op32 = S.linear_elasticity(mesh, adj, det, mu, lam, coeff, real_type=S.FLOAT32)
+    x32 = S.DeviceArray.from_numpy(np.random.default_rng(0).uniform(-1, 1, N).astype(np.float32), np.float32)
+    y32 = S.DeviceArray(N, np.float32)
+    ms32 = S.benchmark(op32, x32, y32, warmup=warmup, reps=steps) / steps
+    B32 = elasticity_algorithmic_bytes(N, E, L, value_bytes=4, geom_bytes=2)
+    out["fp32"] = {"ms_per_apply": ms32, "dof_per_s": N / (ms32 * 1e-3), "algorithmic_bytes": B32,
+                   "GB_per_s": B32 / (ms32 * 1e-3) / 1e9,
+                   "hbm_frac": B32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                   "form": "FP32 vectors, BF16 geometry, element form on the matrix cores (bf16x3 products, "
+                           "FP32 accumulation)"}
     return out
 
 
-def elasticity_cases(S, args):
-    out = {}
-    if args.elasticity_case in ("l8", "both"):
-        out["solve_elasticity"] = elasticity_solve_times(S, with_learned=args.elasticity_learned)
-    if args.elasticity_case in ("l16", "both"):
-        out["solve_elasticity_l16"] = elasticity_solve_times(S, n_macro=24, L=16, coeff_seed=0,
-                                                             with_learned=args.elasticity_learned)
+def laplacian_config2(S, args):
+    """sshex8 Laplacian L=8 on a 16^3 macro cube (2,146,689 DOF): apply (hot and cold caches, and the
+    atomic skeleton option) and the MG-PCG solves."""
+    L, n_macro = 8, 16
+    mel, mpts, el, pts, fff = build_problem(n_macro, L)
+    N, E = pts.shape[1], el.shape[1]
+    mesh = S.Mesh(S.SSHEX8, L, el, N)
+    op = S.laplacian(mesh, fff)
+    x = S.DeviceArray.from_numpy(np.random.default_rng(0).uniform(-1, 1, N))
+    y = S.DeviceArray(N)
+    progress("laplacian config 2 apply")
+    steps = 200
+    ms = S.benchmark(op, x, y, warmup=10, reps=steps) / steps
+    B = algorithmic_bytes(N, E, L)
+    ach = B / (ms * 1e-3) / 1e9
+    out = {"workload": f"sshex8 Laplacian apply, level 8 on a 16^3 macro hex8 unit cube ({N} DOF), FP64",
+           "value": N / (ms * 1e-3), "unit": "DOF/s", "ms_per_apply": ms,
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": profiled_traffic(PROFILE_LAPLACIAN),
+                        "algorithmic_bytes_per_apply": B,
+                        "note": "working set 46 MB fits the 256 MB MALL: back-to-back applies are cache-hot"}}
+    if not args.no_flush:
+        cold = S.benchmark(op, x, y, warmup=1, reps=20, flush_bytes=1 << 30) / 20
+        a2 = B / (cold * 1e-3) / 1e9
+        out["roofline_cold"] = {"achieved": a2, "frac": a2 / HBM_PEAK_GBS, "ms_per_apply": cold,
+                                "note": "1 GiB written between applies (outside the timed events)"}
+    S.set_option("atomic_skeleton", 1)
+    try:
+        ta = S.benchmark(op, x, y, warmup=10, reps=steps) / steps
+    finally:
+        S.set_option("atomic_skeleton", 0)
+    out["apply_atomic_skeleton"] = {"ms_per_apply": ta, "value": N / (ta * 1e-3),
+                                    "note": "FP64 atomics for the macro-element boundary sums (not reproducible)"}
+    if not args.no_solve:
+        out["solve"] = solve_times(S, mesh, fff, pts, n_macro)
+    if not args.no_cpu:
+        progress("cpu baseline (laplacian)")
+        cb = cpu_baseline("laplacian", n_macro, 8.0)
+        if "error" not in cb:
+            out["cpu_baseline"] = {"value": cb["dof_per_s"], "unit": "DOF/s", "cores": cb["threads"], "kind": "port",
+                                   "sample": "full apply of this workload, median of %d after 2 warm-up: the oracle's "
+                                             "restatement of affine_sshex8_laplacian_stencil_apply_fff (stencil + "
+                                             "surface stencil, atomic scatter)" % cb["reps"], **cb}
+        else:
+            out["cpu_baseline"] = cb
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-flush", action="store_true")
-    ap.add_argument("--no-elasticity", action="store_true", help="skip the ~51M DOF elasticity solve")
-    ap.add_argument("--only-elasticity", action="store_true", help="run only the elasticity solve section")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the headline apply (and its solves unless --no-solve): skip the ~50M DOF "
+                         "elasticity solve and the Laplacian workload")
     ap.add_argument("--elasticity-learned", action="store_true",
-                    help="also time the learned smoother in the elasticity solve")
-    ap.add_argument("--elasticity-case", choices=["l8", "l16", "both"], default="both",
-                    help="l8: L=8 on 32^3 macro elements, E=1 (50.9M DOF); l16: L=16 on 24^3 macro elements, "
-                         "E log-uniform per macro element (171.2M DOF)")
+                    help="also time the learned smoother in the elasticity solves")
     ap.add_argument("--mode", choices=["apply", "copy", "elasticity-apply"], default="apply",
                     help="copy: stream-copy 2 GiB of FP64 per step (HBM counter calibration for rocprofv3 --pmc); "
-                         "elasticity-apply: time the sshex8 elasticity operators of --elasticity-case")
+                         "elasticity-apply: only the headline apply (a short run for the profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (this build runs on a single "
@@ -328,42 +384,25 @@ def main():
         n = (2 << 30) // 8  # 2 GiB per array: far beyond the 256 MB MALL
         a = S.DeviceArray.from_numpy(np.ones(n))
         b = S.DeviceArray(n)
-        import time as _t
         S.synchronize()
-        t0 = _t.perf_counter()
+        t0 = time.perf_counter()
         for _ in range(args.steps):
             S.lib.sfem_amd_blas_copy(S.FLOAT64, n, S._dptr(a), S._dptr(b), None)
         S.synchronize()
-        dt = (_t.perf_counter() - t0) / args.steps
+        dt = (time.perf_counter() - t0) / args.steps
         print(json.dumps({"mode": "copy", "bytes_per_step": 16 * n, "GB/s": 16 * n / dt / 1e9}))
         return
-    if args.mode == "elasticity-apply":
-        out = {}
-        if args.elasticity_case in ("l8", "both"):
-            out["elasticity_apply_l8"] = elasticity_apply_times(S, 32, 8, args.steps)
-        if args.elasticity_case in ("l16", "both"):
-            out["elasticity_apply_l16"] = elasticity_apply_times(S, 24, 16, args.steps, coeff_seed=0)
-        print(json.dumps(out))
-        return
-    if args.only_elasticity:
-        print(json.dumps(elasticity_cases(S, args)))
-        return
-    L, n_macro = 8, 16
-    mel, mpts, el, pts, fff = build_problem(n_macro, L)
-    N, E = pts.shape[1], el.shape[1]
-    mesh = S.Mesh(S.SSHEX8, L, el, N)
-    op = S.laplacian(mesh, fff)
-    x = S.DeviceArray.from_numpy(np.random.default_rng(0).uniform(-1, 1, N))
-    y = S.DeviceArray(N)
 
-    progress(f"apply benchmark: {args.steps} steps")
-    total_ms = S.benchmark(op, x, y, warmup=args.warmup, reps=args.steps)
-    ms = total_ms / args.steps
-    B = algorithmic_bytes(N, E, L)
+    if args.mode == "elasticity-apply":
+        args.no_solve = args.no_extra = args.no_cpu = True
+    h = config5_apply(S, args.steps, args.warmup)
+    N, E, L = h["N"], h["E"], h["L"]
+    ms = h["total_ms"] / args.steps
+    B = elasticity_algorithmic_bytes(N, E, L)
     achieved = B / (ms * 1e-3) / 1e9
     result = {
         "metric": METRIC,
-        "value": N * args.steps / (total_ms * 1e-3),
+        "value": N * args.steps / (h["total_ms"] * 1e-3),
         "unit": "DOF/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -374,41 +413,44 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "sshex8 Laplacian apply, level 8 on a 16^3 macro hex8 unit cube "
-                               f"({N} DOF, {E} macro elements), FP64 vectors, float32 FFF, int32 ids",
-                   "dofs": N, "macro_elements": E, "level": L},
+        "config": {"workload": "sshex8 linear elasticity apply y += A x, level 16 on a 24^3 macro hex8 unit cube "
+                               f"({N} DOF, {h['nodes']} nodes, {E} macro elements), Young's modulus log-uniform in "
+                               "[0.1, 10] per macro element (seed 0), nu 0.3; FP64 vectors, float32 adjugate and "
+                               "determinant, int32 ids",
+                   "dofs": N, "nodes": h["nodes"], "macro_elements": E, "level": L,
+                   "form": "stencil form (27-point 3x3-block stencil per macro element, DPP-broadcast coefficients) "
+                           "+ fixed-order skeleton gather"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
-                     "traffic_source": "profiles/sshex8_laplacian_apply_L8_16cube.md (rocprofv3 --pmc FETCH_SIZE, "
-                                       "WRITE_SIZE per dispatch, both kernels of one apply)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(PROFILE_ELASTICITY),
+                     "traffic_source": "profiles/sshex8_elasticity_apply_L16_24cube.md (rocprofv3 --pmc FETCH_SIZE, "
+                                       "WRITE_SIZE per dispatch, both kernels of one apply, copy-calibrated)",
                      "algorithmic_bytes_per_apply": B,
-                     "note": "working set 46 MB fits the 256 MB MALL: back-to-back applies are cache-hot; "
-                             "see roofline_cold for caches overwritten before each apply"},
+                     "note": "achieved over the whole apply (stencil kernel + skeleton gather), HIP events on the "
+                             "launch stream; the 3.0 GB working set exceeds the 256 MB MALL"},
+        "apply_fp32": h["fp32"],
     }
-    if not args.no_flush:
-        reps = 20
-        cold = S.benchmark(op, x, y, warmup=1, reps=reps, flush_bytes=1 << 30) / reps
-        ach = B / (cold * 1e-3) / 1e9
-        result["roofline_cold"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                   "frac": ach / HBM_PEAK_GBS, "ms_per_apply": cold,
-                                   "note": "1 GiB written between applies (outside the timed events)"}
-    S.set_option("atomic_skeleton", 1)
-    try:
-        ta = S.benchmark(op, x, y, warmup=args.warmup, reps=args.steps) / args.steps
-    finally:
-        S.set_option("atomic_skeleton", 0)
-    ach = B / (ta * 1e-3) / 1e9
-    result["apply_atomic_skeleton"] = {
-        "ms_per_apply": ta, "value": N / (ta * 1e-3), "unit": "DOF/s", "achieved": ach, "frac": ach / HBM_PEAK_GBS,
-        "note": "option atomic_skeleton: macro-element boundary sums with FP64 atomics inside the line-sweep "
-                "kernel (one launch, no partials); not bitwise reproducible, so not the headline value"}
+    # the small Laplacian workload first: its cache-resident apply is sensitive to clocks and
+    # memory state after the large solves
+    if not args.no_extra:
+        result["laplacian_config2"] = laplacian_config2(S, args)
     if not args.no_solve:
-        result["solve"] = solve_times(S, mesh, fff, pts, n_macro)
-        if not args.no_elasticity:
-            result.update(elasticity_cases(S, args))
+        result["solve_config5"] = elasticity_solve_times(S, n_macro=24, L=16, coeff_seed=0,
+                                                         with_learned=args.elasticity_learned)
+        if not args.no_extra:
+            result["solve_elasticity_l8"] = elasticity_solve_times(S, with_learned=args.elasticity_learned)
     if not args.no_cpu:
-        progress("cpu baseline")
-        result["cpu_baseline"] = cpu_baseline(el, fff, N, L)
+        progress("cpu baseline (elasticity slab)")
+        cb = cpu_baseline("elasticity", 2, 10.0)
+        if "error" in cb:
+            result["cpu_baseline"] = {"value": None, "unit": "DOF/s", "cores": None, "kind": "port", **cb}
+        else:
+            result["cpu_baseline"] = {
+                "value": cb["dof_per_s"], "unit": "DOF/s", "cores": cb["threads"], "kind": "port",
+                "sample": "full applies of a 24 x 24 x 2 slab of the 24^3 macro cube at level 16 (%d DOF, the same "
+                          "per-element work), median of %d after 2 warm-up; the oracle's restatement of "
+                          "affine_sshex8_linear_elasticity_apply_macro_adjugate (per macro element: gather, 24x24 "
+                          "matrix per micro element, atomic scatter)" % (cb["dofs"], cb["reps"]),
+                **{k: cb[k] for k in ("cpu_model", "stream_triad_GBs", "build", "omp", "median_s", "dofs")}}
     print(json.dumps(result))
 
 

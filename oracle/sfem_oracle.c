@@ -885,6 +885,83 @@ void oracle_sshex8_laplacian_apply(int L, ptrdiff_t ne, const int32_t *el, const
     }
 }
 
+/* The reference's timed sshex8 Laplacian path, affine_sshex8_laplacian_stencil_apply_fff
+ * (operators/sshex8/sshex8_laplacian.cpp:698-769): per macro element gather its (L+1)^3 values, build
+ * the 8x8 sub-element matrix (hex8_sub_fff_0 + hex8_laplacian_matrix_fff_integral), turn it into the
+ * 27-point stencil (hex8_matrix_to_stencil, operators/stencil/stencil3.hpp:205-233: the stencil
+ * entry of offset d sums A[c][c + d] over the corners c), apply that stencil to the (L-1)^3 interior
+ * nodes (sshex8_stencil, :167-184, slice by slice), and give the nodes on the element's faces, edges
+ * and corners the contributions of the micro elements present (sshex8_surface_stencil,
+ * operators/stencil/sshex8_skeleton_stencil.hpp:1683-: one generated routine per boundary region,
+ * restated here as the octant sum of A's row for the node's corner), then scatter with atomics. */
+void oracle_sshex8_laplacian_stencil_apply(int L, ptrdiff_t ne, const int32_t *el, const float *fff, const float *coeff,
+                                           const double *x, double *y) {
+    const int N1 = L + 1, nxe = oracle_sshex8_nxe(L);
+#pragma omp parallel
+    {
+        double *eu = (double *)malloc(sizeof(double) * nxe);
+        double *ev = (double *)malloc(sizeof(double) * nxe);
+#pragma omp for
+        for (ptrdiff_t e = 0; e < ne; e++) {
+            const double h = 1.0 / L, c = coef(coeff, e);
+            double f[6], K[64], st[27];
+            for (int k = 0; k < 6; k++) f[k] = (double)fff[e * 6 + k] * h * c;
+            oracle_hex8_laplacian_matrix(f, K);
+            for (int k = 0; k < 27; k++) st[k] = 0;
+            for (int a = 0; a < 8; a++)
+                for (int b = 0; b < 8; b++) {
+                    const int dx = hex8_cx[b] - hex8_cx[a], dy = hex8_cy[b] - hex8_cy[a],
+                              dz = hex8_cz[b] - hex8_cz[a];
+                    st[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1] += K[a * 8 + b];
+                }
+            for (int d = 0; d < nxe; d++) {
+                eu[d] = x[el[(size_t)d * ne + e]];
+                ev[d] = 0;
+            }
+            /* interior: the 27-point stencil, slice by slice */
+            for (int z = 1; z < L; z++)
+                for (int yy = 1; yy < L; yy++)
+                    for (int xx = 1; xx < L; xx++) {
+                        double acc = 0;
+                        for (int dz = -1; dz <= 1; dz++)
+                            for (int dy = -1; dy <= 1; dy++)
+                                for (int dx = -1; dx <= 1; dx++)
+                                    acc += st[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1] *
+                                           eu[((z + dz) * N1 + yy + dy) * N1 + xx + dx];
+                        ev[(z * N1 + yy) * N1 + xx] += acc;
+                    }
+            /* faces, edges, corners: the micro elements present around the node */
+            for (int z = 0; z <= L; z++)
+                for (int yy = 0; yy <= L; yy++)
+                    for (int xx = 0; xx <= L; xx++) {
+                        if (xx > 0 && xx < L && yy > 0 && yy < L && z > 0 && z < L) continue;
+                        double acc = 0;
+                        for (int a = 0; a < 8; a++) {
+                            /* micro element whose corner a is this node */
+                            const int x0 = xx - hex8_cx[a], y0 = yy - hex8_cy[a], z0 = z - hex8_cz[a];
+                            if (x0 < 0 || x0 >= L || y0 < 0 || y0 >= L || z0 < 0 || z0 >= L) continue;
+                            for (int b = 0; b < 8; b++)
+                                acc += K[a * 8 + b] *
+                                       eu[((z0 + hex8_cz[b]) * N1 + y0 + hex8_cy[b]) * N1 + x0 + hex8_cx[b]];
+                        }
+                        ev[(z * N1 + yy) * N1 + xx] += acc;
+                    }
+            for (int d = 0; d < nxe; d++) {
+#pragma omp atomic update
+                y[el[(size_t)d * ne + e]] += ev[d];
+            }
+        }
+        free(eu);
+        free(ev);
+    }
+}
+
+/* STREAM triad a = b + s c over n doubles (host bandwidth of the CPU baseline's machine). */
+void oracle_stream_triad(ptrdiff_t n, double *a, const double *b, const double *c, double s) {
+#pragma omp parallel for schedule(static)
+    for (ptrdiff_t i = 0; i < n; i++) a[i] = b[i] + s * c[i];
+}
+
 /* Stored sub-element matrices applied to every micro element ("ss:em"):
  * operators/stencil/sshex8_stencil_element_matrix_apply.cpp:10-78 (bs 1, [E][64]) and :80-160
  * (bs 3, [E][576], index component*8 + node; vectors interleaved). */

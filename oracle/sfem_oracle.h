@@ -109,6 +109,11 @@ void oracle_sshex8_laplacian_apply(int L, ptrdiff_t nelements, const int32_t *el
                                    const float *coeff, const double *x, double *y);
 /* "ss:em": stored sub-element matrices applied to every micro element (bs 1: [E][64]; bs 3:
  * [E][576], vectors interleaved).  operators/stencil/sshex8_stencil_element_matrix_apply.cpp:10-160 */
+/* the reference's stencil + surface-stencil evaluation of the same operator (the timed CPU path) */
+void oracle_sshex8_laplacian_stencil_apply(int L, ptrdiff_t nelements, const int32_t *elements, const float *fff,
+                                           const float *coeff, const double *x, double *y);
+/* STREAM triad a = b + s c (OpenMP) */
+void oracle_stream_triad(ptrdiff_t n, double *a, const double *b, const double *c, double s);
 void oracle_sshex8_em_apply(int L, ptrdiff_t nelements, const int32_t *elements, int bs, const double *matrices,
                             const double *x, double *y);
 /* operators/sshex8/sshex8_laplacian.cpp:595-634 (from the macro FFF words) */

@@ -66,6 +66,7 @@ def _load():
         "oracle_hex8_laplacian_apply": (None, [_pd, _i32p, _f32p, _NullableF32, _f64p, _f64p]),
         "oracle_hex8_laplacian_diag": (None, [_pd, _i32p, _f32p, _NullableF32, _f64p]),
         "oracle_sshex8_laplacian_apply": (None, [C.c_int, _pd, _i32p, _f32p, _NullableF32, _f64p, _f64p]),
+        "oracle_sshex8_laplacian_stencil_apply": (None, [C.c_int, _pd, _i32p, _f32p, _NullableF32, _f64p, _f64p]),
         "oracle_sshex8_laplacian_diag": (None, [C.c_int, _pd, _i32p, _f32p, _NullableF32, _f64p]),
         "oracle_sshex8_em_apply": (None, [C.c_int, _pd, _i32p, C.c_int, _f64p, _f64p, _f64p]),
         "oracle_sshex8_laplacian_element_matrices": (None, [C.c_int, _pd, _f32p, _NullableF32, _f64p]),

@@ -457,3 +457,28 @@ def test_em_apply_with_geometry_matrices_equals_operators():
     O.lib.oracle_sshex8_em_apply(L, E, el, 3, K3, x3, y1)
     O.lib.oracle_sshex8_elasticity_apply(L, E, el, adj, det, mu, lam, coeff, x3, y2)
     np.testing.assert_allclose(y1, y2, rtol=1e-12, atol=1e-13 * np.abs(y2).max())
+
+
+@pytest.mark.parametrize("L", [2, 3, 8])
+def test_sshex8_laplacian_stencil_path_equals_element_path(L):
+    """The stencil + surface-stencil evaluation (the reference's timed path,
+    affine_sshex8_laplacian_stencil_apply_fff) gives the element-by-element operator; with unit
+    FFF on [0,L]^3 it reproduces the -6 known answer of operators/stencil/tests/sfem_StencilTest.cpp."""
+    mel, mpts = _cube_problem("hex8", n=2)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    fff = O.hex8_fff(mel, mpts)
+    coeff = RNG.uniform(0.5, 2.0, mel.shape[1]).astype(np.float32)
+    x = RNG.uniform(-1, 1, pts.shape[1])
+    y1, y2 = np.zeros_like(x), np.full_like(x, 0.25)
+    O.lib.oracle_sshex8_laplacian_apply(L, mel.shape[1], el, fff, coeff, x, y1)
+    O.lib.oracle_sshex8_laplacian_stencil_apply(L, mel.shape[1], el, fff, coeff, x, y2)
+    np.testing.assert_allclose(y2 - 0.25, y1, rtol=1e-12, atol=1e-13 * np.abs(y1).max())
+    pts1 = np.array([[L * a, L * b, L * c] for (a, b, c) in O.HEX8_CORNERS]).T.astype(np.float32)
+    el1, fp1 = O.sshex8_refine(L, np.arange(8, dtype=np.int32).reshape(8, 1), pts1)
+    u = (fp1.astype(np.float64) ** 2).sum(0)
+    y = np.zeros_like(u)
+    O.lib.oracle_sshex8_laplacian_stencil_apply(L, 1, el1, O.hex8_fff(np.arange(8, dtype=np.int32).reshape(8, 1), pts1),
+                                                None, u, y)
+    interior = np.all((fp1 > 0.5) & (fp1 < L - 0.5), axis=0)
+    if interior.any():
+        np.testing.assert_allclose(y[interior], -6.0, rtol=0, atol=1e-10)
