@@ -161,6 +161,10 @@ extern bool g_atomic_skeleton;
 // element form (matrix-core / node-wise kernels) everywhere; 2 stencil form wherever supported
 // (option "elasticity_form")
 extern int g_elasticity_form;
+// Host-side state that a captured HIP graph depends on (library options, switch selections, CNN
+// weights) changes the state epoch; graph-replaying operators re-capture when it moved.
+uint64_t state_epoch();
+void bump_state_epoch();
 // Zeroes / sets constrained dofs; defined in runtime.cpp.
 int constrained_set_value(const OpBase *constrained, int rt, double v, void *x, hipStream_t s);
 int mesh_counts(const sfem_amd_mesh *m, hipStream_t s);

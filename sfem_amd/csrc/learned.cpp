@@ -141,6 +141,7 @@ int sfem_amd_cnn_set_weights(sfem_amd_cnn *c, const float *w) {
     SFEM_AMD_REQUIRE(c && w, "cnn_set_weights: arguments");
     SFEM_AMD_HIP_CHECK(hipMemcpy(c->W.p, w, sizeof(float) * c->nparams, hipMemcpyHostToDevice));
     c->w16_valid = false;
+    bump_state_epoch();
     return SFEM_SUCCESS;
 }
 
@@ -157,7 +158,10 @@ int sfem_amd_cnn_forward(sfem_amd_cnn *c, int L, ptrdiff_t E, const float *z, fl
             SFEM_AMD_HIP_CHECK(hipMemcpy(h.data(), c->W.p, sizeof(float) * c->nparams, hipMemcpyDeviceToHost));
             std::vector<uint16_t> hb(kCnnPackedWeights);
             cnn_pack_weights(h.data(), hb.data());
-            if (c->W16.upload(hb.data(), hb.size())) return SFEM_FAILURE;
+            // allocated once: a graph captured earlier keeps a valid pointer (it re-captures anyway,
+            // since weight changes move the state epoch)
+            if (c->W16.n != hb.size() && c->W16.alloc(hb.size())) return SFEM_FAILURE;
+            SFEM_AMD_HIP_CHECK(hipMemcpy(c->W16.p, hb.data(), sizeof(uint16_t) * hb.size(), hipMemcpyHostToDevice));
             c->w16_valid = true;
         }
         return cnn_infer_mfma(L, E, c->W16.p, z, out, s);
@@ -207,6 +211,7 @@ int sfem_amd_cnn_adam_step(sfem_amd_cnn *c, double lr, void *stream) {
     SFEM_AMD_REQUIRE(c, "cnn_adam_step: null");
     c->step++;
     c->w16_valid = false;
+    bump_state_epoch();
     return cnn_adam((int)c->nparams, c->grad.p, c->W.p, c->m.p, c->v.p, (float)lr, 0.9f, 0.999f, 1e-8f, c->step,
                     as_stream(stream));
 }

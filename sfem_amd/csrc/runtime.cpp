@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -151,6 +152,10 @@ static int mesh_owner(const sfem_amd_mesh *m, hipStream_t s) {
     if (m->own.alloc((size_t)m->E * m->nv)) return SFEM_FAILURE;
     return node_owner_mask(m->E, m->nv, m->nnodes, m->idx.p, m->own.p, s);
 }
+static std::atomic<uint64_t> g_state_epoch{0};
+uint64_t state_epoch() { return g_state_epoch.load(std::memory_order_acquire); }
+void bump_state_epoch() { g_state_epoch.fetch_add(1, std::memory_order_acq_rel); }
+
 bool g_fused_chebyshev = false;
 bool g_atomic_skeleton = false;
 int g_elasticity_form = 0;
@@ -781,6 +786,7 @@ struct GraphOp final : OpBase {
         void *cy{nullptr};
         hipStream_t cs{nullptr};
         int calls{0};
+        uint64_t epoch{0};  // the state epoch of the capture
     } cap[2];  // [mode]: general / zero initial guess
     int replays{0};
     bool failed{false};
@@ -800,6 +806,14 @@ struct GraphOp final : OpBase {
     }
     int apply(const void *x, void *y, hipStream_t s) override {
         Capture &c = cap[zero_guess ? 1 : 0];
+        const uint64_t now = state_epoch();
+        if (c.exec && c.epoch != now) {
+            // options, switches or weights changed since the capture: drop it, run the next call
+            // eagerly (lazy set-up such as a weight repack happens outside a capture), then capture
+            (void)hipGraphExecDestroy(c.exec);
+            c.exec = nullptr;
+            c.calls = 0;
+        }
         if (c.calls++ == 0 || failed) return inner->apply(x, y, s);
         if (!c.exec) {
             if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -822,6 +836,7 @@ struct GraphOp final : OpBase {
             c.cx = x;
             c.cy = y;
             c.cs = s;
+            c.epoch = now;
         }
         if (x == c.cx && y == c.cy && s == c.cs) {
             replays++;
@@ -878,15 +893,18 @@ int sfem_amd_set_option(const char *name, int value) {
     SFEM_AMD_REQUIRE(name, "set_option: name");
     if (std::strcmp(name, "fused_chebyshev") == 0) {
         g_fused_chebyshev = value != 0;
+        bump_state_epoch();
         return SFEM_SUCCESS;
     }
     if (std::strcmp(name, "atomic_skeleton") == 0) {
         g_atomic_skeleton = value != 0;
+        bump_state_epoch();
         return SFEM_SUCCESS;
     }
     if (std::strcmp(name, "elasticity_form") == 0) {
         SFEM_AMD_REQUIRE(value >= 0 && value <= 2, "set_option: elasticity_form is 0 (auto), 1 (element) or 2 (stencil)");
         g_elasticity_form = value;
+        bump_state_epoch();
         return SFEM_SUCCESS;
     }
     set_last_error(std::string("set_option: unknown option ") + name);
@@ -1754,7 +1772,16 @@ int sfem_amd_op_benchmark(sfem_amd_op *op, const void *x, void *y, int warmup, i
         if (op->impl->apply(x, y, s)) return SFEM_FAILURE;
     DevArray<double> scratch;
     if (flush_bytes && scratch.alloc((flush_bytes + 7) / 8)) return SFEM_FAILURE;
-    std::vector<hipEvent_t> ev(2 * (size_t)(flush_bytes ? reps : 1));
+    // the events are destroyed on every exit path
+    struct Events {
+        std::vector<hipEvent_t> v;
+        ~Events() {
+            for (auto &e : v)
+                if (e) (void)hipEventDestroy(e);
+        }
+    } events;
+    std::vector<hipEvent_t> &ev = events.v;
+    ev.assign(2 * (size_t)(flush_bytes ? reps : 1), nullptr);
     for (auto &e : ev) SFEM_AMD_HIP_CHECK(hipEventCreate(&e));
     int err = SFEM_SUCCESS;
     if (!flush_bytes) {
@@ -1770,15 +1797,15 @@ int sfem_amd_op_benchmark(sfem_amd_op *op, const void *x, void *y, int warmup, i
         }
     }
     SFEM_AMD_HIP_CHECK(hipStreamSynchronize(s));
+    if (err) return err;  // some events were never recorded
     double total = 0;
     for (size_t i = 0; i < ev.size(); i += 2) {
         float ms = 0;
         SFEM_AMD_HIP_CHECK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
         total += ms;
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
     *total_ms = total;
-    return err;
+    return SFEM_SUCCESS;
 }
 
 int sfem_amd_dense_op_create(int rt, ptrdiff_t n, const double *matrix, sfem_amd_op **out) {
@@ -1830,6 +1857,7 @@ int sfem_amd_switch_select(sfem_amd_op *op, int which) {
     CHECK_HANDLE(op);
     auto *sw = dynamic_cast<SwitchOp *>(op->impl.get());
     SFEM_AMD_REQUIRE(sw && (which == 0 || which == 1), "switch_select: not a switch operator or index not 0/1");
+    if (sw->active != which) bump_state_epoch();
     sw->active = which;
     return SFEM_SUCCESS;
 }
@@ -1852,6 +1880,7 @@ int sfem_amd_cg_add_fallback(sfem_amd_op *cg, sfem_amd_op *sw, double max_ratio)
         if (swp->active == 0 && prev > 0 && now > max_ratio * prev) {
             swp->active = 1;
             swp->events++;
+            bump_state_epoch();
         }
     };
     auto chain = [hook](std::function<void(int, double, double)> &slot) {

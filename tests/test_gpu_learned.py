@@ -196,3 +196,51 @@ def test_forced_fallback_switches_once():
     # a switched hierarchy can be reset to the learned smoother
     h.select_smoothers(0)
     assert S.switch_state(h.switches[0])[0] == 0
+
+
+def test_graph_replayed_learned_hierarchy_follows_switches_and_fallback():
+    """A V-cycle replayed as a HIP graph must not replay stale host decisions: after the switch
+    moves from the learned smoother to its Chebyshev3 fallback (by hand, or by the CG's fallback
+    hook) the graph is re-captured, and its output equals the eager V-cycle in the same state.
+    A flexible CG with fallback ratio 0 gives the same iterations and fallback events with the
+    graph-replayed preconditioner as with the eager one."""
+    mesh, fff, bc, coeff, N = _poisson_problem(L=4, n_macro=2, coeff_seed=None)
+    cnn = S.CNN(16, 4, seed=0)
+    h = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, learned=cnn, coarse="dense")
+    P_eager = h.preconditioner(graph=False)
+    P_graph = h.preconditioner(graph=True)
+    rng = np.random.default_rng(4)
+    r = S.DeviceArray.from_numpy(rng.uniform(-1, 1, N))
+
+    def both():
+        ze, zg = S.DeviceArray(N), S.DeviceArray(N)
+        P_eager.apply(r, ze)
+        P_graph.apply(r, zg)
+        S.synchronize()
+        return ze.numpy(), zg.numpy()
+
+    outs = {}
+    for which in (0, 1, 0):
+        h.select_smoothers(which)
+        for _ in range(3):  # first call eager, then captured, then replayed
+            ze, zg = both()
+            err = np.linalg.norm(zg - ze) / np.linalg.norm(ze)
+            assert err < 1e-5, (which, err)
+        outs.setdefault(which, ze)
+    # the two smoothers give different cycles: a stale replay would be caught above
+    assert np.linalg.norm(outs[0] - outs[1]) > 1e-3 * np.linalg.norm(outs[1])
+    h64 = gmg.SSGMG(mesh, "laplacian", fff, bc)
+    rhs = np.random.default_rng(9).uniform(0, 1, N)
+    rhs[bc] = 0
+    results = []
+    for P in (P_eager, P_graph):
+        h.select_smoothers(0)
+        solver = learned.flexible_cg_with_fallback(h64.fine.A, P, h.switches, rtol=1e-8, max_it=300,
+                                                   fallback_ratio=0.0)
+        x = S.DeviceArray(N)
+        solver.apply(S.DeviceArray.from_numpy(rhs), x)
+        active, events = S.switch_state(h.switches[0])
+        results.append((solver.iterations, solver.last_status, active))
+    print(f"eager vs graph-replayed (iterations, status, active smoother): {results}")
+    assert results[0] == results[1]
+    assert results[0][1] == 0 and results[0][2] == 1
