@@ -62,7 +62,9 @@ const char *sfem_amd_last_error(void);
  * arithmetic) or in element form (the shared 24x24 micro-element matrix, on the matrix cores for even
  * levels); 0 picks the stencil form for FP64 operators at levels >= 10, 1 forces the element form,
  * 2 the stencil form (levels 2-6, 8, 10, 12, 16).  Both give the reference's operator; the
- * summation order differs. */
+ * summation order differs.  "hex8_patches" (default 1): hex8 elasticity applies on meshes of at least
+ * 2^15 elements run on element patches (512 elements grown over shared nodes, accumulated in LDS,
+ * patch-boundary nodes summed in a fixed order); 0 runs one element per lane with atomics. */
 int sfem_amd_set_option(const char *name, int value);
 /* Number of visible GPUs; 0 when no HIP device is present. */
 int sfem_amd_device_count(void);
@@ -165,7 +167,9 @@ int sfem_amd_element_matrix_op_create(const sfem_amd_mesh *mesh, const double *m
  * (operators/hex8/hex8_linear_elasticity.cpp:625-760),
  * affine_sshex8_linear_elasticity_apply_macro_adjugate
  * (operators/sshex8/sshex8_linear_elasticity.cpp:275-425), cu_linear_elasticity_apply
- * (operators/cuda/cu_linear_elasticity.hpp).  mu, lambda: Lame parameters; coeff scales both. */
+ * (operators/cuda/cu_linear_elasticity.hpp).  mu, lambda: Lame parameters; coeff scales both.
+ * real_type SFEM_AMD_FLOAT32: FP32 vectors with the adjugate and determinant stored as bf16 words
+ * (HEX8: sum-factorised FP32 arithmetic; SSHEX8: per-macro matrices on the matrix cores). */
 int sfem_amd_linear_elasticity_create(const sfem_amd_mesh *mesh, const float *adjugate,
                                       const float *det, double mu, double lambda,
                                       const float *coeff, int real_type, sfem_amd_op **out);

@@ -679,7 +679,8 @@ def norm2(x, stream=None):
 
 def set_option(name, value):
     """Library options, see sfem_amd_set_option ("fused_chebyshev", "atomic_skeleton": 1 or 0,
-    default 0; "elasticity_form": 0 auto, 1 element form, 2 stencil form)."""
+    default 0; "elasticity_form": 0 auto, 1 element form, 2 stencil form; "hex8_patches": 1 (default)
+    or 0)."""
     _check(lib.sfem_amd_set_option(name.encode(), int(value)), "set_option")
 
 

@@ -219,6 +219,22 @@ __host__ __device__ __forceinline__ T cheb3_direction(T p, T rhs, T t, T beta, c
     return first ? r : __builtin_fma(beta, p, r);
 }
 
+// ----------------------------------------------------------------- geometry words
+// Per-element geometry is stored as float32 words (the reference's jacobian_t) or, for the FP32
+// operators, as bf16 words (the upper half of the float32 word, round-to-nearest-even on the host).
+__device__ __forceinline__ float bf16_to_float(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
+
+template <typename G>
+__device__ __forceinline__ double load_geom(const G *g, ptrdiff_t i);
+template <>
+__device__ __forceinline__ double load_geom<float>(const float *g, ptrdiff_t i) {
+    return (double)g[i];
+}
+template <>
+__device__ __forceinline__ double load_geom<uint16_t>(const uint16_t *g, ptrdiff_t i) {
+    return (double)bf16_to_float(g[i]);
+}
+
 // ----------------------------------------------------------------- workgroup barrier for LDS
 // Orders LDS accesses only: __syncthreads() is a workgroup fence over all address spaces, which on
 // this chip waits for every outstanding global load and store (vmcnt(0)) before the barrier.

@@ -129,6 +129,26 @@ struct SkeletonStore {
     ptrdiff_t n{0};
     bool covers{false};  // every mesh node lies in some element (overwriting applies allowed)
 };
+
+// The elements of a tet4/hex8 mesh grouped into patches of at most kP elements, each grown
+// breadth-first over shared nodes from a seed next to the previous patch (mesh_patches).  A patch
+// kernel gathers the patch's nodes once into LDS, accumulates its elements there, adds the nodes that
+// belong to this patch alone straight into y, and writes the others (the patch boundary) to
+// partials in node-sorted order, which skeleton_gather sums per node in patch order.
+struct ElementPatches {
+    static constexpr int kP = 512;
+    ptrdiff_t npatch{0};
+    int maxloc{0};                   // largest node count of a patch
+    double nodes_per_element{0};     // patch node entries / elements (1 would be no duplication)
+    DevArray<int32_t> count;         // [npatch] elements in each patch (its first slots)
+    DevArray<int32_t> perm;          // [npatch * kP] element in each slot (-1 past the patch's end)
+    DevArray<uint16_t> lconn;        // [npatch][nv][kP] local node of each element corner
+    DevArray<int32_t> node_off;      // [npatch + 1] ranges of the patch node lists
+    DevArray<int32_t> nodes;         // mesh node of each patch node
+    DevArray<int32_t> bslot;         // its partial position (node-sorted), -1 for a patch's own node
+    SkeletonStore boundary;          // the shared nodes: node (ascending), off (partial ranges)
+    ptrdiff_t nparts{0};             // partial entries
+};
 }  // namespace sfem_amd
 
 struct sfem_amd_op {
@@ -149,6 +169,8 @@ struct sfem_amd_mesh {
     mutable std::map<int, std::unique_ptr<sfem_amd::SkeletonStore>> skel;
     // sshex8: the macro elements cut into sub-blocks of level Ls (lazy, mesh_subblocks)
     mutable std::map<int, std::unique_ptr<sfem_amd_mesh>> sub;
+    // tet4/hex8: element patches (lazy, mesh_patches)
+    mutable std::unique_ptr<sfem_amd::ElementPatches> patches;
 };
 
 namespace sfem_amd {
@@ -161,6 +183,10 @@ extern bool g_atomic_skeleton;
 // element form (matrix-core / node-wise kernels) everywhere; 2 stencil form wherever supported
 // (option "elasticity_form")
 extern int g_elasticity_form;
+// hex8 elasticity applies on element patches (option "hex8_patches", default on) for meshes of at
+// least kPatchMinElements elements; off: one element per lane with atomics
+extern bool g_hex8_patches;
+constexpr ptrdiff_t kPatchMinElements = (ptrdiff_t)1 << 15;
 // Host-side state that a captured HIP graph depends on (library options, switch selections, CNN
 // weights) changes the state epoch; graph-replaying operators re-capture when it moved.
 uint64_t state_epoch();
@@ -174,5 +200,7 @@ int mesh_skeleton(const sfem_amd_mesh *m, int Lc);
 // The sshex8 mesh's macro elements as (L/Ls)^3 sub-blocks of level Ls (sshex8_subblock_idx), built
 // once per level and shared; null on failure.
 const sfem_amd_mesh *mesh_subblocks(const sfem_amd_mesh *m, int Ls, hipStream_t s);
+// Builds (once) the element patches of a tet4/hex8 mesh; null on failure or for other meshes.
+const ElementPatches *mesh_patches(const sfem_amd_mesh *m);
 sfem_amd_op *wrap_op(std::shared_ptr<OpBase> o);
 }  // namespace sfem_amd

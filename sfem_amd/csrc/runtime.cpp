@@ -159,6 +159,7 @@ void bump_state_epoch() { g_state_epoch.fetch_add(1, std::memory_order_acq_rel);
 bool g_fused_chebyshev = false;
 bool g_atomic_skeleton = false;
 int g_elasticity_form = 0;
+bool g_hex8_patches = true;
 
 const sfem_amd_mesh *mesh_subblocks(const sfem_amd_mesh *m, int Ls, hipStream_t s) {
     auto &slot = m->sub[Ls];
@@ -182,6 +183,150 @@ const sfem_amd_mesh *mesh_subblocks(const sfem_amd_mesh *m, int Ls, hipStream_t 
     }
     slot = std::move(sm);
     return slot.get();
+}
+
+// Element patches of a tet4/hex8 mesh: breadth-first growth over shared nodes, at most kP elements
+// each, the next seed taken from the unassigned neighbours of the previous patch (lowest id first),
+// else the lowest unassigned element.  On a structured box of hexahedra the patches come out as
+// compact blocks (about 1.56 patch nodes per element at 512 elements, against 1.42 for a perfect
+// 8^3 cube and 8 for no sharing).
+static_assert(ElementPatches::kP == kPatchElements, "patch size of the host partition and the kernels");
+const ElementPatches *mesh_patches(const sfem_amd_mesh *m) {
+    if (m->patches) return m->patches.get();
+    if (m->type == SFEM_AMD_SSHEX8 || m->E <= 0 || m->nv <= 0) return nullptr;
+    constexpr int P = ElementPatches::kP;
+    const int nv = m->nv;
+    const ptrdiff_t E = m->E, N = m->nnodes;
+    if ((double)E * nv >= 2147483647.0 || N >= 2147483647) {
+        set_last_error("element patches: mesh exceeds int32 indexing");
+        return nullptr;
+    }
+    std::vector<int32_t> el((size_t)E * nv);  // SoA [nv][E]
+    if (hipMemcpy(el.data(), m->idx.p, sizeof(int32_t) * el.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_last_error("element patches: connectivity download failed");
+        return nullptr;
+    }
+    // node -> elements
+    std::vector<int32_t> n2e_off((size_t)N + 1, 0);
+    for (const int32_t g : el) n2e_off[(size_t)g + 1]++;
+    for (ptrdiff_t g = 0; g < N; g++) n2e_off[g + 1] += n2e_off[g];
+    std::vector<int32_t> n2e((size_t)E * nv), fill(n2e_off.begin(), n2e_off.end() - 1);
+    for (int v = 0; v < nv; v++)
+        for (ptrdiff_t e = 0; e < E; e++) n2e[fill[el[(size_t)v * E + e]]++] = (int32_t)e;
+    // grow the patches
+    std::vector<int32_t> patch_of((size_t)E, -1), order, pstart, queue, hint;
+    order.reserve(E);
+    ptrdiff_t seed = 0;
+    for (;;) {
+        int32_t s0 = -1;
+        while (!hint.empty()) {
+            const int32_t c = hint.back();
+            hint.pop_back();
+            if (patch_of[c] < 0) {
+                s0 = c;
+                break;
+            }
+        }
+        if (s0 < 0) {
+            while (seed < E && patch_of[seed] >= 0) seed++;
+            if (seed >= E) break;
+            s0 = (int32_t)seed;
+        }
+        const int32_t pid = (int32_t)pstart.size();
+        pstart.push_back((int32_t)order.size());
+        queue.assign(1, s0);
+        patch_of[s0] = pid;
+        for (size_t head = 0; head < queue.size(); head++) {
+            const int32_t e = queue[head];
+            for (int v = 0; v < nv && queue.size() < (size_t)P; v++) {
+                const int32_t g = el[(size_t)v * E + e];
+                for (int32_t k = n2e_off[g]; k < n2e_off[g + 1] && queue.size() < (size_t)P; k++) {
+                    const int32_t f = n2e[k];
+                    if (patch_of[f] < 0) {
+                        patch_of[f] = pid;
+                        queue.push_back(f);
+                    }
+                }
+            }
+        }
+        order.insert(order.end(), queue.begin(), queue.end());
+        // unassigned neighbours of this patch, lowest id popped first
+        hint.clear();
+        for (const int32_t e : queue)
+            for (int v = 0; v < nv; v++) {
+                const int32_t g = el[(size_t)v * E + e];
+                for (int32_t k = n2e_off[g]; k < n2e_off[g + 1]; k++)
+                    if (patch_of[n2e[k]] < 0) hint.push_back(n2e[k]);
+            }
+        std::sort(hint.begin(), hint.end(), std::greater<int32_t>());
+        hint.erase(std::unique(hint.begin(), hint.end()), hint.end());
+    }
+    const ptrdiff_t np = (ptrdiff_t)pstart.size();
+    pstart.push_back((int32_t)E);
+    // local numbering, patch node lists and which nodes other patches share
+    std::vector<int32_t> count((size_t)np), perm((size_t)np * P, -1), node_off(1, 0), nodes;
+    std::vector<uint16_t> lconn((size_t)np * nv * P, 0);
+    std::vector<int32_t> local((size_t)N, -1), inside((size_t)N, 0);
+    std::vector<uint8_t> shared_flag;
+    int maxloc = 0;
+    for (ptrdiff_t p = 0; p < np; p++) {
+        const int32_t b = pstart[p], cnt = pstart[p + 1] - b;
+        count[p] = cnt;
+        const size_t first = nodes.size();
+        for (int32_t t = 0; t < cnt; t++) {
+            const int32_t e = order[(size_t)b + t];
+            perm[(size_t)p * P + t] = e;
+            for (int v = 0; v < nv; v++) {
+                const int32_t g = el[(size_t)v * E + e];
+                if (local[g] < 0) {
+                    local[g] = (int32_t)(nodes.size() - first);
+                    nodes.push_back(g);
+                }
+                inside[g]++;
+                lconn[((size_t)p * nv + v) * P + t] = (uint16_t)local[g];
+            }
+        }
+        const int nloc = (int)(nodes.size() - first);
+        if (nloc > 65535) {
+            set_last_error("element patches: more than 65535 nodes in a patch");
+            return nullptr;
+        }
+        maxloc = std::max(maxloc, nloc);
+        for (size_t i = first; i < nodes.size(); i++) {
+            const int32_t g = nodes[i];
+            shared_flag.push_back(inside[g] < n2e_off[g + 1] - n2e_off[g] ? 1 : 0);
+            local[g] = -1;
+            inside[g] = 0;
+        }
+        node_off.push_back((int32_t)nodes.size());
+    }
+    // node-sorted partial positions of the shared entries (patch order within a node)
+    std::vector<int32_t> pcount((size_t)N + 1, 0);
+    for (size_t i = 0; i < nodes.size(); i++)
+        if (shared_flag[i]) pcount[(size_t)nodes[i] + 1]++;
+    std::vector<int32_t> bnode, boff(1, 0), start((size_t)N, -1);
+    for (ptrdiff_t g = 0; g < N; g++)
+        if (pcount[g + 1]) {
+            start[g] = boff.back();
+            bnode.push_back((int32_t)g);
+            boff.push_back(boff.back() + pcount[g + 1]);
+        }
+    std::vector<int32_t> bslot(nodes.size(), -1);
+    for (size_t i = 0; i < nodes.size(); i++)
+        if (shared_flag[i]) bslot[i] = start[nodes[i]]++;
+    auto pt = std::make_unique<ElementPatches>();
+    pt->npatch = np;
+    pt->maxloc = maxloc;
+    pt->nodes_per_element = (double)nodes.size() / (double)E;
+    pt->nparts = boff.back();
+    pt->boundary.n = (ptrdiff_t)bnode.size();
+    if (pt->count.upload(count.data(), count.size()) || pt->perm.upload(perm.data(), perm.size()) ||
+        pt->lconn.upload(lconn.data(), lconn.size()) || pt->node_off.upload(node_off.data(), node_off.size()) ||
+        pt->nodes.upload(nodes.data(), nodes.size()) || pt->bslot.upload(bslot.data(), bslot.size()) ||
+        pt->boundary.node.upload(bnode.data(), bnode.size()) || pt->boundary.off.upload(boff.data(), boff.size()))
+        return nullptr;
+    m->patches = std::move(pt);
+    return m->patches.get();
 }
 
 // ----------------------------------------------------------------- Laplacian
@@ -259,6 +404,12 @@ template <typename T>
 struct ElasticityOp final : OpBase {
     const sfem_amd_mesh *m{nullptr};
     DevArray<float> adj, det, coeff;  // hex8: SoA [9][E], [E]
+    DevArray<uint16_t> adj16, det16;  // hex8, FP32 operator: the same words rounded to bf16
+    // hex8 on element patches: geometry and coefficient words in patch-slot order, shared-node partials
+    DevArray<float> adjp, detp, coeffp;
+    DevArray<uint16_t> adjp16, detp16;
+    DevArray<T> ppart;
+    int patch_state{0};  // 0 not yet tried, 1 ready, -1 not applicable
     DevArray<double> K;               // sshex8: per-macro 24x24 matrices
     DevArray<float> K32;              // sshex8, FP32 operator: K rounded to float
     DevArray<uint32_t> Apk;           // FP32 matrix cores: K32 split into bf16 A operands
@@ -341,11 +492,13 @@ struct ElasticityOp final : OpBase {
     int apply_impl(const void *xv, void *yv, hipStream_t s, int ow, const uint8_t *cmask = nullptr,
                    const ChebStep<T> *cs = nullptr) {
         if (m->type == SFEM_AMD_HEX8) {
+            if (g_hex8_patches && m->E >= kPatchMinElements && patches_ready(s)) return apply_patches(xv, yv, s);
             if constexpr (std::is_same<T, double>::value)
-                return hex8_elasticity_apply(m->E, m->idx.p, adj.p, det.p, mu, lambda, coeff.p, (const double *)xv,
-                                             (double *)yv, s);
-            set_last_error("hex8 elasticity: FP64 only");
-            return SFEM_FAILURE;
+                return hex8_elasticity_apply<double, float>(m->E, m->idx.p, adj.p, det.p, mu, lambda, coeff.p,
+                                                            (const double *)xv, (double *)yv, s);
+            else
+                return hex8_elasticity_apply<float, uint16_t>(m->E, m->idx.p, adj16.p, det16.p, mu, lambda, coeff.p,
+                                                              (const float *)xv, (float *)yv, s);
         }
         const sfem_amd_mesh *wm = work_mesh(s);
         if (!wm) return SFEM_FAILURE;
@@ -379,20 +532,66 @@ struct ElasticityOp final : OpBase {
                                           uses_matrix_cores() ? 1 : 0, wm == m ? 0 : 3, Apk.p,
                                           cs ? *cs : ChebStep<T>());
     }
+    // hex8: the element patches of the mesh and this operator's words in their slot order (first use)
+    bool patches_ready(hipStream_t s) {
+        if (patch_state) return patch_state > 0;
+        patch_state = -1;
+        const ElementPatches *pt = mesh_patches(m);
+        if (!pt || hex8_patch_lds_bytes(pt->maxloc, sizeof(T)) > kPatchLdsLimit) return false;
+        const ptrdiff_t Ep = pt->npatch * ElementPatches::kP;
+        bool ok = ppart.alloc((size_t)pt->nparts * 3 + 3) == SFEM_SUCCESS;
+        if constexpr (std::is_same<T, double>::value)
+            ok = ok && !adjp.alloc((size_t)9 * Ep) && !detp.alloc(Ep) &&
+                 !patch_gather_rows<float>(9, m->E, Ep, pt->perm.p, adj.p, adjp.p, s) &&
+                 !patch_gather_rows<float>(1, m->E, Ep, pt->perm.p, det.p, detp.p, s);
+        else
+            ok = ok && !adjp16.alloc((size_t)9 * Ep) && !detp16.alloc(Ep) &&
+                 !patch_gather_rows<uint16_t>(9, m->E, Ep, pt->perm.p, adj16.p, adjp16.p, s) &&
+                 !patch_gather_rows<uint16_t>(1, m->E, Ep, pt->perm.p, det16.p, detp16.p, s);
+        if (ok && coeff.p)
+            ok = !coeffp.alloc(Ep) && !patch_gather_rows<float>(1, m->E, Ep, pt->perm.p, coeff.p, coeffp.p, s);
+        if (!ok || hipStreamSynchronize(s) != hipSuccess) return false;
+        patch_state = 1;
+        return true;
+    }
+    int apply_patches(const void *xv, void *yv, hipStream_t s) {
+        const ElementPatches *pt = m->patches.get();
+        PatchView pv;
+        pv.npatch = pt->npatch;
+        pv.maxloc = pt->maxloc;
+        pv.count = pt->count.p;
+        pv.node_off = pt->node_off.p;
+        pv.nodes = pt->nodes.p;
+        pv.bslot = pt->bslot.p;
+        pv.lconn = pt->lconn.p;
+        int err;
+        if constexpr (std::is_same<T, double>::value)
+            err = hex8_elasticity_apply_patches<double, float>(pv, adjp.p, detp.p, coeffp.p, mu, lambda,
+                                                               (const double *)xv, (double *)yv, ppart.p, s);
+        else
+            err = hex8_elasticity_apply_patches<float, uint16_t>(pv, adjp16.p, detp16.p, coeffp.p, mu, lambda,
+                                                                 (const float *)xv, (float *)yv, ppart.p, s);
+        if (err) return err;
+        SkeletonMap sk{};
+        sk.node = pt->boundary.node.p;
+        sk.off = pt->boundary.off.p;
+        sk.n = pt->boundary.n;
+        return skeleton_gather<T>(sk, 3, ppart.p, (T *)yv, s, 0, 1);
+    }
+    int hex8_diag(int block, void *d, hipStream_t s) {
+        if constexpr (std::is_same<T, double>::value)
+            return hex8_elasticity_diag<double, float>(m->E, m->idx.p, adj.p, det.p, mu, lambda, coeff.p, block,
+                                                       (double *)d, s);
+        else
+            return hex8_elasticity_diag<float, uint16_t>(m->E, m->idx.p, adj16.p, det16.p, mu, lambda, coeff.p,
+                                                         block, (float *)d, s);
+    }
     int diag(void *d, hipStream_t s) override {
-        if (m->type == SFEM_AMD_HEX8) {
-            if constexpr (std::is_same<T, double>::value)
-                return hex8_elasticity_diag(m->E, m->idx.p, adj.p, det.p, mu, lambda, coeff.p, 0, (double *)d, s);
-            return SFEM_FAILURE;
-        }
+        if (m->type == SFEM_AMD_HEX8) return hex8_diag(0, d, s);
         return sshex8_elasticity_diag<T>(m->level, m->E, m->idx.p, K.p, 0, (T *)d, s);
     }
     int block_diag(void *d, hipStream_t s) override {
-        if (m->type == SFEM_AMD_HEX8) {
-            if constexpr (std::is_same<T, double>::value)
-                return hex8_elasticity_diag(m->E, m->idx.p, adj.p, det.p, mu, lambda, coeff.p, 1, (double *)d, s);
-            return SFEM_FAILURE;
-        }
+        if (m->type == SFEM_AMD_HEX8) return hex8_diag(1, d, s);
         return sshex8_elasticity_diag<T>(m->level, m->E, m->idx.p, K.p, 1, (T *)d, s);
     }
 };
@@ -901,6 +1100,11 @@ int sfem_amd_set_option(const char *name, int value) {
         bump_state_epoch();
         return SFEM_SUCCESS;
     }
+    if (std::strcmp(name, "hex8_patches") == 0) {
+        g_hex8_patches = value != 0;
+        bump_state_epoch();
+        return SFEM_SUCCESS;
+    }
     if (std::strcmp(name, "elasticity_form") == 0) {
         SFEM_AMD_REQUIRE(value >= 0 && value <= 2, "set_option: elasticity_form is 0 (auto), 1 (element) or 2 (stencil)");
         g_elasticity_form = value;
@@ -1178,9 +1382,26 @@ int sfem_amd_linear_elasticity_create(const sfem_amd_mesh *mesh, const float *ad
         op->mu = mu;
         op->lambda = lambda;
         if (mesh->type == SFEM_AMD_HEX8) {
-            if (soa_upload(op->adj, adj, E, 9) || op->det.upload(det, E)) return SFEM_FAILURE;
             if (coeff && op->coeff.upload(coeff, E)) return SFEM_FAILURE;
-            return SFEM_SUCCESS;
+            if (rt == SFEM_AMD_FLOAT64)
+                return (soa_upload(op->adj, adj, E, 9) || op->det.upload(det, E)) ? SFEM_FAILURE : SFEM_SUCCESS;
+            // FP32 operator: bf16 SoA words
+            if (device_pointer(adj)) {
+                SFEM_AMD_REQUIRE(device_pointer(det), "elasticity_create: adjugate and det in the same memory");
+                if (op->adj16.alloc((size_t)9 * E) || op->det16.alloc((size_t)E) ||
+                    aos_to_bf16_soa_device(adj, nullptr, E, 9, op->adj16.p, nullptr) ||
+                    aos_to_bf16_soa_device(det, nullptr, E, 1, op->det16.p, nullptr))
+                    return SFEM_FAILURE;
+                SFEM_AMD_HIP_CHECK(hipDeviceSynchronize());
+                return SFEM_SUCCESS;
+            }
+            std::vector<uint16_t> ha((size_t)9 * E), hd(E);
+            for (ptrdiff_t e = 0; e < E; e++) {
+                for (int q = 0; q < 9; q++) ha[(size_t)q * E + e] = to_bf16(adj[e * 9 + q]);
+                hd[e] = to_bf16(det[e]);
+            }
+            return (op->adj16.upload(ha.data(), ha.size()) || op->det16.upload(hd.data(), hd.size())) ? SFEM_FAILURE
+                                                                                                     : SFEM_SUCCESS;
         }
         // sshex8: per-macro 24x24 matrices, built on device from the float32 words (FP64
         // path) or from BF16-rounded words (FP32 path)
@@ -1220,10 +1441,9 @@ int sfem_amd_linear_elasticity_create(const sfem_amd_mesh *mesh, const float *ad
         if (setup(op)) return SFEM_FAILURE;
         *out = wrap(op);
     } else {
-        SFEM_AMD_REQUIRE(mesh->type == SFEM_AMD_SSHEX8, "FP32 elasticity: sshex8 only");
         auto op = std::make_shared<ElasticityOp<float>>();
         if (setup(op)) return SFEM_FAILURE;
-        op->matrix_cores = true;  // its BF16 geometry already bounds the accuracy (~1e-2)
+        op->matrix_cores = true;  // sshex8: its BF16 geometry already bounds the accuracy (~1e-2)
         *out = wrap(op);
     }
     return SFEM_SUCCESS;

@@ -17,19 +17,6 @@
 
 namespace sfem_amd {
 
-__device__ __forceinline__ float bf16_to_float(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
-
-template <typename G>
-__device__ __forceinline__ double load_geom(const G *g, ptrdiff_t i);
-template <>
-__device__ __forceinline__ double load_geom<float>(const float *g, ptrdiff_t i) {
-    return (double)g[i];
-}
-template <>
-__device__ __forceinline__ double load_geom<uint16_t>(const uint16_t *g, ptrdiff_t i) {
-    return (double)bf16_to_float(g[i]);
-}
-
 template <typename T>
 __device__ __forceinline__ void atomic_add(T *p, T v) {
     atomicAdd(p, v);

@@ -89,8 +89,10 @@ class SSGMG:
         meshes = [fine_mesh]
         for l in derefinement_levels(fine_mesh.level)[1:]:
             meshes.append(meshes[-1].derefine(l))
-        for m in meshes:
-            self.levels.append(self._make_level(m, geometry, coeff, bc_nodes[bc_nodes < m.nnodes]))
+        extended = macro_box is not None and super_level(macro_box[0]) > 0
+        for i, m in enumerate(meshes):
+            coarsest = i == len(meshes) - 1 and not extended
+            self.levels.append(self._make_level(m, geometry, coeff, bc_nodes[bc_nodes < m.nnodes], coarsest))
         # nested transfers inside the macro hierarchy
         for i in range(len(self.levels) - 1):
             f, co = self.levels[i], self.levels[i + 1]
@@ -102,10 +104,13 @@ class SSGMG:
         self._finish(coarse_rtol, coarse_max_it, max_it, atol)
 
     # -- helpers
-    def _make_level(self, m, geometry, coeff, bc):
+    def _make_level(self, m, geometry, coeff, bc, coarsest=False):
         lv = Level()
         lv.mesh = m
-        rt = self.real_type if m.element_type == SSHEX8 else FLOAT64
+        # HEX8 levels: the FP32 hierarchy runs its elasticity operator in FP32 (bf16 geometry words)
+        # except on the coarsest level, whose solve stays FP64; the HEX8 Laplacian is FP64 only
+        fp32_ok = m.element_type == SSHEX8 or (self.kind == "elasticity" and not coarsest)
+        rt = self.real_type if fp32_ok else FLOAT64
         if self.kind == "laplacian":
             lv.op = laplacian(m, geometry, coeff, real_type=rt)
         else:
@@ -176,9 +181,10 @@ class SSGMG:
         self._super = (top, el, pts, perm)
         prev = None
         link_fine = self.levels[-1]
-        for l in derefinement_levels(S)[1:]:
+        super_levels = derefinement_levels(S)[1:]
+        for l in super_levels:
             m = top.derefine(l)
-            lv = self._make_level(m, sgeom, scoeff, sbc_top[sbc_top < m.nnodes])
+            lv = self._make_level(m, sgeom, scoeff, sbc_top[sbc_top < m.nnodes], l == super_levels[-1])
             rt = self.real_type
             bs = self.block_size
             if prev is None:

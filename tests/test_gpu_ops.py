@@ -167,23 +167,69 @@ def test_empty_mesh_apply_is_noop():
 
 
 # ---------------------------------------------------------------- elasticity
-@pytest.mark.parametrize("n", [6, 65])  # one lane per Gauss point (small) / per element (>= 2^18 elements)
-def test_hex8_elasticity_parity(n):
+# one lane per Gauss point (small meshes), element patches (>= 2^15 elements), one lane per element
+@pytest.mark.parametrize("n,patches", [(6, 1), (65, 1), (65, 0)])
+def test_hex8_elasticity_parity(n, patches):
     el, pts = skewed_box(n, perturb=0.02 * 6 / n)
     adj, det = O.hex8_adj_det(el, pts)
     coeff = np.random.default_rng(4).uniform(0.1, 10, el.shape[1]).astype(np.float32)
     mu, lam = O.lame(1.0, 0.3)
     n = pts.shape[1] * 3
     x = np.random.default_rng(0).uniform(-1, 1, n)
-    op = S.linear_elasticity(S.Mesh(S.HEX8, 1, el, pts.shape[1]), adj, det, mu, lam, coeff)
-    yo = np.zeros(n)
-    O.lib.oracle_hex8_elasticity_apply(el.shape[1], el, adj, det, mu, lam, coeff, x, yo)
-    assert relerr(host(gpu_apply(op, dev(x))), yo) < RTOL
+    S.set_option("hex8_patches", patches)
+    try:
+        op = S.linear_elasticity(S.Mesh(S.HEX8, 1, el, pts.shape[1]), adj, det, mu, lam, coeff)
+        yo = np.full(n, 0.25)
+        y = dev(yo)  # accumulate semantics: y += A x
+        O.lib.oracle_hex8_elasticity_apply(el.shape[1], el, adj, det, mu, lam, coeff, x, yo)
+        op.apply(dev(x), y)
+        S.synchronize()
+    finally:
+        S.set_option("hex8_patches", 1)
+    assert relerr(host(y), yo) < RTOL
     do = np.zeros(n)
     O.lib.oracle_hex8_elasticity_diag(el.shape[1], el, adj, det, mu, lam, coeff, do)
     d = zeros(n)
     op.diag(d)
     assert relerr(host(d), do) < RTOL
+
+
+@pytest.mark.parametrize("n,patches", [(6, 1), (40, 1), (40, 0)])
+def test_hex8_elasticity_fp32_bf16_error_stated(n, patches):
+    """FP32 hex8 elasticity (bf16 adjugate and determinant words, FP32 sum-factorised arithmetic):
+    against the FP64 oracle on the same bf16-rounded words the error is FP32 rounding (< 1e-5); against
+    the FP64 oracle on the float32 words it adds the geometry rounding (< 1e-2).  Diagonal and block
+    diagonal likewise."""
+    el, pts = skewed_box(n, perturb=0.02 * 6 / n)
+    adj, det = O.hex8_adj_det(el, pts)
+    E = el.shape[1]
+    coeff = np.random.default_rng(4).uniform(0.1, 10, E).astype(np.float32)
+    mu, lam = O.lame(1.0, 0.3)
+    N = pts.shape[1] * 3
+    x = np.random.default_rng(0).uniform(-1, 1, N)
+    op = S.linear_elasticity(S.Mesh(S.HEX8, 1, el, pts.shape[1]), adj, det, mu, lam, coeff, real_type=S.FLOAT32)
+    S.set_option("hex8_patches", patches)
+    try:
+        y = host(gpu_apply(op, dev(x, np.float32))).astype(np.float64)
+    finally:
+        S.set_option("hex8_patches", 1)
+    yb, yo = np.zeros(N), np.zeros(N)
+    O.lib.oracle_hex8_elasticity_apply(E, el, bf16_round(adj), bf16_round(det), mu, lam, coeff, x, yb)
+    O.lib.oracle_hex8_elasticity_apply(E, el, adj, det, mu, lam, coeff, x, yo)
+    e_arith, e_geom = relerr(y, yb), relerr(y, yo)
+    print(f"hex8 elasticity FP32/bf16 n={n}: {e_arith:.2e} vs bf16 words, {e_geom:.2e} vs the FP64 oracle")
+    assert e_arith < 1e-5 and e_geom < 1e-2
+    db = np.zeros(N)
+    O.lib.oracle_hex8_elasticity_diag(E, el, bf16_round(adj), bf16_round(det), mu, lam, coeff, db)
+    d = zeros(N, np.float32)
+    op.diag(d)
+    assert relerr(host(d).astype(np.float64), db) < 1e-6
+    b6 = zeros(N // 3 * 6, np.float32)
+    op.block_diag_sym(b6)
+    b64 = zeros(N // 3 * 6)
+    S.linear_elasticity(S.Mesh(S.HEX8, 1, el, pts.shape[1]), bf16_round(adj), bf16_round(det), mu, lam,
+                        coeff).block_diag_sym(b64)
+    assert relerr(host(b6).astype(np.float64), host(b64)) < 1e-6
 
 
 @pytest.mark.parametrize("L,n", [(1, 2), (2, 2), (4, 2), (6, 1), (8, 2), (12, 1), (16, 1), (16, 2)])

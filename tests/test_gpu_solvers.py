@@ -334,6 +334,52 @@ def test_structured_extension_below_macro_mesh(real_type):
     assert solver.last_status == 0 and res < 1e-8
 
 
+def test_hex8_elasticity_mg_pcg_fp64_and_mixed_match_oracle():
+    """hex8 linear elasticity on a 16^3 box (the 128^3 workload's problem at reduced size: E = 1,
+    nu = 0.3, clamped at z = 0, body force (0, 0, -1)).  Geometric MG below the hex8 grid (super-mesh
+    levels 8, 4, 2, 1), FP64 MG-PCG and the FP32 hierarchy (FP32 hex8 operator with bf16 geometry)
+    inside FP64 flexible CG; both solutions against the oracle's CG solve of the same system."""
+    n = 16
+    el, pts = O.box_mesh(n + 1, n + 1, n + 1)
+    adj, det = O.hex8_adj_det(el, pts)
+    N = pts.shape[1]
+    E = el.shape[1]
+    nodes = np.where(pts[2] < 1e-6)[0].astype(np.int32)
+    mu, lam = O.lame(1.0, 0.3)
+    mesh = S.Mesh(S.HEX8, 1, el, N)
+    h64 = gmg.SSGMG(mesh, "elasticity", (adj, det), nodes, mu=mu, lam=lam, macro_box=(n, 1.0))
+    h32 = gmg.SSGMG(mesh, "elasticity", (adj, det), nodes, mu=mu, lam=lam, macro_box=(n, 1.0),
+                    real_type=S.FLOAT32)
+    assert [lv.mesh.level for lv in h64.levels] == [1, 8, 4, 2, 1]
+    assert h32.levels[0].op.real_type == S.FLOAT32 and h32.levels[-1].op.real_type == S.FLOAT64
+    b = np.zeros((N, 3))
+    b[:, 2] = -1
+    b[nodes] = 0
+    b = b.ravel()
+    dofs = (nodes[:, None].astype(np.int64) * 3 + np.arange(3)).ravel()
+    A_o = oracle_constrained(lambda x, y: O.lib.oracle_hex8_elasticity_apply(E, el, adj, det, mu, lam, None, x, y),
+                             dofs)
+    d = np.zeros(3 * N)
+    O.lib.oracle_hex8_elasticity_diag(E, el, adj, det, mu, lam, None, d)
+    d[dofs] = 1
+    xo, it_o, st_o, _ = SO.cg(A_o, b, np.zeros_like(b), rtol=1e-13, max_it=5000,
+                              precond=lambda r, z: z.__iadd__(r / d))
+    assert st_o == 0
+    out = {}
+    for name, P, flexible in (("FP64", h64.mg, False), ("mixed", S.precision_bridge(h32.mg), True)):
+        solver = S.cg(h64.fine.A, P, flexible=flexible, rtol=1e-10, max_it=300)
+        x = zeros(3 * N)
+        solver.apply(dev(b), x)
+        res = true_residual(h64.fine.A, dev(b), x)
+        diff = np.linalg.norm(host(x) - xo) / np.linalg.norm(xo)
+        out[name] = solver.iterations
+        print(f"hex8 elasticity {name} MG-PCG: {solver.iterations} its, residual {res:.2e}, "
+              f"difference to the oracle solve {diff:.2e}")
+        assert solver.last_status == 0 and res < (1e-8 if name == "FP64" else 1e-10)
+        assert diff < 1e-7
+    assert out["mixed"] <= out["FP64"] + 3
+
+
 def test_structured_extension_elasticity():
     L, n = 4, 4
     mel, mpts = O.box_mesh(n + 1, n + 1, n + 1)
