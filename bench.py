@@ -306,6 +306,93 @@ def config5_apply(S, steps, warmup):
     return out
 
 
+def hex8_elasticity_config3(S, args, n=128, rtol=1e-10):
+    """hex8 linear elasticity on a 128^3 hex8 unit cube (6,440,067 DOF): E = 1, nu = 0.3, clamped at
+    z = 0, body force (0, 0, -1).  The FP64 apply and the FP32 one (bf16 adjugate and determinant words)
+    on element patches, the FP32 apply's error against the FP64 one, and the time to true relative
+    residual rtol of FP64 MG-PCG and of the FP32 hierarchy in FP64 flexible CG.  The multigrid
+    continues below the hex8 grid on the super-mesh levels 8, 4, 2, 1 of an 8^3 box (graph-replayed
+    V-cycles, dense coarse solve)."""
+    from sfem_amd import gmg, learned
+    progress(f"config 3: hex8 {n}^3 mesh")
+    t0 = time.perf_counter()
+    el, pts = S.box_mesh(n + 1, n + 1, n + 1, element_type=S.HEX8)
+    adj, det = S.adjugate_det(S.HEX8, 1, el, pts)
+    nn, E = pts.shape[1], el.shape[1]
+    N = 3 * nn
+    mesh = S.Mesh(S.HEX8, 1, el, nn)
+    bc = np.where(pts[2] < 1e-6)[0].astype(np.int32)
+    nu = 0.3
+    mu, lam = 1.0 / (2 * (1 + nu)), nu / ((1 + nu) * (1 - 2 * nu))
+    mesh_s = time.perf_counter() - t0
+    out = {"workload": f"hex8 linear elasticity on a {n}^3 hex8 unit cube ({N} DOF), E=1, nu=0.3, clamped z=0, "
+                       "body force (0,0,-1)", "dofs": N, "elements": E, "mesh_seconds": mesh_s}
+    x64 = S.DeviceArray.from_numpy(np.random.default_rng(0).uniform(-1, 1, N))
+    x32 = S.DeviceArray.from_numpy(x64.numpy().astype(np.float32), np.float32)
+    ys = {}
+    for name, rt, x, vb, gb in (("fp64", S.FLOAT64, x64, 8, 4), ("fp32", S.FLOAT32, x32, 4, 2)):
+        op = S.linear_elasticity(mesh, adj, det, mu, lam, real_type=rt)
+        y = S.DeviceArray(N, x.dtype)
+        t0 = time.perf_counter()
+        op.apply(x, y)  # first use builds the element patches
+        S.synchronize()
+        first = time.perf_counter() - t0
+        y.zero()
+        op.apply(x, y)
+        S.synchronize()
+        ys[name] = y.numpy().astype(np.float64)
+        reps = max(args.steps, 20)
+        ms = S.benchmark(op, x, y, warmup=3, reps=reps) / reps
+        B = 2 * vb * N + 4 * 8 * E + gb * 10 * E
+        out["apply_" + name] = {"ms_per_apply": ms, "dof_per_s": N / (ms * 1e-3), "algorithmic_bytes": B,
+                                "GB_per_s": B / (ms * 1e-3) / 1e9, "hbm_frac": B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                "first_apply_seconds": first}
+    out["apply_fp32"]["rel_error_vs_fp64"] = float(np.linalg.norm(ys["fp32"] - ys["fp64"]) /
+                                                   np.linalg.norm(ys["fp64"]))
+    out["apply_fp32"]["note"] = ("FP32 sum-factorised arithmetic on bf16 geometry words; error against the FP64 apply, "
+                                 "itself within 1e-12 of the oracle (tests/test_gpu_ops.py)")
+    if args.no_solve:
+        return out
+    rhs = np.zeros((nn, 3))
+    rhs[:, 2] = -1
+    rhs[bc] = 0
+    rhs = rhs.ravel()
+    b = S.DeviceArray.from_numpy(rhs)
+    common = dict(mu=mu, lam=lam, macro_box=(n, 1.0), coarse="dense")
+    progress("config 3: FP64 and FP32 hierarchies")
+    t0 = time.perf_counter()
+    h64 = gmg.SSGMG(mesh, "elasticity", (adj, det), bc, **common)
+    setups = {"fp64": time.perf_counter() - t0}
+    t0 = time.perf_counter()
+    h32 = gmg.SSGMG(mesh, "elasticity", (adj, det), bc, real_type=S.FLOAT32, **common)
+    setups["mixed"] = time.perf_counter() - t0
+    A64 = h64.fine.A
+    out["levels"] = [f"{'hex8' if lv.mesh.element_type == S.HEX8 else 'sshex8'} L={lv.mesh.level} "
+                     f"({3 * lv.mesh.nnodes} DOF)" for lv in h64.levels]
+    for name, h, rt in (("fp64", h64, S.FLOAT64), ("mixed", h32, S.FLOAT32)):
+        progress(f"config 3 solve: {name}")
+        P = h.preconditioner(graph=True)
+        if rt == S.FLOAT64:
+            solver = S.cg(A64, P, rtol=rtol, max_it=1000, stop_on_residual=True)
+        else:
+            solver = learned.flexible_cg_with_fallback(A64, P, h.switches, rtol=rtol, max_it=1000)
+        x = S.DeviceArray(N)
+        solver.apply(b, x)  # warm-up (captures the graph)
+        x.zero()
+        S.synchronize()
+        t0 = time.perf_counter()
+        solver.apply(b, x)
+        S.synchronize()
+        t = time.perf_counter() - t0
+        r = S.DeviceArray(N)
+        A64.apply(x, r)
+        res = float(np.linalg.norm(rhs - r.numpy()) / np.linalg.norm(rhs))
+        out["solve_" + name] = {"seconds": t, "iterations": solver.iterations, "status": solver.last_status,
+                                "true_rel_residual": res, "setup_seconds": setups[name]}
+    out["speedup_mixed_vs_fp64"] = out["solve_fp64"]["seconds"] / out["solve_mixed"]["seconds"]
+    return out
+
+
 def laplacian_config2(S, args):
     """sshex8 Laplacian L=8 on a 16^3 macro cube (2,146,689 DOF): apply (hot and cold caches, and the
     atomic skeleton option) and the MG-PCG solves."""
@@ -367,9 +454,10 @@ def main():
                          "elasticity solve and the Laplacian workload")
     ap.add_argument("--elasticity-learned", action="store_true",
                     help="also time the learned smoother in the elasticity solves")
-    ap.add_argument("--mode", choices=["apply", "copy", "elasticity-apply"], default="apply",
+    ap.add_argument("--mode", choices=["apply", "copy", "elasticity-apply", "hex8-elasticity"], default="apply",
                     help="copy: stream-copy 2 GiB of FP64 per step (HBM counter calibration for rocprofv3 --pmc); "
-                         "elasticity-apply: only the headline apply (a short run for the profiler)")
+                         "elasticity-apply: only the headline apply (a short run for the profiler); "
+                         "hex8-elasticity: only the 128^3 hex8 elasticity applies and solves")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (this build runs on a single "
@@ -393,6 +481,9 @@ def main():
         print(json.dumps({"mode": "copy", "bytes_per_step": 16 * n, "GB/s": 16 * n / dt / 1e9}))
         return
 
+    if args.mode == "hex8-elasticity":
+        print(json.dumps({"mode": "hex8-elasticity", **hex8_elasticity_config3(S, args)}))
+        return
     if args.mode == "elasticity-apply":
         args.no_solve = args.no_extra = args.no_cpu = True
     h = config5_apply(S, args.steps, args.warmup)
@@ -433,6 +524,7 @@ def main():
     # memory state after the large solves
     if not args.no_extra:
         result["laplacian_config2"] = laplacian_config2(S, args)
+        result["hex8_elasticity_config3"] = hex8_elasticity_config3(S, args)
     if not args.no_solve:
         result["solve_config5"] = elasticity_solve_times(S, n_macro=24, L=16, coeff_seed=0,
                                                          with_learned=args.elasticity_learned)
