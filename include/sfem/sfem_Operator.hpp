@@ -27,6 +27,41 @@ public:
 template <typename T>
 using SharedOperator = std::shared_ptr<Operator<T>>;
 
+// y := f(y) on one vector (algebra/sfem_Operator.hpp:35-42): constraints, scalings, projections
+template <typename T>
+class InPlaceOperator {
+public:
+    virtual ~InPlaceOperator() = default;
+    virtual int apply(T *const y) = 0;
+    virtual std::ptrdiff_t size() const = 0;
+    virtual ExecutionSpace execution_space() const = 0;
+};
+
+template <typename T>
+using SharedInPlaceOperator = std::shared_ptr<InPlaceOperator<T>>;
+
+template <typename T>
+class LambdaInPlaceOperator final : public InPlaceOperator<T> {
+public:
+    std::ptrdiff_t size_{0};
+    std::function<void(T *const)> apply_;
+    ExecutionSpace execution_space_;
+    LambdaInPlaceOperator(std::ptrdiff_t n, std::function<void(T *const)> f, ExecutionSpace es)
+        : size_(n), apply_(std::move(f)), execution_space_(es) {}
+    int apply(T *const y) override {
+        apply_(y);
+        return SFEM_SUCCESS;
+    }
+    std::ptrdiff_t size() const override { return size_; }
+    ExecutionSpace execution_space() const override { return execution_space_; }
+};
+
+template <typename T>
+inline SharedInPlaceOperator<T> make_in_place_op(std::ptrdiff_t n, std::function<void(T *const)> f,
+                                                 ExecutionSpace es) {
+    return std::make_shared<LambdaInPlaceOperator<T>>(n, std::move(f), es);
+}
+
 template <typename T>
 class LambdaOperator final : public Operator<T> {
 public:
@@ -80,6 +115,29 @@ public:
 private:
     SharedOperator<T> l_, r_;
 };
+
+// An in-place operator after an operator (algebra/sfem_Operator.hpp:99-117): r applied to (x, y),
+// then l to y
+template <typename T>
+class InPlaceAfterOperator final : public Operator<T> {
+public:
+    InPlaceAfterOperator(SharedInPlaceOperator<T> l, SharedOperator<T> r) : l_(std::move(l)), r_(std::move(r)) {
+        assert(l_->size() == r_->rows());
+    }
+    int apply(const T *const x, T *const y) override { return r_->apply(x, y) | l_->apply(y); }
+    std::ptrdiff_t rows() const override { return l_->size(); }
+    std::ptrdiff_t cols() const override { return r_->cols(); }
+    ExecutionSpace execution_space() const override { return l_->execution_space(); }
+
+private:
+    SharedInPlaceOperator<T> l_;
+    SharedOperator<T> r_;
+};
+
+template <typename T>
+inline SharedOperator<T> operator*(const SharedInPlaceOperator<T> &l, const SharedOperator<T> &r) {
+    return std::make_shared<InPlaceAfterOperator<T>>(l, r);
+}
 
 template <typename T>
 inline SharedOperator<T> operator+(const SharedOperator<T> &l, const SharedOperator<T> &r) {

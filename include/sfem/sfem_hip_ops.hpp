@@ -33,4 +33,31 @@ inline SharedOperator<T> make_device_op(sfem_amd_op *op, void *stream = nullptr,
     return std::make_shared<DeviceOperator<T>>(op, stream, std::move(keep));
 }
 
+// y[c] = value on the constrained dofs c of a constrained operator (sfem_amd_constrained_op_create),
+// in place: constraint_nodes_to_value_vec (operators/boundary_conditions/dirichlet.cpp:95-106) as an
+// InPlaceOperator, e.g. to zero the Dirichlet rows of a right-hand side on the device
+template <typename T>
+class DeviceConstraintValue final : public InPlaceOperator<T> {
+public:
+    DeviceConstraintValue(std::shared_ptr<DeviceOperator<T>> constrained, double value, void *stream = nullptr)
+        : c_(std::move(constrained)), value_(value), stream_(stream) {}
+    int apply(T *const y) override { return sfem_amd_constrained_op_set_value(c_->handle(), value_, y, stream_); }
+    std::ptrdiff_t size() const override { return c_->rows(); }
+    ExecutionSpace execution_space() const override { return EXECUTION_SPACE_DEVICE; }
+
+private:
+    std::shared_ptr<DeviceOperator<T>> c_;
+    double value_;
+    void *stream_;
+};
+
+// `constrained` must view a constrained operator handle (a make_device_op of sfem_amd_constrained_op_create)
+template <typename T>
+inline SharedInPlaceOperator<T> make_constraint_value_op(const SharedOperator<T> &constrained, double value,
+                                                         void *stream = nullptr) {
+    auto d = std::dynamic_pointer_cast<DeviceOperator<T>>(constrained);
+    if (!d) fatal("make_constraint_value_op: not a device operator");
+    return std::make_shared<DeviceConstraintValue<T>>(std::move(d), value, stream);
+}
+
 }  // namespace sfem

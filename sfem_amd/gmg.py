@@ -64,11 +64,16 @@ class SSGMG:
     macro mesh made by box_mesh(n+1, n+1, n+1, extent, extent, extent)."""
 
     def __init__(self, fine_mesh, kind, geometry, bc_nodes, coeff=None, real_type=FLOAT64, smoother="cheb3",
-                 cheb_degree=3, eig_its=1000, eig_tol=1e-5, scale_eig_max=1.02, coarse_rtol=1e-8,
+                 cheb_degree=3, eig_its=1000, eig_tol=1e-5, scale_eig_max=1.02, coarse_rtol=None,
                  coarse_max_it=10000, mu=None, lam=None, bc_components=None, max_it=1, atol=0.0, macro_box=None,
                  learned=None, learned_levels=1, learned_steps=1, learned_mode=1, learned_damping=1.0,
                  coarse="cg"):
         self.kind = kind
+        # coarse CG tolerance: the reference's default hierarchy (smoother "jacobi",
+        # create_gmg_default_smoothers_and_solver) uses 1e-6, mgsolve's Chebyshev3 driver 1e-8
+        # (drivers/mgsolve.cpp:208-214)
+        if coarse_rtol is None:
+            coarse_rtol = 1e-6 if smoother == "jacobi" else 1e-8
         # coarsest solve: "cg" (the reference's FP64 PCG with inverse-diagonal preconditioning) or
         # "dense" (a precomputed inverse applied as a dense matrix: no host synchronisation, so the
         # V-cycle can be replayed as a HIP graph, see preconditioner(graph=True))

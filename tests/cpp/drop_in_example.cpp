@@ -81,11 +81,12 @@ int main(int argc, char **argv) {
         else
             rhs[(size_t)i] = 1;
     }
-    for (const int32_t b : bc)
-        for (int c = 0; c < bs; c++) rhs[(size_t)b * bs + c] = 0;
     double *d_rhs = (double *)sfem_amd_malloc(sizeof(double) * ndofs);
     double *d_x = (double *)sfem_amd_malloc(sizeof(double) * ndofs);
     sfem_amd_memcpy_h2d(d_rhs, rhs.data(), sizeof(double) * ndofs);
+    // the Dirichlet rows of the right-hand side zeroed on the device by an in-place operator
+    auto zero_bc = sfem::make_constraint_value_op<sfem::real_t>(A, 0.0);
+    if (zero_bc->size() != ndofs || zero_bc->apply(d_rhs) != SFEM_SUCCESS) return 1;
     sfem_amd_memset(d_x, 0, sizeof(double) * ndofs, nullptr);
 
     auto cg = sfem::create_cg<sfem::real_t>(A, sfem::EXECUTION_SPACE_DEVICE);

@@ -166,3 +166,18 @@ def test_cpp_create_ssgmg_mg_pcg_matches_oracle(kind, tmp_path):
     assert abs(its - it_o) <= 1
     assert res < 1e-10
     assert diff < 1e-8
+
+
+def test_cpp_operator_algebra_semantics(tmp_path):
+    """tests/cpp/operator_algebra.cpp (host lambdas only): OperatorAdd, OperatorCompose and an
+    InPlaceOperator composed after an operator follow the reference's rules."""
+    if not os.path.exists(os.path.join(LIBDIR, "libsfem_amd.so")):
+        pytest.skip("libsfem_amd.so not built")
+    exe = os.path.join(str(tmp_path), "operator_algebra")
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "operator_algebra.cpp"), "-L", LIBDIR, "-lsfem_amd",
+           f"-Wl,-rpath,{LIBDIR}", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
