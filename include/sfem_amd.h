@@ -340,6 +340,12 @@ int sfem_amd_op_last_status(const sfem_amd_op *op);
  * weights == NULL: He-normal initialisation from `seed`.  Micro-grid levels 1..8. */
 typedef struct sfem_amd_cnn sfem_amd_cnn;
 int sfem_amd_cnn_create(int hidden, int layers, const float *weights, unsigned seed, sfem_amd_cnn **out);
+/* The same network with `channels` (1 or 3) input and output channels: channels -> hidden -> ... ->
+ * channels.  The vector network smooths linear elasticity, one channel per displacement component;
+ * its activations and the z / out arrays below are [E][channels][(L+1)^3]; it runs the FP32 path. */
+int sfem_amd_cnn_create_channels(int channels, int hidden, int layers, const float *weights, unsigned seed,
+                                 sfem_amd_cnn **out);
+int sfem_amd_cnn_channels(const sfem_amd_cnn *cnn);
 int sfem_amd_cnn_num_params(const sfem_amd_cnn *cnn);
 int sfem_amd_cnn_get_weights(const sfem_amd_cnn *cnn, float *weights_host);
 int sfem_amd_cnn_set_weights(sfem_amd_cnn *cnn, const float *weights_host);
@@ -356,12 +362,14 @@ int sfem_amd_cnn_loss_grad(sfem_amd_cnn *cnn, int level, ptrdiff_t nelements, co
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradient. */
 int sfem_amd_cnn_adam_step(sfem_amd_cnn *cnn, double lr, void *stream);
 /* out[e][v] = vec[idx[e][v]*bs + comp] * (scale ? scale[idx*bs + comp] : 1) as float32, for an
- * sshex8 mesh (device; vec/scale of real type rt). */
+ * sshex8 mesh (device; vec/scale of real type rt).  comp = -1: every component, as the channels of
+ * out [E][bs][(L+1)^3]. */
 int sfem_amd_gather_elements(const sfem_amd_mesh *mesh, int real_type, int bs, int comp, const void *vec,
                              const void *scale, float *out, void *stream);
 /* Learned smoother on an sshex8 level: `steps` times
  *     r = b - A x;  x += damping * avg_e f(D^-1 r |_e)
- * (per component for bs > 1; shared nodes average their elements' corrections; constrained dofs
+ * (for bs > 1: one evaluation of a network with bs channels, or the scalar network on each component;
+ * shared nodes average their elements' corrections; constrained dofs
  * untouched when A is a constrained operator).  diag: device diagonal of A (copied).  The op
  * keeps a shared reference to A; mesh and cnn must outlive it.  mode as sfem_amd_cnn_forward. */
 int sfem_amd_learned_smoother_create(sfem_amd_op *A, const sfem_amd_mesh *mesh, sfem_amd_cnn *cnn,

@@ -6,9 +6,10 @@ element on the (L+1)^3 micro-grid, zero padding outside the element:
     g(z) = conv_n( relu( ... relu( conv_1(z) ) ... ) )      3x3x3 convolutions, no biases
     f(z) = g(z) - g(-z)                                      odd, so f(a z) = a f(z) for all a
 
-Input z = D^-1 r restricted to the element (one channel), output: the element's correction.
-Weights W_l[c_out][c_in][27] with offset k = (dz+1)*9 + (dy+1)*3 + (dx+1), node index
-lidx = z*(L+1)^2 + y*(L+1) + x.  This module gives forward, backward (weights and input) and the
+Input z = D^-1 r restricted to the element, output: the element's correction; one channel for a
+scalar problem ([E][n] arrays), or one channel per component for a vector problem (the 3-channel
+network of linear elasticity, [E][3][n] arrays).  Weights W_l[c_out][c_in][27] with offset
+k = (dz+1)*9 + (dy+1)*3 + (dx+1), node index lidx = z*(L+1)^2 + y*(L+1) + x.  This module gives forward, backward (weights and input) and the
 normalised squared-error loss; tests/test_oracle_cnn.py pins it by finite differences, and the
 GPU kernels are checked against it.
 """
@@ -54,19 +55,20 @@ def conv_backward(x, W, dout, L):
 
 
 def g_forward(z, Ws, L):
-    """z: [E][n]; returns output [E][n] and the cache of pre-activations."""
-    a = z[:, None, :]
+    """z: [E][n] (one channel) or [E][C][n]; returns the output of the same shape and the cache of
+    the layer inputs."""
+    a = z[:, None, :] if z.ndim == 2 else z
     cache = []
     for i, W in enumerate(Ws):
         inp = a if i == 0 else np.maximum(a, 0.0)
         cache.append(inp)
         a = conv(inp, W, L)
-    return a[:, 0, :], cache
+    return (a[:, 0, :] if z.ndim == 2 else a), cache
 
 
 def g_backward(dout, Ws, cache, L):
     grads = [None] * len(Ws)
-    d = dout[:, None, :]
+    d = dout[:, None, :] if dout.ndim == 2 else dout
     for i in reversed(range(len(Ws))):
         inp = cache[i]
         dW, dinp = conv_backward(inp, Ws[i], d, L)
@@ -75,7 +77,7 @@ def g_backward(dout, Ws, cache, L):
             d = dinp * (inp > 0)  # inp = relu(pre); relu' on the pre-activation
         else:
             d = dinp
-    return grads, d[:, 0, :]
+    return grads, (d[:, 0, :] if dout.ndim == 2 else d)
 
 
 def forward(z, Ws, L):
@@ -96,10 +98,11 @@ def loss_and_grads(z, target, Ws, L):
     return loss, [a + b for a, b in zip(gp, gm)]
 
 
-def init_weights(hidden=16, layers=4, seed=0):
-    """He-style initialisation scaled for the 27-point stencils."""
+def init_weights(hidden=16, layers=4, seed=0, channels=1):
+    """He-style initialisation scaled for the 27-point stencils; `channels` input and output
+    channels (1, or 3 for the vector network)."""
     rng = np.random.default_rng(seed)
-    chans = [1] + [hidden] * (layers - 1) + [1]
+    chans = [channels] + [hidden] * (layers - 1) + [channels]
     Ws = []
     for i in range(layers):
         fan_in = chans[i] * 27
@@ -107,6 +110,6 @@ def init_weights(hidden=16, layers=4, seed=0):
     return Ws
 
 
-def n_params(hidden=16, layers=4):
-    chans = [1] + [hidden] * (layers - 1) + [1]
+def n_params(hidden=16, layers=4, channels=1):
+    chans = [channels] + [hidden] * (layers - 1) + [channels]
     return sum(27 * chans[i] * chans[i + 1] for i in range(layers))

@@ -98,6 +98,8 @@ SIGNATURES = {
     "sfem_amd_op_benchmark": (_i, [_vp, _vp, _vp, _i, _i, C.c_size_t, _vp, C.POINTER(_d)]),
     "sfem_amd_element_matrix_op_create": (_i, [_vp, _vp, _i, _i, C.POINTER(_vp)]),
     "sfem_amd_cnn_create": (_i, [_i, _i, _vp, C.c_uint, C.POINTER(_vp)]),
+    "sfem_amd_cnn_create_channels": (_i, [_i, _i, _i, _vp, C.c_uint, C.POINTER(_vp)]),
+    "sfem_amd_cnn_channels": (_i, [_vp]),
     "sfem_amd_cnn_num_params": (_i, [_vp]),
     "sfem_amd_cnn_get_weights": (_i, [_vp, _vp]),
     "sfem_amd_cnn_set_weights": (_i, [_vp, _vp]),
@@ -555,15 +557,16 @@ def solver_bridge(solver):
 
 class CNN:
     """The learned smoother's network (sfem_amd_cnn): 3x3x3 convolutions without biases,
-    1 -> hidden -> ... -> 1 channels with ReLU between, f(z) = g(z) - g(-z) per macro element."""
+    channels -> hidden -> ... -> channels with ReLU between, f(z) = g(z) - g(-z) per macro element;
+    channels 1 (scalar problems) or 3 (linear elasticity, one channel per component)."""
 
-    def __init__(self, hidden=16, layers=4, weights=None, seed=0):
+    def __init__(self, hidden=16, layers=4, weights=None, seed=0, channels=1):
         h = _vp()
         w = None if weights is None else _host(weights, np.float32)
-        _check(lib.sfem_amd_cnn_create(hidden, layers, None if w is None else _np_ptr(w), seed, C.byref(h)),
-               "cnn_create")
+        _check(lib.sfem_amd_cnn_create_channels(channels, hidden, layers, None if w is None else _np_ptr(w), seed,
+                                                C.byref(h)), "cnn_create")
         self.h = h
-        self.hidden, self.layers = hidden, layers
+        self.hidden, self.layers, self.channels = hidden, layers, channels
 
     @property
     def num_params(self):
@@ -581,17 +584,17 @@ class CNN:
         _check(lib.sfem_amd_cnn_set_weights(self.h, _np_ptr(w)), "cnn_set_weights")
 
     def forward(self, level, z, out, mode=0, stream=None):
-        n = (level + 1) ** 3
+        n = (level + 1) ** 3 * self.channels
         if z.numel() % n or out.numel() != z.numel() or z.element_size() != 4 or out.element_size() != 4:
-            raise SfemError("cnn_forward: float32 [E][(L+1)^3] input and output expected")
+            raise SfemError("cnn_forward: float32 [E][channels][(L+1)^3] input and output expected")
         _check(lib.sfem_amd_cnn_forward(self.h, level, z.numel() // n, _dptr(z), _dptr(out), mode,
                                         _stream_ptr(stream)), "cnn_forward")
         return out
 
     def loss_grad(self, level, z, target, want_grad=True, stream=None):
-        n = (level + 1) ** 3
+        n = (level + 1) ** 3 * self.channels
         if z.numel() % n or target.numel() != z.numel():
-            raise SfemError("cnn_loss_grad: float32 [E][(L+1)^3] input and target expected")
+            raise SfemError("cnn_loss_grad: float32 [E][channels][(L+1)^3] input and target expected")
         loss = C.c_double()
         g = np.zeros(self.num_params, np.float32) if want_grad else None
         _check(lib.sfem_amd_cnn_loss_grad(self.h, level, z.numel() // n, _dptr(z), _dptr(target), C.byref(loss),
@@ -608,7 +611,8 @@ class CNN:
 
 
 def gather_elements(mesh, vec, out, block_size=1, component=0, scale=None, stream=None):
-    """out[e][v] = vec[idx[e][v]*bs + comp] (* scale[...]) as float32 (sshex8 meshes)."""
+    """out[e][v] = vec[idx[e][v]*bs + comp] (* scale[...]) as float32 (sshex8 meshes); component -1:
+    all components as channels, out [E][bs][(L+1)^3]."""
     rt = FLOAT64 if vec.element_size() == 8 else FLOAT32
     _check(lib.sfem_amd_gather_elements(mesh.h, rt, block_size, component, _dptr(vec),
                                         None if scale is None else _dptr(scale), _dptr(out), _stream_ptr(stream)),

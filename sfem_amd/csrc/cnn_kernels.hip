@@ -206,27 +206,31 @@ __global__ void adam_kernel(const int n, const float *__restrict__ g, float *__r
     w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// out[e][v] = vec[idx[e][v]*bs + comp] * (scale ? scale[...] : 1)
+// out[e * ld + off + v] = vec[idx[e][v]*bs + comp] * (scale ? scale[...] : 1)  (ld = nxe, off = 0: [E][nxe];
+// ld = bs * nxe, off = comp * nxe: the channel comp of [E][bs][nxe])
 template <typename T>
 __global__ void gather_elements_kernel(const ptrdiff_t E, const int nxe, const int32_t *__restrict__ idx, const int bs,
                                        const int comp, const T *__restrict__ vec, const T *__restrict__ scale,
-                                       float *__restrict__ out) {
+                                       const int ld, const int off, float *__restrict__ out) {
     const ptrdiff_t n = E * nxe;
     for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
         const ptrdiff_t d = (ptrdiff_t)idx[t] * bs + comp;
-        out[t] = (float)(scale ? vec[d] * scale[d] : vec[d]);
+        const ptrdiff_t e = t / nxe;
+        out[e * ld + off + (t - e * nxe)] = (float)(scale ? vec[d] * scale[d] : vec[d]);
     }
 }
 
-// vec[idx[e][v]*bs + comp] += alpha * val[e][v] / count[idx]
+// vec[idx[e][v]*bs + comp] += alpha * val[e * ld + off + v] / count[idx]
 template <typename T>
 __global__ void scatter_elements_kernel(const ptrdiff_t E, const int nxe, const int32_t *__restrict__ idx, const int bs,
                                         const int comp, const uint16_t *__restrict__ count, const float alpha,
-                                        const float *__restrict__ val, T *__restrict__ vec) {
+                                        const int ld, const int off, const float *__restrict__ val,
+                                        T *__restrict__ vec) {
     const ptrdiff_t n = E * nxe;
     for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
         const int32_t g = idx[t];
-        atomicAdd(&vec[(ptrdiff_t)g * bs + comp], (T)(alpha * val[t] / (float)count[g]));
+        const ptrdiff_t e = t / nxe;
+        atomicAdd(&vec[(ptrdiff_t)g * bs + comp], (T)(alpha * val[e * ld + off + (t - e * nxe)] / (float)count[g]));
     }
 }
 
@@ -320,30 +324,34 @@ int cnn_adam(int n, const float *g, float *w, float *m, float *v, float lr, floa
 
 template <typename T>
 int gather_elements(ptrdiff_t E, int nxe, const int32_t *idx, int bs, int comp, const T *vec, const T *scale,
-                    float *out, hipStream_t s) {
+                    float *out, hipStream_t s, bool channels) {
     if (E == 0) return SFEM_SUCCESS;
-    gather_elements_kernel<T><<<grid_for(E * nxe), kBlock, 0, s>>>(E, nxe, idx, bs, comp, vec, scale, out);
+    gather_elements_kernel<T><<<grid_for(E * nxe), kBlock, 0, s>>>(E, nxe, idx, bs, comp, vec, scale,
+                                                                   channels ? bs * nxe : nxe, channels ? comp * nxe : 0,
+                                                                   out);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
 }
 
 template <typename T>
 int scatter_elements(ptrdiff_t E, int nxe, const int32_t *idx, int bs, int comp, const uint16_t *count, float alpha,
-                     const float *val, T *vec, hipStream_t s) {
+                     const float *val, T *vec, hipStream_t s, bool channels) {
     if (E == 0) return SFEM_SUCCESS;
-    scatter_elements_kernel<T><<<grid_for(E * nxe), kBlock, 0, s>>>(E, nxe, idx, bs, comp, count, alpha, val, vec);
+    scatter_elements_kernel<T><<<grid_for(E * nxe), kBlock, 0, s>>>(E, nxe, idx, bs, comp, count, alpha,
+                                                                    channels ? bs * nxe : nxe,
+                                                                    channels ? comp * nxe : 0, val, vec);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
 }
 
 template int gather_elements<double>(ptrdiff_t, int, const int32_t *, int, int, const double *, const double *, float *,
-                                     hipStream_t);
+                                     hipStream_t, bool);
 template int gather_elements<float>(ptrdiff_t, int, const int32_t *, int, int, const float *, const float *, float *,
-                                    hipStream_t);
+                                    hipStream_t, bool);
 template int scatter_elements<double>(ptrdiff_t, int, const int32_t *, int, int, const uint16_t *, float, const float *,
-                                      double *, hipStream_t);
+                                      double *, hipStream_t, bool);
 template int scatter_elements<float>(ptrdiff_t, int, const int32_t *, int, int, const uint16_t *, float, const float *,
-                                     float *, hipStream_t);
+                                     float *, hipStream_t, bool);
 
 // ------------------------------------------------------------------ fused BF16 MFMA inference
 // One macro element per block (512 threads = 8 waves); the element's

@@ -21,12 +21,15 @@ int cnn_sq_err(ptrdiff_t n, const float *o, const float *t, double *ws, double *
 int cnn_dloss(ptrdiff_t n, const float *o, const float *t, float inv_norm, float *dpm, hipStream_t s);
 int cnn_adam(int n, const float *g, float *w, float *m, float *v, float lr, float b1, float b2, float eps, int step,
              hipStream_t s);
+// Element values of component comp of a bs-interleaved vector: out [E][nxe], or with channels the
+// channel comp of out [E][bs][nxe]
 template <typename T>
 int gather_elements(ptrdiff_t E, int nxe, const int32_t *idx, int bs, int comp, const T *vec, const T *scale,
-                    float *out, hipStream_t s);
+                    float *out, hipStream_t s, bool channels = false);
+// vec[node * bs + comp] += alpha * val / count[node] over the element values (layout as gather_elements)
 template <typename T>
 int scatter_elements(ptrdiff_t E, int nxe, const int32_t *idx, int bs, int comp, const uint16_t *count, float alpha,
-                     const float *val, T *vec, hipStream_t s);
+                     const float *val, T *vec, hipStream_t s, bool channels = false);
 // fused BF16 MFMA inference (hidden 16, 4 layers, L <= 8); W16: bf16 weights of the 4 layers in
 // MFMA lane order (kCnnPackedWeights entries, see cnn_pack_weights)
 constexpr int kCnnPackedWeights = 512 + 2 * 7168 + 512;

@@ -11,6 +11,7 @@ using namespace sfem_amd;
 
 struct sfem_amd_cnn {
     int hidden{16}, layers{4};
+    int io{1};                    // input and output channels: 1, or 3 (one per displacement component)
     std::vector<int> chans;       // layer l: chans[l] -> chans[l+1]
     std::vector<size_t> offset;   // weight offset of layer l
     size_t nparams{0};
@@ -43,7 +44,7 @@ int check_level(int L) {
     return SFEM_SUCCESS;
 }
 
-// FP32 forward of g on the batch x2 ([2E][1][nxe]); keeps every layer's output in acts.
+// FP32 forward of g on the batch x2 ([2E][io][nxe]); keeps every layer's output in acts.
 int forward_g(sfem_amd_cnn *c, int L, ptrdiff_t E2, hipStream_t s) {
     const size_t nxe = nxe_of(L);
     const float *in = c->x2.p;
@@ -57,7 +58,7 @@ int forward_g(sfem_amd_cnn *c, int L, ptrdiff_t E2, hipStream_t s) {
 }
 
 int forward_f(sfem_amd_cnn *c, int L, ptrdiff_t E, const float *z, float *out, hipStream_t s) {
-    const size_t n = (size_t)E * nxe_of(L);
+    const size_t n = (size_t)E * c->io * nxe_of(L);
     if (ensure(c->x2, 2 * n)) return SFEM_FAILURE;
     if (cnn_make_pm((ptrdiff_t)n, z, c->x2.p, s)) return SFEM_FAILURE;
     if (forward_g(c, L, 2 * E, s)) return SFEM_FAILURE;
@@ -97,13 +98,20 @@ void cnn_pack_weights(const float *W, uint16_t *packed) {
 extern "C" {
 
 int sfem_amd_cnn_create(int hidden, int layers, const float *weights, unsigned seed, sfem_amd_cnn **out) {
+    return sfem_amd_cnn_create_channels(1, hidden, layers, weights, seed, out);
+}
+
+int sfem_amd_cnn_create_channels(int channels, int hidden, int layers, const float *weights, unsigned seed,
+                                 sfem_amd_cnn **out) {
     SFEM_AMD_REQUIRE(hidden >= 1 && hidden <= 16 && layers >= 2 && layers <= 8, "cnn: hidden in 1..16, layers in 2..8");
+    SFEM_AMD_REQUIRE(channels == 1 || channels == 3, "cnn: 1 or 3 input/output channels");
     auto c = std::make_unique<sfem_amd_cnn>();
     c->hidden = hidden;
     c->layers = layers;
-    c->chans.push_back(1);
+    c->io = channels;
+    c->chans.push_back(channels);
     for (int l = 0; l < layers - 1; l++) c->chans.push_back(hidden);
-    c->chans.push_back(1);
+    c->chans.push_back(channels);
     size_t off = 0;
     for (int l = 0; l < layers; l++) {
         c->offset.push_back(off);
@@ -130,6 +138,7 @@ int sfem_amd_cnn_create(int hidden, int layers, const float *weights, unsigned s
 }
 
 int sfem_amd_cnn_num_params(const sfem_amd_cnn *c) { return c ? (int)c->nparams : -1; }
+int sfem_amd_cnn_channels(const sfem_amd_cnn *c) { return c ? c->io : -1; }
 
 int sfem_amd_cnn_get_weights(const sfem_amd_cnn *c, float *w) {
     SFEM_AMD_REQUIRE(c && w, "cnn_get_weights: arguments");
@@ -152,7 +161,8 @@ int sfem_amd_cnn_forward(sfem_amd_cnn *c, int L, ptrdiff_t E, const float *z, fl
     if (check_level(L)) return SFEM_FAILURE;
     hipStream_t s = as_stream(stream);
     if (mode == 1) {
-        SFEM_AMD_REQUIRE(c->hidden == 16 && c->layers == 4, "cnn_forward: the MFMA path is built for 4 layers of 16");
+        SFEM_AMD_REQUIRE(c->hidden == 16 && c->layers == 4 && c->io == 1,
+                         "cnn_forward: the MFMA path is built for the scalar network of 4 layers of 16");
         if (!c->w16_valid) {
             std::vector<float> h(c->nparams);
             SFEM_AMD_HIP_CHECK(hipMemcpy(h.data(), c->W.p, sizeof(float) * c->nparams, hipMemcpyDeviceToHost));
@@ -174,7 +184,7 @@ int sfem_amd_cnn_loss_grad(sfem_amd_cnn *c, int L, ptrdiff_t E, const float *z, 
     SFEM_AMD_REQUIRE(c && z && target && loss, "cnn_loss_grad: arguments");
     if (check_level(L)) return SFEM_FAILURE;
     hipStream_t s = as_stream(stream);
-    const size_t nxe = nxe_of(L), n = (size_t)E * nxe;
+    const size_t nxe = nxe_of(L), n = (size_t)E * c->io * nxe;
     if (ensure(c->out2, n) || ensure(c->dpm, 2 * n)) return SFEM_FAILURE;
     if (forward_f(c, L, E, z, c->out2.p, s)) return SFEM_FAILURE;
     double err = 0, norm = 0;
@@ -219,12 +229,17 @@ int sfem_amd_cnn_adam_step(sfem_amd_cnn *c, double lr, void *stream) {
 int sfem_amd_gather_elements(const sfem_amd_mesh *mesh, int rt, int bs, int comp, const void *vec, const void *scale,
                              float *out, void *stream) {
     SFEM_AMD_REQUIRE(mesh && vec && out && mesh->type == SFEM_AMD_SSHEX8, "gather_elements: sshex8 mesh and vectors");
-    SFEM_AMD_REQUIRE(comp >= 0 && comp < bs, "gather_elements: component");
-    if (rt == SFEM_AMD_FLOAT64)
-        return gather_elements<double>(mesh->E, mesh->nv, mesh->idx.p, bs, comp, (const double *)vec,
-                                       (const double *)scale, out, as_stream(stream));
-    return gather_elements<float>(mesh->E, mesh->nv, mesh->idx.p, bs, comp, (const float *)vec, (const float *)scale, out,
-                                  as_stream(stream));
+    SFEM_AMD_REQUIRE((comp >= 0 && comp < bs) || comp == -1, "gather_elements: component");
+    const int c0 = comp < 0 ? 0 : comp, c1 = comp < 0 ? bs : comp + 1;
+    for (int c = c0; c < c1; c++) {
+        const int err = rt == SFEM_AMD_FLOAT64
+                                ? gather_elements<double>(mesh->E, mesh->nv, mesh->idx.p, bs, c, (const double *)vec,
+                                                          (const double *)scale, out, as_stream(stream), comp < 0)
+                                : gather_elements<float>(mesh->E, mesh->nv, mesh->idx.p, bs, c, (const float *)vec,
+                                                         (const float *)scale, out, as_stream(stream), comp < 0);
+        if (err) return err;
+    }
+    return SFEM_SUCCESS;
 }
 
 }  // extern "C"
@@ -253,11 +268,25 @@ struct LearnedSmootherOp final : OpBase {
             if (A->apply_overwrite(x, r.p, s)) return SFEM_FAILURE;
             if (blas_axpby<T>(n, T(1), b, T(-1), r.p, s)) return SFEM_FAILURE;  // r = b - A x
             if (blas_values<T>(n, T(0), corr.p, s)) return SFEM_FAILURE;
-            for (int c = 0; c < bs; c++) {
-                if (gather_elements<T>(mesh->E, mesh->nv, mesh->idx.p, bs, c, r.p, invdiag.p, z.p, s)) return SFEM_FAILURE;
+            if (cnn->io == bs && bs > 1) {
+                // the vector network: all components as channels of one evaluation
+                for (int c = 0; c < bs; c++)
+                    if (gather_elements<T>(mesh->E, mesh->nv, mesh->idx.p, bs, c, r.p, invdiag.p, z.p, s, true))
+                        return SFEM_FAILURE;
                 if (sfem_amd_cnn_forward(cnn, mesh->level, mesh->E, z.p, out.p, mode, s)) return SFEM_FAILURE;
-                if (scatter_elements<T>(mesh->E, mesh->nv, mesh->idx.p, bs, c, mesh->count.p, 1.f, out.p, corr.p, s))
-                    return SFEM_FAILURE;
+                for (int c = 0; c < bs; c++)
+                    if (scatter_elements<T>(mesh->E, mesh->nv, mesh->idx.p, bs, c, mesh->count.p, 1.f, out.p, corr.p, s,
+                                            true))
+                        return SFEM_FAILURE;
+            } else {
+                // the scalar network on each component in turn
+                for (int c = 0; c < bs; c++) {
+                    if (gather_elements<T>(mesh->E, mesh->nv, mesh->idx.p, bs, c, r.p, invdiag.p, z.p, s))
+                        return SFEM_FAILURE;
+                    if (sfem_amd_cnn_forward(cnn, mesh->level, mesh->E, z.p, out.p, mode, s)) return SFEM_FAILURE;
+                    if (scatter_elements<T>(mesh->E, mesh->nv, mesh->idx.p, bs, c, mesh->count.p, 1.f, out.p, corr.p, s))
+                        return SFEM_FAILURE;
+                }
             }
             if (constrained && constrained_set_value(A.get(), rt, 0.0, corr.p, s)) return SFEM_FAILURE;
             if (blas_axpby<T>(n, (T)damping, corr.p, T(1), x, s)) return SFEM_FAILURE;
@@ -275,6 +304,8 @@ extern "C" int sfem_amd_learned_smoother_create(sfem_amd_op *A, const sfem_amd_m
     SFEM_AMD_REQUIRE(mesh->type == SFEM_AMD_SSHEX8, "learned_smoother: sshex8 level required");
     if (check_level(mesh->level)) return SFEM_FAILURE;
     SFEM_AMD_REQUIRE(A->impl->rows() == mesh->nnodes * bs, "learned_smoother: operator size != mesh dofs");
+    SFEM_AMD_REQUIRE(cnn->io == 1 || cnn->io == bs, "learned_smoother: network channels must be 1 or the block size");
+    SFEM_AMD_REQUIRE(cnn->io == 1 || mode == 0, "learned_smoother: the vector network runs the FP32 path (mode 0)");
     if (mesh_counts(mesh, nullptr)) return SFEM_FAILURE;
     auto make = [&](auto op) -> int {
         using T = typename std::remove_reference<decltype(*op)>::type;
@@ -288,7 +319,7 @@ extern "C" int sfem_amd_learned_smoother_create(sfem_amd_op *A, const sfem_amd_m
         op->mode = mode;
         op->constrained = is_constrained(A->impl.get());
         const ptrdiff_t n = A->impl->rows();
-        const size_t ne = (size_t)mesh->E * mesh->nv;
+        const size_t ne = (size_t)mesh->E * mesh->nv * cnn->io;
         if (op->r.alloc(n) || op->corr.alloc(n) || op->invdiag.alloc(n) || op->z.alloc(ne) || op->out.alloc(ne))
             return SFEM_FAILURE;
         *out = wrap_op(op);
