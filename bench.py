@@ -233,14 +233,23 @@ def elasticity_solve_times(S, n_macro=32, L=8, rtol=1e-10, with_learned=False, c
     setups["mixed"] = time.perf_counter() - t0
     cases.append(("mixed", h32, S.FLOAT32))
     if with_learned:
-        progress("elasticity: training the learned smoother")
+        # the 3-channel network acts on the finest level with micro-grid level <= 8; its training pairs
+        # come from the plain FP32 V-cycle started on that level
+        progress("elasticity: training the learned smoother (3-channel network)")
         t0 = time.perf_counter()
-        data = learned.collect_pairs(h32, n_problems=2, cycles=2, seed=0)
-        cnn = S.CNN(16, 4, seed=0)
-        learned.train(cnn, data, steps=300, lr=2e-3, batch_elements=16, seed=0)
+        if L > 8:
+            lm = mesh.derefine(8)
+            ht = gmg.SSGMG(lm, "elasticity", (adj, det), bc[bc < lm.nnodes], real_type=S.FLOAT32, **common)
+        else:
+            ht = h32
+        data = learned.collect_pairs(ht, n_problems=2, cycles=2, seed=0, channels=3)
+        del ht
+        cnn = S.CNN(16, 4, seed=0, channels=3)
+        hist = learned.train(cnn, data, steps=300, lr=2e-3, batch_elements=16, seed=0)
         hl = gmg.SSGMG(mesh, "elasticity", (adj, det), bc, real_type=S.FLOAT32, learned=cnn, learned_steps=1,
                        **common)
         setups["mixed_learned"] = time.perf_counter() - t0
+        out["learned_training_loss_last20"] = float(np.mean(hist[-20:]))
         cases.append(("mixed_learned", hl, S.FLOAT32))
 
     def true_res(x):
@@ -268,6 +277,7 @@ def elasticity_solve_times(S, n_macro=32, L=8, rtol=1e-10, with_learned=False, c
                      "true_rel_residual": true_res(x), "setup_seconds": setups[name]}
         if h.switches:
             out[name]["fallback_active"] = int(S.switch_state(h.switches[0])[0])
+            out[name]["fallback_events"] = int(S.switch_state(h.switches[0])[1])
     out["speedup_mixed_vs_fp64"] = out["fp64"]["seconds"] / out["mixed"]["seconds"]
     if with_learned:
         out["speedup_mixed_learned_vs_fp64"] = out["fp64"]["seconds"] / out["mixed_learned"]["seconds"]
@@ -393,6 +403,106 @@ def hex8_elasticity_config3(S, args, n=128, rtol=1e-10):
     return out
 
 
+def learned_config4(S, args, n_macro=32, L=8, n_train=32, train_macro=8, held_out=(1000, 1001), rtol=1e-10):
+    """Variable-coefficient sshex8 Poisson, level 8 on a 32^3 macro unit cube (16,974,593 DOF), zero
+    Dirichlet on all faces, RHS U[0,1]; per-macro-element coefficients log-normal with log10 contrast
+    U[0,3] (learned.coefficient_field).  The learned-smoother protocol: training pairs (D^-1 r, e)
+    after one plain V-cycle (FP32 hierarchy, Chebyshev3 on D^-1 A) on n_train seeded problems of an
+    8^3-macro copy (the network acts per macro element, so its training problems need not be the full
+    mesh; the full protocol takes 2,000), Adam on the project's kernels; then on held-out seeds at full
+    size, the time to true relative residual rtol of FP64 MG-PCG, of the FP32 hierarchy in FP64
+    flexible CG, and of the latter with the trained network (2 steps) on the finest level.  Every
+    hierarchy continues below the macro mesh (super levels), dense coarse solve, graph-replayed
+    V-cycles."""
+    from sfem_amd import gmg, learned
+    progress(f"config 4: training set ({n_train} problems on a {train_macro}^3 macro copy)")
+    t0 = time.perf_counter()
+    tmel, tmpts = S.box_mesh(train_macro + 1, train_macro + 1, train_macro + 1, element_type=S.HEX8)
+    tel, tpts = S.sshex8_refine(L, tmel, tmpts)
+    tfff = S.fff(S.HEX8, 1, tmel, tmpts)
+    tmesh = S.Mesh(S.SSHEX8, L, tel, tpts.shape[1])
+    tbc = np.where((tpts.min(0) < 1e-6) | (tpts.max(0) > 1 - 1e-6))[0].astype(np.int32)
+
+    def build_train(seed):
+        k, _ = learned.coefficient_field(seed, train_macro ** 3)
+        return gmg.SSGMG(tmesh, "laplacian", tfff, tbc, coeff=k, real_type=S.FLOAT32, smoother="cheb3_jacobi",
+                         macro_box=(train_macro, 1.0))
+
+    data = learned.protocol_pairs(build_train, range(n_train), cycles=1)
+    t_pairs = time.perf_counter() - t0
+    cnn = S.CNN(16, 4, seed=0)
+    hist = learned.train(cnn, data, steps=600, lr=2e-3, batch_elements=32, seed=0)
+    t_train = time.perf_counter() - t0 - t_pairs
+    test_small = learned.protocol_pairs(build_train, [s + 5000 for s in held_out], cycles=1)
+    out = {"workload": f"sshex8 Poisson L={L} on a {n_macro}^3 macro cube, log-normal per-macro-element coefficient "
+                       "(log10 contrast U[0,3]), zero Dirichlet on all faces, RHS U[0,1] (seeded)",
+           "training": {"problems": n_train, "of_protocol": 2000, "train_mesh": f"{train_macro}^3 macro, L={L}",
+                        "pairs": len(data), "pairs_seconds": t_pairs, "train_seconds": t_train, "adam_steps": 600,
+                        "loss_last20": float(np.mean(hist[-20:])),
+                        "held_out_loss": learned.evaluate(cnn, test_small)}}
+    del data, test_small
+    progress(f"config 4: {n_macro}^3 macro mesh")
+    mel, mpts = S.box_mesh(n_macro + 1, n_macro + 1, n_macro + 1, element_type=S.HEX8)
+    mesh, pts_d = S.sshex8_refine_device(L, mel, mpts)
+    fff = S.fff(S.HEX8, 1, mel, mpts)
+    N = mesh.nnodes
+    p = pts_d.numpy().reshape(3, N)
+    bc = np.where((p.min(0) < 1e-6) | (p.max(0) > 1 - 1e-6))[0].astype(np.int32)
+    del pts_d, p
+    out["dofs"] = N
+    box = (n_macro, 1.0)
+    rows = []
+    for seed in held_out:
+        k, c = learned.coefficient_field(seed, n_macro ** 3)
+        rhs = np.random.default_rng(seed).uniform(0, 1, N)
+        rhs[bc] = 0
+        b = S.DeviceArray.from_numpy(rhs)
+        common = dict(coeff=k, smoother="cheb3_jacobi", macro_box=box, coarse="dense")
+        row = {"seed": seed, "log10_contrast": c}
+        h64 = A64 = None
+        for name in ("fp64", "mixed", "mixed_learned"):
+            progress(f"config 4 seed {seed}: {name}")
+            t0 = time.perf_counter()
+            if name == "fp64":
+                h = h64 = gmg.SSGMG(mesh, "laplacian", fff, bc, **common)
+                A64 = h64.fine.A
+                solver = S.cg(A64, h.preconditioner(graph=True), rtol=rtol, max_it=500, stop_on_residual=True)
+            elif name == "mixed":
+                h = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, **common)
+                solver = S.cg(A64, h.preconditioner(graph=True), flexible=True, rtol=rtol, max_it=500)
+            else:
+                h = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, learned=cnn, learned_steps=2, **common)
+                solver = learned.flexible_cg_with_fallback(A64, h.preconditioner(graph=True), h.switches, rtol=rtol,
+                                                           max_it=500)
+            setup = time.perf_counter() - t0
+            x = S.DeviceArray(N)
+            solver.apply(b, x)  # warm-up (captures the graph)
+            h.select_smoothers(0)
+            ev0 = sum(S.switch_state(sw)[1] for sw in h.switches)
+            x.zero()
+            S.synchronize()
+            t0 = time.perf_counter()
+            solver.apply(b, x)
+            S.synchronize()
+            t = time.perf_counter() - t0
+            r = S.DeviceArray(N)
+            A64.apply(x, r)
+            res = float(np.linalg.norm(rhs - r.numpy()) / np.linalg.norm(rhs))
+            row[name] = {"seconds": t, "iterations": solver.iterations, "status": solver.last_status,
+                         "true_rel_residual": res, "setup_seconds": setup}
+            if h.switches:
+                row[name]["fallback_events"] = int(sum(S.switch_state(sw)[1] for sw in h.switches) - ev0)
+            del solver, h
+        del h64, A64
+        row["speedup_mixed_vs_fp64"] = row["fp64"]["seconds"] / row["mixed"]["seconds"]
+        row["speedup_mixed_learned_vs_fp64"] = row["fp64"]["seconds"] / row["mixed_learned"]["seconds"]
+        rows.append(row)
+    out["held_out"] = rows
+    out["mean_speedup_mixed_learned_vs_fp64"] = float(np.mean([r["speedup_mixed_learned_vs_fp64"] for r in rows]))
+    out["mean_speedup_mixed_vs_fp64"] = float(np.mean([r["speedup_mixed_vs_fp64"] for r in rows]))
+    return out
+
+
 def laplacian_config2(S, args):
     """sshex8 Laplacian L=8 on a 16^3 macro cube (2,146,689 DOF): apply (hot and cold caches, and the
     atomic skeleton option) and the MG-PCG solves."""
@@ -452,12 +562,14 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="only the headline apply (and its solves unless --no-solve): skip the ~50M DOF "
                          "elasticity solve and the Laplacian workload")
-    ap.add_argument("--elasticity-learned", action="store_true",
-                    help="also time the learned smoother in the elasticity solves")
-    ap.add_argument("--mode", choices=["apply", "copy", "elasticity-apply", "hex8-elasticity"], default="apply",
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned smoother (3-channel network) in the elasticity solves")
+    ap.add_argument("--mode", choices=["apply", "copy", "elasticity-apply", "hex8-elasticity", "learned"],
+                    default="apply",
                     help="copy: stream-copy 2 GiB of FP64 per step (HBM counter calibration for rocprofv3 --pmc); "
                          "elasticity-apply: only the headline apply (a short run for the profiler); "
-                         "hex8-elasticity: only the 128^3 hex8 elasticity applies and solves")
+                         "hex8-elasticity: only the 128^3 hex8 elasticity applies and solves; "
+                         "learned: only the variable-coefficient learned-smoother protocol")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (this build runs on a single "
@@ -481,6 +593,9 @@ def main():
         print(json.dumps({"mode": "copy", "bytes_per_step": 16 * n, "GB/s": 16 * n / dt / 1e9}))
         return
 
+    if args.mode == "learned":
+        print(json.dumps({"mode": "learned", **learned_config4(S, args)}))
+        return
     if args.mode == "hex8-elasticity":
         print(json.dumps({"mode": "hex8-elasticity", **hex8_elasticity_config3(S, args)}))
         return
@@ -525,11 +640,13 @@ def main():
     if not args.no_extra:
         result["laplacian_config2"] = laplacian_config2(S, args)
         result["hex8_elasticity_config3"] = hex8_elasticity_config3(S, args)
+        if not args.no_solve:
+            result["learned_config4"] = learned_config4(S, args)
     if not args.no_solve:
         result["solve_config5"] = elasticity_solve_times(S, n_macro=24, L=16, coeff_seed=0,
-                                                         with_learned=args.elasticity_learned)
+                                                         with_learned=not args.no_learned)
         if not args.no_extra:
-            result["solve_elasticity_l8"] = elasticity_solve_times(S, with_learned=args.elasticity_learned)
+            result["solve_elasticity_l8"] = elasticity_solve_times(S, with_learned=not args.no_learned)
     if not args.no_cpu:
         progress("cpu baseline (elasticity slab)")
         cb = cpu_baseline("elasticity", 2, 10.0)
