@@ -1370,15 +1370,21 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
                   NT = S::NT, NC = S::NC;
     constexpr int XR = 2 * G;      // x planes in LDS: G being read, G arriving
     constexpr int IR = 3 * G + 1;  // node-id ring: planes z - 1 .. z + 3G - 1 of a step starting at z
-    __shared__ double stab[S::NTYPES][NC];  // the nine kinds' tables (S_0lo read from here)
-    __shared__ double sx[XR][3][P2 + 1];    // halo-padded planes (+ a spare cell for idle lanes)
+    // LDS: the per-kind tables (S_m, S_0, S_p of the nine row kinds) are only read once into each
+    // lane's registers, so they share their space with the x planes and the id ring that follow them
+    constexpr size_t SX_BYTES = sizeof(double) * XR * 3 * (P2 + 1), SG_BYTES = sizeof(int32_t) * IR * S::BLOCK,
+                     ST_BYTES = sizeof(double) * S::NTYPES * 243;
+    constexpr size_t U_BYTES = ST_BYTES > SX_BYTES + SG_BYTES ? ST_BYTES : SX_BYTES + SG_BYTES;
+    __shared__ __align__(16) unsigned char ubuf[U_BYTES];
+    __shared__ double slo[S::NTYPES][81];  // S_0lo of the nine kinds (the z = 0 and z = L planes)
     __shared__ double sK[576];
-    __shared__ int32_t sg[IR][S::BLOCK];    // this thread's node ids of planes z, by z % IR
+    double(*stab)[243] = reinterpret_cast<double(*)[243]>(ubuf);
+    double(*sx)[3][P2 + 1] = reinterpret_cast<double(*)[3][P2 + 1]>(ubuf);           // halo-padded planes
+    int32_t(*sg)[S::BLOCK] = reinterpret_cast<int32_t(*)[S::BLOCK]>(ubuf + SX_BYTES);  // ids of planes z, by z % IR
 
     const ptrdiff_t e = blockIdx.x;
     const int tid = threadIdx.x, r = tid >> 4, k = tid & 15;
     for (int i = tid; i < 576; i += S::BLOCK) sK[i] = Kall[e * 576 + i];
-    for (int i = tid; i < XR * 3 * (P2 + 1); i += S::BLOCK) (&sx[0][0][0])[i] = 0;  // the halo stays zero
 
     // this lane's column and its row's kind
     int cx = 1, cy = 1, kind = -1;
@@ -1417,31 +1423,41 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
     // value (into LDS or the stored data) before it issues its stores, and carries no loaded
     // register into the next step (node ids travel through the thread's own LDS ring).
     TIO x0[G][3];
+    int32_t g0[2 * G];
 #pragma unroll
     for (int g = 0; g < G; g++) {
-        const int32_t ga = row[(g <= L ? g : L) * PLANE], gb = row[(G + g <= L ? G + g : L) * PLANE];
-        sg[g % IR][tid] = ga;
-        sg[(G + g) % IR][tid] = gb;
+        g0[g] = row[(g <= L ? g : L) * PLANE];
+        g0[G + g] = row[(G + g <= L ? G + g : L) * PLANE];
 #pragma unroll
-        for (int c = 0; c < 3; c++) x0[g][c] = x[(ptrdiff_t)ga * 3 + c];
+        for (int c = 0; c < 3; c++) x0[g][c] = x[(ptrdiff_t)g0[g] * 3 + c];
     }
-    __syncthreads();  // K staged, planes zeroed
-    for (int t = tid; t < S::NTYPES * NC; t += S::BLOCK) {
-        const int kd = t / NC, n = t - kd * NC, b = n / 81, q = n - b * 81, nb = q / 9;
-        stab[kd][n] = st_coefficient_q(sK, b, nb % 3 - 1, nb / 3 - 1, (q % 9) / 3, q % 3, st_kind_qmask(kd));
+    __syncthreads();  // K staged
+    for (int t = tid; t < S::NTYPES * 324; t += S::BLOCK) {
+        const int kd = t / 324, n = t - kd * 324, b = n / 81, q = n - b * 81, nb = q / 9;
+        const double v = st_coefficient_q(sK, b, nb % 3 - 1, nb / 3 - 1, (q % 9) / 3, q % 3, st_kind_qmask(kd));
+        if (b < 3)
+            stab[kd][n] = v;
+        else
+            slo[kd][q] = v;
     }
     __syncthreads();  // tables ready
     double tab[NT];
-    const double *lo = stab[kind >= 0 ? kind : 0] + 3 * 81;  // S_0lo of this row's kind
+    const double *lo = slo[kind >= 0 ? kind : 0];  // S_0lo of this row's kind
 #pragma unroll
     for (int m = 0; m < NT; m++) {
         const int n = m * 16 + k;
         tab[m] = (kind >= 0 && n < 3 * 81) ? stab[kind][n] : 0.0;
     }
+    __syncthreads();  // the tables' space becomes the x planes and the id ring
+    for (int i = tid; i < XR * 3 * (P2 + 1); i += S::BLOCK) (&sx[0][0][0])[i] = 0;  // the halo stays zero
+    __syncthreads();
 #pragma unroll
-    for (int g = 0; g < G; g++)
+    for (int g = 0; g < G; g++) {
+        sg[g % IR][tid] = g0[g];
+        sg[(G + g) % IR][tid] = g0[G + g];
 #pragma unroll
         for (int c = 0; c < 3; c++) sx[g][c][active ? me : P2] = (double)x0[g][c];
+    }
 
     double acc_lo[3] = {0, 0, 0}, acc_mid[3] = {0, 0, 0}, acc_hi[3] = {0, 0, 0};
     for (int p0 = 0; p0 <= L; p0 += G) {

@@ -208,10 +208,12 @@ class SSGMG:
 
     def _finish(self, coarse_rtol, coarse_max_it, max_it, atol):
         cfg = self._learned
-        for i, lv in enumerate(self.levels[:-1]):
+        eligible = 0  # the learned smoother goes on the `levels` finest sshex8 levels of micro-grid level <= 8
+        for lv in self.levels[:-1]:
             lv.smoother = self._make_smoother(lv)
-            if (cfg["cnn"] is not None and i < cfg["levels"] and lv.mesh.element_type == SSHEX8
-                    and lv.mesh.level <= 8):
+            if (cfg["cnn"] is not None and lv.mesh.element_type == SSHEX8 and lv.mesh.level <= 8
+                    and eligible < cfg["levels"]):
+                eligible += 1
                 from .learned import make_learned_smoother
                 lv.cheb = lv.smoother
                 lv.learned = make_learned_smoother(lv, cfg["cnn"], steps=cfg["steps"], damping=cfg["damping"],
