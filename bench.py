@@ -43,6 +43,7 @@ def progress(msg):
 
 PROFILE_LAPLACIAN = os.path.join(ROOT, "profiles", "sshex8_laplacian_apply_L8_16cube.md")
 PROFILE_ELASTICITY = os.path.join(ROOT, "profiles", "sshex8_elasticity_apply_L16_24cube.md")
+PROFILE_HEX8 = os.path.join(ROOT, "profiles", "hex8_elasticity_apply_128cube.md")
 
 
 def profiled_traffic(path):
@@ -357,6 +358,9 @@ def hex8_elasticity_config3(S, args, n=128, rtol=1e-10):
         out["apply_" + name] = {"ms_per_apply": ms, "dof_per_s": N / (ms * 1e-3), "algorithmic_bytes": B,
                                 "GB_per_s": B / (ms * 1e-3) / 1e9, "hbm_frac": B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                 "first_apply_seconds": first}
+    out["apply_fp64"]["traffic"] = profiled_traffic(PROFILE_HEX8)
+    out["apply_fp64"]["traffic_source"] = ("profiles/hex8_elasticity_apply_128cube.md (rocprofv3 --pmc FETCH_SIZE, "
+                                           "WRITE_SIZE per dispatch, patch kernel + gather, copy-calibrated)")
     out["apply_fp32"]["rel_error_vs_fp64"] = float(np.linalg.norm(ys["fp32"] - ys["fp64"]) /
                                                    np.linalg.norm(ys["fp64"]))
     out["apply_fp32"]["note"] = ("FP32 sum-factorised arithmetic on bf16 geometry words; error against the FP64 apply, "

@@ -20,9 +20,10 @@
  *   sshex8 apply .......... brute-force dense assembly of the refined hex8 mesh
  *   transfers ............. constants prolongate exactly, restriction == P^T,
  *                           Galerkin identity (algebra/sfem_Multigrid.hpp:112-173)
- *   sshex8 numbering ...... invariants only (prefix property, counts, coordinates);
- *                           the live numbering lives in the absent smesh submodule:
- *                           parity unpinned against it.
+ *   sshex8 numbering ...... known-answer tables derived by hand from the in-tree text
+ *                           (mesh/sfem_hex8_mesh_graph.cpp:346-735,994-1060) for one, two and
+ *                           permuted elements at L = 2, 3, 4, plus invariants (prefix property,
+ *                           counts, coordinates); the live smesh numbering is absent here.
  */
 #ifndef SFEM_ORACLE_H
 #define SFEM_ORACLE_H
