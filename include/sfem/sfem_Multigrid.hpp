@@ -32,6 +32,7 @@ public:
 
     std::shared_ptr<BLAS<T>> blas_;
     bool verbose{false};
+    bool debug{false};  // with verbose: the residual norm of every level's cycle, not only the finest
     std::vector<double> residual_history;  // finest residual norm per cycle
     // Called after each cycle with (cycle, finest residual norm); may replace smoothers.
     std::function<void(int, double)> on_cycle;
@@ -147,6 +148,8 @@ private:
                 if (verbose) std::printf("Multigrid %d\t%g\n", iterations_, nr);
                 if (on_cycle) on_cycle(iterations_, nr);
                 if (nr < (double)atol_) return CYCLE_CONVERGED;
+            } else if (level > 0 && verbose && debug) {
+                std::printf("  level %d residual %g\n", level, (double)blas_->norm2(mem.size(), mem.work->data()));
             }
             // restriction
             if (!restriction_overwrites_output) blas_->zeros(coarse.rhs->size(), coarse.rhs->data());

@@ -54,6 +54,8 @@ public:
     void set_atol(T v) { atol = v; }
     void set_rtol(T v) { rtol = v; }
     void set_verbose(bool v) { verbose = v; }
+    // verbose output every check_each iterations (and the first), as algebra/sfem_cg.hpp:187
+    bool report(int it) const { return it == 1 || check_each <= 1 || it % check_each == 0; }
     void set_op(const SharedOperator<T> &op) override {
         apply_op = op;
         n_dofs = op->rows();
@@ -65,6 +67,14 @@ public:
     std::ptrdiff_t cols() const override { return n_dofs; }
     ExecutionSpace execution_space() const override { return EXECUTION_SPACE_DEVICE; }
 
+    // apply(n, b, x) (algebra/sfem_cg.hpp:193): the same solve on the first n entries
+    int apply(const std::ptrdiff_t n, const T *const b, T *const x) {
+        const std::ptrdiff_t keep = n_dofs;
+        n_dofs = n;
+        const int ret = apply(b, x);
+        n_dofs = keep;
+        return ret;
+    }
     int apply(const T *const b, T *const x) override {
         if (n_dofs < 0 || !apply_op || !blas) return SFEM_FAILURE;
         history.clear();
@@ -149,7 +159,7 @@ private:
             blas->axpby(n, T(1), r, beta, p);
             const T r_norm = std::sqrt(rtr_new);
             history.push_back((double)(r_norm / r_norm0));
-            if (verbose) std::printf("%d: residual abs: %g, rel: %g\n", iterations_ + 1, (double)r_norm, (double)(r_norm / r_norm0));
+            if (verbose && report(iterations_ + 1)) std::printf("%d: residual abs: %g, rel: %g\n", iterations_ + 1, (double)r_norm, (double)(r_norm / r_norm0));
             // the reference breaks without incrementing its counter (:351-355)
             if (r_norm < atol || rtr_new == 0 || r_norm / r_norm0 < rtol) return SFEM_SUCCESS;
         }
@@ -198,7 +208,7 @@ private:
             const T anorm = std::sqrt(std::abs(rtz));
             const T rnorm = anorm / std::sqrt(rtz0);
             history.push_back((double)rnorm);
-            if (verbose) std::printf("%d: sqrt(r.z) abs: %g, rel: %g\n", iterations_ + 1, (double)anorm, (double)rnorm);
+            if (verbose && report(iterations_ + 1)) std::printf("%d: sqrt(r.z) abs: %g, rel: %g\n", iterations_ + 1, (double)anorm, (double)rnorm);
             if (anorm < atol || rnorm < rtol) return SFEM_SUCCESS;
         }
         return SFEM_FAILURE;
@@ -234,7 +244,7 @@ private:
             const T rprev = rnorm;
             rnorm = std::sqrt(rtr);
             history.push_back((double)(rnorm / ref));
-            if (verbose) std::printf("%d: residual rel: %g\n", iterations_ + 1, (double)(rnorm / ref));
+            if (verbose && report(iterations_ + 1)) std::printf("%d: residual rel: %g\n", iterations_ + 1, (double)(rnorm / ref));
             if (on_iteration) on_iteration(iterations_, (double)rprev, (double)rnorm);
             if (rnorm / ref < rtol || rnorm < atol) {
                 // confirm with the true residual (recomputed in the operator's precision)

@@ -93,7 +93,7 @@ int main(int argc, char **argv) {
     cg->set_preconditioner_op(mg);
     cg->set_rtol(1e-12);
     cg->set_max_it(500);
-    const int status = cg->apply(d_rhs, d_x);
+    const int status = cg->apply(ndofs, d_rhs, d_x);  // apply(n, b, x), as the reference's drivers call it
 
     std::vector<double> x((size_t)ndofs);
     sfem_amd_memcpy_d2h(x.data(), d_x, sizeof(double) * ndofs);
