@@ -67,8 +67,9 @@ class SSGMG:
                  cheb_degree=3, eig_its=1000, eig_tol=1e-5, scale_eig_max=1.02, coarse_rtol=None,
                  coarse_max_it=10000, mu=None, lam=None, bc_components=None, max_it=1, atol=0.0, macro_box=None,
                  learned=None, learned_levels=1, learned_steps=1, learned_mode=1, learned_damping=1.0,
-                 coarse="cg"):
+                 coarse="cg", cycle_type=1):
         self.kind = kind
+        self.cycle_type = cycle_type  # 1: V-cycle, 2: W-cycle (algebra/sfem_Multigrid.hpp:242-346)
         # coarse CG tolerance: the reference's default hierarchy (smoother "jacobi",
         # create_gmg_default_smoothers_and_solver) uses 1e-6, mgsolve's Chebyshev3 driver 1e-8
         # (drivers/mgsolve.cpp:208-214)
@@ -229,7 +230,7 @@ class SSGMG:
             c.A.diag(d)
             c.precond = inverse_diagonal(d, 1.0)
             c.smoother = cg(c.A, c.precond, rtol=coarse_rtol, atol=1e-30, max_it=coarse_max_it)
-        self.mg = Multigrid(self.real_type, 1, max_it, atol)
+        self.mg = Multigrid(self.real_type, self.cycle_type, max_it, atol)
         last = len(self.levels) - 1
         for i, lv in enumerate(self.levels):
             A, sm = lv.A, lv.smoother

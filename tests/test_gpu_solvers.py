@@ -206,6 +206,35 @@ def test_single_vcycle_matches_oracle():
     assert err < 1e-9
 
 
+@pytest.mark.parametrize("cycle_type", [1, 2])
+def test_multigrid_as_solver_matches_oracle(cycle_type):
+    """Multigrid as the solver (5 cycles, V or W): the iterate and the finest-level residual norm of
+    every cycle (computed after the pre-smoothing, algebra/sfem_Multigrid.hpp:242-346) against the
+    oracle's cycle."""
+    L = 4
+    mel, mpts, el, pts, fff, N, nodes, coeff = _ss_poisson(L, 3, coeff_seed=4)
+    h = gmg.SSGMG(S.Mesh(S.SSHEX8, L, el, N), "laplacian", fff, nodes, coeff=coeff, coarse_rtol=1e-12,
+                  cycle_type=cycle_type, max_it=5, atol=0.0)
+    lams = [S.cheb3_eig_max(lv.smoother) for lv in h.levels[:-1]]
+    mg_o = _oracle_hierarchy(L, mel, mpts, el, pts, fff, coeff, nodes, lams)
+    mg_o.max_it, mg_o.cycle_type = 5, cycle_type
+    b = np.random.default_rng(6).uniform(-1, 1, N)
+    b[nodes] = 0
+    xo = mg_o(b, np.zeros(N))
+    x = zeros(N)
+    h.mg.apply(dev(b), x)
+    S.synchronize()
+    hist = h.mg.history()
+    err = np.linalg.norm(host(x) - xo) / np.linalg.norm(xo)
+    herr = np.max(np.abs(hist - np.array(mg_o.history)) / np.array(mg_o.history))
+    print(f"{'V' if cycle_type == 1 else 'W'}-cycle solver: iterate {err:.2e}, residual history {herr:.2e}, "
+          f"residual reduction {hist[-1] / hist[0]:.2e} over {len(hist) - 1} norms (jumping coefficients, unscaled "
+          f"Chebyshev3)")
+    # the cycle repeats cycle_type times on every level, the finest included (one norm each)
+    assert len(hist) == 5 * cycle_type and len(mg_o.history) == 5 * cycle_type
+    assert err < 1e-9 and herr < 1e-8
+
+
 @pytest.mark.parametrize("coeff_seed", [None, 5])
 def test_mg_pcg_poisson_converges_and_matches_oracle(coeff_seed):
     L = 8
