@@ -1417,8 +1417,8 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
     const bool interior = kind == 0;
     const int32_t *row = idx + e * NXE + cy * N1 + cx;  // node (cx, cy, z) at row[z * PLANE]
     const int me = (cy + 1) * H + cx + 1;               // its cell in the halo-padded plane
-    // Every lane's column lies in the element (idle lanes sit on a real node too), so no load needs
-    // a condition.  Loads and stores share one wait counter on this chip, and a wait while both are
+    // Every lane's column lies in the element (idle lanes sit on a real node too), so the x and id
+    // loads need no condition; y is read only where it is updated in place.  Loads and stores share one wait counter on this chip, and a wait while both are
     // pending covers all of them: so each step issues all its loads first, consumes every loaded
     // value (into LDS or the stored data) before it issues its stores, and carries no loaded
     // register into the next step (node ids travel through the thread's own LDS ring).
@@ -1473,8 +1473,10 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
             const int32_t gq = sg[q % IR][tid];
             const int slot = sk_slot(L, cx, cy, q);
             spos[g] = sk_pos[e * NSK + (slot >= 0 ? slot : 0)];
+            // only the nodes updated in place read y (skeleton nodes go to the partials)
+            const bool read_y = !overwrite && active && interior && q > 0 && q < L;
 #pragma unroll
-            for (int c = 0; c < 3; c++) yold[g][c] = overwrite ? TIO(0) : y[(ptrdiff_t)gq * 3 + c];
+            for (int c = 0; c < 3; c++) yold[g][c] = read_y ? y[(ptrdiff_t)gq * 3 + c] : TIO(0);
         }
 #pragma unroll
         for (int g = 0; g < G; g++) {
