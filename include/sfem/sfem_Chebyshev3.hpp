@@ -1,12 +1,14 @@
 // sfem::Chebyshev3<T> (algebra/sfem_Chebyshev3.hpp:57-303): three-term Chebyshev
 // smoother over [scale_eig_min*lmax, lmax], lmax = scale_eig_max * eig_max.
-// Extension (not in the reference): with `inv_diag` set before init(), the recurrence and the
-// power method run on D^-1 A (Jacobi-scaled Chebyshev), for coefficients that vary between
-// elements by orders of magnitude.
+// Extensions (not in the reference): with `inv_diag` set before init(), the recurrence and the
+// eigenvalue estimate run on D^-1 A (Jacobi-scaled Chebyshev), for coefficients that vary between
+// elements by orders of magnitude; with `lanczos_steps` > 0, init() takes eig_max from at most
+// that many Lanczos steps (relative tolerance eigen_solver_tol) instead of the power method.
 #pragma once
 
 #include "sfem_Buffer.hpp"
 #include "sfem_MatrixFreeLinearSolver.hpp"
+#include "sfem_Lanczos.hpp"
 #include "sfem_PowerMethod.hpp"
 
 namespace sfem {
@@ -33,6 +35,8 @@ public:
 
     T eigen_solver_tol{1e-6};
     int eigen_solver_max_it{1000};
+    int lanczos_steps{0};  // > 0: eig_max by Lanczos (extension)
+    int eigen_iterations{0};
     int max_it{3};
     int iterations_{0};
     T eig_max{0};
@@ -77,7 +81,17 @@ public:
             op->apply(x, y);
             if (inv_diag) blas->xypaz(n_dofs, inv_diag->data(), y, T(0), y);
         };
-        eig_max = power_method->max_eigen_value(apply_fn, eigen_solver_max_it, eigen_solver_tol, n_dofs, ev, work);
+        if (lanczos_steps > 0) {
+            Lanczos<T> lanczos;
+            lanczos.blas = blas;
+            eig_max = lanczos.max_eigen_value([this](const T *const x, T *const y) { op->apply(x, y); },
+                                              inv_diag ? inv_diag->data() : nullptr, lanczos_steps, eigen_solver_tol,
+                                              n_dofs, guess);
+            eigen_iterations = lanczos.last_iterations;
+        } else {
+            eig_max = power_method->max_eigen_value(apply_fn, eigen_solver_max_it, eigen_solver_tol, n_dofs, ev, work);
+            eigen_iterations = power_method->last_iterations;
+        }
         auto b = blas;
         p_ = Buffer<T>::own(n_dofs, work, [b](void *q) { b->destroy(q); });
         temp_ = Buffer<T>::own(n_dofs, ev, [b](void *q) { b->destroy(q); });

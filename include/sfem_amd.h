@@ -294,6 +294,18 @@ double sfem_amd_cheb3_eig_max(const sfem_amd_op *cheb);
 int sfem_amd_cheb3_diag_create(sfem_amd_op *A, const void *inv_diag, int max_it, double scale_eig_max,
                                double scale_eig_min, int eigen_solver_max_it, double eigen_solver_tol,
                                int initial_guess_zero, const void *init_vector, sfem_amd_op **out);
+/* Chebyshev3 (on A, or on D^-1 A with inv_diag non-NULL) whose eig_max comes from at most
+ * `lanczos_steps` Lanczos steps, stopping when the estimate changes by at most `tol` relative
+ * (extension: the reference's Chebyshev3 uses PowerMethod only, algebra/sfem_Chebyshev3.hpp:165-175,
+ * which sfem_amd_cheb3_create keeps).  The scaled process runs in the D inner product, where
+ * D^-1 A is self-adjoint.  init_vector as for sfem_amd_cheb3_create (its D-scaled form starts the
+ * process). */
+int sfem_amd_cheb3_lanczos_create(sfem_amd_op *A, const void *inv_diag, int max_it, double scale_eig_max,
+                                  double scale_eig_min, int lanczos_steps, double tol, int initial_guess_zero,
+                                  const void *init_vector, sfem_amd_op **out);
+/* Iterations the eigenvalue estimate of a Chebyshev3 smoother took (power or Lanczos); -1 for
+ * other operators (extension). */
+int sfem_amd_cheb3_eigen_iterations(const sfem_amd_op *cheb);
 /* StationaryIteration with (scaled) Jacobi: x += omega D^-1 (b - A x), `steps` sweeps
  * (algebra/sfem_Stationary.hpp:50-80, algebra/sfem_ShiftableJacobi.hpp).  diag: device. */
 int sfem_amd_jacobi_create(sfem_amd_op *A, const void *diag, double omega, int steps,

@@ -84,6 +84,8 @@ SIGNATURES = {
     "sfem_amd_cg_create": (_i, [_vp, _vp, _i, _d, _d, _i, C.POINTER(_vp)]),
     "sfem_amd_cheb3_create": (_i, [_vp, _i, _d, _d, _i, _d, _i, _vp, C.POINTER(_vp)]),
     "sfem_amd_cheb3_eig_max": (_d, [_vp]),
+    "sfem_amd_cheb3_lanczos_create": (_i, [_vp, _vp, _i, _d, _d, _i, _d, _i, _vp, C.POINTER(_vp)]),
+    "sfem_amd_cheb3_eigen_iterations": (_i, [_vp]),
     "sfem_amd_jacobi_create": (_i, [_vp, _vp, _d, _i, C.POINTER(_vp)]),
     "sfem_amd_inverse_diagonal_create": (_i, [_i, _pd, _vp, _d, C.POINTER(_vp)]),
     "sfem_amd_permutation_create": (_i, [_pd, _vp, _i, _i, _i, _vp, C.POINTER(_vp)]),
@@ -248,6 +250,23 @@ class DeviceArray:
 
     def zero(self, stream=None):
         _check(lib.sfem_amd_memset(self.ptr, 0, self.nbytes, _stream_ptr(stream)), "memset")
+        return self
+
+    def _real_type(self):
+        if self.dtype not in (np.float64, np.float32):
+            raise SfemError("DeviceArray: a float64 or float32 vector required")
+        return FLOAT64 if self.dtype == np.float64 else FLOAT32
+
+    def fill(self, value, stream=None):
+        """Every entry = value (on the device)."""
+        _check(lib.sfem_amd_blas_values(self._real_type(), self.n, float(value), self.ptr, _stream_ptr(stream)),
+               "blas_values")
+        return self
+
+    def reciprocal(self, alpha=1.0, stream=None):
+        """x = alpha / x entrywise (on the device)."""
+        _check(lib.sfem_amd_blas_reciprocal(self._real_type(), self.n, float(alpha), self.ptr, _stream_ptr(stream)),
+               "blas_reciprocal")
         return self
 
     def copy_from(self, a):
@@ -493,8 +512,14 @@ def cg(A, precond=None, flexible=False, rtol=1e-10, atol=1e-16, max_it=10000, st
 
 
 def cheb3(A, max_it=3, scale_eig_max=1.0, scale_eig_min=0.06, eigen_solver_max_it=1000, eigen_solver_tol=1e-6,
-          initial_guess_zero=False, init_vector=None, inv_diag=None):
-    """Chebyshev3 smoother; with inv_diag (device D^-1) it runs on D^-1 A (an extension)."""
+          initial_guess_zero=False, init_vector=None, inv_diag=None, lanczos_steps=0):
+    """Chebyshev3 smoother; with inv_diag (device D^-1) it runs on D^-1 A (an extension).  eig_max
+    from the power method (the reference's), or with lanczos_steps > 0 from at most that many
+    Lanczos steps (an extension; eigen_solver_tol is then a relative tolerance)."""
+    if lanczos_steps > 0:
+        return _new_op("sfem_amd_cheb3_lanczos_create", A.h, _dptr(inv_diag) if inv_diag is not None else None,
+                       max_it, scale_eig_max, scale_eig_min, lanczos_steps, eigen_solver_tol,
+                       int(initial_guess_zero), _dptr(init_vector), keep=[A])
     if inv_diag is not None:
         return _new_op("sfem_amd_cheb3_diag_create", A.h, _dptr(inv_diag), max_it, scale_eig_max, scale_eig_min,
                        eigen_solver_max_it, eigen_solver_tol, int(initial_guess_zero), _dptr(init_vector), keep=[A])
@@ -504,6 +529,11 @@ def cheb3(A, max_it=3, scale_eig_max=1.0, scale_eig_min=0.06, eigen_solver_max_i
 
 def cheb3_eig_max(op):
     return lib.sfem_amd_cheb3_eig_max(op.h)
+
+
+def cheb3_eigen_iterations(op):
+    """Iterations of the smoother's eigenvalue estimate (power method or Lanczos)."""
+    return lib.sfem_amd_cheb3_eigen_iterations(op.h)
 
 
 def jacobi(A, diag, omega=1.0, steps=3):

@@ -1713,7 +1713,7 @@ int sfem_amd_cg_create(sfem_amd_op *A, sfem_amd_op *P, int flexible, double rtol
 extern "C++" {
 template <typename T>
 static int cheb3_create_t(sfem_amd_op *A, int max_it, double smax, double smin, int eig_its, double eig_tol, int zero,
-                          const void *init, sfem_amd_op **out, const void *inv_diag = nullptr) {
+                          const void *init, sfem_amd_op **out, const void *inv_diag = nullptr, int lanczos_steps = 0) {
     auto c = std::make_shared<sfem::Chebyshev3<T>>();
     c->default_init();
     c->set_op(view<T>(A->impl, true));
@@ -1729,13 +1729,14 @@ static int cheb3_create_t(sfem_amd_op *A, int max_it, double smax, double smin, 
     c->scale_eig_min = (T)smin;
     c->eigen_solver_max_it = eig_its;
     c->eigen_solver_tol = (T)eig_tol;
+    c->lanczos_steps = lanczos_steps;
     c->is_initial_guess_zero = zero != 0;
     if (init)
         c->init((const T *)init);
     else
         c->init_with_ones();
     if (!std::isfinite((double)c->eig_max) || c->eig_max <= 0) {
-        set_last_error("cheb3: power method failed");
+        set_last_error(lanczos_steps > 0 ? "cheb3: Lanczos estimate failed" : "cheb3: power method failed");
         return SFEM_FAILURE;
     }
     auto s = std::make_shared<SolverOp<T>>();
@@ -1763,6 +1764,24 @@ int sfem_amd_cheb3_diag_create(sfem_amd_op *A, const void *inv_diag, int max_it,
     if (A->impl->real_type() == SFEM_AMD_FLOAT64)
         return cheb3_create_t<double>(A, max_it, smax, smin, eig_its, eig_tol, zero, init, out, inv_diag);
     return cheb3_create_t<float>(A, max_it, smax, smin, eig_its, eig_tol, zero, init, out, inv_diag);
+}
+
+int sfem_amd_cheb3_lanczos_create(sfem_amd_op *A, const void *inv_diag, int max_it, double smax, double smin,
+                                  int lanczos_steps, double tol, int zero, const void *init, sfem_amd_op **out) {
+    CHECK_HANDLE(A);
+    SFEM_AMD_REQUIRE(lanczos_steps > 0, "cheb3_lanczos_create: lanczos_steps > 0 required");
+    if (A->impl->real_type() == SFEM_AMD_FLOAT64)
+        return cheb3_create_t<double>(A, max_it, smax, smin, 0, tol, zero, init, out, inv_diag, lanczos_steps);
+    return cheb3_create_t<float>(A, max_it, smax, smin, 0, tol, zero, init, out, inv_diag, lanczos_steps);
+}
+
+int sfem_amd_cheb3_eigen_iterations(const sfem_amd_op *op) {
+    if (!op) return -1;
+    if (auto *s = dynamic_cast<SolverOp<double> *>(op->impl.get()))
+        if (auto *c = dynamic_cast<sfem::Chebyshev3<double> *>(s->op.get())) return c->eigen_iterations;
+    if (auto *s = dynamic_cast<SolverOp<float> *>(op->impl.get()))
+        if (auto *c = dynamic_cast<sfem::Chebyshev3<float> *>(s->op.get())) return c->eigen_iterations;
+    return -1;
 }
 
 double sfem_amd_cheb3_eig_max(const sfem_amd_op *op) {

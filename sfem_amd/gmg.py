@@ -67,7 +67,7 @@ class SSGMG:
                  cheb_degree=3, eig_its=1000, eig_tol=1e-5, scale_eig_max=1.02, coarse_rtol=None,
                  coarse_max_it=10000, mu=None, lam=None, bc_components=None, max_it=1, atol=0.0, macro_box=None,
                  learned=None, learned_levels=1, learned_steps=1, learned_mode=1, learned_damping=1.0,
-                 coarse="cg", cycle_type=1):
+                 coarse="cg", cycle_type=1, lanczos_steps=0):
         self.kind = kind
         self.cycle_type = cycle_type  # 1: V-cycle, 2: W-cycle (algebra/sfem_Multigrid.hpp:242-346)
         # coarse CG tolerance: the reference's default hierarchy (smoother "jacobi",
@@ -87,8 +87,10 @@ class SSGMG:
         self.switches = []
         bs = self.block_size = 3 if kind == "elasticity" else 1
         self.real_type = real_type
+        # lanczos_steps > 0 (extension): Chebyshev3 takes lambda_max from at most that many Lanczos
+        # steps (eig_tol relative) instead of the power method
         self._smoother_cfg = dict(kind=smoother, degree=cheb_degree, eig_its=eig_its, eig_tol=eig_tol,
-                                  smax=scale_eig_max)
+                                  smax=scale_eig_max, lanczos=lanczos_steps)
         self.mu, self.lam, self.bc_components = mu, lam, bc_components
         bc_nodes = np.asarray(bc_nodes, np.int32)
         self.levels = []
@@ -131,15 +133,19 @@ class SSGMG:
         cfg = self._smoother_cfg
         dtype = np.float64 if lv.op.real_type == FLOAT64 else np.float32
         if cfg["kind"] in ("cheb3", "cheb3_jacobi"):
-            init = DeviceArray.from_numpy(np.ones(lv.n, dtype))
+            # start vector: ones with the constrained dofs zeroed; D^-1 as the reciprocal of the
+            # constrained operator's diagonal (both built on the device)
+            init = DeviceArray(lv.n, dtype)
+            init.fill(1.0)
             set_constrained_value(lv.A, 0.0, init)
             inv = None
             if cfg["kind"] == "cheb3_jacobi":
-                d = DeviceArray(lv.n, dtype)
-                lv.A.diag(d)
-                inv = DeviceArray.from_numpy((1.0 / d.numpy()).astype(dtype))
+                inv = DeviceArray(lv.n, dtype)
+                lv.A.diag(inv)
+                inv.reciprocal()
             return cheb3(lv.A, max_it=cfg["degree"], scale_eig_max=cfg["smax"], eigen_solver_max_it=cfg["eig_its"],
-                         eigen_solver_tol=cfg["eig_tol"], init_vector=init, inv_diag=inv)
+                         eigen_solver_tol=cfg["eig_tol"], init_vector=init, inv_diag=inv,
+                         lanczos_steps=cfg["lanczos"])
         if self.block_size == 3:
             # reference default for vector problems: block-Jacobi on the 3x3 nodal blocks, omega 1/3
             # (algebra/mg/sfem_GeometricMultigrid.cpp:114-131), constrained rows as unit rows

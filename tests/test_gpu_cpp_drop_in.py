@@ -181,3 +181,17 @@ def test_cpp_operator_algebra_semantics(tmp_path):
     assert r.returncode == 0, r.stderr
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr
+
+
+def test_cpp_lanczos_closed_forms(tmp_path):
+    """tests/cpp/lanczos_host.cpp (host BLAS): the Lanczos bound of include/sfem/sfem_Lanczos.hpp
+    against closed-form spectra: the tridiagonal bisection, the 1-D Laplacian (to convergence and
+    as a lower bound when cut short), D^-1 A similar to it, and decoupled rows left out by a zero
+    start entry."""
+    exe = os.path.join(str(tmp_path), "lanczos_host")
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "lanczos_host.cpp"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout + r.stderr

@@ -135,6 +135,61 @@ def test_jacobi_scaled_chebyshev_parity():
     assert np.linalg.norm(host(x) - xo) / np.linalg.norm(xo) < 1e-12
 
 
+@pytest.mark.parametrize("scaled", [False, True])
+def test_lanczos_eig_max_matches_dense_spectrum(scaled):
+    """Chebyshev3 with its bound from Lanczos (extension): lambda_max of the constrained sshex8
+    Laplacian (L=4 on 3^3, coefficients over three decades), or of D^-1 A, against the dense
+    spectrum of the free block (numpy eigvalsh of the oracle's matrix, symmetrically scaled);
+    the power method's estimate lies below it; the smoother with that bound matches the oracle's
+    Chebyshev3 step for step."""
+    L = 4
+    mel, mpts, el, pts, fff, N, nodes, coeff = _ss_poisson(L, 3, coeff_seed=4)
+    E = mel.shape[1]
+    A_o = oracle_constrained(lambda x, y: O.lib.oracle_sshex8_laplacian_apply(L, E, el, fff, coeff, x, y), nodes)
+    free = np.setdiff1d(np.arange(N), nodes)
+    M = np.zeros((free.size, free.size))
+    for j, c in enumerate(free):
+        e = np.zeros(N)
+        e[c] = 1
+        y = np.zeros(N)
+        A_o(e, y)
+        M[:, j] = y[free]
+    d = np.zeros(N)
+    O.lib.oracle_sshex8_laplacian_diag(L, E, el, fff, coeff, d)
+    d[nodes] = 1
+    if scaled:
+        s = 1 / np.sqrt(d[free])
+        M = s[:, None] * M * s[None, :]
+    lam_dense = float(np.linalg.eigvalsh(0.5 * (M + M.T))[-1])
+    v0 = np.ones(N)
+    v0[nodes] = 0
+    A = S.constrained(S.laplacian(S.Mesh(S.SSHEX8, L, el, N), fff, coeff), nodes)
+    inv = dev(1.0 / d) if scaled else None
+    ch = S.cheb3(A, max_it=3, scale_eig_max=1.02, eigen_solver_tol=1e-13, init_vector=dev(v0), inv_diag=inv,
+                 lanczos_steps=free.size)
+    lam = S.cheb3_eig_max(ch)
+    steps = S.cheb3_eigen_iterations(ch)
+    pw = S.cheb3(A, max_it=3, scale_eig_max=1.02, eigen_solver_max_it=1000, eigen_solver_tol=1e-5,
+                 init_vector=dev(v0), inv_diag=inv)
+    lam_pw = S.cheb3_eig_max(pw)
+    print(f"lambda_max dense {lam_dense:.15g}, Lanczos {lam:.15g} ({steps} steps), power {lam_pw:.15g} "
+          f"({S.cheb3_eigen_iterations(pw)} its)")
+    assert abs(lam - lam_dense) < 1e-10 * lam_dense
+    assert steps < 200
+    assert lam_pw <= lam_dense * (1 + 1e-12)
+    rng = np.random.default_rng(5)
+    b = rng.uniform(-1, 1, N)
+    b[nodes] = 0
+    x0 = rng.uniform(-1, 1, N)
+    x0[nodes] = 0
+    xo = SO.Chebyshev3(A_o, N, max_it=3, scale_eig_max=1.02, eig_max=lam,
+                       inv_diag=(1.0 / d) if scaled else None)(b, x0.copy())
+    x = dev(x0)
+    ch.apply(dev(b), x)
+    S.synchronize()
+    assert np.linalg.norm(host(x) - xo) / np.linalg.norm(xo) < 1e-12
+
+
 def _oracle_hierarchy(L, mel, mpts, el, pts, fff, coeff, nodes, lams):
     """Oracle V-cycle mirroring sfem_amd.gmg.SSGMG (same eig_max values)."""
     E = mel.shape[1]
