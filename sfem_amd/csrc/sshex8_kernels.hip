@@ -210,6 +210,203 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
 
 }
 
+// The same operator with the block's memory traffic staged through LDS (levels whose block
+// tile fits in 64 KB).  A thread per x-line makes each of the line's loads and stores a
+// 64-lane access spread over ~30 cache lines (lanes one line apart); here the block's node
+// slots are read and written with consecutive threads on consecutive slots of the
+// element-major connectivity, whose ids run in long consecutive stretches, instead:
+//   1. per slot: its id, x, and the old y (interior nodes, accumulating) or the skeleton slot
+//      position, into LDS (one unrolled batch of loads per thread);
+//   2. the line sweep of the kernel above, from LDS x into the LDS results (each node has one
+//      line thread, so no two threads touch a result);
+//   3. per slot: interior nodes store their y, skeleton nodes their partial (or, without
+//      partials, add to y atomically).
+template <int L>
+struct LapStagedShape {
+    static constexpr int N1 = L + 1, NXE = N1 * N1 * N1, LINES = N1 * N1;
+    static constexpr int EPB = LapShape<L>::EPB, BLOCK = LapShape<L>::BLOCK;
+    static constexpr int SLOTS = EPB * NXE, PER_THREAD = (SLOTS + BLOCK - 1) / BLOCK;
+    template <typename T>
+    static constexpr size_t lds_bytes() {
+        return (size_t)SLOTS * (2 * sizeof(T) + sizeof(int32_t)) + (size_t)EPB * 64 * sizeof(double);
+    }
+};
+
+// Used for FP32 vectors: at 113 VGPRs the staged kernel runs 4 waves per SIMD and beats the
+// line-thread loads (L=8 on 32^3: 165 vs 189 us); in FP64 both kernels hold ~200 VGPRs (2 waves
+// per SIMD, the window and the masked stencil halves) and the three phases of the staged one
+// overlap worse (280 vs 242 us), so FP64 keeps the line-thread kernel.
+template <typename T, int L>
+constexpr bool lap_staged_used() {
+    return std::is_same<T, float>::value && LapStagedShape<L>::template lds_bytes<T>() <= 64 * 1024;
+}
+
+template <typename T, typename G, int L>
+__global__ void __launch_bounds__(LapShape<L>::BLOCK)
+    sshex8_laplacian_apply_staged_kernel(const ptrdiff_t E, const int32_t *__restrict__ idx, const G *__restrict__ fff,
+                                         const ptrdiff_t fstride, const float *__restrict__ coeff,
+                                         const double *__restrict__ Kem, const T *__restrict__ x, T *__restrict__ y,
+                                         T *__restrict__ part, const int32_t *__restrict__ sk_pos,
+                                         const int overwrite) {
+    using S = LapStagedShape<L>;
+    constexpr int N1 = S::N1, NXE = S::NXE, LINES = S::LINES, EPB = S::EPB, NSK = sk_count(L);
+    __shared__ T su[EPB][NXE];      // x of the block's nodes
+    __shared__ T sy[EPB][NXE];      // old y (interior, accumulating) or 0, then the results
+    __shared__ int32_t st[EPB][NXE];  // store target: node id (interior; skeleton without partials) or slot position
+    __shared__ double sK[EPB][64];
+
+    const int tid = threadIdx.x;
+    const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
+    const ptrdiff_t nslots = (E - e0 < EPB ? E - e0 : EPB) * (ptrdiff_t)NXE;
+    T *su0 = &su[0][0], *sy0 = &sy[0][0];
+    int32_t *st0 = &st[0][0];
+
+    // 1. ids, then x and old y or slot positions, of this thread's slots
+    {
+        int32_t id[S::PER_THREAD];
+#pragma unroll
+        for (int k = 0; k < S::PER_THREAD; k++) {
+            const int sl = tid + k * S::BLOCK;
+            id[k] = sl < nslots ? idx[e0 * NXE + sl] : 0;
+        }
+        T xv[S::PER_THREAD], yv[S::PER_THREAD];
+        int32_t tg[S::PER_THREAD];
+#pragma unroll
+        for (int k = 0; k < S::PER_THREAD; k++) {
+            const int sl = tid + k * S::BLOCK;
+            const int le = sl / NXE, n = sl - le * NXE;
+            const int cx = n % N1, cy = (n / N1) % N1, cz = n / (N1 * N1);
+            const int slot = sk_slot(L, cx, cy, cz);
+            const bool ok = sl < nslots;
+            xv[k] = ok ? x[id[k]] : T(0);
+            yv[k] = (ok && slot < 0 && !overwrite) ? y[id[k]] : T(0);
+            tg[k] = (ok && slot >= 0 && part) ? sk_pos[(e0 + le) * NSK + slot] : id[k];
+        }
+#pragma unroll
+        for (int k = 0; k < S::PER_THREAD; k++) {
+            const int sl = tid + k * S::BLOCK;
+            if (sl < S::SLOTS) {
+                su0[sl] = xv[k];
+                sy0[sl] = yv[k];
+                st0[sl] = tg[k];
+            }
+        }
+    }
+    lap_block_matrices<L, EPB, S::BLOCK, G>(E, e0, fff, fstride, coeff, Kem, sK);
+    __syncthreads();
+
+    // 2. the line sweep
+    const int le = tid / LINES, line = tid - le * LINES;
+    const int yy = line % N1, zz = line / N1;
+    if (le < EPB && e0 + le < E) {
+        T Sa[3][3][2], Sb[3][3][2];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++)
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    Sa[a][b][c] = 0;
+                    Sb[a][b][c] = 0;
+                }
+        const double *K = sK[le];
+#pragma unroll
+        for (int cz = 0; cz < 2; cz++) {
+            const bool okz = cz ? (zz > 0) : (zz < L);
+#pragma unroll
+            for (int cy = 0; cy < 2; cy++) {
+                const bool oky = cy ? (yy > 0) : (yy < L);
+                const double m = (okz && oky) ? 1.0 : 0.0;
+#pragma unroll
+                for (int dz = -cz; dz <= 1 - cz; dz++)
+#pragma unroll
+                    for (int dy = -cy; dy <= 1 - cy; dy++) {
+                        const int ra = hex8_corner(0, cy, cz), rb = hex8_corner(1, cy, cz);
+#pragma unroll
+                        for (int dx = 0; dx < 2; dx++) {
+                            Sa[dz + 1][dy + 1][dx] += (T)(m * K[ra * 8 + hex8_corner(dx, cy + dy, cz + dz)]);
+                            Sb[dz + 1][dy + 1][dx] += (T)(m * K[rb * 8 + hex8_corner(dx, cy + dy, cz + dz)]);
+                        }
+                    }
+            }
+        }
+        const int yl = yy > 0 ? yy - 1 : 0, yh = yy < L ? yy + 1 : L;
+        const int zl = zz > 0 ? zz - 1 : 0, zh = zz < L ? zz + 1 : L;
+        const int ys[3] = {yl, yy, yh}, zs[3] = {zl, zz, zh};
+        const T *__restrict__ u = su[le];
+        T *__restrict__ out = sy[le];
+        T res[N1];
+        T w[3][3][3];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                w[0][a][b] = 0;
+                w[1][a][b] = u[lidx(L, 0, ys[b], zs[a])];
+                w[2][a][b] = u[lidx(L, 1, ys[b], zs[a])];
+            }
+#pragma unroll
+        for (int i = 0; i <= L; i++) {
+            T v = 0;
+            if (i < L) {
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        v = fma(Sa[a][b][0], w[1][a][b], v);
+                        v = fma(Sa[a][b][1], w[2][a][b], v);
+                    }
+            }
+            if (i > 0) {
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        v = fma(Sb[a][b][0], w[0][a][b], v);
+                        v = fma(Sb[a][b][1], w[1][a][b], v);
+                    }
+            }
+            res[i] = v;
+            if (i < L) {
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+#pragma unroll
+                    for (int b = 0; b < 3; b++) {
+                        w[0][a][b] = w[1][a][b];
+                        w[1][a][b] = w[2][a][b];
+                        w[2][a][b] = (i + 2 <= L) ? u[lidx(L, i + 2 <= L ? i + 2 : L, ys[b], zs[a])] : T(0);
+                    }
+            }
+        }
+        // the line's results after the sweep (no LDS store inside it, so the window may be re-read)
+#pragma unroll
+        for (int i = 0; i <= L; i++) {
+            const int n = lidx(L, i, yy, zz);
+            out[n] += res[i];
+        }
+    }
+    __syncthreads();
+
+    // 3. the stores
+#pragma unroll
+    for (int k = 0; k < S::PER_THREAD; k++) {
+        const int sl = tid + k * S::BLOCK;
+        if (sl < nslots) {
+            const int n = sl % NXE;
+            const int cx = n % N1, cy = (n / N1) % N1, cz = n / (N1 * N1);
+            const bool skel = sk_slot(L, cx, cy, cz) >= 0;
+            const T v = sy0[sl];
+            const int32_t t = st0[sl];
+            if (!skel)
+                y[t] = v;
+            else if (part)
+                part[t] = v;
+            else
+                atomic_add(&y[t], v);
+        }
+    }
+}
+
 template <typename T>
 struct MfmaAcc;
 // Operands: lane (c, g) = (lane % 16, lane / 16) holds A[c][g] and B[g][c].  Result element r
@@ -1106,8 +1303,12 @@ int sshex8_laplacian_apply(int L, ptrdiff_t E, const int32_t *idx, const G *fff,
     case LV: {                                                                                                    \
         using Sh = LapShape<LV>;                                                                                  \
         const unsigned grid = (unsigned)((E + Sh::EPB - 1) / Sh::EPB);                                            \
-        sshex8_laplacian_apply_kernel<T, G, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, fff, fstride, coeff, Kem, x, y, part, \
-                                                                             sk.pos, overwrite);                     \
+        if constexpr (lap_staged_used<T, LV>())                                                                   \
+            sshex8_laplacian_apply_staged_kernel<T, G, LV><<<grid, Sh::BLOCK, 0, s>>>(                            \
+                E, idx, fff, fstride, coeff, Kem, x, y, part, sk.pos, overwrite);                                 \
+        else                                                                                                      \
+            sshex8_laplacian_apply_kernel<T, G, LV><<<grid, Sh::BLOCK, 0, s>>>(E, idx, fff, fstride, coeff, Kem, x, y, \
+                                                                                 part, sk.pos, overwrite);        \
         break;                                                                                                    \
     }
         SFEM_AMD_SSHEX8_LEVELS(CASE)
