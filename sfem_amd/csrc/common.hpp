@@ -246,6 +246,15 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD b % 8), each with its
+// own L2.  This renumbering gives XCD k the k-th contiguous range of the grid, so workgroups on
+// neighbouring elements (consecutive ids, sharing boundary nodes and the cache lines of their
+// vector entries) meet in one L2 instead of writing back partial lines from several.
+__device__ __forceinline__ ptrdiff_t xcd_contiguous_block(ptrdiff_t b, ptrdiff_t n) {
+    const ptrdiff_t per = n / 8, rem = n % 8, xcd = b % 8;
+    return xcd * per + (xcd < rem ? xcd : rem) + b / 8;
+}
+
 // ----------------------------------------------------------------- launch helpers
 constexpr int kBlock = 256;
 inline unsigned grid_for(ptrdiff_t n, int block = kBlock, unsigned cap = 1u << 20) {

@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
     const int le = tid / LINES;
     const int line = tid - le * LINES;
     const int yy = line % N1, zz = line / N1;
-    const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
+    const ptrdiff_t e0 = xcd_contiguous_block(blockIdx.x, gridDim.x) * EPB;
     const ptrdiff_t e = e0 + le;
     const bool active = (le < EPB) && (e < E);
 
@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(LapShape<L>::BLOCK)
     __shared__ double sK[EPB][64];
 
     const int tid = threadIdx.x;
-    const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
+    const ptrdiff_t e0 = xcd_contiguous_block(blockIdx.x, gridDim.x) * EPB;
     const ptrdiff_t nslots = (E - e0 < EPB ? E - e0 : EPB) * (ptrdiff_t)NXE;
     T *su0 = &su[0][0], *sy0 = &sy[0][0];
     int32_t *st0 = &st[0][0];
@@ -779,7 +779,7 @@ __global__ void __launch_bounds__(256)
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int c = lane & 15, g = lane >> 4;
-    const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
+    const ptrdiff_t e0 = xcd_contiguous_block(blockIdx.x, gridDim.x) * EPB;
 
     // node k = tid + 256 i of the block: ids first, then all values at once; interior nodes
     // start from their y (accumulate), skeleton nodes from 0
@@ -937,7 +937,7 @@ __global__ void __launch_bounds__(256)
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int c = lane & 15, g = lane >> 4;
-    const ptrdiff_t e0 = (ptrdiff_t)blockIdx.x * EPB;
+    const ptrdiff_t e0 = xcd_contiguous_block(blockIdx.x, gridDim.x) * EPB;
 
     int32_t gid[NPT], pos[NPT];  // node ids; node-sorted skeleton slots (-1: interior)
 #pragma unroll
@@ -1583,7 +1583,7 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
     double(*sx)[3][P2 + 1] = reinterpret_cast<double(*)[3][P2 + 1]>(ubuf);           // halo-padded planes
     int32_t(*sg)[S::BLOCK] = reinterpret_cast<int32_t(*)[S::BLOCK]>(ubuf + SX_BYTES);  // ids of planes z, by z % IR
 
-    const ptrdiff_t e = blockIdx.x;
+    const ptrdiff_t e = xcd_contiguous_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x, r = tid >> 4, k = tid & 15;
     for (int i = tid; i < 576; i += S::BLOCK) sK[i] = Kall[e * 576 + i];
 
