@@ -517,6 +517,29 @@ def learned_config4(S, args, n_macro=32, L=8, n_train=32, train_macro=8, held_ou
     return out
 
 
+def laplacian_apply_only(S, args):
+    """sshex8 Laplacian apply alone, level 8 on an n^3 macro cube (n = 16: the 2,146,689-DOF
+    workload; n = 32: 16,974,593 DOF, the variable-coefficient workload's size), FP64 and FP32/BF16,
+    mesh and FFF built on the device: a short run for the profiler."""
+    L, n = 8, args.n_macro
+    mel, mpts = S.box_mesh(n + 1, n + 1, n + 1, element_type=S.HEX8)
+    mesh, pts_d = S.sshex8_refine_device(L, mel, mpts)
+    fff = S.mesh_fff(mesh, pts_d)
+    N, E = mesh.nnodes, n ** 3
+    out = {"workload": f"sshex8 Laplacian apply y += A x, level 8 on a {n}^3 macro hex8 unit cube ({N} DOF)",
+           "dofs": N}
+    for name, rt, dt, vb in (("fp64", S.FLOAT64, np.float64, 8), ("fp32", S.FLOAT32, np.float32, 4)):
+        op = S.laplacian(mesh, fff, real_type=rt)
+        x = S.DeviceArray.from_numpy(np.random.default_rng(0).uniform(-1, 1, N).astype(dt))
+        y = S.DeviceArray(N, dt)
+        progress(f"laplacian apply {name}: {args.steps} steps")
+        ms = S.benchmark(op, x, y, warmup=args.warmup, reps=args.steps) / args.steps
+        B = algorithmic_bytes(N, E, L, value_bytes=vb, geom_bytes=4 if vb == 8 else 2)
+        out[name] = {"ms_per_apply": ms, "dof_per_s": N / (ms * 1e-3), "algorithmic_bytes": B,
+                     "GB_per_s": B / (ms * 1e-3) / 1e9, "hbm_frac": B / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    return out
+
+
 def laplacian_config2(S, args):
     """sshex8 Laplacian L=8 on a 16^3 macro cube (2,146,689 DOF): apply (hot and cold caches, and the
     atomic skeleton option) and the MG-PCG solves."""
@@ -578,10 +601,14 @@ def main():
                          "elasticity solve and the Laplacian workload")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned smoother (3-channel network) in the elasticity solves")
-    ap.add_argument("--mode", choices=["apply", "copy", "elasticity-apply", "hex8-elasticity", "learned"],
+    ap.add_argument("--n-macro", type=int, default=16,
+                    help="laplacian-apply: macro elements per direction of the level-8 cube")
+    ap.add_argument("--mode", choices=["apply", "copy", "elasticity-apply", "laplacian-apply", "hex8-elasticity",
+                                       "learned"],
                     default="apply",
                     help="copy: stream-copy 2 GiB of FP64 per step (HBM counter calibration for rocprofv3 --pmc); "
                          "elasticity-apply: only the headline apply (a short run for the profiler); "
+                         "laplacian-apply: only the sshex8 Laplacian apply on --n-macro^3 (FP64 and FP32); "
                          "hex8-elasticity: only the 128^3 hex8 elasticity applies and solves; "
                          "learned: only the variable-coefficient learned-smoother protocol")
     args = ap.parse_args()
@@ -609,6 +636,9 @@ def main():
 
     if args.mode == "learned":
         print(json.dumps({"mode": "learned", **learned_config4(S, args)}))
+        return
+    if args.mode == "laplacian-apply":
+        print(json.dumps({"mode": "laplacian-apply", **laplacian_apply_only(S, args)}))
         return
     if args.mode == "hex8-elasticity":
         print(json.dumps({"mode": "hex8-elasticity", **hex8_elasticity_config3(S, args)}))
