@@ -129,6 +129,34 @@ def test_sshex8_laplacian_fp32_bf16_error_stated():
     assert err < 1e-2
 
 
+@pytest.mark.parametrize("L,n", [(1, 3), (2, 3), (3, 2), (4, 3), (5, 2), (6, 2), (8, 3), (10, 1), (12, 1), (16, 2)])
+def test_sshex8_laplacian_fp32_every_level(L, n):
+    """The FP32 apply (LDS-staged kernel up to L=12, line-thread kernel at L=16) against the FP64
+    oracle on the same geometry words (the FP32 operator folds the per-element coefficient into
+    the FFF and rounds the products to bf16), accumulate semantics: only the FP32 arithmetic
+    differs (< 1e-5); two applies give the same words."""
+    mel, mpts = skewed_box(n, perturb=0.02)
+    el, pts = O.sshex8_refine(L, mel, mpts)
+    fff = O.hex8_fff(mel, mpts)
+    coeff = np.random.default_rng(L).uniform(0.1, 10, mel.shape[1]).astype(np.float32)
+    N = pts.shape[1]
+    x = np.random.default_rng(1).uniform(-1, 1, N).astype(np.float32)
+    yo = np.zeros(N)
+    words = bf16_round((fff.astype(np.float64) * coeff[:, None]).astype(np.float32))
+    O.lib.oracle_sshex8_laplacian_apply(L, mel.shape[1], el, words, None, x.astype(np.float64), yo)
+    op = S.laplacian(S.Mesh(S.SSHEX8, L, el, N), fff, coeff, real_type=S.FLOAT32)
+    outs = []
+    for _ in range(2):
+        y0 = dev(np.full(N, 0.5, np.float32), np.float32)
+        op.apply(dev(x, np.float32), y0)
+        S.synchronize()
+        outs.append(host(y0))
+    err = relerr(outs[0].astype(np.float64) - 0.5, yo)
+    print(f"L={L}: FP32 apply vs FP64 oracle on the same bf16 words: {err:.3e}")
+    assert err < 1e-5
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize("L,n", [(2, 3), (4, 2), (8, 3)])
 def test_sshex8_laplacian_atomic_skeleton(L, n):
     """Option "atomic_skeleton": accumulating applies add the skeleton contributions with atomics and
