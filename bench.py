@@ -417,7 +417,7 @@ def hex8_elasticity_config3(S, args, n=128, rtol=1e-10):
     return out
 
 
-def learned_config4(S, args, n_macro=32, L=8, n_train=32, train_macro=8, held_out=(1000, 1001), rtol=1e-10):
+def learned_config4(S, args, n_macro=32, L=8, n_train=None, train_macro=8, held_out=(1000, 1001), rtol=1e-10):
     """Variable-coefficient sshex8 Poisson, level 8 on a 32^3 macro unit cube (16,974,593 DOF), zero
     Dirichlet on all faces, RHS U[0,1]; per-macro-element coefficients log-normal with log10 contrast
     U[0,3] (learned.coefficient_field).  The learned-smoother protocol: training pairs (D^-1 r, e)
@@ -429,6 +429,9 @@ def learned_config4(S, args, n_macro=32, L=8, n_train=32, train_macro=8, held_ou
     hierarchy continues below the macro mesh (super levels), dense coarse solve, graph-replayed
     V-cycles."""
     from sfem_amd import gmg, learned
+    if n_train is None:
+        n_train = args.protocol_problems
+    adam_steps = args.adam_steps
     progress(f"config 4: training set ({n_train} problems on a {train_macro}^3 macro copy)")
     t0 = time.perf_counter()
     tmel, tmpts = S.box_mesh(train_macro + 1, train_macro + 1, train_macro + 1, element_type=S.HEX8)
@@ -442,16 +445,24 @@ def learned_config4(S, args, n_macro=32, L=8, n_train=32, train_macro=8, held_ou
         return gmg.SSGMG(tmesh, "laplacian", tfff, tbc, coeff=k, real_type=S.FLOAT32, smoother="cheb3_jacobi",
                          macro_box=(train_macro, 1.0))
 
-    data = learned.protocol_pairs(build_train, range(n_train), cycles=1)
+    data = None
+    for c0 in range(0, n_train, 250):  # in chunks, with a progress line each
+        part = learned.protocol_pairs(build_train, range(c0, min(n_train, c0 + 250)), cycles=1)
+        if data is None:
+            data = part
+        else:
+            data.pairs.extend(part.pairs)
+        if n_train > 250:
+            progress(f"config 4: pairs of {min(n_train, c0 + 250)} problems")
     t_pairs = time.perf_counter() - t0
     cnn = S.CNN(16, 4, seed=0)
-    hist = learned.train(cnn, data, steps=600, lr=2e-3, batch_elements=32, seed=0)
+    hist = learned.train(cnn, data, steps=adam_steps, lr=2e-3, batch_elements=32, seed=0)
     t_train = time.perf_counter() - t0 - t_pairs
     test_small = learned.protocol_pairs(build_train, [s + 5000 for s in held_out], cycles=1)
     out = {"workload": f"sshex8 Poisson L={L} on a {n_macro}^3 macro cube, log-normal per-macro-element coefficient "
                        "(log10 contrast U[0,3]), zero Dirichlet on all faces, RHS U[0,1] (seeded)",
            "training": {"problems": n_train, "of_protocol": 2000, "train_mesh": f"{train_macro}^3 macro, L={L}",
-                        "pairs": len(data), "pairs_seconds": t_pairs, "train_seconds": t_train, "adam_steps": 600,
+                        "pairs": len(data), "pairs_seconds": t_pairs, "train_seconds": t_train, "adam_steps": adam_steps,
                         "loss_last20": float(np.mean(hist[-20:])),
                         "held_out_loss": learned.evaluate(cnn, test_small)}}
     del data, test_small
@@ -601,6 +612,9 @@ def main():
                          "elasticity solve and the Laplacian workload")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned smoother (3-channel network) in the elasticity solves")
+    ap.add_argument("--protocol-problems", type=int, default=32,
+                    help="learned-smoother protocol: seeded training problems (the full protocol: 2000)")
+    ap.add_argument("--adam-steps", type=int, default=600, help="learned-smoother protocol: Adam steps")
     ap.add_argument("--n-macro", type=int, default=16,
                     help="laplacian-apply: macro elements per direction of the level-8 cube")
     ap.add_argument("--mode", choices=["apply", "copy", "elasticity-apply", "laplacian-apply", "hex8-elasticity",
