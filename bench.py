@@ -425,7 +425,8 @@ def learned_config4(S, args, n_macro=32, L=8, n_train=None, train_macro=8, held_
     8^3-macro copy (the network acts per macro element, so its training problems need not be the full
     mesh; the full protocol takes 2,000), Adam on the project's kernels; then on held-out seeds at full
     size, the time to true relative residual rtol of FP64 MG-PCG, of the FP32 hierarchy in FP64
-    flexible CG, and of the latter with the trained network (2 steps) on the finest level.  Every
+    flexible CG, and of the latter with the trained network (--learned-steps per smoothing) on the
+    finest level.  Every
     hierarchy continues below the macro mesh (super levels), dense coarse solve, graph-replayed
     V-cycles."""
     from sfem_amd import gmg, learned
@@ -463,6 +464,7 @@ def learned_config4(S, args, n_macro=32, L=8, n_train=None, train_macro=8, held_
                        "(log10 contrast U[0,3]), zero Dirichlet on all faces, RHS U[0,1] (seeded)",
            "training": {"problems": n_train, "of_protocol": 2000, "train_mesh": f"{train_macro}^3 macro, L={L}",
                         "pairs": len(data), "pairs_seconds": t_pairs, "train_seconds": t_train, "adam_steps": adam_steps,
+                        "learned_steps_per_smoothing": args.learned_steps,
                         "loss_last20": float(np.mean(hist[-20:])),
                         "held_out_loss": learned.evaluate(cnn, test_small)}}
     del data, test_small
@@ -496,7 +498,8 @@ def learned_config4(S, args, n_macro=32, L=8, n_train=None, train_macro=8, held_
                 h = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, **common)
                 solver = S.cg(A64, h.preconditioner(graph=True), flexible=True, rtol=rtol, max_it=500)
             else:
-                h = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, learned=cnn, learned_steps=2, **common)
+                h = gmg.SSGMG(mesh, "laplacian", fff, bc, real_type=S.FLOAT32, learned=cnn,
+                              learned_steps=args.learned_steps, **common)
                 solver = learned.flexible_cg_with_fallback(A64, h.preconditioner(graph=True), h.switches, rtol=rtol,
                                                            max_it=500)
             setup = time.perf_counter() - t0
@@ -615,6 +618,8 @@ def main():
     ap.add_argument("--protocol-problems", type=int, default=32,
                     help="learned-smoother protocol: seeded training problems (the full protocol: 2000)")
     ap.add_argument("--adam-steps", type=int, default=600, help="learned-smoother protocol: Adam steps")
+    ap.add_argument("--learned-steps", type=int, default=2,
+                    help="learned-smoother protocol: CNN steps per smoothing in the held-out solves")
     ap.add_argument("--n-macro", type=int, default=16,
                     help="laplacian-apply: macro elements per direction of the level-8 cube")
     ap.add_argument("--mode", choices=["apply", "copy", "elasticity-apply", "laplacian-apply", "hex8-elasticity",
