@@ -1476,8 +1476,11 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
 // from lane n % 16 of the row through the DPP row broadcast (v_fmac_f64 with row_newbcast), so the
 // 243 FMAs of a node need no scalar loads and no per-lane masks.  A row's columns share one kind
 // of (x, y) position, whose micro elements (xy quadrants, bit qy*2 + qx) define its table: interior
-// columns (all four), the four faces y = 0, x = L, y = L, x = 0 (two quadrants each, L - 1 columns)
-// and the four corner columns (one quadrant, one column).  Register table blocks: S_m, S_0, S_p;
+// columns (all four) and the four faces y = 0, x = L, y = L, x = 0 (two quadrants each, L - 1
+// columns).  The four corner columns (one quadrant, one column each) would take a row and a
+// table apiece for one node per plane; instead the sweep saves each corner's 2x2 block of columns
+// and, after it, the block's threads apply the element matrix to the corner columns' micro
+// elements (L per column) from LDS.  Register table blocks: S_m, S_0, S_p;
 // the z-face block S_0lo (a node on z = 0 sees S_0lo, on z = L S_0 - S_0lo) stays in LDS, read on
 // those two planes only.  Arithmetic is FP64 for both vector precisions (FP32 vectors
 // are converted on load and store: on this chip FP32 FMA without packing runs at the FP64 rate).
@@ -1491,15 +1494,15 @@ struct StDpp {
     static constexpr int N1 = L + 1, PLANE = N1 * N1, H = N1 + 2, P2 = H * H;  // halo-padded plane
     static constexpr int NI = (L - 1) * (L - 1);
     static constexpr int RI = (NI + 15) / 16;  // rows of interior columns
-    static constexpr int ROWS = RI + 8;        // + 4 face rows + 4 corner rows
+    static constexpr int ROWS = RI + 4;        // + the four face rows
     static constexpr int BLOCK = 64 * ((ROWS + 3) / 4);
     static constexpr int NC = 4 * 81;           // coefficients per kind (S_m, S_0, S_p, S_0lo)
     static constexpr int NT = (3 * 81 + 15) / 16;  // register table per lane (S_m, S_0, S_p)
-    static constexpr int NTYPES = 9;
+    static constexpr int NTYPES = 5;  // row kinds: interior, the four faces
 };
 
-// quadrant mask of row kind t: 0 interior, 1-4 faces y = 0, x = L, y = L, x = 0, 5-8 corners
-// (0, 0), (L, 0), (L, L), (0, L)
+// quadrant mask of column kind t: 0 interior, 1-4 faces y = 0, x = L, y = L, x = 0, 5-8 corners
+// (0, 0), (L, 0), (L, L), (0, L) (the stencil kernel uses kinds 0-4)
 __host__ __device__ constexpr inline int st_kind_qmask(int t) {
     constexpr int q[9] = {0b1111, 0b1100, 0b0101, 0b0011, 0b1010, 0b1000, 0b0100, 0b0001, 0b0010};
     return q[t];
@@ -1571,7 +1574,7 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
                   NT = S::NT, NC = S::NC;
     constexpr int XR = 2 * G;      // x planes in LDS: G being read, G arriving
     constexpr int IR = 3 * G + 1;  // node-id ring: planes z - 1 .. z + 3G - 1 of a step starting at z
-    // LDS: the per-kind tables (S_m, S_0, S_p of the nine row kinds) are only read once into each
+    // LDS: the per-kind tables (S_m, S_0, S_p of the five row kinds) are only read once into each
     // lane's registers, so they share their space with the x planes and the id ring that follow them
     constexpr size_t SX_BYTES = sizeof(double) * XR * 3 * (P2 + 1), SG_BYTES = sizeof(int32_t) * IR * S::BLOCK,
                      ST_BYTES = sizeof(double) * S::NTYPES * 243;
@@ -1579,6 +1582,9 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
     __shared__ __align__(16) unsigned char ubuf[U_BYTES];
     __shared__ double slo[S::NTYPES][81];  // S_0lo of the nine kinds (the z = 0 and z = L planes)
     __shared__ double sK[576];
+    __shared__ double csave[4][4][3][N1];        // x of each corner's 2x2 block of columns, by plane
+    __shared__ double clo[4][L][3], cup[4][L][3];  // corner rows of the micro elements of the corner columns
+    __shared__ int32_t cpos[4 * N1];             // skeleton slot positions of the corner nodes
     double(*stab)[243] = reinterpret_cast<double(*)[243]>(ubuf);
     double(*sx)[3][P2 + 1] = reinterpret_cast<double(*)[3][P2 + 1]>(ubuf);           // halo-padded planes
     int32_t(*sg)[S::BLOCK] = reinterpret_cast<int32_t(*)[S::BLOCK]>(ubuf + SX_BYTES);  // ids of planes z, by z % IR
@@ -1586,10 +1592,14 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
     const ptrdiff_t e = xcd_contiguous_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x, r = tid >> 4, k = tid & 15;
     for (int i = tid; i < 576; i += S::BLOCK) sK[i] = Kall[e * 576 + i];
+    if (tid < 4 * N1) {
+        const int c = tid / N1, z = tid - c * N1;
+        cpos[tid] = sk_pos[e * NSK + sk_slot(L, (c == 1 || c == 2) ? L : 0, c >= 2 ? L : 0, z)];
+    }
 
     // this lane's column and its row's kind
     int cx = 1, cy = 1, kind = -1;
-    bool active = false;
+    bool active = false, loader = false;
     if (r < S::RI) {
         const int ci = r * 16 + k;
         kind = 0;
@@ -1602,22 +1612,30 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
         const int f = r - S::RI, t = 1 + min(k, L - 2);
         kind = 1 + f;
         active = k < L - 1;
-        if (f == 0) { cx = t; cy = 0; }
+        // the row's first idle lane stages the x of corner column f ((0,0), (L,0), (L,L), (0,L)): the
+        // neighbouring face and interior columns read it from the plane
+        loader = k == L - 1;
+        if (loader) {
+            cx = (f == 1 || f == 2) ? L : 0;
+            cy = f >= 2 ? L : 0;
+        } else if (f == 0) { cx = t; cy = 0; }
         else if (f == 1) { cx = L; cy = t; }
         else if (f == 2) { cx = t; cy = L; }
         else { cx = 0; cy = t; }
-    } else if (r < S::RI + 8) {
-        const int c = r - S::RI - 4;
-        kind = 5 + c;
-        active = k == 0;
-        if (c == 0) { cx = 0; cy = 0; }
-        else if (c == 1) { cx = L; cy = 0; }
-        else if (c == 2) { cx = L; cy = L; }
-        else { cx = 0; cy = L; }
     }
     const bool interior = kind == 0;
     const int32_t *row = idx + e * NXE + cy * N1 + cx;  // node (cx, cy, z) at row[z * PLANE]
     const int me = (cy + 1) * H + cx + 1;               // its cell in the halo-padded plane
+    // the corner columns' 2x2 blocks this column belongs to: bit c for corner c ((0,0), (L,0), (L,L),
+    // (0,L)), its cell there (dy * 2 + dx) in bits 4 + 2c
+    unsigned cmem = 0;
+    if (active || loader) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int dx = abs(cx - ((c == 1 || c == 2) ? L : 0)), dy = abs(cy - (c >= 2 ? L : 0));
+            if (dx <= 1 && dy <= 1) cmem |= (1u << c) | ((unsigned)(dy * 2 + dx) << (4 + 2 * c));
+        }
+    }
     // Every lane's column lies in the element (idle lanes sit on a real node too), so the x and id
     // loads need no condition; y is read only where it is updated in place.  Loads and stores share one wait counter on this chip, and a wait while both are
     // pending covers all of them: so each step issues all its loads first, consumes every loaded
@@ -1657,7 +1675,13 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
         sg[g % IR][tid] = g0[g];
         sg[(G + g) % IR][tid] = g0[G + g];
 #pragma unroll
-        for (int c = 0; c < 3; c++) sx[g][c][active ? me : P2] = (double)x0[g][c];
+        for (int c = 0; c < 3; c++) sx[g][c][(active || loader) ? me : P2] = (double)x0[g][c];
+        if (cmem)
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++)
+                if ((cmem >> cc) & 1u)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) csave[cc][(cmem >> (4 + 2 * cc)) & 3u][c][g] = (double)x0[g][c];
     }
 
     double acc_lo[3] = {0, 0, 0}, acc_mid[3] = {0, 0, 0}, acc_hi[3] = {0, 0, 0};
@@ -1731,12 +1755,18 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
             }
         }
         // consume the loads: x of the next G planes into the free buffers (idle lanes: the spare
-        // cell), the ids into the ring
+        // cell) and, for the columns of a corner's 2x2 block, into its save area; the ids into the ring
 #pragma unroll
         for (int g = 0; g < G; g++) {
             const int z1 = p0 + G + g;
 #pragma unroll
-            for (int c = 0; c < 3; c++) sx[z1 % XR][c][active ? me : P2] = (double)xn[g][c];
+            for (int c = 0; c < 3; c++) sx[z1 % XR][c][(active || loader) ? me : P2] = (double)xn[g][c];
+            if (cmem && z1 <= L)
+#pragma unroll
+                for (int cc = 0; cc < 4; cc++)
+                    if ((cmem >> cc) & 1u)
+#pragma unroll
+                        for (int c = 0; c < 3; c++) csave[cc][(cmem >> (4 + 2 * cc)) & 3u][c][z1] = (double)xn[g][c];
             sg[(p0 + 2 * G + g) % IR][tid] = gid2[g];
         }
         // the stores of the completed planes z = p - 1 (lanes with nothing to store write the
@@ -1759,6 +1789,33 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
         const ptrdiff_t ps = (ptrdiff_t)sk_pos[e * NSK + sk_slot(L, cx, cy, L)] * 3;
 #pragma unroll
         for (int c = 0; c < 3; c++) part[ps + c] = (TIO)acc_lo[c];
+    }
+    // the corner columns, in element form from the saved blocks: thread (c, m) applies the rows of
+    // the two corner nodes of micro element m of corner column c (its lower and upper corner) to
+    // the micro element's 24 values; each corner node's two contributions are summed after
+    lds_barrier();  // every wave's saved blocks
+    // one row (c, m, lz, i) per thread: 4 corners x L micro elements x 2 corner nodes x 3 components
+    for (int t = tid; t < 24 * L; t += S::BLOCK) {
+        const int c = t / (6 * L), rem = t - c * 6 * L, m = rem / 6, lz = (rem / 3) & 1, i = rem % 3;
+        const int ccx = (c == 1 || c == 2) ? L : 0, ccy = c >= 2 ? L : 0;
+        const int mx = ccx ? L - 1 : 0, my = ccy ? L - 1 : 0, lx = ccx - mx, ly = ccy - my;
+        const double *kr = sK + (i * 8 + hex8_corner(lx, ly, lz)) * 24;
+        double a = 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int cell = abs(my + hex8_cy(q) - ccy) * 2 + abs(mx + hex8_cx(q) - ccx), z = m + hex8_cz(q);
+#pragma unroll
+            for (int j = 0; j < 3; j++) a = fma(kr[j * 8 + q], csave[c][cell][j][z], a);
+        }
+        (lz ? cup : clo)[c][m][i] = a;
+    }
+    lds_barrier();
+    for (int t = tid; t < 4 * N1; t += S::BLOCK) {
+        const int c = t / N1, z = t - c * N1;
+        const ptrdiff_t ps = (ptrdiff_t)cpos[t] * 3;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            part[ps + i] = (TIO)((z < L ? clo[c][z][i] : 0.0) + (z > 0 ? cup[c][z - 1][i] : 0.0));
     }
 }
 
