@@ -702,6 +702,8 @@ def main():
     # memory state after the large solves
     if not args.no_extra:
         result["laplacian_config2"] = laplacian_config2(S, args)
+        # the Laplacian apply at the variable-coefficient workload's size (beyond the MALL)
+        result["laplacian_apply_32cube"] = laplacian_apply_only(S, argparse.Namespace(**{**vars(args), "n_macro": 32}))
         result["hex8_elasticity_config3"] = hex8_elasticity_config3(S, args)
         if not args.no_solve:
             result["learned_config4"] = learned_config4(S, args)
