@@ -33,6 +33,7 @@ public:
     int iterations_{0};
     bool verbose{false};
     bool apply_overwrites_output{false};
+    bool preconditioner_overwrites_output{false};  // (extension) see precondition()
     bool flexible{false};
     // fixed-preconditioner PCG that stops on ||r||/||b|| (recursive residual) instead of the
     // reference's sqrt(r.z)/sqrt(r0.z0)
@@ -123,9 +124,11 @@ private:
         return rtr;
     }
     // z = P r from z = 0 whatever the operator does; a solver or multigrid preconditioner is told
-    // that its initial guess is zero for this apply (it may skip products with it)
+    // that its initial guess is zero for this apply (it may skip products with it).  With
+    // preconditioner_overwrites_output the operator's apply writes all of z itself (from a zero
+    // start), and z is not cleared first.
     void precondition(std::ptrdiff_t n, const T *r, T *z) {
-        blas->zeros(n, z);
+        if (!preconditioner_overwrites_output) blas->zeros(n, z);
         auto *mf = dynamic_cast<MatrixFreeLinearSolver<T> *>(preconditioner_op.get());
         if (mf) mf->set_initial_guess_zero(true);
         preconditioner_op->apply(r, z);

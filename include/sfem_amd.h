@@ -276,7 +276,9 @@ int sfem_amd_cheb3_start(int rt, ptrdiff_t n, double alpha, const void *rhs, con
  * sqrt(r.z)/sqrt(r0.z0), :430-437); mode 1: flexible PCG (beta from r_new.(z_new - z_old); the
  * reference has none) stopping on the true residual ||b - A x||/||b||; mode 2: fixed-
  * preconditioner PCG stopping on ||r||/||b||.  Applying the returned op solves A x = b
- * (x: initial guess and output).  The op keeps shared references to A and precond. */
+ * (x: initial guess and output).  The op keeps shared references to A and precond, whose
+ * overwriting apply it calls for z = P r (a zero fill and the apply, unless the operator writes
+ * z directly, as the precision bridge does). */
 int sfem_amd_cg_create(sfem_amd_op *A, sfem_amd_op *precond, int mode, double rtol,
                        double atol, int max_it, sfem_amd_op **out);
 /* sfem::Chebyshev3<real_t> (algebra/sfem_Chebyshev3.hpp:57-303) of degree `max_it`; eig_max
@@ -396,9 +398,9 @@ int sfem_amd_dense_op_create(int real_type, ptrdiff_t n, const double *matrix, s
  * (x, y, stream) relaunch the graph and others run eagerly.  For launch-bound sequences without
  * host synchronisation, e.g. a preconditioning V-cycle (max_it 1) with a dense coarse solve. */
 int sfem_amd_graph_op_create(sfem_amd_op *op, sfem_amd_op **out);
-/* captured: graphs instantiated (one per initial-guess mode: the zero-guess apply a CG makes of its
- * preconditioner records a different sequence), 0 not yet, -1 capture failed (eager); replays:
- * graph launches. */
+/* captured: graphs instantiated (one per initial-guess mode and per accumulating or overwriting
+ * apply: the zero-guess overwriting apply a CG makes of its preconditioner records a different
+ * sequence), 0 not yet, -1 capture failed (eager); replays: graph launches. */
 int sfem_amd_graph_op_stats(const sfem_amd_op *op, int *captured, int *replays);
 /* An op that runs `primary` until switched to `fallback` (same size and real type). */
 int sfem_amd_switch_create(sfem_amd_op *primary, sfem_amd_op *fallback, sfem_amd_op **out);

@@ -894,6 +894,18 @@ struct PrecisionBridge final : OpBase {
         if (err) return SFEM_FAILURE;
         return cast_accumulate(n, yf.p, (double *)y, s);  // y += (double) yf
     }
+    // y = (double) op32((float) x): the result is cast into y, which is neither cleared nor read
+    int apply_overwrite(const void *x, void *y, hipStream_t s) override {
+        const ptrdiff_t n = rows();
+        if ((ptrdiff_t)xf.n != n && (xf.alloc(n) || yf.alloc(n))) return SFEM_FAILURE;
+        if (blas_cast<double, float>(n, (const double *)x, xf.p, s)) return SFEM_FAILURE;
+        if (blas_values<float>(n, 0.f, yf.p, s)) return SFEM_FAILURE;
+        op32->set_initial_guess_zero(true);
+        const int err = op32->apply(xf.p, yf.p, s);
+        op32->set_initial_guess_zero(false);
+        if (err) return SFEM_FAILURE;
+        return blas_cast<float, double>(n, yf.p, (double *)y, s);
+    }
 };
 
 // FP32 -> FP64 operator -> FP32: y += (float) op64((double) x).  Used for the FP64
@@ -986,7 +998,7 @@ struct GraphOp final : OpBase {
         hipStream_t cs{nullptr};
         int calls{0};
         uint64_t epoch{0};  // the state epoch of the capture
-    } cap[2];  // [mode]: general / zero initial guess
+    } cap[4];  // [2 * overwrite + zero initial guess]
     int replays{0};
     bool failed{false};
     bool zero_guess{false};
@@ -994,7 +1006,11 @@ struct GraphOp final : OpBase {
         for (auto &c : cap)
             if (c.exec) (void)hipGraphExecDestroy(c.exec);
     }
-    int captured() const { return (cap[0].exec ? 1 : 0) + (cap[1].exec ? 1 : 0); }
+    int captured() const {
+        int n = 0;
+        for (const auto &c : cap) n += c.exec ? 1 : 0;
+        return n;
+    }
     int real_type() const override { return inner->real_type(); }
     ptrdiff_t rows() const override { return inner->rows(); }
     ptrdiff_t cols() const override { return inner->cols(); }
@@ -1003,8 +1019,12 @@ struct GraphOp final : OpBase {
         zero_guess = v;
         inner->set_initial_guess_zero(v);
     }
-    int apply(const void *x, void *y, hipStream_t s) override {
-        Capture &c = cap[zero_guess ? 1 : 0];
+    int apply(const void *x, void *y, hipStream_t s) override { return run(x, y, s, false); }
+    int apply_overwrite(const void *x, void *y, hipStream_t s) override { return run(x, y, s, true); }
+    // the inner operator's apply (accumulating) or apply_overwrite, replayed from its own capture
+    int run(const void *x, void *y, hipStream_t s, bool overwrite) {
+        auto eager = [&] { return overwrite ? inner->apply_overwrite(x, y, s) : inner->apply(x, y, s); };
+        Capture &c = cap[(overwrite ? 2 : 0) + (zero_guess ? 1 : 0)];
         const uint64_t now = state_epoch();
         if (c.exec && c.epoch != now) {
             // options, switches or weights changed since the capture: drop it, run the next call
@@ -1013,14 +1033,14 @@ struct GraphOp final : OpBase {
             c.exec = nullptr;
             c.calls = 0;
         }
-        if (c.calls++ == 0 || failed) return inner->apply(x, y, s);
+        if (c.calls++ == 0 || failed) return eager();
         if (!c.exec) {
             if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
                 failed = true;
                 (void)hipGetLastError();
-                return inner->apply(x, y, s);
+                return eager();
             }
-            const int err = inner->apply(x, y, s);
+            const int err = eager();
             hipGraph_t graph = nullptr;
             const hipError_t e_end = hipStreamEndCapture(s, &graph);
             if (err || e_end != hipSuccess || !graph ||
@@ -1029,7 +1049,7 @@ struct GraphOp final : OpBase {
                 c.exec = nullptr;
                 failed = true;
                 (void)hipGetLastError();
-                return inner->apply(x, y, s);  // nothing ran during the capture: run it now
+                return eager();  // nothing ran during the capture: run it now
             }
             (void)hipGraphDestroy(graph);
             c.cx = x;
@@ -1042,7 +1062,7 @@ struct GraphOp final : OpBase {
             SFEM_AMD_HIP_CHECK(hipGraphLaunch(c.exec, s));
             return SFEM_SUCCESS;
         }
-        return inner->apply(x, y, s);
+        return eager();
     }
 };
 
@@ -1684,7 +1704,12 @@ static int cg_create_t(sfem_amd_op *A, sfem_amd_op *P, int flexible, double rtol
     cg->default_init();
     cg->set_op(view<T>(A->impl, true));
     cg->apply_overwrites_output = true;
-    if (P) cg->set_preconditioner_op(view<T>(P->impl));
+    if (P) {
+        // z = P r from zero through the operator's overwriting apply: its default is the zero fill
+        // and the apply, operators that can write z directly (the precision bridge) skip the fill
+        cg->set_preconditioner_op(view<T>(P->impl, true));
+        cg->preconditioner_overwrites_output = true;
+    }
     cg->flexible = flexible == 1;
     cg->stop_on_residual_norm = flexible == 2;
     cg->rtol = (T)rtol;
