@@ -622,12 +622,16 @@ def main():
                     help="learned-smoother protocol: CNN steps per smoothing in the held-out solves")
     ap.add_argument("--n-macro", type=int, default=16,
                     help="laplacian-apply: macro elements per direction of the level-8 cube")
-    ap.add_argument("--mode", choices=["apply", "copy", "elasticity-apply", "laplacian-apply", "hex8-elasticity",
-                                       "learned"],
+    ap.add_argument("--case", choices=["l8", "l16"], default="l16",
+                    help="elasticity-solve: L=8 on 32^3 (E=1) or L=16 on 24^3 (E log-uniform)")
+    ap.add_argument("--mode", choices=["apply", "copy", "elasticity-apply", "laplacian-apply", "elasticity-solve",
+                                       "laplacian-solve", "hex8-elasticity", "learned"],
                     default="apply",
                     help="copy: stream-copy 2 GiB of FP64 per step (HBM counter calibration for rocprofv3 --pmc); "
                          "elasticity-apply: only the headline apply (a short run for the profiler); "
                          "laplacian-apply: only the sshex8 Laplacian apply on --n-macro^3 (FP64 and FP32); "
+                         "elasticity-solve: only the elasticity MG-PCG solves of --case; "
+                         "laplacian-solve: only the MG-PCG solves of the 2.15M-DOF Laplacian workload; "
                          "hex8-elasticity: only the 128^3 hex8 elasticity applies and solves; "
                          "learned: only the variable-coefficient learned-smoother protocol")
     args = ap.parse_args()
@@ -655,6 +659,16 @@ def main():
 
     if args.mode == "learned":
         print(json.dumps({"mode": "learned", **learned_config4(S, args)}))
+        return
+    if args.mode == "elasticity-solve":
+        kw = dict(n_macro=32, L=8) if args.case == "l8" else dict(n_macro=24, L=16, coeff_seed=0)
+        print(json.dumps({"mode": "elasticity-solve",
+                          **elasticity_solve_times(S, with_learned=not args.no_learned, **kw)}))
+        return
+    if args.mode == "laplacian-solve":
+        mel, mpts, el, pts, fff = build_problem(16, 8)
+        mesh = S.Mesh(S.SSHEX8, 8, el, pts.shape[1])
+        print(json.dumps({"mode": "laplacian-solve", **solve_times(S, mesh, fff, pts, 16)}))
         return
     if args.mode == "laplacian-apply":
         print(json.dumps({"mode": "laplacian-apply", **laplacian_apply_only(S, args)}))
