@@ -439,21 +439,51 @@ __global__ void skeleton_gather_kernel(const ptrdiff_t n, const int32_t *__restr
                                        const int32_t *__restrict__ off, const int32_t *__restrict__ ent,
                                        const T *__restrict__ part, T *__restrict__ y, const int overwrite,
                                        const uint8_t *__restrict__ cmask, const T *cx, const ChebStep<T> cs) {
+    // the first NU partials of a node (face nodes have 2, edge nodes 4) are loaded together with its
+    // old value, right after its offsets: two memory round trips per node instead of one per partial
+    constexpr int NU = 4;
     for (ptrdiff_t t = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (ptrdiff_t)gridDim.x * blockDim.x) {
         const int j0 = off[t], j1 = off[t + 1];
-        T acc[BS];
-#pragma unroll
-        for (int c = 0; c < BS; c++) acc[c] = 0;
-        for (int j = j0; j < j1; j++) {
-            const ptrdiff_t q = (ent ? (ptrdiff_t)ent[j] : (ptrdiff_t)j) * BS;
-#pragma unroll
-            for (int c = 0; c < BS; c++) acc[c] += part[q + c];
-        }
         const ptrdiff_t g = (ptrdiff_t)node[t] * BS;
         const unsigned cm = cmask ? cmask[t] : 0u;
+        T acc[BS], yo[BS];
+#pragma unroll
+        for (int c = 0; c < BS; c++) acc[c] = yo[c] = 0;
+        if (!ent) {
+            // absent slots re-read the node's first partial (a valid index: every skeleton node has
+            // one) and are zeroed after the load, so that no load waits behind a branch
+            T pv[NU][BS];
+#pragma unroll
+            for (int k = 0; k < NU; k++) {
+                const ptrdiff_t q = (ptrdiff_t)(j0 + k < j1 ? j0 + k : j0) * BS;
+#pragma unroll
+                for (int c = 0; c < BS; c++) pv[k][c] = part[q + c];
+            }
+            if (!overwrite)
+#pragma unroll
+                for (int c = 0; c < BS; c++) yo[c] = y[g + c];
+            // in slot order, as the loop below (adding the zeros of absent slots leaves the sum's
+            // words unchanged: it starts at +0 and never becomes -0)
+#pragma unroll
+            for (int k = 0; k < NU; k++)
+#pragma unroll
+                for (int c = 0; c < BS; c++) acc[c] += j0 + k < j1 ? pv[k][c] : T(0);
+            for (int j = j0 + NU; j < j1; j++)
+#pragma unroll
+                for (int c = 0; c < BS; c++) acc[c] += part[(ptrdiff_t)j * BS + c];
+        } else {
+            if (!overwrite)
+#pragma unroll
+                for (int c = 0; c < BS; c++) yo[c] = y[g + c];
+            for (int j = j0; j < j1; j++) {
+                const ptrdiff_t q = (ptrdiff_t)ent[j] * BS;
+#pragma unroll
+                for (int c = 0; c < BS; c++) acc[c] += part[q + c];
+            }
+        }
 #pragma unroll
         for (int c = 0; c < BS; c++) {
-            const T v = ((cm >> c) & 1u) ? (cx ? cx[g + c] : T(0)) : (overwrite ? acc[c] : y[g + c] + acc[c]);
+            const T v = ((cm >> c) & 1u) ? (cx ? cx[g + c] : T(0)) : (overwrite ? acc[c] : yo[c] + acc[c]);
             if (cs.p) {
                 const T pi = cheb3_direction<T>(cs.first ? T(0) : cs.p[g + c], cs.rhs[g + c], v, cs.beta, cs.dinv, g + c,
                                                 cs.first);
