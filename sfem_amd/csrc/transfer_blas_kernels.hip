@@ -114,34 +114,58 @@ __global__ void __launch_bounds__(256) sshex8_prolongate_lds_kernel(const ptrdif
     }
     __syncthreads();
     const uint8_t *ow = own + e * nxe;
-    for (int v = threadIdx.x; v < nxe; v += blockDim.x) {
-        if (!ow[v]) continue;
-        const int x = v % N1, y = (v / N1) % N1, z = v / (N1 * N1);
-        int cx = x / s, cy = y / s, cz = z / s;
-        if (cx == Lc) cx--;
-        if (cy == Lc) cy--;
-        if (cz == Lc) cz--;
-        const double tx = (double)(x - cx * s) / s, ty = (double)(y - cy * s) / s, tz = (double)(z - cz * s) / s;
-        double acc[VS];
+    // NB fine nodes per thread at a time: their owner flags and ids, then (adding) their keep flags
+    // and old values, each batch of loads issued together (unowned nodes read valid entries they
+    // do not use)
+    constexpr int NB = 4;
+    for (int v0 = threadIdx.x; v0 < nxe; v0 += NB * blockDim.x) {
+        uint8_t o[NB];
+        ptrdiff_t gg[NB];
 #pragma unroll
-        for (int d = 0; d < VS; d++) acc[d] = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int bx = hex8_cx(k), by = hex8_cy(k), bz = hex8_cz(k);
-            const double w = (bx ? tx : 1 - tx) * (by ? ty : 1 - ty) * (bz ? tz : 1 - tz);
-            if (w == 0) continue;
-            const double *src = tr_smem + lidx(Lc, cx + bx, cy + by, cz + bz) * VS;
-#pragma unroll
-            for (int d = 0; d < VS; d++) acc[d] += w * src[d];
+        for (int b = 0; b < NB; b++) {
+            const int v = v0 + b * blockDim.x;
+            const bool in = v < nxe;
+            o[b] = in ? ow[v] : 0;
+            gg[b] = row[in ? v : 0];
         }
-        const ptrdiff_t g = row[v];
-        const unsigned km = keep ? keep[g] : 0u;
+        unsigned km[NB];
+        T old[NB][VS];
 #pragma unroll
-        for (int d = 0; d < VS; d++) {
-            if (!add)
-                to[g * VS + d] = (T)acc[d];
-            else if (!((km >> d) & 1u))
-                to[g * VS + d] = to[g * VS + d] + (T)acc[d];
+        for (int b = 0; b < NB; b++) {
+            km[b] = (add && keep) ? keep[gg[b]] : 0u;
+#pragma unroll
+            for (int d = 0; d < VS; d++) old[b][d] = add ? to[gg[b] * VS + d] : T(0);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            if (!o[b]) continue;
+            const int v = v0 + b * blockDim.x;
+            const int x = v % N1, y = (v / N1) % N1, z = v / (N1 * N1);
+            int cx = x / s, cy = y / s, cz = z / s;
+            if (cx == Lc) cx--;
+            if (cy == Lc) cy--;
+            if (cz == Lc) cz--;
+            const double tx = (double)(x - cx * s) / s, ty = (double)(y - cy * s) / s, tz = (double)(z - cz * s) / s;
+            double acc[VS];
+#pragma unroll
+            for (int d = 0; d < VS; d++) acc[d] = 0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int bx = hex8_cx(k), by = hex8_cy(k), bz = hex8_cz(k);
+                const double w = (bx ? tx : 1 - tx) * (by ? ty : 1 - ty) * (bz ? tz : 1 - tz);
+                if (w == 0) continue;
+                const double *src = tr_smem + lidx(Lc, cx + bx, cy + by, cz + bz) * VS;
+#pragma unroll
+                for (int d = 0; d < VS; d++) acc[d] += w * src[d];
+            }
+            const ptrdiff_t g = gg[b];
+#pragma unroll
+            for (int d = 0; d < VS; d++) {
+                if (!add)
+                    to[g * VS + d] = (T)acc[d];
+                else if (!((km[b] >> d) & 1u))
+                    to[g * VS + d] = old[b][d] + (T)acc[d];
+            }
         }
     }
 }
@@ -165,11 +189,29 @@ __global__ void __launch_bounds__(256) sshex8_restrict_lds_kernel(const ptrdiff_
     const int N1 = Lf + 1, nxe = N1 * N1 * N1, s = Lf / Lc, C1 = Lc + 1, cnxe = C1 * C1 * C1;
     const int nskc = sk_count(Lc);
     const int32_t *row = idx + e * nxe;
-    for (int v = threadIdx.x; v < nxe; v += blockDim.x) {
-        const ptrdiff_t g = row[v];
-        const double inv = 1.0 / (double)count[g];
+    // NB fine nodes per thread at a time: their ids, then their counts and values together
+    constexpr int NB = 4;
+    for (int v0 = threadIdx.x; v0 < nxe; v0 += NB * blockDim.x) {
+        ptrdiff_t gg[NB];
 #pragma unroll
-        for (int d = 0; d < VS; d++) tr_smem[v * VS + d] = (double)from[g * VS + d] * inv;
+        for (int b = 0; b < NB; b++) {
+            const int v = v0 + b * blockDim.x;
+            gg[b] = row[v < nxe ? v : 0];
+        }
+        double inv[NB], fv[NB][VS];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            inv[b] = 1.0 / (double)count[gg[b]];
+#pragma unroll
+            for (int d = 0; d < VS; d++) fv[b][d] = (double)from[gg[b] * VS + d];
+        }
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int v = v0 + b * blockDim.x;
+            if (v < nxe)
+#pragma unroll
+                for (int d = 0; d < VS; d++) tr_smem[v * VS + d] = fv[b][d] * inv[b];
+        }
     }
     __syncthreads();
     for (int c = threadIdx.x; c < cnxe; c += blockDim.x) {
