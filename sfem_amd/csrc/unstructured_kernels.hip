@@ -293,12 +293,29 @@ __global__ void __launch_bounds__(256)
     const int off = node_off[p], nloc = node_off[p + 1] - off;
     double *sy = reinterpret_cast<double *>(patch_lds);
     T *sx = reinterpret_cast<T *>(sy + 3 * nloc);
-    for (int i = threadIdx.x; i < nloc; i += blockDim.x) {
-        const ptrdiff_t g = nodes[off + i];
+    // NB patch nodes per thread at a time: their ids, then their x values, each batch issued together
+    constexpr int NB = 4;
+    for (int i0 = threadIdx.x; i0 < nloc; i0 += NB * blockDim.x) {
+        ptrdiff_t gg[NB];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            sx[i * 3 + c] = x[g * 3 + c];
-            sy[i * 3 + c] = 0;
+        for (int b = 0; b < NB; b++) {
+            const int i = i0 + b * blockDim.x;
+            gg[b] = nodes[off + (i < nloc ? i : 0)];
+        }
+        T xv[NB][3];
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) xv[b][c] = x[gg[b] * 3 + c];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int i = i0 + b * blockDim.x;
+            if (i < nloc)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    sx[i * 3 + c] = xv[b][c];
+                    sy[i * 3 + c] = 0;
+                }
         }
     }
     __syncthreads();
@@ -317,16 +334,33 @@ __global__ void __launch_bounds__(256)
             [&](int n, int d, T val) { atomicAdd(&sy[l[n] * 3 + d], (double)val); });
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nloc; i += blockDim.x) {
-        const ptrdiff_t g = nodes[off + i];
-        const ptrdiff_t b = bslot[off + i];
+    // the same batches: ids and boundary slots, then the old y (read for every node of the batch: a
+    // valid entry, used for the patch's own nodes), then the stores
+    for (int i0 = threadIdx.x; i0 < nloc; i0 += NB * blockDim.x) {
+        ptrdiff_t gg[NB], bb[NB];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const T v = (T)sy[i * 3 + c];
-            if (b < 0)
-                y[g * 3 + c] += v;
-            else
-                part[b * 3 + c] = v;
+        for (int b = 0; b < NB; b++) {
+            const int i = i0 + b * blockDim.x, ii = i < nloc ? i : 0;
+            gg[b] = nodes[off + ii];
+            bb[b] = bslot[off + ii];
+        }
+        T yo[NB][3];
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) yo[b][c] = y[gg[b] * 3 + c];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int i = i0 + b * blockDim.x;
+            if (i >= nloc) continue;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const T v = (T)sy[i * 3 + c];
+                if (bb[b] < 0)
+                    y[gg[b] * 3 + c] = yo[b][c] + v;
+                else
+                    part[bb[b] * 3 + c] = v;
+            }
         }
     }
 }
