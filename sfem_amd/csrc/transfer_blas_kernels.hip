@@ -106,18 +106,34 @@ __global__ void __launch_bounds__(256) sshex8_prolongate_lds_kernel(const ptrdif
     if (e >= E) return;  // uniform per block
     const int N1 = Lf + 1, nxe = N1 * N1 * N1, s = Lf / Lc, C1 = Lc + 1, cnxe = C1 * C1 * C1;
     const int32_t *row = idx + e * nxe;
-    for (int c = threadIdx.x; c < cnxe; c += blockDim.x) {
-        const int X = c % C1, Y = (c / C1) % C1, Z = c / (C1 * C1);
-        const ptrdiff_t cg = row[lidx(Lf, X * s, Y * s, Z * s)];
+    // NB nodes per thread at a time (coarse nodes here, fine nodes below): their ids, then their values
+    constexpr int NB = 4;
+    for (int c0 = threadIdx.x; c0 < cnxe; c0 += NB * blockDim.x) {
+        ptrdiff_t cg[NB];
 #pragma unroll
-        for (int d = 0; d < VS; d++) tr_smem[c * VS + d] = (double)from[cg * VS + d];
+        for (int b = 0; b < NB; b++) {
+            const int c = c0 + b * blockDim.x, cc = c < cnxe ? c : 0;
+            const int X = cc % C1, Y = (cc / C1) % C1, Z = cc / (C1 * C1);
+            cg[b] = row[lidx(Lf, X * s, Y * s, Z * s)];
+        }
+        T fv[NB][VS];
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+            for (int d = 0; d < VS; d++) fv[b][d] = from[cg[b] * VS + d];
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int c = c0 + b * blockDim.x;
+            if (c < cnxe)
+#pragma unroll
+                for (int d = 0; d < VS; d++) tr_smem[c * VS + d] = (double)fv[b][d];
+        }
     }
     __syncthreads();
     const uint8_t *ow = own + e * nxe;
     // NB fine nodes per thread at a time: their owner flags and ids, then (adding) their keep flags
     // and old values, each batch of loads issued together (unowned nodes read valid entries they
     // do not use)
-    constexpr int NB = 4;
     for (int v0 = threadIdx.x; v0 < nxe; v0 += NB * blockDim.x) {
         uint8_t o[NB];
         ptrdiff_t gg[NB];
@@ -216,6 +232,11 @@ __global__ void __launch_bounds__(256) sshex8_restrict_lds_kernel(const ptrdiff_
     __syncthreads();
     for (int c = threadIdx.x; c < cnxe; c += blockDim.x) {
         const int X = c % C1, Y = (c / C1) % C1, Z = c / (C1 * C1);
+        // the node's id and skeleton slot position are loaded before the sums from LDS, which hide
+        // their latency
+        const int q = sk_slot(Lc, X, Y, Z);
+        const ptrdiff_t cg = row[lidx(Lf, X * s, Y * s, Z * s)];
+        const ptrdiff_t slot = q < 0 ? 0 : (VS == 1 ? (ptrdiff_t)cpos[e * nskc + q] : (e * nskc + q));
         double acc[VS];
 #pragma unroll
         for (int d = 0; d < VS; d++) acc[d] = 0;
@@ -237,13 +258,10 @@ __global__ void __launch_bounds__(256) sshex8_restrict_lds_kernel(const ptrdiff_
                 }
             }
         }
-        const int q = sk_slot(Lc, X, Y, Z);
         if (q < 0) {
-            const ptrdiff_t cg = row[lidx(Lf, X * s, Y * s, Z * s)];
 #pragma unroll
             for (int d = 0; d < VS; d++) to[cg * VS + d] = overwrite ? (T)acc[d] : to[cg * VS + d] + (T)acc[d];
         } else {
-            const ptrdiff_t slot = VS == 1 ? (ptrdiff_t)cpos[e * nskc + q] : (e * nskc + q);
 #pragma unroll
             for (int d = 0; d < VS; d++) part[slot * VS + d] = (T)acc[d];
         }
