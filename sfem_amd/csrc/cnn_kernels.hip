@@ -478,10 +478,25 @@ __global__ void __launch_bounds__(512, 2) cnn_infer_kernel(const ptrdiff_t E, co
 
     // the input on its haloed grid; actX zeroed (the layers write only the interior points, so its
     // halo stays zero for both signs)
-    for (int q = tid; q < NH; q += S::THREADS) {
-        const int px = q % P1 - 1, py = (q / P1) % P1 - 1, pz = q / (P1 * P1) - 1;
-        const bool in = (unsigned)px <= (unsigned)L && (unsigned)py <= (unsigned)L && (unsigned)pz <= (unsigned)L;
-        zs[q] = in ? zin[(size_t)e * S::NXE + (pz * S::N1 + py) * S::N1 + px] : 0.f;
+    {
+        // all of this thread's input loads at once (halo and tail points read the element's first
+        // value, a valid address, and store zero)
+        constexpr int NBI = (NH + S::THREADS - 1) / S::THREADS;
+        float zv[NBI];
+#pragma unroll
+        for (int b = 0; b < NBI; b++) {
+            const int q = tid + b * S::THREADS;
+            const int px = q % P1 - 1, py = (q / P1) % P1 - 1, pz = q / (P1 * P1) - 1;
+            const bool in = q < NH && (unsigned)px <= (unsigned)L && (unsigned)py <= (unsigned)L &&
+                            (unsigned)pz <= (unsigned)L;
+            const float v = zin[(size_t)e * S::NXE + (in ? (pz * S::N1 + py) * S::N1 + px : 0)];
+            zv[b] = in ? v : 0.f;
+        }
+#pragma unroll
+        for (int b = 0; b < NBI; b++) {
+            const int q = tid + b * S::THREADS;
+            if (q < NH) zs[q] = zv[b];
+        }
     }
     for (int i = tid; i < 2 * NA; i += S::THREADS) ((mf_u4 *)actX)[i] = mf_u4{0u, 0u, 0u, 0u};
     // weights in lane order: layer 1 [64][8], layers 2-3 [14][64][8], layer 4 [2][64][4]
