@@ -64,7 +64,9 @@ const char *sfem_amd_last_error(void);
  * 2 the stencil form (levels 2-6, 8, 10, 12, 16).  Both give the reference's operator; the
  * summation order differs.  "hex8_patches" (default 1): hex8 elasticity applies on meshes of at least
  * 2^15 elements run on element patches (512 elements grown over shared nodes, accumulated in LDS,
- * patch-boundary nodes summed in a fixed order); 0 runs one element per lane with atomics. */
+ * patch-boundary nodes summed in a fixed order); 0 runs one element per lane with atomics.
+ * "mixed_cg" (default 1): flexible CGs created afterwards around a bridged FP32 preconditioner keep
+ * z in FP32 (see sfem_amd_cg_create; the same iterates); 0 runs them through the bridge's casts. */
 int sfem_amd_set_option(const char *name, int value);
 /* Number of visible GPUs; 0 when no HIP device is present. */
 int sfem_amd_device_count(void);
@@ -278,7 +280,12 @@ int sfem_amd_cheb3_start(int rt, ptrdiff_t n, double alpha, const void *rhs, con
  * preconditioner PCG stopping on ||r||/||b||.  Applying the returned op solves A x = b
  * (x: initial guess and output).  The op keeps shared references to A and precond, whose
  * overwriting apply it calls for z = P r (a zero fill and the apply, unless the operator writes
- * z directly, as the precision bridge does). */
+ * z directly, as the precision bridge does).  Mode 1 with an FP64 A and a precision bridge over an
+ * FP32 preconditioner (or a graph op over one) keeps z in FP32: the CG update also writes the
+ * FP32 residual the preconditioner reads, the dots and the direction update read z in FP32, and
+ * the bridge's FP32 part is replayed as a graph of its own when precond was a graph op; the
+ * iterates are those of the FP64 steps on (double) z, bit for bit (set_option "mixed_cg" 0 runs
+ * the FP64 steps through the bridge instead). */
 int sfem_amd_cg_create(sfem_amd_op *A, sfem_amd_op *precond, int mode, double rtol,
                        double atol, int max_it, sfem_amd_op **out);
 /* sfem::Chebyshev3<real_t> (algebra/sfem_Chebyshev3.hpp:57-303) of degree `max_it`; eig_max

@@ -157,6 +157,7 @@ uint64_t state_epoch() { return g_state_epoch.load(std::memory_order_acquire); }
 void bump_state_epoch() { g_state_epoch.fetch_add(1, std::memory_order_acq_rel); }
 
 bool g_fused_chebyshev = false;
+bool g_mixed_cg = true;
 bool g_atomic_skeleton = false;
 int g_elasticity_form = 0;
 bool g_hex8_patches = true;
@@ -1083,6 +1084,132 @@ struct SwitchOp final : OpBase {
         return ops[0]->zero_guess_ignores_x() && ops[1]->zero_guess_ignores_x();
     }
 };
+
+// zf = P32(rf) from zf = 0 on a precision bridge's own FP32 buffers (rf: its xf, zf: its yf): the
+// FP32 part of the bridge's overwriting apply, without the casts around it
+struct BridgeCore final : OpBase {
+    std::shared_ptr<PrecisionBridge> bridge;
+    int real_type() const override { return SFEM_AMD_FLOAT32; }
+    ptrdiff_t rows() const override { return bridge->rows(); }
+    int apply(const void *x, void *y, hipStream_t s) override { return apply_overwrite(x, y, s); }
+    int apply_overwrite(const void *x, void *y, hipStream_t s) override {
+        if (blas_values<float>(rows(), 0.f, (float *)y, s)) return SFEM_FAILURE;
+        bridge->op32->set_initial_guess_zero(true);
+        const int err = bridge->op32->apply(x, y, s);
+        bridge->op32->set_initial_guess_zero(false);
+        return err;
+    }
+};
+
+// Flexible PCG (the steps of sfem::ConjugateGradient<double>::flexible_pcg) for an FP32
+// preconditioner behind a precision bridge, with the preconditioned residual kept in FP32: the CG
+// update also writes the FP32 residual the preconditioner reads, and the dots and the direction
+// update read z in FP32, so neither of the bridge's casts nor an FP64 copy of z is made (32 of the
+// 140 bytes per unknown that an iteration's vector operations move).  Each step is the FP64 step
+// on (double) z, which the bridge's cast makes exactly: the iterates are the same.
+struct MixedFlexibleCG final : sfem::MatrixFreeLinearSolver<double> {
+    std::shared_ptr<OpBase> A;                // FP64 operator
+    std::shared_ptr<PrecisionBridge> bridge;  // its xf and yf hold rf and zf
+    std::shared_ptr<OpBase> core;             // BridgeCore, replayed as a graph when the CG's was
+    double rtol{1e-10}, atol{1e-16};
+    int max_it{10000};
+    int residual_replacement{50};
+    int iterations_{0};
+    int last_status{SFEM_FAILURE};
+    std::vector<double> history;
+    std::function<void(int, double, double)> on_iteration;
+    DevArray<double> r_, p_, Ap_;
+
+    int iterations() const override { return iterations_; }
+    void set_op(const sfem::SharedOperator<double> &) override {}
+    void set_preconditioner_op(const sfem::SharedOperator<double> &) override {}
+    void set_max_it(int it) override { max_it = it; }
+    void set_n_dofs(std::ptrdiff_t) override {}
+    std::ptrdiff_t rows() const override { return A->rows(); }
+    std::ptrdiff_t cols() const override { return A->rows(); }
+    sfem::ExecutionSpace execution_space() const override { return sfem::EXECUTION_SPACE_DEVICE; }
+
+    int apply(const double *const b, double *const x) override {
+        history.clear();
+        last_status = solve(b, x);
+        return last_status;
+    }
+
+private:
+    int residual(const double *b, const double *x, double *r, hipStream_t s) {
+        if (A->apply_overwrite(x, r, s)) return SFEM_FAILURE;
+        return blas_axpby<double>(rows(), 1.0, b, -1.0, r, s);
+    }
+    double dot(const double *x, const double *y, hipStream_t s) {
+        double v = NAN;
+        return blas_dot<double>(rows(), x, y, reduction_ws(), &v, s) ? NAN : v;
+    }
+    double dot(const double *x, const float *y, hipStream_t s) {
+        double v = NAN;
+        return blas_dot_mixed(rows(), x, y, reduction_ws(), &v, s) ? NAN : v;
+    }
+    // rf = (float) r was written; zf = P32(rf)
+    int precondition(hipStream_t s) { return core->apply_overwrite(bridge->xf.p, bridge->yf.p, s); }
+
+    int solve(const double *b, double *x) {
+        const ptrdiff_t n = rows();
+        hipStream_t s = as_stream(nullptr);
+        if ((ptrdiff_t)r_.n != n && (r_.alloc(n) || p_.alloc(n) || Ap_.alloc(n))) return SFEM_FAILURE;
+        if ((ptrdiff_t)bridge->xf.n != n && (bridge->xf.alloc(n) || bridge->yf.alloc(n))) return SFEM_FAILURE;
+        if (!reduction_ws()) return SFEM_FAILURE;
+        double *r = r_.p, *p = p_.p, *Ap = Ap_.p;
+        float *rf = bridge->xf.p, *zf = bridge->yf.p;
+        const double bnorm = std::sqrt(dot(b, b, s));
+        if (residual(b, x, r, s)) return SFEM_FAILURE;
+        double rnorm = std::sqrt(dot(r, r, s));
+        iterations_ = 0;
+        const double ref = bnorm > 0 ? bnorm : 1.0;
+        if (rnorm == 0 || rnorm / ref < rtol || rnorm < atol) return SFEM_SUCCESS;
+        if (blas_cast<double, float>(n, r, rf, s) || precondition(s)) return SFEM_FAILURE;
+        if (blas_cast<float, double>(n, zf, p, s)) return SFEM_FAILURE;
+        double rtz = dot(r, zf, s);
+        for (iterations_ = 0; iterations_ < max_it; iterations_++) {
+            if (A->apply_overwrite(p, Ap, s)) return SFEM_FAILURE;
+            const double ptAp = dot(p, Ap, s);
+            const double alpha = rtz / ptAp;
+            const bool replace = residual_replacement > 0 && (iterations_ + 1) % residual_replacement == 0;
+            double rtr = 0, rtzold = 0;
+            if (cg_update_mixed(n, alpha, p, Ap, x, r, rf, replace ? nullptr : zf, reduction_ws(), &rtr, &rtzold, s))
+                return SFEM_FAILURE;
+            bool cast_r = false;  // rf no longer holds (float) r
+            if (replace) {
+                if (residual(b, x, r, s)) return SFEM_FAILURE;
+                rtr = dot(r, r, s);
+                rtzold = dot(r, zf, s);
+                cast_r = true;
+            }
+            const double rprev = rnorm;
+            rnorm = std::sqrt(rtr);
+            history.push_back(rnorm / ref);
+            if (on_iteration) on_iteration(iterations_, rprev, rnorm);
+            if (rnorm / ref < rtol || rnorm < atol) {
+                // confirm with the true residual
+                if (residual(b, x, r, s)) return SFEM_FAILURE;
+                const double true_norm = std::sqrt(dot(r, r, s));
+                if (true_norm / ref < rtol || true_norm < atol) {
+                    iterations_++;
+                    if (!history.empty()) history.back() = true_norm / ref;
+                    return SFEM_SUCCESS;
+                }
+                rnorm = true_norm;
+                rtzold = dot(r, zf, s);
+                cast_r = true;
+            }
+            if (cast_r && blas_cast<double, float>(n, r, rf, s)) return SFEM_FAILURE;
+            if (precondition(s)) return SFEM_FAILURE;
+            const double rtz_new = dot(r, zf, s);
+            const double beta = (rtz_new - rtzold) / rtz;
+            rtz = rtz_new;
+            if (blas_axpby_mixed(n, 1.0, zf, beta, p, s)) return SFEM_FAILURE;
+        }
+        return SFEM_FAILURE;
+    }
+};
 }  // namespace
 
 #define CHECK_HANDLE(h)                                  \
@@ -1113,6 +1240,10 @@ int sfem_amd_set_option(const char *name, int value) {
     if (std::strcmp(name, "fused_chebyshev") == 0) {
         g_fused_chebyshev = value != 0;
         bump_state_epoch();
+        return SFEM_SUCCESS;
+    }
+    if (std::strcmp(name, "mixed_cg") == 0) {
+        g_mixed_cg = value != 0;
         return SFEM_SUCCESS;
     }
     if (std::strcmp(name, "atomic_skeleton") == 0) {
@@ -1700,6 +1831,47 @@ extern "C++" {
 template <typename T>
 static int cg_create_t(sfem_amd_op *A, sfem_amd_op *P, int flexible, double rtol, double atol, int max_it,
                        sfem_amd_op **out) {
+    if constexpr (std::is_same<T, double>::value) {
+        // flexible CG around an FP32 preconditioner behind a precision bridge (itself, or replayed as
+        // a graph): the same iterates with z kept in FP32 (option "mixed_cg", default 1)
+        if (P && flexible == 1 && g_mixed_cg) {
+            std::shared_ptr<PrecisionBridge> br = std::dynamic_pointer_cast<PrecisionBridge>(P->impl);
+            bool graph = false;
+            if (!br) {
+                if (auto g = std::dynamic_pointer_cast<GraphOp>(P->impl)) {
+                    br = std::dynamic_pointer_cast<PrecisionBridge>(g->inner);
+                    graph = br != nullptr;
+                }
+            }
+            if (br) {
+                auto cg = std::make_shared<MixedFlexibleCG>();
+                cg->A = A->impl;
+                cg->bridge = br;
+                auto core = std::make_shared<BridgeCore>();
+                core->bridge = br;
+                if (graph) {
+                    auto g = std::make_shared<GraphOp>();
+                    g->inner = core;
+                    cg->core = g;
+                } else {
+                    cg->core = core;
+                }
+                cg->rtol = rtol;
+                cg->atol = atol;
+                cg->max_it = max_it;
+                auto s = std::make_shared<SolverOp<double>>();
+                s->op = cg;
+                s->n = A->impl->rows();
+                MixedFlexibleCG *raw = cg.get();
+                s->its = [raw] { return raw->iterations(); };
+                s->status = [raw] { return raw->last_status; };
+                s->hist = [raw] { return &raw->history; };
+                s->keep.push_back(P->impl);
+                *out = wrap(s);
+                return SFEM_SUCCESS;
+            }
+        }
+    }
     auto cg = std::make_shared<sfem::ConjugateGradient<T>>();
     cg->default_init();
     cg->set_op(view<T>(A->impl, true));
@@ -2155,6 +2327,11 @@ int sfem_amd_cg_add_fallback(sfem_amd_op *cg, sfem_amd_op *sw, double max_ratio)
         };
     };
     if (auto *s = dynamic_cast<SolverOp<double> *>(cg->impl.get())) {
+        if (auto *m = dynamic_cast<MixedFlexibleCG *>(s->op.get())) {
+            chain(m->on_iteration);
+            s->keep.push_back(swp);
+            return SFEM_SUCCESS;
+        }
         auto *c = dynamic_cast<sfem::ConjugateGradient<double> *>(s->op.get());
         SFEM_AMD_REQUIRE(c && c->flexible, "cg_add_fallback: needs a flexible CG (mode 1)");
         chain(c->on_iteration);

@@ -433,10 +433,11 @@ int sshex8_to_hex8(ptrdiff_t E, int L, const int32_t *ss, int32_t *h8, hipStream
 }
 
 // ------------------------------------------------------------------ BLAS-1
-template <typename T>
-__global__ void axpby_kernel(const ptrdiff_t n, const T a, const T *__restrict__ x, const T b, T *__restrict__ y) {
+// y = a x + b y (x may be FP32 beside an FP64 y: converted exactly, the same arithmetic)
+template <typename TX, typename T>
+__global__ void axpby_kernel(const ptrdiff_t n, const T a, const TX *__restrict__ x, const T b, T *__restrict__ y) {
     for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
-        y[i] = a * x[i] + b * y[i];
+        y[i] = a * (T)x[i] + b * y[i];
 }
 template <typename T>
 __global__ void zaxpby_kernel(const ptrdiff_t n, const T a, const T *__restrict__ x, const T b, const T *__restrict__ y,
@@ -612,9 +613,9 @@ __device__ __forceinline__ double block_sum(double v) {
     return r;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock) dot_partial_kernel(const ptrdiff_t n, const T *__restrict__ x,
-                                                             const T *__restrict__ y, double *__restrict__ part) {
+template <typename TX, typename TY>
+__global__ void __launch_bounds__(kBlock) dot_partial_kernel(const ptrdiff_t n, const TX *__restrict__ x,
+                                                             const TY *__restrict__ y, double *__restrict__ part) {
     double acc = 0;
     for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x)
         acc += (double)x[i] * (double)y[i];
@@ -631,17 +632,19 @@ __global__ void __launch_bounds__(kBlock) sum_partials_kernel(const int nb, cons
 }
 
 // CG fused update (algebra/sfem_cg.hpp:26-72 host fusion): x += a p; r -= a Ap; partial (r,r) and,
-// with z, partial (r,z) of the updated r at part[kRedBlocks + block]
-template <typename T>
+// with z, partial (r,z) of the updated r at part[kRedBlocks + block]; with rf, the updated r rounded
+// to FP32 into rf (the input of an FP32 preconditioner)
+template <typename T, typename TZ>
 __global__ void __launch_bounds__(kBlock) cg_update_kernel(const ptrdiff_t n, const T a, const T *__restrict__ p,
                                                            const T *__restrict__ Ap, T *__restrict__ x,
-                                                           T *__restrict__ r, const T *__restrict__ z,
-                                                           double *__restrict__ part) {
+                                                           T *__restrict__ r, const TZ *__restrict__ z,
+                                                           float *__restrict__ rf, double *__restrict__ part) {
     double acc = 0, acz = 0;
     for (ptrdiff_t i = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (ptrdiff_t)gridDim.x * blockDim.x) {
         x[i] += a * p[i];
         const T ri = r[i] - a * Ap[i];
         r[i] = ri;
+        if (rf) rf[i] = (float)ri;
         acc += (double)ri * (double)ri;
         if (z) acz += (double)ri * (double)z[i];
     }
@@ -682,7 +685,7 @@ __global__ void copy_dofs_kernel(const ptrdiff_t n, const int32_t *__restrict__ 
 template <typename T>
 int blas_axpby(ptrdiff_t n, T a, const T *x, T b, T *y, hipStream_t s) {
     if (n <= 0) return SFEM_SUCCESS;
-    axpby_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, a, x, b, y);
+    axpby_kernel<T, T><<<grid_for(n), kBlock, 0, s>>>(n, a, x, b, y);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
 }
@@ -798,7 +801,7 @@ static int finish_sums(int nb, int k, double *ws, double *out, hipStream_t s) {
 template <typename T>
 int blas_dot(ptrdiff_t n, const T *x, const T *y, double *ws, double *result_host, hipStream_t s) {
     const int nb = red_blocks(n);
-    dot_partial_kernel<T><<<nb, kBlock, 0, s>>>(n, x, y, ws);
+    dot_partial_kernel<T, T><<<nb, kBlock, 0, s>>>(n, x, y, ws);
     return finish_sums(nb, 1, ws, result_host, s);
 }
 
@@ -848,7 +851,7 @@ int blas_dot_noalloc(ptrdiff_t n, const T *x, const T *y, double *result, hipStr
     }
     double *ws = scratch_ptr();
     if (!ws) return SFEM_FAILURE;
-    dot_partial_kernel<T><<<nb, kBlock, 0, s>>>(n, x, y, ws);
+    dot_partial_kernel<T, T><<<nb, kBlock, 0, s>>>(n, x, y, ws);
     sum_partials_kernel<<<1, kBlock, 0, s>>>(nb, ws, result);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
@@ -858,13 +861,40 @@ template <typename T>
 int cg_update(ptrdiff_t n, T a, const T *p, const T *Ap, T *x, T *r, double *ws, double *rtr_host, hipStream_t s,
               const T *z, double *rz_host) {
     const int nb = red_blocks(n);
-    cg_update_kernel<T><<<nb, kBlock, 0, s>>>(n, a, p, Ap, x, r, rtr_host ? z : nullptr, rtr_host ? ws : nullptr);
+    cg_update_kernel<T, T><<<nb, kBlock, 0, s>>>(n, a, p, Ap, x, r, rtr_host ? z : nullptr, nullptr,
+                                                rtr_host ? ws : nullptr);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     if (!rtr_host) return SFEM_SUCCESS;
     double out[2] = {0, 0};
     if (finish_sums(nb, z ? 2 : 1, ws, out, s)) return SFEM_FAILURE;
     *rtr_host = out[0];
     if (z && rz_host) *rz_host = out[1];
+    return SFEM_SUCCESS;
+}
+
+// The steps of a flexible CG whose preconditioner runs in FP32 (z stays FP32; the outer vectors
+// are FP64): each is the FP64 step with z converted exactly, so the results are those of the
+// all-FP64 steps on (double) z.
+int blas_dot_mixed(ptrdiff_t n, const double *x, const float *y, double *ws, double *result_host, hipStream_t s) {
+    const int nb = red_blocks(n);
+    dot_partial_kernel<double, float><<<nb, kBlock, 0, s>>>(n, x, y, ws);
+    return finish_sums(nb, 1, ws, result_host, s);
+}
+int blas_axpby_mixed(ptrdiff_t n, double a, const float *x, double b, double *y, hipStream_t s) {
+    axpby_kernel<float, double><<<grid_for(n), kBlock, 0, s>>>(n, a, x, b, y);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+// x += a p; r -= a Ap; rf = (float) r; *rtr_host = (r, r) and, with zf, *rz_host = (r, zf)
+int cg_update_mixed(ptrdiff_t n, double a, const double *p, const double *Ap, double *x, double *r, float *rf,
+                    const float *zf, double *ws, double *rtr_host, double *rz_host, hipStream_t s) {
+    const int nb = red_blocks(n);
+    cg_update_kernel<double, float><<<nb, kBlock, 0, s>>>(n, a, p, Ap, x, r, zf, rf, ws);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    double out[2] = {0, 0};
+    if (finish_sums(nb, zf ? 2 : 1, ws, out, s)) return SFEM_FAILURE;
+    *rtr_host = out[0];
+    if (zf && rz_host) *rz_host = out[1];
     return SFEM_SUCCESS;
 }
 

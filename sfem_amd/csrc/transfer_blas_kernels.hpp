@@ -62,6 +62,12 @@ template <typename T>
 // with z (and rtr_host): also (r, z) of the updated r into *rz_host
 int cg_update(ptrdiff_t n, T a, const T *p, const T *Ap, T *x, T *r, double *ws, double *rtr_host, hipStream_t s,
               const T *z = nullptr, double *rz_host = nullptr);
+// flexible CG with an FP32 preconditioner: (x, y) for FP64 x and FP32 y; y = a x + b y for FP32 x;
+// the CG update that also writes rf = (float) r and, with zf, returns (r, zf) in *rz_host
+int blas_dot_mixed(ptrdiff_t n, const double *x, const float *y, double *ws, double *result_host, hipStream_t s);
+int blas_axpby_mixed(ptrdiff_t n, double a, const float *x, double b, double *y, hipStream_t s);
+int cg_update_mixed(ptrdiff_t n, double a, const double *p, const double *Ap, double *x, double *r, float *rf,
+                    const float *zf, double *ws, double *rtr_host, double *rz_host, hipStream_t s);
 template <typename T> int dofs_set_value(ptrdiff_t n, const int32_t *dofs, T v, T *x, hipStream_t s);
 template <typename T> int dofs_copy(ptrdiff_t n, const int32_t *dofs, const T *src, T *dst, hipStream_t s);
 size_t reduction_workspace_doubles();

@@ -327,6 +327,55 @@ def test_mixed_precision_flexible_cg_reaches_fp64_tolerance():
     assert solver.last_status == 0 and res < 1e-10
 
 
+@pytest.mark.parametrize("precond", ["vcycle", "vcycle_graph", "jacobi"])
+def test_mixed_flexible_cg_keeps_z_in_fp32_same_iterates(precond):
+    """A flexible CG around an FP32 preconditioner behind a precision bridge keeps z in FP32 (option
+    mixed_cg, the default): the CG update writes the FP32 residual and the dots and the direction
+    update read z in FP32.  Its iterates are those of the generic flexible CG on the bridge's
+    (double) z, bit for bit: iteration count, residual history and solution.  The FP32 Jacobi case
+    runs past the residual replacement (every 50 iterations)."""
+    L = 8
+    mel, mpts, el, pts, fff, N, nodes, coeff = _ss_poisson(L, 4, coeff_seed=7)
+    mesh = S.Mesh(S.SSHEX8, L, el, N)
+    h64 = gmg.SSGMG(mesh, "laplacian", fff, nodes, coeff=coeff)
+    # (a graph replay needs a V-cycle without host synchronisation: the dense coarse solve)
+    h32 = gmg.SSGMG(mesh, "laplacian", fff, nodes, coeff=coeff, real_type=S.FLOAT32, coarse="dense")
+    if precond == "jacobi":
+        d32 = zeros(N, np.float32)
+        h32.fine.A.diag(d32)
+        S.synchronize()
+        P = S.precision_bridge(S.inverse_diagonal(d32))
+    else:
+        P = S.precision_bridge(h32.mg)
+        if precond == "vcycle_graph":
+            P = S.graph_op(P)
+    b = dev(np.where(np.isin(np.arange(N), nodes), 0.0, np.random.default_rng(0).uniform(0, 1, N)))
+    runs = {}
+    for mode in (1, 0):
+        S.set_option("mixed_cg", mode)
+        try:
+            solver = S.cg(h64.fine.A, P, flexible=True, rtol=1e-10, max_it=1000)
+        finally:
+            S.set_option("mixed_cg", 1)
+        sols = []
+        for _ in range(2):  # the second solve replays the graphs
+            x = zeros(N)
+            solver.apply(b, x)
+            S.synchronize()
+            sols.append(host(x))
+        runs[mode] = (solver.iterations, solver.last_status, solver.history(), sols)
+    its, st, hist, sols = runs[1]
+    print(f"{precond}: {its} iterations (generic flexible CG {runs[0][0]})")
+    assert st == 0 and runs[0][1] == 0
+    assert its == runs[0][0]
+    if precond == "jacobi":
+        assert its > 50
+    assert np.array_equal(hist, runs[0][2])
+    for a, g in zip(sols, runs[0][3]):
+        assert np.array_equal(a, g)
+    assert true_residual(h64.fine.A, b, dev(sols[1])) < 1e-10
+
+
 def test_elasticity_mg_pcg_converges():
     L = 4
     mel, mpts = O.box_mesh(4, 4, 4)
@@ -524,7 +573,18 @@ def test_graph_replayed_vcycle_matches_eager(real_type):
         assert np.linalg.norm(b - host(rr)) / np.linalg.norm(b) < 1e-10
         its.append(solver.iterations)
     assert its[0] == its[1]
-    # the CG applies its preconditioner in zero-initial-guess mode: a second graph, replayed
+    # the CG applies its preconditioner in zero-initial-guess mode: a second graph, replayed (around
+    # the bridged FP32 V-cycle the flexible CG keeps z in FP32 and replays a graph of the bridge's
+    # FP32 part of its own; with the option mixed_cg 0 it replays this one)
+    if real_type:
+        S.set_option("mixed_cg", 0)
+        try:
+            solver = S.cg(A64, P_graph, flexible=True, rtol=1e-10, max_it=200)
+        finally:
+            S.set_option("mixed_cg", 1)
+        x = zeros(N)
+        solver.apply(dev(b), x)
+        assert solver.iterations == its[1]
     captured, replays2 = S.graph_op_stats(P_graph)
     assert captured == 2 and replays2 >= replays + its[1] - 2
 
