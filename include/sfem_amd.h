@@ -403,7 +403,9 @@ int sfem_amd_dense_op_create(int real_type, ptrdiff_t n, const double *matrix, s
 /* Wraps `op` so its apply replays a captured HIP graph (no counterpart in the reference): the
  * first apply runs eagerly, the second is captured and launched, later applies with the same
  * (x, y, stream) relaunch the graph and others run eagerly.  For launch-bound sequences without
- * host synchronisation, e.g. a preconditioning V-cycle (max_it 1) with a dense coarse solve. */
+ * host synchronisation, e.g. a preconditioning V-cycle (max_it 1) with a dense coarse solve; an
+ * op whose apply synchronises with the host (a coarse CG) cannot be captured, and its second apply
+ * returns the capture error. */
 int sfem_amd_graph_op_create(sfem_amd_op *op, sfem_amd_op **out);
 /* captured: graphs instantiated (one per initial-guess mode and per accumulating or overwriting
  * apply: the zero-guess overwriting apply a CG makes of its preconditioner records a different

@@ -989,7 +989,9 @@ struct DenseOp final : OpBase {
 // kernel attributes), the second captures the launch sequence on its stream and launches the
 // graph; later calls with the same (x, y, stream) and mode relaunch it, other arguments run
 // eagerly.  The inner apply must not synchronise with the host (e.g. a preconditioning V-cycle
-// with a direct coarse solve).  A failed capture leaves the operator eager.
+// with a direct coarse solve).  A capture that cannot begin leaves the operator eager; an inner
+// apply that synchronises inside the capture invalidates it, and the HIP runtime then reports
+// that error on the next launch on the stream, so such an apply fails instead of running eagerly.
 struct GraphOp final : OpBase {
     std::shared_ptr<OpBase> inner;
     struct Capture {
