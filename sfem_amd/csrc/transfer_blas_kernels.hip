@@ -1,5 +1,6 @@
 // gfx950 kernels: sshex8 transfers, BLAS-1 (deterministic reductions), fused solver
 // updates and Dirichlet helpers.  T = double (plain method) or float (FP32 preconditioner).
+#include <initializer_list>
 #include <mutex>
 #include <string>
 
@@ -682,9 +683,174 @@ __global__ void copy_dofs_kernel(const ptrdiff_t n, const int32_t *__restrict__ 
         dst[dofs[i]] = src[dofs[i]];
 }
 
+// ---- streaming kernels over 16-byte vectors
+// When every array of a streaming operation is 16-byte aligned, each thread moves 16 bytes per
+// array (4 floats or 2 doubles) with one load or store instruction: fewer, wider memory
+// instructions per byte (a one-float-per-thread copy of 171M floats runs at 3.7 TB/s on MI355X,
+// a 2-GiB FP64 copy at 6.3 TB/s).  The element arithmetic is the scalar kernels', so the results
+// are the same; the last n % W elements go to the first threads of block 0.  Used for FP32 arrays:
+// FP64 ones (2 doubles per thread) made the FP64 multigrid solve of 171M unknowns 3.5% slower
+// than one double per thread, which already streams at the FP64 copy's rate.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <typename T>
+using vec16_t = typename Vec16<T>::type;
+
+template <typename T>
+static inline bool aligned16(std::initializer_list<const void *> ps) {
+    if (sizeof(T) != 4) return false;
+    for (const void *q : ps)
+        if (q && ((uintptr_t)q & 15u) != 0) return false;
+    return true;
+}
+
+template <typename T, typename F>
+__global__ void __launch_bounds__(kBlock) stream_vec_kernel(const ptrdiff_t n, const F f) {
+    constexpr int W = 16 / sizeof(T);
+    const ptrdiff_t nv = n / W;
+    for (ptrdiff_t v = (ptrdiff_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (ptrdiff_t)gridDim.x * blockDim.x)
+        f.vec(v);
+    const ptrdiff_t t = nv * W + threadIdx.x;
+    if (blockIdx.x == 0 && t < n) f.one(t);
+}
+template <typename T, typename F>
+static int launch_stream_vec(ptrdiff_t n, const F &f, hipStream_t s) {
+    constexpr int W = 16 / sizeof(T);
+    stream_vec_kernel<T, F><<<grid_for(n / W + 1), kBlock, 0, s>>>(n, f);
+    SFEM_AMD_HIP_CHECK(hipGetLastError());
+    return SFEM_SUCCESS;
+}
+#define SFEM_VEC_LOAD(V, ptr, v) (reinterpret_cast<const V *>(ptr)[v])
+#define SFEM_VEC_STORE(V, ptr, v, val) (reinterpret_cast<V *>(ptr)[v] = (val))
+
+// y = a x + b y
+template <typename T>
+struct AxpbyF {
+    T a, b;
+    const T *x;
+    T *y;
+    __device__ void one(ptrdiff_t i) const { y[i] = a * x[i] + b * y[i]; }
+    __device__ void vec(ptrdiff_t v) const {
+        using V = vec16_t<T>;
+        const V xv = SFEM_VEC_LOAD(V, x, v), yv = SFEM_VEC_LOAD(V, y, v);
+        V o;
+#pragma unroll
+        for (int k = 0; k < (int)(16 / sizeof(T)); k++) o[k] = a * xv[k] + b * yv[k];
+        SFEM_VEC_STORE(V, y, v, o);
+    }
+};
+template <typename T>
+struct ValuesF {
+    T a;
+    T *x;
+    __device__ void one(ptrdiff_t i) const { x[i] = a; }
+    __device__ void vec(ptrdiff_t v) const {
+        using V = vec16_t<T>;
+        V o;
+#pragma unroll
+        for (int k = 0; k < (int)(16 / sizeof(T)); k++) o[k] = a;
+        SFEM_VEC_STORE(V, x, v, o);
+    }
+};
+template <typename T>
+struct CopyF {
+    const T *x;
+    T *y;
+    __device__ void one(ptrdiff_t i) const { y[i] = x[i]; }
+    __device__ void vec(ptrdiff_t v) const {
+        using V = vec16_t<T>;
+        SFEM_VEC_STORE(V, y, v, SFEM_VEC_LOAD(V, x, v));
+    }
+};
+// Chebyshev3 steps, as cheb3_update_kernel / cheb3_update_diag_kernel (DIAG) and cheb3_start_kernel
+template <typename T, bool DIAG>
+struct Cheb3UpdateF {
+    T alpha, beta;
+    const T *rhs, *Ax, *dinv;
+    T *p, *x;
+    __device__ void one(ptrdiff_t i) const {
+        const T pi = cheb3_direction<T>(p[i], rhs[i], Ax[i], beta, DIAG ? dinv : nullptr, i, 0);
+        p[i] = pi;
+        x[i] = __builtin_fma(-alpha, pi, x[i]);
+    }
+    __device__ void vec(ptrdiff_t v) const {
+        using V = vec16_t<T>;
+        const V pv = SFEM_VEC_LOAD(V, p, v), rv = SFEM_VEC_LOAD(V, rhs, v), av = SFEM_VEC_LOAD(V, Ax, v),
+                xv = SFEM_VEC_LOAD(V, x, v);
+        V dv;
+        if (DIAG) dv = SFEM_VEC_LOAD(V, dinv, v);
+        V po, xo;
+#pragma unroll
+        for (int k = 0; k < (int)(16 / sizeof(T)); k++) {
+            const T dk = DIAG ? dv[k] : T(0);
+            const T pi = cheb3_direction<T>(pv[k], rv[k], av[k], beta, DIAG ? &dk : nullptr, 0, 0);
+            po[k] = pi;
+            xo[k] = __builtin_fma(-alpha, pi, xv[k]);
+        }
+        SFEM_VEC_STORE(V, p, v, po);
+        SFEM_VEC_STORE(V, x, v, xo);
+    }
+};
+template <typename T>
+struct Cheb3StartF {
+    T alpha;
+    const T *rhs, *dinv;
+    int have_ax;
+    T *p, *x;
+    __device__ void one(ptrdiff_t i) const {
+        if (have_ax) {
+            const T pi = cheb3_direction<T>(T(0), rhs[i], p[i], T(0), dinv, i, 1);
+            p[i] = pi;
+            x[i] = __builtin_fma(-alpha, pi, x[i]);
+        } else {
+            T pi = -rhs[i];
+            if (dinv) pi = dinv[i] * pi;
+            p[i] = pi;
+            x[i] = -(alpha * pi);
+        }
+    }
+    __device__ void vec(ptrdiff_t v) const {
+        using V = vec16_t<T>;
+        constexpr int W = 16 / sizeof(T);
+        const V rv = SFEM_VEC_LOAD(V, rhs, v);
+        V dv, pv, xv, po, xo;
+        if (dinv) dv = SFEM_VEC_LOAD(V, dinv, v);
+        if (have_ax) {
+            pv = SFEM_VEC_LOAD(V, p, v);
+            xv = SFEM_VEC_LOAD(V, x, v);
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                const T dk = dinv ? dv[k] : T(0);
+                const T pi = cheb3_direction<T>(T(0), rv[k], pv[k], T(0), dinv ? &dk : nullptr, 0, 1);
+                po[k] = pi;
+                xo[k] = __builtin_fma(-alpha, pi, xv[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < W; k++) {
+                T pi = -rv[k];
+                if (dinv) pi = dv[k] * pi;
+                po[k] = pi;
+                xo[k] = -(alpha * pi);
+            }
+        }
+        SFEM_VEC_STORE(V, p, v, po);
+        SFEM_VEC_STORE(V, x, v, xo);
+    }
+};
+
 template <typename T>
 int blas_axpby(ptrdiff_t n, T a, const T *x, T b, T *y, hipStream_t s) {
     if (n <= 0) return SFEM_SUCCESS;
+    if (aligned16<T>({x, y})) return launch_stream_vec<T>(n, AxpbyF<T>{a, b, x, y}, s);
     axpby_kernel<T, T><<<grid_for(n), kBlock, 0, s>>>(n, a, x, b, y);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
@@ -706,6 +872,7 @@ int blas_scal(ptrdiff_t n, T a, T *x, hipStream_t s) {
 template <typename T>
 int blas_values(ptrdiff_t n, T a, T *x, hipStream_t s) {
     if (n <= 0) return SFEM_SUCCESS;
+    if (aligned16<T>({x})) return launch_stream_vec<T>(n, ValuesF<T>{a, x}, s);
     values_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, a, x);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
@@ -713,6 +880,7 @@ int blas_values(ptrdiff_t n, T a, T *x, hipStream_t s) {
 template <typename T>
 int blas_copy(ptrdiff_t n, const T *x, T *y, hipStream_t s) {
     if (n <= 0) return SFEM_SUCCESS;
+    if (aligned16<T>({x, y})) return launch_stream_vec<T>(n, CopyF<T>{x, y}, s);
     copy_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, x, y);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
@@ -741,6 +909,8 @@ int blas_cast(ptrdiff_t n, const S *x, D *y, hipStream_t s) {
 template <typename T>
 int cheb3_update(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, T *p, T *x, hipStream_t s) {
     if (n <= 0) return SFEM_SUCCESS;
+    if (aligned16<T>({rhs, Ax, p, x}))
+        return launch_stream_vec<T>(n, Cheb3UpdateF<T, false>{alpha, beta, rhs, Ax, nullptr, p, x}, s);
     cheb3_update_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, alpha, beta, rhs, Ax, p, x);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
@@ -749,6 +919,8 @@ int cheb3_update(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, T *p, 
 template <typename T>
 int cheb3_start(ptrdiff_t n, T alpha, const T *rhs, const T *dinv, int have_ax, T *p, T *x, hipStream_t s) {
     if (n <= 0) return SFEM_SUCCESS;
+    if (aligned16<T>({rhs, dinv, p, x}))
+        return launch_stream_vec<T>(n, Cheb3StartF<T>{alpha, rhs, dinv, have_ax, p, x}, s);
     cheb3_start_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, alpha, rhs, dinv, have_ax, p, x);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
@@ -756,6 +928,9 @@ int cheb3_start(ptrdiff_t n, T alpha, const T *rhs, const T *dinv, int have_ax, 
 template <typename T>
 int cheb3_update_diag(ptrdiff_t n, T alpha, T beta, const T *rhs, const T *Ax, const T *dinv, T *p, T *x,
                       hipStream_t s) {
+    if (n <= 0) return SFEM_SUCCESS;
+    if (aligned16<T>({rhs, Ax, dinv, p, x}))
+        return launch_stream_vec<T>(n, Cheb3UpdateF<T, true>{alpha, beta, rhs, Ax, dinv, p, x}, s);
     cheb3_update_diag_kernel<T><<<grid_for(n), kBlock, 0, s>>>(n, alpha, beta, rhs, Ax, dinv, p, x);
     SFEM_AMD_HIP_CHECK(hipGetLastError());
     return SFEM_SUCCESS;
