@@ -1538,26 +1538,6 @@ __host__ __device__ constexpr inline int st_kind_qmask(int t) {
     return q[t];
 }
 
-// coefficient of block b at neighbour (dx, dy), rows i, cols j, for the quadrants in qmask
-__device__ inline double st_coefficient_q(const double *K, int b, int dx, int dy, int i, int j, int qmask) {
-    const int dz = b == 0 ? -1 : (b == 2 ? 1 : 0);
-    const bool below = b == 0 || b == 1, above = b == 1 || b == 2 || b == 3;
-    double v = 0;
-    for (int oz = 0; oz < 2; oz++) {
-        if (!(oz ? above : below)) continue;
-        const int cz = 1 - oz, nz = cz + dz;
-        if (nz < 0 || nz > 1) continue;
-        for (int qy = 0; qy < 2; qy++)
-            for (int qx = 0; qx < 2; qx++) {
-                if (!((qmask >> (qy * 2 + qx)) & 1)) continue;
-                const int cx = 1 - qx, cy = 1 - qy, nx = cx + dx, ny = cy + dy;
-                if (nx < 0 || nx > 1 || ny < 0 || ny > 1) continue;
-                v += K[(i * 8 + hex8_corner(cx, cy, cz)) * 24 + j * 8 + hex8_corner(nx, ny, nz)];
-            }
-    }
-    return v;
-}
-
 // The FMAs of neighbour NB (values v of its three components) into the outputs at z = p + 1
 // (block S_m), p (block S_0, unless MID is false) and p - 1 (block S_p), output component fastest
 // within each input component, so consecutive FMAs update different accumulators.
@@ -1681,13 +1661,46 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
         for (int c = 0; c < 3; c++) x0[g][c] = x[(ptrdiff_t)g0[g] * 3 + c];
     }
     __syncthreads();  // K staged
-    for (int t = tid; t < S::NTYPES * 324; t += S::BLOCK) {
-        const int kd = t / 324, n = t - kd * 324, b = n / 81, q = n - b * 81, nb = q / 9;
-        const double v = st_coefficient_q(sK, b, nb % 3 - 1, nb / 3 - 1, (q % 9) / 3, q % 3, st_kind_qmask(kd));
-        if (b < 3)
-            stab[kd][n] = v;
-        else
-            slo[kd][q] = v;
+    // Table entry (block b: S_m, S_0, S_p; neighbour (dx, dy); row i, column j) of a row kind: the
+    // sum, over the node's micro elements that the kind has (z side oz: below for S_m, above for
+    // S_p, both for S_0; xy quadrants of st_kind_qmask) and that hold the neighbour, of the element
+    // matrix entry coupling them, in the order (oz, qy, qx).  S_0lo: S_0's upper side (oz = 1), the
+    // z = 0 plane's in-plane block.  One thread per entry reads the eight micro elements' entries
+    // once and sums them for every kind.
+    for (int t = tid; t < 243; t += S::BLOCK) {
+        const int b = t / 81, q = t - b * 81, nb = q / 9, i = (q % 9) / 3, j = q % 3;
+        const int dx = nb % 3 - 1, dy = nb / 3 - 1, dz = b - 1;
+        double c[2][4];
+        unsigned ok = 0;  // bit oz * 4 + qy * 2 + qx: the neighbour lies in that micro element
+#pragma unroll
+        for (int oz = 0; oz < 2; oz++)
+#pragma unroll
+            for (int qy = 0; qy < 2; qy++)
+#pragma unroll
+                for (int qx = 0; qx < 2; qx++) {
+                    const int cz = 1 - oz, cy = 1 - qy, cx = 1 - qx, nz = cz + dz, ny = cy + dy, nx = cx + dx;
+                    const bool in = nx >= 0 && nx <= 1 && ny >= 0 && ny <= 1 && nz >= 0 && nz <= 1;
+                    c[oz][qy * 2 + qx] =
+                            in ? sK[(i * 8 + hex8_corner(cx, cy, cz)) * 24 + j * 8 + hex8_corner(nx, ny, nz)] : 0.0;
+                    ok |= (unsigned)in << (oz * 4 + qy * 2 + qx);
+                }
+#pragma unroll
+        for (int kd = 0; kd < S::NTYPES; kd++) {
+            const unsigned qmask = (unsigned)st_kind_qmask(kd);
+            double v = 0, vlo = 0;
+#pragma unroll
+            for (int oz = 0; oz < 2; oz++) {
+                if (!(oz ? b >= 1 : b <= 1)) continue;  // S_m: below only, S_p: above only
+#pragma unroll
+                for (int qq = 0; qq < 4; qq++)
+                    if (((qmask >> qq) & 1u) && ((ok >> (oz * 4 + qq)) & 1u)) {
+                        v += c[oz][qq];
+                        if (oz) vlo += c[oz][qq];
+                    }
+            }
+            stab[kd][t] = v;
+            if (b == 1) slo[kd][q] = vlo;
+        }
     }
     __syncthreads();  // tables ready
     double tab[NT];
