@@ -1516,7 +1516,7 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
 // are converted on load and store: on this chip FP32 FMA without packing runs at the FP64 rate).
 template <int K>
 __device__ __forceinline__ void fmac_row_bcast(double &acc, double coef, double v) {
-    asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" : "+v"(acc) : "v"(coef), "v"(v), "i"(K));
+    asm volatile("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" : "+v"(acc) : "v"(coef), "v"(v), "i"(K));
 }
 
 template <int L>
@@ -1765,20 +1765,38 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
             // (in-plane block S_0lo from LDS instead of S_0), 2 z = L (S_0 - S_0lo)
             auto sweep = [&](auto face) {
                 constexpr int FACE = decltype(face)::value;
-#define ST_NB(NB)                                                                                       \
+                // the next neighbour's three values are read from LDS before this neighbour's FMAs
+                // (the FMAs are asm statements, which the compiler does not move loads across)
+#define ST_LD(NB, V)                                                                                    \
     {                                                                                                   \
         constexpr int o = (NB / 3 - 1) * H + (NB % 3 - 1);                                              \
-        const double v[3] = {pl0[me + o], pl1[me + o], pl2[me + o]};                                    \
-        st_dpp_nbr<NB, NT, FACE != 1>(acc_hi, acc_mid, acc_lo, tab, v);                                 \
+        V[0] = pl0[me + o];                                                                             \
+        V[1] = pl1[me + o];                                                                             \
+        V[2] = pl2[me + o];                                                                             \
+    }
+#define ST_NB(NB, V)                                                                                    \
+    {                                                                                                   \
+        st_dpp_nbr<NB, NT, FACE != 1>(acc_hi, acc_mid, acc_lo, tab, V);                                 \
         if (FACE) {                                                                                     \
             const double sgn = FACE == 1 ? 1.0 : -1.0;                                                  \
             _Pragma("unroll") for (int i = 0; i < 3; i++)                                               \
                 _Pragma("unroll") for (int j = 0; j < 3; j++)                                           \
-                    acc_mid[i] = fma(sgn * lo[NB * 9 + i * 3 + j], v[j], acc_mid[i]);                   \
+                    acc_mid[i] = fma(sgn * lo[NB * 9 + i * 3 + j], V[j], acc_mid[i]);                   \
         }                                                                                               \
     }
-                ST_NB(0) ST_NB(1) ST_NB(2) ST_NB(3) ST_NB(4) ST_NB(5) ST_NB(6) ST_NB(7) ST_NB(8)
+                double va[3], vb[3];
+                ST_LD(0, va)
+                ST_LD(1, vb) ST_NB(0, va)
+                ST_LD(2, va) ST_NB(1, vb)
+                ST_LD(3, vb) ST_NB(2, va)
+                ST_LD(4, va) ST_NB(3, vb)
+                ST_LD(5, vb) ST_NB(4, va)
+                ST_LD(6, va) ST_NB(5, vb)
+                ST_LD(7, vb) ST_NB(6, va)
+                ST_LD(8, va) ST_NB(7, vb)
+                ST_NB(8, va)
 #undef ST_NB
+#undef ST_LD
             };
             if (p <= L) {
                 if (p == 0)
