@@ -1510,9 +1510,10 @@ int sshex8_elasticity_apply(int L, ptrdiff_t E, const int32_t *idx, const T *K, 
 // columns).  The four corner columns (one quadrant, one column each) would take a row and a
 // table apiece for one node per plane; instead the sweep saves each corner's 2x2 block of columns
 // and, after it, the block's threads apply the element matrix to the corner columns' micro
-// elements (L per column) from LDS.  Register table blocks: S_m, S_0, S_p;
-// the z-face block S_0lo (a node on z = 0 sees S_0lo, on z = L S_0 - S_0lo) stays in LDS, read on
-// those two planes only.  Arithmetic is FP64 for both vector precisions (FP32 vectors
+// elements (L per column) from LDS.  Register table blocks: S_m, S_0, S_p; on the z = 0 and
+// z = L planes the register slots holding S_0 are reloaded with the in-plane block of the elements
+// above (S_0lo) and below (S_0up) the node, and restored after z = 0, so every plane runs the same
+// 243 FMAs.  Arithmetic is FP64 for both vector precisions (FP32 vectors
 // are converted on load and store: on this chip FP32 FMA without packing runs at the FP64 rate).
 template <int K>
 __device__ __forceinline__ void fmac_row_bcast(double &acc, double coef, double v) {
@@ -1526,7 +1527,7 @@ struct StDpp {
     static constexpr int RI = (NI + 15) / 16;  // rows of interior columns
     static constexpr int ROWS = RI + 4;        // + the four face rows
     static constexpr int BLOCK = 64 * ((ROWS + 3) / 4);
-    static constexpr int NC = 4 * 81;           // coefficients per kind (S_m, S_0, S_p, S_0lo)
+    static constexpr int NC = 3 * 81;           // coefficients per kind (S_m, S_0, S_p)
     static constexpr int NT = (3 * 81 + 15) / 16;  // register table per lane (S_m, S_0, S_p)
     static constexpr int NTYPES = 5;  // row kinds: interior, the four faces
 };
@@ -1539,37 +1540,37 @@ __host__ __device__ constexpr inline int st_kind_qmask(int t) {
 }
 
 // The FMAs of neighbour NB (values v of its three components) into the outputs at z = p + 1
-// (block S_m), p (block S_0, unless MID is false) and p - 1 (block S_p), output component fastest
+// (block S_m), p (block S_0) and p - 1 (block S_p), output component fastest
 // within each input component, so consecutive FMAs update different accumulators.
-template <int NB, int NT, bool MID>
+template <int NB, int NT>
 __device__ __forceinline__ void st_dpp_nbr(double (&ah)[3], double (&am)[3], double (&al)[3], const double (&tab)[NT],
                                            const double v[3]) {
     fmac_row_bcast<(0 + NB * 9 + 0) % 16>(ah[0], tab[(0 + NB * 9 + 0) / 16], v[0]);
-    if (MID) fmac_row_bcast<(81 + NB * 9 + 0) % 16>(am[0], tab[(81 + NB * 9 + 0) / 16], v[0]);
+    fmac_row_bcast<(81 + NB * 9 + 0) % 16>(am[0], tab[(81 + NB * 9 + 0) / 16], v[0]);
     fmac_row_bcast<(162 + NB * 9 + 0) % 16>(al[0], tab[(162 + NB * 9 + 0) / 16], v[0]);
     fmac_row_bcast<(0 + NB * 9 + 3) % 16>(ah[1], tab[(0 + NB * 9 + 3) / 16], v[0]);
-    if (MID) fmac_row_bcast<(81 + NB * 9 + 3) % 16>(am[1], tab[(81 + NB * 9 + 3) / 16], v[0]);
+    fmac_row_bcast<(81 + NB * 9 + 3) % 16>(am[1], tab[(81 + NB * 9 + 3) / 16], v[0]);
     fmac_row_bcast<(162 + NB * 9 + 3) % 16>(al[1], tab[(162 + NB * 9 + 3) / 16], v[0]);
     fmac_row_bcast<(0 + NB * 9 + 6) % 16>(ah[2], tab[(0 + NB * 9 + 6) / 16], v[0]);
-    if (MID) fmac_row_bcast<(81 + NB * 9 + 6) % 16>(am[2], tab[(81 + NB * 9 + 6) / 16], v[0]);
+    fmac_row_bcast<(81 + NB * 9 + 6) % 16>(am[2], tab[(81 + NB * 9 + 6) / 16], v[0]);
     fmac_row_bcast<(162 + NB * 9 + 6) % 16>(al[2], tab[(162 + NB * 9 + 6) / 16], v[0]);
     fmac_row_bcast<(0 + NB * 9 + 1) % 16>(ah[0], tab[(0 + NB * 9 + 1) / 16], v[1]);
-    if (MID) fmac_row_bcast<(81 + NB * 9 + 1) % 16>(am[0], tab[(81 + NB * 9 + 1) / 16], v[1]);
+    fmac_row_bcast<(81 + NB * 9 + 1) % 16>(am[0], tab[(81 + NB * 9 + 1) / 16], v[1]);
     fmac_row_bcast<(162 + NB * 9 + 1) % 16>(al[0], tab[(162 + NB * 9 + 1) / 16], v[1]);
     fmac_row_bcast<(0 + NB * 9 + 4) % 16>(ah[1], tab[(0 + NB * 9 + 4) / 16], v[1]);
-    if (MID) fmac_row_bcast<(81 + NB * 9 + 4) % 16>(am[1], tab[(81 + NB * 9 + 4) / 16], v[1]);
+    fmac_row_bcast<(81 + NB * 9 + 4) % 16>(am[1], tab[(81 + NB * 9 + 4) / 16], v[1]);
     fmac_row_bcast<(162 + NB * 9 + 4) % 16>(al[1], tab[(162 + NB * 9 + 4) / 16], v[1]);
     fmac_row_bcast<(0 + NB * 9 + 7) % 16>(ah[2], tab[(0 + NB * 9 + 7) / 16], v[1]);
-    if (MID) fmac_row_bcast<(81 + NB * 9 + 7) % 16>(am[2], tab[(81 + NB * 9 + 7) / 16], v[1]);
+    fmac_row_bcast<(81 + NB * 9 + 7) % 16>(am[2], tab[(81 + NB * 9 + 7) / 16], v[1]);
     fmac_row_bcast<(162 + NB * 9 + 7) % 16>(al[2], tab[(162 + NB * 9 + 7) / 16], v[1]);
     fmac_row_bcast<(0 + NB * 9 + 2) % 16>(ah[0], tab[(0 + NB * 9 + 2) / 16], v[2]);
-    if (MID) fmac_row_bcast<(81 + NB * 9 + 2) % 16>(am[0], tab[(81 + NB * 9 + 2) / 16], v[2]);
+    fmac_row_bcast<(81 + NB * 9 + 2) % 16>(am[0], tab[(81 + NB * 9 + 2) / 16], v[2]);
     fmac_row_bcast<(162 + NB * 9 + 2) % 16>(al[0], tab[(162 + NB * 9 + 2) / 16], v[2]);
     fmac_row_bcast<(0 + NB * 9 + 5) % 16>(ah[1], tab[(0 + NB * 9 + 5) / 16], v[2]);
-    if (MID) fmac_row_bcast<(81 + NB * 9 + 5) % 16>(am[1], tab[(81 + NB * 9 + 5) / 16], v[2]);
+    fmac_row_bcast<(81 + NB * 9 + 5) % 16>(am[1], tab[(81 + NB * 9 + 5) / 16], v[2]);
     fmac_row_bcast<(162 + NB * 9 + 5) % 16>(al[1], tab[(162 + NB * 9 + 5) / 16], v[2]);
     fmac_row_bcast<(0 + NB * 9 + 8) % 16>(ah[2], tab[(0 + NB * 9 + 8) / 16], v[2]);
-    if (MID) fmac_row_bcast<(81 + NB * 9 + 8) % 16>(am[2], tab[(81 + NB * 9 + 8) / 16], v[2]);
+    fmac_row_bcast<(81 + NB * 9 + 8) % 16>(am[2], tab[(81 + NB * 9 + 8) / 16], v[2]);
     fmac_row_bcast<(162 + NB * 9 + 8) % 16>(al[2], tab[(162 + NB * 9 + 8) / 16], v[2]);
 }
 
@@ -1582,6 +1583,7 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
     using S = StDpp<L>;
     constexpr int N1 = S::N1, PLANE = S::PLANE, H = S::H, P2 = S::P2, NXE = N1 * PLANE, NSK = sk_count(L),
                   NT = S::NT, NC = S::NC;
+    constexpr int ST_SLICE0 = 80, ST_SLICE = 96;  // register-table slots 5..10: entries 80..175
     constexpr int XR = 2 * G;      // x planes in LDS: G being read, G arriving
     constexpr int IR = 3 * G + 1;  // node-id ring: planes z - 1 .. z + 3G - 1 of a step starting at z
     // LDS: the per-kind tables (S_m, S_0, S_p of the five row kinds) are only read once into each
@@ -1590,7 +1592,10 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
                      ST_BYTES = sizeof(double) * S::NTYPES * 243;
     constexpr size_t U_BYTES = ST_BYTES > SX_BYTES + SG_BYTES ? ST_BYTES : SX_BYTES + SG_BYTES;
     __shared__ __align__(16) unsigned char ubuf[U_BYTES];
-    __shared__ double slo[S::NTYPES][81];  // S_0lo of the nine kinds (the z = 0 and z = L planes)
+    // register-table slots 5..10 of each kind (entries 80..175: S_0 and its neighbours in S_m, S_p)
+    // with the in-plane block of the z = 0 plane (variant 0: the elements above), of the inner
+    // planes (1: both sides) and of the z = L plane (2: below)
+    __shared__ double s0v[3][S::NTYPES][ST_SLICE];
     __shared__ double sK[576];
     __shared__ double csave[4][4][3][N1];        // x of each corner's 2x2 block of columns, by plane
     __shared__ double clo[4][L][3], cup[4][L][3];  // corner rows of the micro elements of the corner columns
@@ -1664,9 +1669,9 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
     // Table entry (block b: S_m, S_0, S_p; neighbour (dx, dy); row i, column j) of a row kind: the
     // sum, over the node's micro elements that the kind has (z side oz: below for S_m, above for
     // S_p, both for S_0; xy quadrants of st_kind_qmask) and that hold the neighbour, of the element
-    // matrix entry coupling them, in the order (oz, qy, qx).  S_0lo: S_0's upper side (oz = 1), the
-    // z = 0 plane's in-plane block.  One thread per entry reads the eight micro elements' entries
-    // once and sums them for every kind.
+    // matrix entry coupling them, in the order (oz, qy, qx).  The z = 0 and z = L planes' in-plane
+    // blocks: S_0's upper (oz = 1) and lower (oz = 0) sides.  One thread per entry reads the eight
+    // micro elements' entries once and sums them for every kind.
     for (int t = tid; t < 243; t += S::BLOCK) {
         const int b = t / 81, q = t - b * 81, nb = q / 9, i = (q % 9) / 3, j = q % 3;
         const int dx = nb % 3 - 1, dy = nb / 3 - 1, dz = b - 1;
@@ -1687,7 +1692,7 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
 #pragma unroll
         for (int kd = 0; kd < S::NTYPES; kd++) {
             const unsigned qmask = (unsigned)st_kind_qmask(kd);
-            double v = 0, vlo = 0;
+            double v = 0, vside[2] = {0, 0};
 #pragma unroll
             for (int oz = 0; oz < 2; oz++) {
                 if (!(oz ? b >= 1 : b <= 1)) continue;  // S_m: below only, S_p: above only
@@ -1695,16 +1700,20 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
                 for (int qq = 0; qq < 4; qq++)
                     if (((qmask >> qq) & 1u) && ((ok >> (oz * 4 + qq)) & 1u)) {
                         v += c[oz][qq];
-                        if (oz) vlo += c[oz][qq];
+                        vside[oz] += c[oz][qq];
                     }
             }
             stab[kd][t] = v;
-            if (b == 1) slo[kd][q] = vlo;
+            if (t >= ST_SLICE0 && t < ST_SLICE0 + ST_SLICE) {
+                const bool inplane = b == 1;
+                s0v[0][kd][t - ST_SLICE0] = inplane ? vside[1] : v;
+                s0v[1][kd][t - ST_SLICE0] = v;
+                s0v[2][kd][t - ST_SLICE0] = inplane ? vside[0] : v;
+            }
         }
     }
     __syncthreads();  // tables ready
     double tab[NT];
-    const double *lo = slo[kind >= 0 ? kind : 0];  // S_0lo of this row's kind
 #pragma unroll
     for (int m = 0; m < NT; m++) {
         const int n = m * 16 + k;
@@ -1761,12 +1770,19 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
             const int buf = p % XR;
             const double *pl0 = &sx[buf][0][0], *pl1 = &sx[buf][1][0], *pl2 = &sx[buf][2][0];
             // all lanes run the FMAs (the table lanes of every row must be active for the broadcast);
-            // the outputs of the planes outside [0, L] are never stored.  FACE: 0 inner plane, 1 z = 0
-            // (in-plane block S_0lo from LDS instead of S_0), 2 z = L (S_0 - S_0lo)
-            auto sweep = [&](auto face) {
-                constexpr int FACE = decltype(face)::value;
+            // the outputs of the planes outside [0, L] are never stored.  The z = 0, inner and z = L
+            // planes differ in the in-plane block only: its register slots are reloaded on entering
+            // them (no table rows are read otherwise)
+            if (p == 0 || p == 1 || p == L) {
+                const int var = p == 0 ? 0 : (p == L ? 2 : 1);
+                const double *sv = s0v[var][kind >= 0 ? kind : 0];
+#pragma unroll
+                for (int m = ST_SLICE0 / 16; m < (ST_SLICE0 + ST_SLICE) / 16; m++)
+                    tab[m] = kind >= 0 && m * 16 + k < 3 * 81 ? sv[m * 16 + k - ST_SLICE0] : 0.0;
+            }
+            if (p <= L) {
                 // the next neighbour's three values are read from LDS before this neighbour's FMAs
-                // (the FMAs are asm statements, which the compiler does not move loads across)
+                // (volatile asm statements, which the compiler does not move loads across)
 #define ST_LD(NB, V)                                                                                    \
     {                                                                                                   \
         constexpr int o = (NB / 3 - 1) * H + (NB % 3 - 1);                                              \
@@ -1774,16 +1790,7 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
         V[1] = pl1[me + o];                                                                             \
         V[2] = pl2[me + o];                                                                             \
     }
-#define ST_NB(NB, V)                                                                                    \
-    {                                                                                                   \
-        st_dpp_nbr<NB, NT, FACE != 1>(acc_hi, acc_mid, acc_lo, tab, V);                                 \
-        if (FACE) {                                                                                     \
-            const double sgn = FACE == 1 ? 1.0 : -1.0;                                                  \
-            _Pragma("unroll") for (int i = 0; i < 3; i++)                                               \
-                _Pragma("unroll") for (int j = 0; j < 3; j++)                                           \
-                    acc_mid[i] = fma(sgn * lo[NB * 9 + i * 3 + j], V[j], acc_mid[i]);                   \
-        }                                                                                               \
-    }
+#define ST_NB(NB, V) st_dpp_nbr<NB, NT>(acc_hi, acc_mid, acc_lo, tab, V);
                 double va[3], vb[3];
                 ST_LD(0, va)
                 ST_LD(1, vb) ST_NB(0, va)
@@ -1797,14 +1804,6 @@ __global__ void __launch_bounds__(StDpp<L>::BLOCK)
                 ST_NB(8, va)
 #undef ST_NB
 #undef ST_LD
-            };
-            if (p <= L) {
-                if (p == 0)
-                    sweep(std::integral_constant<int, 1>());
-                else if (p == L)
-                    sweep(std::integral_constant<int, 2>());
-                else
-                    sweep(std::integral_constant<int, 0>());
             }
             // plane p - 1 is complete
 #pragma unroll
