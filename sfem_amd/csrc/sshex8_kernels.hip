@@ -892,18 +892,44 @@ __global__ void __launch_bounds__(256)
         xb[u] = lidx(L, mx, my, mz);
         yb[u] = mz * Y::PL + my * Y::RS + mx;
     }
+    // B columns of tile u in colour `colour` (in range for every lane: qq, gc clamped; invalid
+    // columns are never scattered)
+    auto load_b = [&](int colour, int u, uint32_t(&pk)[8]) {
+        const uint32_t *xp = sxl + xb[u] + lidx(L, colour & 1, (colour >> 1) & 1, colour >> 2);
+#pragma unroll
+        for (int j = 0; j < 8; j++) pk[j] = xp[lidx(L, hex8_cx(j), hex8_cy(j), hex8_cz(j))];
+    };
+    // With one tile per wave and colour the next colour's columns are read from LDS before this
+    // colour's products (x in LDS does not change in the loop), and each tile's node accumulators
+    // before its products (the two row tiles hold different components, and no other wave touches
+    // these nodes in this colour): the LDS latency overlaps the matrix-core work
+    uint32_t pkn[8];
+    if (TPW == 1 && wsub < TPC) load_b(0, 0, pkn);
 #pragma unroll 1
     for (int colour = 0; colour < 8; colour++) {
         const int px = colour & 1, py = (colour >> 1) & 1, pz = colour >> 2;
-        const int xc = lidx(L, px, py, pz), yc = pz * Y::PL + py * Y::RS + px;
+        const int yc = pz * Y::PL + py * Y::RS + px;
 #pragma unroll
         for (int u = 0; u < TPW; u++) {
             if (wsub + u * WPE >= TPC) break;  // wave-uniform
-            // in range for every lane (qq, gc clamped); invalid columns are never scattered
-            const uint32_t *xp = sxl + xb[u] + xc;
             uint32_t pk[8];
+            if (TPW == 1) {
 #pragma unroll
-            for (int j = 0; j < 8; j++) pk[j] = xp[lidx(L, hex8_cx(j), hex8_cy(j), hex8_cz(j))];
+                for (int j = 0; j < 8; j++) pk[j] = pkn[j];
+                if (colour < 7) load_b(colour + 1, 0, pkn);
+            } else {
+                load_b(colour, u, pk);
+            }
+            float yv[2][4];
+#pragma unroll
+            for (int tt = 0; tt < 2; tt++)
+                if (valid[u] && row_ok[tt]) {
+                    const float *yp = syl[tt] + yb[u] + yc;
+                    yv[tt][0] = yp[0];
+                    yv[tt][1] = yp[1];
+                    yv[tt][2] = yp[Y::RS + 1];
+                    yv[tt][3] = yp[Y::RS];
+                }
             el_b8 bhi, blo;
             el_unpack8(pk, bhi, blo);
 #pragma unroll
@@ -914,10 +940,10 @@ __global__ void __launch_bounds__(256)
                 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[tt], bhi, d, 0, 0, 0);
                 if (valid[u] && row_ok[tt]) {
                     float *yp = syl[tt] + yb[u] + yc;
-                    yp[0] += d[0];
-                    yp[1] += d[1];
-                    yp[Y::RS + 1] += d[2];
-                    yp[Y::RS] += d[3];
+                    yp[0] = yv[tt][0] + d[0];
+                    yp[1] = yv[tt][1] + d[1];
+                    yp[Y::RS + 1] = yv[tt][2] + d[2];
+                    yp[Y::RS] = yv[tt][3] + d[3];
                 }
             }
         }
