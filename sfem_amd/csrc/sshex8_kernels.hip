@@ -1050,6 +1050,23 @@ __global__ void __launch_bounds__(256)
             const bool valid = active && q < NC;
             const int qq = valid ? q : 0;
             const int mx = 2 * (qq % H) + px, my = 2 * ((qq / H) % H) + py, mz = 2 * (qq / (H * H)) + pz;
+            // the lane's accumulators (distinct (component, corner) rows: distinct nodes or planes,
+            // untouched by other waves in this colour), read before the products so that the LDS
+            // latency overlaps them
+            double yv[4][2];
+            if (valid) {
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+#pragma unroll
+                    for (int tt = 0; tt < 2; tt++) {
+                        const int R = 16 * tt + M::row(g, r);  // component R / 8, corner R % 8
+                        if (R < 24) {
+                            const int k = R & 7;
+                            yv[r][tt] = sy[le][R >> 3][(mz + hex8_cz(k)) * Y::PL + (my + hex8_cy(k)) * Y::RS + mx +
+                                                      hex8_cx(k)];
+                        }
+                    }
+            }
             typename M::v4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
 #pragma unroll
             for (int s = 0; s < 6; s++) {
@@ -1064,11 +1081,11 @@ __global__ void __launch_bounds__(256)
                 for (int r = 0; r < 4; r++) {
 #pragma unroll
                     for (int tt = 0; tt < 2; tt++) {
-                        const int R = 16 * tt + M::row(g, r);  // component R / 8, corner R % 8
+                        const int R = 16 * tt + M::row(g, r);
                         if (R < 24) {
                             const int k = R & 7;
-                            sy[le][R >> 3][(mz + hex8_cz(k)) * Y::PL + (my + hex8_cy(k)) * Y::RS + mx + hex8_cx(k)] +=
-                                    tt ? d1[r] : d0[r];
+                            sy[le][R >> 3][(mz + hex8_cz(k)) * Y::PL + (my + hex8_cy(k)) * Y::RS + mx + hex8_cx(k)] =
+                                    yv[r][tt] + (tt ? d1[r] : d0[r]);
                         }
                     }
                 }
