@@ -578,3 +578,37 @@ def test_blas_reductions_deterministic_and_accurate():
         assert abs(d1 - a @ b) <= 1e-12 * max(1.0, np.abs(a) @ np.abs(b))
         if n:
             assert abs(S.norm2(da) - np.linalg.norm(a)) < 1e-12 * np.linalg.norm(a)
+
+
+@pytest.mark.parametrize("offset", [0, 1, 4])
+def test_fp32_streaming_kernels_aligned_and_not(offset):
+    """FP32 axpby, fill, copy and the Chebyshev3 steps take 16 bytes per thread and array when every
+    array is 16-byte aligned (the last n % 4 elements by the first threads of block 0) and one
+    float per thread otherwise: both against numpy in FP32 arithmetic, on arrays starting `offset`
+    floats into their allocation (4: aligned, 1: not), with n = 1001."""
+    n = 1001
+    rng = np.random.default_rng(offset)
+    host_in = {k: rng.uniform(-1, 1, n + 8).astype(np.float32) for k in ("x", "y", "rhs", "ax", "dinv", "p", "z")}
+    arr = {k: dev(v, np.float32) for k, v in host_in.items()}
+    ptr = {k: C.c_void_p(a.ptr + 4 * offset) for k, a in arr.items()}
+    sl = slice(offset, offset + n)
+    f = np.float32
+    a, b = f(0.75), f(-1.25)
+    assert S.lib.sfem_amd_blas_axpby(S.FLOAT32, n, float(a), ptr["x"], float(b), ptr["y"], None) == 0
+    assert S.lib.sfem_amd_blas_copy(S.FLOAT32, n, ptr["x"], ptr["z"], None) == 0
+    assert S.lib.sfem_amd_blas_values(S.FLOAT32, n, 2.5, ptr["dinv"], None) == 0
+    alpha, beta = f(0.3), f(0.6)
+    assert S.lib.sfem_amd_cheb3_update(S.FLOAT32, n, float(alpha), float(beta), ptr["rhs"], ptr["ax"], ptr["p"],
+                                       ptr["x"], None) == 0
+    S.synchronize()
+    got = {k: host(v) for k, v in arr.items()}
+    x0, y0, rhs, ax, p0 = (host_in[k][sl] for k in ("x", "y", "rhs", "ax", "p"))
+    np.testing.assert_allclose(got["y"][sl], a * x0 + b * y0, rtol=1e-6, atol=1e-7)
+    np.testing.assert_array_equal(got["z"][sl], x0)
+    np.testing.assert_array_equal(got["dinv"][sl], np.full(n, 2.5, np.float32))
+    pn = (beta * p0 + (ax - rhs)).astype(np.float32)  # fma(beta, p, Ax - rhs) in FP32
+    np.testing.assert_allclose(got["p"][sl], pn, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(got["x"][sl], x0 - alpha * got["p"][sl], rtol=1e-6, atol=1e-7)
+    for k in ("x", "y", "z", "dinv", "p"):  # nothing outside [offset, offset + n) written
+        np.testing.assert_array_equal(got[k][:offset], host_in[k][:offset])
+        np.testing.assert_array_equal(got[k][offset + n:], host_in[k][offset + n:])
